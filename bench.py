@@ -1,0 +1,200 @@
+#!/usr/bin/env python3
+"""Benchmark of the Stokes-matvec FMM (the layer potential GMRES applies every iteration).
+
+Workload (the FMM-alone configuration): 2^20 boundary points on 4096 closed circles with
+centres U[0,1]^2 and radii U[0.01,0.05], 256 points each, complex CN(0,1) densities from a
+fixed seed; kernel = CADAPDR's GW (DR signs), FP64, evaluated at every point.
+
+One step = one complete FMM evaluation from raw device-resident coordinates: tree build
+(binning, sort, leaf table) + P2M, M2M, M2L+L2L, L2P, P2P.  `value` is points per second of
+that whole evaluation (Mpts/s).  The apply-only rate (tree kept, as inside one GMRES solve)
+is reported beside it.
+
+Prints ONE JSON line.  Single GPU only.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# FP64 peak of MI355X (vector = matrix) and HBM3E bandwidth
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+
+
+def flops_model(info, n, p):
+    """Algorithmic flop convention: complex add 2, complex*real 2, complex*complex 6,
+    reciprocal 6, divide 12; per stage counts from the tree."""
+    return {
+        "p2p": 50.0 * info["near_pairs"],
+        "m2l": (8.0 * p * p + 34.0 * p) * info["m2l_pairs"],
+        "p2m": 26.0 * p * n,
+        "l2p": 26.0 * p * n,
+        "m2m": (8.0 * p * p + 30.0 * p) * info["boxes"],
+    }
+
+
+def cpu_baseline(z, dz, nd, h, dens, levmax, p, ncurves_total, budget_s=20.0):
+    """Time the reference CADAPDR (compiled in place from its sources into oracle/_ref) or, when
+    that library is absent, the C restatement, on a bounded sample of the same workload: the
+    first S curves with LEVMAX reduced so that points per leaf box match."""
+    import oracle
+    kind = "reference" if oracle.ref_available() else "port"
+    tol = 2.0 ** (-p)
+    m = int(nd[0])
+    S = 256
+    best = None
+    while True:
+        npts = S * m
+        lev = max(2, levmax - int(round(np.log(ncurves_total / S) / np.log(4.0))))
+        args = (oracle.DR, lev, nd[:S], z.real[:npts], z.imag[:npts], dz[:npts], dens[:npts], h[:S], tol)
+        t0 = time.perf_counter()
+        if kind == "reference":
+            oracle.ref_cadap(*args)
+        else:
+            oracle.cadap(*args)
+        dt = time.perf_counter() - t0
+        best = (npts, lev, dt)
+        if dt > budget_s / 4 or S * 4 > ncurves_total:
+            break
+        S *= 4
+    npts, lev, dt = best
+    return {"value": round(npts / dt / 1e6, 4), "unit": "Mpts/s", "cores": 1, "kind": kind,
+            "sample": f"one CADAPDR call (DR, TOL 2^-{p}, LEVMAX {lev}) on the first {npts} points "
+                      f"({npts // m} curves) of the same workload, {dt:.2f} s on 1 host core"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", choices=["curves", "uniform"], default="curves")
+    ap.add_argument("--levmax", type=int, default=8)
+    ap.add_argument("--nterms", type=int, default=36)
+    ap.add_argument("--npts", type=int, default=1 << 20)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-check", action="store_true", help="skip the accuracy spot check")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on one GPU only (the FMM is measured on one MI355X); "
+              "--gpus above 1 is not supported", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import electricbubbles_amd as eb
+    from electricbubbles_amd.workloads import complex_normal, curves_cfg, uniform_cfg, weights
+
+    if not torch.cuda.is_available():
+        print("bench.py: no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(20260303)
+    n = args.npts
+    if args.workload == "curves":
+        ncurves = n // 256
+        z, dz, nd, h = curves_cfg(ncurves, 256, rng)
+    else:
+        ncurves = 0
+        z, dz, nd, h = uniform_cfg(n, rng)
+    dens = complex_normal(n, rng)
+    p, L = args.nterms, args.levmax
+
+    x = torch.tensor(z.real.copy(), device=dev)
+    y = torch.tensor(z.imag.copy(), device=dev)
+    w = torch.tensor(weights(dz, nd, h), device=dev)
+    d = torch.tensor(dens, device=dev)
+    out = torch.empty(n, dtype=torch.complex128, device=dev)
+    stream = torch.cuda.current_stream()
+
+    plan = eb.FMMPlan(x, y, w, L, p, eb.VARIANT_DR, eb.MODE_FMM, stream=stream)
+    for _ in range(args.warmup):
+        plan.rebuild(x, y, w)
+        plan.apply(d, out)
+    torch.cuda.synchronize()
+
+    # timed: K complete evaluations (tree build + far + near)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.rebuild(x, y, w)
+        plan.apply(d, out)
+    torch.cuda.synchronize()
+    t_full = (time.perf_counter() - t0) / args.steps
+
+    # apply only (tree kept: the GMRES inner loop)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.apply(d, out)
+    torch.cuda.synchronize()
+    t_apply = (time.perf_counter() - t0) / args.steps
+
+    # per-stage times with HIP events on the plan's stream (outside the timed loops)
+    plan.enable_timing(True)
+    reps = 5
+    acc = None
+    for _ in range(reps):
+        plan.apply(d, out)
+        s = plan.stage_ms()
+        acc = s if acc is None else {k: acc[k] + s[k] for k in s}
+    stage = {k: v / reps for k, v in acc.items()}
+    plan.enable_timing(False)
+    info = plan.info()
+    fl = flops_model(info, n, p)
+    dom = max(("p2p", "m2l"), key=lambda k: stage[k])
+    achieved = fl[dom] / (stage[dom] * 1e-3) / 1e12
+    total_fl = sum(fl.values())
+
+    check = None
+    if not args.no_check:
+        import oracle
+        tgt = np.sort(rng.choice(n, 64, replace=False)).astype(np.int32)
+        ref = oracle.direct(oracle.DR, nd, z.real, z.imag, dz, dens, h, tgt)
+        got = out.cpu().numpy()[tgt]
+        check = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(z, dz, nd, h, dens, L, p, max(ncurves, 1) if ncurves else 4096)
+
+    value = n / t_full / 1e6
+    line = {
+        "metric": "FMM Mpts/s @N=2^20",
+        "value": round(value, 3),
+        "unit": "Mpts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_full * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"FMM alone, {n} points on {ncurves} circles x 256 (r U[0.01,0.05], "
+                               f"centres U[0,1]^2)" if ncurves else f"FMM alone, {n} uniform points",
+                   "kernel": "CADAPDR GW (DR signs), sources = targets",
+                   "levmax": L, "nterms": p, "leaves_nonempty": info["leaves"], "max_leaf": info["max_leaf"]},
+        "apply_only": {"value": round(n / t_apply / 1e6, 3), "unit": "Mpts/s", "ms": round(t_apply * 1e3, 4)},
+        "stage_ms": {k: round(v, 4) for k, v in stage.items()},
+        "fp64_fraction_total": round(total_fl / (t_apply) / 1e12 / PEAK_FP64_TFLOPS, 4),
+        "roofline": {"bound": "mfma", "kernel": dom, "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
+                     "flops_per_launch": fl[dom]},
+        "accuracy_vs_direct": check,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    plan.close()
+
+
+if __name__ == "__main__":
+    main()

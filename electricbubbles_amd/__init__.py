@@ -1,0 +1,299 @@
+"""electricbubbles_amd -- MI355X (gfx950) implementation of ElectricBubbles' Stokes-matvec path.
+
+The native library ``libeb.so`` (HIP kernels + C-ABI, ``include/eb_cadap.h``) does all the
+work; this module is a thin ctypes layer over that C-ABI:
+
+* :func:`cadapdr`, :func:`cadapmod`, :func:`cadap` -- the reference's CADAPDR / CADAPMOD /
+  CADAP interface (cadapdrop.f:3-5) on host arrays, returning ``(gradw, inform, ier)``;
+* :class:`FMMPlan` -- the device-resident plan/apply API on torch tensors (device memory and
+  streams only; no torch types cross the C-ABI);
+* :func:`tables` / :func:`lists` -- the reference's per-level binning tables and
+  interaction lists computed on the GPU (tree parity);
+* spectral / RK / FASMVP-epilogue wrappers (see :mod:`electricbubbles_amd.spectral`).
+
+There is no CPU fallback: if ``libeb.so`` is missing, or no GPU is visible, the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libeb.so")
+
+EB_OK = 0
+EB_ERR_TERMS = 4
+EB_ERR_WORKSPACE = 8
+EB_ERR_UNSUPPORTED = 16
+EB_ERR_ARG = 17
+EB_ERR_HIP = 18
+EB_ERR_NOMEM = 19
+
+VARIANT_DR, VARIANT_MOD, VARIANT_PLUS = 0, 1, 2
+MODE_AUTO, MODE_FMM, MODE_DIRECT = 0, 1, 2
+
+_vp = ctypes.c_void_p
+_i32 = ctypes.c_int32
+_ip = ctypes.POINTER(ctypes.c_int32)
+_dp = ctypes.POINTER(ctypes.c_double)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+
+# exported symbols of include/eb_cadap.h and include/eb_spectral.h
+EXPORTS = (
+    "eb_last_error", "eb_version", "eb_cadapdr", "eb_cadapmod", "eb_cadap", "eb_set_mode",
+    "eb_fmm_plan_create", "eb_fmm_plan_rebuild", "eb_fmm_apply", "eb_fmm_plan_destroy", "eb_fmm_plan_info",
+    "eb_fmm_plan_ntbox", "eb_fmm_enable_timing", "eb_fmm_stage_ms", "eb_cadap_tables",
+    "eb_cadap_lists",
+)
+
+_lib = None
+
+
+class EBError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"eb status {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    """Load libeb.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBPATH):
+        raise RuntimeError(f"{LIBPATH} is not built: run `python -c 'import __graft_entry__ as g; g.build()'`")
+    L = ctypes.CDLL(LIBPATH)
+    L.eb_last_error.restype = ctypes.c_char_p
+    dropin_args = [_ip, _ip, _ip, _dp, _dp, _dp, _ip, _ip, _ip, _dp, _dp, _ip, _ip, _dp, _ip,
+                   _dp, _dp, _ip, _dp, _dp, _dp, _dp, _ip, _ip, _dp, _dp]
+    for name in ("eb_cadapdr", "eb_cadapmod", "eb_cadap"):
+        f = getattr(L, name)
+        f.argtypes = dropin_args
+        f.restype = _i32
+    L.eb_set_mode.argtypes = [ctypes.c_int]
+    L.eb_fmm_plan_create.argtypes = [ctypes.POINTER(_vp), _i32, _vp, _vp, _vp, _i32, _i32, _i32, _i32, _vp]
+    L.eb_fmm_apply.argtypes = [_vp, _vp, _vp, _vp]
+    L.eb_fmm_plan_rebuild.argtypes = [_vp, _vp, _vp, _vp, _vp]
+    L.eb_fmm_plan_destroy.argtypes = [_vp]
+    L.eb_fmm_plan_info.argtypes = [_vp, _dp, _i64p]
+    L.eb_fmm_plan_ntbox.argtypes = [_vp, _ip]
+    L.eb_fmm_enable_timing.argtypes = [_vp, _i32]
+    L.eb_fmm_stage_ms.argtypes = [_vp, _dp]
+    L.eb_cadap_tables.argtypes = [_i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _ip, _ip, _vp]
+    L.eb_cadap_lists.argtypes = [_i32, _vp, _vp, _vp, _vp, _vp]
+    _lib = L
+    return L
+
+
+def last_error() -> str:
+    return lib().eb_last_error().decode(errors="replace")
+
+
+def _check(rc):
+    if rc != EB_OK:
+        raise EBError(rc, last_error())
+
+
+def nterms_for(tol: float) -> int:
+    """NTERMS = -NINT(log(TOL)/log(2)) (cadapdrop.f:126)."""
+    v = np.log(tol) / np.log(2.0)
+    return -int(np.sign(v) * np.floor(abs(v) + 0.5))
+
+
+# ----------------------------------------------------------------------------------------
+# drop-in interface on host arrays
+# ----------------------------------------------------------------------------------------
+
+def _as_c(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def _call_dropin(fn, iflag1, iflag2, levmax, xat, yat, dzdt, nd, dens, h, tol, leniw, lenw, k0=1):
+    nd = _as_c(nd, np.int32)
+    kbod = k0 + nd.size - 1
+    n = int(nd.sum())
+    x = _as_c(xat, np.float64)
+    y = _as_c(yat, np.float64)
+    dz = _as_c(dzdt, np.complex128).view(np.float64)
+    de = _as_c(dens, np.complex128).view(np.float64)
+    hh = _as_c(h, np.float64)
+    gw = np.zeros(2 * n)
+    inform = np.zeros(6, np.int32)
+    ier = np.zeros(5, np.int32)
+    dummy_d = np.zeros(2)
+    dummy_i = np.zeros(1, np.int32)
+    I = lambda v: ctypes.byref(ctypes.c_int32(int(v)))  # noqa: E731
+    D = lambda v: ctypes.byref(ctypes.c_double(float(v)))  # noqa: E731
+    dp = lambda a: a.ctypes.data_as(_dp)  # noqa: E731
+    ip = lambda a: a.ctypes.data_as(_ip)  # noqa: E731
+    rc = fn(I(iflag1), I(iflag2), I(levmax), dp(x), dp(y), dp(dz), I(k0), I(kbod), ip(nd), dp(de), dp(hh),
+            ip(dummy_i), I(leniw), dp(dummy_d), I(lenw), dp(dummy_d), dp(dummy_d), I(0), dp(gw),
+            dp(dummy_d), dp(dummy_d), D(tol), ip(inform), ip(ier), D(0.0), D(0.0))
+    return rc, gw.view(np.complex128), inform, ier
+
+
+def _dropin(name, levmax, xat, yat, dzdt, nd, dens, h, tol, iflag1=0, iflag2=0, leniw=None, lenw=None):
+    """Shared body of cadapdr / cadapmod / cadap.
+
+    LENIW / LENW default to the workspaces FASMVP passes (stokesbubb.f:503, 564-566).
+    Returns (gradw, inform[:2], ier) exactly like the Fortran routine; ier[0] != 0 is an error
+    (the reference's callers STOP on it, stokesbubb.f:573-580)."""
+    if leniw is None:
+        leniw = 1000000
+    if lenw is None:
+        lenw = 1000000
+    rc, gw, inform, ier = _call_dropin(getattr(lib(), name), iflag1, iflag2, levmax, xat, yat, dzdt, nd,
+                                       dens, h, tol, leniw, lenw)
+    if rc >= EB_ERR_UNSUPPORTED:
+        raise EBError(rc, last_error())
+    return gw, inform[:2].copy(), ier.copy()
+
+
+def cadapdr(levmax, xat, yat, dzdt, nd, dens, h, tol, **kw):
+    """CADAPDR (cadapdrop.f:3): GW = phi - z conj(phi') - conj(psi) at the sources."""
+    return _dropin("eb_cadapdr", levmax, xat, yat, dzdt, nd, dens, h, tol, **kw)
+
+
+def cadapmod(levmax, xat, yat, dzdt, nd, dens, h, tol, **kw):
+    """CADAPMOD (cadapmod.f:3): GW = phi + z conj(phi') + conj(psi) at the sources."""
+    return _dropin("eb_cadapmod", levmax, xat, yat, dzdt, nd, dens, h, tol, **kw)
+
+
+def cadap(levmax, xat, yat, dzdt, nd, dens, h, tol, **kw):
+    """CADAP (cadap.f:3): as CADAPMOD with +conj(dens) dz in QB."""
+    return _dropin("eb_cadap", levmax, xat, yat, dzdt, nd, dens, h, tol, **kw)
+
+
+def set_mode(mode: int) -> None:
+    _check(lib().eb_set_mode(mode))
+
+
+# ----------------------------------------------------------------------------------------
+# device-resident plan (torch tensors are used only as device memory)
+# ----------------------------------------------------------------------------------------
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("electricbubbles_amd: no GPU visible (torch.cuda.is_available() is False)")
+    return torch
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        import torch
+        stream = torch.cuda.current_stream()
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+class FMMPlan:
+    """Binned geometry for repeated layer-potential evaluations (one GMRES solve).
+
+    x, y: float64 device tensors (n,); wdz: complex128 device tensor (n,) = DZDT * H.
+    apply(dens) returns GW (complex128, n) on the same stream."""
+
+    def __init__(self, x, y, wdz, levmax, nterms, variant=VARIANT_DR, mode=MODE_FMM, stream=None):
+        torch = _torch()
+        self._torch = torch
+        self.n = int(x.numel())
+        assert x.dtype == torch.float64 and y.dtype == torch.float64 and wdz.dtype == torch.complex128
+        assert x.is_cuda and y.is_cuda and wdz.is_cuda and y.numel() == self.n and wdz.numel() == self.n
+        self._keep = (x.contiguous(), y.contiguous(), wdz.contiguous())
+        self.levmax, self.nterms, self.variant, self.mode = levmax, nterms, variant, mode
+        self.stream = stream
+        h = ctypes.c_void_p()
+        _check(lib().eb_fmm_plan_create(ctypes.byref(h), self.n, _ptr(self._keep[0]), _ptr(self._keep[1]),
+                                        _ptr(self._keep[2]), levmax, nterms, variant, mode,
+                                        _stream_ptr(stream)))
+        self._h = h
+
+    def rebuild(self, x, y, wdz):
+        """Re-bin moved points (same n) in place."""
+        self._keep = (x.contiguous(), y.contiguous(), wdz.contiguous())
+        _check(lib().eb_fmm_plan_rebuild(self._h, _ptr(self._keep[0]), _ptr(self._keep[1]),
+                                         _ptr(self._keep[2]), _stream_ptr(self.stream)))
+
+    def apply(self, dens, out=None):
+        torch = self._torch
+        assert dens.dtype == torch.complex128 and dens.numel() == self.n and dens.is_cuda
+        dens = dens.contiguous()
+        if out is None:
+            out = torch.empty(self.n, dtype=torch.complex128, device=dens.device)
+        _check(lib().eb_fmm_apply(self._h, _ptr(dens), _ptr(out), _stream_ptr(self.stream)))
+        return out
+
+    def info(self):
+        r = np.zeros(3)
+        c = np.zeros(8, np.int64)
+        _check(lib().eb_fmm_plan_info(self._h, r.ctypes.data_as(_dp), c.ctypes.data_as(_i64p)))
+        keys = ("levmax", "nterms", "kmax", "near_pairs", "m2l_pairs", "boxes", "leaves", "max_leaf")
+        d = dict(zip(keys, (int(v) for v in c)))
+        d.update(rscal=float(r[0]), rminx=float(r[1]), rminy=float(r[2]))
+        return d
+
+    def ntbox(self):
+        a = np.zeros(self.levmax + 1, np.int32)
+        _check(lib().eb_fmm_plan_ntbox(self._h, a.ctypes.data_as(_ip)))
+        return a
+
+    def enable_timing(self, on=True):
+        _check(lib().eb_fmm_enable_timing(self._h, 1 if on else 0))
+
+    def stage_ms(self):
+        a = np.zeros(6)
+        _check(lib().eb_fmm_stage_ms(self._h, a.ctypes.data_as(_dp)))
+        return dict(zip(("charges", "p2m", "m2m", "m2l", "l2p", "p2p"), a.tolist()))
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().eb_fmm_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def tables(ilev, x, y, stream=None):
+    """ASSIGN1 tables (cadap2.f:252-409) of level ilev computed on the GPU; 1-based numpy arrays."""
+    torch = _torch()
+    x = x.contiguous()
+    y = y.contiguous()
+    n = x.numel()
+    nb = 4 ** ilev
+    dev = x.device
+    ibox = torch.empty(n, dtype=torch.int32, device=dev)
+    icnt = torch.empty(nb, dtype=torch.int32, device=dev)
+    ioffst = torch.empty(nb + 1, dtype=torch.int32, device=dev)
+    iatadr = torch.empty(n, dtype=torch.int32, device=dev)
+    iloc = torch.empty(nb, dtype=torch.int32, device=dev)
+    ntbox = ctypes.c_int32()
+    nsmax = ctypes.c_int32()
+    _check(lib().eb_cadap_tables(ilev, n, _ptr(x), _ptr(y), _ptr(ibox), _ptr(icnt), _ptr(ioffst), _ptr(iatadr),
+                                 _ptr(iloc), ctypes.byref(ntbox), ctypes.byref(nsmax), _stream_ptr(stream)))
+    return dict(ibox=ibox.cpu().numpy(), icnt=icnt.cpu().numpy(), ioffst=ioffst.cpu().numpy(),
+                iatadr=iatadr.cpu().numpy(), iloc=iloc.cpu().numpy(), ntbox=ntbox.value, nsmax=nsmax.value)
+
+
+def lists(ilev, device="cuda", stream=None):
+    """MKINTL / MKNBOR lists of every box of level ilev (GPU).  Returns (intl list, nbor list)."""
+    torch = _torch()
+    nb = 4 ** ilev
+    intl = torch.zeros(nb * 27, dtype=torch.int32, device=device)
+    nintl = torch.zeros(nb, dtype=torch.int32, device=device)
+    nbor = torch.zeros(nb * 9, dtype=torch.int32, device=device)
+    nnbor = torch.zeros(nb, dtype=torch.int32, device=device)
+    _check(lib().eb_cadap_lists(ilev, _ptr(intl), _ptr(nintl), _ptr(nbor), _ptr(nnbor), _stream_ptr(stream)))
+    intl = intl.cpu().numpy().reshape(nb, 27)
+    nintl = nintl.cpu().numpy()
+    nbor = nbor.cpu().numpy().reshape(nb, 9)
+    nnbor = nnbor.cpu().numpy()
+    return ([intl[b, : nintl[b]].copy() for b in range(nb)], [nbor[b, : nnbor[b]].copy() for b in range(nb)])

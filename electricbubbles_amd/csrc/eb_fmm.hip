@@ -1,0 +1,1119 @@
+// eb_fmm.hip -- MI355X (gfx950) complex-variable FMM for the Stokes layer potential of the
+// reference's CADAPDR family (cadapdrop.f, cadapmod.f, cadap.f; kernels in cadapdrop2.f and
+// cadap2.f), evaluated at the sources (IFLAG1 = 0) in free space (IFLAG2 = 0).
+//
+// Geometry, boxes and lists are the reference's: points are scaled into [0,64]^2 exactly as
+// cadapdrop.f:145-205 does, level l is the uniform 2^l x 2^l grid of ASSIGN1
+// (cadap2.f:292-322), the far field of a box at level l comes from the MKINTL interaction
+// list (cadap2.f:780-828) and the near field from the MKNBOR 3x3 neighbourhood at the
+// finest level (cadap2.f:705-742, cadapdrop2.f:623-722).  The reference re-forms every
+// multipole from the sources at every level and evaluates every local at every level; here
+// the same per-level expansions are produced hierarchically (P2M at the finest level, exact
+// M2M/L2L translations between levels, L2P once), which reproduces the reference up to
+// rounding and truncation.
+//
+// Expansions are stored scaled by the box half-width r of their level
+// (alpha_k = A_k / r^k, lambda_m = L_m r^m), which makes the M2L operator depend on the
+// box offset only, not on the level.  M2L is a real Pascal-matrix product between two
+// diagonal complex scalings (the factorisation of SLSHIFT, cadap2.f:1175-1213).
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+#include <vector>
+
+#include "eb_internal.h"
+
+using namespace eb;
+
+// ------------------------------------------------------------------------------------------
+// error reporting
+// ------------------------------------------------------------------------------------------
+static thread_local char g_err[512];
+extern "C" void eb_set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+extern "C" const char* eb_last_error(void) { return g_err; }
+extern "C" int eb_version(void) { return 1; }
+
+#define EB_MAXLEV 10
+#define EB_MAXKMAX 64
+
+// ------------------------------------------------------------------------------------------
+// device helpers
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ int pidx(int ix, int iy, int W) { return (iy + 3) * W + (ix + 3); }
+
+// [start, end) of the sorted points inside box (ix, iy) of level l, from the leaf table.
+__device__ __forceinline__ void box_range(const int* __restrict__ leaf_start, int L, int l, int ix, int iy,
+                                          int& s, int& e) {
+  const uint32_t m = morton((uint32_t)ix, (uint32_t)iy);
+  const int sh = 2 * (L - l);
+  s = leaf_start[m << sh];
+  e = leaf_start[(m + 1) << sh];
+}
+
+struct Geom {
+  double rscal, rminx, rminy, pad;
+};
+
+// ------------------------------------------------------------------------------------------
+// tree build kernels
+// ------------------------------------------------------------------------------------------
+constexpr int BB_THREADS = 256;
+
+// partial min/max of x and y (cadapdrop.f:147-158)
+__global__ __launch_bounds__(BB_THREADS) void k_bbox_part(const double* __restrict__ x, const double* __restrict__ y,
+                                                          int n, double4* __restrict__ part) {
+  double mnx = x[0], mxx = x[0], mny = y[0], mxy = y[0];
+  for (int i = blockIdx.x * BB_THREADS + threadIdx.x; i < n; i += gridDim.x * BB_THREADS) {
+    double a = x[i], b = y[i];
+    mnx = fmin(mnx, a);
+    mxx = fmax(mxx, a);
+    mny = fmin(mny, b);
+    mxy = fmax(mxy, b);
+  }
+  __shared__ double4 s[BB_THREADS];
+  s[threadIdx.x] = make_double4(mnx, mxx, mny, mxy);
+  __syncthreads();
+  for (int off = BB_THREADS / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) {
+      double4 a = s[threadIdx.x], b = s[threadIdx.x + off];
+      s[threadIdx.x] = make_double4(fmin(a.x, b.x), fmax(a.y, b.y), fmin(a.z, b.z), fmax(a.w, b.w));
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+// RSCAL = 64 / max(RMAXX-RMINX, RMAXY-RMINY)  (cadapdrop.f:165-171)
+__global__ void k_bbox_final(const double4* __restrict__ part, int np, Geom* __restrict__ g) {
+  if (threadIdx.x != 0) return;
+  double4 a = part[0];
+  for (int i = 1; i < np; ++i) {
+    double4 b = part[i];
+    a = make_double4(fmin(a.x, b.x), fmax(a.y, b.y), fmin(a.z, b.z), fmax(a.w, b.w));
+  }
+  double rsx = a.y - a.x, rsy = a.w - a.z;
+  double r = (rsx > rsy) ? rsx : rsy;
+  g->rscal = 64.0 / r;
+  g->rminx = a.x;
+  g->rminy = a.z;
+}
+
+// Scaled coordinates (cadapdrop.f:203-205) and the finest-level Morton key of ASSIGN1's box
+// (cadap2.f:312-319).  H = 64/2^L is a power of two, so X/H is exact and the integer
+// truncation matches the reference bit for bit; coarser levels are key >> 2(L-l).
+__global__ void k_keys(const double* __restrict__ x, const double* __restrict__ y, int n, int L,
+                       const Geom* __restrict__ g, double2* __restrict__ zs0, uint32_t* __restrict__ keys,
+                       int32_t* __restrict__ vals) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double rs = g->rscal;
+  const double X = (x[i] - g->rminx) * rs;
+  const double Y = (y[i] - g->rminy) * rs;
+  zs0[i] = make_double2(X, Y);
+  const int nside = 1 << L;
+  const double h = 64.0 / nside;
+  int ix = (int)(X / h), iy = (int)(Y / h);
+  ix = ix >= nside ? nside - 1 : ix;
+  iy = iy >= nside ? nside - 1 : iy;
+  ix = ix < 0 ? 0 : ix;
+  iy = iy < 0 ? 0 : iy;
+  keys[i] = morton((uint32_t)ix, (uint32_t)iy);
+  vals[i] = i;
+}
+
+// leaf_start[m] = first sorted position whose key is >= m, m = 0..4^L
+__global__ void k_leaf_start(const uint32_t* __restrict__ keys, int n, int nleaf, int* __restrict__ leaf_start) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m > nleaf) return;
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (keys[mid] < (uint32_t)m) lo = mid + 1;
+    else hi = mid;
+  }
+  leaf_start[m] = lo;
+}
+
+__global__ void k_gather(int n, const int32_t* __restrict__ perm, const double2* __restrict__ zs0,
+                         const double2* __restrict__ wdz, double2* __restrict__ zs, double2* __restrict__ wdzs) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int o = perm[i];
+  zs[i] = zs0[o];
+  wdzs[i] = wdz[o];
+}
+
+__global__ void k_zero_int(int* __restrict__ a, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) a[i] = 0;
+}
+
+// number of non-empty boxes per level (the reference's NTBOX, cadap2.f:331-336): a leaf is
+// counted at level l when it holds points and is the first non-empty leaf of its ancestor.
+__global__ void k_ntbox(const int* __restrict__ leaf_start, int L, int* __restrict__ ntbox) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= (1 << (2 * L))) return;
+  const int s = leaf_start[m];
+  if (leaf_start[m + 1] == s) return;
+  for (int l = 1; l <= L; ++l) {
+    const int sh = 2 * (L - l);
+    const int first = (m >> sh) << sh;
+    if (leaf_start[first] == s) atomicAdd(&ntbox[l], 1);
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// apply kernels
+// ------------------------------------------------------------------------------------------
+
+// QA = DENS*DZDT*H/(2 pi i), QB = (sqb*conj(DENS)*DZDT + DENS*conj(DZDT))*H/(2 pi i)
+// (cadapdrop.f:218-220, cadap.f:215), in sorted order.  QC = -conj(z) QA is formed where used.
+__global__ void k_charges(int n, const int32_t* __restrict__ perm, const double2* __restrict__ dens,
+                          const double2* __restrict__ wdzs, double sqb, double2* __restrict__ qa,
+                          double2* __restrict__ qb) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double2 d = dens[perm[i]];
+  const double2 w = wdzs[i];
+  const double c = -0.15915494309189533577;  // -1/(2 pi): 1/(2 pi i) = -i/(2 pi)
+  const double2 dw = cmul(d, w);
+  qa[i] = make_double2(-c * dw.y, c * dw.x);
+  const double2 t = cadd(cscale(cmul(cconj(d), w), sqb), cmul(d, cconj(w)));
+  qb[i] = make_double2(-c * t.y, c * t.x);
+}
+
+// P2M at the finest level (SLMKXP, cadap2.f:444-466), one 64-lane block per leaf box.
+// Sources of a chunk write their powers t^m (t = (z-c)/r) into LDS; lane k then forms
+//   alpha_{k+1} = -(1/r) sum qa t^k,  beta_{k+1} = -(1/r) sum qb t^k + k (1/r^2) sum qc t^(k-1).
+template <int KMAX>
+__global__ __launch_bounds__(64) void k_p2m(int L, int p, const int* __restrict__ leaf_start,
+                                            const double2* __restrict__ zs, const double2* __restrict__ qa,
+                                            const double2* __restrict__ qb, double2* __restrict__ mp, int W,
+                                            int NBP) {
+  constexpr int CH = 32;
+  __shared__ double2 P[CH][KMAX + 1];
+  __shared__ double2 A[CH], B[CH], E[CH];
+  const int nside = 1 << L;
+  const int ix = blockIdx.x % nside, iy = blockIdx.x / nside;
+  const uint32_t mk = morton((uint32_t)ix, (uint32_t)iy);
+  const int s = leaf_start[mk], e = leaf_start[mk + 1];
+  const double h = 64.0 / nside, r = 0.5 * h, ir = 2.0 / h;
+  const double2 c = make_double2((ix + 0.5) * h, (iy + 0.5) * h);
+  const int lane = threadIdx.x;
+  double2 a1 = make_double2(0, 0), a2 = make_double2(0, 0);
+  for (int c0 = s; c0 < e; c0 += CH) {
+    const int nj = min(CH, e - c0);
+    if (lane < nj) {
+      const int j = c0 + lane;
+      const double2 z = zs[j];
+      const double2 t = make_double2((z.x - c.x) * ir, (z.y - c.y) * ir);
+      const double2 q1 = qa[j], q2 = qb[j];
+      A[lane] = make_double2(-q1.x * ir, -q1.y * ir);
+      B[lane] = make_double2(-q2.x * ir, -q2.y * ir);
+      const double2 qc = cmul(make_double2(-z.x, z.y), q1);  // -conj(z) qa
+      E[lane] = make_double2(qc.x * ir * ir, qc.y * ir * ir);
+      double2 pw = make_double2(1.0, 0.0);
+      P[lane][0] = pw;
+#pragma unroll 8
+      for (int k = 1; k < KMAX; ++k) {
+        pw = cmul(pw, t);
+        P[lane][k] = pw;
+      }
+    }
+    __syncthreads();
+    if (lane < KMAX) {
+      const double fk = (double)lane;
+      for (int jj = 0; jj < nj; ++jj) {
+        const double2 pw = P[jj][lane];
+        cfma(a1, A[jj], pw);
+        cfma(a2, B[jj], pw);
+        if (lane > 0) {
+          const double2 t2 = cmul(E[jj], P[jj][lane - 1]);
+          rfma(a2, fk, t2);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (lane < KMAX) {
+    if (lane >= p) {
+      a1 = make_double2(0, 0);
+      a2 = make_double2(0, 0);
+    }
+    const int pb = pidx(ix, iy, W);
+    mp[(size_t)lane * NBP + pb] = a1;
+    mp[(size_t)(KMAX + lane) * NBP + pb] = a2;
+  }
+}
+
+// M2M: parent alpha_{n+1} = sum_c sum_k TM[c][k][n] child alpha_{k+1}, with
+// TM[c][k][n] = C(n,k) 2^-(k+1) delta_c^(n-k), delta_c = (child centre - parent centre)/r_parent.
+template <int KMAX>
+__global__ __launch_bounds__(64) void k_m2m(int L, int l, const int* __restrict__ leaf_start,
+                                            const double2* __restrict__ mpc, int Wc, int NBPc,
+                                            double2* __restrict__ mpp, int Wp, int NBPp,
+                                            const double2* __restrict__ TM) {
+  const int nside = 1 << l;
+  const int ix = blockIdx.x % nside, iy = blockIdx.x / nside;
+  const int lane = threadIdx.x;
+  int s, e;
+  box_range(leaf_start, L, l, ix, iy, s, e);
+  double2 a1 = make_double2(0, 0), a2 = make_double2(0, 0);
+  if (e > s) {
+    for (int c = 0; c < 4; ++c) {
+      const int pc = pidx(2 * ix + (c & 1), 2 * iy + (c >> 1), Wc);
+      const double2* tm = TM + (size_t)c * KMAX * KMAX;
+#pragma unroll 4
+      for (int k = 0; k < KMAX; ++k) {
+        const double2 m1 = mpc[(size_t)k * NBPc + pc];
+        const double2 m2 = mpc[(size_t)(KMAX + k) * NBPc + pc];
+        const double2 w = tm[k * KMAX + (lane < KMAX ? lane : 0)];
+        cfma(a1, w, m1);
+        cfma(a2, w, m2);
+      }
+    }
+  }
+  if (lane < KMAX) {
+    const int pb = pidx(ix, iy, Wp);
+    mpp[(size_t)lane * NBPp + pb] = a1;
+    mpp[(size_t)(KMAX + lane) * NBPp + pb] = a2;
+  }
+}
+
+// M2L (+ L2L from the parent) for every non-empty box of level l.  One thread owns
+// (target box, series s, block of MB local coefficients); blockIdx.y fixes the target's
+// parity class, the series and the coefficient block, so the 27 list offsets, the scaling
+// powers and the Pascal coefficients are wave-uniform (scalar loads):
+//   lambda_m += (-u)^m sum_k C(m+k,k) u^(k+1) alpha_{k+1},   u = r / (c_t - c_s).
+template <int KMAX, int MB>
+__global__ __launch_bounds__(256) void k_m2l(int L, int l, const int* __restrict__ leaf_start,
+                                             const double2* __restrict__ mp, double2* __restrict__ lo, int W,
+                                             int NBP, const double2* __restrict__ lop, int Wp, int NBPp,
+                                             const double* __restrict__ Ct, const double2* __restrict__ U,
+                                             const double2* __restrict__ NU, const double2* __restrict__ WL,
+                                             const int* __restrict__ offs) {
+  constexpr int NB = KMAX / MB;
+  const int cls = blockIdx.y & 3;
+  const int rest = blockIdx.y >> 2;
+  const int s = rest % 2;
+  const int b = rest / 2;
+  const int half = 1 << (l - 1);
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= half * half) return;
+  const int ix = 2 * (t % half) + (cls & 1);
+  const int iy = 2 * (t / half) + (cls >> 1);
+  int bs, be;
+  box_range(leaf_start, L, l, ix, iy, bs, be);
+  if (be == bs) return;
+  const int pt = pidx(ix, iy, W);
+  const int m0 = b * MB;
+  double2 acc[MB];
+#pragma unroll
+  for (int i = 0; i < MB; ++i) acc[i] = make_double2(0, 0);
+  if (lop != nullptr) {
+    // L2L: lambda_m = sum_{m' >= m} C(m',m) delta^(m'-m) 2^-m lambda^parent_m'
+    const int pp = pidx(ix >> 1, iy >> 1, Wp);
+    const double2* wl = WL + (size_t)cls * KMAX * KMAX;
+    for (int mq = m0; mq < KMAX; ++mq) {
+      const double2 a = lop[(size_t)(s * KMAX + mq) * NBPp + pp];
+#pragma unroll
+      for (int i = 0; i < MB; ++i) cfma(acc[i], wl[mq * KMAX + m0 + i], a);
+    }
+  }
+  const double2* mps = mp + (size_t)s * KMAX * NBP;
+  for (int o = 0; o < 27; ++o) {
+    const int dx = offs[(cls * 27 + o) * 2 + 0];
+    const int dy = offs[(cls * 27 + o) * 2 + 1];
+    const int ps = pt + dy * W + dx;
+    const double2* uu = U + (size_t)(cls * 27 + o) * KMAX;
+    double2 yv[MB];
+#pragma unroll
+    for (int i = 0; i < MB; ++i) yv[i] = make_double2(0, 0);
+#pragma unroll 4
+    for (int k = 0; k < KMAX; ++k) {
+      const double2 a = mps[(size_t)k * NBP + ps];
+      const double2 xk = cmul(uu[k], a);
+      const double* ck = Ct + k * KMAX + m0;
+#pragma unroll
+      for (int i = 0; i < MB; ++i) {
+        yv[i].x = fma(ck[i], xk.x, yv[i].x);
+        yv[i].y = fma(ck[i], xk.y, yv[i].y);
+      }
+    }
+    const double2* nu = NU + (size_t)(cls * 27 + o) * KMAX + m0;
+#pragma unroll
+    for (int i = 0; i < MB; ++i) cfma(acc[i], nu[i], yv[i]);
+  }
+  double2* out = lo + (size_t)(s * KMAX + m0) * NBP + pt;
+#pragma unroll
+  for (int i = 0; i < MB; ++i) out[(size_t)i * NBP] = acc[i];
+  (void)NB;
+}
+
+// L2P at the finest level (SLLVALDR, cadapdrop2.f:331-348): one thread per point.
+//   GW_far = phi + sgn (z conj(phi') + conj(psi))
+template <int KMAX>
+__global__ __launch_bounds__(256) void k_l2p(int n, int L, const uint32_t* __restrict__ keys,
+                                             const double2* __restrict__ zs, const double2* __restrict__ lo, int W,
+                                             int NBP, double sgn, double2* __restrict__ gwf) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t key = keys[i];
+  const int ix = (int)compact2(key), iy = (int)compact2(key >> 1);
+  const int nside = 1 << L;
+  const double h = 64.0 / nside, ir = 2.0 / h;
+  const double2 z = zs[i];
+  const double2 u = make_double2((z.x - (ix + 0.5) * h) * ir, (z.y - (iy + 0.5) * h) * ir);
+  const int pb = pidx(ix, iy, W);
+  double2 phi = make_double2(0, 0), dphi = make_double2(0, 0), psi = make_double2(0, 0);
+#pragma unroll 4
+  for (int m = KMAX - 1; m >= 0; --m) {
+    const double2 l1 = lo[(size_t)m * NBP + pb];
+    const double2 l2 = lo[(size_t)(KMAX + m) * NBP + pb];
+    dphi = cadd(cmul(dphi, u), phi);
+    phi = cadd(cmul(phi, u), l1);
+    psi = cadd(cmul(psi, u), l2);
+  }
+  const double2 phip = cscale(dphi, ir);
+  // z conj(phi') + conj(psi)
+  const double2 zc = cmul(z, cconj(phip));
+  const double2 t = make_double2(zc.x + psi.x, zc.y - psi.y);
+  gwf[i] = make_double2(phi.x + sgn * t.x, phi.y + sgn * t.y);
+}
+
+// One source-target pair of the near field, in the translation-invariant form
+//   qa/d + sgn (conj(qb)/conj(d) - conj(qa) d/conj(d)^2),  d = z_src - z_tgt
+// (SLDIRDR / SLSELFDR, cadapdrop2.f:586-599, 741-767, with QC = -conj(z) QA folded in).
+// acc1 collects qa/d, acc2 the bracket.
+__device__ __forceinline__ void p2p_pair(double2 zi, double2 zj, double2 qa, double2 qb, double2& acc1,
+                                         double2& acc2) {
+  const double dx = zj.x - zi.x, dy = zj.y - zi.y;
+  const double ir = rcp_nr(fma(dx, dx, dy * dy));
+  const double wx = dx * ir, wy = dy * ir;  // w = 1/conj(d)
+  // acc1 += qa * conj(w)
+  acc1.x = fma(qa.x, wx, fma(qa.y, wy, acc1.x));
+  acc1.y = fma(qa.y, wx, fma(-qa.x, wy, acc1.y));
+  // e = d * w = d / conj(d)
+  const double ex = fma(dx, wx, -dy * wy);
+  const double ey = fma(dx, wy, dy * wx);
+  // v = conj(qb) - conj(qa) * e
+  const double vx = fma(-qa.x, ex, fma(-qa.y, ey, qb.x));
+  const double vy = fma(-qa.x, ey, fma(qa.y, ex, -qb.y));
+  // acc2 += v * w
+  acc2.x = fma(vx, wx, fma(-vy, wy, acc2.x));
+  acc2.y = fma(vx, wy, fma(vy, wx, acc2.y));
+}
+
+// Near field at the finest level (DONN0DR, cadapdrop2.f:623-722) plus the final scaling
+// GW *= RSCAL (cadapdrop.f:370-373) and the scatter back to the caller's order.
+// One 64-lane block per target leaf; with T < 64 targets the lanes split into G = 64/T
+// source slices per target and the slices are summed through LDS.
+__global__ __launch_bounds__(64) void k_p2p(int L, const int* __restrict__ leaf_start,
+                                            const double2* __restrict__ zs, const double2* __restrict__ qa,
+                                            const double2* __restrict__ qb, const double2* __restrict__ gwf,
+                                            const int32_t* __restrict__ perm, const Geom* __restrict__ g,
+                                            double sgn, double2* __restrict__ out) {
+  const int nside = 1 << L;
+  const int ix = blockIdx.x % nside, iy = blockIdx.x / nside;
+  const uint32_t mk = morton((uint32_t)ix, (uint32_t)iy);
+  const int ts = leaf_start[mk], te = leaf_start[mk + 1];
+  const int T = te - ts;
+  if (T == 0) return;
+  const int lane = threadIdx.x;
+  int sb[9], eb[9];
+  int nb = 0;
+  // MKNBOR order: column-outer, row-inner (cadap2.f:734-740)
+  for (int dx = -1; dx <= 1; ++dx)
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int jx = ix + dx, jy = iy + dy;
+      if (jx < 0 || jy < 0 || jx >= nside || jy >= nside) continue;
+      const uint32_t m = morton((uint32_t)jx, (uint32_t)jy);
+      sb[nb] = leaf_start[m];
+      eb[nb] = leaf_start[m + 1];
+      ++nb;
+    }
+  const double rs = g->rscal;
+  __shared__ double2 red[64];
+  if (T < 64) {
+    const int G = 64 / T;
+    const int tt = lane % T, gs = lane / T;
+    double2 acc1 = make_double2(0, 0), acc2 = make_double2(0, 0);
+    const int i = ts + tt;
+    if (gs < G) {
+      const double2 zi = zs[i];
+      for (int q = 0; q < nb; ++q) {
+        for (int j = sb[q] + gs; j < eb[q]; j += G) {
+          if (j == i) continue;
+          p2p_pair(zi, zs[j], qa[j], qb[j], acc1, acc2);
+        }
+      }
+    }
+    red[lane] = make_double2(acc1.x + sgn * acc2.x, acc1.y + sgn * acc2.y);
+    __syncthreads();
+    if (lane < T) {
+      double2 sum = gwf ? gwf[i] : make_double2(0, 0);
+      for (int q = 0; q < G; ++q) sum = cadd(sum, red[lane + q * T]);
+      out[perm[i]] = make_double2(sum.x * rs, sum.y * rs);
+    }
+  } else {
+    for (int t0 = 0; t0 < T; t0 += 64) {
+      const int i = ts + t0 + lane;
+      if (i >= te) continue;
+      const double2 zi = zs[i];
+      double2 acc1 = make_double2(0, 0), acc2 = make_double2(0, 0);
+      for (int q = 0; q < nb; ++q)
+        for (int j = sb[q]; j < eb[q]; ++j) {
+          if (j == i) continue;
+          p2p_pair(zi, zs[j], qa[j], qb[j], acc1, acc2);
+        }
+      double2 sum = gwf ? gwf[i] : make_double2(0, 0);
+      sum.x += acc1.x + sgn * acc2.x;
+      sum.y += acc1.y + sgn * acc2.y;
+      out[perm[i]] = make_double2(sum.x * rs, sum.y * rs);
+    }
+  }
+}
+
+// O(N^2) direct evaluation of the same kernel (small N), sources staged through LDS.
+__global__ __launch_bounds__(256) void k_direct(int n, const double2* __restrict__ zs, const double2* __restrict__ qa,
+                                                const double2* __restrict__ qb, const int32_t* __restrict__ perm,
+                                                const Geom* __restrict__ g, double sgn, double2* __restrict__ out) {
+  __shared__ double2 sz[256], sa[256], sbq[256];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const double2 zi = i < n ? zs[i] : make_double2(0, 0);
+  double2 acc1 = make_double2(0, 0), acc2 = make_double2(0, 0);
+  for (int t0 = 0; t0 < n; t0 += 256) {
+    const int j = t0 + threadIdx.x;
+    if (j < n) {
+      sz[threadIdx.x] = zs[j];
+      sa[threadIdx.x] = qa[j];
+      sbq[threadIdx.x] = qb[j];
+    }
+    __syncthreads();
+    const int m = min(256, n - t0);
+    if (i < n)
+      for (int jj = 0; jj < m; ++jj) {
+        if (t0 + jj == i) continue;
+        p2p_pair(zi, sz[jj], sa[jj], sbq[jj], acc1, acc2);
+      }
+    __syncthreads();
+  }
+  if (i < n) {
+    const double rs = g->rscal;
+    out[perm[i]] = make_double2((acc1.x + sgn * acc2.x) * rs, (acc1.y + sgn * acc2.y) * rs);
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// host-side tables (long double, then rounded)
+// ------------------------------------------------------------------------------------------
+typedef long double LD;
+struct LC {
+  LD x, y;
+};
+static inline LC lmul(LC a, LC b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+
+struct Tables {
+  std::vector<double> Ct;     // [KMAX][KMAX]   Ct[k*KMAX+m] = C(m+k, k) (k < p, m < p) else 0
+  std::vector<double2> U;     // [4][27][KMAX]  u^(k+1)
+  std::vector<double2> NU;    // [4][27][KMAX]  (-u)^m
+  std::vector<double2> WL;    // [4][KMAX][KMAX] L2L  [c][m'][m]
+  std::vector<double2> TM;    // [4][KMAX][KMAX] M2M  [c][k][n]
+  std::vector<int> offs;      // [4][27][2]
+};
+
+static LD binom(int n, int k) {
+  if (k < 0 || k > n) return 0;
+  LD r = 1;
+  for (int i = 1; i <= k; ++i) r = r * (LD)(n - k + i) / (LD)i;
+  return r;
+}
+
+static void build_tables(int p, int KMAX, Tables& T) {
+  T.Ct.assign((size_t)KMAX * KMAX, 0.0);
+  for (int k = 0; k < p; ++k)
+    for (int m = 0; m < p; ++m) T.Ct[(size_t)k * KMAX + m] = (double)binom(m + k, k);
+  T.U.assign((size_t)4 * 27 * KMAX, make_double2(0, 0));
+  T.NU.assign((size_t)4 * 27 * KMAX, make_double2(0, 0));
+  T.offs.assign(4 * 27 * 2, 0);
+  for (int c = 0; c < 4; ++c) {
+    int dxs[27], dys[27];
+    int no = intl_offsets(c & 1, c >> 1, dxs, dys);
+    for (int o = 0; o < no; ++o) {
+      T.offs[(c * 27 + o) * 2 + 0] = dxs[o];
+      T.offs[(c * 27 + o) * 2 + 1] = dys[o];
+      // z0 = c_t - c_s = -(dx + i dy) * 2r  ->  u = r / z0 = -1 / (2 (dx + i dy))
+      LD den = 2.0L * ((LD)dxs[o] * dxs[o] + (LD)dys[o] * dys[o]);
+      LC u = {-(LD)dxs[o] / den, (LD)dys[o] / den};
+      LC nu = {-u.x, -u.y};
+      LC pw = u, pn = {1, 0};
+      for (int k = 0; k < KMAX; ++k) {
+        T.U[(size_t)(c * 27 + o) * KMAX + k] = make_double2((double)pw.x, (double)pw.y);
+        T.NU[(size_t)(c * 27 + o) * KMAX + k] = make_double2((double)pn.x, (double)pn.y);
+        pw = lmul(pw, u);
+        pn = lmul(pn, nu);
+      }
+    }
+  }
+  T.WL.assign((size_t)4 * KMAX * KMAX, make_double2(0, 0));
+  T.TM.assign((size_t)4 * KMAX * KMAX, make_double2(0, 0));
+  for (int c = 0; c < 4; ++c) {
+    // child centre - parent centre = (+-r_c, +-r_c);  delta = that / r_parent = (+-1 +- i)/2
+    LC d = {(c & 1) ? 0.5L : -0.5L, (c >> 1) ? 0.5L : -0.5L};
+    std::vector<LC> dp(2 * KMAX + 1);
+    dp[0] = {1, 0};
+    for (int i = 1; i <= 2 * KMAX; ++i) dp[i] = lmul(dp[i - 1], d);
+    for (int mq = 0; mq < p; ++mq)
+      for (int m = 0; m <= mq; ++m) {
+        LD s = binom(mq, m) * powl(0.5L, (LD)m);
+        LC v = {s * dp[mq - m].x, s * dp[mq - m].y};
+        T.WL[((size_t)c * KMAX + mq) * KMAX + m] = make_double2((double)v.x, (double)v.y);
+      }
+    for (int k = 0; k < p; ++k)
+      for (int n = k; n < p; ++n) {
+        LD s = binom(n, k) * powl(0.5L, (LD)(k + 1));
+        LC v = {s * dp[n - k].x, s * dp[n - k].y};
+        T.TM[((size_t)c * KMAX + k) * KMAX + n] = make_double2((double)v.x, (double)v.y);
+      }
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// plan
+// ------------------------------------------------------------------------------------------
+struct eb_fmm_plan_s {
+  int n = 0, L = 1, p = 0, kmax = 0, variant = 0, mode = EB_MODE_FMM;
+  double sgn = -1.0, sqb = -1.0;
+  hipStream_t st = nullptr;
+  // device
+  Geom* geom = nullptr;
+  double4* bbpart = nullptr;
+  double2* zs0 = nullptr;
+  double2* zs = nullptr;
+  double2* wdzs = nullptr;
+  double2* qa = nullptr;
+  double2* qb = nullptr;
+  double2* gwf = nullptr;
+  uint32_t* keys = nullptr;
+  uint32_t* keys2 = nullptr;
+  int32_t* perm = nullptr;
+  int32_t* vals2 = nullptr;
+  int* sorttmp = nullptr;
+  int* leaf_start = nullptr;
+  int* ntbox = nullptr;
+  double2* mp = nullptr;
+  double2* lo = nullptr;
+  size_t lvl_off[EB_MAXLEV + 2] = {0};
+  int lvl_W[EB_MAXLEV + 2] = {0};
+  int lvl_NBP[EB_MAXLEV + 2] = {0};
+  double* dCt = nullptr;
+  double2 *dU = nullptr, *dNU = nullptr, *dWL = nullptr, *dTM = nullptr;
+  int* doffs = nullptr;
+  // timing
+  bool timing = false;
+  hipEvent_t ev[7] = {};
+  double stage_ms[6] = {0};
+};
+
+static void plan_free(eb_fmm_plan P) {
+  if (!P) return;
+  void* ptrs[] = {P->geom, P->bbpart, P->zs0, P->zs, P->wdzs, P->qa, P->qb, P->gwf, P->keys, P->keys2,
+                  P->perm, P->vals2, P->sorttmp, P->leaf_start, P->ntbox, P->mp, P->lo, P->dCt,
+                  P->dU, P->dNU, P->dWL, P->dTM, P->doffs};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+  for (auto& e : P->ev)
+    if (e) (void)hipEventDestroy(e);
+  delete P;
+}
+
+template <typename T>
+static int dmalloc(T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc((void**)p, sizeof(T) * count);
+  if (e != hipSuccess) {
+    eb_set_error("hipMalloc(%zu bytes) failed: %s", sizeof(T) * count, hipGetErrorString(e));
+    return EB_ERR_NOMEM;
+  }
+  return EB_OK;
+}
+
+static int pick_kmax(int p) {
+  if (p <= 16) return 16;
+  if (p <= 32) return 32;
+  if (p <= 48) return 48;
+  return 64;
+}
+
+#define EB_TRY(x)            \
+  do {                       \
+    int _rc = (x);           \
+    if (_rc != EB_OK) {      \
+      plan_free(P);          \
+      return _rc;            \
+    }                        \
+  } while (0)
+
+// Bin the points (tree build): bounding box and RSCAL, scaled coordinates and finest-level
+// keys, stable sort by key, leaf table, gather into sorted order, non-empty counts per level.
+static int build_tree(eb_fmm_plan P, const double* x, const double* y, const double* wdz) {
+  hipStream_t st = P->st;
+  const int n = P->n, L = P->L;
+  const int nleaf = 1 << (2 * L);
+  const int bbblocks = std::min(1024, (n + BB_THREADS - 1) / BB_THREADS);
+  hipLaunchKernelGGL(k_bbox_part, dim3(bbblocks), dim3(BB_THREADS), 0, st, x, y, n, P->bbpart);
+  hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(64), 0, st, P->bbpart, bbblocks, P->geom);
+  hipLaunchKernelGGL(k_keys, dim3((n + 255) / 256), dim3(256), 0, st, x, y, n, L, P->geom, P->zs0, P->keys,
+                     P->perm);
+  EB_HIP_CHECK(hipGetLastError());
+  int rc = radix_sort_pairs(P->keys, P->perm, P->keys2, P->vals2, n, 2 * L, P->sorttmp, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_leaf_start, dim3((nleaf + 1 + 255) / 256), dim3(256), 0, st, P->keys, n, nleaf,
+                     P->leaf_start);
+  hipLaunchKernelGGL(k_gather, dim3((n + 255) / 256), dim3(256), 0, st, n, P->perm, P->zs0,
+                     (const double2*)wdz, P->zs, P->wdzs);
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, P->ntbox, EB_MAXLEV + 2);
+  hipLaunchKernelGGL(k_ntbox, dim3((nleaf + 255) / 256), dim3(256), 0, st, P->leaf_start, L, P->ntbox);
+  EB_HIP_CHECK(hipGetLastError());
+  return EB_OK;
+}
+
+extern "C" int eb_fmm_plan_create(eb_fmm_plan* out, int32_t n, const double* x, const double* y,
+                                  const double* wdz, int32_t levmax, int32_t nterms, int32_t variant,
+                                  int32_t mode, void* stream) {
+  if (!out) {
+    eb_set_error("eb_fmm_plan_create: null plan pointer");
+    return EB_ERR_ARG;
+  }
+  *out = nullptr;
+  if (n < 2 || !x || !y || !wdz) {
+    eb_set_error("eb_fmm_plan_create: need n >= 2 points and non-null x, y, wdz (n=%d)", n);
+    return EB_ERR_ARG;
+  }
+  if (levmax < 1 || levmax > EB_MAXLEV) {
+    eb_set_error("eb_fmm_plan_create: levmax=%d outside 1..%d", levmax, EB_MAXLEV);
+    return EB_ERR_ARG;
+  }
+  if (nterms < 1 || nterms > 60) {
+    eb_set_error("eb_fmm_plan_create: nterms=%d outside 1..60", nterms);
+    return EB_ERR_TERMS;
+  }
+  if (variant < 0 || variant > 2 || (mode != EB_MODE_FMM && mode != EB_MODE_DIRECT)) {
+    eb_set_error("eb_fmm_plan_create: bad variant %d or mode %d", variant, mode);
+    return EB_ERR_ARG;
+  }
+  eb_fmm_plan P = new eb_fmm_plan_s();
+  P->n = n;
+  P->L = levmax;
+  P->p = nterms;
+  P->kmax = pick_kmax(nterms);
+  P->variant = variant;
+  P->mode = mode;
+  P->sgn = (variant == EB_VARIANT_DR) ? -1.0 : 1.0;
+  P->sqb = (variant == EB_VARIANT_PLUS) ? 1.0 : -1.0;
+  P->st = (hipStream_t)stream;
+  hipStream_t st = P->st;
+  const int L = P->L, K = P->kmax;
+  const int nleaf = 1 << (2 * L);
+  const int bbblocks = std::min(1024, (n + BB_THREADS - 1) / BB_THREADS);
+
+  EB_TRY(dmalloc(&P->geom, 1));
+  EB_TRY(dmalloc(&P->bbpart, bbblocks));
+  EB_TRY(dmalloc(&P->zs0, n));
+  EB_TRY(dmalloc(&P->zs, n));
+  EB_TRY(dmalloc(&P->wdzs, n));
+  EB_TRY(dmalloc(&P->qa, n));
+  EB_TRY(dmalloc(&P->qb, n));
+  EB_TRY(dmalloc(&P->gwf, n));
+  EB_TRY(dmalloc(&P->keys, n));
+  EB_TRY(dmalloc(&P->keys2, n));
+  EB_TRY(dmalloc(&P->perm, n));
+  EB_TRY(dmalloc(&P->vals2, n));
+  EB_TRY(dmalloc(&P->sorttmp, radix_sort_temp_ints(n)));
+  EB_TRY(dmalloc(&P->leaf_start, nleaf + 1));
+  EB_TRY(dmalloc(&P->ntbox, EB_MAXLEV + 2));
+
+  EB_TRY(build_tree(P, x, y, wdz));
+
+  if (mode == EB_MODE_FMM && L >= 2) {
+    size_t off = 0;
+    for (int l = 2; l <= L; ++l) {
+      const int W = (1 << l) + 6;
+      P->lvl_W[l] = W;
+      P->lvl_NBP[l] = W * W;
+      P->lvl_off[l] = off;
+      off += (size_t)2 * K * W * W;
+    }
+    EB_TRY(dmalloc(&P->mp, off));
+    EB_TRY(dmalloc(&P->lo, off));
+    // the 3-box halo of every level must read as zero multipoles
+    EB_TRY((hipMemsetAsync(P->mp, 0, sizeof(double2) * off, st) == hipSuccess)
+               ? EB_OK
+               : (eb_set_error("hipMemsetAsync failed"), EB_ERR_HIP));
+    Tables T;
+    build_tables(P->p, K, T);
+    EB_TRY(dmalloc(&P->dCt, T.Ct.size()));
+    EB_TRY(dmalloc(&P->dU, T.U.size()));
+    EB_TRY(dmalloc(&P->dNU, T.NU.size()));
+    EB_TRY(dmalloc(&P->dWL, T.WL.size()));
+    EB_TRY(dmalloc(&P->dTM, T.TM.size()));
+    EB_TRY(dmalloc(&P->doffs, T.offs.size()));
+    bool ok = hipMemcpyAsync(P->dCt, T.Ct.data(), sizeof(double) * T.Ct.size(), hipMemcpyHostToDevice, st) == hipSuccess &&
+              hipMemcpyAsync(P->dU, T.U.data(), sizeof(double2) * T.U.size(), hipMemcpyHostToDevice, st) == hipSuccess &&
+              hipMemcpyAsync(P->dNU, T.NU.data(), sizeof(double2) * T.NU.size(), hipMemcpyHostToDevice, st) == hipSuccess &&
+              hipMemcpyAsync(P->dWL, T.WL.data(), sizeof(double2) * T.WL.size(), hipMemcpyHostToDevice, st) == hipSuccess &&
+              hipMemcpyAsync(P->dTM, T.TM.data(), sizeof(double2) * T.TM.size(), hipMemcpyHostToDevice, st) == hipSuccess &&
+              hipMemcpyAsync(P->doffs, T.offs.data(), sizeof(int) * T.offs.size(), hipMemcpyHostToDevice, st) == hipSuccess;
+    // the host vectors die at scope end: wait for the copies
+    ok = ok && hipStreamSynchronize(st) == hipSuccess;
+    if (!ok) {
+      eb_set_error("eb_fmm_plan_create: table upload failed");
+      plan_free(P);
+      return EB_ERR_HIP;
+    }
+  }
+  *out = P;
+  return EB_OK;
+}
+
+template <int K>
+static void launch_far(eb_fmm_plan P) {
+  hipStream_t st = P->st;
+  const int L = P->L;
+  constexpr int MB = 16;
+  if (P->timing) (void)hipEventRecord(P->ev[1], st);
+  // P2M at the finest level
+  hipLaunchKernelGGL(k_p2m<K>, dim3(1 << (2 * L)), dim3(64), 0, st, L, P->p, P->leaf_start, P->zs, P->qa, P->qb,
+                     P->mp + P->lvl_off[L], P->lvl_W[L], P->lvl_NBP[L]);
+  if (P->timing) (void)hipEventRecord(P->ev[2], st);
+  for (int l = L - 1; l >= 2; --l)
+    hipLaunchKernelGGL(k_m2m<K>, dim3(1 << (2 * l)), dim3(64), 0, st, L, l, P->leaf_start, P->mp + P->lvl_off[l + 1],
+                       P->lvl_W[l + 1], P->lvl_NBP[l + 1], P->mp + P->lvl_off[l], P->lvl_W[l], P->lvl_NBP[l], P->dTM);
+  if (P->timing) (void)hipEventRecord(P->ev[3], st);
+  for (int l = 2; l <= L; ++l) {
+    const int half = 1 << (l - 1);
+    const int nt = half * half;
+    dim3 grid((nt + 255) / 256, 4 * 2 * (K / MB));
+    const double2* lop = (l > 2) ? P->lo + P->lvl_off[l - 1] : nullptr;
+    hipLaunchKernelGGL((k_m2l<K, MB>), grid, dim3(256), 0, st, L, l, P->leaf_start, P->mp + P->lvl_off[l],
+                       P->lo + P->lvl_off[l], P->lvl_W[l], P->lvl_NBP[l], lop, (l > 2) ? P->lvl_W[l - 1] : 0,
+                       (l > 2) ? P->lvl_NBP[l - 1] : 0, P->dCt, P->dU, P->dNU, P->dWL, P->doffs);
+  }
+  if (P->timing) (void)hipEventRecord(P->ev[4], st);
+  hipLaunchKernelGGL(k_l2p<K>, dim3((P->n + 255) / 256), dim3(256), 0, st, P->n, L, P->keys, P->zs,
+                     P->lo + P->lvl_off[L], P->lvl_W[L], P->lvl_NBP[L], P->sgn, P->gwf);
+  if (P->timing) (void)hipEventRecord(P->ev[5], st);
+}
+
+extern "C" int eb_fmm_apply(eb_fmm_plan P, const double* dens, double* gradw, void* stream) {
+  if (!P || !dens || !gradw) {
+    eb_set_error("eb_fmm_apply: null argument");
+    return EB_ERR_ARG;
+  }
+  if (stream) P->st = (hipStream_t)stream;
+  hipStream_t st = P->st;
+  const int n = P->n;
+  if (P->timing) {
+    for (auto& e : P->ev)
+      if (!e) EB_HIP_CHECK(hipEventCreate(&e));
+    EB_HIP_CHECK(hipEventRecord(P->ev[0], st));
+  }
+  hipLaunchKernelGGL(k_charges, dim3((n + 255) / 256), dim3(256), 0, st, n, P->perm, (const double2*)dens, P->wdzs,
+                     P->sqb, P->qa, P->qb);
+  if (P->mode == EB_MODE_DIRECT) {
+    if (P->timing)
+      for (int i = 1; i <= 5; ++i) (void)hipEventRecord(P->ev[i], st);
+    hipLaunchKernelGGL(k_direct, dim3((n + 255) / 256), dim3(256), 0, st, n, P->zs, P->qa, P->qb, P->perm, P->geom,
+                       P->sgn, (double2*)gradw);
+  } else {
+    const bool far = P->L >= 2;
+    if (far) {
+      switch (P->kmax) {
+        case 16: launch_far<16>(P); break;
+        case 32: launch_far<32>(P); break;
+        case 48: launch_far<48>(P); break;
+        default: launch_far<64>(P); break;
+      }
+    } else if (P->timing) {
+      for (int i = 1; i <= 5; ++i) (void)hipEventRecord(P->ev[i], st);
+    }
+    hipLaunchKernelGGL(k_p2p, dim3(1 << (2 * P->L)), dim3(64), 0, st, P->L, P->leaf_start, P->zs, P->qa, P->qb,
+                       far ? P->gwf : nullptr, P->perm, P->geom, P->sgn, (double2*)gradw);
+  }
+  EB_HIP_CHECK(hipGetLastError());
+  if (P->timing) {
+    EB_HIP_CHECK(hipEventRecord(P->ev[6], st));
+    EB_HIP_CHECK(hipEventSynchronize(P->ev[6]));
+    for (int i = 0; i < 6; ++i) {
+      float ms = 0;
+      EB_HIP_CHECK(hipEventElapsedTime(&ms, P->ev[i], P->ev[i + 1]));
+      P->stage_ms[i] = ms;
+    }
+  }
+  return EB_OK;
+}
+
+extern "C" int eb_fmm_plan_rebuild(eb_fmm_plan P, const double* x, const double* y, const double* wdz,
+                                   void* stream) {
+  if (!P || !x || !y || !wdz) {
+    eb_set_error("eb_fmm_plan_rebuild: null argument");
+    return EB_ERR_ARG;
+  }
+  if (stream) P->st = (hipStream_t)stream;
+  return build_tree(P, x, y, wdz);
+}
+
+extern "C" int eb_fmm_plan_destroy(eb_fmm_plan P) {
+  if (P && P->st) (void)hipStreamSynchronize(P->st);
+  plan_free(P);
+  return EB_OK;
+}
+
+extern "C" int eb_fmm_enable_timing(eb_fmm_plan P, int32_t on) {
+  if (!P) return EB_ERR_ARG;
+  P->timing = on != 0;
+  return EB_OK;
+}
+
+extern "C" int eb_fmm_stage_ms(eb_fmm_plan P, double* ms6) {
+  if (!P || !ms6) return EB_ERR_ARG;
+  for (int i = 0; i < 6; ++i) ms6[i] = P->stage_ms[i];
+  return EB_OK;
+}
+
+extern "C" int eb_fmm_plan_info(eb_fmm_plan P, double* reals3, int64_t* counts8) {
+  if (!P) return EB_ERR_ARG;
+  EB_HIP_CHECK(hipStreamSynchronize(P->st));
+  Geom g;
+  EB_HIP_CHECK(hipMemcpy(&g, P->geom, sizeof(Geom), hipMemcpyDeviceToHost));
+  if (reals3) {
+    reals3[0] = g.rscal;
+    reals3[1] = g.rminx;
+    reals3[2] = g.rminy;
+  }
+  if (!counts8) return EB_OK;
+  const int L = P->L, nside = 1 << L;
+  std::vector<int> ls((size_t)(1 << (2 * L)) + 1);
+  EB_HIP_CHECK(hipMemcpy(ls.data(), P->leaf_start, sizeof(int) * ls.size(), hipMemcpyDeviceToHost));
+  auto cnt = [&](int l, int ix, int iy) -> long long {
+    if (ix < 0 || iy < 0 || ix >= (1 << l) || iy >= (1 << l)) return 0;
+    uint32_t m = morton((uint32_t)ix, (uint32_t)iy);
+    int sh = 2 * (L - l);
+    return ls[(size_t)(m + 1) << sh] - ls[(size_t)m << sh];
+  };
+  long long near = 0, maxleaf = 0, nonempty_leaf = 0;
+  for (int iy = 0; iy < nside; ++iy)
+    for (int ix = 0; ix < nside; ++ix) {
+      long long t = cnt(L, ix, iy);
+      if (!t) continue;
+      ++nonempty_leaf;
+      maxleaf = std::max(maxleaf, t);
+      long long s = 0;
+      for (int dx = -1; dx <= 1; ++dx)
+        for (int dy = -1; dy <= 1; ++dy) s += cnt(L, ix + dx, iy + dy);
+      near += t * s - t;
+    }
+  long long m2l = 0, boxes = 0;
+  if (P->mode == EB_MODE_FMM)
+    for (int l = 2; l <= L; ++l) {
+      const int ns = 1 << l;
+      for (int iy = 0; iy < ns; ++iy)
+        for (int ix = 0; ix < ns; ++ix) {
+          if (!cnt(l, ix, iy)) continue;
+          ++boxes;
+          int dxs[27], dys[27];
+          int no = intl_offsets(ix & 1, iy & 1, dxs, dys);
+          for (int o = 0; o < no; ++o)
+            if (cnt(l, ix + dxs[o], iy + dys[o])) ++m2l;
+        }
+    }
+  if (P->mode == EB_MODE_DIRECT) near = (long long)P->n * (P->n - 1);
+  counts8[0] = L;
+  counts8[1] = P->p;
+  counts8[2] = P->kmax;
+  counts8[3] = near;
+  counts8[4] = m2l;
+  counts8[5] = boxes;
+  counts8[6] = nonempty_leaf;
+  counts8[7] = maxleaf;
+  return EB_OK;
+}
+
+// per-level non-empty box counts (levels 1..levmax) of a plan, host copy
+extern "C" int eb_fmm_plan_ntbox(eb_fmm_plan P, int32_t* ntbox) {
+  if (!P || !ntbox) return EB_ERR_ARG;
+  EB_HIP_CHECK(hipMemcpyAsync(ntbox, P->ntbox, sizeof(int) * (P->L + 1), hipMemcpyDeviceToHost, P->st));
+  EB_HIP_CHECK(hipStreamSynchronize(P->st));
+  return EB_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// reference tables on the GPU (parity interface)
+// ------------------------------------------------------------------------------------------
+__global__ void k_tab_box(const double* __restrict__ x, const double* __restrict__ y, int n, int l,
+                          const Geom* __restrict__ g, int32_t* __restrict__ ibox, int* __restrict__ icnt,
+                          int* __restrict__ first, uint32_t* __restrict__ keys, int32_t* __restrict__ vals) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double rs = g->rscal;
+  const double X = (x[i] - g->rminx) * rs;
+  const double Y = (y[i] - g->rminy) * rs;
+  const int nside = 1 << l;
+  const double h = 64.0 / nside;
+  int ix = (int)(X / h), iy = (int)(Y / h);
+  ix = ix >= nside ? nside - 1 : ix;
+  iy = iy >= nside ? nside - 1 : iy;
+  ix = ix < 0 ? 0 : ix;
+  iy = iy < 0 ? 0 : iy;
+  const int b = iy * nside + ix;  // 0-based row-major (cadap2.f:320-322)
+  ibox[i] = b + 1;
+  atomicAdd(&icnt[b], 1);
+  atomicMin(&first[b], i);
+  keys[i] = (uint32_t)b;
+  vals[i] = i;
+}
+
+__global__ void k_tab_flags(const int32_t* __restrict__ ibox, const int* __restrict__ first, int n,
+                            int* __restrict__ flag) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  flag[i] = (first[ibox[i] - 1] == i) ? 1 : 0;
+}
+
+__global__ void k_tab_iloc(const int32_t* __restrict__ ibox, const int* __restrict__ flag,
+                           const int* __restrict__ rank, int n, int32_t* __restrict__ iloc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (flag[i]) iloc[ibox[i] - 1] = rank[i] + 1;
+}
+
+__global__ void k_tab_final(const int32_t* __restrict__ vals, int n, int32_t* __restrict__ iatadr) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  iatadr[i] = vals[i] + 1;
+}
+
+__global__ void k_max(const int* __restrict__ a, int n, int* __restrict__ out) {
+  int m = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) m = max(m, a[i]);
+  atomicMax(out, m);
+}
+
+extern "C" int eb_cadap_tables(int32_t ilev, int32_t n, const double* x, const double* y, int32_t* ibox,
+                               int32_t* icnt, int32_t* ioffst, int32_t* iatadr, int32_t* iloc, int32_t* ntbox,
+                               int32_t* nsmax, void* stream) {
+  if (ilev < 1 || ilev > 15 || n < 1) {
+    eb_set_error("eb_cadap_tables: ilev=%d n=%d", ilev, n);
+    return EB_ERR_ARG;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  const int nb = 1 << (2 * ilev);
+  const int bbblocks = std::min(1024, (n + BB_THREADS - 1) / BB_THREADS);
+  Geom* g = nullptr;
+  double4* part = nullptr;
+  int *first = nullptr, *flag = nullptr, *rank = nullptr, *tmp = nullptr, *dmax = nullptr;
+  uint32_t *k0 = nullptr, *k1 = nullptr;
+  int32_t *v0 = nullptr, *v1 = nullptr;
+  int rc = EB_OK;
+  auto cleanup = [&]() {
+    void* ptrs[] = {g, part, first, flag, rank, tmp, dmax, k0, k1, v0, v1};
+    for (void* q : ptrs)
+      if (q) (void)hipFree(q);
+  };
+  if ((rc = dmalloc(&g, 1)) || (rc = dmalloc(&part, bbblocks)) || (rc = dmalloc(&first, nb)) ||
+      (rc = dmalloc(&flag, n)) || (rc = dmalloc(&rank, n)) || (rc = dmalloc(&tmp, radix_sort_temp_ints(n))) ||
+      (rc = dmalloc(&dmax, 1)) || (rc = dmalloc(&k0, n)) || (rc = dmalloc(&k1, n)) || (rc = dmalloc(&v0, n)) ||
+      (rc = dmalloc(&v1, n))) {
+    cleanup();
+    return rc;
+  }
+  hipLaunchKernelGGL(k_bbox_part, dim3(bbblocks), dim3(BB_THREADS), 0, st, x, y, n, part);
+  hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(64), 0, st, part, bbblocks, g);
+  bool ok = hipMemsetAsync(icnt, 0, sizeof(int) * nb, st) == hipSuccess &&
+            hipMemsetAsync(iloc, 0, sizeof(int) * nb, st) == hipSuccess &&
+            hipMemsetAsync(first, 0x7f, sizeof(int) * nb, st) == hipSuccess &&
+            hipMemsetAsync(dmax, 0, sizeof(int), st) == hipSuccess;
+  if (!ok) {
+    cleanup();
+    eb_set_error("eb_cadap_tables: memset failed");
+    return EB_ERR_HIP;
+  }
+  const int gb = (n + 255) / 256;
+  hipLaunchKernelGGL(k_tab_box, dim3(gb), dim3(256), 0, st, x, y, n, ilev, g, ibox, icnt, first, k0, v0);
+  if ((rc = scan_exclusive(icnt, ioffst, nb, true, st)) ||
+      (rc = radix_sort_pairs(k0, v0, k1, v1, n, 2 * ilev, tmp, st))) {
+    cleanup();
+    return rc;
+  }
+  hipLaunchKernelGGL(k_tab_final, dim3(gb), dim3(256), 0, st, v0, n, iatadr);
+  hipLaunchKernelGGL(k_tab_flags, dim3(gb), dim3(256), 0, st, ibox, first, n, flag);
+  if ((rc = scan_exclusive(flag, rank, n, false, st))) {
+    cleanup();
+    return rc;
+  }
+  hipLaunchKernelGGL(k_tab_iloc, dim3(gb), dim3(256), 0, st, ibox, flag, rank, n, iloc);
+  hipLaunchKernelGGL(k_max, dim3(std::min(1024, (nb + 255) / 256)), dim3(256), 0, st, icnt, nb, dmax);
+  int last_rank = 0, last_flag = 0, hmax = 0;
+  ok = hipGetLastError() == hipSuccess &&
+       hipMemcpyAsync(&last_rank, rank + (n - 1), sizeof(int), hipMemcpyDeviceToHost, st) == hipSuccess &&
+       hipMemcpyAsync(&last_flag, flag + (n - 1), sizeof(int), hipMemcpyDeviceToHost, st) == hipSuccess &&
+       hipMemcpyAsync(&hmax, dmax, sizeof(int), hipMemcpyDeviceToHost, st) == hipSuccess &&
+       hipStreamSynchronize(st) == hipSuccess;
+  cleanup();
+  if (!ok) {
+    eb_set_error("eb_cadap_tables: kernel or copy failed");
+    return EB_ERR_HIP;
+  }
+  if (ntbox) *ntbox = last_rank + last_flag;
+  if (nsmax) *nsmax = hmax;
+  return EB_OK;
+}
+
+// MKINTL / MKNBOR for every box of a level, with the domain clipping of cadap2.f:727-730,801-815
+__global__ void k_lists(int l, int32_t* __restrict__ intl, int32_t* __restrict__ nintl, int32_t* __restrict__ nbor,
+                        int32_t* __restrict__ nnbor) {
+  const int nside = 1 << l;
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nside * nside) return;
+  const int ix = b % nside, iy = b / nside;
+  int dxs[27], dys[27];
+  const int no = intl_offsets(ix & 1, iy & 1, dxs, dys);
+  int c = 0;
+  for (int o = 0; o < no; ++o) {
+    const int jx = ix + dxs[o], jy = iy + dys[o];
+    if (jx < 0 || jy < 0 || jx >= nside || jy >= nside) continue;
+    intl[(size_t)b * 27 + c++] = jy * nside + jx + 1;
+  }
+  nintl[b] = c;
+  c = 0;
+  for (int dx = -1; dx <= 1; ++dx)
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int jx = ix + dx, jy = iy + dy;
+      if (jx < 0 || jy < 0 || jx >= nside || jy >= nside) continue;
+      nbor[(size_t)b * 9 + c++] = jy * nside + jx + 1;
+    }
+  nnbor[b] = c;
+}
+
+extern "C" int eb_cadap_lists(int32_t ilev, int32_t* intl, int32_t* nintl, int32_t* nbor, int32_t* nnbor,
+                              void* stream) {
+  if (ilev < 1 || ilev > 12) {
+    eb_set_error("eb_cadap_lists: ilev=%d", ilev);
+    return EB_ERR_ARG;
+  }
+  const int nb = 1 << (2 * ilev);
+  hipLaunchKernelGGL(k_lists, dim3((nb + 255) / 256), dim3(256), 0, (hipStream_t)stream, ilev, intl, nintl, nbor,
+                     nnbor);
+  EB_HIP_CHECK(hipGetLastError());
+  EB_HIP_CHECK(hipStreamSynchronize((hipStream_t)stream));
+  return EB_OK;
+}

@@ -1,0 +1,158 @@
+// eb_sort.hip -- stable LSD radix sort of (uint32 key, int32 value) pairs and a single-block
+// exclusive scan, both hand-written for gfx950 (wave64).  Used by the tree build: stability
+// is what makes the per-box source order equal to the reference's source-index order
+// (ASSIGN1, cadap2.f:346-353).
+#include "eb_internal.h"
+
+namespace eb {
+
+constexpr int RS_THREADS = 256;
+constexpr int RS_ITEMS = 16;                  // elements per thread per pass
+constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
+constexpr int RS_BITS = 8;
+constexpr int RS_RADIX = 1 << RS_BITS;
+
+__global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint32_t* __restrict__ keys, int n, int shift,
+                                                        int nblk, int* __restrict__ hist) {
+  __shared__ int h[RS_RADIX];
+  for (int d = threadIdx.x; d < RS_RADIX; d += RS_THREADS) h[d] = 0;
+  __syncthreads();
+  const int base = blockIdx.x * RS_TILE;
+  for (int r = 0; r < RS_ITEMS; ++r) {
+    int e = base + r * RS_THREADS + threadIdx.x;
+    if (e < n) atomicAdd(&h[(keys[e] >> shift) & (RS_RADIX - 1)], 1);
+  }
+  __syncthreads();
+  for (int d = threadIdx.x; d < RS_RADIX; d += RS_THREADS) hist[d * nblk + blockIdx.x] = h[d];
+}
+
+// Each block re-reads its tile in order, RS_THREADS elements per round; within a round the
+// rank among equal digits is found with wave ballots (64-lane match) plus per-wave counts.
+__global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint32_t* __restrict__ kin,
+                                                           const int32_t* __restrict__ vin,
+                                                           uint32_t* __restrict__ kout,
+                                                           int32_t* __restrict__ vout, int n, int shift,
+                                                           int nblk, const int* __restrict__ hscan) {
+  constexpr int NW = RS_THREADS / EB_WAVE;
+  __shared__ int base[RS_RADIX];
+  __shared__ int wcnt[NW][RS_RADIX];
+  const int lane = threadIdx.x & (EB_WAVE - 1);
+  const int wid = threadIdx.x / EB_WAVE;
+  for (int d = threadIdx.x; d < RS_RADIX; d += RS_THREADS) base[d] = hscan[d * nblk + blockIdx.x];
+  const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+  const int tile = blockIdx.x * RS_TILE;
+  for (int r = 0; r < RS_ITEMS; ++r) {
+    for (int d = threadIdx.x; d < NW * RS_RADIX; d += RS_THREADS) (&wcnt[0][0])[d] = 0;
+    __syncthreads();
+    const int e = tile + r * RS_THREADS + threadIdx.x;
+    const bool valid = e < n;
+    uint32_t k = 0;
+    int32_t v = 0;
+    int dig = 0;
+    if (valid) {
+      k = kin[e];
+      v = vin[e];
+      dig = (k >> shift) & (RS_RADIX - 1);
+    }
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < RS_BITS; ++b) {
+      const bool bit = (dig >> b) & 1;
+      const uint64_t m = __ballot(bit);
+      peers &= bit ? m : ~m;
+    }
+    const int rank = __popcll(peers & lt);
+    if (valid && rank == 0) wcnt[wid][dig] = __popcll(peers);
+    __syncthreads();
+    if (valid) {
+      int off = base[dig] + rank;
+      for (int w = 0; w < wid; ++w) off += wcnt[w][dig];
+      kout[off] = k;
+      vout[off] = v;
+    }
+    __syncthreads();
+    for (int d = threadIdx.x; d < RS_RADIX; d += RS_THREADS) {
+      int t = 0;
+#pragma unroll
+      for (int w = 0; w < NW; ++w) t += wcnt[w][d];
+      base[d] += t;
+    }
+    __syncthreads();
+  }
+}
+
+// Exclusive scan of n ints in one block of 1024 threads; out[n] receives the total when
+// out has n+1 entries (write_total).
+__global__ __launch_bounds__(1024) void k_scan1(const int* __restrict__ in, int* __restrict__ out, int n,
+                                               int write_total) {
+  __shared__ int part[1024];
+  const int t = threadIdx.x;
+  const int chunk = (n + 1023) / 1024;
+  const int s = t * chunk;
+  const int e = min(n, s + chunk);
+  int sum = 0;
+  for (int i = s; i < e; ++i) sum += in[i];
+  part[t] = sum;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    int v = (t >= off) ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int run = part[t] - sum;
+  for (int i = s; i < e; ++i) {
+    int x = in[i];
+    out[i] = run;
+    run += x;
+  }
+  if (write_total && t == 1023) out[n] = part[1023];
+}
+
+int scan_exclusive(const int* in, int* out, int n, bool write_total, hipStream_t st) {
+  hipLaunchKernelGGL(k_scan1, dim3(1), dim3(1024), 0, st, in, out, n, write_total ? 1 : 0);
+  EB_HIP_CHECK(hipGetLastError());
+  return EB_OK;
+}
+
+size_t radix_sort_temp_ints(int n) {
+  int nblk = (n + RS_TILE - 1) / RS_TILE;
+  return (size_t)2 * RS_RADIX * nblk;
+}
+
+// Sorts (k0, v0) by the low `nbits` of the keys; the result ends in (k0, v0).  k1/v1 are
+// scratch of n entries, tmp holds radix_sort_temp_ints(n) ints.
+int radix_sort_pairs(uint32_t* k0, int32_t* v0, uint32_t* k1, int32_t* v1, int n, int nbits, int* tmp,
+                     hipStream_t st) {
+  if (n <= 0) return EB_OK;
+  const int nblk = (n + RS_TILE - 1) / RS_TILE;
+  int* hist = tmp;
+  int* hscan = tmp + RS_RADIX * nblk;
+  int passes = (nbits + RS_BITS - 1) / RS_BITS;
+  if (passes == 0) passes = 1;
+  if (passes & 1) passes += 1;  // an even number of passes leaves the result in (k0, v0)
+  uint32_t* ka = k0;
+  int32_t* va = v0;
+  uint32_t* kb = k1;
+  int32_t* vb = v1;
+  for (int p = 0; p < passes; ++p) {
+    const int shift = p * RS_BITS;
+    if (shift < 32) {
+      hipLaunchKernelGGL(k_rs_hist, dim3(nblk), dim3(RS_THREADS), 0, st, ka, n, shift, nblk, hist);
+      EB_HIP_CHECK(hipGetLastError());
+      int rc = scan_exclusive(hist, hscan, RS_RADIX * nblk, false, st);
+      if (rc) return rc;
+      hipLaunchKernelGGL(k_rs_scatter, dim3(nblk), dim3(RS_THREADS), 0, st, ka, va, kb, vb, n, shift, nblk,
+                         hscan);
+      EB_HIP_CHECK(hipGetLastError());
+    } else {
+      EB_HIP_CHECK(hipMemcpyAsync(kb, ka, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st));
+      EB_HIP_CHECK(hipMemcpyAsync(vb, va, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, st));
+    }
+    uint32_t* tk = ka; ka = kb; kb = tk;
+    int32_t* tv = va; va = vb; vb = tv;
+  }
+  return EB_OK;
+}
+
+}  // namespace eb

@@ -1,0 +1,148 @@
+/*
+ * eb_cadap.h -- C-ABI of the MI355X (gfx950) Stokes-matvec path of ElectricBubbles.
+ *
+ * Two layers:
+ *
+ * 1. Drop-in entry points with the exact 26-argument Fortran interface of the reference's
+ *    complex-variable FMM (all arguments by reference, COMPLEX*16 = interleaved double pairs,
+ *    ND(K0:*) and H(K0:*) point at element K0).  A thin ISO_C_BINDING shim
+ *    (electricbubbles_amd/fortran/eb_cadap_shim.f90) exports the reference names
+ *    CADAPDR / CADAPMOD / CADAP and forwards here, so the reference drivers link unchanged.
+ *    Host buffers in, host buffers out; the call blocks until GRADW is back in host memory.
+ *
+ * 2. A device-resident, stream-ordered plan/apply API used by the drop-in layer itself, by
+ *    the benchmark and by batched callers: geometry is binned once per plan (the tree is a
+ *    pure function of the point coordinates), and each apply evaluates the layer potential
+ *    for a new density.
+ *
+ * Error behaviour mirrors the reference: the int return value equals IER(1) (0 ok, 4 too many
+ * terms, 8 workspace too small); build-specific failures use EB_ERR_* (>= 16) and leave a
+ * message for eb_last_error().
+ */
+#ifndef EB_CADAP_H
+#define EB_CADAP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes (0, 4, 8 are the reference's IER(1) values, cadapdrop.f:89-103) */
+#define EB_OK             0
+#define EB_ERR_TERMS      4    /* NTERMS > 60 (cadapdrop.f:129-133) */
+#define EB_ERR_WORKSPACE  8    /* LENW / LENIW too small (cadapdrop.f:191-197, 283-290, 312-319) */
+#define EB_ERR_UNSUPPORTED 16  /* IFLAG1=1 (targets) or IFLAG2=1 (wall): not built */
+#define EB_ERR_ARG        17   /* invalid argument (n < 1, LEVMAX out of range, ...) */
+#define EB_ERR_HIP        18   /* HIP runtime error */
+#define EB_ERR_NOMEM      19
+
+/* Kernel variants: sign conventions of the three reference drivers. */
+#define EB_VARIANT_DR   0  /* CADAPDR  (cadapdrop.f): GW = phi - z conj(phi') - conj(psi) */
+#define EB_VARIANT_MOD  1  /* CADAPMOD (cadapmod.f):  GW = phi + z conj(phi') + conj(psi) */
+#define EB_VARIANT_PLUS 2  /* CADAP    (cadap.f):     as MOD, with +conj(dens) dz in QB (cadap.f:215) */
+
+/* Evaluation modes of the drop-in layer. */
+#define EB_MODE_AUTO   0   /* direct O(N^2) sum below EB_DIRECT_MAX points, FMM above */
+#define EB_MODE_FMM    1
+#define EB_MODE_DIRECT 2
+
+const char *eb_last_error(void);
+int eb_version(void);
+
+/* ---------------------------------------------------------------------------------------
+ * Layer 1: drop-in replacements.  Interface of CADAPDR, cadapdrop.f:3-5:
+ *   (IFLAG1, IFLAG2, LEVMAX, XAT, YAT, DZDT, K0, KBOD, ND, DENS, H, IWORK, LENIW, CWORK,
+ *    LENW, XTARG, YTARG, NTARG, GRADW, PHIP, PHIPP, TOL, INFORM, IER, XLEFT, SLENGTH)
+ * Supported: IFLAG1 = 0 (evaluate at the sources, self excluded), IFLAG2 = 0 (free space),
+ * which is what every caller on the path passes (stokesbubb.f:559-571, electric.f:835-837).
+ * With IFLAG1 = 0 the target arrays and NTARG are not read (FASMVP leaves NTARG unset).
+ * IWORK/CWORK are not used, but LENIW/LENW are checked exactly as the reference does, so
+ * the same inputs fail the same way (IER(2..5) = need/have).
+ * --------------------------------------------------------------------------------------- */
+int32_t eb_cadapdr(const int32_t *iflag1, const int32_t *iflag2, const int32_t *levmax,
+                   const double *xat, const double *yat, const double *dzdt,
+                   const int32_t *k0, const int32_t *kbod, const int32_t *nd,
+                   const double *dens, const double *h, int32_t *iwork, const int32_t *leniw,
+                   double *cwork, const int32_t *lenw, const double *xtarg, const double *ytarg,
+                   const int32_t *ntarg, double *gradw, double *phip, double *phipp,
+                   const double *tol, int32_t *inform, int32_t *ier, const double *xleft,
+                   const double *slength);
+/* CADAPMOD, cadapmod.f:3-5 (same arguments) */
+int32_t eb_cadapmod(const int32_t *iflag1, const int32_t *iflag2, const int32_t *levmax,
+                    const double *xat, const double *yat, const double *dzdt,
+                    const int32_t *k0, const int32_t *kbod, const int32_t *nd,
+                    const double *dens, const double *h, int32_t *iwork, const int32_t *leniw,
+                    double *cwork, const int32_t *lenw, const double *xtarg, const double *ytarg,
+                    const int32_t *ntarg, double *gradw, double *phip, double *phipp,
+                    const double *tol, int32_t *inform, int32_t *ier, const double *xleft,
+                    const double *slength);
+/* CADAP, cadap.f:3-5 (same arguments) */
+int32_t eb_cadap(const int32_t *iflag1, const int32_t *iflag2, const int32_t *levmax,
+                 const double *xat, const double *yat, const double *dzdt,
+                 const int32_t *k0, const int32_t *kbod, const int32_t *nd,
+                 const double *dens, const double *h, int32_t *iwork, const int32_t *leniw,
+                 double *cwork, const int32_t *lenw, const double *xtarg, const double *ytarg,
+                 const int32_t *ntarg, double *gradw, double *phip, double *phipp,
+                 const double *tol, int32_t *inform, int32_t *ier, const double *xleft,
+                 const double *slength);
+
+/* Evaluation mode used by the drop-in layer (process-wide; default EB_MODE_AUTO). */
+int eb_set_mode(int mode);
+
+/* ---------------------------------------------------------------------------------------
+ * Layer 2: device-resident plan / apply.  All pointers are device pointers; `stream` is a
+ * hipStream_t (NULL = default stream).  Complex arrays are interleaved (re, im) doubles.
+ * --------------------------------------------------------------------------------------- */
+typedef struct eb_fmm_plan_s *eb_fmm_plan;
+
+/* Bin n points (x, y) into the reference's uniform grids of levels 1..levmax
+ * (cadap2.f:252-409) and prepare the expansions.  wdz[i] = DZDT(i) * H(body of i): the
+ * quadrature-weighted tangent.  nterms = p (the reference uses -NINT(log2 TOL),
+ * cadapdrop.f:126).  mode: EB_MODE_FMM or EB_MODE_DIRECT. */
+int eb_fmm_plan_create(eb_fmm_plan *plan, int32_t n, const double *x, const double *y,
+                       const double *wdz, int32_t levmax, int32_t nterms, int32_t variant,
+                       int32_t mode, void *stream);
+/* Re-bin new coordinates / weights of the same n into an existing plan (no allocation);
+ * used when the geometry moves (every Runge-Kutta stage). */
+int eb_fmm_plan_rebuild(eb_fmm_plan plan, const double *x, const double *y, const double *wdz,
+                        void *stream);
+/* gradw[i] = GW at point i for density dens (the CADAPDR result with IFLAG1 = 0). */
+int eb_fmm_apply(eb_fmm_plan plan, const double *dens, double *gradw, void *stream);
+int eb_fmm_plan_destroy(eb_fmm_plan plan);
+
+/* Plan facts: [0] rscal, [1] rminx, [2] rminy; counts: [0] levmax, [1] nterms, [2] kmax
+ * (padded p), [3] #ordered near-field pairs, [4] #M2L pairs (both boxes non-empty),
+ * [5] #non-empty boxes over levels 2..levmax, [6] #non-empty leaf boxes, [7] max leaf count.
+ * Synchronises the plan's stream. */
+int eb_fmm_plan_info(eb_fmm_plan plan, double *reals3, int64_t *counts8);
+/* Non-empty boxes per level, ntbox[0..levmax] (entry l = the reference's NTBOX of level l,
+ * cadap2.f:331-336; entry 0 unused).  Synchronises the plan's stream. */
+int eb_fmm_plan_ntbox(eb_fmm_plan plan, int32_t *ntbox);
+
+/* Per-stage timing of the last apply on the plan's stream: milliseconds of
+ * [charges, P2M, M2M, M2L+L2L, L2P, P2P] measured with HIP events when enabled. */
+int eb_fmm_enable_timing(eb_fmm_plan plan, int32_t on);
+int eb_fmm_stage_ms(eb_fmm_plan plan, double *ms6);
+
+/* ---------------------------------------------------------------------------------------
+ * Tree parity: the reference's per-level binning tables, computed on the GPU.
+ * For level ilev (1..15), n points (device x, y), with the bounding box and RSCAL of
+ * cadapdrop.f:145-171 recomputed on the device.  Outputs (device, 1-based like the
+ * reference): ibox[n] (ASSIGN1 IBOX), icnt[4^ilev] (source histogram), ioffst[4^ilev+1],
+ * iatadr[n] (stable by source index), iloc[4^ilev] (first-touch numbering); host scalars
+ * ntbox, nsmax.  Synchronises `stream`.
+ * --------------------------------------------------------------------------------------- */
+int eb_cadap_tables(int32_t ilev, int32_t n, const double *x, const double *y,
+                    int32_t *ibox, int32_t *icnt, int32_t *ioffst, int32_t *iatadr,
+                    int32_t *iloc, int32_t *ntbox, int32_t *nsmax, void *stream);
+/* MKINTL (cadap2.f:780-828) and MKNBOR (cadap2.f:705-742) for every box of level ilev,
+ * generated by the same device routine the M2L and near-field kernels use.
+ * intl[4^ilev * 27] / nintl[4^ilev], nbor[4^ilev * 9] / nnbor[4^ilev] (device, 1-based ids). */
+int eb_cadap_lists(int32_t ilev, int32_t *intl, int32_t *nintl, int32_t *nbor, int32_t *nnbor,
+                   void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,427 @@
+"""Checkers for the Stokes-matvec path (TEST INFRASTRUCTURE ONLY).
+
+Two CPU checkers live here:
+
+* ``liboracle.so`` -- a plain-C restatement of the reference algorithm (``eb_oracle.c``),
+  function by function, with file:line citations of the Fortran it follows;
+* ``_ref/libref.so`` -- the reference Fortran itself, compiled in place from its sources by
+  ``oracle/Makefile`` (only where the sources are present; the built library travels with
+  the tree like any other built ``.so``).
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py``
+may import this package, and only as the checker.  The product package
+(``electricbubbles_amd``) never imports it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REF_SRC = os.environ.get("EB_REFERENCE_DIR", "/root/reference")
+LIBORACLE = os.path.join(HERE, "liboracle.so")
+LIBREF = os.path.join(HERE, "_ref", "libref.so")
+
+DR, MOD, PLUS = 0, 1, 2
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_llp = ctypes.POINTER(ctypes.c_longlong)
+
+
+def build(ref: bool | None = None) -> None:
+    """Compile liboracle.so (always) and _ref/libref.so (when the reference sources exist)."""
+    subprocess.run(["make", "-s", "-C", HERE, "oracle"], check=True)
+    if ref is None:
+        ref = os.path.isdir(REF_SRC)
+    if ref:
+        subprocess.run(["make", "-s", "-C", HERE, "ref", f"REF={REF_SRC}"], check=True)
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+def _c_int(v):
+    return ctypes.byref(ctypes.c_int(int(v)))
+
+
+def _c_double(v):
+    return ctypes.byref(ctypes.c_double(float(v)))
+
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBORACLE):
+            raise RuntimeError(f"{LIBORACLE} is not built; run oracle.build()")
+        _lib = ctypes.CDLL(LIBORACLE)
+    return _lib
+
+
+def ref_available() -> bool:
+    return os.path.exists(LIBREF)
+
+
+def ref():
+    """The compiled reference (lazy binding: the plot routines of matplot.f stay unresolved)."""
+    global _ref
+    if _ref is None:
+        if not os.path.exists(LIBREF):
+            raise RuntimeError(f"{LIBREF} is not built (needs the reference sources)")
+        _ref = ctypes.CDLL(LIBREF, mode=os.RTLD_LAZY)
+    return _ref
+
+
+def _cplx(a):
+    """complex128 array -> contiguous interleaved float64 view"""
+    a = np.ascontiguousarray(a, dtype=np.complex128)
+    return a.view(np.float64)
+
+
+# ------------------------------------------------------------------------------------
+# C restatement
+# ------------------------------------------------------------------------------------
+
+def scale(x, y):
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    n = x.size
+    zs = np.empty(2 * n)
+    rs = ctypes.c_double()
+    rmin = np.empty(2)
+    lib().ora_scale(n, _d(x), _d(y), _d(zs), ctypes.byref(rs), _d(rmin))
+    return zs.view(np.complex128), rs.value, rmin
+
+
+def assign1(ilev, zs):
+    """ASSIGN1 tables for scaled points zs (complex array). Returns dict of 1-based tables."""
+    zsd = _cplx(zs)
+    n = zs.size
+    nb = 4 ** ilev
+    ibox = np.empty(n, np.int32)
+    icnt = np.empty(nb, np.int32)
+    ioffst = np.empty(nb + 1, np.int32)
+    iatadr = np.empty(n, np.int32)
+    iloc = np.empty(nb, np.int32)
+    ntbox = ctypes.c_int()
+    nsmax = ctypes.c_int()
+    lib().ora_assign1(ilev, n, _d(zsd), _i(ibox), _i(icnt), _i(ioffst), _i(iatadr), _i(iloc),
+                      ctypes.byref(ntbox), ctypes.byref(nsmax))
+    return dict(ibox=ibox, icnt=icnt, ioffst=ioffst, iatadr=iatadr, iloc=iloc,
+                ntbox=ntbox.value, nsmax=nsmax.value)
+
+
+def mkintl(ilev, ibox):
+    out = np.zeros(30, np.int32)
+    n = ctypes.c_int()
+    lib().ora_mkintl(ilev, ibox, _i(out), ctypes.byref(n))
+    return out[: n.value].copy()
+
+
+def mknbor(ilev, ibox):
+    out = np.zeros(30, np.int32)
+    n = ctypes.c_int()
+    lib().ora_mknbor(ilev, ibox, _i(out), ctypes.byref(n))
+    return out[: n.value].copy()
+
+
+def cadap(variant, levmax, nd, x, y, dzdt, dens, h, tol):
+    """Restated CADAPDR/CADAPMOD/CADAP (IFLAG1=0, IFLAG2=0). Returns (gradw, inform, ier, counts)."""
+    nd = np.ascontiguousarray(nd, np.int32)
+    h = np.ascontiguousarray(h, np.float64)
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    dz = _cplx(dzdt)
+    de = _cplx(dens)
+    n = int(nd.sum())
+    gw = np.zeros(2 * n)
+    inform = np.zeros(2, np.int32)
+    ier = np.zeros(5, np.int32)
+    counts = np.zeros(4, np.int64)
+    lib().ora_cadap(variant, levmax, nd.size, _i(nd), _d(x), _d(y), _d(dz), _d(de), _d(h),
+                    ctypes.c_double(tol), _d(gw), _i(inform), _i(ier),
+                    counts.ctypes.data_as(_llp))
+    return gw.view(np.complex128), inform, ier, counts
+
+
+def direct(variant, nd, x, y, dzdt, dens, h, targets=None):
+    """O(N^2) closed-form sum (long double). targets: 0-based indices (default all)."""
+    nd = np.ascontiguousarray(nd, np.int32)
+    h = np.ascontiguousarray(h, np.float64)
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    dz = _cplx(dzdt)
+    de = _cplx(dens)
+    n = int(nd.sum())
+    if targets is None:
+        targets = np.arange(n, dtype=np.int32)
+    targets = np.ascontiguousarray(targets, np.int32)
+    gw = np.zeros(2 * targets.size)
+    lib().ora_direct(variant, nd.size, _i(nd), _d(x), _d(y), _d(dz), _d(de), _d(h),
+                     targets.size, _i(targets), _d(gw))
+    return gw.view(np.complex128)
+
+
+def fasmvp_epilogue(nd, x, y, dz, dsdth, rkappa, zk, u, stress):
+    nd = np.ascontiguousarray(nd, np.int32)
+    k = nd.size
+    nbk = int(nd.sum())
+    w = np.zeros(2 * nbk)
+    za = np.zeros(2 * k)
+    pk = np.zeros(k)
+    lib().ora_fasmvp_epilogue(k, _i(nd), _d(np.ascontiguousarray(x, np.float64)),
+                              _d(np.ascontiguousarray(y, np.float64)), _d(_cplx(dz)),
+                              _d(np.ascontiguousarray(dsdth, np.float64)),
+                              _d(np.ascontiguousarray(rkappa, np.float64)), _d(_cplx(zk)),
+                              _d(np.ascontiguousarray(u, np.float64)), _d(_cplx(stress)),
+                              _d(w), _d(za), _d(pk))
+    return w, za.view(np.complex128), pk
+
+
+def dft(c, sign):
+    a = _cplx(c).copy()
+    lib().ora_dft(a.size // 2, _d(a), sign)
+    return a.view(np.complex128)
+
+
+def fdifff(f):
+    f = _cplx(f)
+    out = np.empty_like(f)
+    lib().ora_fdifff(f.size // 2, _d(f), _d(out))
+    return out.view(np.complex128)
+
+
+def extractdat(nd, rku):
+    nd = np.ascontiguousarray(nd, np.int32)
+    k = nd.size
+    nbk = int(nd.sum())
+    rku = np.ascontiguousarray(rku, np.float64)
+    x, y = np.empty(nbk), np.empty(nbk)
+    zk = np.empty(2 * k)
+    dz = np.empty(2 * nbk)
+    tha, dsda, rkappa = np.empty(nbk), np.empty(nbk), np.empty(nbk)
+    ckern = np.empty(2 * nbk)
+    area = np.empty(k)
+    xl, sl = ctypes.c_double(), ctypes.c_double()
+    lib().ora_extractdat(k, _i(nd), _d(rku), _d(x), _d(y), _d(zk), _d(dz), _d(tha), _d(dsda),
+                         _d(rkappa), _d(ckern), _d(area), ctypes.byref(xl), ctypes.byref(sl))
+    return dict(x=x, y=y, zk=zk.view(np.complex128), dz=dz.view(np.complex128), tha=tha,
+                dsda=dsda, rkappa=rkappa, ckern=ckern.view(np.complex128), area=area,
+                xleft=xl.value, slength=sl.value)
+
+
+def equit(tha, un):
+    tha = np.ascontiguousarray(tha, np.float64)
+    un = np.ascontiguousarray(un, np.float64)
+    t = np.empty_like(tha)
+    lib().ora_equit(tha.size, _d(tha), _d(un), _d(t))
+    return t
+
+
+def filter_(cn):
+    a = _cplx(cn).copy()
+    lib().ora_filter(a.size // 2, _d(a))
+    return a.view(np.complex128)
+
+
+def ftail(nd, tha, un, dz, dsda):
+    nd = np.ascontiguousarray(nd, np.int32)
+    nbk = int(nd.sum())
+    rkup = np.empty(2 * nbk)
+    lib().ora_ftail(nd.size, _i(nd), _d(np.ascontiguousarray(tha, np.float64)),
+                    _d(np.ascontiguousarray(un, np.float64)), _d(_cplx(dz)),
+                    _d(np.ascontiguousarray(dsda, np.float64)), _d(rkup))
+    return rkup
+
+
+def rk2_stage1(dt, ybar, dy):
+    ybar = np.ascontiguousarray(ybar, np.float64)
+    dy = np.ascontiguousarray(dy, np.float64)
+    phi0, y = np.empty_like(ybar), np.empty_like(ybar)
+    lib().ora_rk2_stage1(ybar.size, ctypes.c_double(dt), _d(ybar), _d(dy), _d(phi0), _d(y))
+    return phi0, y
+
+
+def rk2_stage2(dt, phi0, dy, ybar):
+    ybar = np.ascontiguousarray(ybar, np.float64).copy()
+    lib().ora_rk2_stage2(ybar.size, ctypes.c_double(dt), _d(np.ascontiguousarray(phi0, np.float64)),
+                         _d(np.ascontiguousarray(dy, np.float64)), _d(ybar))
+    return ybar
+
+
+# ------------------------------------------------------------------------------------
+# The reference library (Fortran calling convention: everything by reference)
+# ------------------------------------------------------------------------------------
+
+def ref_cadap(variant, levmax, nd, x, y, dzdt, dens, h, tol, lenw=None, leniw=None):
+    """Call the reference CADAPDR / CADAPMOD / CADAP with IFLAG1=0, IFLAG2=0."""
+    r = ref()
+    fn = {DR: r.cadapdr_, MOD: r.cadapmod_, PLUS: r.cadap_}[variant]
+    nd = np.ascontiguousarray(nd, np.int32)
+    n = int(nd.sum())
+    p = -int(np.round(np.log(tol) / np.log(2.0)))
+    if lenw is None:
+        lenw = 8 * n + 2 * (p + 1) * min(n, 4 ** levmax) + 16
+    if leniw is None:
+        leniw = 5 * 4 ** levmax + 2 * n + 16
+    iwork = np.zeros(leniw, np.int32)
+    cwork = np.zeros(2 * lenw)
+    gw = np.zeros(2 * n)
+    dummy = np.zeros(4)
+    inform = np.zeros(6, np.int32)
+    ier = np.zeros(5, np.int32)
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    dz = _cplx(dzdt)
+    de = _cplx(dens)
+    h = np.ascontiguousarray(h, np.float64)
+    fn(_c_int(0), _c_int(0), _c_int(levmax), _d(x), _d(y), _d(dz), _c_int(1), _c_int(nd.size),
+       _i(nd), _d(de), _d(h), _i(iwork), _c_int(leniw), _d(cwork), _c_int(lenw),
+       _d(dummy), _d(dummy), _c_int(0), _d(gw), _d(dummy), _d(dummy), _c_double(tol),
+       _i(inform), _i(ier), _c_double(0.0), _c_double(0.0))
+    return gw.view(np.complex128), inform[:2].copy(), ier.copy()
+
+
+def ref_assign1(ilev, zs):
+    r = ref()
+    n = zs.size
+    nb = 4 ** ilev
+    zsd = _cplx(zs)
+    ioffst = np.zeros(nb + 1, np.int32)
+    itofst = np.zeros(nb + 1, np.int32)
+    ibox = np.zeros(n, np.int32)
+    icnt = np.zeros(nb, np.int32)
+    icnt2 = np.zeros(nb, np.int32)
+    iatadr = np.zeros(n, np.int32)
+    itradr = np.zeros(1, np.int32)
+    iloc = np.zeros(nb, np.int32)
+    ntbox = ctypes.c_int()
+    nsmax = ctypes.c_int()
+    zt = np.zeros(2)
+    r.assign1_(_c_int(0), _c_int(ilev), _d(zsd), _i(ioffst), _d(zt), _c_int(0), _i(ibox),
+               _c_int(n), _i(icnt), _i(icnt2), _i(iatadr), _i(itradr), _i(itofst), _i(iloc),
+               ctypes.byref(ntbox), ctypes.byref(nsmax))
+    return dict(ibox=ibox, icnt=icnt, ioffst=ioffst, iatadr=iatadr, iloc=iloc,
+                ntbox=ntbox.value, nsmax=nsmax.value)
+
+
+def ref_mkintl(ilev, ibox):
+    out = np.zeros(30, np.int32)
+    n = ctypes.c_int()
+    ref().mkintl_(_c_int(ilev), _c_int(ibox), _i(out), ctypes.byref(n))
+    return out[: n.value].copy()
+
+
+def ref_mknbor(ilev, ibox):
+    out = np.zeros(30, np.int32)
+    n = ctypes.c_int()
+    ref().mknbor_(_c_int(ilev), _c_int(ibox), _i(out), ctypes.byref(n))
+    return out[: n.value].copy()
+
+
+def _wsave(n):
+    w = np.zeros(2 * (4 * n + 15) + 64)
+    ref().dcffti_(_c_int(n), _d(w))
+    return w
+
+
+def ref_dcfftf(c):
+    a = _cplx(c).copy()
+    n = a.size // 2
+    w = _wsave(n)
+    ref().dcfftf_(_c_int(n), _d(a), _d(w))
+    return a.view(np.complex128)
+
+
+def ref_dcfftb(c):
+    a = _cplx(c).copy()
+    n = a.size // 2
+    w = _wsave(n)
+    ref().dcfftb_(_c_int(n), _d(a), _d(w))
+    return a.view(np.complex128)
+
+
+def ref_fdifff(f):
+    f = _cplx(f).copy()
+    n = f.size // 2
+    w = _wsave(n)
+    out = np.zeros_like(f)
+    ref().fdifff_(_d(f), _d(out), _c_int(n), _d(w))
+    return out.view(np.complex128)
+
+
+def ref_filter(cn):
+    a = _cplx(cn).copy()
+    ref().filter_(_c_int(a.size // 2), _d(a))
+    return a.view(np.complex128)
+
+
+def ref_equit(tha, un):
+    tha = np.ascontiguousarray(tha, np.float64)
+    un = np.ascontiguousarray(un, np.float64)
+    n = tha.size
+    t = np.zeros(n)
+    zf = np.zeros(2 * n)
+    zfi = np.zeros(2 * n)
+    w = _wsave(n)
+    ref().equit_(_c_int(n), _d(tha), _d(un), _d(t), _d(zf), _d(zfi), _d(w))
+    return t
+
+
+def ref_extractdat(nd, rku):
+    nd = np.ascontiguousarray(nd, np.int32)
+    k = nd.size
+    nbk = int(nd.sum())
+    ifmax = 2 * (4 * int(nd.max()) + 15) // 2 + 64   # complex words per body in WSAVE
+    rku = np.ascontiguousarray(rku, np.float64)
+    x, y = np.zeros(nbk), np.zeros(nbk)
+    z = np.zeros(2 * nbk)
+    zk = np.zeros(2 * k)
+    dz = np.zeros(2 * nbk)
+    tha, dsda, rkappa = np.zeros(nbk), np.zeros(nbk), np.zeros(nbk)
+    ckern = np.zeros(2 * nbk)
+    za = np.zeros(2 * int(nd.max()))
+    zai = np.zeros(2 * int(nd.max()))
+    wsave = np.zeros(2 * ifmax * k)
+    zc = np.zeros(2 * k)
+    xl, sl = ctypes.c_double(), ctypes.c_double()
+    zbond = np.zeros(2)
+    ref().extractdat_(_c_int(k), _i(nd), _c_int(nbk), _d(x), _d(y), _d(z), _d(zk), _d(dz),
+                      _d(tha), _d(dsda), _d(rkappa), _d(ckern), _d(rku), _d(za), _d(zai),
+                      _d(wsave), _c_int(ifmax), _d(zc), ctypes.byref(xl), ctypes.byref(sl),
+                      _d(zbond), _c_int(0))
+    return dict(x=x, y=y, zk=zk.view(np.complex128), dz=dz.view(np.complex128), tha=tha,
+                dsda=dsda, rkappa=rkappa, ckern=ckern.view(np.complex128),
+                xleft=xl.value, slength=sl.value)
+
+
+def ref_fasmvp(nd, x, y, dz, dsdth, rkappa, zk, u):
+    """Reference FASMVP (stokesbubb.f:496-614): calls CADAPDR (LEVMAX 7, TOL 1e-13) itself."""
+    nd = np.ascontiguousarray(nd, np.int32)
+    k = nd.size
+    nbk = int(nd.sum())
+    h = (2 * np.pi / nd).astype(np.float64)
+    w = np.zeros(2 * nbk)
+    za = np.zeros(2 * k)
+    pk = np.zeros(k)
+    stress = np.zeros(2 * nbk)
+    dens = np.zeros(2 * nbk)
+    ref().fasmvp_(_i(nd), _c_int(nbk), _c_int(k), _d(np.ascontiguousarray(x, np.float64)),
+                  _d(np.ascontiguousarray(y, np.float64)), _d(_cplx(dz)),
+                  _d(np.ascontiguousarray(dsdth, np.float64)),
+                  _d(np.ascontiguousarray(rkappa, np.float64)), _d(_cplx(zk)),
+                  _d(np.ascontiguousarray(u, np.float64)), _d(w), _d(h), _d(za), _d(pk),
+                  _d(stress), _d(dens))
+    return w, stress.view(np.complex128), za.view(np.complex128), pk
