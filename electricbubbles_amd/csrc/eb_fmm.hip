@@ -255,37 +255,43 @@ __global__ __launch_bounds__(64) void k_p2m(int L, int p, const int* __restrict_
 }
 
 // M2M: parent alpha_{n+1} = sum_c sum_k TM[c][k][n] child alpha_{k+1}, with
-// TM[c][k][n] = C(n,k) 2^-(k+1) delta_c^(n-k), delta_c = (child centre - parent centre)/r_parent.
-template <int KMAX>
-__global__ __launch_bounds__(64) void k_m2m(int L, int l, const int* __restrict__ leaf_start,
-                                            const double2* __restrict__ mpc, int Wc, int NBPc,
-                                            double2* __restrict__ mpp, int Wp, int NBPp,
-                                            const double2* __restrict__ TM) {
+// TM[c][k][n] = C(n,k) 2^-(k+1) delta_c^(n-k), delta_c = (child centre - parent centre)/r_parent
+// (lower triangular: n >= k).  One thread per (parent box, series, block of MB outputs); lanes
+// are consecutive parents, so the weights are wave-uniform (scalar loads).  Empty parents
+// write zeros: the M2L pass reads every box of the level.
+template <int KMAX, int MB>
+__global__ __launch_bounds__(256) void k_m2m(int L, int l, const int* __restrict__ leaf_start,
+                                             const double2* __restrict__ mpc, int Wc, int NBPc,
+                                             double2* __restrict__ mpp, int Wp, int NBPp,
+                                             const double2* __restrict__ TM) {
+  const int s = blockIdx.y & 1;
+  const int n0 = (blockIdx.y >> 1) * MB;
   const int nside = 1 << l;
-  const int ix = blockIdx.x % nside, iy = blockIdx.x / nside;
-  const int lane = threadIdx.x;
-  int s, e;
-  box_range(leaf_start, L, l, ix, iy, s, e);
-  double2 a1 = make_double2(0, 0), a2 = make_double2(0, 0);
-  if (e > s) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nside * nside) return;
+  const int ix = t % nside, iy = t / nside;
+  int bs, be;
+  box_range(leaf_start, L, l, ix, iy, bs, be);
+  double2 acc[MB];
+#pragma unroll
+  for (int i = 0; i < MB; ++i) acc[i] = make_double2(0, 0);
+  if (be > bs) {
+    const int kend = min(KMAX, n0 + MB);
     for (int c = 0; c < 4; ++c) {
       const int pc = pidx(2 * ix + (c & 1), 2 * iy + (c >> 1), Wc);
-      const double2* tm = TM + (size_t)c * KMAX * KMAX;
+      const double2* mc = mpc + (size_t)s * KMAX * NBPc + pc;
+      const double2* tm = TM + (size_t)c * KMAX * KMAX + n0;
 #pragma unroll 4
-      for (int k = 0; k < KMAX; ++k) {
-        const double2 m1 = mpc[(size_t)k * NBPc + pc];
-        const double2 m2 = mpc[(size_t)(KMAX + k) * NBPc + pc];
-        const double2 w = tm[k * KMAX + (lane < KMAX ? lane : 0)];
-        cfma(a1, w, m1);
-        cfma(a2, w, m2);
+      for (int k = 0; k < kend; ++k) {
+        const double2 a = mc[(size_t)k * NBPc];
+#pragma unroll
+        for (int i = 0; i < MB; ++i) cfma(acc[i], tm[k * KMAX + i], a);
       }
     }
   }
-  if (lane < KMAX) {
-    const int pb = pidx(ix, iy, Wp);
-    mpp[(size_t)lane * NBPp + pb] = a1;
-    mpp[(size_t)(KMAX + lane) * NBPp + pb] = a2;
-  }
+  double2* out = mpp + (size_t)(s * KMAX + n0) * NBPp + pidx(ix, iy, Wp);
+#pragma unroll
+  for (int i = 0; i < MB; ++i) out[(size_t)i * NBPp] = acc[i];
 }
 
 // M2L (+ L2L from the parent) for every non-empty box of level l.  One thread owns
@@ -795,9 +801,12 @@ static void launch_far(eb_fmm_plan P) {
   hipLaunchKernelGGL(k_p2m<K>, dim3(1 << (2 * L)), dim3(64), 0, st, L, P->p, P->leaf_start, P->zs, P->qa, P->qb,
                      P->mp + P->lvl_off[L], P->lvl_W[L], P->lvl_NBP[L]);
   if (P->timing) (void)hipEventRecord(P->ev[2], st);
-  for (int l = L - 1; l >= 2; --l)
-    hipLaunchKernelGGL(k_m2m<K>, dim3(1 << (2 * l)), dim3(64), 0, st, L, l, P->leaf_start, P->mp + P->lvl_off[l + 1],
-                       P->lvl_W[l + 1], P->lvl_NBP[l + 1], P->mp + P->lvl_off[l], P->lvl_W[l], P->lvl_NBP[l], P->dTM);
+  for (int l = L - 1; l >= 2; --l) {
+    const int nb = 1 << (2 * l);
+    hipLaunchKernelGGL((k_m2m<K, MB>), dim3((nb + 255) / 256, 2 * (K / MB)), dim3(256), 0, st, L, l, P->leaf_start,
+                       P->mp + P->lvl_off[l + 1], P->lvl_W[l + 1], P->lvl_NBP[l + 1], P->mp + P->lvl_off[l],
+                       P->lvl_W[l], P->lvl_NBP[l], P->dTM);
+  }
   if (P->timing) (void)hipEventRecord(P->ev[3], st);
   for (int l = 2; l <= L; ++l) {
     const int half = 1 << (l - 1);

@@ -24,6 +24,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 REF_SRC = os.environ.get("EB_REFERENCE_DIR", "/root/reference")
 LIBORACLE = os.path.join(HERE, "liboracle.so")
 LIBREF = os.path.join(HERE, "_ref", "libref.so")
+# the reference with CADAPDR/CADAPMOD/CADAP bound to the GPU shim instead of its own FMM
+LIBREF_ON_EB = os.path.join(HERE, "_ref", "libref_on_eb.so")
 
 DR, MOD, PLUS = 0, 1, 2
 
@@ -39,6 +41,7 @@ def build(ref: bool | None = None) -> None:
         ref = os.path.isdir(REF_SRC)
     if ref:
         subprocess.run(["make", "-s", "-C", HERE, "ref", f"REF={REF_SRC}"], check=True)
+        subprocess.run(["make", "-s", "-C", HERE, "ref_on_eb", f"REF={REF_SRC}"], check=True)
 
 
 def _d(a):
@@ -425,3 +428,78 @@ def ref_fasmvp(nd, x, y, dz, dsdth, rkappa, zk, u):
                   _d(np.ascontiguousarray(u, np.float64)), _d(w), _d(h), _d(za), _d(pk),
                   _d(stress), _d(dens))
     return w, stress.view(np.complex128), za.view(np.complex128), pk
+
+
+# ------------------------------------------------------------------------------------
+# The reference's own right-hand side and time step, on a chosen library handle
+# ------------------------------------------------------------------------------------
+
+class RefSim:
+    """Drive the reference's F (electric.f:968-1109) and RK2 (electric.f:762-786) in-process.
+
+    `path` is libref.so (the reference as written) or libref_on_eb.so (the same objects with
+    CADAPDR / CADAPMOD / CADAP bound to the GPU shim).  COMMON blocks are set the way
+    MANYDROP sets them (electric.f:52-56, 67-70): /problem/ nd, k; /flowinfo/ cap, zbond, E0;
+    /wallinfo/ iwall.  KMAX = 150 fixes the /problem/ layout (electric.f:10)."""
+
+    KMAX = 150
+
+    def __init__(self, path):
+        self.lib = ctypes.CDLL(path, mode=os.RTLD_LAZY)
+
+    def _block(self, name, nbytes):
+        addr = ctypes.addressof(ctypes.c_char.in_dll(self.lib, name))
+        return (ctypes.c_char * nbytes).from_address(addr)
+
+    def setup(self, nd, cap=0.0, zbond=0j, e0=0.0):
+        nd = np.asarray(nd, np.int32)
+        k = nd.size
+        prob = self._block("problem_", 16 * self.KMAX + 4 * self.KMAX + 12)
+        ndv = np.frombuffer(prob, dtype=np.int32, count=self.KMAX, offset=16 * self.KMAX)
+        ndv[:] = 0
+        ndv[:k] = nd
+        ints = np.frombuffer(prob, dtype=np.int32, count=3, offset=20 * self.KMAX)
+        ints[:] = [k, int(nd.sum()), 2 * int(nd.sum())]
+        flow = np.frombuffer(self._block("flowinfo_", 32), dtype=np.float64, count=4)
+        flow[:] = [cap, zbond.real, zbond.imag, e0]
+        wall = self._block("wallinfo_", 24)
+        np.frombuffer(wall, dtype=np.float64, count=2)[:] = 0.0
+        np.frombuffer(wall, dtype=np.int32, count=1, offset=16)[:] = 0
+        self.k = k
+        self.neq = 2 * int(nd.sum())
+
+    def rhs(self, ybar, t=0.0):
+        ybar = np.ascontiguousarray(ybar, np.float64).copy()
+        dy = np.zeros_like(ybar)
+        self.lib.f_(_c_int(self.k), _c_int(self.neq), _c_double(t), _d(ybar), _d(dy))
+        return dy
+
+    def rk2(self, ybar, dt, nsteps, t0=0.0):
+        ybar = np.ascontiguousarray(ybar, np.float64).copy()
+        y = np.zeros_like(ybar)
+        dy = np.zeros_like(ybar)
+        phi = np.zeros(4 * ybar.size)
+        t = t0
+        for _ in range(nsteps):
+            self.lib.rk2_(_c_int(self.k), _c_int(self.neq), _c_double(t), _c_double(dt), _d(phi), _d(ybar),
+                          _d(y), _d(dy))
+            t += dt
+        return ybar
+
+    def fasmvp(self, nd, x, y, dz, dsdth, rkappa, zk, u):
+        nd = np.ascontiguousarray(nd, np.int32)
+        k = nd.size
+        nbk = int(nd.sum())
+        h = (2 * np.pi / nd).astype(np.float64)
+        w = np.zeros(2 * nbk)
+        za = np.zeros(2 * k)
+        pk = np.zeros(k)
+        stress = np.zeros(2 * nbk)
+        dens = np.zeros(2 * nbk)
+        self.lib.fasmvp_(_i(nd), _c_int(nbk), _c_int(k), _d(np.ascontiguousarray(x, np.float64)),
+                         _d(np.ascontiguousarray(y, np.float64)), _d(_cplx(dz)),
+                         _d(np.ascontiguousarray(dsdth, np.float64)),
+                         _d(np.ascontiguousarray(rkappa, np.float64)), _d(_cplx(zk)),
+                         _d(np.ascontiguousarray(u, np.float64)), _d(w), _d(h), _d(za), _d(pk),
+                         _d(stress), _d(dens))
+        return w, stress.view(np.complex128)

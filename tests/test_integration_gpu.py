@@ -1,0 +1,71 @@
+"""The unchanged reference drivers on the GPU FMM.
+
+oracle/_ref/libref_on_eb.so holds the reference's own objects with cadapdrop.o, cadapmod.o and
+cadap.o left out, linked against the ISO_C_BINDING shim (electricbubbles_amd/libeb_fortran.so):
+FASMVP, INTERFACENEW, F and RK2 run exactly as written and reach the GPU through the
+CADAPDR / CADAPMOD names.  The same calls on oracle/_ref/libref.so (the reference as written)
+are the checker.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from electricbubbles_amd.workloads import perturbed_circle
+from tests.conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+
+def _need_libs():
+    for p in (oracle.LIBREF, oracle.LIBREF_ON_EB):
+        if not os.path.exists(p):
+            pytest.skip(f"{p} not built (it is compiled from the reference sources)")
+
+
+@pytest.fixture(scope="module")
+def sims(tmp_path_factory):
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    _need_libs()
+    cwd = os.getcwd()
+    os.chdir(tmp_path_factory.mktemp("fortran_units"))  # the reference writes fort.13 here
+    try:
+        yield oracle.RefSim(oracle.LIBREF), oracle.RefSim(oracle.LIBREF_ON_EB)
+    finally:
+        os.chdir(cwd)
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def test_fasmvp_through_shim_matches_reference(sims):
+    ref, gpu = sims
+    g = np.load(os.path.join(GOLDEN, "geometry_two_drops.npz"))
+    args = (g["nd"], g["x"], g["y"], g["dz"], g["dsda"], g["rkappa"], g["zk"], g["u"])
+    w_ref, s_ref = ref.fasmvp(*args)
+    w_gpu, s_gpu = gpu.fasmvp(*args)
+    assert rel(w_gpu, w_ref) < 1e-10
+    assert rel(w_gpu, g["w"]) < 1e-10
+
+
+def test_single_drop_trajectory_matches_reference(sims):
+    """Config-1 drop r = 1 + 0.05 cos 2 theta, 256 points, 10 Heun steps of dt 1e-3: the
+    reference's F and RK2 with the GPU FMM against the reference as written (shapes 1e-8)."""
+    ref, gpu = sims
+    z, _, nd, _ = perturbed_circle(256, [(2, 0.05)])
+    y0 = z.view(np.float64).copy()
+    out = []
+    for s in (ref, gpu):
+        s.setup(nd, cap=0.0, e0=1.0)
+        out.append(s.rk2(y0, 1e-3, 10))
+    y_ref, y_gpu = out
+    assert np.abs(y_gpu - y_ref).max() < 1e-8
+    r_ref = np.abs(y_ref.view(np.complex128))
+    r_gpu = np.abs(y_gpu.view(np.complex128))
+    d_ref = (r_ref.max() - r_ref.min()) / (r_ref.max() + r_ref.min())
+    d_gpu = (r_gpu.max() - r_gpu.min()) / (r_gpu.max() + r_gpu.min())
+    assert abs(d_gpu - d_ref) < 1e-8
