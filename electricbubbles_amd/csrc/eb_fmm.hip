@@ -92,19 +92,34 @@ __global__ __launch_bounds__(BB_THREADS) void k_bbox_part(const double* __restri
   if (threadIdx.x == 0) part[blockIdx.x] = s[0];
 }
 
-// RSCAL = 64 / max(RMAXX-RMINX, RMAXY-RMINY)  (cadapdrop.f:165-171)
-__global__ void k_bbox_final(const double4* __restrict__ part, int np, Geom* __restrict__ g) {
-  if (threadIdx.x != 0) return;
+// RSCAL = 64 / max(RMAXX-RMINX, RMAXY-RMINY)  (cadapdrop.f:165-171).  min/max are exact, so
+// the reduction order does not change the result.  One block of BB_THREADS threads.
+__global__ __launch_bounds__(BB_THREADS) void k_bbox_final(const double4* __restrict__ part, int np,
+                                                           Geom* __restrict__ g) {
+  __shared__ double4 s[BB_THREADS];
   double4 a = part[0];
-  for (int i = 1; i < np; ++i) {
-    double4 b = part[i];
+  for (int i = threadIdx.x; i < np; i += BB_THREADS) {
+    const double4 b = part[i];
     a = make_double4(fmin(a.x, b.x), fmax(a.y, b.y), fmin(a.z, b.z), fmax(a.w, b.w));
   }
-  double rsx = a.y - a.x, rsy = a.w - a.z;
-  double r = (rsx > rsy) ? rsx : rsy;
-  g->rscal = 64.0 / r;
-  g->rminx = a.x;
-  g->rminy = a.z;
+  s[threadIdx.x] = a;
+  __syncthreads();
+  for (int off = BB_THREADS / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) {
+      const double4 b = s[threadIdx.x + off];
+      a = s[threadIdx.x];
+      s[threadIdx.x] = make_double4(fmin(a.x, b.x), fmax(a.y, b.y), fmin(a.z, b.z), fmax(a.w, b.w));
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    a = s[0];
+    const double rsx = a.y - a.x, rsy = a.w - a.z;
+    const double r = (rsx > rsy) ? rsx : rsy;
+    g->rscal = 64.0 / r;
+    g->rminx = a.x;
+    g->rminy = a.z;
+  }
 }
 
 // Scaled coordinates (cadapdrop.f:203-205) and the finest-level Morton key of ASSIGN1's box
@@ -158,15 +173,18 @@ __global__ void k_zero_int(int* __restrict__ a, int n) {
 
 // number of non-empty boxes per level (the reference's NTBOX, cadap2.f:331-336): a leaf is
 // counted at level l when it holds points and is the first non-empty leaf of its ancestor.
-__global__ void k_ntbox(const int* __restrict__ leaf_start, int L, int* __restrict__ ntbox) {
+// Counts are aggregated per wave before the atomic add.
+__global__ __launch_bounds__(256) void k_ntbox(const int* __restrict__ leaf_start, int L, int* __restrict__ ntbox) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= (1 << (2 * L))) return;
-  const int s = leaf_start[m];
-  if (leaf_start[m + 1] == s) return;
+  const int nleaf = 1 << (2 * L);
+  const bool in = m < nleaf;
+  const int s = in ? leaf_start[m] : 0;
+  const bool nonempty = in && leaf_start[m + 1] != s;
   for (int l = 1; l <= L; ++l) {
     const int sh = 2 * (L - l);
-    const int first = (m >> sh) << sh;
-    if (leaf_start[first] == s) atomicAdd(&ntbox[l], 1);
+    const bool first = nonempty && leaf_start[(m >> sh) << sh] == s;
+    const uint64_t b = __ballot(first);
+    if ((threadIdx.x & (EB_WAVE - 1)) == 0 && b) atomicAdd(&ntbox[l], __popcll(b));
   }
 }
 
@@ -277,15 +295,20 @@ __global__ __launch_bounds__(256) void k_m2m(int L, int l, const int* __restrict
   for (int i = 0; i < MB; ++i) acc[i] = make_double2(0, 0);
   if (be > bs) {
     const int kend = min(KMAX, n0 + MB);
-    for (int c = 0; c < 4; ++c) {
-      const int pc = pidx(2 * ix + (c & 1), 2 * iy + (c >> 1), Wc);
-      const double2* mc = mpc + (size_t)s * KMAX * NBPc + pc;
-      const double2* tm = TM + (size_t)c * KMAX * KMAX + n0;
-#pragma unroll 4
-      for (int k = 0; k < kend; ++k) {
-        const double2 a = mc[(size_t)k * NBPc];
+    const double2* mc[4];
 #pragma unroll
-        for (int i = 0; i < MB; ++i) cfma(acc[i], tm[k * KMAX + i], a);
+    for (int c = 0; c < 4; ++c)
+      mc[c] = mpc + (size_t)s * KMAX * NBPc + pidx(2 * ix + (c & 1), 2 * iy + (c >> 1), Wc);
+#pragma unroll 2
+    for (int k = 0; k < kend; ++k) {
+      double2 a[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) a[c] = mc[c][(size_t)k * NBPc];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const double2* tm = TM + ((size_t)c * KMAX + k) * KMAX + n0;
+#pragma unroll
+        for (int i = 0; i < MB; ++i) cfma(acc[i], tm[i], a[c]);
       }
     }
   }
@@ -362,6 +385,167 @@ __global__ __launch_bounds__(256) void k_m2l(int L, int l, const int* __restrict
 #pragma unroll
   for (int i = 0; i < MB; ++i) out[(size_t)i * NBP] = acc[i];
   (void)NB;
+}
+
+// ------------------------------------------------------------------------------------------
+// M2L on FP64 matrix cores, every level in one launch.
+//
+// For a target box t and a source box in its interaction list (offset o, u = r/(c_t - c_s)):
+//   lambda_m(t) += (-u)^m  sum_k C(m+k,k) [u^(k+1) alpha_{k+1}(s)]
+// The bracket (prescale) and (-u)^m (postscale) are complex diagonal scalings done on the
+// VALU; the middle is a product with the real, constant Pascal matrix C, done with
+// v_mfma_f64_16x16x4f64: A = C (16 local coefficients x 4 multipole coefficients, staged in
+// LDS), B = the prescaled multipoles as real columns.  A wave owns 4 target boxes of one
+// parity class (so o and u are wave-uniform); its 16 B columns are (target, series, re/im).
+// Operand layout of the instruction (lane l, register r):
+//   A[l%16][l/16],  B[l/16][l%16],  D[l/16 + 4r][l%16].
+// ------------------------------------------------------------------------------------------
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+struct M2LLevels {
+  int nent;                   // number of levels handled (2..L)
+  int L;
+  int total;                  // logical blocks
+  int lev[EB_MAXLEV];
+  int blk0[EB_MAXLEV + 1];    // first logical block of each level
+  int W[EB_MAXLEV];
+  int NBP[EB_MAXLEV];
+  long long off[EB_MAXLEV];   // offset of the level's expansions (double2 elements)
+};
+
+// value of lane l^1 (DPP quad_perm [1,0,3,2])
+__device__ __forceinline__ double swap_adjacent(double v) {
+  int lo = __double2loint(v), hi = __double2hiint(v);
+  lo = __builtin_amdgcn_mov_dpp(lo, 0xB1, 0xF, 0xF, false);
+  hi = __builtin_amdgcn_mov_dpp(hi, 0xB1, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+template <int KMAX, int KS, int MT>
+__global__ __launch_bounds__(256) void k_m2l_mfma(M2LLevels LS, const int* __restrict__ leaf_start,
+                                                  const double2* __restrict__ mp, double2* __restrict__ lo,
+                                                  const double* __restrict__ Ct, const double2* __restrict__ U,
+                                                  const double2* __restrict__ NU, const int* __restrict__ offs) {
+  static_assert(KS * 4 <= KMAX && MT * 16 <= KMAX, "tile exceeds the padded expansion length");
+  constexpr int CS = MT * 16 + 4;  // padded LDS row of the Pascal tile
+  __shared__ double Cs[KS * 4 * CS];
+  // XCD-aware order: consecutive logical blocks (neighbouring rows, all four parity
+  // classes) run on the same XCD and share its L2.
+  const int nphys = gridDim.x;
+  const int b = (blockIdx.x % 8) * (nphys / 8) + blockIdx.x / 8;
+  for (int i = threadIdx.x; i < KS * 4 * MT * 16; i += blockDim.x) {
+    const int k = i / (MT * 16), m = i % (MT * 16);
+    Cs[k * CS + m] = Ct[k * KMAX + m];
+  }
+  __syncthreads();
+  if (b >= LS.total) return;
+  int e = 0;
+  while (e + 1 < LS.nent && b >= LS.blk0[e + 1]) ++e;
+  const int l = LS.lev[e], W = LS.W[e], NBP = LS.NBP[e];
+  const int local = b - LS.blk0[e];
+  const int cls = local & 3, g = local >> 2;
+  const int half = 1 << (l - 1);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int col = lane & 15, kr = lane >> 4;
+  const int t = col >> 2, s = (col >> 1) & 1, comp = col & 1;
+  const int q = (g * 4 + wv) * 4 + t;
+  const bool valid = q < half * half;
+  const int qq = valid ? q : 0;
+  const int ix = 2 * (qq % half) + (cls & 1), iy = 2 * (qq / half) + (cls >> 1);
+  int bs = 0, be = 0;
+  if (valid) box_range(leaf_start, LS.L, l, ix, iy, bs, be);
+  const bool live = valid && be > bs;
+  const int pt = pidx(ix, iy, W);
+  const double2* mps = mp + LS.off[e] + (size_t)s * KMAX * NBP;
+  double lam[MT][4];
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) lam[mt][r] = 0.0;
+  for (int o = 0; o < 27; ++o) {
+    const int dx = offs[(cls * 27 + o) * 2 + 0];
+    const int dy = offs[(cls * 27 + o) * 2 + 1];
+    const double2* src = mps + pt + dy * W + dx;
+    const double2* uu = U + (size_t)(cls * 27 + o) * KMAX;
+    double xb[KS];
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+      const int k = 4 * ks + kr;
+      const double2 a = src[(size_t)k * NBP];
+      const double2 w = uu[k];
+      xb[ks] = comp ? fma(w.x, a.y, w.y * a.x) : fma(w.x, a.x, -w.y * a.y);
+    }
+    v4d acc[MT];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) acc[mt] = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt)
+        acc[mt] = __builtin_amdgcn_mfma_f64_16x16x4f64(Cs[(4 * ks + kr) * CS + 16 * mt + col], xb[ks], acc[mt],
+                                                       0, 0, 0);
+    const double2* nu = NU + (size_t)(cls * 27 + o) * KMAX;
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const double2 w = nu[16 * mt + kr + 4 * r];
+        const double v = acc[mt][r];
+        const double vo = swap_adjacent(v);
+        const double yre = comp ? vo : v, yim = comp ? v : vo;
+        lam[mt][r] += comp ? fma(w.x, yim, w.y * yre) : fma(w.x, yre, -w.y * yim);
+      }
+  }
+  if (live) {
+    double* out = (double*)(lo + LS.off[e]);
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = 16 * mt + kr + 4 * r;
+        out[2 * ((size_t)(s * KMAX + m) * NBP + pt) + comp] = lam[mt][r];
+      }
+  }
+}
+
+// L2L: lambda_l(child) += sum_{m' >= m} C(m',m) delta^(m'-m) 2^-m lambda_{l-1}(parent)_m'.
+// One thread per (child box, series, block of MB coefficients); the class (child position)
+// is fixed per block so the weights are wave-uniform.
+template <int KMAX, int MB>
+__global__ __launch_bounds__(256) void k_l2l(int L, int l, const int* __restrict__ leaf_start,
+                                             double2* __restrict__ lo, int W, int NBP,
+                                             const double2* __restrict__ lop, int Wp, int NBPp,
+                                             const double2* __restrict__ WL) {
+  const int cls = blockIdx.y & 3;
+  const int rest = blockIdx.y >> 2;
+  const int s = rest & 1;
+  const int m0 = (rest >> 1) * MB;
+  const int half = 1 << (l - 1);
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= half * half) return;
+  const int ix = 2 * (t % half) + (cls & 1);
+  const int iy = 2 * (t / half) + (cls >> 1);
+  int bs, be;
+  box_range(leaf_start, L, l, ix, iy, bs, be);
+  if (be == bs) return;
+  const int pp = pidx(ix >> 1, iy >> 1, Wp);
+  const double2* wl = WL + (size_t)cls * KMAX * KMAX;
+  const double2* par = lop + (size_t)s * KMAX * NBPp + pp;
+  double2 acc[MB];
+#pragma unroll
+  for (int i = 0; i < MB; ++i) acc[i] = make_double2(0, 0);
+#pragma unroll 4
+  for (int mq = m0; mq < KMAX; ++mq) {
+    const double2 a = par[(size_t)mq * NBPp];
+#pragma unroll
+    for (int i = 0; i < MB; ++i) cfma(acc[i], wl[mq * KMAX + m0 + i], a);
+  }
+  double2* out = lo + (size_t)(s * KMAX + m0) * NBP + pidx(ix, iy, W);
+#pragma unroll
+  for (int i = 0; i < MB; ++i) {
+    const double2 v = out[(size_t)i * NBP];
+    out[(size_t)i * NBP] = cadd(v, acc[i]);
+  }
 }
 
 // L2P at the finest level (SLLVALDR, cadapdrop2.f:331-348): one thread per point.
@@ -677,7 +861,7 @@ static int build_tree(eb_fmm_plan P, const double* x, const double* y, const dou
   const int nleaf = 1 << (2 * L);
   const int bbblocks = std::min(1024, (n + BB_THREADS - 1) / BB_THREADS);
   hipLaunchKernelGGL(k_bbox_part, dim3(bbblocks), dim3(BB_THREADS), 0, st, x, y, n, P->bbpart);
-  hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(64), 0, st, P->bbpart, bbblocks, P->geom);
+  hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(BB_THREADS), 0, st, P->bbpart, bbblocks, P->geom);
   hipLaunchKernelGGL(k_keys, dim3((n + 255) / 256), dim3(256), 0, st, x, y, n, L, P->geom, P->zs0, P->keys,
                      P->perm);
   EB_HIP_CHECK(hipGetLastError());
@@ -791,6 +975,25 @@ extern "C" int eb_fmm_plan_create(eb_fmm_plan* out, int32_t n, const double* x, 
   return EB_OK;
 }
 
+// (KS, MT) = (ceil(p/4), ceil(p/16)) for the padded length K
+template <int K, int KS>
+static void launch_m2l_ks(eb_fmm_plan P, const M2LLevels& LS, int grid) {
+  constexpr int MT = K / 16;
+  hipLaunchKernelGGL((k_m2l_mfma<K, KS, MT>), dim3(grid), dim3(256), 0, P->st, LS, P->leaf_start, P->mp, P->lo,
+                     P->dCt, P->dU, P->dNU, P->doffs);
+}
+
+template <int K>
+static void launch_m2l_mfma(eb_fmm_plan P, const M2LLevels& LS, int grid) {
+  const int ks = (P->p + 3) / 4;
+  switch (ks - (K / 4 - 3)) {  // the four k-step counts that round up to K
+    case 0: launch_m2l_ks<K, K / 4 - 3>(P, LS, grid); break;
+    case 1: launch_m2l_ks<K, K / 4 - 2>(P, LS, grid); break;
+    case 2: launch_m2l_ks<K, K / 4 - 1>(P, LS, grid); break;
+    default: launch_m2l_ks<K, K / 4>(P, LS, grid); break;
+  }
+}
+
 template <int K>
 static void launch_far(eb_fmm_plan P) {
   hipStream_t st = P->st;
@@ -808,14 +1011,33 @@ static void launch_far(eb_fmm_plan P) {
                        P->lvl_W[l], P->lvl_NBP[l], P->dTM);
   }
   if (P->timing) (void)hipEventRecord(P->ev[3], st);
-  for (int l = 2; l <= L; ++l) {
+  {
+    // M2L for every level in one launch, 16 target boxes (4 waves x 4) per block
+    M2LLevels LS{};
+    LS.L = L;
+    int tot = 0;
+    for (int l = 2; l <= L; ++l) {
+      const int e = l - 2;
+      const int half = 1 << (l - 1);
+      LS.lev[e] = l;
+      LS.blk0[e] = tot;
+      LS.W[e] = P->lvl_W[l];
+      LS.NBP[e] = P->lvl_NBP[l];
+      LS.off[e] = (long long)P->lvl_off[l];
+      tot += 4 * ((half * half + 15) / 16);
+    }
+    LS.nent = L - 1;
+    LS.blk0[LS.nent] = tot;
+    LS.total = tot;
+    const int grid = (tot + 7) / 8 * 8;
+    launch_m2l_mfma<K>(P, LS, grid);
+  }
+  for (int l = 3; l <= L; ++l) {
     const int half = 1 << (l - 1);
-    const int nt = half * half;
-    dim3 grid((nt + 255) / 256, 4 * 2 * (K / MB));
-    const double2* lop = (l > 2) ? P->lo + P->lvl_off[l - 1] : nullptr;
-    hipLaunchKernelGGL((k_m2l<K, MB>), grid, dim3(256), 0, st, L, l, P->leaf_start, P->mp + P->lvl_off[l],
-                       P->lo + P->lvl_off[l], P->lvl_W[l], P->lvl_NBP[l], lop, (l > 2) ? P->lvl_W[l - 1] : 0,
-                       (l > 2) ? P->lvl_NBP[l - 1] : 0, P->dCt, P->dU, P->dNU, P->dWL, P->doffs);
+    dim3 grid((half * half + 255) / 256, 4 * 2 * (K / MB));
+    hipLaunchKernelGGL((k_l2l<K, MB>), grid, dim3(256), 0, st, L, l, P->leaf_start, P->lo + P->lvl_off[l],
+                       P->lvl_W[l], P->lvl_NBP[l], P->lo + P->lvl_off[l - 1], P->lvl_W[l - 1], P->lvl_NBP[l - 1],
+                       P->dWL);
   }
   if (P->timing) (void)hipEventRecord(P->ev[4], st);
   hipLaunchKernelGGL(k_l2p<K>, dim3((P->n + 255) / 256), dim3(256), 0, st, P->n, L, P->keys, P->zs,
@@ -1029,24 +1251,24 @@ extern "C" int eb_cadap_tables(int32_t ilev, int32_t n, const double* x, const d
   const int bbblocks = std::min(1024, (n + BB_THREADS - 1) / BB_THREADS);
   Geom* g = nullptr;
   double4* part = nullptr;
-  int *first = nullptr, *flag = nullptr, *rank = nullptr, *tmp = nullptr, *dmax = nullptr;
+  int *first = nullptr, *flag = nullptr, *rank = nullptr, *tmp = nullptr, *dmax = nullptr, *stmp = nullptr;
   uint32_t *k0 = nullptr, *k1 = nullptr;
   int32_t *v0 = nullptr, *v1 = nullptr;
   int rc = EB_OK;
   auto cleanup = [&]() {
-    void* ptrs[] = {g, part, first, flag, rank, tmp, dmax, k0, k1, v0, v1};
+    void* ptrs[] = {g, part, first, flag, rank, tmp, dmax, k0, k1, v0, v1, stmp};
     for (void* q : ptrs)
       if (q) (void)hipFree(q);
   };
   if ((rc = dmalloc(&g, 1)) || (rc = dmalloc(&part, bbblocks)) || (rc = dmalloc(&first, nb)) ||
       (rc = dmalloc(&flag, n)) || (rc = dmalloc(&rank, n)) || (rc = dmalloc(&tmp, radix_sort_temp_ints(n))) ||
       (rc = dmalloc(&dmax, 1)) || (rc = dmalloc(&k0, n)) || (rc = dmalloc(&k1, n)) || (rc = dmalloc(&v0, n)) ||
-      (rc = dmalloc(&v1, n))) {
+      (rc = dmalloc(&v1, n)) || (rc = dmalloc(&stmp, scan_temp_ints(std::max(n, nb))))) {
     cleanup();
     return rc;
   }
   hipLaunchKernelGGL(k_bbox_part, dim3(bbblocks), dim3(BB_THREADS), 0, st, x, y, n, part);
-  hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(64), 0, st, part, bbblocks, g);
+  hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(BB_THREADS), 0, st, part, bbblocks, g);
   bool ok = hipMemsetAsync(icnt, 0, sizeof(int) * nb, st) == hipSuccess &&
             hipMemsetAsync(iloc, 0, sizeof(int) * nb, st) == hipSuccess &&
             hipMemsetAsync(first, 0x7f, sizeof(int) * nb, st) == hipSuccess &&
@@ -1058,14 +1280,14 @@ extern "C" int eb_cadap_tables(int32_t ilev, int32_t n, const double* x, const d
   }
   const int gb = (n + 255) / 256;
   hipLaunchKernelGGL(k_tab_box, dim3(gb), dim3(256), 0, st, x, y, n, ilev, g, ibox, icnt, first, k0, v0);
-  if ((rc = scan_exclusive(icnt, ioffst, nb, true, st)) ||
+  if ((rc = scan_exclusive(icnt, ioffst, nb, true, stmp, st)) ||
       (rc = radix_sort_pairs(k0, v0, k1, v1, n, 2 * ilev, tmp, st))) {
     cleanup();
     return rc;
   }
   hipLaunchKernelGGL(k_tab_final, dim3(gb), dim3(256), 0, st, v0, n, iatadr);
   hipLaunchKernelGGL(k_tab_flags, dim3(gb), dim3(256), 0, st, ibox, first, n, flag);
-  if ((rc = scan_exclusive(flag, rank, n, false, st))) {
+  if ((rc = scan_exclusive(flag, rank, n, false, stmp, st))) {
     cleanup();
     return rc;
   }

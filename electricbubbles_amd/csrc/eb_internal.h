@@ -10,7 +10,8 @@
 extern "C" void eb_set_error(const char* fmt, ...);
 
 namespace eb {
-int scan_exclusive(const int* in, int* out, int n, bool write_total, hipStream_t st);
+size_t scan_temp_ints(int n);
+int scan_exclusive(const int* in, int* out, int n, bool write_total, int* tmp, hipStream_t st);
 size_t radix_sort_temp_ints(int n);
 int radix_sort_pairs(uint32_t* k0, int32_t* v0, uint32_t* k1, int32_t* v1, int n, int nbits, int* tmp,
                      hipStream_t st);

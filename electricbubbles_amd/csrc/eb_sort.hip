@@ -109,15 +109,90 @@ __global__ __launch_bounds__(1024) void k_scan1(const int* __restrict__ in, int*
   if (write_total && t == 1023) out[n] = part[1023];
 }
 
-int scan_exclusive(const int* in, int* out, int n, bool write_total, hipStream_t st) {
-  hipLaunchKernelGGL(k_scan1, dim3(1), dim3(1024), 0, st, in, out, n, write_total ? 1 : 0);
+// Three-phase exclusive scan for larger arrays: per-tile sums, a scan of the tile sums, then
+// each tile scanned again with its offset.  Tiles of SC_TILE ints are staged through LDS so
+// every global access is coalesced.
+constexpr int SC_THREADS = 256;
+constexpr int SC_ITEMS = 8;
+constexpr int SC_TILE = SC_THREADS * SC_ITEMS;
+
+__global__ __launch_bounds__(SC_THREADS) void k_scan_tilesum(const int* __restrict__ in, int n, int* __restrict__ sums) {
+  __shared__ int red[SC_THREADS];
+  const int base = blockIdx.x * SC_TILE;
+  int s = 0;
+  for (int i = threadIdx.x; i < SC_TILE; i += SC_THREADS) {
+    const int e = base + i;
+    if (e < n) s += in[e];
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = SC_THREADS / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) sums[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(SC_THREADS) void k_scan_tile(const int* __restrict__ in, int* __restrict__ out, int n,
+                                                          const int* __restrict__ offs, int write_total) {
+  __shared__ int tile[SC_TILE];
+  __shared__ int part[SC_THREADS];
+  const int base = blockIdx.x * SC_TILE;
+  for (int i = threadIdx.x; i < SC_TILE; i += SC_THREADS) {
+    const int e = base + i;
+    tile[i] = e < n ? in[e] : 0;
+  }
+  __syncthreads();
+  int loc[SC_ITEMS];
+  int s = 0;
+#pragma unroll
+  for (int j = 0; j < SC_ITEMS; ++j) {
+    loc[j] = s;
+    s += tile[threadIdx.x * SC_ITEMS + j];
+  }
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 1; off < SC_THREADS; off <<= 1) {
+    const int v = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  const int run = offs[blockIdx.x] + part[threadIdx.x] - s;
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < SC_ITEMS; ++j) tile[threadIdx.x * SC_ITEMS + j] = run + loc[j];
+  __syncthreads();
+  for (int i = threadIdx.x; i < SC_TILE; i += SC_THREADS) {
+    const int e = base + i;
+    if (e < n) out[e] = tile[i];
+  }
+  if (write_total && blockIdx.x == gridDim.x - 1 && threadIdx.x == SC_THREADS - 1)
+    out[n] = offs[blockIdx.x] + part[SC_THREADS - 1];
+}
+
+size_t scan_temp_ints(int n) { return (size_t)2 * ((n + SC_TILE - 1) / SC_TILE) + 1; }
+
+// Exclusive scan; `tmp` holds scan_temp_ints(n) ints owned by the caller.
+int scan_exclusive(const int* in, int* out, int n, bool write_total, int* tmp, hipStream_t st) {
+  const int nt = (n + SC_TILE - 1) / SC_TILE;
+  if (nt <= 1 || tmp == nullptr) {
+    hipLaunchKernelGGL(k_scan1, dim3(1), dim3(1024), 0, st, in, out, n, write_total ? 1 : 0);
+    EB_HIP_CHECK(hipGetLastError());
+    return EB_OK;
+  }
+  int* sums = tmp;
+  int* offs = tmp + nt;
+  hipLaunchKernelGGL(k_scan_tilesum, dim3(nt), dim3(SC_THREADS), 0, st, in, n, sums);
+  hipLaunchKernelGGL(k_scan1, dim3(1), dim3(1024), 0, st, sums, offs, nt, 0);
+  hipLaunchKernelGGL(k_scan_tile, dim3(nt), dim3(SC_THREADS), 0, st, in, out, n, offs, write_total ? 1 : 0);
   EB_HIP_CHECK(hipGetLastError());
   return EB_OK;
 }
 
 size_t radix_sort_temp_ints(int n) {
   int nblk = (n + RS_TILE - 1) / RS_TILE;
-  return (size_t)2 * RS_RADIX * nblk;
+  return (size_t)2 * RS_RADIX * nblk + scan_temp_ints(RS_RADIX * nblk);
 }
 
 // Sorts (k0, v0) by the low `nbits` of the keys; the result ends in (k0, v0).  k1/v1 are
@@ -140,7 +215,7 @@ int radix_sort_pairs(uint32_t* k0, int32_t* v0, uint32_t* k1, int32_t* v1, int n
     if (shift < 32) {
       hipLaunchKernelGGL(k_rs_hist, dim3(nblk), dim3(RS_THREADS), 0, st, ka, n, shift, nblk, hist);
       EB_HIP_CHECK(hipGetLastError());
-      int rc = scan_exclusive(hist, hscan, RS_RADIX * nblk, false, st);
+      int rc = scan_exclusive(hist, hscan, RS_RADIX * nblk, false, tmp + 2 * RS_RADIX * nblk, st);
       if (rc) return rc;
       hipLaunchKernelGGL(k_rs_scatter, dim3(nblk), dim3(RS_THREADS), 0, st, ka, va, kb, vb, n, shift, nblk,
                          hscan);
