@@ -173,19 +173,23 @@ __global__ void k_zero_int(int* __restrict__ a, int n) {
 
 // number of non-empty boxes per level (the reference's NTBOX, cadap2.f:331-336): a leaf is
 // counted at level l when it holds points and is the first non-empty leaf of its ancestor.
-// Counts are aggregated per wave before the atomic add.
+// Counts are summed per block in LDS before one atomic add per level.
 __global__ __launch_bounds__(256) void k_ntbox(const int* __restrict__ leaf_start, int L, int* __restrict__ ntbox) {
+  __shared__ int cnt[EB_MAXLEV + 1];
+  if (threadIdx.x <= EB_MAXLEV) cnt[threadIdx.x] = 0;
+  __syncthreads();
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   const int nleaf = 1 << (2 * L);
-  const bool in = m < nleaf;
-  const int s = in ? leaf_start[m] : 0;
-  const bool nonempty = in && leaf_start[m + 1] != s;
-  for (int l = 1; l <= L; ++l) {
-    const int sh = 2 * (L - l);
-    const bool first = nonempty && leaf_start[(m >> sh) << sh] == s;
-    const uint64_t b = __ballot(first);
-    if ((threadIdx.x & (EB_WAVE - 1)) == 0 && b) atomicAdd(&ntbox[l], __popcll(b));
+  if (m < nleaf) {
+    const int s = leaf_start[m];
+    if (leaf_start[m + 1] != s)
+      for (int l = 1; l <= L; ++l) {
+        const int sh = 2 * (L - l);
+        if (leaf_start[(m >> sh) << sh] == s) atomicAdd(&cnt[l], 1);
+      }
   }
+  __syncthreads();
+  if (threadIdx.x >= 1 && threadIdx.x <= L && cnt[threadIdx.x]) atomicAdd(&ntbox[threadIdx.x], cnt[threadIdx.x]);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -272,145 +276,36 @@ __global__ __launch_bounds__(64) void k_p2m(int L, int p, const int* __restrict_
   }
 }
 
-// M2M: parent alpha_{n+1} = sum_c sum_k TM[c][k][n] child alpha_{k+1}, with
-// TM[c][k][n] = C(n,k) 2^-(k+1) delta_c^(n-k), delta_c = (child centre - parent centre)/r_parent
-// (lower triangular: n >= k).  One thread per (parent box, series, block of MB outputs); lanes
-// are consecutive parents, so the weights are wave-uniform (scalar loads).  Empty parents
-// write zeros: the M2L pass reads every box of the level.
-template <int KMAX, int MB>
-__global__ __launch_bounds__(256) void k_m2m(int L, int l, const int* __restrict__ leaf_start,
-                                             const double2* __restrict__ mpc, int Wc, int NBPc,
-                                             double2* __restrict__ mpp, int Wp, int NBPp,
-                                             const double2* __restrict__ TM) {
-  const int s = blockIdx.y & 1;
-  const int n0 = (blockIdx.y >> 1) * MB;
-  const int nside = 1 << l;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nside * nside) return;
-  const int ix = t % nside, iy = t / nside;
-  int bs, be;
-  box_range(leaf_start, L, l, ix, iy, bs, be);
-  double2 acc[MB];
-#pragma unroll
-  for (int i = 0; i < MB; ++i) acc[i] = make_double2(0, 0);
-  if (be > bs) {
-    const int kend = min(KMAX, n0 + MB);
-    const double2* mc[4];
-#pragma unroll
-    for (int c = 0; c < 4; ++c)
-      mc[c] = mpc + (size_t)s * KMAX * NBPc + pidx(2 * ix + (c & 1), 2 * iy + (c >> 1), Wc);
-#pragma unroll 2
-    for (int k = 0; k < kend; ++k) {
-      double2 a[4];
-#pragma unroll
-      for (int c = 0; c < 4; ++c) a[c] = mc[c][(size_t)k * NBPc];
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        const double2* tm = TM + ((size_t)c * KMAX + k) * KMAX + n0;
-#pragma unroll
-        for (int i = 0; i < MB; ++i) cfma(acc[i], tm[i], a[c]);
-      }
-    }
-  }
-  double2* out = mpp + (size_t)(s * KMAX + n0) * NBPp + pidx(ix, iy, Wp);
-#pragma unroll
-  for (int i = 0; i < MB; ++i) out[(size_t)i * NBPp] = acc[i];
-}
-
-// M2L (+ L2L from the parent) for every non-empty box of level l.  One thread owns
-// (target box, series s, block of MB local coefficients); blockIdx.y fixes the target's
-// parity class, the series and the coefficient block, so the 27 list offsets, the scaling
-// powers and the Pascal coefficients are wave-uniform (scalar loads):
-//   lambda_m += (-u)^m sum_k C(m+k,k) u^(k+1) alpha_{k+1},   u = r / (c_t - c_s).
-template <int KMAX, int MB>
-__global__ __launch_bounds__(256) void k_m2l(int L, int l, const int* __restrict__ leaf_start,
-                                             const double2* __restrict__ mp, double2* __restrict__ lo, int W,
-                                             int NBP, const double2* __restrict__ lop, int Wp, int NBPp,
-                                             const double* __restrict__ Ct, const double2* __restrict__ U,
-                                             const double2* __restrict__ NU, const double2* __restrict__ WL,
-                                             const int* __restrict__ offs) {
-  constexpr int NB = KMAX / MB;
-  const int cls = blockIdx.y & 3;
-  const int rest = blockIdx.y >> 2;
-  const int s = rest % 2;
-  const int b = rest / 2;
-  const int half = 1 << (l - 1);
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= half * half) return;
-  const int ix = 2 * (t % half) + (cls & 1);
-  const int iy = 2 * (t / half) + (cls >> 1);
-  int bs, be;
-  box_range(leaf_start, L, l, ix, iy, bs, be);
-  if (be == bs) return;
-  const int pt = pidx(ix, iy, W);
-  const int m0 = b * MB;
-  double2 acc[MB];
-#pragma unroll
-  for (int i = 0; i < MB; ++i) acc[i] = make_double2(0, 0);
-  if (lop != nullptr) {
-    // L2L: lambda_m = sum_{m' >= m} C(m',m) delta^(m'-m) 2^-m lambda^parent_m'
-    const int pp = pidx(ix >> 1, iy >> 1, Wp);
-    const double2* wl = WL + (size_t)cls * KMAX * KMAX;
-    for (int mq = m0; mq < KMAX; ++mq) {
-      const double2 a = lop[(size_t)(s * KMAX + mq) * NBPp + pp];
-#pragma unroll
-      for (int i = 0; i < MB; ++i) cfma(acc[i], wl[mq * KMAX + m0 + i], a);
-    }
-  }
-  const double2* mps = mp + (size_t)s * KMAX * NBP;
-  for (int o = 0; o < 27; ++o) {
-    const int dx = offs[(cls * 27 + o) * 2 + 0];
-    const int dy = offs[(cls * 27 + o) * 2 + 1];
-    const int ps = pt + dy * W + dx;
-    const double2* uu = U + (size_t)(cls * 27 + o) * KMAX;
-    double2 yv[MB];
-#pragma unroll
-    for (int i = 0; i < MB; ++i) yv[i] = make_double2(0, 0);
-#pragma unroll 4
-    for (int k = 0; k < KMAX; ++k) {
-      const double2 a = mps[(size_t)k * NBP + ps];
-      const double2 xk = cmul(uu[k], a);
-      const double* ck = Ct + k * KMAX + m0;
-#pragma unroll
-      for (int i = 0; i < MB; ++i) {
-        yv[i].x = fma(ck[i], xk.x, yv[i].x);
-        yv[i].y = fma(ck[i], xk.y, yv[i].y);
-      }
-    }
-    const double2* nu = NU + (size_t)(cls * 27 + o) * KMAX + m0;
-#pragma unroll
-    for (int i = 0; i < MB; ++i) cfma(acc[i], nu[i], yv[i]);
-  }
-  double2* out = lo + (size_t)(s * KMAX + m0) * NBP + pt;
-#pragma unroll
-  for (int i = 0; i < MB; ++i) out[(size_t)i * NBP] = acc[i];
-  (void)NB;
-}
-
 // ------------------------------------------------------------------------------------------
-// M2L on FP64 matrix cores, every level in one launch.
+// Translations on FP64 matrix cores: M2M (upward), M2L (interaction lists), L2L (downward).
 //
-// For a target box t and a source box in its interaction list (offset o, u = r/(c_t - c_s)):
-//   lambda_m(t) += (-u)^m  sum_k C(m+k,k) [u^(k+1) alpha_{k+1}(s)]
-// The bracket (prescale) and (-u)^m (postscale) are complex diagonal scalings done on the
-// VALU; the middle is a product with the real, constant Pascal matrix C, done with
-// v_mfma_f64_16x16x4f64: A = C (16 local coefficients x 4 multipole coefficients, staged in
-// LDS), B = the prescaled multipoles as real columns.  A wave owns 4 target boxes of one
-// parity class (so o and u are wave-uniform); its 16 B columns are (target, series, re/im).
-// Operand layout of the instruction (lane l, register r):
+// With expansions scaled by the half-width of their box, each translation is
+//   out_m = post_m * sum_k Mat[k][m] * (pre_k * in_k)
+// with complex diagonal pre/post scalings and a real Pascal matrix Mat:
+//   M2L  (SLSHIFT, cadap2.f:1175-1213): Mat[k][m] = C(m+k,k), pre_k = u^(k+1), post_m = (-u)^m,
+//        u = r/(c_target - c_source), 27 sources per target (MKINTL, cadap2.f:780-828);
+//   M2M: Mat[k][n] = C(n,k), pre_k = 2^-(k+1) d^-k, post_n = d^n, 4 children per parent;
+//   L2L: Mat[m][j] = C(m,j), pre_m = d^m, post_j = (2d)^-j, 1 parent per child;
+// d = (child centre - parent centre)/r_parent = (+-1 +- i)/2.
+// The real product runs on v_mfma_f64_16x16x4f64 with A = Mat (LDS) and B = the prescaled
+// input as real columns; the scalings run on the VALU.  A wave owns 4 target boxes of one
+// parity class (so the source offsets and scalings are wave-uniform); its 16 B columns are
+// (target, series, re/im).  Operand layout (lane l, register r):
 //   A[l%16][l/16],  B[l/16][l%16],  D[l/16 + 4r][l%16].
 // ------------------------------------------------------------------------------------------
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-struct M2LLevels {
-  int nent;                   // number of levels handled (2..L)
+enum { XL_M2L = 0, XL_M2M = 1, XL_L2L = 2 };
+
+struct XlatLevels {
+  int nent;                   // target levels handled by the launch
   int L;
   int total;                  // logical blocks
   int lev[EB_MAXLEV];
-  int blk0[EB_MAXLEV + 1];    // first logical block of each level
-  int W[EB_MAXLEV];
-  int NBP[EB_MAXLEV];
-  long long off[EB_MAXLEV];   // offset of the level's expansions (double2 elements)
+  int blk0[EB_MAXLEV + 1];    // first logical block of each entry
+  int Wt[EB_MAXLEV], NBPt[EB_MAXLEV];
+  int Ws[EB_MAXLEV], NBPs[EB_MAXLEV];
+  long long offt[EB_MAXLEV], offs[EB_MAXLEV];  // expansion offsets (double2 elements)
 };
 
 // value of lane l^1 (DPP quad_perm [1,0,3,2])
@@ -421,29 +316,42 @@ __device__ __forceinline__ double swap_adjacent(double v) {
   return __hiloint2double(hi, lo);
 }
 
-template <int KMAX, int KS, int MT>
-__global__ __launch_bounds__(256) void k_m2l_mfma(M2LLevels LS, const int* __restrict__ leaf_start,
-                                                  const double2* __restrict__ mp, double2* __restrict__ lo,
-                                                  const double* __restrict__ Ct, const double2* __restrict__ U,
-                                                  const double2* __restrict__ NU, const int* __restrict__ offs) {
+template <int MODE, int KMAX, int KS, int MT>
+__global__ __launch_bounds__(256) void k_xlat(XlatLevels LS, const int* __restrict__ leaf_start,
+                                              const double2* __restrict__ src, double2* __restrict__ dst,
+                                              const double* __restrict__ Mat, const double2* __restrict__ pre,
+                                              const double2* __restrict__ post, const int* __restrict__ offs) {
   static_assert(KS * 4 <= KMAX && MT * 16 <= KMAX, "tile exceeds the padded expansion length");
-  constexpr int CS = MT * 16 + 4;  // padded LDS row of the Pascal tile
-  __shared__ double Cs[KS * 4 * CS];
-  // XCD-aware order: consecutive logical blocks (neighbouring rows, all four parity
-  // classes) run on the same XCD and share its L2.
+  constexpr int NS = MODE == XL_M2L ? 27 : (MODE == XL_M2M ? 4 : 1);
+  constexpr int NK = KS * 4, NM = MT * 16, CS = NM + 4;
+  __shared__ double Cs[NK * CS];
+  __shared__ double2 Ps[NS * NK];
+  __shared__ double2 Qs[NS * NM];
+  // XCD-aware order: consecutive logical blocks (neighbouring rows, all four parity classes)
+  // are dispatched to the same XCD and share its L2.
   const int nphys = gridDim.x;
   const int b = (blockIdx.x % 8) * (nphys / 8) + blockIdx.x / 8;
-  for (int i = threadIdx.x; i < KS * 4 * MT * 16; i += blockDim.x) {
-    const int k = i / (MT * 16), m = i % (MT * 16);
-    Cs[k * CS + m] = Ct[k * KMAX + m];
+  const int cls = b & 3;
+  for (int i = threadIdx.x; i < NK * NM; i += blockDim.x) {
+    const int k = i / NM, m = i % NM;
+    Cs[k * CS + m] = Mat[k * KMAX + m];
+  }
+  for (int i = threadIdx.x; i < NS * NK; i += blockDim.x) {
+    const int o = i / NK, k = i % NK;
+    const int row = MODE == XL_M2L ? cls * 27 + o : (MODE == XL_M2M ? o : cls);
+    Ps[i] = pre[(size_t)row * KMAX + k];
+  }
+  for (int i = threadIdx.x; i < NS * NM; i += blockDim.x) {
+    const int o = i / NM, m = i % NM;
+    const int row = MODE == XL_M2L ? cls * 27 + o : (MODE == XL_M2M ? o : cls);
+    Qs[i] = post[(size_t)row * KMAX + m];
   }
   __syncthreads();
   if (b >= LS.total) return;
   int e = 0;
   while (e + 1 < LS.nent && b >= LS.blk0[e + 1]) ++e;
-  const int l = LS.lev[e], W = LS.W[e], NBP = LS.NBP[e];
-  const int local = b - LS.blk0[e];
-  const int cls = local & 3, g = local >> 2;
+  const int l = LS.lev[e], Wt = LS.Wt[e], NBPt = LS.NBPt[e], Ws = LS.Ws[e], NBPs = LS.NBPs[e];
+  const int g = (b - LS.blk0[e]) >> 2;
   const int half = 1 << (l - 1);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int col = lane & 15, kr = lane >> 4;
@@ -452,28 +360,45 @@ __global__ __launch_bounds__(256) void k_m2l_mfma(M2LLevels LS, const int* __res
   const bool valid = q < half * half;
   const int qq = valid ? q : 0;
   const int ix = 2 * (qq % half) + (cls & 1), iy = 2 * (qq / half) + (cls >> 1);
-  int bs = 0, be = 0;
-  if (valid) box_range(leaf_start, LS.L, l, ix, iy, bs, be);
-  const bool live = valid && be > bs;
-  const int pt = pidx(ix, iy, W);
-  const double2* mps = mp + LS.off[e] + (size_t)s * KMAX * NBP;
+  bool live = valid;
+  if (MODE != XL_M2M && valid) {
+    int bs, be;
+    box_range(leaf_start, LS.L, l, ix, iy, bs, be);
+    live = be > bs;
+  }
+  const int pt = pidx(ix, iy, Wt);
+  const double2* srcs = src + LS.offs[e] + (size_t)s * KMAX * NBPs;
+  auto src_box = [&](int o) -> int {
+    if (MODE == XL_M2L) return pt + offs[(cls * 27 + o) * 2 + 1] * Ws + offs[(cls * 27 + o) * 2 + 0];
+    if (MODE == XL_M2M) return pidx(2 * ix + (o & 1), 2 * iy + (o >> 1), Ws);
+    return pidx(ix >> 1, iy >> 1, Ws);
+  };
+  double2 an[KS];
+  {
+    const int ps = src_box(0);
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) an[ks] = srcs[(size_t)(4 * ks + kr) * NBPs + ps];
+  }
   double lam[MT][4];
 #pragma unroll
   for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
     for (int r = 0; r < 4; ++r) lam[mt][r] = 0.0;
-  for (int o = 0; o < 27; ++o) {
-    const int dx = offs[(cls * 27 + o) * 2 + 0];
-    const int dy = offs[(cls * 27 + o) * 2 + 1];
-    const double2* src = mps + pt + dy * W + dx;
-    const double2* uu = U + (size_t)(cls * 27 + o) * KMAX;
+#pragma unroll(MODE == XL_M2L ? 1 : NS)
+  for (int o = 0; o < NS; ++o) {
+    double2 a[KS];
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) a[ks] = an[ks];
+    if (o + 1 < NS) {  // prefetch the next source while this one is multiplied
+      const int ps = src_box(o + 1);
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) an[ks] = srcs[(size_t)(4 * ks + kr) * NBPs + ps];
+    }
     double xb[KS];
 #pragma unroll
     for (int ks = 0; ks < KS; ++ks) {
-      const int k = 4 * ks + kr;
-      const double2 a = src[(size_t)k * NBP];
-      const double2 w = uu[k];
-      xb[ks] = comp ? fma(w.x, a.y, w.y * a.x) : fma(w.x, a.x, -w.y * a.y);
+      const double2 w = Ps[o * NK + 4 * ks + kr];
+      xb[ks] = comp ? fma(w.x, a[ks].y, w.y * a[ks].x) : fma(w.x, a[ks].x, -w.y * a[ks].y);
     }
     v4d acc[MT];
 #pragma unroll
@@ -484,12 +409,11 @@ __global__ __launch_bounds__(256) void k_m2l_mfma(M2LLevels LS, const int* __res
       for (int mt = 0; mt < MT; ++mt)
         acc[mt] = __builtin_amdgcn_mfma_f64_16x16x4f64(Cs[(4 * ks + kr) * CS + 16 * mt + col], xb[ks], acc[mt],
                                                        0, 0, 0);
-    const double2* nu = NU + (size_t)(cls * 27 + o) * KMAX;
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const double2 w = nu[16 * mt + kr + 4 * r];
+        const double2 w = Qs[o * NM + 16 * mt + kr + 4 * r];
         const double v = acc[mt][r];
         const double vo = swap_adjacent(v);
         const double yre = comp ? vo : v, yim = comp ? v : vo;
@@ -497,54 +421,15 @@ __global__ __launch_bounds__(256) void k_m2l_mfma(M2LLevels LS, const int* __res
       }
   }
   if (live) {
-    double* out = (double*)(lo + LS.off[e]);
+    double* out = (double*)(dst + LS.offt[e]);
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int m = 16 * mt + kr + 4 * r;
-        out[2 * ((size_t)(s * KMAX + m) * NBP + pt) + comp] = lam[mt][r];
+        const size_t at = 2 * ((size_t)(s * KMAX + m) * NBPt + pt) + comp;
+        out[at] = MODE == XL_L2L ? out[at] + lam[mt][r] : lam[mt][r];
       }
-  }
-}
-
-// L2L: lambda_l(child) += sum_{m' >= m} C(m',m) delta^(m'-m) 2^-m lambda_{l-1}(parent)_m'.
-// One thread per (child box, series, block of MB coefficients); the class (child position)
-// is fixed per block so the weights are wave-uniform.
-template <int KMAX, int MB>
-__global__ __launch_bounds__(256) void k_l2l(int L, int l, const int* __restrict__ leaf_start,
-                                             double2* __restrict__ lo, int W, int NBP,
-                                             const double2* __restrict__ lop, int Wp, int NBPp,
-                                             const double2* __restrict__ WL) {
-  const int cls = blockIdx.y & 3;
-  const int rest = blockIdx.y >> 2;
-  const int s = rest & 1;
-  const int m0 = (rest >> 1) * MB;
-  const int half = 1 << (l - 1);
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= half * half) return;
-  const int ix = 2 * (t % half) + (cls & 1);
-  const int iy = 2 * (t / half) + (cls >> 1);
-  int bs, be;
-  box_range(leaf_start, L, l, ix, iy, bs, be);
-  if (be == bs) return;
-  const int pp = pidx(ix >> 1, iy >> 1, Wp);
-  const double2* wl = WL + (size_t)cls * KMAX * KMAX;
-  const double2* par = lop + (size_t)s * KMAX * NBPp + pp;
-  double2 acc[MB];
-#pragma unroll
-  for (int i = 0; i < MB; ++i) acc[i] = make_double2(0, 0);
-#pragma unroll 4
-  for (int mq = m0; mq < KMAX; ++mq) {
-    const double2 a = par[(size_t)mq * NBPp];
-#pragma unroll
-    for (int i = 0; i < MB; ++i) cfma(acc[i], wl[mq * KMAX + m0 + i], a);
-  }
-  double2* out = lo + (size_t)(s * KMAX + m0) * NBP + pidx(ix, iy, W);
-#pragma unroll
-  for (int i = 0; i < MB; ++i) {
-    const double2 v = out[(size_t)i * NBP];
-    out[(size_t)i * NBP] = cadd(v, acc[i]);
   }
 }
 
@@ -712,11 +597,13 @@ struct LC {
 static inline LC lmul(LC a, LC b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
 
 struct Tables {
-  std::vector<double> Ct;     // [KMAX][KMAX]   Ct[k*KMAX+m] = C(m+k, k) (k < p, m < p) else 0
-  std::vector<double2> U;     // [4][27][KMAX]  u^(k+1)
-  std::vector<double2> NU;    // [4][27][KMAX]  (-u)^m
-  std::vector<double2> WL;    // [4][KMAX][KMAX] L2L  [c][m'][m]
-  std::vector<double2> TM;    // [4][KMAX][KMAX] M2M  [c][k][n]
+  std::vector<double> Mat;    // [3][KMAX][KMAX]  M2L C(m+k,k); M2M C(n,k); L2L C(m,j)  (k-major)
+  std::vector<double2> U;     // [4][27][KMAX]  M2L pre  u^(k+1)
+  std::vector<double2> NU;    // [4][27][KMAX]  M2L post (-u)^m
+  std::vector<double2> PM;    // [4][KMAX]      M2M pre  2^-(k+1) d_c^-k
+  std::vector<double2> QM;    // [4][KMAX]      M2M post d_c^n
+  std::vector<double2> PL;    // [4][KMAX]      L2L pre  d_c^m
+  std::vector<double2> QL;    // [4][KMAX]      L2L post (2 d_c)^-j
   std::vector<int> offs;      // [4][27][2]
 };
 
@@ -727,10 +614,26 @@ static LD binom(int n, int k) {
   return r;
 }
 
+static LC lpow(LC z, int n) {
+  LC r = {1, 0};
+  for (int i = 0; i < n; ++i) r = lmul(r, z);
+  return r;
+}
+
+static LC linv(LC z) {
+  const LD d = z.x * z.x + z.y * z.y;
+  return {z.x / d, -z.y / d};
+}
+
 static void build_tables(int p, int KMAX, Tables& T) {
-  T.Ct.assign((size_t)KMAX * KMAX, 0.0);
+  const size_t KK = (size_t)KMAX * KMAX;
+  T.Mat.assign(3 * KK, 0.0);
   for (int k = 0; k < p; ++k)
-    for (int m = 0; m < p; ++m) T.Ct[(size_t)k * KMAX + m] = (double)binom(m + k, k);
+    for (int m = 0; m < p; ++m) {
+      T.Mat[0 * KK + (size_t)k * KMAX + m] = (double)binom(m + k, k);
+      T.Mat[1 * KK + (size_t)k * KMAX + m] = (double)binom(m, k);  // [k][n] = C(n,k)
+      T.Mat[2 * KK + (size_t)k * KMAX + m] = (double)binom(k, m);  // [m][j] = C(m,j)
+    }
   T.U.assign((size_t)4 * 27 * KMAX, make_double2(0, 0));
   T.NU.assign((size_t)4 * 27 * KMAX, make_double2(0, 0));
   T.offs.assign(4 * 27 * 2, 0);
@@ -753,26 +656,25 @@ static void build_tables(int p, int KMAX, Tables& T) {
       }
     }
   }
-  T.WL.assign((size_t)4 * KMAX * KMAX, make_double2(0, 0));
-  T.TM.assign((size_t)4 * KMAX * KMAX, make_double2(0, 0));
+  T.PM.assign((size_t)4 * KMAX, make_double2(0, 0));
+  T.QM = T.PM;
+  T.PL = T.PM;
+  T.QL = T.PM;
   for (int c = 0; c < 4; ++c) {
-    // child centre - parent centre = (+-r_c, +-r_c);  delta = that / r_parent = (+-1 +- i)/2
-    LC d = {(c & 1) ? 0.5L : -0.5L, (c >> 1) ? 0.5L : -0.5L};
-    std::vector<LC> dp(2 * KMAX + 1);
-    dp[0] = {1, 0};
-    for (int i = 1; i <= 2 * KMAX; ++i) dp[i] = lmul(dp[i - 1], d);
-    for (int mq = 0; mq < p; ++mq)
-      for (int m = 0; m <= mq; ++m) {
-        LD s = binom(mq, m) * powl(0.5L, (LD)m);
-        LC v = {s * dp[mq - m].x, s * dp[mq - m].y};
-        T.WL[((size_t)c * KMAX + mq) * KMAX + m] = make_double2((double)v.x, (double)v.y);
-      }
-    for (int k = 0; k < p; ++k)
-      for (int n = k; n < p; ++n) {
-        LD s = binom(n, k) * powl(0.5L, (LD)(k + 1));
-        LC v = {s * dp[n - k].x, s * dp[n - k].y};
-        T.TM[((size_t)c * KMAX + k) * KMAX + n] = make_double2((double)v.x, (double)v.y);
-      }
+    // child centre - parent centre = (+-r_c, +-r_c);  d = that / r_parent = (+-1 +- i)/2
+    const LC d = {(c & 1) ? 0.5L : -0.5L, (c >> 1) ? 0.5L : -0.5L};
+    const LC di = linv(d);
+    const LC d2i = linv(LC{2 * d.x, 2 * d.y});
+    for (int k = 0; k < KMAX; ++k) {
+      LC a = lpow(di, k);
+      const LD h = powl(0.5L, (LD)(k + 1));
+      T.PM[(size_t)c * KMAX + k] = make_double2((double)(h * a.x), (double)(h * a.y));
+      LC b = lpow(d, k);
+      T.QM[(size_t)c * KMAX + k] = make_double2((double)b.x, (double)b.y);
+      T.PL[(size_t)c * KMAX + k] = make_double2((double)b.x, (double)b.y);
+      LC q = lpow(d2i, k);
+      T.QL[(size_t)c * KMAX + k] = make_double2((double)q.x, (double)q.y);
+    }
   }
 }
 
@@ -804,8 +706,8 @@ struct eb_fmm_plan_s {
   size_t lvl_off[EB_MAXLEV + 2] = {0};
   int lvl_W[EB_MAXLEV + 2] = {0};
   int lvl_NBP[EB_MAXLEV + 2] = {0};
-  double* dCt = nullptr;
-  double2 *dU = nullptr, *dNU = nullptr, *dWL = nullptr, *dTM = nullptr;
+  double* dMat = nullptr;
+  double2 *dU = nullptr, *dNU = nullptr, *dPM = nullptr, *dQM = nullptr, *dPL = nullptr, *dQL = nullptr;
   int* doffs = nullptr;
   // timing
   bool timing = false;
@@ -816,8 +718,8 @@ struct eb_fmm_plan_s {
 static void plan_free(eb_fmm_plan P) {
   if (!P) return;
   void* ptrs[] = {P->geom, P->bbpart, P->zs0, P->zs, P->wdzs, P->qa, P->qb, P->gwf, P->keys, P->keys2,
-                  P->perm, P->vals2, P->sorttmp, P->leaf_start, P->ntbox, P->mp, P->lo, P->dCt,
-                  P->dU, P->dNU, P->dWL, P->dTM, P->doffs};
+                  P->perm, P->vals2, P->sorttmp, P->leaf_start, P->ntbox, P->mp, P->lo, P->dMat,
+                  P->dU, P->dNU, P->dPM, P->dQM, P->dPL, P->dQL, P->doffs};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   for (auto& e : P->ev)
@@ -871,8 +773,6 @@ static int build_tree(eb_fmm_plan P, const double* x, const double* y, const dou
                      P->leaf_start);
   hipLaunchKernelGGL(k_gather, dim3((n + 255) / 256), dim3(256), 0, st, n, P->perm, P->zs0,
                      (const double2*)wdz, P->zs, P->wdzs);
-  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, P->ntbox, EB_MAXLEV + 2);
-  hipLaunchKernelGGL(k_ntbox, dim3((nleaf + 255) / 256), dim3(256), 0, st, P->leaf_start, L, P->ntbox);
   EB_HIP_CHECK(hipGetLastError());
   return EB_OK;
 }
@@ -951,17 +851,21 @@ extern "C" int eb_fmm_plan_create(eb_fmm_plan* out, int32_t n, const double* x, 
                : (eb_set_error("hipMemsetAsync failed"), EB_ERR_HIP));
     Tables T;
     build_tables(P->p, K, T);
-    EB_TRY(dmalloc(&P->dCt, T.Ct.size()));
+    EB_TRY(dmalloc(&P->dMat, T.Mat.size()));
     EB_TRY(dmalloc(&P->dU, T.U.size()));
     EB_TRY(dmalloc(&P->dNU, T.NU.size()));
-    EB_TRY(dmalloc(&P->dWL, T.WL.size()));
-    EB_TRY(dmalloc(&P->dTM, T.TM.size()));
+    EB_TRY(dmalloc(&P->dPM, T.PM.size()));
+    EB_TRY(dmalloc(&P->dQM, T.QM.size()));
+    EB_TRY(dmalloc(&P->dPL, T.PL.size()));
+    EB_TRY(dmalloc(&P->dQL, T.QL.size()));
     EB_TRY(dmalloc(&P->doffs, T.offs.size()));
-    bool ok = hipMemcpyAsync(P->dCt, T.Ct.data(), sizeof(double) * T.Ct.size(), hipMemcpyHostToDevice, st) == hipSuccess &&
+    bool ok = hipMemcpyAsync(P->dMat, T.Mat.data(), sizeof(double) * T.Mat.size(), hipMemcpyHostToDevice, st) == hipSuccess &&
               hipMemcpyAsync(P->dU, T.U.data(), sizeof(double2) * T.U.size(), hipMemcpyHostToDevice, st) == hipSuccess &&
               hipMemcpyAsync(P->dNU, T.NU.data(), sizeof(double2) * T.NU.size(), hipMemcpyHostToDevice, st) == hipSuccess &&
-              hipMemcpyAsync(P->dWL, T.WL.data(), sizeof(double2) * T.WL.size(), hipMemcpyHostToDevice, st) == hipSuccess &&
-              hipMemcpyAsync(P->dTM, T.TM.data(), sizeof(double2) * T.TM.size(), hipMemcpyHostToDevice, st) == hipSuccess &&
+              hipMemcpyAsync(P->dPM, T.PM.data(), sizeof(double2) * T.PM.size(), hipMemcpyHostToDevice, st) == hipSuccess &&
+              hipMemcpyAsync(P->dQM, T.QM.data(), sizeof(double2) * T.QM.size(), hipMemcpyHostToDevice, st) == hipSuccess &&
+              hipMemcpyAsync(P->dPL, T.PL.data(), sizeof(double2) * T.PL.size(), hipMemcpyHostToDevice, st) == hipSuccess &&
+              hipMemcpyAsync(P->dQL, T.QL.data(), sizeof(double2) * T.QL.size(), hipMemcpyHostToDevice, st) == hipSuccess &&
               hipMemcpyAsync(P->doffs, T.offs.data(), sizeof(int) * T.offs.size(), hipMemcpyHostToDevice, st) == hipSuccess;
     // the host vectors die at scope end: wait for the copies
     ok = ok && hipStreamSynchronize(st) == hipSuccess;
@@ -975,22 +879,49 @@ extern "C" int eb_fmm_plan_create(eb_fmm_plan* out, int32_t n, const double* x, 
   return EB_OK;
 }
 
-// (KS, MT) = (ceil(p/4), ceil(p/16)) for the padded length K
-template <int K, int KS>
-static void launch_m2l_ks(eb_fmm_plan P, const M2LLevels& LS, int grid) {
-  constexpr int MT = K / 16;
-  hipLaunchKernelGGL((k_m2l_mfma<K, KS, MT>), dim3(grid), dim3(256), 0, P->st, LS, P->leaf_start, P->mp, P->lo,
-                     P->dCt, P->dU, P->dNU, P->doffs);
+// Fill the level table of one translation launch: target levels [l0, l1] (one entry each),
+// sources at level l + dl.
+static XlatLevels xlat_levels(eb_fmm_plan P, int l0, int l1, int dl) {
+  XlatLevels LS{};
+  LS.L = P->L;
+  int tot = 0, e = 0;
+  for (int l = l0; l <= l1; ++l, ++e) {
+    const int half = 1 << (l - 1);
+    LS.lev[e] = l;
+    LS.blk0[e] = tot;
+    LS.Wt[e] = P->lvl_W[l];
+    LS.NBPt[e] = P->lvl_NBP[l];
+    LS.offt[e] = (long long)P->lvl_off[l];
+    LS.Ws[e] = P->lvl_W[l + dl];
+    LS.NBPs[e] = P->lvl_NBP[l + dl];
+    LS.offs[e] = (long long)P->lvl_off[l + dl];
+    tot += 4 * ((half * half + 15) / 16);  // 16 target boxes per block, per parity class
+  }
+  LS.nent = e;
+  LS.blk0[e] = tot;
+  LS.total = tot;
+  return LS;
 }
 
-template <int K>
-static void launch_m2l_mfma(eb_fmm_plan P, const M2LLevels& LS, int grid) {
-  const int ks = (P->p + 3) / 4;
-  switch (ks - (K / 4 - 3)) {  // the four k-step counts that round up to K
-    case 0: launch_m2l_ks<K, K / 4 - 3>(P, LS, grid); break;
-    case 1: launch_m2l_ks<K, K / 4 - 2>(P, LS, grid); break;
-    case 2: launch_m2l_ks<K, K / 4 - 1>(P, LS, grid); break;
-    default: launch_m2l_ks<K, K / 4>(P, LS, grid); break;
+template <int MODE, int K, int KS>
+static void launch_xlat_ks(eb_fmm_plan P, const XlatLevels& LS, const double2* src, double2* dst,
+                           const double2* pre, const double2* post) {
+  constexpr int MT = K / 16;
+  const int grid = (LS.total + 7) / 8 * 8;
+  const double* mat = P->dMat + (size_t)MODE * K * K;
+  hipLaunchKernelGGL((k_xlat<MODE, K, KS, MT>), dim3(grid), dim3(256), 0, P->st, LS, P->leaf_start, src, dst, mat,
+                     pre, post, P->doffs);
+}
+
+// (KS, MT) = (ceil(p/4), K/16): the four k-step counts that round up to K
+template <int MODE, int K>
+static void launch_xlat(eb_fmm_plan P, const XlatLevels& LS, const double2* src, double2* dst,
+                        const double2* pre, const double2* post) {
+  switch ((P->p + 3) / 4 - (K / 4 - 3)) {
+    case 0: launch_xlat_ks<MODE, K, K / 4 - 3>(P, LS, src, dst, pre, post); break;
+    case 1: launch_xlat_ks<MODE, K, K / 4 - 2>(P, LS, src, dst, pre, post); break;
+    case 2: launch_xlat_ks<MODE, K, K / 4 - 1>(P, LS, src, dst, pre, post); break;
+    default: launch_xlat_ks<MODE, K, K / 4>(P, LS, src, dst, pre, post); break;
   }
 }
 
@@ -998,47 +929,19 @@ template <int K>
 static void launch_far(eb_fmm_plan P) {
   hipStream_t st = P->st;
   const int L = P->L;
-  constexpr int MB = 16;
   if (P->timing) (void)hipEventRecord(P->ev[1], st);
   // P2M at the finest level
   hipLaunchKernelGGL(k_p2m<K>, dim3(1 << (2 * L)), dim3(64), 0, st, L, P->p, P->leaf_start, P->zs, P->qa, P->qb,
                      P->mp + P->lvl_off[L], P->lvl_W[L], P->lvl_NBP[L]);
   if (P->timing) (void)hipEventRecord(P->ev[2], st);
-  for (int l = L - 1; l >= 2; --l) {
-    const int nb = 1 << (2 * l);
-    hipLaunchKernelGGL((k_m2m<K, MB>), dim3((nb + 255) / 256, 2 * (K / MB)), dim3(256), 0, st, L, l, P->leaf_start,
-                       P->mp + P->lvl_off[l + 1], P->lvl_W[l + 1], P->lvl_NBP[l + 1], P->mp + P->lvl_off[l],
-                       P->lvl_W[l], P->lvl_NBP[l], P->dTM);
-  }
+  // upward pass, level by level
+  for (int l = L - 1; l >= 2; --l)
+    launch_xlat<XL_M2M, K>(P, xlat_levels(P, l, l, +1), P->mp, P->mp, P->dPM, P->dQM);
   if (P->timing) (void)hipEventRecord(P->ev[3], st);
-  {
-    // M2L for every level in one launch, 16 target boxes (4 waves x 4) per block
-    M2LLevels LS{};
-    LS.L = L;
-    int tot = 0;
-    for (int l = 2; l <= L; ++l) {
-      const int e = l - 2;
-      const int half = 1 << (l - 1);
-      LS.lev[e] = l;
-      LS.blk0[e] = tot;
-      LS.W[e] = P->lvl_W[l];
-      LS.NBP[e] = P->lvl_NBP[l];
-      LS.off[e] = (long long)P->lvl_off[l];
-      tot += 4 * ((half * half + 15) / 16);
-    }
-    LS.nent = L - 1;
-    LS.blk0[LS.nent] = tot;
-    LS.total = tot;
-    const int grid = (tot + 7) / 8 * 8;
-    launch_m2l_mfma<K>(P, LS, grid);
-  }
-  for (int l = 3; l <= L; ++l) {
-    const int half = 1 << (l - 1);
-    dim3 grid((half * half + 255) / 256, 4 * 2 * (K / MB));
-    hipLaunchKernelGGL((k_l2l<K, MB>), grid, dim3(256), 0, st, L, l, P->leaf_start, P->lo + P->lvl_off[l],
-                       P->lvl_W[l], P->lvl_NBP[l], P->lo + P->lvl_off[l - 1], P->lvl_W[l - 1], P->lvl_NBP[l - 1],
-                       P->dWL);
-  }
+  // interaction lists of every level in one launch, then the downward pass
+  launch_xlat<XL_M2L, K>(P, xlat_levels(P, 2, L, 0), P->mp, P->lo, P->dU, P->dNU);
+  for (int l = 3; l <= L; ++l)
+    launch_xlat<XL_L2L, K>(P, xlat_levels(P, l, l, -1), P->lo, P->lo, P->dPL, P->dQL);
   if (P->timing) (void)hipEventRecord(P->ev[4], st);
   hipLaunchKernelGGL(k_l2p<K>, dim3((P->n + 255) / 256), dim3(256), 0, st, P->n, L, P->keys, P->zs,
                      P->lo + P->lvl_off[L], P->lvl_W[L], P->lvl_NBP[L], P->sgn, P->gwf);
@@ -1182,6 +1085,10 @@ extern "C" int eb_fmm_plan_info(eb_fmm_plan P, double* reals3, int64_t* counts8)
 // per-level non-empty box counts (levels 1..levmax) of a plan, host copy
 extern "C" int eb_fmm_plan_ntbox(eb_fmm_plan P, int32_t* ntbox) {
   if (!P || !ntbox) return EB_ERR_ARG;
+  const int nleaf = 1 << (2 * P->L);
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, P->st, P->ntbox, EB_MAXLEV + 2);
+  hipLaunchKernelGGL(k_ntbox, dim3((nleaf + 255) / 256), dim3(256), 0, P->st, P->leaf_start, P->L, P->ntbox);
+  EB_HIP_CHECK(hipGetLastError());
   EB_HIP_CHECK(hipMemcpyAsync(ntbox, P->ntbox, sizeof(int) * (P->L + 1), hipMemcpyDeviceToHost, P->st));
   EB_HIP_CHECK(hipStreamSynchronize(P->st));
   return EB_OK;
