@@ -40,6 +40,7 @@ def flops_model(info, n, p):
         "p2m": 26.0 * p * n,
         "l2p": 26.0 * p * n,
         "m2m": (8.0 * p * p + 30.0 * p) * info["boxes"],
+        "l2l": (8.0 * p * p + 30.0 * p) * info["boxes"],
     }
 
 
@@ -80,7 +81,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", choices=["curves", "uniform"], default="curves")
     ap.add_argument("--levmax", type=int, default=8)
-    ap.add_argument("--nterms", type=int, default=36)
+    ap.add_argument("--nterms", type=int, default=32)
     ap.add_argument("--npts", type=int, default=1 << 20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-check", action="store_true", help="skip the accuracy spot check")

@@ -246,9 +246,9 @@ class FMMPlan:
         _check(lib().eb_fmm_enable_timing(self._h, 1 if on else 0))
 
     def stage_ms(self):
-        a = np.zeros(6)
+        a = np.zeros(7)
         _check(lib().eb_fmm_stage_ms(self._h, a.ctypes.data_as(_dp)))
-        return dict(zip(("charges", "p2m", "m2m", "m2l", "l2p", "p2p"), a.tolist()))
+        return dict(zip(("charges", "p2m", "m2m", "m2l", "l2l", "l2p", "p2p"), a.tolist()))
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:

@@ -213,41 +213,49 @@ __global__ void k_charges(int n, const int32_t* __restrict__ perm, const double2
 }
 
 // P2M at the finest level (SLMKXP, cadap2.f:444-466), one 64-lane block per leaf box.
-// Sources of a chunk write their powers t^m (t = (z-c)/r) into LDS; lane k then forms
+// A chunk of up to 16 sources writes its powers t^m (t = (z-c)/r) into LDS, four lanes per
+// source each covering a quarter of the exponents; lane k then forms
 //   alpha_{k+1} = -(1/r) sum qa t^k,  beta_{k+1} = -(1/r) sum qb t^k + k (1/r^2) sum qc t^(k-1).
 template <int KMAX>
 __global__ __launch_bounds__(64) void k_p2m(int L, int p, const int* __restrict__ leaf_start,
                                             const double2* __restrict__ zs, const double2* __restrict__ qa,
                                             const double2* __restrict__ qb, double2* __restrict__ mp, int W,
                                             int NBP) {
-  constexpr int CH = 32;
+  constexpr int CH = 16, QK = KMAX / 4;
   __shared__ double2 P[CH][KMAX + 1];
   __shared__ double2 A[CH], B[CH], E[CH];
   const int nside = 1 << L;
   const int ix = blockIdx.x % nside, iy = blockIdx.x / nside;
   const uint32_t mk = morton((uint32_t)ix, (uint32_t)iy);
   const int s = leaf_start[mk], e = leaf_start[mk + 1];
-  const double h = 64.0 / nside, r = 0.5 * h, ir = 2.0 / h;
+  const double h = 64.0 / nside, ir = 2.0 / h;
   const double2 c = make_double2((ix + 0.5) * h, (iy + 0.5) * h);
   const int lane = threadIdx.x;
+  const int js = lane & (CH - 1), qk = lane / CH;
   double2 a1 = make_double2(0, 0), a2 = make_double2(0, 0);
   for (int c0 = s; c0 < e; c0 += CH) {
     const int nj = min(CH, e - c0);
-    if (lane < nj) {
-      const int j = c0 + lane;
+    if (js < nj) {
+      const int j = c0 + js;
       const double2 z = zs[j];
       const double2 t = make_double2((z.x - c.x) * ir, (z.y - c.y) * ir);
-      const double2 q1 = qa[j], q2 = qb[j];
-      A[lane] = make_double2(-q1.x * ir, -q1.y * ir);
-      B[lane] = make_double2(-q2.x * ir, -q2.y * ir);
-      const double2 qc = cmul(make_double2(-z.x, z.y), q1);  // -conj(z) qa
-      E[lane] = make_double2(qc.x * ir * ir, qc.y * ir * ir);
-      double2 pw = make_double2(1.0, 0.0);
-      P[lane][0] = pw;
-#pragma unroll 8
-      for (int k = 1; k < KMAX; ++k) {
+      if (qk == 0) {
+        const double2 q1 = qa[j], q2 = qb[j];
+        A[js] = make_double2(-q1.x * ir, -q1.y * ir);
+        B[js] = make_double2(-q2.x * ir, -q2.y * ir);
+        const double2 qc = cmul(make_double2(-z.x, z.y), q1);  // -conj(z) qa
+        E[js] = make_double2(qc.x * ir * ir, qc.y * ir * ir);
+      }
+      // pw = t^(qk*QK) by squaring, then QK consecutive powers
+      double2 pw = make_double2(1.0, 0.0), bb = t;
+      for (int ee = qk * QK; ee > 0; ee >>= 1) {
+        if (ee & 1) pw = cmul(pw, bb);
+        bb = cmul(bb, bb);
+      }
+#pragma unroll
+      for (int k = 0; k < QK; ++k) {
+        P[js][qk * QK + k] = pw;
         pw = cmul(pw, t);
-        P[lane][k] = pw;
       }
     }
     __syncthreads();
@@ -257,10 +265,7 @@ __global__ __launch_bounds__(64) void k_p2m(int L, int p, const int* __restrict_
         const double2 pw = P[jj][lane];
         cfma(a1, A[jj], pw);
         cfma(a2, B[jj], pw);
-        if (lane > 0) {
-          const double2 t2 = cmul(E[jj], P[jj][lane - 1]);
-          rfma(a2, fk, t2);
-        }
+        if (lane > 0) rfma(a2, fk, cmul(E[jj], P[jj][lane - 1]));
       }
     }
     __syncthreads();
@@ -467,11 +472,14 @@ __global__ __launch_bounds__(256) void k_l2p(int n, int L, const uint32_t* __res
 // One source-target pair of the near field, in the translation-invariant form
 //   qa/d + sgn (conj(qb)/conj(d) - conj(qa) d/conj(d)^2),  d = z_src - z_tgt
 // (SLDIRDR / SLSELFDR, cadapdrop2.f:586-599, 741-767, with QC = -conj(z) QA folded in).
-// acc1 collects qa/d, acc2 the bracket.
-__device__ __forceinline__ void p2p_pair(double2 zi, double2 zj, double2 qa, double2 qb, double2& acc1,
+// acc1 collects qa/d, acc2 the bracket.  `self` (source == target) contributes exactly zero:
+// the reciprocal is forced to 0, which zeroes both terms without a branch.
+__device__ __forceinline__ void p2p_pair(double2 zi, double2 zj, double2 qa, double2 qb, bool self, double2& acc1,
                                          double2& acc2) {
   const double dx = zj.x - zi.x, dy = zj.y - zi.y;
-  const double ir = rcp_nr(fma(dx, dx, dy * dy));
+  const double r2 = fma(dx, dx, dy * dy);
+  double ir = rcp_nr(self ? 1.0 : r2);
+  ir = self ? 0.0 : ir;
   const double wx = dx * ir, wy = dy * ir;  // w = 1/conj(d)
   // acc1 += qa * conj(w)
   acc1.x = fma(qa.x, wx, fma(qa.y, wy, acc1.x));
@@ -489,13 +497,18 @@ __device__ __forceinline__ void p2p_pair(double2 zi, double2 zj, double2 qa, dou
 
 // Near field at the finest level (DONN0DR, cadapdrop2.f:623-722) plus the final scaling
 // GW *= RSCAL (cadapdrop.f:370-373) and the scatter back to the caller's order.
-// One 64-lane block per target leaf; with T < 64 targets the lanes split into G = 64/T
-// source slices per target and the slices are summed through LDS.
+// One 64-lane block per target leaf.  The sources of the 3x3 neighbourhood (MKNBOR,
+// cadap2.f:705-742) are staged through LDS in chunks of P2P_CH points; with T < 64 targets
+// the lanes split into G = 64/T source slices per target, summed through LDS at the end.
+constexpr int P2P_CH = 256;
+
 __global__ __launch_bounds__(64) void k_p2p(int L, const int* __restrict__ leaf_start,
                                             const double2* __restrict__ zs, const double2* __restrict__ qa,
                                             const double2* __restrict__ qb, const double2* __restrict__ gwf,
                                             const int32_t* __restrict__ perm, const Geom* __restrict__ g,
                                             double sgn, double2* __restrict__ out) {
+  __shared__ double2 sz[P2P_CH], sa[P2P_CH], sq[P2P_CH];
+  __shared__ double2 red[64];
   const int nside = 1 << L;
   const int ix = blockIdx.x % nside, iy = blockIdx.x / nside;
   const uint32_t mk = morton((uint32_t)ix, (uint32_t)iy);
@@ -503,8 +516,9 @@ __global__ __launch_bounds__(64) void k_p2p(int L, const int* __restrict__ leaf_
   const int T = te - ts;
   if (T == 0) return;
   const int lane = threadIdx.x;
-  int sb[9], eb[9];
-  int nb = 0;
+  int sb[9], cum[10];
+  int nb = 0, self0 = 0;
+  cum[0] = 0;
   // MKNBOR order: column-outer, row-inner (cadap2.f:734-740)
   for (int dx = -1; dx <= 1; ++dx)
     for (int dy = -1; dy <= 1; ++dy) {
@@ -512,48 +526,49 @@ __global__ __launch_bounds__(64) void k_p2p(int L, const int* __restrict__ leaf_
       if (jx < 0 || jy < 0 || jx >= nside || jy >= nside) continue;
       const uint32_t m = morton((uint32_t)jx, (uint32_t)jy);
       sb[nb] = leaf_start[m];
-      eb[nb] = leaf_start[m + 1];
+      cum[nb + 1] = cum[nb] + (leaf_start[m + 1] - sb[nb]);
+      if (dx == 0 && dy == 0) self0 = cum[nb];  // flat position of the leaf's own points
       ++nb;
     }
+  const int S = cum[nb];
   const double rs = g->rscal;
-  __shared__ double2 red[64];
-  if (T < 64) {
-    const int G = 64 / T;
-    const int tt = lane % T, gs = lane / T;
+  const int G = T < 64 ? 64 / T : 1;
+  const int TT = T < 64 ? T : 64;
+  const int tt = lane % TT, gs = lane / TT;
+  for (int t0 = 0; t0 < T; t0 += 64) {
+    const int i = ts + t0 + tt;
+    const bool act = gs < G && t0 + tt < T;
+    const double2 zi = act ? zs[i] : make_double2(0, 0);
+    const int fself = self0 + t0 + tt;  // flat index of point i among the sources
     double2 acc1 = make_double2(0, 0), acc2 = make_double2(0, 0);
-    const int i = ts + tt;
-    if (gs < G) {
-      const double2 zi = zs[i];
-      for (int q = 0; q < nb; ++q) {
-        for (int j = sb[q] + gs; j < eb[q]; j += G) {
-          if (j == i) continue;
-          p2p_pair(zi, zs[j], qa[j], qb[j], acc1, acc2);
-        }
+    for (int c0 = 0; c0 < S; c0 += P2P_CH) {
+      const int n = min(P2P_CH, S - c0);
+      __syncthreads();
+      for (int f = lane; f < n; f += 64) {
+        const int flat = c0 + f;
+        int q = 0;
+        while (q + 1 < nb && flat >= cum[q + 1]) ++q;
+        const int j = sb[q] + (flat - cum[q]);
+        sz[f] = zs[j];
+        sa[f] = qa[j];
+        sq[f] = qb[j];
+      }
+      __syncthreads();
+      if (act) {
+        const int fs = fself - c0;
+#pragma unroll 4
+        for (int f = gs; f < n; f += G) p2p_pair(zi, sz[f], sa[f], sq[f], f == fs, acc1, acc2);
       }
     }
     red[lane] = make_double2(acc1.x + sgn * acc2.x, acc1.y + sgn * acc2.y);
     __syncthreads();
-    if (lane < T) {
-      double2 sum = gwf ? gwf[i] : make_double2(0, 0);
-      for (int q = 0; q < G; ++q) sum = cadd(sum, red[lane + q * T]);
-      out[perm[i]] = make_double2(sum.x * rs, sum.y * rs);
+    if (lane < TT && t0 + lane < T) {
+      const int it = ts + t0 + lane;
+      double2 sum = gwf ? gwf[it] : make_double2(0, 0);
+      for (int q = 0; q < G; ++q) sum = cadd(sum, red[lane + q * TT]);
+      out[perm[it]] = make_double2(sum.x * rs, sum.y * rs);
     }
-  } else {
-    for (int t0 = 0; t0 < T; t0 += 64) {
-      const int i = ts + t0 + lane;
-      if (i >= te) continue;
-      const double2 zi = zs[i];
-      double2 acc1 = make_double2(0, 0), acc2 = make_double2(0, 0);
-      for (int q = 0; q < nb; ++q)
-        for (int j = sb[q]; j < eb[q]; ++j) {
-          if (j == i) continue;
-          p2p_pair(zi, zs[j], qa[j], qb[j], acc1, acc2);
-        }
-      double2 sum = gwf ? gwf[i] : make_double2(0, 0);
-      sum.x += acc1.x + sgn * acc2.x;
-      sum.y += acc1.y + sgn * acc2.y;
-      out[perm[i]] = make_double2(sum.x * rs, sum.y * rs);
-    }
+    __syncthreads();
   }
 }
 
@@ -576,8 +591,7 @@ __global__ __launch_bounds__(256) void k_direct(int n, const double2* __restrict
     const int m = min(256, n - t0);
     if (i < n)
       for (int jj = 0; jj < m; ++jj) {
-        if (t0 + jj == i) continue;
-        p2p_pair(zi, sz[jj], sa[jj], sbq[jj], acc1, acc2);
+        p2p_pair(zi, sz[jj], sa[jj], sbq[jj], t0 + jj == i, acc1, acc2);
       }
     __syncthreads();
   }
@@ -711,8 +725,8 @@ struct eb_fmm_plan_s {
   int* doffs = nullptr;
   // timing
   bool timing = false;
-  hipEvent_t ev[7] = {};
-  double stage_ms[6] = {0};
+  hipEvent_t ev[8] = {};
+  double stage_ms[7] = {0};
 };
 
 static void plan_free(eb_fmm_plan P) {
@@ -940,12 +954,13 @@ static void launch_far(eb_fmm_plan P) {
   if (P->timing) (void)hipEventRecord(P->ev[3], st);
   // interaction lists of every level in one launch, then the downward pass
   launch_xlat<XL_M2L, K>(P, xlat_levels(P, 2, L, 0), P->mp, P->lo, P->dU, P->dNU);
+  if (P->timing) (void)hipEventRecord(P->ev[4], st);
   for (int l = 3; l <= L; ++l)
     launch_xlat<XL_L2L, K>(P, xlat_levels(P, l, l, -1), P->lo, P->lo, P->dPL, P->dQL);
-  if (P->timing) (void)hipEventRecord(P->ev[4], st);
+  if (P->timing) (void)hipEventRecord(P->ev[5], st);
   hipLaunchKernelGGL(k_l2p<K>, dim3((P->n + 255) / 256), dim3(256), 0, st, P->n, L, P->keys, P->zs,
                      P->lo + P->lvl_off[L], P->lvl_W[L], P->lvl_NBP[L], P->sgn, P->gwf);
-  if (P->timing) (void)hipEventRecord(P->ev[5], st);
+  if (P->timing) (void)hipEventRecord(P->ev[6], st);
 }
 
 extern "C" int eb_fmm_apply(eb_fmm_plan P, const double* dens, double* gradw, void* stream) {
@@ -965,7 +980,7 @@ extern "C" int eb_fmm_apply(eb_fmm_plan P, const double* dens, double* gradw, vo
                      P->sqb, P->qa, P->qb);
   if (P->mode == EB_MODE_DIRECT) {
     if (P->timing)
-      for (int i = 1; i <= 5; ++i) (void)hipEventRecord(P->ev[i], st);
+      for (int i = 1; i <= 6; ++i) (void)hipEventRecord(P->ev[i], st);
     hipLaunchKernelGGL(k_direct, dim3((n + 255) / 256), dim3(256), 0, st, n, P->zs, P->qa, P->qb, P->perm, P->geom,
                        P->sgn, (double2*)gradw);
   } else {
@@ -978,16 +993,16 @@ extern "C" int eb_fmm_apply(eb_fmm_plan P, const double* dens, double* gradw, vo
         default: launch_far<64>(P); break;
       }
     } else if (P->timing) {
-      for (int i = 1; i <= 5; ++i) (void)hipEventRecord(P->ev[i], st);
+      for (int i = 1; i <= 6; ++i) (void)hipEventRecord(P->ev[i], st);
     }
     hipLaunchKernelGGL(k_p2p, dim3(1 << (2 * P->L)), dim3(64), 0, st, P->L, P->leaf_start, P->zs, P->qa, P->qb,
                        far ? P->gwf : nullptr, P->perm, P->geom, P->sgn, (double2*)gradw);
   }
   EB_HIP_CHECK(hipGetLastError());
   if (P->timing) {
-    EB_HIP_CHECK(hipEventRecord(P->ev[6], st));
-    EB_HIP_CHECK(hipEventSynchronize(P->ev[6]));
-    for (int i = 0; i < 6; ++i) {
+    EB_HIP_CHECK(hipEventRecord(P->ev[7], st));
+    EB_HIP_CHECK(hipEventSynchronize(P->ev[7]));
+    for (int i = 0; i < 7; ++i) {
       float ms = 0;
       EB_HIP_CHECK(hipEventElapsedTime(&ms, P->ev[i], P->ev[i + 1]));
       P->stage_ms[i] = ms;
@@ -1018,9 +1033,9 @@ extern "C" int eb_fmm_enable_timing(eb_fmm_plan P, int32_t on) {
   return EB_OK;
 }
 
-extern "C" int eb_fmm_stage_ms(eb_fmm_plan P, double* ms6) {
-  if (!P || !ms6) return EB_ERR_ARG;
-  for (int i = 0; i < 6; ++i) ms6[i] = P->stage_ms[i];
+extern "C" int eb_fmm_stage_ms(eb_fmm_plan P, double* ms7) {
+  if (!P || !ms7) return EB_ERR_ARG;
+  for (int i = 0; i < 7; ++i) ms7[i] = P->stage_ms[i];
   return EB_OK;
 }
 

@@ -121,9 +121,9 @@ int eb_fmm_plan_info(eb_fmm_plan plan, double *reals3, int64_t *counts8);
 int eb_fmm_plan_ntbox(eb_fmm_plan plan, int32_t *ntbox);
 
 /* Per-stage timing of the last apply on the plan's stream: milliseconds of
- * [charges, P2M, M2M, M2L+L2L, L2P, P2P] measured with HIP events when enabled. */
+ * [charges, P2M, M2M, M2L, L2L, L2P, P2P] measured with HIP events when enabled. */
 int eb_fmm_enable_timing(eb_fmm_plan plan, int32_t on);
-int eb_fmm_stage_ms(eb_fmm_plan plan, double *ms6);
+int eb_fmm_stage_ms(eb_fmm_plan plan, double *ms7);
 
 /* ---------------------------------------------------------------------------------------
  * Tree parity: the reference's per-level binning tables, computed on the GPU.
