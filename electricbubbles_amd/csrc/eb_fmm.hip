@@ -532,15 +532,23 @@ __global__ __launch_bounds__(64) void k_p2p(int L, const int* __restrict__ leaf_
     }
   const int S = cum[nb];
   const double rs = g->rscal;
-  const int G = T < 64 ? 64 / T : 1;
-  const int TT = T < 64 ? T : 64;
-  const int tt = lane % TT, gs = lane / TT;
-  for (int t0 = 0; t0 < T; t0 += 64) {
-    const int i = ts + t0 + tt;
-    const bool act = gs < G && t0 + tt < T;
-    const double2 zi = act ? zs[i] : make_double2(0, 0);
-    const int fself = self0 + t0 + tt;  // flat index of point i among the sources
-    double2 acc1 = make_double2(0, 0), acc2 = make_double2(0, 0);
+  // each lane owns two consecutive targets (register blocking halves the LDS reads per
+  // pair); with TP = ceil(T/2) < 64 target pairs the lanes split into G = 64/TP slices
+  const int TP = (T + 1) >> 1;
+  const int TPc = TP < 64 ? TP : 64;
+  const int G = TP < 64 ? 64 / TP : 1;
+  const int tp = lane % TPc, gs = lane / TPc;
+  __shared__ double2 red2[64];
+  for (int t0 = 0; t0 < T; t0 += 128) {
+    const int i0 = ts + t0 + 2 * tp, i1 = i0 + 1;
+    const bool act = gs < G && t0 + 2 * tp < T;
+    const bool has1 = act && i1 < te;
+    const double2 z0 = act ? zs[i0] : make_double2(0, 0);
+    const double2 z1 = has1 ? zs[i1] : z0;
+    const int fself0 = self0 + t0 + 2 * tp;   // flat index of point i0 among the sources
+    const int fself1 = has1 ? fself0 + 1 : -1;
+    double2 a10 = make_double2(0, 0), a20 = make_double2(0, 0);
+    double2 a11 = make_double2(0, 0), a21 = make_double2(0, 0);
     for (int c0 = 0; c0 < S; c0 += P2P_CH) {
       const int n = min(P2P_CH, S - c0);
       __syncthreads();
@@ -555,18 +563,28 @@ __global__ __launch_bounds__(64) void k_p2p(int L, const int* __restrict__ leaf_
       }
       __syncthreads();
       if (act) {
-        const int fs = fself - c0;
-#pragma unroll 4
-        for (int f = gs; f < n; f += G) p2p_pair(zi, sz[f], sa[f], sq[f], f == fs, acc1, acc2);
+        const int fs0 = fself0 - c0, fs1 = fself1 - c0;
+#pragma unroll 2
+        for (int f = gs; f < n; f += G) {
+          const double2 zj = sz[f], qj = sa[f], bj = sq[f];
+          p2p_pair(z0, zj, qj, bj, f == fs0, a10, a20);
+          p2p_pair(z1, zj, qj, bj, f == fs1, a11, a21);
+        }
       }
     }
-    red[lane] = make_double2(acc1.x + sgn * acc2.x, acc1.y + sgn * acc2.y);
+    red[lane] = make_double2(a10.x + sgn * a20.x, a10.y + sgn * a20.y);
+    red2[lane] = make_double2(a11.x + sgn * a21.x, a11.y + sgn * a21.y);
     __syncthreads();
-    if (lane < TT && t0 + lane < T) {
-      const int it = ts + t0 + lane;
-      double2 sum = gwf ? gwf[it] : make_double2(0, 0);
-      for (int q = 0; q < G; ++q) sum = cadd(sum, red[lane + q * TT]);
-      out[perm[it]] = make_double2(sum.x * rs, sum.y * rs);
+    if (lane < TPc && t0 + 2 * lane < T) {
+      const int j0 = ts + t0 + 2 * lane, j1 = j0 + 1;
+      double2 s0 = gwf ? gwf[j0] : make_double2(0, 0);
+      for (int q = 0; q < G; ++q) s0 = cadd(s0, red[lane + q * TPc]);
+      out[perm[j0]] = make_double2(s0.x * rs, s0.y * rs);
+      if (j1 < te) {
+        double2 s1 = gwf ? gwf[j1] : make_double2(0, 0);
+        for (int q = 0; q < G; ++q) s1 = cadd(s1, red2[lane + q * TPc]);
+        out[perm[j1]] = make_double2(s1.x * rs, s1.y * rs);
+      }
     }
     __syncthreads();
   }
