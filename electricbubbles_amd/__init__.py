@@ -297,3 +297,8 @@ def lists(ilev, device="cuda", stream=None):
     nbor = nbor.cpu().numpy().reshape(nb, 9)
     nnbor = nnbor.cpu().numpy()
     return ([intl[b, : nintl[b]].copy() for b in range(nb)], [nbor[b, : nnbor[b]].copy() for b in range(nb)])
+
+
+from . import spectral  # noqa: E402
+
+EXPORTS = EXPORTS + spectral.SPECTRAL_EXPORTS

@@ -1,0 +1,850 @@
+// eb_spectral.hip -- per-body spectral geometry, RK2 stage updates and the Stokes matvec
+// epilogue, batched over all bodies (drops) of one or many simulations.
+//
+// One 256-thread workgroup owns one body; the body's sequence lives in LDS for the whole
+// transform chain (FFT, multiply, inverse FFT), so each kernel reads its inputs from HBM once
+// and writes its outputs once.  The FFT is a mixed-radix Stockham auto-sort in which every
+// thread forms whole outputs of a stage (sum over the radix with twiddles from a table of
+// N-th roots of unity), so any length works, including the large prime factors that body
+// lengths rounded to even produce (e.g. 1234 = 2 x 617).
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "../../include/eb_spectral.h"
+#include "eb_internal.h"
+
+using namespace eb;
+
+#define EB_MAXFAC 32
+constexpr int SP_THREADS = 256;
+
+struct eb_bodies_s {
+  int nbody = 0, n = 0, nmax = 0;
+  std::vector<int> nd, off;
+  int* d_nd = nullptr;     // [nbody]
+  int* d_off = nullptr;    // [nbody + 1]
+  int* d_fac = nullptr;    // [nbody][EB_MAXFAC]
+  int* d_nfac = nullptr;   // [nbody]
+  int* d_twoff = nullptr;  // [nbody] offset of the body's root table in d_tw
+  double2* d_tw = nullptr; // exp(-2 pi i j / N), j < N, one table per distinct N
+  double* d_red = nullptr; // [nbody][4] xmin, xmax, ymin, ymax
+};
+
+// ------------------------------------------------------------------------------------------
+// block-level FFT in LDS
+// ------------------------------------------------------------------------------------------
+
+// Stockham stage with radix R: out[o] = sum_r in[j + r N/R] W^(r (k + q Ns)),
+// W = exp(sign 2 pi i / (Ns R)), where o = (j / Ns) Ns R + q Ns + (j % Ns), k = j % Ns.
+// The result of all stages is in natural order.  a holds the input and receives the output;
+// b is scratch of N entries.
+__device__ void fft_block(double2* a, double2* b, int N, const int* __restrict__ fac, int nfac,
+                          const double2* __restrict__ tw, int sign) {
+  double2* in = a;
+  double2* out = b;
+  int Ns = 1;
+  for (int f = 0; f < nfac; ++f) {
+    const int R = fac[f];
+    const int NR = N / R;
+    const int step = N / (Ns * R);
+    for (int o = threadIdx.x; o < N; o += blockDim.x) {
+      const int q = (o / Ns) % R;
+      const int j = (o / (Ns * R)) * Ns + (o % Ns);
+      const int e = ((o % Ns) + q * Ns) * step;
+      double2 acc = in[j];
+      int idx = e;
+      for (int r = 1; r < R; ++r) {
+        double2 w = tw[idx];
+        if (sign > 0) w.y = -w.y;
+        cfma(acc, in[j + r * NR], w);
+        idx += e;
+        if (idx >= N) idx -= N;
+      }
+      out[o] = acc;
+    }
+    __syncthreads();
+    double2* t = in;
+    in = out;
+    out = t;
+    Ns *= R;
+  }
+  if (in != a) {
+    for (int o = threadIdx.x; o < N; o += blockDim.x) a[o] = in[o];
+    __syncthreads();
+  }
+}
+
+// wavenumber of coefficient i under FDIFFF's rules (dcfft.f:1404-1422): 0 for the mean and,
+// for even N, the Nyquist coefficient
+__device__ __forceinline__ double fdiff_k(int i, int N) {
+  if (i == 0) return 0.0;
+  if ((N & 1) == 0) {
+    const int h = N / 2;
+    if (i < h) return (double)i;
+    if (i == h) return 0.0;
+    return (double)(i - N);
+  }
+  return (i <= (N - 1) / 2) ? (double)i : (double)(i - N);
+}
+
+// FILTER (electric.f:1152-1187) on coefficients c[0..N) in LDS: exp(-10 (i/n)^25) with the
+// ratio formed in single precision, components below 1e-13 zeroed, Nyquist zeroed.
+__device__ __forceinline__ double filt_factor(int i, int n) {
+  const double ratio = (double)((float)i / (float)n);
+  double r = 1.0, b = ratio;  // ratio**25 by repeated squaring
+  for (int e = 25; e > 0; e >>= 1) {
+    if (e & 1) r *= b;
+    b *= b;
+  }
+  return exp(-10.0 * r);
+}
+
+__device__ void filter_block(double2* c, int N) {
+  const int n = N / 2;
+  const double eps = 1e-13;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    int m;
+    if (i <= n) m = i;
+    else m = N - i;  // i = N - m, m = 1..n-1
+    double2 v = c[i];
+    if (i == n) {
+      v = make_double2(0, 0);
+    } else {
+      const double f = filt_factor(m, n);
+      v.x *= f;
+      v.y *= f;
+      if (fabs(v.x) < eps) v.x = 0.0;
+      if (fabs(v.y) < eps) v.y = 0.0;
+    }
+    c[i] = v;
+  }
+  __syncthreads();
+}
+
+__device__ double block_sum(double v, double* red) {
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+    __syncthreads();
+  }
+  const double s = red[0];
+  __syncthreads();
+  return s;
+}
+
+struct BodyView {
+  const int* nd;
+  const int* off;
+  const int* fac;
+  const int* nfac;
+  const int* twoff;
+  const double2* tw;
+};
+
+static BodyView view(eb_bodies B) { return BodyView{B->d_nd, B->d_off, B->d_fac, B->d_nfac, B->d_twoff, B->d_tw}; }
+
+// ------------------------------------------------------------------------------------------
+// kernels (one block per body; dynamic LDS holds 3 sequences of nmax entries)
+// ------------------------------------------------------------------------------------------
+
+__global__ __launch_bounds__(SP_THREADS) void k_dcfft(BodyView B, double2* __restrict__ c, int sign) {
+  extern __shared__ double2 lds[];
+  const int k = blockIdx.x, N = B.nd[k], o0 = B.off[k];
+  double2* a = lds;
+  double2* b = lds + N;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) a[i] = c[o0 + i];
+  __syncthreads();
+  fft_block(a, b, N, B.fac + k * EB_MAXFAC, B.nfac[k], B.tw + B.twoff[k], sign);
+  for (int i = threadIdx.x; i < N; i += blockDim.x) c[o0 + i] = a[i];
+}
+
+__global__ __launch_bounds__(SP_THREADS) void k_fdifff(BodyView B, const double2* __restrict__ f,
+                                                        double2* __restrict__ df) {
+  extern __shared__ double2 lds[];
+  const int k = blockIdx.x, N = B.nd[k], o0 = B.off[k];
+  double2* a = lds;
+  double2* b = lds + N;
+  const int* fac = B.fac + k * EB_MAXFAC;
+  const double2* tw = B.tw + B.twoff[k];
+  for (int i = threadIdx.x; i < N; i += blockDim.x) a[i] = f[o0 + i];
+  __syncthreads();
+  fft_block(a, b, N, fac, B.nfac[k], tw, -1);
+  const double invn = 1.0 / N;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    const double w = fdiff_k(i, N) * invn;  // (i k / N) * F
+    const double2 v = a[i];
+    a[i] = make_double2(-w * v.y, w * v.x);
+  }
+  __syncthreads();
+  fft_block(a, b, N, fac, B.nfac[k], tw, +1);
+  for (int i = threadIdx.x; i < N; i += blockDim.x) df[o0 + i] = a[i];
+}
+
+__global__ __launch_bounds__(SP_THREADS) void k_filter(BodyView B, double2* __restrict__ c) {
+  extern __shared__ double2 lds[];
+  const int k = blockIdx.x, N = B.nd[k], o0 = B.off[k];
+  for (int i = threadIdx.x; i < N; i += blockDim.x) lds[i] = c[o0 + i];
+  __syncthreads();
+  filter_block(lds, N);
+  for (int i = threadIdx.x; i < N; i += blockDim.x) c[o0 + i] = lds[i];
+}
+
+// EQUIT + FOURINT (electric.f:1270-1331) of body k into T (real, LDS or global); a, b, c are
+// LDS sequences.  Returns after T is written for every point.
+__device__ void equit_block(const BodyView& B, int k, const double* __restrict__ tha, const double* __restrict__ un,
+                            double2* a, double2* b, double2* c, double* tout) {
+  const int N = B.nd[k], o0 = B.off[k], n = N / 2;
+  const int* fac = B.fac + k * EB_MAXFAC;
+  const double2* tw = B.tw + B.twoff[k];
+  for (int i = threadIdx.x; i < N; i += blockDim.x) a[i] = make_double2(-tha[o0 + i] * un[o0 + i], 0.0);
+  __syncthreads();
+  fft_block(a, b, N, fac, B.nfac[k], tw, -1);
+  const double invn = 1.0 / N;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    double2 v = a[i];
+    v = make_double2(v.x * invn, v.y * invn);
+    if (i == n) v = make_double2(0, 0);  // zf(n+1) = 0
+    a[i] = v;
+  }
+  __syncthreads();
+  // FOURINT: zfi(i) = zf(i)/(i*eye) for 1 <= i <= n-1, the mirrored ones with -, zero elsewhere
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    const double2 v = a[i];
+    double2 r = make_double2(0, 0);
+    if (i >= 1 && i <= n - 1) {
+      const double ii = (double)i;
+      r = make_double2(v.y / ii, -v.x / ii);
+    } else if (i >= N - n + 1 && i <= N - 1) {
+      const double ii = (double)(N - i);
+      r = make_double2(-v.y / ii, v.x / ii);
+    }
+    c[i] = r;
+  }
+  __syncthreads();
+  fft_block(c, b, N, fac, B.nfac[k], tw, +1);
+  const double2 zf0 = a[0];
+  const double dadt = zf0.x;
+  const double dalph = 2.0 * 3.14159265358979323846 / N;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    const double alpha = dalph * (double)i;
+    tout[i] = -(zf0.x * alpha + c[i].x) + alpha * dadt;
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(SP_THREADS) void k_equit(BodyView B, const double* __restrict__ tha,
+                                                       const double* __restrict__ un, double* __restrict__ t) {
+  extern __shared__ double2 lds[];
+  const int k = blockIdx.x, N = B.nd[k], o0 = B.off[k];
+  equit_block(B, k, tha, un, lds, lds + N, lds + 2 * N, t + o0);
+}
+
+__global__ __launch_bounds__(SP_THREADS) void k_ftail(BodyView B, const double* __restrict__ tha,
+                                                       const double* __restrict__ un, const double2* __restrict__ dz,
+                                                       const double* __restrict__ dsda, double2* __restrict__ rkup) {
+  extern __shared__ double2 lds[];
+  const int k = blockIdx.x, N = B.nd[k], o0 = B.off[k];
+  double2* a = lds;
+  double2* b = lds + N;
+  double2* c = lds + 2 * N;
+  double* T = (double*)b;  // b is free between the EQUIT transforms and the next FFT
+  equit_block(B, k, tha, un, a, b, c, T);
+  const int* fac = B.fac + k * EB_MAXFAC;
+  const double2* tw = B.tw + B.twoff[k];
+  // zf = (-Un i dz - T dz) / dsda  (electric.f:1084-1086); T is read from b before the FFT
+  // below reuses b as scratch
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    const double2 d = dz[o0 + i];
+    const double u = un[o0 + i], tt = T[i], s = dsda[o0 + i];
+    a[i] = make_double2((u * d.y - tt * d.x) / s, (-u * d.x - tt * d.y) / s);
+  }
+  __syncthreads();
+  fft_block(a, b, N, fac, B.nfac[k], tw, -1);
+  const double invn = 1.0 / N;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) a[i] = make_double2(a[i].x * invn, a[i].y * invn);
+  __syncthreads();
+  filter_block(a, N);
+  fft_block(a, b, N, fac, B.nfac[k], tw, +1);
+  for (int i = threadIdx.x; i < N; i += blockDim.x) rkup[o0 + i] = a[i];
+}
+
+// EXTRACTDAT (electric.f:636-716), one block per body.  One forward FFT of z gives both
+// derivatives (the reference applies FDIFFF twice; the mean and Nyquist rules are the same).
+__global__ __launch_bounds__(SP_THREADS) void k_extractdat(BodyView B, const double2* __restrict__ z,
+                                                            double* __restrict__ x, double* __restrict__ y,
+                                                            double2* __restrict__ zk, double2* __restrict__ dz,
+                                                            double* __restrict__ tha, double* __restrict__ dsda,
+                                                            double* __restrict__ rkappa, double2* __restrict__ ckern,
+                                                            double* __restrict__ area, double* __restrict__ red4) {
+  extern __shared__ double2 lds[];
+  __shared__ double red[SP_THREADS];
+  const int k = blockIdx.x, N = B.nd[k], o0 = B.off[k];
+  double2* a = lds;
+  double2* b = lds + N;
+  double2* c = lds + 2 * N;
+  const int* fac = B.fac + k * EB_MAXFAC;
+  const double2* tw = B.tw + B.twoff[k];
+  for (int i = threadIdx.x; i < N; i += blockDim.x) a[i] = z[o0 + i];
+  __syncthreads();
+  fft_block(a, b, N, fac, B.nfac[k], tw, -1);
+  const double invn = 1.0 / N;
+  if (threadIdx.x == 0 && zk) zk[k] = make_double2(a[0].x * invn, a[0].y * invn);
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    const double kk = fdiff_k(i, N);
+    const double2 v = make_double2(a[i].x * invn, a[i].y * invn);
+    c[i] = make_double2(-kk * kk * v.x, -kk * kk * v.y);  // (ik)^2 F / N
+    a[i] = make_double2(-kk * v.y, kk * v.x);             // (ik) F / N
+  }
+  __syncthreads();
+  fft_block(a, b, N, fac, B.nfac[k], tw, +1);  // z_alpha
+  fft_block(c, b, N, fac, B.nfac[k], tw, +1);  // z_alpha_alpha
+  double sarea = 0.0, xmn = 1e300, xmx = -1e300, ymn = 1e300, ymx = -1e300;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    const int g = o0 + i;
+    const double2 zz = z[g];
+    const double xdot = a[i].x, ydot = a[i].y, xddot = c[i].x, yddot = c[i].y;
+    const double dsdt = sqrt(xdot * xdot + ydot * ydot);
+    const double kap = -(xdot * yddot - ydot * xddot) / (dsdt * dsdt * dsdt);
+    if (x) x[g] = zz.x;
+    if (y) y[g] = zz.y;
+    if (rkappa) rkappa[g] = kap;
+    if (ckern) {
+      // -(xdd + i ydd) / (xd + i yd)
+      const double den = xdot * xdot + ydot * ydot;
+      ckern[g] = make_double2(-(xddot * xdot + yddot * ydot) / den, -(yddot * xdot - xddot * ydot) / den);
+    }
+    if (tha) tha[g] = -kap * dsdt;
+    if (dz) dz[g] = make_double2(-xdot, -ydot);
+    if (dsda) dsda[g] = hypot(xdot, ydot);
+    sarea += zz.y * (-xdot);
+    xmn = fmin(xmn, zz.x);
+    xmx = fmax(xmx, zz.x);
+    ymn = fmin(ymn, zz.y);
+    ymx = fmax(ymx, zz.y);
+  }
+  const double s = block_sum(sarea, red);
+  if (threadIdx.x == 0 && area) area[k] = 2.0 * 3.14159265358979323846 * (s * invn);
+  // min / max through the same LDS buffer
+  red[threadIdx.x] = xmn;
+  __syncthreads();
+  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + off]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) red4[4 * k + 0] = red[0];
+  __syncthreads();
+  red[threadIdx.x] = xmx;
+  __syncthreads();
+  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + off]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) red4[4 * k + 1] = red[0];
+  __syncthreads();
+  red[threadIdx.x] = ymn;
+  __syncthreads();
+  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + off]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) red4[4 * k + 2] = red[0];
+  __syncthreads();
+  red[threadIdx.x] = ymx;
+  __syncthreads();
+  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + off]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) red4[4 * k + 3] = red[0];
+}
+
+// xleft = min x, slength = max(xmax - xmin, ymax) over all bodies (electric.f:710-711)
+__global__ void k_extract_final(const double* __restrict__ red4, int nbody, double* __restrict__ out2) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double xmn = 1e10, xmx = -1e10, ymx = -1e10;
+  for (int k = 0; k < nbody; ++k) {
+    xmn = fmin(xmn, red4[4 * k]);
+    xmx = fmax(xmx, red4[4 * k + 1]);
+    ymx = fmax(ymx, red4[4 * k + 3]);
+  }
+  out2[0] = xmn;
+  out2[1] = fmax(xmx - xmn, ymx);
+}
+
+__global__ void k_rk2_1(int neq, double dt, const double* __restrict__ ybar, const double* __restrict__ dy,
+                        double* __restrict__ phi0, double* __restrict__ y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= neq) return;
+  // separate roundings, as the reference's two statements (no fused multiply-add)
+  const double p = __dmul_rn(dt, dy[i]);
+  phi0[i] = p;
+  y[i] = __dadd_rn(ybar[i], p);
+}
+
+__global__ void k_rk2_2(int neq, double dt, const double* __restrict__ phi0, const double* __restrict__ dy,
+                        double* __restrict__ ybar) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= neq) return;
+  const double p1 = __dmul_rn(dt, dy[i]);
+  ybar[i] = __dadd_rn(__dadd_rn(ybar[i], __dmul_rn(0.5, phi0[i])), __dmul_rn(0.5, p1));
+}
+
+// ------------------------------------------------------------------------------------------
+// FASMVP epilogue (stokesbubb.f:527-549, 585-611)
+// ------------------------------------------------------------------------------------------
+
+// per body: za = h sum dens, pk = (2h/pi) sum Re(dens dz / (z - zk)^2)
+__global__ __launch_bounds__(SP_THREADS) void k_fasmvp_sums(BodyView B, const double* __restrict__ x,
+                                                             const double* __restrict__ y, const double2* __restrict__ dz,
+                                                             const double2* __restrict__ zk, const double* __restrict__ h,
+                                                             const double2* __restrict__ dens, double2* __restrict__ za,
+                                                             double* __restrict__ pk) {
+  __shared__ double red[SP_THREADS];
+  const int k = blockIdx.x, N = B.nd[k], o0 = B.off[k];
+  const double2 c = zk[k];
+  double sr = 0, si = 0, sp = 0;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    const int g = o0 + i;
+    const double2 d = dens[g];
+    sr += d.x;
+    si += d.y;
+    const double2 zd = make_double2(x[g] - c.x, y[g] - c.y);
+    const double2 zd2 = cmul(zd, zd);
+    const double2 num = cmul(d, dz[g]);
+    const double den = zd2.x * zd2.x + zd2.y * zd2.y;
+    sp += (num.x * zd2.x + num.y * zd2.y) / den;  // Re(num / zd2)
+  }
+  const double tr = block_sum(sr, red), ti = block_sum(si, red), tp = block_sum(sp, red);
+  if (threadIdx.x == 0) {
+    const double hk = h[k];
+    za[k] = make_double2(hk * tr, hk * ti);
+    pk[k] = 2.0 * hk * tp / 3.14159265358979323846;
+  }
+}
+
+// w = dens + za + GW + self - (i/2)(pinf - pk_k) z, with pinf = -pk_1 and pk_k shifted by pinf
+__global__ void k_fasmvp_point(BodyView B, int nbody, int n, const double* __restrict__ x, const double* __restrict__ y,
+                               const double2* __restrict__ dz, const double* __restrict__ dsdth,
+                               const double* __restrict__ rkappa, const double* __restrict__ h,
+                               const double2* __restrict__ dens, const double2* __restrict__ za,
+                               const double* __restrict__ pk, double2* __restrict__ gw, double2* __restrict__ w,
+                               const int* __restrict__ body_of) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  const int k = body_of[g];
+  const double pi = 3.14159265358979323846;
+  const double pinf = -pk[0];
+  const double pkk = pk[k] + pinf;
+  const double2 zu = dens[g];
+  const double2 d = dz[g];
+  const double s = dsdth[g];
+  const double2 zn = make_double2(d.y / s, -d.x / s);  // -i dz / dsdth
+  const double qwi = h[k];
+  const double c0 = qwi * s * 0.5 * rkappa[g] / pi;
+  const double2 zn2 = cmul(zn, zn);
+  // zself = zu c0 - conj(zu) c0 zn^2
+  const double2 t = cmul(make_double2(zu.x, -zu.y), zn2);
+  const double2 zself = make_double2(c0 * (zu.x - t.x), c0 * (zu.y - t.y));
+  // -0.5 i (pinf - pkk) z
+  const double a = -0.5 * (pinf - pkk);
+  const double2 pr = make_double2(-a * y[g], a * x[g]);
+  const double2 st = make_double2(gw[g].x + zself.x + pr.x, gw[g].y + zself.y + pr.y);
+  gw[g] = st;
+  const double2 zak = za[k];
+  w[g] = make_double2(zu.x + zak.x + st.x, zu.y + zak.y + st.y);
+}
+
+__global__ void k_body_of(const int* __restrict__ off, int nbody, int* __restrict__ body_of) {
+  const int k = blockIdx.x;
+  if (k >= nbody) return;
+  for (int i = off[k] + threadIdx.x; i < off[k + 1]; i += blockDim.x) body_of[i] = k;
+}
+
+__global__ void k_wdz(int n, const double2* __restrict__ dz, const double* __restrict__ h,
+                      const int* __restrict__ body_of, double2* __restrict__ wdz) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  const double hk = h[body_of[g]];
+  wdz[g] = make_double2(dz[g].x * hk, dz[g].y * hk);
+}
+
+__global__ void k_shift_pk(int nbody, double* __restrict__ pk_out, const double* __restrict__ pk) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nbody) return;
+  pk_out[k] = pk[k] - pk[0];
+}
+
+// ------------------------------------------------------------------------------------------
+// host side
+// ------------------------------------------------------------------------------------------
+template <typename T>
+static int dalloc(T** p, size_t count) {
+  *p = nullptr;
+  if (hipMalloc((void**)p, sizeof(T) * (count ? count : 1)) != hipSuccess) {
+    eb_set_error("hipMalloc(%zu bytes) failed", sizeof(T) * count);
+    return EB_ERR_NOMEM;
+  }
+  return EB_OK;
+}
+
+static std::vector<int> factorize(int n) {
+  std::vector<int> f;
+  while (n % 4 == 0) {
+    f.push_back(4);
+    n /= 4;
+  }
+  while (n % 2 == 0) {
+    f.push_back(2);
+    n /= 2;
+  }
+  for (int p = 3; p * p <= n; p += 2)
+    while (n % p == 0) {
+      f.push_back(p);
+      n /= p;
+    }
+  if (n > 1) f.push_back(n);
+  return f;
+}
+
+static void free_bodies(eb_bodies B) {
+  if (!B) return;
+  void* ptrs[] = {B->d_nd, B->d_off, B->d_fac, B->d_nfac, B->d_twoff, B->d_tw, B->d_red};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+  delete B;
+}
+
+extern "C" int eb_bodies_create(eb_bodies* out, int32_t nbody, const int32_t* nd, void* stream) {
+  if (!out || nbody < 1 || !nd) {
+    eb_set_error("eb_bodies_create: bad arguments");
+    return EB_ERR_ARG;
+  }
+  *out = nullptr;
+  eb_bodies B = new eb_bodies_s();
+  B->nbody = nbody;
+  B->nd.assign(nd, nd + nbody);
+  B->off.assign(nbody + 1, 0);
+  for (int k = 0; k < nbody; ++k) {
+    if (nd[k] < 2 || nd[k] > EB_SPECTRAL_NMAX) {
+      eb_set_error("eb_bodies_create: body %d has %d points (2..%d supported)", k, nd[k], EB_SPECTRAL_NMAX);
+      free_bodies(B);
+      return EB_ERR_ARG;
+    }
+    B->off[k + 1] = B->off[k] + nd[k];
+    B->nmax = std::max(B->nmax, (int)nd[k]);
+  }
+  B->n = B->off[nbody];
+  {
+    int dev = 0, maxlds = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess &&
+        (size_t)maxlds < sizeof(double2) * 3 * (size_t)B->nmax) {
+      eb_set_error("eb_bodies_create: a body of %d points needs %zu B of LDS, the device allows %d", B->nmax,
+                   sizeof(double2) * 3 * (size_t)B->nmax, maxlds);
+      free_bodies(B);
+      return EB_ERR_ARG;
+    }
+  }
+  std::vector<int> fac((size_t)nbody * EB_MAXFAC, 1), nfac(nbody), twoff(nbody);
+  std::map<int, int> tab;  // N -> offset
+  std::vector<double2> tw;
+  for (int k = 0; k < nbody; ++k) {
+    const int N = nd[k];
+    std::vector<int> f = factorize(N);
+    nfac[k] = (int)f.size();
+    for (size_t i = 0; i < f.size(); ++i) fac[(size_t)k * EB_MAXFAC + i] = f[i];
+    auto it = tab.find(N);
+    if (it == tab.end()) {
+      const int o = (int)tw.size();
+      tab[N] = o;
+      for (int j = 0; j < N; ++j) {
+        const long double a = -6.28318530717958647692528676655900577L * (long double)j / (long double)N;
+        tw.push_back(make_double2((double)cosl(a), (double)sinl(a)));
+      }
+      twoff[k] = o;
+    } else {
+      twoff[k] = it->second;
+    }
+  }
+  hipStream_t st = (hipStream_t)stream;
+  int rc;
+  if ((rc = dalloc(&B->d_nd, nbody)) || (rc = dalloc(&B->d_off, nbody + 1)) ||
+      (rc = dalloc(&B->d_fac, fac.size())) || (rc = dalloc(&B->d_nfac, nbody)) || (rc = dalloc(&B->d_twoff, nbody)) ||
+      (rc = dalloc(&B->d_tw, tw.size())) || (rc = dalloc(&B->d_red, 4 * (size_t)nbody))) {
+    free_bodies(B);
+    return rc;
+  }
+  bool ok = hipMemcpyAsync(B->d_nd, nd, sizeof(int) * nbody, hipMemcpyHostToDevice, st) == hipSuccess &&
+            hipMemcpyAsync(B->d_off, B->off.data(), sizeof(int) * (nbody + 1), hipMemcpyHostToDevice, st) == hipSuccess &&
+            hipMemcpyAsync(B->d_fac, fac.data(), sizeof(int) * fac.size(), hipMemcpyHostToDevice, st) == hipSuccess &&
+            hipMemcpyAsync(B->d_nfac, nfac.data(), sizeof(int) * nbody, hipMemcpyHostToDevice, st) == hipSuccess &&
+            hipMemcpyAsync(B->d_twoff, twoff.data(), sizeof(int) * nbody, hipMemcpyHostToDevice, st) == hipSuccess &&
+            hipMemcpyAsync(B->d_tw, tw.data(), sizeof(double2) * tw.size(), hipMemcpyHostToDevice, st) == hipSuccess &&
+            hipStreamSynchronize(st) == hipSuccess;
+  if (!ok) {
+    eb_set_error("eb_bodies_create: upload failed");
+    free_bodies(B);
+    return EB_ERR_HIP;
+  }
+  *out = B;
+  return EB_OK;
+}
+
+extern "C" int eb_bodies_destroy(eb_bodies B) {
+  free_bodies(B);
+  return EB_OK;
+}
+
+extern "C" int eb_bodies_npoints(eb_bodies B) { return B ? B->n : 0; }
+
+static size_t lds_bytes(eb_bodies B, int nseq) { return sizeof(double2) * (size_t)B->nmax * nseq; }
+
+extern "C" int eb_dcfft(eb_bodies B, double* c, int32_t sign, void* stream) {
+  if (!B || !c || (sign != 1 && sign != -1)) return (eb_set_error("eb_dcfft: bad arguments"), EB_ERR_ARG);
+  hipLaunchKernelGGL(k_dcfft, dim3(B->nbody), dim3(SP_THREADS), lds_bytes(B, 2), (hipStream_t)stream, view(B),
+                     (double2*)c, sign);
+  EB_HIP_CHECK(hipGetLastError());
+  return EB_OK;
+}
+
+extern "C" int eb_fdifff(eb_bodies B, const double* f, double* df, void* stream) {
+  if (!B || !f || !df) return (eb_set_error("eb_fdifff: bad arguments"), EB_ERR_ARG);
+  hipLaunchKernelGGL(k_fdifff, dim3(B->nbody), dim3(SP_THREADS), lds_bytes(B, 2), (hipStream_t)stream, view(B),
+                     (const double2*)f, (double2*)df);
+  EB_HIP_CHECK(hipGetLastError());
+  return EB_OK;
+}
+
+extern "C" int eb_filter(eb_bodies B, double* cn, void* stream) {
+  if (!B || !cn) return (eb_set_error("eb_filter: bad arguments"), EB_ERR_ARG);
+  hipLaunchKernelGGL(k_filter, dim3(B->nbody), dim3(SP_THREADS), lds_bytes(B, 1), (hipStream_t)stream, view(B),
+                     (double2*)cn);
+  EB_HIP_CHECK(hipGetLastError());
+  return EB_OK;
+}
+
+extern "C" int eb_equit(eb_bodies B, const double* tha, const double* un, double* t, void* stream) {
+  if (!B || !tha || !un || !t) return (eb_set_error("eb_equit: bad arguments"), EB_ERR_ARG);
+  hipLaunchKernelGGL(k_equit, dim3(B->nbody), dim3(SP_THREADS), lds_bytes(B, 3), (hipStream_t)stream, view(B), tha,
+                     un, t);
+  EB_HIP_CHECK(hipGetLastError());
+  return EB_OK;
+}
+
+extern "C" int eb_ftail(eb_bodies B, const double* tha, const double* un, const double* dz, const double* dsda,
+                        double* rkup, void* stream) {
+  if (!B || !tha || !un || !dz || !dsda || !rkup) return (eb_set_error("eb_ftail: bad arguments"), EB_ERR_ARG);
+  const size_t lds = lds_bytes(B, 3);
+  hipLaunchKernelGGL(k_ftail, dim3(B->nbody), dim3(SP_THREADS), lds, (hipStream_t)stream, view(B), tha, un,
+                     (const double2*)dz, dsda, (double2*)rkup);
+  EB_HIP_CHECK(hipGetLastError());
+  return EB_OK;
+}
+
+extern "C" int eb_extractdat(eb_bodies B, const double* rku, double* x, double* y, double* zk, double* dz,
+                             double* tha, double* dsda, double* rkappa, double* ckern, double* area,
+                             double* xleft_slength, void* stream) {
+  if (!B || !rku) return (eb_set_error("eb_extractdat: bad arguments"), EB_ERR_ARG);
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_extractdat, dim3(B->nbody), dim3(SP_THREADS), lds_bytes(B, 3), st, view(B),
+                     (const double2*)rku, x, y, (double2*)zk, (double2*)dz, tha, dsda, rkappa, (double2*)ckern,
+                     area, B->d_red);
+  if (xleft_slength) hipLaunchKernelGGL(k_extract_final, dim3(1), dim3(64), 0, st, B->d_red, B->nbody, xleft_slength);
+  EB_HIP_CHECK(hipGetLastError());
+  return EB_OK;
+}
+
+extern "C" int eb_rk2_stage1(int32_t neq, double dt, const double* ybar, const double* dy, double* phi0, double* y,
+                             void* stream) {
+  if (neq < 1) return EB_OK;
+  hipLaunchKernelGGL(k_rk2_1, dim3((neq + 255) / 256), dim3(256), 0, (hipStream_t)stream, neq, dt, ybar, dy, phi0, y);
+  EB_HIP_CHECK(hipGetLastError());
+  return EB_OK;
+}
+
+extern "C" int eb_rk2_stage2(int32_t neq, double dt, const double* phi0, const double* dy, double* ybar,
+                             void* stream) {
+  if (neq < 1) return EB_OK;
+  hipLaunchKernelGGL(k_rk2_2, dim3((neq + 255) / 256), dim3(256), 0, (hipStream_t)stream, neq, dt, phi0, dy, ybar);
+  EB_HIP_CHECK(hipGetLastError());
+  return EB_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// matvec
+// ------------------------------------------------------------------------------------------
+struct eb_matvec_s {
+  eb_bodies B = nullptr;
+  eb_fmm_plan fmm = nullptr;
+  int n = 0;
+  double *x = nullptr, *y = nullptr, *dsdth = nullptr, *rkappa = nullptr, *h = nullptr, *pk = nullptr;
+  double2 *dz = nullptr, *zk = nullptr, *wdz = nullptr, *gw = nullptr, *za = nullptr;
+  int* body_of = nullptr;
+};
+
+static void free_mv(eb_matvec M) {
+  if (!M) return;
+  if (M->fmm) eb_fmm_plan_destroy(M->fmm);
+  void* ptrs[] = {M->x, M->y, M->dsdth, M->rkappa, M->h, M->pk, M->dz, M->zk, M->wdz, M->gw, M->za, M->body_of};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+  delete M;
+}
+
+extern "C" int eb_matvec_create(eb_matvec* out, eb_bodies B, const double* x, const double* y, const double* dz,
+                                const double* dsdth, const double* rkappa, const double* zk, int32_t levmax,
+                                int32_t nterms, int32_t mode, void* stream) {
+  if (!out || !B || !x || !y || !dz || !dsdth || !rkappa || !zk)
+    return (eb_set_error("eb_matvec_create: bad arguments"), EB_ERR_ARG);
+  *out = nullptr;
+  hipStream_t st = (hipStream_t)stream;
+  eb_matvec M = new eb_matvec_s();
+  M->B = B;
+  const int n = B->n, nb = B->nbody;
+  M->n = n;
+  int rc;
+  if ((rc = dalloc(&M->x, n)) || (rc = dalloc(&M->y, n)) || (rc = dalloc(&M->dsdth, n)) ||
+      (rc = dalloc(&M->rkappa, n)) || (rc = dalloc(&M->h, nb)) || (rc = dalloc(&M->pk, nb)) ||
+      (rc = dalloc(&M->dz, n)) || (rc = dalloc(&M->zk, nb)) || (rc = dalloc(&M->wdz, n)) || (rc = dalloc(&M->gw, n)) ||
+      (rc = dalloc(&M->za, nb)) || (rc = dalloc(&M->body_of, n))) {
+    free_mv(M);
+    return rc;
+  }
+  std::vector<double> h(nb);
+  for (int k = 0; k < nb; ++k) h[k] = 2.0 * 3.14159265358979323846 / B->nd[k];  // stokesbubb.f:446-448
+  bool ok = hipMemcpyAsync(M->x, x, sizeof(double) * n, hipMemcpyDeviceToDevice, st) == hipSuccess &&
+            hipMemcpyAsync(M->y, y, sizeof(double) * n, hipMemcpyDeviceToDevice, st) == hipSuccess &&
+            hipMemcpyAsync(M->dsdth, dsdth, sizeof(double) * n, hipMemcpyDeviceToDevice, st) == hipSuccess &&
+            hipMemcpyAsync(M->rkappa, rkappa, sizeof(double) * n, hipMemcpyDeviceToDevice, st) == hipSuccess &&
+            hipMemcpyAsync(M->dz, dz, sizeof(double2) * n, hipMemcpyDeviceToDevice, st) == hipSuccess &&
+            hipMemcpyAsync(M->zk, zk, sizeof(double2) * nb, hipMemcpyDeviceToDevice, st) == hipSuccess &&
+            hipMemcpyAsync(M->h, h.data(), sizeof(double) * nb, hipMemcpyHostToDevice, st) == hipSuccess;
+  if (!ok) {
+    free_mv(M);
+    return (eb_set_error("eb_matvec_create: copy failed"), EB_ERR_HIP);
+  }
+  hipLaunchKernelGGL(k_body_of, dim3(nb), dim3(256), 0, st, B->d_off, nb, M->body_of);
+  hipLaunchKernelGGL(k_wdz, dim3((n + 255) / 256), dim3(256), 0, st, n, (const double2*)M->dz, M->h, M->body_of,
+                     M->wdz);
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess) {
+    free_mv(M);
+    return (eb_set_error("eb_matvec_create: kernel failed"), EB_ERR_HIP);
+  }
+  rc = eb_fmm_plan_create(&M->fmm, n, M->x, M->y, (const double*)M->wdz, levmax, nterms, EB_VARIANT_DR, mode, stream);
+  if (rc != EB_OK) {
+    M->fmm = nullptr;
+    free_mv(M);
+    return rc;
+  }
+  *out = M;
+  return EB_OK;
+}
+
+extern "C" int eb_matvec_apply(eb_matvec M, const double* xx, double* yy, double* stress, double* za, double* pk,
+                               void* stream) {
+  if (!M || !xx || !yy) return (eb_set_error("eb_matvec_apply: bad arguments"), EB_ERR_ARG);
+  hipStream_t st = (hipStream_t)stream;
+  eb_bodies B = M->B;
+  const double2* dens = (const double2*)xx;  // dens(i) = dcmplx(u(2i-1), u(2i)) (stokesbubb.f:527-529)
+  int rc = eb_fmm_apply(M->fmm, xx, (double*)M->gw, stream);
+  if (rc != EB_OK) return rc;
+  hipLaunchKernelGGL(k_fasmvp_sums, dim3(B->nbody), dim3(SP_THREADS), 0, st, view(B), M->x, M->y,
+                     (const double2*)M->dz, (const double2*)M->zk, M->h, dens, M->za, M->pk);
+  hipLaunchKernelGGL(k_fasmvp_point, dim3((M->n + 255) / 256), dim3(256), 0, st, view(B), B->nbody, M->n, M->x, M->y,
+                     (const double2*)M->dz, M->dsdth, M->rkappa, M->h, dens, (const double2*)M->za, M->pk, M->gw,
+                     (double2*)yy, M->body_of);
+  if (stress) EB_HIP_CHECK(hipMemcpyAsync(stress, M->gw, sizeof(double2) * M->n, hipMemcpyDeviceToDevice, st));
+  if (za) EB_HIP_CHECK(hipMemcpyAsync(za, M->za, sizeof(double2) * B->nbody, hipMemcpyDeviceToDevice, st));
+  if (pk) hipLaunchKernelGGL(k_shift_pk, dim3((B->nbody + 255) / 256), dim3(256), 0, st, B->nbody, pk, M->pk);
+  EB_HIP_CHECK(hipGetLastError());
+  return EB_OK;
+}
+
+extern "C" int eb_matvec_destroy(eb_matvec M) {
+  free_mv(M);
+  return EB_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// FASMVP drop-in on host arrays.  The geometry is the same for every matvec of one GMRES
+// solve (MATVEC passes the /inteqn/ arrays unchanged, stokesbubb.f:449-455), so the bodies,
+// the FMM plan and the device copies are kept while (ND, X, Y, DZ, DSDTH, RKAPPA, ZK, H)
+// do not change.
+// ------------------------------------------------------------------------------------------
+namespace {
+struct FasCtx {
+  std::mutex mu;
+  hipStream_t st = nullptr;
+  eb_bodies B = nullptr;
+  eb_matvec M = nullptr;
+  std::vector<int> nd;
+  std::vector<double> key;
+  double *dgeo = nullptr, *du = nullptr, *dw = nullptr, *dst = nullptr, *dza = nullptr, *dpk = nullptr;
+};
+FasCtx& fctx() {
+  static FasCtx* c = new FasCtx();
+  return *c;
+}
+}  // namespace
+
+extern "C" int32_t eb_fasmvp(const int32_t* nd, const int32_t* nbk, const int32_t* k, const double* x, const double* y,
+                             const double* dz, const double* dsdth, const double* rkappa, const double* zk,
+                             const double* u, double* w, const double* h, double* za, double* pk, double* stress,
+                             double* dens) {
+  const int K = *k, N = *nbk;
+  (void)h;  // h(k) = 2 pi / nd(k), as MATVEC sets it (stokesbubb.f:446-448)
+  FasCtx& c = fctx();
+  std::lock_guard<std::mutex> lk(c.mu);
+  if (!c.st && hipStreamCreate(&c.st) != hipSuccess) return (eb_set_error("eb_fasmvp: no HIP stream"), EB_ERR_HIP);
+  std::vector<double> key;
+  key.reserve(7 * (size_t)N + 2 * K);
+  key.insert(key.end(), x, x + N);
+  key.insert(key.end(), y, y + N);
+  key.insert(key.end(), dz, dz + 2 * N);
+  key.insert(key.end(), dsdth, dsdth + N);
+  key.insert(key.end(), rkappa, rkappa + N);
+  key.insert(key.end(), zk, zk + 2 * K);
+  std::vector<int> ndv(nd, nd + K);
+  int rc;
+  if (!c.M || ndv != c.nd || key != c.key) {
+    if (c.M) eb_matvec_destroy(c.M);
+    if (c.B) eb_bodies_destroy(c.B);
+    c.M = nullptr;
+    c.B = nullptr;
+    void* ptrs[] = {c.dgeo, c.du, c.dw, c.dst, c.dza, c.dpk};
+    for (void* q : ptrs)
+      if (q) (void)hipFree(q);
+    if ((rc = eb_bodies_create(&c.B, K, nd, c.st)) != EB_OK) return rc;
+    if ((rc = dalloc(&c.dgeo, key.size())) || (rc = dalloc(&c.du, 2 * (size_t)N)) || (rc = dalloc(&c.dw, 2 * (size_t)N)) ||
+        (rc = dalloc(&c.dst, 2 * (size_t)N)) || (rc = dalloc(&c.dza, 2 * (size_t)K)) || (rc = dalloc(&c.dpk, K)))
+      return rc;
+    if (hipMemcpyAsync(c.dgeo, key.data(), sizeof(double) * key.size(), hipMemcpyHostToDevice, c.st) != hipSuccess)
+      return (eb_set_error("eb_fasmvp: copy failed"), EB_ERR_HIP);
+    double* g = c.dgeo;
+    const int mode = N <= 2048 ? EB_MODE_DIRECT : EB_MODE_FMM;
+    rc = eb_matvec_create(&c.M, c.B, g, g + N, g + 2 * N, g + 4 * N, g + 5 * N, g + 6 * N, 7, 43, mode, c.st);
+    if (rc != EB_OK) {
+      c.M = nullptr;
+      return rc;
+    }
+    c.nd = ndv;
+    c.key.swap(key);
+  }
+  if (hipMemcpyAsync(c.du, u, sizeof(double) * 2 * N, hipMemcpyHostToDevice, c.st) != hipSuccess)
+    return (eb_set_error("eb_fasmvp: copy failed"), EB_ERR_HIP);
+  if ((rc = eb_matvec_apply(c.M, c.du, c.dw, c.dst, c.dza, c.dpk, c.st)) != EB_OK) return rc;
+  bool ok = hipMemcpyAsync(w, c.dw, sizeof(double) * 2 * N, hipMemcpyDeviceToHost, c.st) == hipSuccess &&
+            (!stress || hipMemcpyAsync(stress, c.dst, sizeof(double) * 2 * N, hipMemcpyDeviceToHost, c.st) == hipSuccess) &&
+            (!za || hipMemcpyAsync(za, c.dza, sizeof(double) * 2 * K, hipMemcpyDeviceToHost, c.st) == hipSuccess) &&
+            (!pk || hipMemcpyAsync(pk, c.dpk, sizeof(double) * K, hipMemcpyDeviceToHost, c.st) == hipSuccess) &&
+            hipStreamSynchronize(c.st) == hipSuccess;
+  if (!ok) return (eb_set_error("eb_fasmvp: copy back failed"), EB_ERR_HIP);
+  if (dens) memcpy(dens, u, sizeof(double) * 2 * N);
+  return EB_OK;
+}

@@ -1,0 +1,91 @@
+/*
+ * eb_spectral.h -- C-ABI of the per-body spectral geometry, the Runge-Kutta stage update and
+ * the Stokes matvec (FASMVP epilogue + GPU FMM), batched over all bodies (drops) of one or
+ * many simulations on one MI355X.
+ *
+ * Bodies are stored back to back: body k owns points off[k] .. off[k]+nd[k]-1 of every
+ * per-point array.  Complex arrays are interleaved (re, im) doubles, as COMPLEX*16 in the
+ * reference.  All data pointers are device pointers; `stream` is a hipStream_t (NULL = default).
+ * Transforms keep the reference's conventions: DCFFTF is the unnormalised forward DFT
+ * C_j = sum_k c_k exp(-2 pi i jk/N), DCFFTB the unnormalised backward one (dcfft.f:1-78,
+ * 715-791).  Any body length 2 <= N <= EB_SPECTRAL_NMAX is accepted (mixed radix with a
+ * direct stage for large prime factors).
+ */
+#ifndef EB_SPECTRAL_H
+#define EB_SPECTRAL_H
+
+#include <stdint.h>
+
+#include "eb_cadap.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EB_SPECTRAL_NMAX 3200  /* three LDS sequences of N complex doubles per body: 150 KB */
+
+typedef struct eb_bodies_s *eb_bodies;
+
+/* Layout of nbody bodies with nd[k] points each (host array); builds the per-length twiddle
+ * tables and factorisations on the device. */
+int eb_bodies_create(eb_bodies *bodies, int32_t nbody, const int32_t *nd, void *stream);
+int eb_bodies_destroy(eb_bodies bodies);
+int eb_bodies_npoints(eb_bodies bodies);
+
+/* DCFFTF (sign = -1, dcfft.f:1-78) / DCFFTB (sign = +1, dcfft.f:715-791) on every body, in place. */
+int eb_dcfft(eb_bodies bodies, double *c, int32_t sign, void *stream);
+/* FDIFFF (dcfft.f:1371-1427): spectral derivative of every body. f and df may alias. */
+int eb_fdifff(eb_bodies bodies, const double *f, double *df, void *stream);
+/* FILTER (electric.f:1152-1187) on the Fourier coefficients of every body, in place. */
+int eb_filter(eb_bodies bodies, double *cn, void *stream);
+/* EQUIT + FOURINT (electric.f:1270-1331): tangential velocity T from tha and Un (real arrays). */
+int eb_equit(eb_bodies bodies, const double *tha, const double *un, double *t, void *stream);
+
+/* EXTRACTDAT (electric.f:636-716) for all bodies from the RK state rku (2*nbk interleaved):
+ * x, y, dz, tha, dsda, rkappa, ckern per point; zk (complex) and area per body;
+ * xleft_slength[2] = {xleft, slength}.  Any output pointer may be NULL. */
+int eb_extractdat(eb_bodies bodies, const double *rku, double *x, double *y, double *zk, double *dz,
+                  double *tha, double *dsda, double *rkappa, double *ckern, double *area,
+                  double *xleft_slength, void *stream);
+
+/* Tail of F (electric.f:1076-1103): EQUIT, the frame velocity (-Un i dz - T dz)/dsda, FFT,
+ * FILTER, inverse FFT -> rkup (2*nbk interleaved). */
+int eb_ftail(eb_bodies bodies, const double *tha, const double *un, const double *dz,
+             const double *dsda, double *rkup, void *stream);
+
+/* RK2 (Heun) stage updates (electric.f:773-783) over neq doubles:
+ * stage 1: phi0 = dt*dy, y = ybar + phi0;  stage 2: ybar += 0.5*phi0 + 0.5*dt*dy. */
+int eb_rk2_stage1(int32_t neq, double dt, const double *ybar, const double *dy, double *phi0, double *y,
+                  void *stream);
+int eb_rk2_stage2(int32_t neq, double dt, const double *phi0, const double *dy, double *ybar,
+                  void *stream);
+
+/* ---------------------------------------------------------------------------------------
+ * The Stokes matvec of the GMRES solve: MATVEC (stokesbubb.f:419-465) + FASMVP
+ * (stokesbubb.f:496-614) with the layer potential from the GPU FMM (eb_fmm_plan).
+ * Geometry (x, y, dz = DZ, dsdth = |z_alpha|, rkappa, zk per body) is bound at creation;
+ * each apply maps the interleaved density xx (2*nbk) to yy (2*nbk).  h_k = 2 pi / nd_k.
+ * --------------------------------------------------------------------------------------- */
+typedef struct eb_matvec_s *eb_matvec;
+
+int eb_matvec_create(eb_matvec *mv, eb_bodies bodies, const double *x, const double *y, const double *dz,
+                     const double *dsdth, const double *rkappa, const double *zk, int32_t levmax,
+                     int32_t nterms, int32_t mode, void *stream);
+/* yy = A xx.  When non-NULL, stress (nbk complex) receives GW + self + pressure terms, za / pk
+ * (per body) the reference's za and pk (after the pinf shift). */
+int eb_matvec_apply(eb_matvec mv, const double *xx, double *yy, double *stress, double *za, double *pk,
+                    void *stream);
+int eb_matvec_destroy(eb_matvec mv);
+
+/* Drop-in for FASMVP (stokesbubb.f:496-498) on host arrays, Fortran convention (all by
+ * reference): (ND, NBK, K, X, Y, DZ, DSDTH, RKAPPA, ZK, U, W, H, ZA, PK, STRESS, DENS).
+ * The FMM runs with the reference's LEVMAX 7 and TOL 1e-13 (stokesbubb.f:564-567). */
+int32_t eb_fasmvp(const int32_t *nd, const int32_t *nbk, const int32_t *k, const double *x, const double *y,
+                  const double *dz, const double *dsdth, const double *rkappa, const double *zk,
+                  const double *u, double *w, const double *h, double *za, double *pk, double *stress,
+                  double *dens);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,224 @@
+"""GPU parity of the per-body spectral geometry, the RK2 stage update and the FASMVP matvec.
+
+Checkers: outputs of the compiled reference (tests/golden/spectral.npz,
+geometry_two_drops.npz), the C restatement (oracle.*), numpy's FFT (the exact DFT definition
+DCFFTF follows) and closed forms (trigonometric polynomials, ellipses).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from electricbubbles_amd.workloads import perturbed_circle
+from tests.conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as _t
+    if not _t.cuda.is_available():
+        pytest.skip("no GPU")
+    return _t
+
+
+@pytest.fixture(scope="module")
+def sp(torch):
+    from electricbubbles_amd import spectral
+    return spectral
+
+
+def g(name):
+    return np.load(os.path.join(GOLDEN, name + ".npz"))
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+def cat(arrs):
+    return np.concatenate([np.asarray(a) for a in arrs])
+
+
+LENGTHS = [8, 12, 30, 64, 100, 256, 1234]
+
+
+def test_dcfft_and_fdifff_batched_match_reference(torch, sp):
+    t = g("spectral")
+    nd = np.array(LENGTHS, np.int32)
+    B = sp.Bodies(nd)
+    c = torch.tensor(cat([t[f"c_{n}"] for n in LENGTHS]), device="cuda")
+    fwd = B.dcfft(c, -1).cpu().numpy()
+    bwd = B.dcfft(c, +1).cpu().numpy()
+    dif = B.fdifff(c).cpu().numpy()
+    off = np.concatenate([[0], np.cumsum(nd)])
+    for k, n in enumerate(LENGTHS):
+        s = slice(off[k], off[k + 1])
+        assert rel(fwd[s], t[f"fwd_{n}"]) < 1e-13, n
+        assert rel(bwd[s], t[f"bwd_{n}"]) < 1e-13, n
+        assert rel(dif[s], t[f"dif_{n}"]) < 1e-12, n
+
+
+def test_dcfft_many_lengths_against_numpy(torch, sp):
+    """Every even length the batched sweep can draw (128..2048 log-uniform), plus odd and prime ones."""
+    rng = np.random.default_rng(2)
+    nd = np.round(np.exp(rng.uniform(np.log(128), np.log(2048), 300)) / 2).astype(np.int32) * 2
+    nd = np.concatenate([nd, [2, 3, 5, 7, 97, 1009, 2 * 1013, 3000, 3200]]).astype(np.int32)
+    data = [rng.normal(size=n) + 1j * rng.normal(size=n) for n in nd]
+    B = sp.Bodies(nd)
+    c = torch.tensor(cat(data), device="cuda")
+    fwd = B.dcfft(c, -1).cpu().numpy()
+    bwd = B.dcfft(c, +1).cpu().numpy()
+    off = np.concatenate([[0], np.cumsum(nd)])
+    for k, n in enumerate(nd):
+        s = slice(off[k], off[k + 1])
+        assert rel(fwd[s], np.fft.fft(data[k])) < 1e-12, n
+        assert rel(bwd[s], n * np.fft.ifft(data[k])) < 1e-12, n
+
+
+def test_fdifff_exact_on_trig_polynomials(torch, sp):
+    nd = np.array([16, 30, 64, 250, 2048], np.int32)
+    f, d = [], []
+    for n in nd:
+        a = 2 * np.pi * np.arange(n) / n
+        f.append(np.exp(3j * a) + 0.5 * np.cos(5 * a) - 0.25j * np.sin(2 * a))
+        d.append(3j * np.exp(3j * a) - 2.5 * np.sin(5 * a) - 0.5j * np.cos(2 * a))
+    B = sp.Bodies(nd)
+    out = B.fdifff(torch.tensor(cat(f), device="cuda")).cpu().numpy()
+    off = np.concatenate([[0], np.cumsum(nd)])
+    for k, n in enumerate(nd):
+        # rounding of the transform is amplified by wavenumbers up to n/2
+        err = np.abs(out[off[k]:off[k + 1]] - d[k]).max()
+        assert err < 5e-15 * n, (n, err)
+
+
+def test_filter_matches_reference(torch, sp):
+    t = g("spectral")
+    for n in (64, 256):
+        B = sp.Bodies(np.array([n], np.int32))
+        got = B.filter(torch.tensor(t[f"filt_in_{n}"], device="cuda")).cpu().numpy()
+        ref = t[f"filt_out_{n}"]
+        for part in (np.real, np.imag):
+            a, b = part(got), part(ref)
+            both_small = (np.abs(a) < 2e-13) & (np.abs(b) < 2e-13)
+            close = np.abs(a - b) <= 1e-14 * np.abs(b) + 1e-300
+            assert np.all(both_small | close), n
+
+
+def test_equit_matches_reference(torch, sp):
+    t = g("spectral")
+    nd = np.array([64, 256], np.int32)
+    B = sp.Bodies(nd)
+    tha = torch.tensor(cat([t["equit_tha_64"], t["equit_tha_256"]]), device="cuda")
+    un = torch.tensor(cat([t["equit_un_64"], t["equit_un_256"]]), device="cuda")
+    got = B.equit(tha, un).cpu().numpy()
+    assert rel(got[:64], t["equit_t_64"]) < 1e-12
+    assert rel(got[64:], t["equit_t_256"]) < 1e-12
+
+
+def test_extractdat_matches_reference(torch, sp):
+    t = g("geometry_two_drops")
+    B = sp.Bodies(t["nd"])
+    o = B.extractdat(torch.tensor(t["rku"], device="cuda"))
+    for k in ("x", "y", "zk", "dz", "tha", "dsda", "rkappa", "ckern"):
+        assert rel(o[k].cpu().numpy(), t[k]) < 1e-12, k
+    xls = o["xls"].cpu().numpy()
+    assert xls[0] == float(t["xleft"]) and abs(xls[1] - float(t["slength"])) < 1e-14
+    ora = oracle.extractdat(t["nd"], t["rku"])
+    assert np.abs(o["area"].cpu().numpy() - ora["area"]).max() < 1e-13
+
+
+def test_extractdat_ellipses_closed_forms(torch, sp):
+    nd = np.array([128, 200, 512], np.int32)
+    ab = [(1.5, 0.8), (1.0, 0.3), (2.0, 1.9)]
+    z = []
+    for n, (a, b) in zip(nd, ab):
+        th = 2 * np.pi * np.arange(n) / n
+        z.append(a * np.cos(th) + 1j * b * np.sin(th) + 0.1)
+    B = sp.Bodies(nd)
+    o = B.extractdat(torch.tensor(cat(z).view(np.float64).copy(), device="cuda"))
+    area = o["area"].cpu().numpy()
+    kap = o["rkappa"].cpu().numpy()
+    off = np.concatenate([[0], np.cumsum(nd)])
+    for k, (n, (a, b)) in enumerate(zip(nd, ab)):
+        assert abs(area[k] - np.pi * a * b) < 1e-11
+        th = 2 * np.pi * np.arange(n) / n
+        exact = a * b / (a * a * np.sin(th) ** 2 + b * b * np.cos(th) ** 2) ** 1.5
+        assert np.abs(kap[off[k]:off[k + 1]] + exact).max() < 1e-8 * exact.max()
+
+
+def test_ftail_matches_oracle(torch, sp):
+    t = g("geometry_two_drops")
+    rng = np.random.default_rng(8)
+    un = rng.normal(size=t["x"].size) * 0.05
+    B = sp.Bodies(t["nd"])
+    got = B.ftail(torch.tensor(t["tha"], device="cuda"), torch.tensor(un, device="cuda"),
+                  torch.tensor(t["dz"], device="cuda"), torch.tensor(t["dsda"], device="cuda")).cpu().numpy()
+    ref = oracle.ftail(t["nd"], t["tha"], un, t["dz"], t["dsda"]).view(np.complex128)
+    assert rel(got, ref) < 1e-12
+
+
+def test_rk2_stages_match_oracle(torch, sp):
+    rng = np.random.default_rng(1)
+    ybar, dy1, dy2 = rng.normal(size=(3, 1000))
+    phi0, y = sp.rk2_stage1(1e-3, torch.tensor(ybar, device="cuda"), torch.tensor(dy1, device="cuda"))
+    ophi0, oy = oracle.rk2_stage1(1e-3, ybar, dy1)
+    np.testing.assert_array_equal(phi0.cpu().numpy(), ophi0)
+    np.testing.assert_array_equal(y.cpu().numpy(), oy)
+    yb = torch.tensor(ybar, device="cuda")
+    sp.rk2_stage2(1e-3, phi0, torch.tensor(dy2, device="cuda"), yb)
+    np.testing.assert_array_equal(yb.cpu().numpy(), oracle.rk2_stage2(1e-3, ophi0, dy2, ybar))
+
+
+def test_fasmvp_dropin_matches_reference(torch, sp):
+    t = g("geometry_two_drops")
+    w, stress, za, pk = sp.fasmvp(t["nd"], t["x"], t["y"], t["dz"], t["dsda"], t["rkappa"], t["zk"], t["u"])
+    assert rel(w, t["w"]) < 1e-10
+    assert rel(stress, t["stress"]) < 1e-10
+    assert rel(za, t["za"]) < 1e-13
+    np.testing.assert_allclose(pk, t["pk"], rtol=1e-12, atol=1e-14)
+
+
+def test_matvec_device_api_matches_dropin_and_is_real_linear(torch, sp):
+    t = g("geometry_two_drops")
+    dev = "cuda"
+    B = sp.Bodies(t["nd"])
+    M = sp.MatVec(B, torch.tensor(t["x"], device=dev), torch.tensor(t["y"], device=dev),
+                  torch.tensor(t["dz"], device=dev), torch.tensor(t["dsda"], device=dev),
+                  torch.tensor(t["rkappa"], device=dev), torch.tensor(t["zk"], device=dev), levmax=7, nterms=43,
+                  mode=1)
+    u = torch.tensor(t["u"], device=dev)
+    yy = M.apply(u).cpu().numpy()
+    assert rel(yy, t["w"]) < 1e-10
+    rng = np.random.default_rng(4)
+    v1 = torch.tensor(rng.normal(size=t["u"].size), device=dev)
+    v2 = torch.tensor(rng.normal(size=t["u"].size), device=dev)
+    lhs = M.apply(0.3 * v1 + v2).cpu().numpy()
+    rhs = 0.3 * M.apply(v1).cpu().numpy() + M.apply(v2).cpu().numpy()
+    assert rel(lhs, rhs) < 1e-13
+
+
+def test_many_drop_matvec_against_oracle(torch, sp):
+    """64 drops of uneven N (the many-drop configuration's shapes at smaller radius spread)."""
+    rng = np.random.default_rng(64)
+    zs, nds = [], []
+    for k in range(64):
+        n = int(2 ** round(np.log2(512 * rng.uniform(0.5, 1.5))))
+        n = min(1024, max(128, n))
+        c = rng.uniform(-20, 20) + 1j * rng.uniform(-20, 20)
+        modes = [(m, rng.uniform(-0.05, 0.05)) for m in rng.choice(np.arange(2, 8), 3, replace=False)]
+        z, _, _, _ = perturbed_circle(n, modes, center=c, radius=rng.uniform(0.5, 1.5))
+        zs.append(z)
+        nds.append(n)
+    nd = np.array(nds, np.int32)
+    rku = cat(zs).view(np.float64).copy()
+    ex = oracle.extractdat(nd, rku)
+    u = rng.normal(size=2 * int(nd.sum()))
+    w_gpu, stress, za, pk = sp.fasmvp(nd, ex["x"], ex["y"], ex["dz"], ex["dsda"], ex["rkappa"], ex["zk"], u)
+    h = 2 * np.pi / nd
+    gw, _, ier, _ = oracle.cadap(oracle.DR, 7, nd, ex["x"], ex["y"], ex["dz"], u.view(np.complex128), h, 1e-13)
+    w_ref, za_ref, pk_ref = oracle.fasmvp_epilogue(nd, ex["x"], ex["y"], ex["dz"], ex["dsda"], ex["rkappa"],
+                                                   ex["zk"], u, gw)
+    assert rel(w_gpu, w_ref) < 1e-10
