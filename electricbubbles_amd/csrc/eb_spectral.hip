@@ -379,20 +379,22 @@ __global__ void k_extract_final(const double* __restrict__ red4, int nbody, doub
 
 __global__ void k_rk2_1(int neq, double dt, const double* __restrict__ ybar, const double* __restrict__ dy,
                         double* __restrict__ phi0, double* __restrict__ y) {
+#pragma clang fp contract(off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= neq) return;
   // separate roundings, as the reference's two statements (no fused multiply-add)
-  const double p = __dmul_rn(dt, dy[i]);
+  const double p = dt * dy[i];
   phi0[i] = p;
-  y[i] = __dadd_rn(ybar[i], p);
+  y[i] = ybar[i] + p;
 }
 
 __global__ void k_rk2_2(int neq, double dt, const double* __restrict__ phi0, const double* __restrict__ dy,
                         double* __restrict__ ybar) {
+#pragma clang fp contract(off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= neq) return;
-  const double p1 = __dmul_rn(dt, dy[i]);
-  ybar[i] = __dadd_rn(__dadd_rn(ybar[i], __dmul_rn(0.5, phi0[i])), __dmul_rn(0.5, p1));
+  const double p1 = dt * dy[i];
+  ybar[i] = ybar[i] + 0.5 * phi0[i] + 0.5 * p1;
 }
 
 // ------------------------------------------------------------------------------------------
