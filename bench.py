@@ -30,6 +30,14 @@ sys.path.insert(0, ROOT)
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 
+# HBM bytes per launch of the dominant kernels, from two rocprofv3 --pmc passes (FETCH_SIZE,
+# WRITE_SIZE) of this script, scaled by the same counters on a 256 MiB copy (fetch x 1.99975,
+# write x 1.000); profiles/fmm_curves_n1048576_L8_p32_hbm_traffic.md has the commands and
+# the per-kernel table.  Keyed by (workload, points, levmax, nterms).
+PROFILED_TRAFFIC_BYTES = {
+    ("curves", 1 << 20, 8, 32): {"m2l": 199325250, "p2p": 227623403},
+}
+
 
 def flops_model(info, n, p):
     """Algorithmic flop convention: complex add 2, complex*real 2, complex*complex 6,
@@ -161,6 +169,7 @@ def main():
     fl = flops_model(info, n, p)
     dom = max(("p2p", "m2l"), key=lambda k: stage[k])
     achieved = fl[dom] / (stage[dom] * 1e-3) / 1e12
+    traffic = PROFILED_TRAFFIC_BYTES.get((args.workload, n, L, p), {}).get(dom)
     total_fl = sum(fl.values())
 
     check = None
@@ -197,7 +206,7 @@ def main():
         "stage_ms": {k: round(v, 4) for k, v in stage.items()},
         "fp64_fraction_total": round(total_fl / (t_apply) / 1e12 / PEAK_FP64_TFLOPS, 4),
         "roofline": {"bound": "mfma", "kernel": dom, "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS,
-                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
+                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
                      "flops_per_launch": fl[dom]},
         "accuracy_vs_direct": check,
         "cpu_baseline": cpu,

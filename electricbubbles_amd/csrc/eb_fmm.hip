@@ -50,6 +50,18 @@ extern "C" int eb_version(void) { return 1; }
 // ------------------------------------------------------------------------------------------
 __device__ __forceinline__ int pidx(int ix, int iy, int W) { return (iy + 3) * W + (ix + 3); }
 
+// Logical block index in an XCD-aware order.  The dispatcher hands physical block i to XCD
+// i % 8; this mapping gives each XCD one contiguous range of logical blocks (neighbouring
+// boxes or points), so lines shared by neighbours are fetched into one L2 instead of eight.
+// The grid must be a multiple of 8 (xcd_grid); blocks past the logical count return.
+// Used where the work per block is uniform (L2P): with uneven leaves (P2P, P2M) a
+// contiguous range per XCD unbalances the XCDs, which costs more than the L2 misses.
+__device__ __forceinline__ int xcd_block() {
+  const int n = (int)gridDim.x;
+  return (int)(blockIdx.x % 8) * (n / 8) + (int)(blockIdx.x / 8);
+}
+static inline int xcd_grid(int nblk) { return (nblk + 7) & ~7; }
+
 // [start, end) of the sorted points inside box (ix, iy) of level l, from the leaf table.
 __device__ __forceinline__ void box_range(const int* __restrict__ leaf_start, int L, int l, int ix, int iy,
                                           int& s, int& e) {
@@ -444,7 +456,7 @@ template <int KMAX>
 __global__ __launch_bounds__(256) void k_l2p(int n, int L, const uint32_t* __restrict__ keys,
                                              const double2* __restrict__ zs, const double2* __restrict__ lo, int W,
                                              int NBP, double sgn, double2* __restrict__ gwf) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint32_t key = keys[i];
   const int ix = (int)compact2(key), iy = (int)compact2(key >> 1);
@@ -976,7 +988,7 @@ static void launch_far(eb_fmm_plan P) {
   for (int l = 3; l <= L; ++l)
     launch_xlat<XL_L2L, K>(P, xlat_levels(P, l, l, -1), P->lo, P->lo, P->dPL, P->dQL);
   if (P->timing) (void)hipEventRecord(P->ev[5], st);
-  hipLaunchKernelGGL(k_l2p<K>, dim3((P->n + 255) / 256), dim3(256), 0, st, P->n, L, P->keys, P->zs,
+  hipLaunchKernelGGL(k_l2p<K>, dim3(xcd_grid((P->n + 255) / 256)), dim3(256), 0, st, P->n, L, P->keys, P->zs,
                      P->lo + P->lvl_off[L], P->lvl_W[L], P->lvl_NBP[L], P->sgn, P->gwf);
   if (P->timing) (void)hipEventRecord(P->ev[6], st);
 }
