@@ -15,6 +15,7 @@ may import this package, and only as the checker.  The product package
 from __future__ import annotations
 
 import ctypes
+import math
 import os
 import subprocess
 
@@ -428,6 +429,163 @@ def ref_fasmvp(nd, x, y, dz, dsdth, rkappa, zk, u):
                   _d(np.ascontiguousarray(u, np.float64)), _d(w), _d(h), _d(za), _d(pk),
                   _d(stress), _d(dens))
     return w, stress.view(np.complex128), za.view(np.complex128), pk
+
+
+# ------------------------------------------------------------------------------------
+# Adaptive quadtree of DAPIF2 (dapif1.f:740-1680, dapif2.f:3-474)
+# ------------------------------------------------------------------------------------
+
+ATREE_TABLES = ("xs", "ys", "nl", "ibox", "xb", "yb", "nbat", "npar", "nchi", "indb", "iat", "jat",
+                "l1", "l2", "jl", "l3", "l4", "jl3", "jl4")
+
+
+def _nint(v):
+    """Fortran NINT: nearest integer, halves away from zero."""
+    return int(math.floor(v + 0.5)) if v >= 0 else -int(math.floor(-v + 0.5))
+
+
+def dapif_lmx(mex=300, tol=1e-11):
+    """Maximum level count DAPIF2 allows (dapif1.f:690-704): N = -NINT(log2 TOL) + 3,
+    LMX = INT((MEX log2 10 - 1)/N + 6)."""
+    n = -_nint(math.log(tol) / math.log(2.0)) + 3
+    rex2 = mex * (math.log(10.0) / math.log(2.0)) - 1
+    return int(rex2 / n + 6)
+
+
+def atree(x, y, napb, lmx=None, eps7=1e-16):
+    """The C restatement of DAPIF2's tree.  Returns a dict of the 1-based tables, the
+    scaled coordinates xs/ys, geom = (rscal, xonew, yonew) and the counts."""
+    L = lib()
+    L.ora_atree_build.restype = ctypes.c_void_p
+    L.ora_atree_build.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_double]
+    L.ora_atree_sizes.argtypes = [ctypes.c_void_p, _ip]
+    L.ora_atree_copy.argtypes = [ctypes.c_void_p] + [ctypes.c_void_p] * 20
+    L.ora_atree_free.argtypes = [ctypes.c_void_p]
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    lmx = dapif_lmx() if lmx is None else lmx
+    h = L.ora_atree_build(x.size, _d(x), _d(y), int(napb), int(lmx), float(eps7))
+    try:
+        sz = np.zeros(10, np.int32)
+        L.ora_atree_sizes(h, _i(sz))
+        ierr, nbox, nlev, ln, ln3, ln4, maxp, jatlen, nat = (int(v) for v in sz[:9])
+        out = dict(ierr=ierr, nbox=nbox, nlev=nlev, len=ln, len3=ln3, len4=ln4, maxp=maxp)
+        if ierr != 0:
+            return out
+        shapes = dict(xs=(nat, np.float64), ys=(nat, np.float64), geom=(3, np.float64), nl=(nlev + 2, np.int32),
+                      ibox=(nat, np.int32), xb=(nbox, np.float64), yb=(nbox, np.float64))
+        for k in ("nbat", "npar", "nchi", "indb", "iat"):
+            shapes[k] = (nbox, np.int32)
+        shapes.update(jat=(jatlen, np.int32), l1=(nbox + 1, np.int32), l2=(nbox + 1, np.int32),
+                      jl=(ln, np.int32), l3=(nbox + 1, np.int32), l4=(nbox + 1, np.int32),
+                      jl3=(ln3, np.int32), jl4=(ln4, np.int32))
+        order = ("xs", "ys", "geom", "nl", "ibox", "xb", "yb", "nbat", "npar", "nchi", "indb", "iat", "jat",
+                 "l1", "l2", "jl", "l3", "l4", "jl3", "jl4")
+        arrs = {k: np.zeros(shapes[k][0], shapes[k][1]) for k in order}
+        L.ora_atree_copy(h, *(arrs[k].ctypes.data for k in order))
+        out.update(arrs)
+        return out
+    finally:
+        L.ora_atree_free(h)
+
+
+_CLOSE_STUB = ctypes.CFUNCTYPE(None)(lambda: None)
+
+
+def ref_dapif2_scaled(x, y, napb, tol=1e-11, mex=300, eps7=1e-16, seed=0):
+    """Run the reference DAPIF2 (IFLAG7 = 3, NINIRE = 2, the parameters of laplace.f:204-215
+    apart from NAPB) and return the scaled coordinates it leaves in WKSP(1..2 NAT), with
+    INFORM(2) = NBOX and INFORM(3) = number of levels."""
+    r = ref()
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    nat = x.size
+    rng = np.random.default_rng(seed)
+    qa = _cplx(rng.normal(size=nat) + 1j * rng.normal(size=nat)).copy()
+    poten = np.zeros(nat)
+    field = np.zeros(2 * nat)
+    nsp = 4_000_000 + 400 * nat
+    wksp = np.zeros(nsp)
+    iout = np.zeros(2, np.int32)
+    ierr = np.zeros(8, np.int32)
+    inform = np.zeros(8, np.int32)
+    r.dapif2_(_i(iout), _c_int(3), _c_int(nat), _c_int(napb), _c_int(2), _c_int(mex), _i(ierr), _i(inform),
+              _c_double(tol), _c_double(eps7), _d(x), _d(y), _d(qa), _d(poten), _d(field), _d(wksp),
+              _c_int(nsp), _CLOSE_STUB)
+    return dict(ierr=int(ierr[0]), xs=wksp[:nat].copy(), ys=wksp[nat:2 * nat].copy(), nbox=int(inform[1]),
+                nlev=int(inform[2]))
+
+
+def ref_atree(xs, ys, napb, lmx=None, eps7=1e-16):
+    """DAPIF2's table builders called one by one from the reference (DAASSI, DACOMP, DAINDB,
+    DAIJAT, DALI12, DALI34, in the order of dapif1.f:794-957) on scaled coordinates, with
+    arrays of their own.  SHIFT gets one zero word in front, the value DACOMP reads at
+    SHIFT(0) inside DAPIF2's zeroed workspace."""
+    r = ref()
+    xs = np.ascontiguousarray(xs, np.float64)
+    ys = np.ascontiguousarray(ys, np.float64)
+    nat = xs.size
+    lmx = dapif_lmx() if lmx is None else lmx
+    mx = 8 * nat + 64
+    qa = np.zeros(2 * nat)
+    ierr = np.zeros(8, np.int32)
+    nl = np.zeros(128, np.int32)
+    ibox = np.zeros(nat + 1, np.int32)
+    xb = np.zeros(mx + 2)
+    yb = np.zeros(mx + 2)
+    nbat = np.zeros(mx + 2, np.int32)
+    npar = np.zeros(mx + 2, np.int32)
+    nchi = np.zeros(mx + 2, np.int32)
+    nbox = ctypes.c_int()
+    lev = ctypes.c_int()
+    r.daassi_(_c_double(eps7), _c_int(napb), _c_int(nat), ctypes.byref(nbox), _c_int(mx), _i(ierr),
+              ctypes.byref(lev), _c_int(lmx), _i(nl), _d(xs), _d(ys), _d(qa), _i(ibox), _d(xb), _d(yb),
+              _i(nbat), _i(npar), _i(nchi))
+    if ierr[0] != 0:
+        return dict(ierr=int(ierr[0]))
+    shift = np.zeros(mx + 3, np.int32)
+    nbol = ctypes.c_int()
+    r.dacomp_(_c_int(nat), ctypes.byref(nbox), ctypes.byref(nbol), ctypes.byref(lev), _i(nl), _i(ibox), _d(xb),
+              _d(yb), _i(nbat), _i(npar), _i(nchi), _i(shift[1:]))
+    nb = nbox.value
+    Lv = lev.value
+    indb = np.zeros(nb + 2, np.int32)
+    maxp = ctypes.c_int()
+    r.daindb_(_c_int(napb), _c_int(nat), ctypes.byref(maxp), ctypes.byref(nbox), ctypes.byref(lev), _i(nl),
+              _i(nbat), _i(nchi), _i(indb))
+    iat = np.zeros(nb + 2, np.int32)
+    jat = np.zeros(nb + nat + 2, np.int32)
+    r.daijat_(_c_int(napb), _c_int(nat), ctypes.byref(nbox), ctypes.byref(lev), _i(nl), _i(ibox), _i(nbat),
+              _i(indb), _i(iat), _i(jat))
+    l1 = np.zeros(nb + 2, np.int32)
+    l2 = np.zeros(nb + 2, np.int32)
+    jl = np.zeros(36 * (nb + 1) + 2, np.int32)
+    ln = ctypes.c_int()
+    r.dali12_(ctypes.byref(nbox), ctypes.byref(ln), ctypes.byref(lev), _i(nl), _d(xb), _d(yb), _i(npar),
+              _i(nchi), _i(l1), _i(l2), _i(jl))
+    l3 = np.zeros(nb + 2, np.int32)
+    l4 = np.zeros(nb + 2, np.int32)
+    idesc = np.zeros(4 * nb + 64, np.int32)
+    big = 256 * nb + 4096
+    jl3 = np.zeros(big, np.int32)
+    jl4 = np.zeros(big, np.int32)
+    ln3 = ctypes.c_int()
+    ln4 = ctypes.c_int()
+    r.dali34_(_c_int(napb), ctypes.byref(nbox), ctypes.byref(ln3), ctypes.byref(ln4), ctypes.byref(lev), _i(nl),
+              _d(xb), _d(yb), _i(nbat), _i(npar), _i(nchi), _i(indb), _i(l1), _i(l2), _i(jl), _i(l3), _i(l4),
+              _i(idesc), _i(jl3), _i(jl4))
+    lead = int(l1[nb]) - 1
+    jatlen = 0
+    leaves = np.nonzero(indb[:nb] == 1)[0]
+    if leaves.size:
+        last = leaves[-1]
+        jatlen = int(iat[last]) + int(nbat[last])
+    return dict(ierr=0, nbox=nb, nlev=Lv - 1, len=lead, len3=ln3.value, len4=ln4.value, maxp=maxp.value,
+                xs=xs.copy(), ys=ys.copy(), nl=nl[:Lv + 1].copy(), ibox=ibox[:nat].copy(), xb=xb[:nb].copy(),
+                yb=yb[:nb].copy(), nbat=nbat[:nb].copy(), npar=npar[:nb].copy(), nchi=nchi[:nb].copy(),
+                indb=indb[:nb].copy(), iat=iat[:nb].copy(), jat=jat[:jatlen].copy(), l1=l1[:nb + 1].copy(),
+                l2=l2[:nb + 1].copy(), jl=jl[:lead].copy(), l3=l3[:nb + 1].copy(), l4=l4[:nb + 1].copy(),
+                jl3=jl3[:ln3.value].copy(), jl4=jl4[:ln4.value].copy())
 
 
 # ------------------------------------------------------------------------------------

@@ -72,6 +72,19 @@ void ora_ftail(int k, const int *nd, const double *tha, const double *un, const 
 void ora_rk2_stage1(int neq, double dt, const double *ybar, const double *dy, double *phi0, double *y);
 void ora_rk2_stage2(int neq, double dt, const double *phi0, const double *dy, double *ybar);
 
+/* Adaptive quadtree of DAPIF2: scaling (dapif1.f:740-770), DAASSI..DAIJAT (dapif1.f:1165-1680),
+ * DALI12/DALI34 (dapif2.f:3-474).  Built on the host; ora_atree_sizes gives
+ * {ierr, nbox, nlev, len, len3, len4, maxp, jatlen, nat, napb}; ora_atree_copy fills caller arrays
+ * of those sizes (nl: nlev+2, ibox/xs/ys: nat, per-box arrays: nbox, l1..l4: nbox+1, jat: jatlen,
+ * jl: len, jl3: len3, jl4: len4, geom: {rscal, xonew, yonew}).  Any pointer may be NULL. */
+typedef struct ora_atree_s ora_atree;
+ora_atree *ora_atree_build(int nat, const double *xa, const double *ya, int napb, int lmx, double eps7);
+void ora_atree_sizes(const ora_atree *t, int *out);
+void ora_atree_copy(const ora_atree *t, double *xs, double *ys, double *geom, int *nl, int *ibox, double *xb,
+                    double *yb, int *nbat, int *npar, int *nchi, int *indb, int *iat, int *jat, int *l1, int *l2,
+                    int *jl, int *l3, int *l4, int *jl3, int *jl4);
+void ora_atree_free(ora_atree *t);
+
 #ifdef __cplusplus
 }
 #endif

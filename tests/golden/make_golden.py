@@ -2,7 +2,8 @@
 
 Runs only where the reference sources exist (they are compiled in place by oracle/Makefile);
 the fixtures it writes are data -- inputs and the reference's outputs -- and are committed so
-the tests run without the reference.  Usage:  python tests/golden/make_golden.py
+the tests run without the reference.  Usage:  python tests/golden/make_golden.py [atree]
+("atree" rewrites only the adaptive-tree fixtures).
 """
 from __future__ import annotations
 
@@ -129,5 +130,38 @@ def main():
          slength=np.float64(ex["slength"]), u=u, w=w, stress=stress, za=za, pk=pk)
 
 
+def atree_cases():
+    """Point sets for the adaptive tree of DAPIF2: uniform (no list 3/4), curves and a tight
+    cluster in a sparse background (deep, uneven tree: long lists 3 and 4)."""
+    rng = np.random.default_rng(20260202)
+    cases = []
+    z, _, _, _ = curves_cfg(16, 128, rng)
+    cases.append(("curves2048_napb30", z.real, z.imag, 30))
+    z, _, _, _ = uniform_cfg(5000, rng)
+    cases.append(("uniform5000_napb40", z.real, z.imag, 40))
+    c = np.concatenate([rng.normal(0.3, 0.01, (1500, 2)), rng.uniform(0, 1, (1500, 2))])
+    cases.append(("cluster3000_napb40", c[:, 0].copy(), c[:, 1].copy(), 40))
+    z, _, _, _ = curves_cfg(64, 256, rng)
+    cases.append(("curves16384_napb40", z.real, z.imag, 40))
+    return cases
+
+
+def main_atree():
+    """DAPIF2's scaled coordinates (from its workspace) and the tables of DAASSI, DACOMP,
+    DAINDB, DAIJAT, DALI12 and DALI34 run on them."""
+    for name, x, y, napb in atree_cases():
+        w = oracle.ref_dapif2_scaled(x, y, napb)
+        assert w["ierr"] == 0
+        t = oracle.ref_atree(w["xs"], w["ys"], napb)
+        assert t["ierr"] == 0 and t["nbox"] == w["nbox"] and t["nlev"] == w["nlev"]
+        counts = np.array([t[k] for k in ("nbox", "nlev", "len", "len3", "len4", "maxp")], np.int32)
+        save("atree_" + name, x=x, y=y, napb=np.int32(napb), lmx=np.int32(oracle.dapif_lmx()),
+             eps7=np.float64(1e-16), counts=counts, **{k: t[k] for k in oracle.ATREE_TABLES})
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["atree"]:
+        main_atree()
+    else:
+        main()
+        main_atree()
