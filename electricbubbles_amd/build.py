@@ -28,6 +28,8 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-result", "-Wno-unused-variable",
     "-I", os.path.join(ROOT, "include"),
 ]
+# per-source additions to the flags above
+HIP_EXTRA: dict = {}
 
 
 def _run(cmd, cwd=None):
@@ -49,7 +51,7 @@ def build_hip(verbose: bool = False) -> str:
             continue
         obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
         lang = ["-x", "hip"] if src.endswith(".hip") else []
-        _run([HIPCC, *HIP_FLAGS, *lang, "-c", path, "-o", obj])
+        _run([HIPCC, *HIP_FLAGS, *HIP_EXTRA.get(src, []), *lang, "-c", path, "-o", obj])
         objs.append(obj)
     tmp = out + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", tmp])

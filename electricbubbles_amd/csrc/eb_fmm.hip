@@ -512,7 +512,7 @@ __device__ __forceinline__ void p2p_pair(double2 zi, double2 zj, double2 qa, dou
 // One 64-lane block per target leaf.  The sources of the 3x3 neighbourhood (MKNBOR,
 // cadap2.f:705-742) are staged through LDS in chunks of P2P_CH points; with T < 64 targets
 // the lanes split into G = 64/T source slices per target, summed through LDS at the end.
-constexpr int P2P_CH = 256;
+constexpr int P2P_CH = 128;  // 6 KB of LDS per block: LDS does not limit the waves per CU
 
 __global__ __launch_bounds__(64) void k_p2p(int L, const int* __restrict__ leaf_start,
                                             const double2* __restrict__ zs, const double2* __restrict__ qa,
@@ -520,7 +520,9 @@ __global__ __launch_bounds__(64) void k_p2p(int L, const int* __restrict__ leaf_
                                             const int32_t* __restrict__ perm, const Geom* __restrict__ g,
                                             double sgn, double2* __restrict__ out) {
   __shared__ double2 sz[P2P_CH], sa[P2P_CH], sq[P2P_CH];
-  __shared__ double2 red[64];
+  double2* const red = sz;  // the per-slice partial sums reuse the staging buffers
+  double2* const red2 = sa;
+  static_assert(P2P_CH >= 64, "reduction buffers alias the staging buffers");
   const int nside = 1 << L;
   const int ix = blockIdx.x % nside, iy = blockIdx.x / nside;
   const uint32_t mk = morton((uint32_t)ix, (uint32_t)iy);
@@ -550,7 +552,6 @@ __global__ __launch_bounds__(64) void k_p2p(int L, const int* __restrict__ leaf_
   const int TPc = TP < 64 ? TP : 64;
   const int G = TP < 64 ? 64 / TP : 1;
   const int tp = lane % TPc, gs = lane / TPc;
-  __shared__ double2 red2[64];
   for (int t0 = 0; t0 < T; t0 += 128) {
     const int i0 = ts + t0 + 2 * tp, i1 = i0 + 1;
     const bool act = gs < G && t0 + 2 * tp < T;
@@ -584,6 +585,7 @@ __global__ __launch_bounds__(64) void k_p2p(int L, const int* __restrict__ leaf_
         }
       }
     }
+    __syncthreads();
     red[lane] = make_double2(a10.x + sgn * a20.x, a10.y + sgn * a20.y);
     red2[lane] = make_double2(a11.x + sgn * a21.x, a11.y + sgn * a21.y);
     __syncthreads();
