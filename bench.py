@@ -172,6 +172,20 @@ def main():
     traffic = PROFILED_TRAFFIC_BYTES.get((args.workload, n, L, p), {}).get(dom)
     total_fl = sum(fl.values())
 
+    # the adaptive tree of the same points (DAPIF2's rules, at most 40 points per childless box)
+    from electricbubbles_amd.atree import AdaptiveTree
+    at = AdaptiveTree(x, y, 40)
+    at.close()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    reps = 3
+    for _ in range(reps):
+        at = AdaptiveTree(x, y, 40)
+        at_info = {"napb": 40, "boxes": at.nbox, "levels": at.nlev, "max_leaf": at.maxp}
+        at.close()
+    torch.cuda.synchronize()
+    at_info["build_ms"] = round((time.perf_counter() - t0) / reps * 1e3, 3)
+
     check = None
     if not args.no_check:
         import oracle
@@ -204,6 +218,7 @@ def main():
                    "levmax": L, "nterms": p, "leaves_nonempty": info["leaves"], "max_leaf": info["max_leaf"]},
         "apply_only": {"value": round(n / t_apply / 1e6, 3), "unit": "Mpts/s", "ms": round(t_apply * 1e3, 4)},
         "stage_ms": {k: round(v, 4) for k, v in stage.items()},
+        "adaptive_tree": at_info,
         "fp64_fraction_total": round(total_fl / (t_apply) / 1e12 / PEAK_FP64_TFLOPS, 4),
         "roofline": {"bound": "mfma", "kernel": dom, "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
