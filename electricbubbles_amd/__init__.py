@@ -10,7 +10,8 @@ work; this module is a thin ctypes layer over that C-ABI:
 * :func:`tables` / :func:`lists` -- the reference's per-level binning tables and
   interaction lists computed on the GPU (tree parity);
 * spectral / RK / FASMVP-epilogue wrappers (see :mod:`electricbubbles_amd.spectral`);
-* DAPIF2's adaptive quadtree on the GPU (see :mod:`electricbubbles_amd.atree`).
+* DAPIF2's adaptive quadtree on the GPU (see :mod:`electricbubbles_amd.atree`);
+* the device-resident GMRES solve of the Stokes equation (see :mod:`electricbubbles_amd.gmres`).
 
 There is no CPU fallback: if ``libeb.so`` is missing, or no GPU is visible, the calls raise.
 """
@@ -300,6 +301,6 @@ def lists(ilev, device="cuda", stream=None):
     return ([intl[b, : nintl[b]].copy() for b in range(nb)], [nbor[b, : nnbor[b]].copy() for b in range(nb)])
 
 
-from . import atree, spectral  # noqa: E402
+from . import atree, gmres, spectral  # noqa: E402
 
-EXPORTS = EXPORTS + spectral.SPECTRAL_EXPORTS + atree.ATREE_EXPORTS
+EXPORTS = EXPORTS + spectral.SPECTRAL_EXPORTS + atree.ATREE_EXPORTS + gmres.GMRES_EXPORTS

@@ -738,6 +738,7 @@ extern "C" int eb_matvec_create(eb_matvec* out, eb_bodies B, const double* x, co
     free_mv(M);
     return (eb_set_error("eb_matvec_create: kernel failed"), EB_ERR_HIP);
   }
+  if (mode == EB_MODE_AUTO) mode = n <= 2048 ? EB_MODE_DIRECT : EB_MODE_FMM;  // as the CADAPDR drop-in
   rc = eb_fmm_plan_create(&M->fmm, n, M->x, M->y, (const double*)M->wdz, levmax, nterms, EB_VARIANT_DR, mode, stream);
   if (rc != EB_OK) {
     M->fmm = nullptr;
@@ -762,6 +763,18 @@ extern "C" int eb_matvec_apply(eb_matvec M, const double* xx, double* yy, double
                      (const double2*)M->dz, M->dsdth, M->rkappa, M->h, dens, (const double2*)M->za, M->pk, M->gw,
                      (double2*)yy, M->body_of);
   if (stress) EB_HIP_CHECK(hipMemcpyAsync(stress, M->gw, sizeof(double2) * M->n, hipMemcpyDeviceToDevice, st));
+  if (za) EB_HIP_CHECK(hipMemcpyAsync(za, M->za, sizeof(double2) * B->nbody, hipMemcpyDeviceToDevice, st));
+  if (pk) hipLaunchKernelGGL(k_shift_pk, dim3((B->nbody + 255) / 256), dim3(256), 0, st, B->nbody, pk, M->pk);
+  EB_HIP_CHECK(hipGetLastError());
+  return EB_OK;
+}
+
+extern "C" int eb_matvec_parts(eb_matvec M, const double* xx, double* za, double* pk, void* stream) {
+  if (!M || !xx) return (eb_set_error("eb_matvec_parts: bad arguments"), EB_ERR_ARG);
+  hipStream_t st = (hipStream_t)stream;
+  eb_bodies B = M->B;
+  hipLaunchKernelGGL(k_fasmvp_sums, dim3(B->nbody), dim3(SP_THREADS), 0, st, view(B), M->x, M->y,
+                     (const double2*)M->dz, (const double2*)M->zk, M->h, (const double2*)xx, M->za, M->pk);
   if (za) EB_HIP_CHECK(hipMemcpyAsync(za, M->za, sizeof(double2) * B->nbody, hipMemcpyDeviceToDevice, st));
   if (pk) hipLaunchKernelGGL(k_shift_pk, dim3((B->nbody + 255) / 256), dim3(256), 0, st, B->nbody, pk, M->pk);
   EB_HIP_CHECK(hipGetLastError());

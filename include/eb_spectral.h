@@ -65,6 +65,7 @@ int eb_rk2_stage2(int32_t neq, double dt, const double *phi0, const double *dy, 
  * (stokesbubb.f:496-614) with the layer potential from the GPU FMM (eb_fmm_plan).
  * Geometry (x, y, dz = DZ, dsdth = |z_alpha|, rkappa, zk per body) is bound at creation;
  * each apply maps the interleaved density xx (2*nbk) to yy (2*nbk).  h_k = 2 pi / nd_k.
+ * mode: EB_MODE_FMM, EB_MODE_DIRECT, or EB_MODE_AUTO (direct sum up to 2048 points).
  * --------------------------------------------------------------------------------------- */
 typedef struct eb_matvec_s *eb_matvec;
 

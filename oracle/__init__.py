@@ -589,6 +589,83 @@ def ref_atree(xs, ys, napb, lmx=None, eps7=1e-16):
 
 
 # ------------------------------------------------------------------------------------
+# DGMRES (dgmres.f:2-553) of the reference, on a dense operator or on the Stokes MATVEC
+# ------------------------------------------------------------------------------------
+
+_MATVEC_T = ctypes.CFUNCTYPE(None, _ip, _dp, _dp, _ip, _ip, _ip, _dp, _ip)
+
+
+def _dgmres_call(lib_, n, b, x, matvec_ptr, msolve_ptr, maxl, tol, nrmax):
+    lrgw = 1 + n * (maxl + 6) + maxl * (maxl + 3) + 16
+    rgwk = np.zeros(lrgw)
+    igwk = np.zeros(20, np.int32)
+    igwk[0] = maxl
+    igwk[4] = nrmax
+    it = ctypes.c_int()
+    err = ctypes.c_double()
+    ierr = ctypes.c_int()
+    sb = np.ones(n)
+    sx = np.ones(n)
+    dummy_i = np.zeros(4, np.int32)
+    dummy_d = np.zeros(4)
+    lib_.dgmres_.argtypes = [_ip, _dp, _dp, _ip, _ip, _ip, _dp, _ip, ctypes.c_void_p, ctypes.c_void_p, _ip, _dp,
+                             _ip, _ip, _dp, _ip, _ip, _dp, _dp, _dp, _ip, _ip, _ip, _dp, _ip]
+    lib_.dgmres_(_c_int(n), _d(b), _d(x), _c_int(0), _i(dummy_i), _i(dummy_i), _d(dummy_d), _c_int(0),
+                 matvec_ptr, msolve_ptr, _c_int(0), _c_double(tol), _c_int(0), ctypes.byref(it), ctypes.byref(err),
+                 ctypes.byref(ierr), _c_int(0), _d(sb), _d(sx), _d(rgwk), _c_int(lrgw), _i(igwk), _c_int(20),
+                 _d(dummy_d), _i(dummy_i))
+    return x, it.value, err.value, ierr.value
+
+
+def ref_dgmres_dense(a, b, x0, maxl=10, tol=0.0, nrmax=0):
+    """The reference DGMRES (ITOL 0, IUNIT 0, no scaling or preconditioner, IGWK(2..4) = 0)
+    with a MATVEC that multiplies by the dense matrix a.  Returns (x, iter, err, ierr)."""
+    a = np.ascontiguousarray(a, np.float64)
+    n = a.shape[0]
+
+    def mv(pn, px, py, *_):
+        xv = np.ctypeslib.as_array(px, shape=(n,))
+        yv = np.ctypeslib.as_array(py, shape=(n,))
+        yv[:] = a @ xv
+
+    cb = _MATVEC_T(mv)
+    b = np.ascontiguousarray(b, np.float64).copy()
+    x = np.ascontiguousarray(x0, np.float64).copy()
+    ptr = ctypes.cast(cb, ctypes.c_void_p)
+    return _dgmres_call(ref(), n, b, x, ptr, ptr, maxl, tol, nrmax)
+
+
+def ref_dgmres_stokes(nd, z, dz, dsdth, rkappa, zk, rhs, maxl=100, tol=1e-10, nrmax=-1):
+    """SOLVEI's solve (stokesbubb.f:329-358): the reference DGMRES with the reference MATVEC
+    (FASMVP + CADAPDR on the geometry of COMMON /inteqn/ and /problem/), from soln = rhs.
+    Returns (soln, iter, err, ierr)."""
+    r = ref()
+    nd = np.asarray(nd, np.int32)
+    k = nd.size
+    nbk = int(nd.sum())
+    kmax, nmax = 150, 450000
+    prob = (ctypes.c_char * (16 * kmax + 4 * kmax + 12)).from_address(
+        ctypes.addressof(ctypes.c_char.in_dll(r, "problem_")))
+    zkv = np.frombuffer(prob, dtype=np.complex128, count=kmax)
+    zkv[:] = 0
+    zkv[:k] = zk
+    ndv = np.frombuffer(prob, dtype=np.int32, count=kmax, offset=16 * kmax)
+    ndv[:] = 0
+    ndv[:k] = nd
+    np.frombuffer(prob, dtype=np.int32, count=3, offset=20 * kmax)[:] = [k, nbk, 2 * nbk]
+    inteqn = (ctypes.c_char * (48 * nmax)).from_address(ctypes.addressof(ctypes.c_char.in_dll(r, "inteqn_")))
+    np.frombuffer(inteqn, dtype=np.float64, count=nbk, offset=0)[:] = dsdth
+    np.frombuffer(inteqn, dtype=np.float64, count=nbk, offset=8 * nmax)[:] = rkappa
+    np.frombuffer(inteqn, dtype=np.complex128, count=nbk, offset=16 * nmax)[:] = z
+    np.frombuffer(inteqn, dtype=np.complex128, count=nbk, offset=32 * nmax)[:] = dz
+    b = np.ascontiguousarray(rhs, np.float64).copy()
+    x = b.copy()
+    mv = ctypes.cast(r.matvec_, ctypes.c_void_p)
+    ms = ctypes.cast(r.msolve_, ctypes.c_void_p)
+    return _dgmres_call(r, 2 * nbk, b, x, mv, ms, maxl, tol, nrmax)
+
+
+# ------------------------------------------------------------------------------------
 # The reference's own right-hand side and time step, on a chosen library handle
 # ------------------------------------------------------------------------------------
 

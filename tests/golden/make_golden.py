@@ -2,8 +2,8 @@
 
 Runs only where the reference sources exist (they are compiled in place by oracle/Makefile);
 the fixtures it writes are data -- inputs and the reference's outputs -- and are committed so
-the tests run without the reference.  Usage:  python tests/golden/make_golden.py [atree]
-("atree" rewrites only the adaptive-tree fixtures).
+the tests run without the reference.  Usage:  python tests/golden/make_golden.py [atree|gmres]
+("atree" / "gmres" rewrite only the adaptive-tree / GMRES fixtures).
 """
 from __future__ import annotations
 
@@ -159,9 +159,84 @@ def main_atree():
              eps7=np.float64(1e-16), counts=counts, **{k: t[k] for k in oracle.ATREE_TABLES})
 
 
+def gmres_dense_cases():
+    """Dense systems for DGMRES: well conditioned with restarts, non-normal, one solve
+    without restarts, and a hard one that stops at the restart limit (IERR = 1)."""
+    rng = np.random.default_rng(20260404)
+    out = []
+    for name, n, kind, maxl, nrmax in (("well50_maxl10", 50, "well", 10, 0), ("nonnormal80_maxl20", 80, "nn", 20, 0),
+                                        ("well60_maxl100_norestart", 60, "well", 100, -1),
+                                        ("hard200_maxl15_nrmax3", 200, "hard", 15, 3)):
+        if kind == "well":
+            a = np.eye(n) + 0.3 * rng.normal(size=(n, n)) / np.sqrt(n)
+        elif kind == "nn":
+            a = np.eye(n) + 0.2 * np.triu(rng.normal(size=(n, n)), 1) + 0.1 * rng.normal(size=(n, n)) / np.sqrt(n)
+        else:
+            a = np.diag(np.linspace(0.01, 1, n)) + 0.05 * rng.normal(size=(n, n)) / np.sqrt(n)
+        b = rng.normal(size=n)
+        x0 = 0.1 * rng.normal(size=n)
+        out.append((name, a, b, x0, maxl, nrmax))
+    return out
+
+
+def stokes_16drops():
+    """16 perturbed drops on a 4x4 grid (spacing 3), N per drop 128..512: above the direct-sum
+    size, so the FMM path is the operator."""
+    rng = np.random.default_rng(20260405)
+    zs, nds = [], []
+    for k in range(16):
+        n = int(rng.choice([128, 192, 256, 384, 512]))
+        c = 3.0 * (k % 4) + 3.0j * (k // 4)
+        modes = [(m, rng.uniform(-0.04, 0.04)) for m in (2, 3, 5)]
+        z, _, _, _ = perturbed_circle(n, modes, center=c, radius=rng.uniform(0.8, 1.2))
+        zs.append(z)
+        nds.append(n)
+    return np.array(nds, np.int32), np.concatenate(zs).view(np.float64).copy()
+
+
+def main_gmres():
+    """DGMRES of the reference on dense matrices, and SOLVEI's solve (DGMRES with the
+    reference MATVEC: FASMVP + CADAPDR) on two geometries."""
+    cases = gmres_dense_cases()
+    d = {}
+    for name, a, b, x0, maxl, nrmax in cases:
+        x, it, err, ierr = oracle.ref_dgmres_dense(a, b, x0, maxl, 1e-10, nrmax)
+        d.update({f"{name}_a": a, f"{name}_b": b, f"{name}_x0": x0, f"{name}_x": x,
+                  f"{name}_meta": np.array([maxl, nrmax, it, ierr], np.int32), f"{name}_err": np.float64(err)})
+    d["names"] = np.array([c[0] for c in cases])
+    save("gmres_dense", **d)
+    import tempfile
+    cwd = os.getcwd()
+    os.chdir(tempfile.mkdtemp())  # the reference writes fort.13
+    try:
+        sim = oracle.RefSim(oracle.LIBREF)
+        for tag in ("two_drops", "16drops"):
+            if tag == "two_drops":
+                t = np.load(os.path.join(HERE, "geometry_two_drops.npz"))
+                nd, rku = t["nd"], t["rku"]
+            else:
+                nd, rku = stokes_16drops()
+            ex = oracle.ref_extractdat(nd, rku)
+            th = np.concatenate([2 * np.pi * np.arange(m) / m for m in nd])
+            u_true = np.stack([np.cos(th) + 0.3 * np.sin(2 * th), 0.5 * np.sin(th) - 0.2 * np.cos(3 * th)], 1).ravel()
+            sim.setup(nd)
+            rhs = oracle.ref_fasmvp(nd, ex["x"], ex["y"], ex["dz"], ex["dsda"], ex["rkappa"], ex["zk"], u_true)[0]
+            z = ex["x"] + 1j * ex["y"]
+            x, it, err, ierr = oracle.ref_dgmres_stokes(nd, z, ex["dz"], ex["dsda"], ex["rkappa"], ex["zk"], rhs)
+            _, _, za, pk = oracle.ref_fasmvp(nd, ex["x"], ex["y"], ex["dz"], ex["dsda"], ex["rkappa"], ex["zk"], x)
+            print(tag, "reference GMRES iterations", it, "err", err, "ierr", ierr)
+            save("gmres_stokes_" + tag, nd=nd, rku=rku, rhs=rhs, u_true=u_true, x=x,
+                 meta=np.array([it, ierr], np.int32), err=np.float64(err), za=za, pk=pk)
+    finally:
+        os.chdir(cwd)
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["atree"]:
         main_atree()
+    elif sys.argv[1:] == ["gmres"]:
+        main_gmres()
     else:
         main()
         main_atree()
+        main_gmres()
