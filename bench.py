@@ -82,6 +82,62 @@ def cpu_baseline(z, dz, nd, h, dens, levmax, p, ncurves_total, budget_s=20.0):
                       f"({npts // m} curves) of the same workload, {dt:.2f} s on 1 host core"}
 
 
+def stokes_solve_leg(torch, dev, with_cpu):
+    """The Stokes solve of one F evaluation (SOLVEI, stokesbubb.f:294-415) on the many-drop
+    configuration (64 drops, about 28K points): DGMRES with MAXL 100 and TOL 1e-10 on the GPU
+    matvec (FMM with the reference's LEVMAX 7 and p = 43).  The right-hand side is A u for a
+    smooth density u (the SETVELF right-hand side is not part of this build)."""
+    from electricbubbles_amd import gmres, spectral
+    from electricbubbles_amd.workloads import many_drops_cfg
+    nd, rku = many_drops_cfg(64, np.random.default_rng(20260506))
+    B = spectral.Bodies(nd)
+    geo = B.extractdat(torch.tensor(rku, device=dev))
+    mv = spectral.MatVec(B, geo["x"], geo["y"], geo["dz"], geo["dsda"], geo["rkappa"], geo["zk"], levmax=7,
+                         nterms=43, mode=0)
+    th = np.concatenate([2 * np.pi * np.arange(m) / m for m in nd])
+    u_true = torch.tensor(np.stack([np.cos(th) + 0.3 * np.sin(2 * th), 0.5 * np.sin(th)], 1).ravel(), device=dev)
+    rhs = mv.apply(u_true)
+    solver = gmres.GMRES(rhs.numel(), 100)
+    gmres.stokes_solve(mv, solver, rhs)  # warm-up
+    torch.cuda.synchronize()
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        soln, za, pk, it, err, ierr = gmres.stokes_solve(mv, solver, rhs)
+    torch.cuda.synchronize()
+    t_solve = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(10):
+        mv.apply(soln)
+    torch.cuda.synchronize()
+    t_mv = (time.perf_counter() - t0) / 10
+    res = float(torch.linalg.norm(mv.apply(soln) - rhs) / torch.linalg.norm(rhs))
+    out = {"workload": f"many-drop configuration: 64 drops, {int(nd.sum())} points, {2 * int(nd.sum())} unknowns",
+           "levmax": 7, "nterms": 43, "maxl": 100, "tol": 1e-10, "iterations": it, "ierr": ierr,
+           "ms_per_solve": round(t_solve * 1e3, 3), "solves_per_s": round(1.0 / t_solve, 3),
+           "ms_per_matvec": round(t_mv * 1e3, 4), "rel_residual": res}
+    if with_cpu:
+        import tempfile
+        import oracle
+        if oracle.ref_available():
+            g = {k: geo[k].cpu().numpy() for k in ("x", "y", "dz", "dsda", "rkappa", "zk")}
+            cwd = os.getcwd()
+            os.chdir(tempfile.mkdtemp())
+            try:
+                t0 = time.perf_counter()
+                xr, itr, er, ierr_r = oracle.ref_dgmres_stokes(nd, g["x"] + 1j * g["y"], g["dz"], g["dsda"],
+                                                               g["rkappa"], g["zk"], rhs.cpu().numpy())
+                t_ref = time.perf_counter() - t0
+            finally:
+                os.chdir(cwd)
+            x = soln.cpu().numpy()
+            out["cpu_reference"] = {"s_per_solve": round(t_ref, 3), "iterations": itr, "ierr": ierr_r, "cores": 1,
+                                    "kind": "reference",
+                                    "solution_rel_diff": float(np.linalg.norm(x - xr) / np.linalg.norm(xr))}
+    solver.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,6 +149,7 @@ def main():
     ap.add_argument("--npts", type=int, default=1 << 20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-check", action="store_true", help="skip the accuracy spot check")
+    ap.add_argument("--no-solve", action="store_true", help="skip the Stokes GMRES solve leg")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 256 MiB device copy first (the known-size kernel that scales HBM counters)")
     args = ap.parse_args()
@@ -194,6 +251,8 @@ def main():
         got = out.cpu().numpy()[tgt]
         check = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
 
+    solve = None if args.no_solve else stokes_solve_leg(torch, dev, not args.no_cpu)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(z, dz, nd, h, dens, L, p, max(ncurves, 1) if ncurves else 4096)
@@ -219,6 +278,7 @@ def main():
         "apply_only": {"value": round(n / t_apply / 1e6, 3), "unit": "Mpts/s", "ms": round(t_apply * 1e3, 4)},
         "stage_ms": {k: round(v, 4) for k, v in stage.items()},
         "adaptive_tree": at_info,
+        "stokes_solve": solve,
         "fp64_fraction_total": round(total_fl / (t_apply) / 1e12 / PEAK_FP64_TFLOPS, 4),
         "roofline": {"bound": "mfma", "kernel": dom, "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": traffic,

@@ -51,3 +51,23 @@ def complex_normal(n, rng):
 def weights(dzdt, nd, h):
     """Per-point DZDT * H(body), the quadrature-weighted tangent of the plan API."""
     return dzdt * np.repeat(np.asarray(h, np.float64), np.asarray(nd))
+
+
+def many_drops_cfg(ndrops, rng, box=20.0, gap=0.5):
+    """The many-drop configuration: centres U[-box, box]^2 with rejection for a surface gap of
+    at least `gap`, radii U[0.5, 1.5], three random Fourier shape modes of amplitude <= 0.05,
+    N per drop = 2^round(log2(512 r)) clipped to 128..1024.  Returns (nd, rku): points per
+    drop and the interleaved (x, y) state of all points, drop after drop."""
+    zs, nds, placed = [], [], []
+    while len(zs) < ndrops:
+        r = rng.uniform(0.5, 1.5)
+        c = rng.uniform(-box, box) + 1j * rng.uniform(-box, box)
+        if any(abs(c - c2) < r + r2 + gap for c2, r2 in placed):
+            continue
+        placed.append((c, r))
+        n = int(min(1024, max(128, 2 ** round(np.log2(512 * r)))))
+        modes = [(int(m), rng.uniform(-0.05, 0.05)) for m in rng.choice(np.arange(2, 8), 3, replace=False)]
+        z, _, _, _ = perturbed_circle(n, modes, center=c, radius=r)
+        zs.append(z)
+        nds.append(n)
+    return np.array(nds, np.int32), np.concatenate(zs).view(np.float64).copy()

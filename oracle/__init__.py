@@ -662,7 +662,11 @@ def ref_dgmres_stokes(nd, z, dz, dsdth, rkappa, zk, rhs, maxl=100, tol=1e-10, nr
     x = b.copy()
     mv = ctypes.cast(r.matvec_, ctypes.c_void_p)
     ms = ctypes.cast(r.msolve_, ctypes.c_void_p)
-    return _dgmres_call(r, 2 * nbk, b, x, mv, ms, maxl, tol, nrmax)
+    try:
+        return _dgmres_call(r, 2 * nbk, b, x, mv, ms, maxl, tol, nrmax)
+    finally:
+        # FASMVP leaves PRINI(6, 13) set (stokesbubb.f:572): silence the reference's printing again
+        r.prini_(_c_int(0), _c_int(0))
 
 
 # ------------------------------------------------------------------------------------

@@ -6,7 +6,7 @@ import os
 import numpy as np
 import pytest
 
-from electricbubbles_amd.workloads import perturbed_circle
+from electricbubbles_amd.workloads import many_drops_cfg
 from tests.conftest import GOLDEN
 
 pytestmark = pytest.mark.gpu
@@ -82,21 +82,7 @@ def test_stokes_solve_matches_reference(torch, mods, tag):
 def test_stokes_solve_64_drops(torch, mods):
     """The many-drop shape (64 drops of 128..1024 points, about 32K unknowns pairs) on the FMM."""
     gm, sp = mods
-    rng = np.random.default_rng(640)
-    zs, nds, centers = [], [], []
-    while len(zs) < 64:
-        r = rng.uniform(0.5, 1.5)
-        c = rng.uniform(-20, 20) + 1j * rng.uniform(-20, 20)
-        if any(abs(c - c2) < r + r2 + 0.5 for c2, r2 in centers):
-            continue
-        centers.append((c, r))
-        n = int(min(1024, max(128, 2 ** round(np.log2(512 * r)))))
-        modes = [(m, rng.uniform(-0.05, 0.05)) for m in rng.choice(np.arange(2, 8), 3, replace=False)]
-        z, _, _, _ = perturbed_circle(n, modes, center=c, radius=r)
-        zs.append(z)
-        nds.append(n)
-    nd = np.array(nds, np.int32)
-    rku = np.concatenate(zs).view(np.float64).copy()
+    nd, rku = many_drops_cfg(64, np.random.default_rng(640))
     print("stokes solve 64 drops, points", int(nd.sum()), flush=True)
     B, geo, mv = _stokes(torch, sp, nd, rku)
     th = np.concatenate([2 * np.pi * np.arange(m) / m for m in nd])
