@@ -29,6 +29,13 @@ __device__ __forceinline__ void rfma(double2& acc, double s, double2 a) {
 
 // 1/a for a > 0: hardware reciprocal estimate refined by two Newton-Raphson steps
 // y <- y (2 - a y), written as e = 1 - a y ; y = y + y e.
+// 1/a from v_rcp_f64 and one Newton step (relative error below 1e-14; the near field's
+// contribution to GW needs no more)
+__device__ __forceinline__ double rcp_nr1(double a) {
+  const double y = __builtin_amdgcn_rcp(a);
+  return fma(y, fma(-a, y, 1.0), y);
+}
+
 __device__ __forceinline__ double rcp_nr(double a) {
   double y = __builtin_amdgcn_rcp(a);
   double e = fma(-a, y, 1.0);

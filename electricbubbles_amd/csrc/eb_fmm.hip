@@ -484,14 +484,21 @@ __global__ __launch_bounds__(256) void k_l2p(int n, int L, const uint32_t* __res
 // One source-target pair of the near field, in the translation-invariant form
 //   qa/d + sgn (conj(qb)/conj(d) - conj(qa) d/conj(d)^2),  d = z_src - z_tgt
 // (SLDIRDR / SLSELFDR, cadapdrop2.f:586-599, 741-767, with QC = -conj(z) QA folded in).
-// acc1 collects qa/d, acc2 the bracket.  `self` (source == target) contributes exactly zero:
-// the reciprocal is forced to 0, which zeroes both terms without a branch.
+// acc1 collects qa/d, acc2 the bracket.  With MASK, `self` (source == target) contributes
+// exactly zero: the reciprocal is forced to 0, which zeroes both terms without a branch.
+// FAST uses one Newton step on the hardware reciprocal, the exact variant two.
+template <bool MASK, bool FAST>
 __device__ __forceinline__ void p2p_pair(double2 zi, double2 zj, double2 qa, double2 qb, bool self, double2& acc1,
                                          double2& acc2) {
   const double dx = zj.x - zi.x, dy = zj.y - zi.y;
   const double r2 = fma(dx, dx, dy * dy);
-  double ir = rcp_nr(self ? 1.0 : r2);
-  ir = self ? 0.0 : ir;
+  double ir;
+  if (MASK) {
+    ir = FAST ? rcp_nr1(self ? 1.0 : r2) : rcp_nr(self ? 1.0 : r2);
+    ir = self ? 0.0 : ir;
+  } else {
+    ir = FAST ? rcp_nr1(r2) : rcp_nr(r2);
+  }
   const double wx = dx * ir, wy = dy * ir;  // w = 1/conj(d)
   // acc1 += qa * conj(w)
   acc1.x = fma(qa.x, wx, fma(qa.y, wy, acc1.x));
@@ -576,12 +583,28 @@ __global__ __launch_bounds__(64) void k_p2p(int L, const int* __restrict__ leaf_
       }
       __syncthreads();
       if (act) {
+        // chunk positions [sa0, sb0) hold the leaf's own points: only there can a source be
+        // the target itself, so only that range runs the masked pair
         const int fs0 = fself0 - c0, fs1 = fself1 - c0;
+        const int sa0 = min(max(self0 - c0, 0), n), sb0 = min(max(self0 + T - c0, 0), n);
+        auto first = [&](int lo) { return lo + ((gs - lo) % G + G) % G; };  // first f >= lo in this slice
 #pragma unroll 2
-        for (int f = gs; f < n; f += G) {
+        for (int f = first(0); f < sa0; f += G) {
           const double2 zj = sz[f], qj = sa[f], bj = sq[f];
-          p2p_pair(z0, zj, qj, bj, f == fs0, a10, a20);
-          p2p_pair(z1, zj, qj, bj, f == fs1, a11, a21);
+          p2p_pair<false, true>(z0, zj, qj, bj, false, a10, a20);
+          p2p_pair<false, true>(z1, zj, qj, bj, false, a11, a21);
+        }
+#pragma unroll 2
+        for (int f = first(sa0); f < sb0; f += G) {
+          const double2 zj = sz[f], qj = sa[f], bj = sq[f];
+          p2p_pair<true, true>(z0, zj, qj, bj, f == fs0, a10, a20);
+          p2p_pair<true, true>(z1, zj, qj, bj, f == fs1, a11, a21);
+        }
+#pragma unroll 2
+        for (int f = first(sb0); f < n; f += G) {
+          const double2 zj = sz[f], qj = sa[f], bj = sq[f];
+          p2p_pair<false, true>(z0, zj, qj, bj, false, a10, a20);
+          p2p_pair<false, true>(z1, zj, qj, bj, false, a11, a21);
         }
       }
     }
@@ -623,7 +646,7 @@ __global__ __launch_bounds__(256) void k_direct(int n, const double2* __restrict
     const int m = min(256, n - t0);
     if (i < n)
       for (int jj = 0; jj < m; ++jj) {
-        p2p_pair(zi, sz[jj], sa[jj], sbq[jj], t0 + jj == i, acc1, acc2);
+        p2p_pair<true, false>(zi, sz[jj], sa[jj], sbq[jj], t0 + jj == i, acc1, acc2);
       }
     __syncthreads();
   }
