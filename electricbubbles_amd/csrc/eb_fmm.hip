@@ -226,7 +226,7 @@ __global__ void k_charges(int n, const int32_t* __restrict__ perm, const double2
 
 // P2M at the finest level (SLMKXP, cadap2.f:444-466), one 64-lane block per leaf box.
 // A chunk of up to 16 sources writes its powers t^m (t = (z-c)/r) into LDS, four lanes per
-// source each covering a quarter of the exponents; lane k then forms
+// source each covering a quarter of the exponents; output (series, k) then forms
 //   alpha_{k+1} = -(1/r) sum qa t^k,  beta_{k+1} = -(1/r) sum qb t^k + k (1/r^2) sum qc t^(k-1).
 template <int KMAX>
 __global__ __launch_bounds__(64) void k_p2m(int L, int p, const int* __restrict__ leaf_start,
@@ -234,8 +234,9 @@ __global__ __launch_bounds__(64) void k_p2m(int L, int p, const int* __restrict_
                                             const double2* __restrict__ qb, double2* __restrict__ mp, int W,
                                             int NBP) {
   constexpr int CH = 16, QK = KMAX / 4;
+  constexpr int NO = (2 * KMAX + 63) / 64;  // outputs per lane: (series, k) = t / KMAX, t % KMAX
   __shared__ double2 P[CH][KMAX + 1];
-  __shared__ double2 A[CH], B[CH], E[CH];
+  __shared__ double2 AB[2][CH], E[CH];
   const int nside = 1 << L;
   const int ix = blockIdx.x % nside, iy = blockIdx.x / nside;
   const uint32_t mk = morton((uint32_t)ix, (uint32_t)iy);
@@ -244,7 +245,9 @@ __global__ __launch_bounds__(64) void k_p2m(int L, int p, const int* __restrict_
   const double2 c = make_double2((ix + 0.5) * h, (iy + 0.5) * h);
   const int lane = threadIdx.x;
   const int js = lane & (CH - 1), qk = lane / CH;
-  double2 a1 = make_double2(0, 0), a2 = make_double2(0, 0);
+  double2 acc[NO];
+#pragma unroll
+  for (int o = 0; o < NO; ++o) acc[o] = make_double2(0, 0);
   for (int c0 = s; c0 < e; c0 += CH) {
     const int nj = min(CH, e - c0);
     if (js < nj) {
@@ -253,8 +256,8 @@ __global__ __launch_bounds__(64) void k_p2m(int L, int p, const int* __restrict_
       const double2 t = make_double2((z.x - c.x) * ir, (z.y - c.y) * ir);
       if (qk == 0) {
         const double2 q1 = qa[j], q2 = qb[j];
-        A[js] = make_double2(-q1.x * ir, -q1.y * ir);
-        B[js] = make_double2(-q2.x * ir, -q2.y * ir);
+        AB[0][js] = make_double2(-q1.x * ir, -q1.y * ir);
+        AB[1][js] = make_double2(-q2.x * ir, -q2.y * ir);
         const double2 qc = cmul(make_double2(-z.x, z.y), q1);  // -conj(z) qa
         E[js] = make_double2(qc.x * ir * ir, qc.y * ir * ir);
       }
@@ -271,25 +274,27 @@ __global__ __launch_bounds__(64) void k_p2m(int L, int p, const int* __restrict_
       }
     }
     __syncthreads();
-    if (lane < KMAX) {
-      const double fk = (double)lane;
-      for (int jj = 0; jj < nj; ++jj) {
-        const double2 pw = P[jj][lane];
-        cfma(a1, A[jj], pw);
-        cfma(a2, B[jj], pw);
-        if (lane > 0) rfma(a2, fk, cmul(E[jj], P[jj][lane - 1]));
+    // every lane runs the same operations: series 0 takes its factor 0 on the QC term, so
+    // the two halves of the wave do not diverge
+#pragma unroll
+    for (int o = 0; o < NO; ++o) {
+      const int t = lane + 64 * o;
+      if (t < 2 * KMAX) {
+        const int sr = t / KMAX, k = t % KMAX, km1 = k > 0 ? k - 1 : 0;
+        const double fk = sr ? (double)k : 0.0;
+        for (int jj = 0; jj < nj; ++jj) {
+          cfma(acc[o], AB[sr][jj], P[jj][k]);
+          rfma(acc[o], fk, cmul(E[jj], P[jj][km1]));
+        }
       }
     }
     __syncthreads();
   }
-  if (lane < KMAX) {
-    if (lane >= p) {
-      a1 = make_double2(0, 0);
-      a2 = make_double2(0, 0);
-    }
-    const int pb = pidx(ix, iy, W);
-    mp[(size_t)lane * NBP + pb] = a1;
-    mp[(size_t)(KMAX + lane) * NBP + pb] = a2;
+  const int pb = pidx(ix, iy, W);
+#pragma unroll
+  for (int o = 0; o < NO; ++o) {
+    const int t = lane + 64 * o;
+    if (t < 2 * KMAX) mp[(size_t)t * NBP + pb] = (t % KMAX) < p ? acc[o] : make_double2(0, 0);
   }
 }
 
@@ -514,6 +519,27 @@ __device__ __forceinline__ void p2p_pair(double2 zi, double2 zj, double2 qa, dou
   acc2.y = fma(vx, wy, fma(vy, wx, acc2.y));
 }
 
+// Near-field work items: a leaf with T points becomes ceil(T / P2P_TMAX) items of nearly
+// equal target ranges (all with the leaf's whole 3x3 neighbourhood as sources), so that the
+// lanes of a block stay busy on large leaves.  item = (Morton key, first target, end target).
+constexpr int P2P_TMAX = 42;
+
+__global__ void k_p2p_count(int nleaf, const int* __restrict__ leaf_start, int* __restrict__ cnt) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= nleaf) return;
+  const int T = leaf_start[m + 1] - leaf_start[m];
+  cnt[m] = (T + P2P_TMAX - 1) / P2P_TMAX;
+}
+
+__global__ void k_p2p_items(int nleaf, const int* __restrict__ leaf_start, const int* __restrict__ off,
+                            int4* __restrict__ items) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= nleaf) return;
+  const int T = leaf_start[m + 1] - leaf_start[m];
+  const int o = off[m], np = off[m + 1] - o;
+  for (int q = 0; q < np; ++q) items[o + q] = make_int4(m, (int)((long)T * q / np), (int)((long)T * (q + 1) / np), 0);
+}
+
 // Near field at the finest level (DONN0DR, cadapdrop2.f:623-722) plus the final scaling
 // GW *= RSCAL (cadapdrop.f:370-373) and the scatter back to the caller's order.
 // One 64-lane block per target leaf.  The sources of the 3x3 neighbourhood (MKNBOR,
@@ -522,6 +548,7 @@ __device__ __forceinline__ void p2p_pair(double2 zi, double2 zj, double2 qa, dou
 constexpr int P2P_CH = 128;  // 6 KB of LDS per block: LDS does not limit the waves per CU
 
 __global__ __launch_bounds__(64) void k_p2p(int L, const int* __restrict__ leaf_start,
+                                            const int4* __restrict__ items, const int* __restrict__ nitems,
                                             const double2* __restrict__ zs, const double2* __restrict__ qa,
                                             const double2* __restrict__ qb, const double2* __restrict__ gwf,
                                             const int32_t* __restrict__ perm, const Geom* __restrict__ g,
@@ -530,12 +557,14 @@ __global__ __launch_bounds__(64) void k_p2p(int L, const int* __restrict__ leaf_
   double2* const red = sz;  // the per-slice partial sums reuse the staging buffers
   double2* const red2 = sa;
   static_assert(P2P_CH >= 64, "reduction buffers alias the staging buffers");
+  if ((int)blockIdx.x >= *nitems) return;
   const int nside = 1 << L;
-  const int ix = blockIdx.x % nside, iy = blockIdx.x / nside;
-  const uint32_t mk = morton((uint32_t)ix, (uint32_t)iy);
-  const int ts = leaf_start[mk], te = leaf_start[mk + 1];
+  const int4 item = items[blockIdx.x];  // (leaf Morton key, first and end target within the leaf)
+  const uint32_t mk = (uint32_t)item.x;
+  const int ix = (int)compact2(mk), iy = (int)compact2(mk >> 1);
+  const int ls = leaf_start[mk], TL = leaf_start[mk + 1] - ls;  // the whole leaf: its own sources
+  const int ts = ls + item.y, te = ls + item.z;                  // this block's targets
   const int T = te - ts;
-  if (T == 0) return;
   const int lane = threadIdx.x;
   int sb[9], cum[10];
   int nb = 0, self0 = 0;
@@ -565,7 +594,7 @@ __global__ __launch_bounds__(64) void k_p2p(int L, const int* __restrict__ leaf_
     const bool has1 = act && i1 < te;
     const double2 z0 = act ? zs[i0] : make_double2(0, 0);
     const double2 z1 = has1 ? zs[i1] : z0;
-    const int fself0 = self0 + t0 + 2 * tp;   // flat index of point i0 among the sources
+    const int fself0 = self0 + (ts - ls) + t0 + 2 * tp;  // flat index of point i0 among the sources
     const int fself1 = has1 ? fself0 + 1 : -1;
     double2 a10 = make_double2(0, 0), a20 = make_double2(0, 0);
     double2 a11 = make_double2(0, 0), a21 = make_double2(0, 0);
@@ -586,7 +615,7 @@ __global__ __launch_bounds__(64) void k_p2p(int L, const int* __restrict__ leaf_
         // chunk positions [sa0, sb0) hold the leaf's own points: only there can a source be
         // the target itself, so only that range runs the masked pair
         const int fs0 = fself0 - c0, fs1 = fself1 - c0;
-        const int sa0 = min(max(self0 - c0, 0), n), sb0 = min(max(self0 + T - c0, 0), n);
+        const int sa0 = min(max(self0 - c0, 0), n), sb0 = min(max(self0 + TL - c0, 0), n);
         auto first = [&](int lo) { return lo + ((gs - lo) % G + G) % G; };  // first f >= lo in this slice
 #pragma unroll 2
         for (int f = first(0); f < sa0; f += G) {
@@ -769,6 +798,10 @@ struct eb_fmm_plan_s {
   int32_t* vals2 = nullptr;
   int* sorttmp = nullptr;
   int* leaf_start = nullptr;
+  int4* p2p_items = nullptr;  // near-field work items and their count (after the scan total)
+  int* p2p_cnt = nullptr;
+  int* p2p_off = nullptr;
+  int* p2p_tmp = nullptr;
   int* ntbox = nullptr;
   double2* mp = nullptr;
   double2* lo = nullptr;
@@ -787,7 +820,8 @@ struct eb_fmm_plan_s {
 static void plan_free(eb_fmm_plan P) {
   if (!P) return;
   void* ptrs[] = {P->geom, P->bbpart, P->zs0, P->zs, P->wdzs, P->qa, P->qb, P->gwf, P->keys, P->keys2,
-                  P->perm, P->vals2, P->sorttmp, P->leaf_start, P->ntbox, P->mp, P->lo, P->dMat,
+                  P->perm, P->vals2, P->sorttmp, P->leaf_start, P->p2p_items, P->p2p_cnt, P->p2p_off,
+                  P->p2p_tmp, P->ntbox, P->mp, P->lo, P->dMat,
                   P->dU, P->dNU, P->dPM, P->dQM, P->dPL, P->dQL, P->doffs};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
@@ -826,6 +860,10 @@ static int pick_kmax(int p) {
 
 // Bin the points (tree build): bounding box and RSCAL, scaled coordinates and finest-level
 // keys, stable sort by key, leaf table, gather into sorted order, non-empty counts per level.
+// most near-field work items: one per non-empty leaf plus one per extra P2P_TMAX points
+// (sum over leaves of ceil(T/P2P_TMAX) <= leaves + sum (T-1)/P2P_TMAX <= leaves + n/P2P_TMAX)
+static size_t p2p_item_cap(int n, int L) { return ((size_t)1 << (2 * L)) + (size_t)n / P2P_TMAX + 1; }
+
 static int build_tree(eb_fmm_plan P, const double* x, const double* y, const double* wdz) {
   hipStream_t st = P->st;
   const int n = P->n, L = P->L;
@@ -842,6 +880,12 @@ static int build_tree(eb_fmm_plan P, const double* x, const double* y, const dou
                      P->leaf_start);
   hipLaunchKernelGGL(k_gather, dim3((n + 255) / 256), dim3(256), 0, st, n, P->perm, P->zs0,
                      (const double2*)wdz, P->zs, P->wdzs);
+  // near-field work items; their count stays on the device (p2p_off[nleaf])
+  hipLaunchKernelGGL(k_p2p_count, dim3((nleaf + 255) / 256), dim3(256), 0, st, nleaf, P->leaf_start, P->p2p_cnt);
+  rc = scan_exclusive(P->p2p_cnt, P->p2p_off, nleaf, true, P->p2p_tmp, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_p2p_items, dim3((nleaf + 255) / 256), dim3(256), 0, st, nleaf, P->leaf_start, P->p2p_off,
+                     P->p2p_items);
   EB_HIP_CHECK(hipGetLastError());
   return EB_OK;
 }
@@ -899,6 +943,10 @@ extern "C" int eb_fmm_plan_create(eb_fmm_plan* out, int32_t n, const double* x, 
   EB_TRY(dmalloc(&P->vals2, n));
   EB_TRY(dmalloc(&P->sorttmp, radix_sort_temp_ints(n)));
   EB_TRY(dmalloc(&P->leaf_start, nleaf + 1));
+  EB_TRY(dmalloc(&P->p2p_items, p2p_item_cap(n, L)));
+  EB_TRY(dmalloc(&P->p2p_cnt, nleaf));
+  EB_TRY(dmalloc(&P->p2p_off, nleaf + 1));
+  EB_TRY(dmalloc(&P->p2p_tmp, scan_temp_ints(nleaf)));
   EB_TRY(dmalloc(&P->ntbox, EB_MAXLEV + 2));
 
   EB_TRY(build_tree(P, x, y, wdz));
@@ -1050,8 +1098,9 @@ extern "C" int eb_fmm_apply(eb_fmm_plan P, const double* dens, double* gradw, vo
     } else if (P->timing) {
       for (int i = 1; i <= 6; ++i) (void)hipEventRecord(P->ev[i], st);
     }
-    hipLaunchKernelGGL(k_p2p, dim3(1 << (2 * P->L)), dim3(64), 0, st, P->L, P->leaf_start, P->zs, P->qa, P->qb,
-                       far ? P->gwf : nullptr, P->perm, P->geom, P->sgn, (double2*)gradw);
+    hipLaunchKernelGGL(k_p2p, dim3((unsigned)p2p_item_cap(P->n, P->L)), dim3(64), 0, st, P->L, P->leaf_start,
+                       P->p2p_items, P->p2p_off + (1 << (2 * P->L)), P->zs, P->qa, P->qb, far ? P->gwf : nullptr,
+                       P->perm, P->geom, P->sgn, (double2*)gradw);
   }
   EB_HIP_CHECK(hipGetLastError());
   if (P->timing) {
