@@ -457,10 +457,15 @@ __global__ __launch_bounds__(256) void k_xlat(XlatLevels LS, const int* __restri
 
 // L2P at the finest level (SLLVALDR, cadapdrop2.f:331-348): one thread per point.
 //   GW_far = phi + sgn (z conj(phi') + conj(psi))
-template <int KMAX>
+// With PARENT the leaf's own expansion holds only its interaction-list terms, and the
+// parent's complete local expansion is evaluated at the point as well: the same sum as a
+// level-L L2L followed by one evaluation (the L2L shift is exact), without writing the
+// shifted expansions of every leaf.
+template <int KMAX, bool PARENT>
 __global__ __launch_bounds__(256) void k_l2p(int n, int L, const uint32_t* __restrict__ keys,
                                              const double2* __restrict__ zs, const double2* __restrict__ lo, int W,
-                                             int NBP, double sgn, double2* __restrict__ gwf) {
+                                             int NBP, const double2* __restrict__ lop, int Wp, int NBPp, double sgn,
+                                             double2* __restrict__ gwf) {
   const int i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint32_t key = keys[i];
@@ -479,7 +484,25 @@ __global__ __launch_bounds__(256) void k_l2p(int n, int L, const uint32_t* __res
     phi = cadd(cmul(phi, u), l1);
     psi = cadd(cmul(psi, u), l2);
   }
-  const double2 phip = cscale(dphi, ir);
+  double2 phip = cscale(dphi, ir);
+  if (PARENT) {
+    const int px = ix >> 1, py = iy >> 1;
+    const double hp = 2.0 * h, irp = 2.0 / hp;
+    const double2 up = make_double2((z.x - (px + 0.5) * hp) * irp, (z.y - (py + 0.5) * hp) * irp);
+    const int pp = pidx(px, py, Wp);
+    double2 f = make_double2(0, 0), df = make_double2(0, 0), g = make_double2(0, 0);
+#pragma unroll 4
+    for (int m = KMAX - 1; m >= 0; --m) {
+      const double2 l1 = lop[(size_t)m * NBPp + pp];
+      const double2 l2 = lop[(size_t)(KMAX + m) * NBPp + pp];
+      df = cadd(cmul(df, up), f);
+      f = cadd(cmul(f, up), l1);
+      g = cadd(cmul(g, up), l2);
+    }
+    phi = cadd(phi, f);
+    psi = cadd(psi, g);
+    phip = cadd(phip, cscale(df, irp));
+  }
   // z conj(phi') + conj(psi)
   const double2 zc = cmul(z, cconj(phip));
   const double2 t = make_double2(zc.x + psi.x, zc.y - psi.y);
@@ -1058,11 +1081,18 @@ static void launch_far(eb_fmm_plan P) {
   // interaction lists of every level in one launch, then the downward pass
   launch_xlat<XL_M2L, K>(P, xlat_levels(P, 2, L, 0), P->mp, P->lo, P->dU, P->dNU);
   if (P->timing) (void)hipEventRecord(P->ev[4], st);
-  for (int l = 3; l <= L; ++l)
+  // the shift into the leaves is left to L2P, which evaluates the parents' expansions directly
+  for (int l = 3; l <= L - 1; ++l)
     launch_xlat<XL_L2L, K>(P, xlat_levels(P, l, l, -1), P->lo, P->lo, P->dPL, P->dQL);
   if (P->timing) (void)hipEventRecord(P->ev[5], st);
-  hipLaunchKernelGGL(k_l2p<K>, dim3(xcd_grid((P->n + 255) / 256)), dim3(256), 0, st, P->n, L, P->keys, P->zs,
-                     P->lo + P->lvl_off[L], P->lvl_W[L], P->lvl_NBP[L], P->sgn, P->gwf);
+  if (L >= 3)
+    hipLaunchKernelGGL((k_l2p<K, true>), dim3(xcd_grid((P->n + 255) / 256)), dim3(256), 0, st, P->n, L, P->keys,
+                       P->zs, P->lo + P->lvl_off[L], P->lvl_W[L], P->lvl_NBP[L], P->lo + P->lvl_off[L - 1],
+                       P->lvl_W[L - 1], P->lvl_NBP[L - 1], P->sgn, P->gwf);
+  else
+    hipLaunchKernelGGL((k_l2p<K, false>), dim3(xcd_grid((P->n + 255) / 256)), dim3(256), 0, st, P->n, L, P->keys,
+                       P->zs, P->lo + P->lvl_off[L], P->lvl_W[L], P->lvl_NBP[L], P->lo + P->lvl_off[L],
+                       P->lvl_W[L], P->lvl_NBP[L], P->sgn, P->gwf);
   if (P->timing) (void)hipEventRecord(P->ev[6], st);
 }
 
