@@ -347,7 +347,7 @@ __global__ __launch_bounds__(256) void k_xlat(XlatLevels LS, const int* __restri
   constexpr int NS = MODE == XL_M2L ? 27 : (MODE == XL_M2M ? 4 : 1);
   constexpr int NK = KS * 4, NM = MT * 16, CS = NM + 4;
   __shared__ double Cs[NK * CS];
-  __shared__ double2 Ps[NS * NK];
+  __shared__ double2 Ps[NS * NK * 2];  // per (source, k, component): the weights of this lane's column
   __shared__ double2 Qs[NS * NM];
   // XCD-aware order: consecutive logical blocks (neighbouring rows, all four parity classes)
   // are dispatched to the same XCD and share its L2.
@@ -358,10 +358,14 @@ __global__ __launch_bounds__(256) void k_xlat(XlatLevels LS, const int* __restri
     const int k = i / NM, m = i % NM;
     Cs[k * CS + m] = Mat[k * KMAX + m];
   }
+  // a lane of component c forms c ? Im(w a) : Re(w a) = W.x a.x + W.y a.y with
+  // W = c ? (w.y, w.x) : (w.x, -w.y): the component choice sits in the table, not in selects
   for (int i = threadIdx.x; i < NS * NK; i += blockDim.x) {
     const int o = i / NK, k = i % NK;
     const int row = MODE == XL_M2L ? cls * 27 + o : (MODE == XL_M2M ? o : cls);
-    Ps[i] = pre[(size_t)row * KMAX + k];
+    const double2 w = pre[(size_t)row * KMAX + k];
+    Ps[2 * i] = make_double2(w.x, -w.y);
+    Ps[2 * i + 1] = make_double2(w.y, w.x);
   }
   for (int i = threadIdx.x; i < NS * NM; i += blockDim.x) {
     const int o = i / NM, m = i % NM;
@@ -389,39 +393,46 @@ __global__ __launch_bounds__(256) void k_xlat(XlatLevels LS, const int* __restri
     live = be > bs;
   }
   const int pt = pidx(ix, iy, Wt);
-  const double2* srcs = src + LS.offs[e] + (size_t)s * KMAX * NBPs;
   auto src_box = [&](int o) -> int {
     if (MODE == XL_M2L) return pt + offs[(cls * 27 + o) * 2 + 1] * Ws + offs[(cls * 27 + o) * 2 + 0];
     if (MODE == XL_M2M) return pidx(2 * ix + (o & 1), 2 * iy + (o >> 1), Ws);
     return pidx(ix >> 1, iy >> 1, Ws);
   };
-  double2 an[KS];
-  {
-    const int ps = src_box(0);
+  // Source expansions through a buffer resource covering both series of the level: the
+  // per-k row step is a scalar offset (no per-lane address arithmetic per load), and an
+  // offset outside the level reads 0 instead of faulting.
+  const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+      (void*)(src + LS.offs[e]), (short)0, (int)(uint32_t)((size_t)2 * KMAX * NBPs * 16), 0x00020000);
+  const int row_bytes = 4 * NBPs * 16;  // k -> k + 4
+  const int lane_row = (s * KMAX + kr) * NBPs * 16;
+  auto load_src = [&](int o, double2* a) {
+    const int off = lane_row + src_box(o) * 16;
 #pragma unroll
-    for (int ks = 0; ks < KS; ++ks) an[ks] = srcs[(size_t)(4 * ks + kr) * NBPs + ps];
-  }
-  double lam[MT][4];
+    for (int ks = 0; ks < KS; ++ks)
+      a[ks] = __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(rsrc, off, ks * row_bytes, 0));
+  };
+  // out_m = Re/Im(w_m y_m) with y_m split over this lane (v) and its partner (v'): summed
+  // over the sources it is A -+ B' with A = sum w.x v and B = sum w.y v of the partner lane,
+  // so one lane swap after the last source replaces one per source.
+  double sa[MT][4], sb[MT][4];
 #pragma unroll
   for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
-    for (int r = 0; r < 4; ++r) lam[mt][r] = 0.0;
-#pragma unroll(MODE == XL_M2L ? 1 : NS)
-  for (int o = 0; o < NS; ++o) {
-    double2 a[KS];
-#pragma unroll
-    for (int ks = 0; ks < KS; ++ks) a[ks] = an[ks];
-    if (o + 1 < NS) {  // prefetch the next source while this one is multiplied
-      const int ps = src_box(o + 1);
-#pragma unroll
-      for (int ks = 0; ks < KS; ++ks) an[ks] = srcs[(size_t)(4 * ks + kr) * NBPs + ps];
-    }
+    for (int r = 0; r < 4; ++r) sa[mt][r] = sb[mt][r] = 0.0;
+  // One source: prescale the loaded expansion (cur), start loading the next source into the
+  // other register set (nxt; in flight during the products), the Pascal product on the
+  // matrix cores, then the post-scaling.  The empty asm statement is a compiler barrier for
+  // memory that keeps the Pascal tile in LDS, re-read per source instead of held in
+  // 2*KS*MT registers.
+  auto step = [&](int o, const double2* cur, double2* nxt) {
+    __asm__ volatile("" ::: "memory");
     double xb[KS];
 #pragma unroll
     for (int ks = 0; ks < KS; ++ks) {
-      const double2 w = Ps[o * NK + 4 * ks + kr];
-      xb[ks] = comp ? fma(w.x, a[ks].y, w.y * a[ks].x) : fma(w.x, a[ks].x, -w.y * a[ks].y);
+      const double2 w = Ps[2 * (o * NK + 4 * ks + kr) + comp];
+      xb[ks] = fma(w.x, cur[ks].x, w.y * cur[ks].y);
     }
+    if (o + 1 < NS) load_src(o + 1, nxt);
     v4d acc[MT];
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt) acc[mt] = v4d{0.0, 0.0, 0.0, 0.0};
@@ -436,12 +447,28 @@ __global__ __launch_bounds__(256) void k_xlat(XlatLevels LS, const int* __restri
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const double2 w = Qs[o * NM + 16 * mt + kr + 4 * r];
-        const double v = acc[mt][r];
-        const double vo = swap_adjacent(v);
-        const double yre = comp ? vo : v, yim = comp ? v : vo;
-        lam[mt][r] += comp ? fma(w.x, yim, w.y * yre) : fma(w.x, yre, -w.y * yim);
+        sa[mt][r] = fma(w.x, acc[mt][r], sa[mt][r]);
+        sb[mt][r] = fma(w.y, acc[mt][r], sb[mt][r]);
       }
+  };
+  double2 ra[KS], rb[KS];  // two register sets in turn: the prefetch needs no copy
+  load_src(0, ra);
+  int o = 0;
+#pragma unroll 1
+  for (; o + 1 < NS; o += 2) {
+    step(o, ra, rb);
+    step(o + 1, rb, ra);
   }
+  if (o < NS) step(o, ra, rb);
+  const uint32_t sgn_hi = comp ? 0u : 0x80000000u;  // A - B' on component 0, A + B' on 1
+  double lam[MT][4];
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {  // every lane takes part in the swap
+      const double bo = swap_adjacent(sb[mt][r]);
+      lam[mt][r] = sa[mt][r] + __hiloint2double(__double2hiint(bo) ^ (int)sgn_hi, __double2loint(bo));
+    }
   if (live) {
     double* out = (double*)(dst + LS.offt[e]);
 #pragma unroll
