@@ -31,11 +31,11 @@ PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 
 # HBM bytes per launch of the dominant kernels, from two rocprofv3 --pmc passes (FETCH_SIZE,
-# WRITE_SIZE) of this script, scaled by the same counters on a 256 MiB copy (fetch x 1.99975,
+# WRITE_SIZE) of this script, scaled by the same counters on a 256 MiB copy (fetch x 1.99976,
 # write x 1.000); profiles/fmm_curves_n1048576_L8_p32_hbm_traffic.md has the commands and
 # the per-kernel table.  Keyed by (workload, points, levmax, nterms).
 PROFILED_TRAFFIC_BYTES = {
-    ("curves", 1 << 20, 8, 32): {"m2l": 199325250, "p2p": 227623403},
+    ("curves", 1 << 20, 8, 32): {"m2l": 208822278, "p2p": 417546006},
 }
 
 
@@ -138,6 +138,60 @@ def stokes_solve_leg(torch, dev, with_cpu):
     return out
 
 
+def spectral_leg(torch, dev, with_cpu):
+    """Per-body geometry and stage work of F for the batched-sweep shapes: 512 single drops
+    r = 1 + 0.05 cos 2 theta, N log-uniform in 128..2048 (even), all bodies in one launch per
+    stage.  Times EXTRACTDAT (one forward FFT and spectral derivatives per body), the F tail
+    (EQUIT + frame velocity + FFT, FILTER, inverse FFT), one forward DCFFTF of every body and
+    the two RK2 stage updates, each averaged over repeated calls.  Algorithmic bytes per point:
+    EXTRACTDAT reads 16 B and writes 104 B (x, y, dz, tha, dsda, rkappa, ckern); the F tail
+    reads 48 B and writes 16 B; DCFFTF reads and writes 16 B; an RK2 stage reads 3 and writes
+    2 vectors of 2 doubles."""
+    from electricbubbles_amd import spectral
+    from electricbubbles_amd.workloads import perturbed_circle
+    rng = np.random.default_rng(20260707)
+    nd = (np.round(np.exp(rng.uniform(np.log(128), np.log(2048), 512)) / 2) * 2).astype(np.int32)
+    rku = np.concatenate([perturbed_circle(int(n), [(2, 0.05)])[0] for n in nd]).view(np.float64).copy()
+    npts = int(nd.sum())
+    B = spectral.Bodies(nd)
+    r = torch.tensor(rku, device=dev)
+
+    def timed(fn, reps=20):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps
+
+    geo = B.extractdat(r)
+    un = torch.tensor(0.05 * rng.normal(size=npts), device=dev)
+    c = torch.tensor(rng.normal(size=npts) + 1j * rng.normal(size=npts), device=dev)
+    dy = torch.tensor(rng.normal(size=2 * npts), device=dev)
+    t_ex = timed(lambda: B.extractdat(r))
+    t_tail = timed(lambda: B.ftail(geo["tha"], un, geo["dz"], geo["dsda"]))
+    t_fft = timed(lambda: B.dcfft(c, -1))
+    phi0, y = spectral.rk2_stage1(1e-3, r, dy)
+    t_rk = timed(lambda: spectral.rk2_stage1(1e-3, r, dy))
+    gbs = lambda nbytes, t: round(nbytes / t / 1e9, 1)  # noqa: E731
+    out = {"workload": f"512 drops, N log-uniform 128..2048, {npts} points",
+           "extractdat_ms": round(t_ex * 1e3, 4), "extractdat_GBps": gbs(120 * npts, t_ex),
+           "ftail_ms": round(t_tail * 1e3, 4), "ftail_GBps": gbs(64 * npts, t_tail),
+           "dcfft_ms": round(t_fft * 1e3, 4), "dcfft_GBps": gbs(32 * npts, t_fft),
+           "rk2_stage_ms": round(t_rk * 1e3, 4), "rk2_stage_GBps": gbs(80 * npts, t_rk),
+           "hbm_peak_GBps": PEAK_HBM_GBS}
+    if with_cpu:
+        import oracle
+        if oracle.ref_available():
+            t0 = time.perf_counter()
+            oracle.ref_extractdat(nd, rku)
+            t_ref = time.perf_counter() - t0
+            out["cpu_reference_extractdat_ms"] = round(t_ref * 1e3, 3)
+            out["cpu_reference_cores"] = 1
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -150,6 +204,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-check", action="store_true", help="skip the accuracy spot check")
     ap.add_argument("--no-solve", action="store_true", help="skip the Stokes GMRES solve leg")
+    ap.add_argument("--no-spectral", action="store_true", help="skip the spectral geometry leg")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 256 MiB device copy first (the known-size kernel that scales HBM counters)")
     args = ap.parse_args()
@@ -252,6 +307,7 @@ def main():
         check = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
 
     solve = None if args.no_solve else stokes_solve_leg(torch, dev, not args.no_cpu)
+    spec = None if args.no_spectral else spectral_leg(torch, dev, not args.no_cpu)
 
     cpu = None
     if not args.no_cpu:
@@ -279,6 +335,7 @@ def main():
         "stage_ms": {k: round(v, 4) for k, v in stage.items()},
         "adaptive_tree": at_info,
         "stokes_solve": solve,
+        "spectral": spec,
         "fp64_fraction_total": round(total_fl / (t_apply) / 1e12 / PEAK_FP64_TFLOPS, 4),
         "roofline": {"bound": "mfma", "kernel": dom, "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": traffic,

@@ -91,17 +91,27 @@ __global__ __launch_bounds__(TPB) void k_at_minmax(const double* __restrict__ x,
 
 // RMINX .. RMAXY start at +-1.0E30 (a REAL*4 constant); RSCAL = BOXSIZ / max range * 0.99
 // with the REAL*4 0.99 widened to double; XONEW, YONEW are the box centre.
-__global__ void k_at_geom(const double4* __restrict__ part, int np, double* __restrict__ geom) {
+__global__ __launch_bounds__(TPB) void k_at_geom(const double4* __restrict__ part, int np, double* __restrict__ geom) {
 #pragma clang fp contract(off)
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  __shared__ double4 s[TPB];
   const double big = (double)1.0e30f;
-  double mnx = big, mxx = -big, mny = big, mxy = -big;
-  for (int i = 0; i < np; ++i) {
-    mnx = fmin(mnx, part[i].x);
-    mxx = fmax(mxx, part[i].y);
-    mny = fmin(mny, part[i].z);
-    mxy = fmax(mxy, part[i].w);
+  double4 a = make_double4(big, -big, big, -big);
+  for (int i = threadIdx.x; i < np; i += TPB) {
+    const double4 b = part[i];
+    a = make_double4(fmin(a.x, b.x), fmax(a.y, b.y), fmin(a.z, b.z), fmax(a.w, b.w));
   }
+  s[threadIdx.x] = a;
+  __syncthreads();
+  for (int off = TPB / 2; off > 0; off >>= 1) {  // min / max are exact: any order gives the same
+    if (threadIdx.x < off) {
+      const double4 b = s[threadIdx.x + off];
+      a = s[threadIdx.x];
+      s[threadIdx.x] = make_double4(fmin(a.x, b.x), fmax(a.y, b.y), fmin(a.z, b.z), fmax(a.w, b.w));
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  const double mnx = s[0].x, mxx = s[0].y, mny = s[0].z, mxy = s[0].w;
   const double rsx = mxx - mnx, rsy = mxy - mny;
   const double xo = (mxx + mnx) / 2.0, yo = (mxy + mny) / 2.0;
   const double rs = rsx >= rsy ? rsx : rsy;
@@ -138,24 +148,43 @@ __global__ void k_at_zero(int* __restrict__ a, int lo, int hi) {  // a[lo..hi] =
 }
 
 // Child quadrant: x < xp is west, y < yp is south; NW = nc, NE = nc+1, SW = nc+2, SE = nc+3.
+// The counts are aggregated over the wave first: at the coarse levels most points of a wave
+// fall into the same few boxes, and one atomic per distinct box replaces one per point.
 __global__ void k_at_assign(int n, const double* __restrict__ xs, const double* __restrict__ ys,
                             int* __restrict__ ibox, const double* __restrict__ xb, const double* __restrict__ yb,
                             int* __restrict__ nbat, const int* __restrict__ nchi, int napb) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int ib = ibox[i];
-  double xp = 0.0, yp = 0.0;
-  int nc = 1;
-  if (ib != 0) {
-    if (nbat[ib - 1] <= napb) return;
-    xp = xb[ib - 1];
-    yp = yb[ib - 1];
-    nc = nchi[ib - 1];
+  int key = -1;  // child box (1-based) this point moves to, or -1
+  if (i < n) {
+    const int ib = ibox[i];
+    double xp = 0.0, yp = 0.0;
+    int nc = 1;
+    bool move = true;
+    if (ib != 0) {
+      move = nbat[ib - 1] > napb;
+      if (move) {
+        xp = xb[ib - 1];
+        yp = yb[ib - 1];
+        nc = nchi[ib - 1];
+      }
+    }
+    if (move) {
+      const double x = xs[i], y = ys[i];
+      const int q = (x < xp) ? ((y < yp) ? 2 : 0) : ((y < yp) ? 3 : 1);
+      key = nc + q;
+      ibox[i] = key;
+    }
   }
-  const double x = xs[i], y = ys[i];
-  const int q = (x < xp) ? ((y < yp) ? 2 : 0) : ((y < yp) ? 3 : 1);
-  ibox[i] = nc + q;
-  atomicAdd(&nbat[nc + q - 1], 1);
+  const int lane = threadIdx.x & 63;
+  unsigned long long todo = __ballot(key >= 0);
+  for (int round = 0; round < 8 && todo; ++round) {
+    const int leader = __ffsll((long long)todo) - 1;
+    const int lk = __shfl(key, leader);
+    const unsigned long long same = __ballot(key == lk) & todo;
+    if (lane == leader) atomicAdd(&nbat[lk - 1], __popcll(same));
+    todo &= ~same;
+  }
+  if (key >= 0 && ((todo >> lane) & 1ull)) atomicAdd(&nbat[key - 1], 1);  // many distinct boxes
 }
 
 __global__ void k_at_split_flag(const int* __restrict__ nbat, int lo, int m, int napb, int* __restrict__ flag) {
@@ -548,7 +577,7 @@ int build(eb_atree_s* T, const double* x, const double* y, int lmx, double eps7,
   const int np = std::min(1024, nblk(n));
   AT_TRY(grow(T->s_part, np, 0, st));
   hipLaunchKernelGGL(k_at_minmax, dim3(np), dim3(TPB), 0, st, x, y, n, T->s_part.p);
-  hipLaunchKernelGGL(k_at_geom, dim3(1), dim3(64), 0, st, T->s_part.p, np, T->geom.p);
+  hipLaunchKernelGGL(k_at_geom, dim3(1), dim3(TPB), 0, st, T->s_part.p, np, T->geom.p);
   hipLaunchKernelGGL(k_at_scale, dim3(nblk(n)), dim3(TPB), 0, st, x, y, n, T->geom.p, T->xs.p, T->ys.p);
   EB_HIP_CHECK(hipGetLastError());
 
