@@ -1,83 +1,46 @@
 // eb_spectral.hip -- per-body spectral geometry, RK2 stage updates and the Stokes matvec
 // epilogue, batched over all bodies (drops) of one or many simulations.
 //
-// One 256-thread workgroup owns one body; the body's sequence lives in LDS for the whole
+// One 512-thread workgroup owns one body; the body's sequence lives in LDS for the whole
 // transform chain (FFT, multiply, inverse FFT), so each kernel reads its inputs from HBM once
-// and writes its outputs once.  The FFT is a mixed-radix Stockham auto-sort in which every
-// thread forms whole outputs of a stage (sum over the radix with twiddles from a table of
-// N-th roots of unity), so any length works, including the large prime factors that body
-// lengths rounded to even produce (e.g. 1234 = 2 x 617).
+// and writes its outputs once.  The FFT (eb_fft.h) runs in place in LDS: Stockham stages of
+// radix 2, 3, 4, 5 and 8 in registers, direct sums for other small primes, and Bluestein's
+// method for a large prime factor (body lengths rounded to even give e.g. 1942 = 2 x 971).
+// Workgroups take the bodies in order of decreasing cost, so the longest transforms start
+// first.
 #include <math.h>
 #include <string.h>
 
 #include <algorithm>
 #include <map>
 #include <mutex>
+#include <numeric>
 #include <vector>
 
 #include "../../include/eb_spectral.h"
+#include "eb_fft_plan.h"
 #include "eb_internal.h"
 
 using namespace eb;
 
-#define EB_MAXFAC 32
-constexpr int SP_THREADS = 256;
+constexpr int SP_THREADS = ebfft::FFT_THREADS;
 
 struct eb_bodies_s {
-  int nbody = 0, n = 0, nmax = 0;
+  int nbody = 0, n = 0, nmax = 0, ext = 0;
   std::vector<int> nd, off;
-  int* d_nd = nullptr;     // [nbody]
-  int* d_off = nullptr;    // [nbody + 1]
-  int* d_fac = nullptr;    // [nbody][EB_MAXFAC]
-  int* d_nfac = nullptr;   // [nbody]
-  int* d_twoff = nullptr;  // [nbody] offset of the body's root table in d_tw
-  double2* d_tw = nullptr; // exp(-2 pi i j / N), j < N, one table per distinct N
-  double* d_red = nullptr; // [nbody][4] xmin, xmax, ymin, ymax
+  int* d_nd = nullptr;       // [nbody]
+  int* d_off = nullptr;      // [nbody + 1]
+  int* d_plan_of = nullptr;  // [nbody] plan of the body's length
+  int* d_order = nullptr;    // [nbody] bodies by decreasing transform cost
+  ebfft::Plan* d_plans = nullptr;
+  ebfft::Stage* d_stages = nullptr;
+  double2* d_tab = nullptr;  // twiddles, roots, chirps, filter transforms
+  double* d_red = nullptr;   // [nbody][4] xmin, xmax, ymin, ymax
 };
 
 // ------------------------------------------------------------------------------------------
-// block-level FFT in LDS
+// per-body helpers in LDS
 // ------------------------------------------------------------------------------------------
-
-// Stockham stage with radix R: out[o] = sum_r in[j + r N/R] W^(r (k + q Ns)),
-// W = exp(sign 2 pi i / (Ns R)), where o = (j / Ns) Ns R + q Ns + (j % Ns), k = j % Ns.
-// The result of all stages is in natural order.  a holds the input and receives the output;
-// b is scratch of N entries.
-__device__ void fft_block(double2* a, double2* b, int N, const int* __restrict__ fac, int nfac,
-                          const double2* __restrict__ tw, int sign) {
-  double2* in = a;
-  double2* out = b;
-  int Ns = 1;
-  for (int f = 0; f < nfac; ++f) {
-    const int R = fac[f];
-    const int NR = N / R;
-    const int step = N / (Ns * R);
-    for (int o = threadIdx.x; o < N; o += blockDim.x) {
-      const int q = (o / Ns) % R;
-      const int j = (o / (Ns * R)) * Ns + (o % Ns);
-      const int e = ((o % Ns) + q * Ns) * step;
-      double2 acc = in[j];
-      int idx = e;
-      for (int r = 1; r < R; ++r) {
-        double2 w = tw[idx];
-        if (sign > 0) w.y = -w.y;
-        cfma(acc, in[j + r * NR], w);
-        idx += e;
-        if (idx >= N) idx -= N;
-      }
-      out[o] = acc;
-    }
-    __syncthreads();
-    double2* t = in;
-    in = out;
-    out = t;
-    Ns *= R;
-  }
-  if (in != a) {
-    for (int o = threadIdx.x; o < N; o += blockDim.x) a[o] = in[o];
-    __syncthreads();
-  }
-}
 
 // wavenumber of coefficient i under FDIFFF's rules (dcfft.f:1404-1422): 0 for the mean and,
 // for even N, the Nyquist coefficient
@@ -104,7 +67,7 @@ __device__ __forceinline__ double filt_factor(int i, int n) {
   return exp(-10.0 * r);
 }
 
-__device__ void filter_block(double2* c, int N) {
+__device__ __forceinline__ void filter_block(double2* c, int N) {
   const int n = N / 2;
   const double eps = 1e-13;
   for (int i = threadIdx.x; i < N; i += blockDim.x) {
@@ -126,7 +89,7 @@ __device__ void filter_block(double2* c, int N) {
   __syncthreads();
 }
 
-__device__ double block_sum(double v, double* red) {
+__device__ __forceinline__ double block_sum(double v, double* red) {
   red[threadIdx.x] = v;
   __syncthreads();
   for (int off = blockDim.x / 2; off > 0; off >>= 1) {
@@ -141,40 +104,61 @@ __device__ double block_sum(double v, double* red) {
 struct BodyView {
   const int* nd;
   const int* off;
-  const int* fac;
-  const int* nfac;
-  const int* twoff;
-  const double2* tw;
+  const int* plan_of;
+  const int* order;
+  const ebfft::Plan* plans;
+  const ebfft::Stage* stages;
+  const double2* tab;
+  int nmax;
 };
 
-static BodyView view(eb_bodies B) { return BodyView{B->d_nd, B->d_off, B->d_fac, B->d_nfac, B->d_twoff, B->d_tw}; }
+static BodyView view(eb_bodies B) {
+  return BodyView{B->d_nd, B->d_off, B->d_plan_of, B->d_order, B->d_plans, B->d_stages, B->d_tab, B->nmax};
+}
+
+// the body of this workgroup and its transform plan
+struct BodyCtx {
+  int k, N, o0;
+  ebfft::Plan p;
+};
+__device__ __forceinline__ BodyCtx body_ctx(const BodyView& B) {
+  BodyCtx c;
+  c.k = B.order[blockIdx.x];
+  c.N = B.nd[c.k];
+  c.o0 = B.off[c.k];
+  c.p = B.plans[B.plan_of[c.k]];
+  return c;
+}
+
+__device__ __forceinline__ void fft(double2* x, double2* ws, const BodyCtx& c, const BodyView& B, int sign) {
+  ebfft::fft_inplace(x, ws, c.p, B.stages, B.tab, sign);
+}
 
 // ------------------------------------------------------------------------------------------
-// kernels (one block per body; dynamic LDS holds 3 sequences of nmax entries)
+// kernels (one block per body; dynamic LDS holds nseq sequences of nmax entries followed by
+// the Bluestein workspace)
 // ------------------------------------------------------------------------------------------
 
 __global__ __launch_bounds__(SP_THREADS) void k_dcfft(BodyView B, double2* __restrict__ c, int sign) {
   extern __shared__ double2 lds[];
-  const int k = blockIdx.x, N = B.nd[k], o0 = B.off[k];
+  const BodyCtx bc = body_ctx(B);
   double2* a = lds;
-  double2* b = lds + N;
-  for (int i = threadIdx.x; i < N; i += blockDim.x) a[i] = c[o0 + i];
+  for (int i = threadIdx.x; i < bc.N; i += blockDim.x) a[i] = c[bc.o0 + i];
   __syncthreads();
-  fft_block(a, b, N, B.fac + k * EB_MAXFAC, B.nfac[k], B.tw + B.twoff[k], sign);
-  for (int i = threadIdx.x; i < N; i += blockDim.x) c[o0 + i] = a[i];
+  fft(a, lds + B.nmax, bc, B, sign);
+  for (int i = threadIdx.x; i < bc.N; i += blockDim.x) c[bc.o0 + i] = a[i];
 }
 
 __global__ __launch_bounds__(SP_THREADS) void k_fdifff(BodyView B, const double2* __restrict__ f,
                                                         double2* __restrict__ df) {
   extern __shared__ double2 lds[];
-  const int k = blockIdx.x, N = B.nd[k], o0 = B.off[k];
+  const BodyCtx bc = body_ctx(B);
+  const int N = bc.N;
   double2* a = lds;
-  double2* b = lds + N;
-  const int* fac = B.fac + k * EB_MAXFAC;
-  const double2* tw = B.tw + B.twoff[k];
-  for (int i = threadIdx.x; i < N; i += blockDim.x) a[i] = f[o0 + i];
+  double2* ws = lds + B.nmax;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) a[i] = f[bc.o0 + i];
   __syncthreads();
-  fft_block(a, b, N, fac, B.nfac[k], tw, -1);
+  fft(a, ws, bc, B, -1);
   const double invn = 1.0 / N;
   for (int i = threadIdx.x; i < N; i += blockDim.x) {
     const double w = fdiff_k(i, N) * invn;  // (i k / N) * F
@@ -182,40 +166,34 @@ __global__ __launch_bounds__(SP_THREADS) void k_fdifff(BodyView B, const double2
     a[i] = make_double2(-w * v.y, w * v.x);
   }
   __syncthreads();
-  fft_block(a, b, N, fac, B.nfac[k], tw, +1);
-  for (int i = threadIdx.x; i < N; i += blockDim.x) df[o0 + i] = a[i];
+  fft(a, ws, bc, B, +1);
+  for (int i = threadIdx.x; i < N; i += blockDim.x) df[bc.o0 + i] = a[i];
 }
 
 __global__ __launch_bounds__(SP_THREADS) void k_filter(BodyView B, double2* __restrict__ c) {
   extern __shared__ double2 lds[];
-  const int k = blockIdx.x, N = B.nd[k], o0 = B.off[k];
-  for (int i = threadIdx.x; i < N; i += blockDim.x) lds[i] = c[o0 + i];
+  const BodyCtx bc = body_ctx(B);
+  for (int i = threadIdx.x; i < bc.N; i += blockDim.x) lds[i] = c[bc.o0 + i];
   __syncthreads();
-  filter_block(lds, N);
-  for (int i = threadIdx.x; i < N; i += blockDim.x) c[o0 + i] = lds[i];
+  filter_block(lds, bc.N);
+  for (int i = threadIdx.x; i < bc.N; i += blockDim.x) c[bc.o0 + i] = lds[i];
 }
 
-// EQUIT + FOURINT (electric.f:1270-1331) of body k into T (real, LDS or global); a, b, c are
-// LDS sequences.  Returns after T is written for every point.
-__device__ void equit_block(const BodyView& B, int k, const double* __restrict__ tha, const double* __restrict__ un,
-                            double2* a, double2* b, double2* c, double* tout) {
-  const int N = B.nd[k], o0 = B.off[k], n = N / 2;
-  const int* fac = B.fac + k * EB_MAXFAC;
-  const double2* tw = B.tw + B.twoff[k];
+// EQUIT + FOURINT (electric.f:1270-1331) of one body: a and c are LDS sequences, ws the
+// transform workspace.  T(i) is left in c[i].x; returns synchronised.
+__device__ __forceinline__ void equit_block(const BodyCtx& bc, const BodyView& B, const double* __restrict__ tha,
+                            const double* __restrict__ un, double2* a, double2* c, double2* ws) {
+  const int N = bc.N, o0 = bc.o0, n = N / 2;
   for (int i = threadIdx.x; i < N; i += blockDim.x) a[i] = make_double2(-tha[o0 + i] * un[o0 + i], 0.0);
   __syncthreads();
-  fft_block(a, b, N, fac, B.nfac[k], tw, -1);
+  fft(a, ws, bc, B, -1);
   const double invn = 1.0 / N;
   for (int i = threadIdx.x; i < N; i += blockDim.x) {
     double2 v = a[i];
     v = make_double2(v.x * invn, v.y * invn);
     if (i == n) v = make_double2(0, 0);  // zf(n+1) = 0
     a[i] = v;
-  }
-  __syncthreads();
-  // FOURINT: zfi(i) = zf(i)/(i*eye) for 1 <= i <= n-1, the mirrored ones with -, zero elsewhere
-  for (int i = threadIdx.x; i < N; i += blockDim.x) {
-    const double2 v = a[i];
+    // FOURINT: zfi(i) = zf(i)/(i*eye) for 1 <= i <= n-1, the mirrored ones with -, zero elsewhere
     double2 r = make_double2(0, 0);
     if (i >= 1 && i <= n - 1) {
       const double ii = (double)i;
@@ -227,13 +205,13 @@ __device__ void equit_block(const BodyView& B, int k, const double* __restrict__
     c[i] = r;
   }
   __syncthreads();
-  fft_block(c, b, N, fac, B.nfac[k], tw, +1);
+  fft(c, ws, bc, B, +1);
   const double2 zf0 = a[0];
   const double dadt = zf0.x;
   const double dalph = 2.0 * 3.14159265358979323846 / N;
   for (int i = threadIdx.x; i < N; i += blockDim.x) {
     const double alpha = dalph * (double)i;
-    tout[i] = -(zf0.x * alpha + c[i].x) + alpha * dadt;
+    c[i].x = -(zf0.x * alpha + c[i].x) + alpha * dadt;
   }
   __syncthreads();
 }
@@ -241,36 +219,36 @@ __device__ void equit_block(const BodyView& B, int k, const double* __restrict__
 __global__ __launch_bounds__(SP_THREADS) void k_equit(BodyView B, const double* __restrict__ tha,
                                                        const double* __restrict__ un, double* __restrict__ t) {
   extern __shared__ double2 lds[];
-  const int k = blockIdx.x, N = B.nd[k], o0 = B.off[k];
-  equit_block(B, k, tha, un, lds, lds + N, lds + 2 * N, t + o0);
+  const BodyCtx bc = body_ctx(B);
+  double2* a = lds;
+  double2* c = lds + B.nmax;
+  equit_block(bc, B, tha, un, a, c, lds + 2 * B.nmax);
+  for (int i = threadIdx.x; i < bc.N; i += blockDim.x) t[bc.o0 + i] = c[i].x;
 }
 
 __global__ __launch_bounds__(SP_THREADS) void k_ftail(BodyView B, const double* __restrict__ tha,
                                                        const double* __restrict__ un, const double2* __restrict__ dz,
                                                        const double* __restrict__ dsda, double2* __restrict__ rkup) {
   extern __shared__ double2 lds[];
-  const int k = blockIdx.x, N = B.nd[k], o0 = B.off[k];
+  const BodyCtx bc = body_ctx(B);
+  const int N = bc.N, o0 = bc.o0;
   double2* a = lds;
-  double2* b = lds + N;
-  double2* c = lds + 2 * N;
-  double* T = (double*)b;  // b is free between the EQUIT transforms and the next FFT
-  equit_block(B, k, tha, un, a, b, c, T);
-  const int* fac = B.fac + k * EB_MAXFAC;
-  const double2* tw = B.tw + B.twoff[k];
-  // zf = (-Un i dz - T dz) / dsda  (electric.f:1084-1086); T is read from b before the FFT
-  // below reuses b as scratch
+  double2* c = lds + B.nmax;
+  double2* ws = lds + 2 * B.nmax;
+  equit_block(bc, B, tha, un, a, c, ws);
+  // zf = (-Un i dz - T dz) / dsda  (electric.f:1084-1086), T from c[i].x
   for (int i = threadIdx.x; i < N; i += blockDim.x) {
     const double2 d = dz[o0 + i];
-    const double u = un[o0 + i], tt = T[i], s = dsda[o0 + i];
+    const double u = un[o0 + i], tt = c[i].x, s = dsda[o0 + i];
     a[i] = make_double2((u * d.y - tt * d.x) / s, (-u * d.x - tt * d.y) / s);
   }
   __syncthreads();
-  fft_block(a, b, N, fac, B.nfac[k], tw, -1);
+  fft(a, ws, bc, B, -1);
   const double invn = 1.0 / N;
   for (int i = threadIdx.x; i < N; i += blockDim.x) a[i] = make_double2(a[i].x * invn, a[i].y * invn);
   __syncthreads();
   filter_block(a, N);
-  fft_block(a, b, N, fac, B.nfac[k], tw, +1);
+  fft(a, ws, bc, B, +1);
   for (int i = threadIdx.x; i < N; i += blockDim.x) rkup[o0 + i] = a[i];
 }
 
@@ -284,15 +262,14 @@ __global__ __launch_bounds__(SP_THREADS) void k_extractdat(BodyView B, const dou
                                                             double* __restrict__ area, double* __restrict__ red4) {
   extern __shared__ double2 lds[];
   __shared__ double red[SP_THREADS];
-  const int k = blockIdx.x, N = B.nd[k], o0 = B.off[k];
+  const BodyCtx bc = body_ctx(B);
+  const int k = bc.k, N = bc.N, o0 = bc.o0;
   double2* a = lds;
-  double2* b = lds + N;
-  double2* c = lds + 2 * N;
-  const int* fac = B.fac + k * EB_MAXFAC;
-  const double2* tw = B.tw + B.twoff[k];
+  double2* c = lds + B.nmax;
+  double2* ws = lds + 2 * B.nmax;
   for (int i = threadIdx.x; i < N; i += blockDim.x) a[i] = z[o0 + i];
   __syncthreads();
-  fft_block(a, b, N, fac, B.nfac[k], tw, -1);
+  fft(a, ws, bc, B, -1);
   const double invn = 1.0 / N;
   if (threadIdx.x == 0 && zk) zk[k] = make_double2(a[0].x * invn, a[0].y * invn);
   for (int i = threadIdx.x; i < N; i += blockDim.x) {
@@ -302,8 +279,8 @@ __global__ __launch_bounds__(SP_THREADS) void k_extractdat(BodyView B, const dou
     a[i] = make_double2(-kk * v.y, kk * v.x);             // (ik) F / N
   }
   __syncthreads();
-  fft_block(a, b, N, fac, B.nfac[k], tw, +1);  // z_alpha
-  fft_block(c, b, N, fac, B.nfac[k], tw, +1);  // z_alpha_alpha
+  fft(a, ws, bc, B, +1);  // z_alpha
+  fft(c, ws, bc, B, +1);  // z_alpha_alpha
   double sarea = 0.0, xmn = 1e300, xmx = -1e300, ymn = 1e300, ymx = -1e300;
   for (int i = threadIdx.x; i < N; i += blockDim.x) {
     const int g = o0 + i;
@@ -495,31 +472,17 @@ static int dalloc(T** p, size_t count) {
   return EB_OK;
 }
 
-static std::vector<int> factorize(int n) {
-  std::vector<int> f;
-  while (n % 4 == 0) {
-    f.push_back(4);
-    n /= 4;
-  }
-  while (n % 2 == 0) {
-    f.push_back(2);
-    n /= 2;
-  }
-  for (int p = 3; p * p <= n; p += 2)
-    while (n % p == 0) {
-      f.push_back(p);
-      n /= p;
-    }
-  if (n > 1) f.push_back(n);
-  return f;
-}
-
 static void free_bodies(eb_bodies B) {
   if (!B) return;
-  void* ptrs[] = {B->d_nd, B->d_off, B->d_fac, B->d_nfac, B->d_twoff, B->d_tw, B->d_red};
+  void* ptrs[] = {B->d_nd, B->d_off, B->d_plan_of, B->d_order, B->d_plans, B->d_stages, B->d_tab, B->d_red};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   delete B;
+}
+
+template <typename T>
+static bool upload(T* dst, const std::vector<T>& src, hipStream_t st) {
+  return src.empty() || hipMemcpyAsync(dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice, st) == hipSuccess;
 }
 
 extern "C" int eb_bodies_create(eb_bodies* out, int32_t nbody, const int32_t* nd, void* stream) {
@@ -542,53 +505,44 @@ extern "C" int eb_bodies_create(eb_bodies* out, int32_t nbody, const int32_t* nd
     B->nmax = std::max(B->nmax, (int)nd[k]);
   }
   B->n = B->off[nbody];
-  {
-    int dev = 0, maxlds = 0;
-    if (hipGetDevice(&dev) == hipSuccess &&
-        hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess &&
-        (size_t)maxlds < sizeof(double2) * 3 * (size_t)B->nmax) {
-      eb_set_error("eb_bodies_create: a body of %d points needs %zu B of LDS, the device allows %d", B->nmax,
-                   sizeof(double2) * 3 * (size_t)B->nmax, maxlds);
-      free_bodies(B);
-      return EB_ERR_ARG;
-    }
+  // LDS of a workgroup: two sequences of nmax entries, the Bluestein workspace and the
+  // static reduction buffer of SP_THREADS doubles
+  int dev = 0, maxlds = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) {
+    eb_set_error("eb_bodies_create: no device");
+    free_bodies(B);
+    return EB_ERR_HIP;
   }
-  std::vector<int> fac((size_t)nbody * EB_MAXFAC, 1), nfac(nbody), twoff(nbody);
-  std::map<int, int> tab;  // N -> offset
-  std::vector<double2> tw;
-  for (int k = 0; k < nbody; ++k) {
-    const int N = nd[k];
-    std::vector<int> f = factorize(N);
-    nfac[k] = (int)f.size();
-    for (size_t i = 0; i < f.size(); ++i) fac[(size_t)k * EB_MAXFAC + i] = f[i];
-    auto it = tab.find(N);
-    if (it == tab.end()) {
-      const int o = (int)tw.size();
-      tab[N] = o;
-      for (int j = 0; j < N; ++j) {
-        const long double a = -6.28318530717958647692528676655900577L * (long double)j / (long double)N;
-        tw.push_back(make_double2((double)cosl(a), (double)sinl(a)));
-      }
-      twoff[k] = o;
-    } else {
-      twoff[k] = it->second;
-    }
+  const long long avail = ((long long)maxlds - (long long)sizeof(double) * SP_THREADS) / (long long)sizeof(double2);
+  if (avail < 2LL * B->nmax) {
+    eb_set_error("eb_bodies_create: a body of %d points needs %zu B of LDS, the device allows %d", B->nmax,
+                 sizeof(double2) * 2 * (size_t)B->nmax + sizeof(double) * SP_THREADS, maxlds);
+    free_bodies(B);
+    return EB_ERR_ARG;
   }
+  const int ext_budget = (int)std::min<long long>(avail - 2LL * B->nmax, ebfft::EMAX);
+  ebfft::Tables T;
+  std::vector<int> plan_of(nbody), order(nbody);
+  for (int k = 0; k < nbody; ++k) plan_of[k] = ebfft::plan_for(T, nd[k], ext_budget);
+  for (const ebfft::Plan& p : T.plans)
+    if (p.P) B->ext = std::max(B->ext, p.Ns * p.M);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int i, int j) { return T.plans[plan_of[i]].cost > T.plans[plan_of[j]].cost; });
   hipStream_t st = (hipStream_t)stream;
   int rc;
-  if ((rc = dalloc(&B->d_nd, nbody)) || (rc = dalloc(&B->d_off, nbody + 1)) ||
-      (rc = dalloc(&B->d_fac, fac.size())) || (rc = dalloc(&B->d_nfac, nbody)) || (rc = dalloc(&B->d_twoff, nbody)) ||
-      (rc = dalloc(&B->d_tw, tw.size())) || (rc = dalloc(&B->d_red, 4 * (size_t)nbody))) {
+  if ((rc = dalloc(&B->d_nd, nbody)) || (rc = dalloc(&B->d_off, nbody + 1)) || (rc = dalloc(&B->d_plan_of, nbody)) ||
+      (rc = dalloc(&B->d_order, nbody)) || (rc = dalloc(&B->d_plans, T.plans.size())) ||
+      (rc = dalloc(&B->d_stages, T.stages.size())) || (rc = dalloc(&B->d_tab, T.tab.size())) ||
+      (rc = dalloc(&B->d_red, 4 * (size_t)nbody))) {
     free_bodies(B);
     return rc;
   }
-  bool ok = hipMemcpyAsync(B->d_nd, nd, sizeof(int) * nbody, hipMemcpyHostToDevice, st) == hipSuccess &&
-            hipMemcpyAsync(B->d_off, B->off.data(), sizeof(int) * (nbody + 1), hipMemcpyHostToDevice, st) == hipSuccess &&
-            hipMemcpyAsync(B->d_fac, fac.data(), sizeof(int) * fac.size(), hipMemcpyHostToDevice, st) == hipSuccess &&
-            hipMemcpyAsync(B->d_nfac, nfac.data(), sizeof(int) * nbody, hipMemcpyHostToDevice, st) == hipSuccess &&
-            hipMemcpyAsync(B->d_twoff, twoff.data(), sizeof(int) * nbody, hipMemcpyHostToDevice, st) == hipSuccess &&
-            hipMemcpyAsync(B->d_tw, tw.data(), sizeof(double2) * tw.size(), hipMemcpyHostToDevice, st) == hipSuccess &&
-            hipStreamSynchronize(st) == hipSuccess;
+  const bool ok = upload(B->d_nd, B->nd, st) && upload(B->d_off, B->off, st) && upload(B->d_plan_of, plan_of, st) &&
+                  upload(B->d_order, order, st) && upload(B->d_plans, T.plans, st) &&
+                  upload(B->d_stages, T.stages, st) && upload(B->d_tab, T.tab, st) &&
+                  hipStreamSynchronize(st) == hipSuccess;
   if (!ok) {
     eb_set_error("eb_bodies_create: upload failed");
     free_bodies(B);
@@ -605,11 +559,12 @@ extern "C" int eb_bodies_destroy(eb_bodies B) {
 
 extern "C" int eb_bodies_npoints(eb_bodies B) { return B ? B->n : 0; }
 
-static size_t lds_bytes(eb_bodies B, int nseq) { return sizeof(double2) * (size_t)B->nmax * nseq; }
+// dynamic LDS of a spectral kernel: nseq sequences of nmax entries and the Bluestein workspace
+static size_t lds_bytes(eb_bodies B, int nseq) { return sizeof(double2) * ((size_t)B->nmax * nseq + (size_t)B->ext); }
 
 extern "C" int eb_dcfft(eb_bodies B, double* c, int32_t sign, void* stream) {
   if (!B || !c || (sign != 1 && sign != -1)) return (eb_set_error("eb_dcfft: bad arguments"), EB_ERR_ARG);
-  hipLaunchKernelGGL(k_dcfft, dim3(B->nbody), dim3(SP_THREADS), lds_bytes(B, 2), (hipStream_t)stream, view(B),
+  hipLaunchKernelGGL(k_dcfft, dim3(B->nbody), dim3(SP_THREADS), lds_bytes(B, 1), (hipStream_t)stream, view(B),
                      (double2*)c, sign);
   EB_HIP_CHECK(hipGetLastError());
   return EB_OK;
@@ -617,7 +572,7 @@ extern "C" int eb_dcfft(eb_bodies B, double* c, int32_t sign, void* stream) {
 
 extern "C" int eb_fdifff(eb_bodies B, const double* f, double* df, void* stream) {
   if (!B || !f || !df) return (eb_set_error("eb_fdifff: bad arguments"), EB_ERR_ARG);
-  hipLaunchKernelGGL(k_fdifff, dim3(B->nbody), dim3(SP_THREADS), lds_bytes(B, 2), (hipStream_t)stream, view(B),
+  hipLaunchKernelGGL(k_fdifff, dim3(B->nbody), dim3(SP_THREADS), lds_bytes(B, 1), (hipStream_t)stream, view(B),
                      (const double2*)f, (double2*)df);
   EB_HIP_CHECK(hipGetLastError());
   return EB_OK;
@@ -625,7 +580,7 @@ extern "C" int eb_fdifff(eb_bodies B, const double* f, double* df, void* stream)
 
 extern "C" int eb_filter(eb_bodies B, double* cn, void* stream) {
   if (!B || !cn) return (eb_set_error("eb_filter: bad arguments"), EB_ERR_ARG);
-  hipLaunchKernelGGL(k_filter, dim3(B->nbody), dim3(SP_THREADS), lds_bytes(B, 1), (hipStream_t)stream, view(B),
+  hipLaunchKernelGGL(k_filter, dim3(B->nbody), dim3(SP_THREADS), sizeof(double2) * (size_t)B->nmax, (hipStream_t)stream, view(B),
                      (double2*)cn);
   EB_HIP_CHECK(hipGetLastError());
   return EB_OK;
@@ -633,7 +588,7 @@ extern "C" int eb_filter(eb_bodies B, double* cn, void* stream) {
 
 extern "C" int eb_equit(eb_bodies B, const double* tha, const double* un, double* t, void* stream) {
   if (!B || !tha || !un || !t) return (eb_set_error("eb_equit: bad arguments"), EB_ERR_ARG);
-  hipLaunchKernelGGL(k_equit, dim3(B->nbody), dim3(SP_THREADS), lds_bytes(B, 3), (hipStream_t)stream, view(B), tha,
+  hipLaunchKernelGGL(k_equit, dim3(B->nbody), dim3(SP_THREADS), lds_bytes(B, 2), (hipStream_t)stream, view(B), tha,
                      un, t);
   EB_HIP_CHECK(hipGetLastError());
   return EB_OK;
@@ -642,7 +597,7 @@ extern "C" int eb_equit(eb_bodies B, const double* tha, const double* un, double
 extern "C" int eb_ftail(eb_bodies B, const double* tha, const double* un, const double* dz, const double* dsda,
                         double* rkup, void* stream) {
   if (!B || !tha || !un || !dz || !dsda || !rkup) return (eb_set_error("eb_ftail: bad arguments"), EB_ERR_ARG);
-  const size_t lds = lds_bytes(B, 3);
+  const size_t lds = lds_bytes(B, 2);
   hipLaunchKernelGGL(k_ftail, dim3(B->nbody), dim3(SP_THREADS), lds, (hipStream_t)stream, view(B), tha, un,
                      (const double2*)dz, dsda, (double2*)rkup);
   EB_HIP_CHECK(hipGetLastError());
@@ -654,7 +609,7 @@ extern "C" int eb_extractdat(eb_bodies B, const double* rku, double* x, double* 
                              double* xleft_slength, void* stream) {
   if (!B || !rku) return (eb_set_error("eb_extractdat: bad arguments"), EB_ERR_ARG);
   hipStream_t st = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_extractdat, dim3(B->nbody), dim3(SP_THREADS), lds_bytes(B, 3), st, view(B),
+  hipLaunchKernelGGL(k_extractdat, dim3(B->nbody), dim3(SP_THREADS), lds_bytes(B, 2), st, view(B),
                      (const double2*)rku, x, y, (double2*)zk, (double2*)dz, tha, dsda, rkappa, (double2*)ckern,
                      area, B->d_red);
   if (xleft_slength) hipLaunchKernelGGL(k_extract_final, dim3(1), dim3(64), 0, st, B->d_red, B->nbody, xleft_slength);

@@ -78,6 +78,57 @@ def test_dcfft_many_lengths_against_numpy(torch, sp):
         assert rel(bwd[s], n * np.fft.ifft(data[k])) < 1e-12, n
 
 
+def test_dcfft_bluestein_lengths_against_numpy(torch, sp):
+    """The batched sweep's lengths without a longer body: 2P with P prime up to 1021, odd
+    primes and repeated primes, so the largest prime factor runs as a Bluestein stage."""
+    rng = np.random.default_rng(5)
+    nd = np.round(np.exp(rng.uniform(np.log(128), np.log(2048), 200)) / 2).astype(np.int32) * 2
+    special = [2 * 1019, 2 * 1021, 2 * 971, 2 * 17, 17, 1009, 2 * 17 * 17, 3 * 7 * 97, 4 * 11 * 13 * 3, 2047]
+    nd = np.concatenate([nd, special]).astype(np.int32)
+    data = [rng.normal(size=n) + 1j * rng.normal(size=n) for n in nd]
+    B = sp.Bodies(nd)
+    c = torch.tensor(cat(data), device="cuda")
+    fwd = B.dcfft(c, -1).cpu().numpy()
+    bwd = B.dcfft(c, +1).cpu().numpy()
+    dif = B.fdifff(c).cpu().numpy()
+    off = np.concatenate([[0], np.cumsum(nd)])
+    for k, n in enumerate(nd):
+        s = slice(off[k], off[k + 1])
+        assert rel(fwd[s], np.fft.fft(data[k])) < 1e-13, n
+        assert rel(bwd[s], n * np.fft.ifft(data[k])) < 1e-13, n
+        kk = np.fft.fftfreq(n, 1.0 / n)
+        if n % 2 == 0:
+            kk[n // 2] = 0.0
+        ref = np.fft.ifft(1j * kk * np.fft.fft(data[k]))
+        assert np.abs(dif[s] - ref).max() < 1e-13 * n * np.abs(data[k]).max(), n
+
+
+def test_extractdat_and_ftail_bluestein_lengths_match_oracle(torch, sp):
+    nd = np.array([1942, 2026, 578, 1009, 1024], np.int32)
+    rku = cat([perturbed_circle(int(n), [(2, 0.05), (3, 0.02)], center=1.5 * k - 0.5j * k)[0]
+               for k, n in enumerate(nd)]).view(np.float64).copy()
+    B = sp.Bodies(nd)
+    o = B.extractdat(torch.tensor(rku, device="cuda"))
+    ora = oracle.extractdat(nd, rku)
+    for k in ("x", "y", "zk", "dz", "dsda"):
+        assert rel(o[k].cpu().numpy(), ora[k]) < 1e-12, k
+    # second derivatives: transform rounding grows with the squared wavenumber, up to (N/2)^2
+    for k in ("tha", "rkappa"):
+        assert rel(o[k].cpu().numpy(), ora[k]) < 1e-16 * float(nd.max()) ** 2, k
+    assert np.abs(o["area"].cpu().numpy() - ora["area"]).max() < 1e-13
+    # EQUIT and FILTER take N even (the Nyquist rules of electric.f:1284-1299,1162-1184), as the
+    # reference's bodies always are: the F tail runs on the even bodies only
+    ev = np.array([n for n in nd if n % 2 == 0], np.int32)
+    sel = np.concatenate([np.arange(o, o + n) for o, n in zip(np.concatenate([[0], np.cumsum(nd)]), nd) if n % 2 == 0])
+    B2 = sp.Bodies(ev)
+    un = np.random.default_rng(9).normal(size=int(ev.sum())) * 0.05
+    tha, dz, dsda = ora["tha"][sel], ora["dz"][sel], ora["dsda"][sel]
+    got = B2.ftail(torch.tensor(tha, device="cuda"), torch.tensor(un, device="cuda"), torch.tensor(dz, device="cuda"),
+                   torch.tensor(dsda, device="cuda")).cpu().numpy()
+    ref = oracle.ftail(ev, tha, un, dz, dsda).view(np.complex128)
+    assert rel(got, ref) < 1e-12
+
+
 def test_fdifff_exact_on_trig_polynomials(torch, sp):
     nd = np.array([16, 30, 64, 250, 2048], np.int32)
     f, d = [], []
