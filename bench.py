@@ -192,6 +192,102 @@ def spectral_leg(torch, dev, with_cpu):
     return out
 
 
+def sweep_leg(torch, dev, with_cpu):
+    """Batched versus plain Stokes solves at 1e-10 relative (SOLVEI's DGMRES, MAXL 100, TOL
+    1e-10, no restart) for the batched-sweep shapes: 512 independent single-drop simulations,
+    r = 1 + 0.05 cos 2 theta, N log-uniform in 128..2048 rounded to even.  The right-hand
+    side of each is A u for a smooth density u.  Batched: all 512 systems in one batched GMRES
+    over the sweep operator (each simulation's O(N^2) layer potential).  Plain: the
+    per-simulation GPU path (one MatVec, one GMRES workspace, one solve per simulation, the
+    direct sum that EB_MODE_AUTO picks at N <= 2048), timed on every 16th simulation and
+    reported per solve.  The reference leg runs SOLVEI's solve of a few simulations on one
+    host core."""
+    from electricbubbles_amd import gmres, spectral
+    from electricbubbles_amd.workloads import perturbed_circle
+    rng = np.random.default_rng(20260512)
+    nsim = 512
+    nd = (np.round(np.exp(rng.uniform(np.log(128), np.log(2048), nsim)) / 2) * 2).astype(np.int32)
+    rku = np.concatenate([perturbed_circle(int(n), [(2, 0.05)])[0] for n in nd]).view(np.float64).copy()
+    B = spectral.Bodies(nd)
+    geo = B.extractdat(torch.tensor(rku, device=dev))
+    sw = spectral.Sweep(B, np.arange(nsim + 1), geo["x"], geo["y"], geo["dz"], geo["dsda"], geo["rkappa"], geo["zk"])
+    th = np.concatenate([2 * np.pi * np.arange(m) / m for m in nd])
+    u_true = torch.tensor(np.stack([np.cos(th) + 0.3 * np.sin(2 * th), 0.5 * np.sin(th)], 1).ravel(), device=dev)
+    rhs = sw.apply(u_true)
+    gb = gmres.GMRESBatch(sw.unknown_offsets, 100)
+    gmres.sweep_solve(sw, gb, rhs)  # warm-up
+    torch.cuda.synchronize()
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        soln, za, pk, it, err, ierr = gmres.sweep_solve(sw, gb, rhs)
+    torch.cuda.synchronize()
+    t_batch = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(5):
+        sw.apply(soln)
+    torch.cuda.synchronize()
+    t_apply = (time.perf_counter() - t0) / 5
+    res = float(torch.linalg.norm(sw.apply(soln) - rhs) / torch.linalg.norm(rhs))
+    pairs = float((nd.astype(np.float64) * (nd - 1)).sum())
+    # plain path on every 16th simulation
+    off = np.concatenate([[0], np.cumsum(nd)])
+    sample = list(range(0, nsim, 16))
+    t_plain, it_plain, worst = 0.0, [], 0.0
+    for s in sample:
+        p0, p1 = off[s], off[s + 1]
+        Bs = spectral.Bodies(nd[s:s + 1])
+        mv = spectral.MatVec(Bs, geo["x"][p0:p1], geo["y"][p0:p1], geo["dz"][p0:p1], geo["dsda"][p0:p1],
+                             geo["rkappa"][p0:p1], geo["zk"][s:s + 1], mode=0)
+        solver = gmres.GMRES(2 * int(p1 - p0), 100)
+        r = rhs[2 * p0:2 * p1].contiguous()
+        gmres.stokes_solve(mv, solver, r)  # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        xs, _, _, its, _, _ = gmres.stokes_solve(mv, solver, r)
+        torch.cuda.synchronize()
+        t_plain += time.perf_counter() - t0
+        it_plain.append(its)
+        worst = max(worst, float(torch.linalg.norm(xs - soln[2 * p0:2 * p1]) / torch.linalg.norm(xs)))
+        solver.close()
+        mv.close()
+    per_plain = t_plain / len(sample)
+    out = {"workload": f"{nsim} independent single drops, N log-uniform 128..2048 (even), {int(nd.sum())} points",
+           "maxl": 100, "tol": 1e-10, "iterations_mean": round(float(it.mean()), 2), "iterations_max": int(it.max()),
+           "converged": int((ierr == 0).sum()),
+           "batched_ms": round(t_batch * 1e3, 3), "batched_solves_per_s": round(nsim / t_batch, 1),
+           "plain_ms_per_solve": round(per_plain * 1e3, 3), "plain_solves_per_s": round(1.0 / per_plain, 1),
+           "batched_vs_plain": round((nsim / t_batch) * per_plain, 2),
+           "plain_sample": f"every 16th simulation ({len(sample)} solves)",
+           "plain_vs_batched_solution_rel_diff": worst,
+           "sweep_apply_ms": round(t_apply * 1e3, 4),
+           "sweep_apply_TFLOPs": round(50.0 * pairs / t_apply / 1e12, 2), "rel_residual": res}
+    if with_cpu:
+        import tempfile
+        import oracle
+        if oracle.ref_available():
+            cwd = os.getcwd()
+            os.chdir(tempfile.mkdtemp())
+            sim = oracle.RefSim(oracle.LIBREF)
+            t_ref, n_ref = 0.0, 0
+            try:
+                for s in sample[:8]:
+                    p0, p1 = off[s], off[s + 1]
+                    g = {k: geo[k][p0:p1].cpu().numpy() for k in ("x", "y", "dz", "dsda", "rkappa")}
+                    sim.setup(nd[s:s + 1])
+                    t0 = time.perf_counter()
+                    oracle.ref_dgmres_stokes(nd[s:s + 1], g["x"] + 1j * g["y"], g["dz"], g["dsda"], g["rkappa"],
+                                             geo["zk"][s:s + 1].cpu().numpy(), rhs[2 * p0:2 * p1].cpu().numpy())
+                    t_ref += time.perf_counter() - t0
+                    n_ref += 1
+            finally:
+                os.chdir(cwd)
+            out["cpu_reference"] = {"solves_per_s": round(n_ref / t_ref, 2), "cores": 1, "kind": "reference",
+                                    "sample": f"SOLVEI's solve of {n_ref} of the sampled simulations"}
+    gb.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -205,6 +301,7 @@ def main():
     ap.add_argument("--no-check", action="store_true", help="skip the accuracy spot check")
     ap.add_argument("--no-solve", action="store_true", help="skip the Stokes GMRES solve leg")
     ap.add_argument("--no-spectral", action="store_true", help="skip the spectral geometry leg")
+    ap.add_argument("--no-sweep", action="store_true", help="skip the batched-versus-plain sweep leg")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 256 MiB device copy first (the known-size kernel that scales HBM counters)")
     args = ap.parse_args()
@@ -308,6 +405,7 @@ def main():
 
     solve = None if args.no_solve else stokes_solve_leg(torch, dev, not args.no_cpu)
     spec = None if args.no_spectral else spectral_leg(torch, dev, not args.no_cpu)
+    sweep = None if args.no_sweep else sweep_leg(torch, dev, not args.no_cpu)
 
     cpu = None
     if not args.no_cpu:
@@ -336,6 +434,7 @@ def main():
         "adaptive_tree": at_info,
         "stokes_solve": solve,
         "spectral": spec,
+        "sweep": sweep,
         "fp64_fraction_total": round(total_fl / (t_apply) / 1e12 / PEAK_FP64_TFLOPS, 4),
         "roofline": {"bound": "mfma", "kernel": dom, "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": traffic,

@@ -51,6 +51,39 @@ __device__ __forceinline__ double2 crcp(double2 z) {
   return make_double2(z.x * ir, -z.y * ir);
 }
 
+// One source-target pair of the near field, in the translation-invariant form
+//   qa/d + sgn (conj(qb)/conj(d) - conj(qa) d/conj(d)^2),  d = z_src - z_tgt
+// (SLDIRDR / SLSELFDR, cadapdrop2.f:586-599, 741-767, with QC = -conj(z) QA folded in).
+// acc1 collects qa/d, acc2 the bracket.  With MASK, `self` (source == target) contributes
+// exactly zero: the reciprocal is forced to 0, which zeroes both terms without a branch.
+// FAST uses one Newton step on the hardware reciprocal, the exact variant two.
+template <bool MASK, bool FAST>
+__device__ __forceinline__ void p2p_pair(double2 zi, double2 zj, double2 qa, double2 qb, bool self, double2& acc1,
+                                         double2& acc2) {
+  const double dx = zj.x - zi.x, dy = zj.y - zi.y;
+  const double r2 = fma(dx, dx, dy * dy);
+  double ir;
+  if (MASK) {
+    ir = FAST ? rcp_nr1(self ? 1.0 : r2) : rcp_nr(self ? 1.0 : r2);
+    ir = self ? 0.0 : ir;
+  } else {
+    ir = FAST ? rcp_nr1(r2) : rcp_nr(r2);
+  }
+  const double wx = dx * ir, wy = dy * ir;  // w = 1/conj(d)
+  // acc1 += qa * conj(w)
+  acc1.x = fma(qa.x, wx, fma(qa.y, wy, acc1.x));
+  acc1.y = fma(qa.y, wx, fma(-qa.x, wy, acc1.y));
+  // e = d * w = d / conj(d)
+  const double ex = fma(dx, wx, -dy * wy);
+  const double ey = fma(dx, wy, dy * wx);
+  // v = conj(qb) - conj(qa) * e
+  const double vx = fma(-qa.x, ex, fma(-qa.y, ey, qb.x));
+  const double vy = fma(-qa.x, ey, fma(qa.y, ex, -qb.y));
+  // acc2 += v * w
+  acc2.x = fma(vx, wx, fma(-vy, wy, acc2.x));
+  acc2.y = fma(vx, wy, fma(vy, wx, acc2.y));
+}
+
 // ---- Morton helpers (x in the even bits) ---------------------------------------------------
 __host__ __device__ __forceinline__ uint32_t spread2(uint32_t v) {
   v &= 0x0000ffffu;

@@ -536,38 +536,6 @@ __global__ __launch_bounds__(256) void k_l2p(int n, int L, const uint32_t* __res
   gwf[i] = make_double2(phi.x + sgn * t.x, phi.y + sgn * t.y);
 }
 
-// One source-target pair of the near field, in the translation-invariant form
-//   qa/d + sgn (conj(qb)/conj(d) - conj(qa) d/conj(d)^2),  d = z_src - z_tgt
-// (SLDIRDR / SLSELFDR, cadapdrop2.f:586-599, 741-767, with QC = -conj(z) QA folded in).
-// acc1 collects qa/d, acc2 the bracket.  With MASK, `self` (source == target) contributes
-// exactly zero: the reciprocal is forced to 0, which zeroes both terms without a branch.
-// FAST uses one Newton step on the hardware reciprocal, the exact variant two.
-template <bool MASK, bool FAST>
-__device__ __forceinline__ void p2p_pair(double2 zi, double2 zj, double2 qa, double2 qb, bool self, double2& acc1,
-                                         double2& acc2) {
-  const double dx = zj.x - zi.x, dy = zj.y - zi.y;
-  const double r2 = fma(dx, dx, dy * dy);
-  double ir;
-  if (MASK) {
-    ir = FAST ? rcp_nr1(self ? 1.0 : r2) : rcp_nr(self ? 1.0 : r2);
-    ir = self ? 0.0 : ir;
-  } else {
-    ir = FAST ? rcp_nr1(r2) : rcp_nr(r2);
-  }
-  const double wx = dx * ir, wy = dy * ir;  // w = 1/conj(d)
-  // acc1 += qa * conj(w)
-  acc1.x = fma(qa.x, wx, fma(qa.y, wy, acc1.x));
-  acc1.y = fma(qa.y, wx, fma(-qa.x, wy, acc1.y));
-  // e = d * w = d / conj(d)
-  const double ex = fma(dx, wx, -dy * wy);
-  const double ey = fma(dx, wy, dy * wx);
-  // v = conj(qb) - conj(qa) * e
-  const double vx = fma(-qa.x, ex, fma(-qa.y, ey, qb.x));
-  const double vy = fma(-qa.x, ey, fma(qa.y, ex, -qb.y));
-  // acc2 += v * w
-  acc2.x = fma(vx, wx, fma(-vy, wy, acc2.x));
-  acc2.y = fma(vx, wy, fma(vy, wx, acc2.y));
-}
 
 // Near-field work items: a leaf with T points becomes ceil(T / P2P_TMAX) items of nearly
 // equal target ranges (all with the leaf's whole 3x3 neighbourhood as sources), so that the

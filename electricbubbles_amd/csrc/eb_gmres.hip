@@ -177,7 +177,7 @@ int ew_grid(int n) { return std::max(1, std::min(4096, (n + 255) / 256)); }
 // DHEQR column update for column ll (1-based), dheqr.f:100-176 (IJOB > 1, or the full
 // factorisation of a 2x1 matrix for ll = 1 which is the same arithmetic on one column).
 // hes is column-major with leading dimension ldh; q holds (c, s) pairs.  Returns INFO.
-int dheqr_column(std::vector<double>& hes, int ldh, int ll, std::vector<double>& q) {
+int dheqr_column(double* hes, int ldh, int ll, double* q) {
   auto A = [&](int i, int j) -> double& { return hes[(size_t)(j - 1) * ldh + (i - 1)]; };
   for (int k = 1; k <= ll - 1; ++k) {
     const int i = 2 * (k - 1);
@@ -206,7 +206,7 @@ int dheqr_column(std::vector<double>& hes, int ldh, int ll, std::vector<double>&
 }
 
 // DHELS (dhels.f): least-squares solution of the factored Hessenberg system, in place on b.
-void dhels(const std::vector<double>& hes, int ldh, int n, const std::vector<double>& q, std::vector<double>& b) {
+void dhels(const double* hes, int ldh, int n, const double* q, double* b) {
   auto A = [&](int i, int j) { return hes[(size_t)(j - 1) * ldh + (i - 1)]; };
   for (int k = 1; k <= n; ++k) {
     const double c = q[2 * (k - 1)], s = q[2 * (k - 1) + 1];
@@ -290,7 +290,7 @@ int dpigmr_cycle(eb_gmres_s* g, eb_apply_fn apply, void* ctx, double* x, double 
       }
     }
     hes[(size_t)(ll - 1) * ldh + ll] = snormw;  // HES(LL+1, LL)
-    const int info = dheqr_column(hes, ldh, ll, q);
+    const int info = dheqr_column(hes.data(), ldh, ll, q.data());
     if (info == ll) {  // singular Hessenberg: no progress (dpigmr.f:366, 419-423)
       iflag = 2;
       return EB_OK;
@@ -317,7 +317,7 @@ int dpigmr_cycle(eb_gmres_s* g, eb_apply_fn apply, void* ctx, double* x, double 
   const int l = lgmr;
   std::vector<double> b(l + 1, 0.0);
   b[0] = r0nrm;
-  dhels(hes, ldh, l, q, b);
+  dhels(hes.data(), ldh, l, q.data(), b.data());
   if (iflag == 1 && nrmax > 0) {
     // DRLCAL (drlcal.f, KMP = MAXL): the residual of the new iterate as a combination of
     // v_1 .. v_{l+1}, for the next cycle; v_{l+1} is unscaled, hence C / SNORMW.
@@ -464,5 +464,385 @@ extern "C" int eb_stokes_solve(eb_matvec mv, eb_gmres g, const double* rhs, doub
   int rc = eb_gmres_solve(g, eb_matvec_gmres_apply, mv, rhs, soln, 1.0e-10, -1, iter, err, ierr);
   if (rc != EB_OK) return rc;
   if (za || pk) rc = eb_matvec_parts(mv, soln, za, pk, stream);
+  return rc;
+}
+
+// ==========================================================================================
+// Batched GMRES: many independent systems A_s x_s = b_s in lockstep
+// ==========================================================================================
+//
+// System s owns entries [off[s], off[s+1]) of every n-vector; the operator applies all
+// systems at once (a block-diagonal A).  Each system follows DGMRES with IGWK(5) = -1 (one
+// DPIGMR cycle, no restart), SOLVEI's setting (stokesbubb.f:294-415): its own Hessenberg
+// matrix, rotations, norms and tests, in the reference's order, on the host.  On the device
+// one workgroup owns one system, so every reduction is a fixed tree inside one block and one
+// kernel runs the whole modified Gram-Schmidt step of all systems.  Systems that have
+// converged or stopped drop out of the kernels; their Krylov columns stay untouched until
+// the final linear combination, which updates all iterates in one launch.
+namespace {
+
+constexpr int GB_T = 256;
+
+__device__ __forceinline__ double2 block_sum2(double a, double b, double2* s) {
+  s[threadIdx.x] = make_double2(a, b);
+  __syncthreads();
+  for (int o = GB_T / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      s[threadIdx.x].x += s[threadIdx.x + o].x;
+      s[threadIdx.x].y += s[threadIdx.x + o].y;
+    }
+    __syncthreads();
+  }
+  const double2 r = s[0];
+  __syncthreads();
+  return r;
+}
+
+// per-system slots of the scalar array (hld = maxl + 4 doubles per system)
+// [0, maxl): Hessenberg column; maxl: |w|^2 before MGS; maxl+1: |w|^2 after; maxl+2: SUMDSQ;
+// maxl+3: spare.  The residual pass uses slots 0 and 1 for |b|^2 and |r|^2.
+
+// r = b - r (r holds A x); |b|^2 and |r|^2 per system
+__global__ __launch_bounds__(GB_T) void k_gmb_residual(const int* __restrict__ off, const double* __restrict__ b,
+                                                       double* __restrict__ r, double* __restrict__ hs, int hld) {
+  __shared__ double2 red[GB_T];
+  const int s = blockIdx.x, o = off[s], n = off[s + 1] - o;
+  double sb = 0.0, sr = 0.0;
+  for (int i = threadIdx.x; i < n; i += GB_T) {
+    const double bi = b[o + i];
+    const double ri = bi - r[o + i];
+    r[o + i] = ri;
+    sb += bi * bi;
+    sr += ri * ri;
+  }
+  const double2 t = block_sum2(sb, sr, red);
+  if (threadIdx.x == 0) {
+    hs[(size_t)s * hld] = t.x;
+    hs[(size_t)s * hld + 1] = t.y;
+  }
+}
+
+// dst = src * fac[s] for the systems with act[s]
+__global__ void k_gmb_scale(const int* __restrict__ off, const int* __restrict__ act, const double* __restrict__ fac,
+                            const double* __restrict__ src, double* __restrict__ dst) {
+  const int s = blockIdx.x;
+  if (!act[s]) return;
+  const int o = off[s], n = off[s + 1] - o;
+  const double c = fac[s];
+  for (int i = threadIdx.x; i < n; i += blockDim.x) dst[o + i] = src[o + i] * c;
+}
+
+// DORTH's first pass (dorth.f:90-107) on w = V(:, ll) of every active system: |w|^2, then
+// h_i = v_i . w and w -= h_i v_i for i = 0 .. ll-1 (MGS), then |w|^2
+__global__ __launch_bounds__(GB_T) void k_gmb_orth(int ll, const int* __restrict__ off, const int* __restrict__ act,
+                                                   double* __restrict__ V, size_t ldv, double* __restrict__ hs,
+                                                   int hld, int maxl) {
+  __shared__ double2 red[GB_T];
+  const int s = blockIdx.x;
+  if (!act[s]) return;
+  const int o = off[s], n = off[s + 1] - o;
+  double* w = V + (size_t)ll * ldv + o;
+  double* h = hs + (size_t)s * hld;
+  const double* v0 = V + o;
+  double a = 0.0, b = 0.0;
+  for (int i = threadIdx.x; i < n; i += GB_T) {
+    const double wi = w[i];
+    a += wi * wi;
+    b += v0[i] * wi;
+  }
+  double2 t = block_sum2(a, b, red);
+  if (threadIdx.x == 0) {
+    h[maxl] = t.x;
+    h[0] = t.y;
+  }
+  double hprev = t.y;
+  for (int i = 1; i <= ll; ++i) {
+    const double c = -hprev;
+    const double* vi = V + (size_t)(i - 1) * ldv + o;
+    const double* vn = V + (size_t)i * ldv + o;
+    double acc = 0.0;
+    if (i < ll) {
+      for (int j = threadIdx.x; j < n; j += GB_T) {
+        const double wj = w[j] + c * vi[j];
+        w[j] = wj;
+        acc += vn[j] * wj;
+      }
+    } else {
+      for (int j = threadIdx.x; j < n; j += GB_T) {
+        const double wj = w[j] + c * vi[j];
+        w[j] = wj;
+        acc += wj * wj;
+      }
+    }
+    t = block_sum2(acc, 0.0, red);
+    hprev = t.x;
+    if (threadIdx.x == 0) h[i < ll ? i : maxl + 1] = t.x;
+  }
+}
+
+// DORTH's re-orthogonalisation pass (dorth.f:108-121) for the flagged systems: the updated
+// Hessenberg column stays in h[0..ll) and SUMDSQ goes to h[maxl + 2]
+__global__ __launch_bounds__(GB_T) void k_gmb_reorth(int ll, const int* __restrict__ off, const int* __restrict__ flag,
+                                                     double* __restrict__ V, size_t ldv, double* __restrict__ hs,
+                                                     int hld, int maxl) {
+  __shared__ double2 red[GB_T];
+  const int s = blockIdx.x;
+  if (!flag[s]) return;
+  const int o = off[s], n = off[s + 1] - o;
+  double* w = V + (size_t)ll * ldv + o;
+  double* h = hs + (size_t)s * hld;
+  double sumdsq = 0.0;
+  for (int i = 1; i <= ll; ++i) {
+    const double* vi = V + (size_t)(i - 1) * ldv + o;
+    double acc = 0.0;
+    for (int j = threadIdx.x; j < n; j += GB_T) acc += vi[j] * w[j];
+    const double tem = -block_sum2(acc, 0.0, red).x;
+    const double hil = h[i - 1];
+    if (hil + 0.001 * tem == hil) continue;
+    for (int j = threadIdx.x; j < n; j += GB_T) w[j] += tem * vi[j];
+    __syncthreads();  // every lane has read h[i - 1] before it changes
+    if (threadIdx.x == 0) h[i - 1] = hil - tem;
+    sumdsq = sumdsq + tem * tem;
+  }
+  if (threadIdx.x == 0) h[maxl + 2] = sumdsq;
+}
+
+// x_s += sum_{j < l_s} c_s[j] V_s(:, j) in the order of dpigmr.f:411-413
+__global__ void k_gmb_combine(const int* __restrict__ off, const int* __restrict__ lcnt, const double* __restrict__ coef,
+                              int cld, const double* __restrict__ V, size_t ldv, double* __restrict__ x) {
+  const int s = blockIdx.x, l = lcnt[s];
+  if (l <= 0) return;
+  const int o = off[s], n = off[s + 1] - o;
+  const double* c = coef + (size_t)s * cld;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    double z = 0.0;
+    for (int j = 0; j < l; ++j) z += c[j] * V[(size_t)j * ldv + o + i];
+    x[o + i] += z;
+  }
+}
+
+}  // namespace
+
+struct eb_gmres_batch_s {
+  int nsys = 0, maxl = 0, ntot = 0, hld = 0;
+  hipStream_t st = nullptr;
+  std::vector<int> off, maxl_s;
+  int* d_off = nullptr;
+  int* d_act = nullptr;     // per system: 1 while it iterates (also the re-orth flags)
+  int* d_lcnt = nullptr;    // per system: number of Krylov vectors in its correction
+  double* d_fac = nullptr;  // per-system scale factors
+  double* V = nullptr;      // ntot x (maxl + 1)
+  double* r = nullptr;      // residuals (ntot)
+  double* hs = nullptr;     // nsys x hld scalars
+  double* coef = nullptr;   // nsys x maxl combination coefficients
+  // host state
+  std::vector<double> hes, q, h, fac, cf;
+  std::vector<int> act, flag, lcnt;
+};
+
+extern "C" int eb_gmres_batch_create(eb_gmres_batch* out, int32_t nsys, const int32_t* off, int32_t maxl,
+                                     void* stream) {
+  if (!out || nsys < 1 || !off || off[0] != 0) return (eb_set_error("eb_gmres_batch_create: bad arguments"), EB_ERR_ARG);
+  *out = nullptr;
+  if (maxl == 0) maxl = 10;  // dgmres.f: MAXL = IGWK(1), 0 -> 10
+  for (int s = 0; s < nsys; ++s)
+    if (off[s + 1] <= off[s]) return (eb_set_error("eb_gmres_batch_create: system %d is empty", s), EB_ERR_ARG);
+  eb_gmres_batch g = new eb_gmres_batch_s();
+  g->nsys = nsys;
+  g->off.assign(off, off + nsys + 1);
+  g->ntot = off[nsys];
+  g->maxl_s.resize(nsys);
+  int mx = 1;
+  for (int s = 0; s < nsys; ++s) {
+    g->maxl_s[s] = std::max(1, std::min<int>(maxl, off[s + 1] - off[s]));  // MAXL cut to N per system
+    mx = std::max(mx, g->maxl_s[s]);
+  }
+  g->maxl = mx;
+  g->hld = mx + 4;
+  g->st = (hipStream_t)stream;
+  const size_t ns = (size_t)nsys;
+  const bool ok = hipMalloc((void**)&g->d_off, sizeof(int) * (ns + 1)) == hipSuccess &&
+                  hipMalloc((void**)&g->d_act, sizeof(int) * ns) == hipSuccess &&
+                  hipMalloc((void**)&g->d_lcnt, sizeof(int) * ns) == hipSuccess &&
+                  hipMalloc((void**)&g->d_fac, sizeof(double) * ns) == hipSuccess &&
+                  hipMalloc((void**)&g->V, sizeof(double) * (size_t)g->ntot * (mx + 1)) == hipSuccess &&
+                  hipMalloc((void**)&g->r, sizeof(double) * (size_t)g->ntot) == hipSuccess &&
+                  hipMalloc((void**)&g->hs, sizeof(double) * ns * g->hld) == hipSuccess &&
+                  hipMalloc((void**)&g->coef, sizeof(double) * ns * mx) == hipSuccess &&
+                  hipMemcpyAsync(g->d_off, off, sizeof(int) * (ns + 1), hipMemcpyHostToDevice, g->st) == hipSuccess &&
+                  hipStreamSynchronize(g->st) == hipSuccess;
+  if (!ok) {
+    eb_gmres_batch_destroy(g);
+    return (eb_set_error("eb_gmres_batch_create: device allocation failed"), EB_ERR_NOMEM);
+  }
+  g->hes.assign(ns * (mx + 1) * mx, 0.0);
+  g->q.assign(ns * 2 * mx, 0.0);
+  g->h.assign(ns * g->hld, 0.0);
+  g->fac.assign(ns, 0.0);
+  g->cf.assign(ns * mx, 0.0);
+  g->act.assign(ns, 0);
+  g->flag.assign(ns, 0);
+  g->lcnt.assign(ns, 0);
+  *out = g;
+  return EB_OK;
+}
+
+extern "C" int eb_gmres_batch_destroy(eb_gmres_batch g) {
+  if (!g) return EB_OK;
+  if (g->st) (void)hipStreamSynchronize(g->st);
+  void* p[] = {g->d_off, g->d_act, g->d_lcnt, g->d_fac, g->V, g->r, g->hs, g->coef};
+  for (void* q : p)
+    if (q) (void)hipFree(q);
+  delete g;
+  return EB_OK;
+}
+
+extern "C" int eb_gmres_batch_solve(eb_gmres_batch g, eb_apply_fn apply, void* ctx, const double* b, double* x,
+                                    double tol, int32_t* iter, double* err, int32_t* ierr) {
+  if (!g || !apply || !b || !x) return (eb_set_error("eb_gmres_batch_solve: bad arguments"), EB_ERR_ARG);
+  if (tol == 0.0) tol = 500.0 * 1.1102230246251565e-16;  // 500 * D1MACH(3)
+  const int S = g->nsys, hld = g->hld, mx = g->maxl, ldh = mx + 1;
+  const size_t ldv = (size_t)g->ntot;
+  hipStream_t st = g->st;
+  std::vector<double> bnrm(S), r0nrm(S), prod(S, 1.0), rho(S), snormw(S, 0.0), e(S, 0.0);
+  std::vector<int> it(S, 0), ie(S, 0);
+  auto H = [&](int s, int i) -> double& { return g->h[(size_t)s * hld + i]; };
+  auto upload_act = [&](const std::vector<int>& a) {
+    return hipMemcpyAsync(g->d_act, a.data(), sizeof(int) * S, hipMemcpyHostToDevice, st);
+  };
+  auto fetch_h = [&]() -> int {
+    EB_HIP_CHECK(hipMemcpyAsync(g->h.data(), g->hs, sizeof(double) * (size_t)S * hld, hipMemcpyDeviceToHost, st));
+    EB_HIP_CHECK(hipStreamSynchronize(st));
+    return EB_OK;
+  };
+  std::fill(g->hes.begin(), g->hes.end(), 0.0);
+  std::fill(g->q.begin(), g->q.end(), 0.0);
+  std::fill(g->lcnt.begin(), g->lcnt.end(), 0);
+  // r = b - A x, BNRM = |b| per system
+  int rc = apply(ctx, x, g->r, st);
+  if (rc != EB_OK) return rc;
+  hipLaunchKernelGGL(k_gmb_residual, dim3(S), dim3(GB_T), 0, st, g->d_off, b, g->r, g->hs, hld);
+  EB_HIP_CHECK(hipGetLastError());
+  if ((rc = fetch_h()) != EB_OK) return rc;
+  int nact = 0;
+  for (int s = 0; s < S; ++s) {
+    bnrm[s] = sqrt(H(s, 0));
+    r0nrm[s] = sqrt(H(s, 1));
+    e[s] = r0nrm[s] / bnrm[s];  // ISDGMR, ITOL = 0, at the start of the cycle
+    g->act[s] = e[s] <= tol ? 0 : 1;
+    g->fac[s] = g->act[s] ? 1.0 / r0nrm[s] : 0.0;
+    nact += g->act[s];
+  }
+  EB_HIP_CHECK(upload_act(g->act));
+  EB_HIP_CHECK(hipMemcpyAsync(g->d_fac, g->fac.data(), sizeof(double) * S, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_gmb_scale, dim3(S), dim3(256), 0, st, g->d_off, g->d_act, g->d_fac, g->r, g->V);
+  for (int ll = 1; ll <= mx && nact > 0; ++ll) {
+    double* w = g->V + (size_t)ll * ldv;
+    if ((rc = apply(ctx, g->V + (size_t)(ll - 1) * ldv, w, st)) != EB_OK) return rc;
+    hipLaunchKernelGGL(k_gmb_orth, dim3(S), dim3(GB_T), 0, st, ll, g->d_off, g->d_act, g->V, ldv, g->hs, hld, mx);
+    EB_HIP_CHECK(hipGetLastError());
+    if ((rc = fetch_h()) != EB_OK) return rc;
+    int nflag = 0;
+    for (int s = 0; s < S; ++s) {
+      g->flag[s] = 0;
+      if (!g->act[s]) continue;
+      const double vnrm = sqrt(H(s, mx));
+      snormw[s] = sqrt(H(s, mx + 1));
+      if (!(vnrm + 0.001 * snormw[s] != vnrm)) {
+        g->flag[s] = 1;
+        ++nflag;
+      }
+    }
+    if (nflag) {
+      EB_HIP_CHECK(upload_act(g->flag));
+      hipLaunchKernelGGL(k_gmb_reorth, dim3(S), dim3(GB_T), 0, st, ll, g->d_off, g->d_act, g->V, ldv, g->hs, hld, mx);
+      EB_HIP_CHECK(hipGetLastError());
+      if ((rc = fetch_h()) != EB_OK) return rc;
+      for (int s = 0; s < S; ++s)
+        if (g->flag[s]) {
+          const double sumdsq = H(s, mx + 2);
+          if (sumdsq != 0.0) snormw[s] = sqrt(std::max(0.0, snormw[s] * snormw[s] - sumdsq));
+        }
+    }
+    for (int s = 0; s < S; ++s) {
+      g->fac[s] = 0.0;
+      if (!g->act[s]) continue;
+      double* hes = g->hes.data() + (size_t)s * ldh * mx;
+      double* qs = g->q.data() + (size_t)s * 2 * mx;
+      for (int i = 1; i <= ll; ++i) hes[(size_t)(ll - 1) * ldh + (i - 1)] = H(s, i - 1);
+      hes[(size_t)(ll - 1) * ldh + ll] = snormw[s];  // HES(LL+1, LL)
+      const int info = dheqr_column(hes, ldh, ll, qs);
+      it[s] = ll;
+      int done = 0;  // 1: update x with l = ll vectors (converged or out of vectors); 2: stop, no update
+      if (info == ll) {
+        done = 2;  // singular Hessenberg (dpigmr.f:366, 419-423)
+        ie[s] = 2;
+      } else {
+        prod[s] = prod[s] * qs[2 * ll - 1];
+        rho[s] = fabs(prod[s] * r0nrm[s]);
+        e[s] = rho[s] / bnrm[s];
+        if (e[s] <= tol) {
+          done = 1;
+          ie[s] = 0;
+        } else if (ll == g->maxl_s[s]) {
+          if (!(rho[s] < r0nrm[s])) {
+            done = 2;
+            ie[s] = 2;
+          } else {
+            done = 1;
+            ie[s] = 1;  // IGWK(5) = -1: no restart
+          }
+        }
+      }
+      if (done) {
+        g->act[s] = 0;
+        --nact;
+        if (done == 1) {
+          std::vector<double> bb(ll + 1, 0.0);
+          bb[0] = r0nrm[s];
+          dhels(hes, ldh, ll, qs, bb.data());
+          for (int j = 0; j < ll; ++j) g->cf[(size_t)s * mx + j] = bb[j];
+          g->lcnt[s] = ll;
+        }
+      } else {
+        g->fac[s] = 1.0 / snormw[s];
+      }
+    }
+    if (nact > 0) {
+      EB_HIP_CHECK(upload_act(g->act));
+      EB_HIP_CHECK(hipMemcpyAsync(g->d_fac, g->fac.data(), sizeof(double) * S, hipMemcpyHostToDevice, st));
+      hipLaunchKernelGGL(k_gmb_scale, dim3(S), dim3(256), 0, st, g->d_off, g->d_act, g->d_fac, w, w);
+    }
+  }
+  EB_HIP_CHECK(hipMemcpyAsync(g->d_lcnt, g->lcnt.data(), sizeof(int) * S, hipMemcpyHostToDevice, st));
+  EB_HIP_CHECK(hipMemcpyAsync(g->coef, g->cf.data(), sizeof(double) * (size_t)S * mx, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_gmb_combine, dim3(S), dim3(256), 0, st, g->d_off, g->d_lcnt, g->coef, mx, g->V, ldv, x);
+  EB_HIP_CHECK(hipGetLastError());
+  EB_HIP_CHECK(hipStreamSynchronize(st));
+  for (int s = 0; s < S; ++s) {
+    if (iter) iter[s] = it[s];
+    if (err) err[s] = e[s];
+    if (ierr) ierr[s] = ie[s];
+  }
+  return EB_OK;
+}
+
+extern "C" int eb_sweep_gmres_apply(void* sw, const double* x, double* y, void* stream) {
+  return eb_sweep_apply((eb_sweep)sw, x, y, stream);
+}
+
+extern "C" int eb_sweep_solve(eb_sweep sw, eb_gmres_batch g, const double* rhs, double* soln, double* za, double* pk,
+                              int32_t* iter, double* err, int32_t* ierr, void* stream) {
+  if (!sw || !g || !rhs || !soln) return (eb_set_error("eb_sweep_solve: bad arguments"), EB_ERR_ARG);
+  hipStream_t st = (hipStream_t)stream;
+  if (st != g->st) return (eb_set_error("eb_sweep_solve: the GMRES workspace was created on another stream"), EB_ERR_ARG);
+  const int32_t* uo = eb_sweep_unknown_offsets(sw);
+  if ((int)g->off.size() != g->nsys + 1 || !std::equal(g->off.begin(), g->off.end(), uo))
+    return (eb_set_error("eb_sweep_solve: the GMRES workspace does not match the sweep's systems"), EB_ERR_ARG);
+  // soln = rhs as the initial guesses (stokesbubb.f:343-346), ITOL 0, TOL 1e-10, no restart
+  hipLaunchKernelGGL(k_gm_copy, dim3(ew_grid(g->ntot)), dim3(256), 0, st, g->ntot, rhs, soln);
+  int rc = eb_gmres_batch_solve(g, eb_sweep_gmres_apply, sw, rhs, soln, 1.0e-10, iter, err, ierr);
+  if (rc != EB_OK) return rc;
+  if (za || pk) rc = eb_sweep_parts(sw, soln, za, pk, stream);
   return rc;
 }

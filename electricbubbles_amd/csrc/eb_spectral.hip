@@ -742,6 +742,216 @@ extern "C" int eb_matvec_destroy(eb_matvec M) {
 }
 
 // ------------------------------------------------------------------------------------------
+// many independent simulations (eb_sweep)
+// ------------------------------------------------------------------------------------------
+
+// QA, QB of every point (cadapdrop.f:218-220, DR signs) in caller order and unscaled
+// coordinates: the O(N^2) sum below is invariant under CADAPDR's shift and scaling
+__global__ void k_sw_charges(int n, const double2* __restrict__ dens, const double2* __restrict__ wdz,
+                             double2* __restrict__ qa, double2* __restrict__ qb) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double2 d = dens[i], w = wdz[i];
+  const double c = -0.15915494309189533577;  // 1/(2 pi i) = -i/(2 pi)
+  const double2 dw = cmul(d, w);
+  qa[i] = make_double2(-c * dw.y, c * dw.x);
+  const double2 t = cadd(cscale(cmul(cconj(d), w), -1.0), cmul(d, cconj(w)));
+  qb[i] = make_double2(-c * t.y, c * t.x);
+}
+
+// GW of every point from the other points of its own simulation (SLDIRDR / SLSELFDR,
+// cadapdrop2.f:564-620, 724-769).  One 256-thread block per tile = (simulation, 256
+// targets); the simulation's sources pass through LDS in chunks of 256.  Tiles come in order
+// of decreasing simulation size.
+constexpr int SW_T = 256;
+__global__ __launch_bounds__(SW_T) void k_sw_gw(const int2* __restrict__ tiles, const int* __restrict__ soff,
+                                                const double* __restrict__ x, const double* __restrict__ y,
+                                                const double2* __restrict__ qa, const double2* __restrict__ qb,
+                                                double2* __restrict__ gw) {
+  __shared__ double2 sz[SW_T], sa[SW_T], sq[SW_T];
+  const int2 tl = tiles[blockIdx.x];  // (simulation, first target within it)
+  const int o = soff[tl.x], n = soff[tl.x + 1] - o;
+  const int it = tl.y + (int)threadIdx.x;
+  const bool act = it < n;
+  const int i = o + (act ? it : 0);
+  const double2 zi = make_double2(x[i], y[i]);
+  double2 acc1 = make_double2(0, 0), acc2 = make_double2(0, 0);
+  for (int c0 = 0; c0 < n; c0 += SW_T) {
+    const int j = c0 + (int)threadIdx.x;
+    __syncthreads();
+    if (j < n) {
+      sz[threadIdx.x] = make_double2(x[o + j], y[o + j]);
+      sa[threadIdx.x] = qa[o + j];
+      sq[threadIdx.x] = qb[o + j];
+    }
+    __syncthreads();
+    const int m = min(SW_T, n - c0);
+    if (act)
+      for (int jj = 0; jj < m; ++jj) p2p_pair<true, false>(zi, sz[jj], sa[jj], sq[jj], c0 + jj == it, acc1, acc2);
+  }
+  if (act) gw[i] = make_double2(acc1.x - acc2.x, acc1.y - acc2.y);  // DR: GW = phi - z conj(phi') - conj(psi)
+}
+
+// FASMVP's pointwise part (stokesbubb.f:585-611) with pinf = -pk of the simulation's first body
+__global__ void k_sw_point(int n, const double* __restrict__ x, const double* __restrict__ y,
+                           const double2* __restrict__ dz, const double* __restrict__ dsdth,
+                           const double* __restrict__ rkappa, const double* __restrict__ h,
+                           const double2* __restrict__ dens, const double2* __restrict__ za,
+                           const double* __restrict__ pk, const double2* __restrict__ gw, double2* __restrict__ w,
+                           const int* __restrict__ body_of, const int* __restrict__ first) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  const int k = body_of[g];
+  const double pi = 3.14159265358979323846;
+  const double pinf = -pk[first[k]];
+  const double pkk = pk[k] + pinf;
+  const double2 zu = dens[g];
+  const double2 d = dz[g];
+  const double s = dsdth[g];
+  const double2 zn = make_double2(d.y / s, -d.x / s);  // -i dz / dsdth
+  const double c0 = h[k] * s * 0.5 * rkappa[g] / pi;
+  const double2 zn2 = cmul(zn, zn);
+  const double2 t = cmul(make_double2(zu.x, -zu.y), zn2);
+  const double2 zself = make_double2(c0 * (zu.x - t.x), c0 * (zu.y - t.y));
+  const double a = -0.5 * (pinf - pkk);
+  const double2 pr = make_double2(-a * y[g], a * x[g]);
+  const double2 st = make_double2(gw[g].x + zself.x + pr.x, gw[g].y + zself.y + pr.y);
+  const double2 zak = za[k];
+  w[g] = make_double2(zu.x + zak.x + st.x, zu.y + zak.y + st.y);
+}
+
+__global__ void k_sw_shift_pk(int nbody, const int* __restrict__ first, double* __restrict__ pk_out,
+                              const double* __restrict__ pk) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < nbody) pk_out[k] = pk[k] - pk[first[k]];
+}
+
+struct eb_sweep_s {
+  eb_bodies B = nullptr;
+  int nsim = 0, n = 0, ntile = 0;
+  std::vector<int> uoff;  // unknowns per simulation: 2 x points
+  double *x = nullptr, *y = nullptr, *dsdth = nullptr, *rkappa = nullptr, *h = nullptr, *pk = nullptr;
+  double2 *dz = nullptr, *zk = nullptr, *wdz = nullptr, *gw = nullptr, *za = nullptr, *qa = nullptr, *qb = nullptr;
+  int *body_of = nullptr, *first = nullptr, *soff = nullptr;
+  int2* tiles = nullptr;
+};
+
+static void free_sw(eb_sweep S) {
+  if (!S) return;
+  void* ptrs[] = {S->x, S->y, S->dsdth, S->rkappa, S->h, S->pk, S->dz, S->zk, S->wdz, S->gw, S->za, S->qa, S->qb,
+                  S->body_of, S->first, S->soff, S->tiles};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+  delete S;
+}
+
+extern "C" int eb_sweep_create(eb_sweep* out, eb_bodies B, int32_t nsim, const int32_t* sim_off, const double* x,
+                               const double* y, const double* dz, const double* dsdth, const double* rkappa,
+                               const double* zk, void* stream) {
+  if (!out || !B || nsim < 1 || !sim_off || !x || !y || !dz || !dsdth || !rkappa || !zk)
+    return (eb_set_error("eb_sweep_create: bad arguments"), EB_ERR_ARG);
+  *out = nullptr;
+  if (sim_off[0] != 0 || sim_off[nsim] != B->nbody)
+    return (eb_set_error("eb_sweep_create: simulation offsets must run from 0 to the body count"), EB_ERR_ARG);
+  for (int s = 0; s < nsim; ++s)
+    if (sim_off[s + 1] <= sim_off[s]) return (eb_set_error("eb_sweep_create: simulation %d has no body", s), EB_ERR_ARG);
+  hipStream_t st = (hipStream_t)stream;
+  eb_sweep S = new eb_sweep_s();
+  S->B = B;
+  S->nsim = nsim;
+  const int n = B->n, nb = B->nbody;
+  S->n = n;
+  // point ranges, first bodies and tiles (largest simulations first)
+  std::vector<int> soff(nsim + 1), first(nb);
+  S->uoff.resize(nsim + 1);
+  for (int s = 0; s <= nsim; ++s) {
+    soff[s] = B->off[sim_off[s]];
+    S->uoff[s] = 2 * soff[s];
+  }
+  for (int s = 0; s < nsim; ++s)
+    for (int k = sim_off[s]; k < sim_off[s + 1]; ++k) first[k] = sim_off[s];
+  std::vector<int> ord(nsim);
+  std::iota(ord.begin(), ord.end(), 0);
+  std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return soff[a + 1] - soff[a] > soff[b + 1] - soff[b]; });
+  std::vector<int2> tiles;
+  for (int s : ord)
+    for (int t0 = 0; t0 < soff[s + 1] - soff[s]; t0 += SW_T) tiles.push_back(make_int2(s, t0));
+  S->ntile = (int)tiles.size();
+  std::vector<double> h(nb);
+  for (int k = 0; k < nb; ++k) h[k] = 2.0 * 3.14159265358979323846 / B->nd[k];  // stokesbubb.f:446-448
+  int rc;
+  if ((rc = dalloc(&S->x, n)) || (rc = dalloc(&S->y, n)) || (rc = dalloc(&S->dsdth, n)) ||
+      (rc = dalloc(&S->rkappa, n)) || (rc = dalloc(&S->h, nb)) || (rc = dalloc(&S->pk, nb)) ||
+      (rc = dalloc(&S->dz, n)) || (rc = dalloc(&S->zk, nb)) || (rc = dalloc(&S->wdz, n)) || (rc = dalloc(&S->gw, n)) ||
+      (rc = dalloc(&S->za, nb)) || (rc = dalloc(&S->qa, n)) || (rc = dalloc(&S->qb, n)) ||
+      (rc = dalloc(&S->body_of, n)) || (rc = dalloc(&S->first, nb)) || (rc = dalloc(&S->soff, nsim + 1)) ||
+      (rc = dalloc(&S->tiles, tiles.size()))) {
+    free_sw(S);
+    return rc;
+  }
+  const bool ok = hipMemcpyAsync(S->x, x, sizeof(double) * n, hipMemcpyDeviceToDevice, st) == hipSuccess &&
+                  hipMemcpyAsync(S->y, y, sizeof(double) * n, hipMemcpyDeviceToDevice, st) == hipSuccess &&
+                  hipMemcpyAsync(S->dsdth, dsdth, sizeof(double) * n, hipMemcpyDeviceToDevice, st) == hipSuccess &&
+                  hipMemcpyAsync(S->rkappa, rkappa, sizeof(double) * n, hipMemcpyDeviceToDevice, st) == hipSuccess &&
+                  hipMemcpyAsync(S->dz, dz, sizeof(double2) * n, hipMemcpyDeviceToDevice, st) == hipSuccess &&
+                  hipMemcpyAsync(S->zk, zk, sizeof(double2) * nb, hipMemcpyDeviceToDevice, st) == hipSuccess &&
+                  upload(S->h, h, st) && upload(S->first, first, st) && upload(S->soff, soff, st) &&
+                  upload(S->tiles, tiles, st);
+  if (!ok) {
+    free_sw(S);
+    return (eb_set_error("eb_sweep_create: copy failed"), EB_ERR_HIP);
+  }
+  hipLaunchKernelGGL(k_body_of, dim3(nb), dim3(256), 0, st, B->d_off, nb, S->body_of);
+  hipLaunchKernelGGL(k_wdz, dim3((n + 255) / 256), dim3(256), 0, st, n, (const double2*)S->dz, S->h, S->body_of,
+                     S->wdz);
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess) {
+    free_sw(S);
+    return (eb_set_error("eb_sweep_create: kernel failed"), EB_ERR_HIP);
+  }
+  *out = S;
+  return EB_OK;
+}
+
+// za, pk (unshifted) of the density into S->za, S->pk
+static void sweep_sums(eb_sweep S, const double2* dens, hipStream_t st) {
+  eb_bodies B = S->B;
+  hipLaunchKernelGGL(k_fasmvp_sums, dim3(B->nbody), dim3(SP_THREADS), 0, st, view(B), S->x, S->y, S->dz, S->zk, S->h,
+                     dens, S->za, S->pk);
+}
+
+extern "C" int eb_sweep_apply(eb_sweep S, const double* xx, double* yy, void* stream) {
+  if (!S || !xx || !yy) return (eb_set_error("eb_sweep_apply: bad arguments"), EB_ERR_ARG);
+  hipStream_t st = (hipStream_t)stream;
+  const int n = S->n;
+  const double2* dens = (const double2*)xx;  // dens(i) = dcmplx(u(2i-1), u(2i)) (stokesbubb.f:527-529)
+  hipLaunchKernelGGL(k_sw_charges, dim3((n + 255) / 256), dim3(256), 0, st, n, dens, S->wdz, S->qa, S->qb);
+  hipLaunchKernelGGL(k_sw_gw, dim3(S->ntile), dim3(SW_T), 0, st, S->tiles, S->soff, S->x, S->y, S->qa, S->qb, S->gw);
+  sweep_sums(S, dens, st);
+  hipLaunchKernelGGL(k_sw_point, dim3((n + 255) / 256), dim3(256), 0, st, n, S->x, S->y, S->dz, S->dsdth, S->rkappa,
+                     S->h, dens, S->za, S->pk, S->gw, (double2*)yy, S->body_of, S->first);
+  EB_HIP_CHECK(hipGetLastError());
+  return EB_OK;
+}
+
+extern "C" int eb_sweep_parts(eb_sweep S, const double* xx, double* za, double* pk, void* stream) {
+  if (!S || !xx) return (eb_set_error("eb_sweep_parts: bad arguments"), EB_ERR_ARG);
+  hipStream_t st = (hipStream_t)stream;
+  sweep_sums(S, (const double2*)xx, st);
+  const int nb = S->B->nbody;
+  if (za) EB_HIP_CHECK(hipMemcpyAsync(za, S->za, sizeof(double2) * nb, hipMemcpyDeviceToDevice, st));
+  if (pk) hipLaunchKernelGGL(k_sw_shift_pk, dim3((nb + 255) / 256), dim3(256), 0, st, nb, S->first, pk, S->pk);
+  EB_HIP_CHECK(hipGetLastError());
+  return EB_OK;
+}
+
+extern "C" const int32_t* eb_sweep_unknown_offsets(eb_sweep S) { return S ? S->uoff.data() : nullptr; }
+
+extern "C" int eb_sweep_destroy(eb_sweep S) {
+  free_sw(S);
+  return EB_OK;
+}
+
+// ------------------------------------------------------------------------------------------
 // FASMVP drop-in on host arrays.  The geometry is the same for every matvec of one GMRES
 // solve (MATVEC passes the /inteqn/ arrays unchanged, stokesbubb.f:449-455), so the bodies,
 // the FMM plan and the device copies are kept while (ND, X, Y, DZ, DSDTH, RKAPPA, ZK, H)

@@ -19,7 +19,8 @@ _dp = ctypes.POINTER(ctypes.c_double)
 SPECTRAL_EXPORTS = (
     "eb_bodies_create", "eb_bodies_destroy", "eb_bodies_npoints", "eb_dcfft", "eb_fdifff", "eb_filter",
     "eb_equit", "eb_extractdat", "eb_ftail", "eb_rk2_stage1", "eb_rk2_stage2", "eb_matvec_create",
-    "eb_matvec_apply", "eb_matvec_destroy", "eb_fasmvp",
+    "eb_matvec_apply", "eb_matvec_destroy", "eb_fasmvp", "eb_sweep_create", "eb_sweep_apply", "eb_sweep_parts",
+    "eb_sweep_unknown_offsets", "eb_sweep_destroy",
 )
 
 _bound = False
@@ -46,6 +47,12 @@ def _bind():
     L.eb_matvec_destroy.argtypes = [_vp]
     L.eb_fasmvp.argtypes = [_ip, _ip, _ip, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
     L.eb_fasmvp.restype = _i32
+    L.eb_sweep_create.argtypes = [ctypes.POINTER(_vp), _vp, _i32, _ip, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
+    L.eb_sweep_apply.argtypes = [_vp, _vp, _vp, _vp]
+    L.eb_sweep_parts.argtypes = [_vp, _vp, _vp, _vp, _vp]
+    L.eb_sweep_unknown_offsets.argtypes = [_vp]
+    L.eb_sweep_unknown_offsets.restype = _ip
+    L.eb_sweep_destroy.argtypes = [_vp]
     _bound = True
     return L
 
@@ -165,6 +172,49 @@ class MatVec:
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
             _bind().eb_matvec_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Sweep:
+    """Many independent simulations in one operator: simulation s owns bodies
+    sim_off[s]..sim_off[s+1]-1 of `bodies`; apply() is MATVEC + FASMVP of all of them, each
+    simulation's layer potential summed over its own points only (include/eb_spectral.h)."""
+
+    def __init__(self, bodies, sim_off, x, y, dz, dsdth, rkappa, zk, stream=None):
+        self.bodies = bodies
+        self.stream = stream
+        self.sim_off = np.ascontiguousarray(sim_off, np.int32)
+        self.nsim = self.sim_off.size - 1
+        self._keep = tuple(t.contiguous() for t in (x, y, dz, dsdth, rkappa, zk))
+        h = _vp()
+        _check(_bind().eb_sweep_create(ctypes.byref(h), bodies._h, self.nsim, self.sim_off.ctypes.data_as(_ip),
+                                       *(_ptr(t) for t in self._keep), _stream_ptr(stream)))
+        self._h = h
+        uo = _bind().eb_sweep_unknown_offsets(h)
+        self.unknown_offsets = np.array([uo[i] for i in range(self.nsim + 1)], np.int32)
+
+    def apply(self, xx):
+        yy = _torch().empty(2 * self.bodies.n, dtype=_torch().float64, device=xx.device)
+        _check(_bind().eb_sweep_apply(self._h, _ptr(xx.contiguous()), _ptr(yy), _stream_ptr(self.stream)))
+        return yy
+
+    def parts(self, xx):
+        torch = _torch()
+        k = self.bodies.nbody
+        za = torch.empty(k, dtype=torch.complex128, device=xx.device)
+        pk = torch.empty(k, dtype=torch.float64, device=xx.device)
+        _check(_bind().eb_sweep_parts(self._h, _ptr(xx.contiguous()), _ptr(za), _ptr(pk), _stream_ptr(self.stream)))
+        return za, pk
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _bind().eb_sweep_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -62,6 +62,32 @@ int eb_matvec_parts(eb_matvec mv, const double *xx, double *za, double *pk, void
 int eb_stokes_solve(eb_matvec mv, eb_gmres g, const double *rhs, double *soln, double *za, double *pk,
                     int32_t *iter, double *err, int32_t *ierr, void *stream);
 
+/* ---------------------------------------------------------------------------------------
+ * Batched GMRES: nsys independent systems in lockstep, each with DGMRES's arithmetic for
+ * IGWK(5) = -1 (one cycle, no restart: SOLVEI's setting, stokesbubb.f:294-415).  System s
+ * owns entries [off[s], off[s+1]) of every vector (off: nsys + 1 host ints, off[0] = 0); the
+ * operator applies all systems at once.  MAXL is cut to each system's size.  No
+ * counterpart in the reference, which solves one simulation per process.
+ * --------------------------------------------------------------------------------------- */
+typedef struct eb_gmres_batch_s *eb_gmres_batch;
+
+int eb_gmres_batch_create(eb_gmres_batch *g, int32_t nsys, const int32_t *off, int32_t maxl, void *stream);
+/* x: initial guesses in, solutions out.  iter, err, ierr: nsys entries each (may be NULL);
+ * ierr as eb_gmres_solve (1: not converged within MAXL). */
+int eb_gmres_batch_solve(eb_gmres_batch g, eb_apply_fn apply, void *ctx, const double *b, double *x, double tol,
+                         int32_t *iter, double *err, int32_t *ierr);
+int eb_gmres_batch_destroy(eb_gmres_batch g);
+
+/* eb_apply_fn for the sweep operator (ctx = an eb_sweep) */
+int eb_sweep_gmres_apply(void *sw, const double *x, double *y, void *stream);
+
+/* SOLVEI of every simulation of the sweep at once: soln = rhs as the initial guesses,
+ * TOL = 1e-10, the workspace's MAXL, then za and pk of each solution.  The workspace must
+ * have been created with the sweep's unknown offsets (eb_sweep_unknown_offsets) and `stream`.
+ * iter, err, ierr: one entry per simulation. */
+int eb_sweep_solve(eb_sweep sw, eb_gmres_batch g, const double *rhs, double *soln, double *za, double *pk,
+                   int32_t *iter, double *err, int32_t *ierr, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -78,6 +78,29 @@ int eb_matvec_apply(eb_matvec mv, const double *xx, double *yy, double *stress, 
                     void *stream);
 int eb_matvec_destroy(eb_matvec mv);
 
+/* ---------------------------------------------------------------------------------------
+ * Many independent simulations (a parameter sweep) in one operator: simulation s owns the
+ * bodies [sim_off[s], sim_off[s+1]) of `bodies` (sim_off: nsim + 1 host ints, sim_off[0] = 0,
+ * sim_off[nsim] = the body count).  eb_sweep_apply is MATVEC + FASMVP (stokesbubb.f:419-614)
+ * of every simulation at once: each simulation's layer potential sums over its own points
+ * only (the O(N^2) sum that EB_MODE_AUTO also picks for up to 2048 points) and its pressure
+ * term uses its own first body's pk (pinf = -pk(1), stokesbubb.f:546).  Geometry as for
+ * eb_matvec_create, all simulations back to back.  Replaces the reference's one process per
+ * simulation; it has no Fortran counterpart.
+ * --------------------------------------------------------------------------------------- */
+typedef struct eb_sweep_s *eb_sweep;
+
+int eb_sweep_create(eb_sweep *sw, eb_bodies bodies, int32_t nsim, const int32_t *sim_off, const double *x,
+                    const double *y, const double *dz, const double *dsdth, const double *rkappa,
+                    const double *zk, void *stream);
+int eb_sweep_apply(eb_sweep sw, const double *xx, double *yy, void *stream);
+/* za (nbody complex) and pk (nbody, after each simulation's pinf shift) of a density;
+ * either output may be NULL. */
+int eb_sweep_parts(eb_sweep sw, const double *xx, double *za, double *pk, void *stream);
+/* unknowns of each simulation: offsets into the 2*nbk vector (nsim + 1 ints, host memory) */
+const int32_t *eb_sweep_unknown_offsets(eb_sweep sw);
+int eb_sweep_destroy(eb_sweep sw);
+
 /* Drop-in for FASMVP (stokesbubb.f:496-498) on host arrays, Fortran convention (all by
  * reference): (ND, NBK, K, X, Y, DZ, DSDTH, RKAPPA, ZK, U, W, H, ZA, PK, STRESS, DENS).
  * The FMM runs with the reference's LEVMAX 7 and TOL 1e-13 (stokesbubb.f:564-567). */

@@ -2,8 +2,8 @@
 
 Runs only where the reference sources exist (they are compiled in place by oracle/Makefile);
 the fixtures it writes are data -- inputs and the reference's outputs -- and are committed so
-the tests run without the reference.  Usage:  python tests/golden/make_golden.py [atree|gmres]
-("atree" / "gmres" rewrite only the adaptive-tree / GMRES fixtures).
+the tests run without the reference.  Usage:  python tests/golden/make_golden.py [atree|gmres|sweep]
+("atree" / "gmres" / "sweep" rewrite only the adaptive-tree / GMRES / batched-sweep fixtures).
 """
 from __future__ import annotations
 
@@ -231,12 +231,50 @@ def main_gmres():
         os.chdir(cwd)
 
 
+def main_sweep():
+    """SOLVEI's solve by the reference for several single-drop simulations, each solved on its
+    own, as the reference runs one simulation per process: the batched sweep's parity data."""
+    import tempfile
+    from electricbubbles_amd.workloads import perturbed_circle
+    cwd = os.getcwd()
+    os.chdir(tempfile.mkdtemp())  # the reference writes fort.13
+    try:
+        sim = oracle.RefSim(oracle.LIBREF)
+        nds = [128, 130, 256, 322, 512, 578]
+        extra = [(3, 0.02), (4, -0.03), (5, 0.01), (3, -0.04), (6, 0.02), (2, 0.03)]
+        out = {k: [] for k in ("rku", "rhs", "u_true", "x", "za", "pk")}
+        meta, errs = [], []
+        for n, mode in zip(nds, extra):
+            nd = np.array([n], np.int32)
+            rku = perturbed_circle(n, [(2, 0.05), mode])[0].view(np.float64).copy()
+            ex = oracle.ref_extractdat(nd, rku)
+            th = 2 * np.pi * np.arange(n) / n
+            u_true = np.stack([np.cos(th) + 0.3 * np.sin(2 * th), 0.5 * np.sin(th) - 0.2 * np.cos(3 * th)], 1).ravel()
+            sim.setup(nd)
+            rhs = oracle.ref_fasmvp(nd, ex["x"], ex["y"], ex["dz"], ex["dsda"], ex["rkappa"], ex["zk"], u_true)[0]
+            z = ex["x"] + 1j * ex["y"]
+            x, it, err, ierr = oracle.ref_dgmres_stokes(nd, z, ex["dz"], ex["dsda"], ex["rkappa"], ex["zk"], rhs)
+            _, _, za, pk = oracle.ref_fasmvp(nd, ex["x"], ex["y"], ex["dz"], ex["dsda"], ex["rkappa"], ex["zk"], x)
+            print("sweep drop", n, "reference GMRES iterations", it, "err", err, "ierr", ierr)
+            for k, v in (("rku", rku), ("rhs", rhs), ("u_true", u_true), ("x", x), ("za", za), ("pk", pk)):
+                out[k].append(np.asarray(v))
+            meta.append([it, ierr])
+            errs.append(err)
+        save("gmres_sweep", nd=np.array(nds, np.int32), meta=np.array(meta, np.int32), err=np.array(errs),
+             **{k: np.concatenate(v) for k, v in out.items()})
+    finally:
+        os.chdir(cwd)
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["atree"]:
         main_atree()
     elif sys.argv[1:] == ["gmres"]:
         main_gmres()
+    elif sys.argv[1:] == ["sweep"]:
+        main_sweep()
     else:
         main()
         main_atree()
         main_gmres()
+        main_sweep()

@@ -1,0 +1,153 @@
+"""GPU parity of the batched sweep: many independent simulations in one operator
+(include/eb_spectral.h, eb_sweep_*) solved by the batched GMRES (include/eb_gmres.h).
+
+Checkers: the reference's SOLVEI solve of each single-drop simulation on its own
+(tests/golden/gmres_sweep.npz, written by tests/golden/make_golden.py sweep), and the plain
+per-simulation GPU path (spectral.MatVec + gmres.stokes_solve on each simulation alone).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from electricbubbles_amd.workloads import many_drops_cfg, perturbed_circle
+from tests.conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as _t
+    if not _t.cuda.is_available():
+        pytest.skip("no GPU")
+    return _t
+
+
+@pytest.fixture(scope="module")
+def mods(torch):
+    from electricbubbles_amd import gmres, spectral
+    return gmres, spectral
+
+
+def _sweep(torch, sp, nd, rku, sim_off):
+    B = sp.Bodies(nd)
+    geo = B.extractdat(torch.tensor(rku, device="cuda"))
+    sw = sp.Sweep(B, sim_off, geo["x"], geo["y"], geo["dz"], geo["dsda"], geo["rkappa"], geo["zk"])
+    return B, geo, sw
+
+
+def test_sweep_solve_matches_reference(torch, mods):
+    gm, sp = mods
+    f = np.load(os.path.join(GOLDEN, "gmres_sweep.npz"))
+    nd = f["nd"]
+    print("sweep vs reference, drops", list(nd), flush=True)
+    B, geo, sw = _sweep(torch, sp, nd, f["rku"], np.arange(nd.size + 1))
+    gb = gm.GMRESBatch(sw.unknown_offsets, 100)
+    rhs = torch.tensor(f["rhs"], device="cuda")
+    soln, za, pk, it, err, ierr = gm.sweep_solve(sw, gb, rhs)
+    x = soln.cpu().numpy()
+    uo = sw.unknown_offsets
+    for s in range(nd.size):
+        it_ref, ierr_ref = f["meta"][s]
+        assert ierr[s] == ierr_ref == 0 and abs(int(it[s]) - int(it_ref)) <= 1, (s, it[s], it_ref)
+        assert err[s] <= 1e-10
+        seg = slice(uo[s], uo[s + 1])
+        assert np.linalg.norm(x[seg] - f["x"][seg]) <= 1e-9 * np.linalg.norm(f["x"][seg]), s
+        assert np.linalg.norm(x[seg] - f["u_true"][seg]) <= 1e-9 * np.linalg.norm(f["u_true"][seg]), s
+    assert np.abs(za.cpu().numpy() - f["za"]).max() <= 1e-9 * 2 * np.pi * np.abs(x).max()
+    assert np.abs(pk.cpu().numpy() - f["pk"]).max() <= 1e-9 * max(1.0, np.abs(f["pk"]).max())
+    ax = sw.apply(soln).cpu().numpy()
+    for s in range(nd.size):
+        seg = slice(uo[s], uo[s + 1])
+        assert np.linalg.norm(ax[seg] - f["rhs"][seg]) <= 1.5e-10 * np.linalg.norm(f["rhs"][seg]), s
+    gb.close()
+
+
+def _mixed():
+    """Simulations of one and of several drops, with uneven lengths (prime factors included)."""
+    rng = np.random.default_rng(11)
+    nds, zs, sim_off = [], [], [0]
+    for s, nb in enumerate([1, 2, 1, 3, 1]):
+        for b in range(nb):
+            n = int(rng.choice([128, 130, 194, 256, 322, 400]))
+            c = (4.0 * b) + 0.5j * b
+            modes = [(2, 0.05), (int(rng.integers(3, 7)), rng.uniform(-0.03, 0.03))]
+            zs.append(perturbed_circle(n, modes, center=c, radius=rng.uniform(0.7, 1.3))[0])
+            nds.append(n)
+        sim_off.append(sim_off[-1] + nb)
+    return np.array(nds, np.int32), np.concatenate(zs).view(np.float64).copy(), np.array(sim_off, np.int32)
+
+
+def test_sweep_apply_is_the_block_diagonal_of_plain_matvecs(torch, mods):
+    gm, sp = mods
+    nd, rku, sim_off = _mixed()
+    B, geo, sw = _sweep(torch, sp, nd, rku, sim_off)
+    u = torch.tensor(np.random.default_rng(3).normal(size=2 * int(nd.sum())), device="cuda")
+    y = sw.apply(u).cpu().numpy()
+    off = np.concatenate([[0], np.cumsum(nd)])
+    for s in range(sim_off.size - 1):
+        b0, b1 = sim_off[s], sim_off[s + 1]
+        p0, p1 = off[b0], off[b1]
+        Bs = sp.Bodies(nd[b0:b1])
+        g = {k: geo[k][p0:p1] for k in ("x", "y", "dz", "dsda", "rkappa")}
+        mv = sp.MatVec(Bs, g["x"], g["y"], g["dz"], g["dsda"], g["rkappa"], geo["zk"][b0:b1], mode=2)  # direct sum
+        ys = mv.apply(u[2 * p0:2 * p1]).cpu().numpy()
+        assert np.linalg.norm(y[2 * p0:2 * p1] - ys) <= 1e-12 * np.linalg.norm(ys), s
+        mv.close()
+    za, pk = sw.parts(u)
+    assert np.all(pk.cpu().numpy()[sim_off[:-1]] == 0.0)  # each simulation's first body sets its pinf
+
+
+def test_sweep_solve_matches_plain_solves(torch, mods):
+    gm, sp = mods
+    nd, rku, sim_off = _mixed()
+    B, geo, sw = _sweep(torch, sp, nd, rku, sim_off)
+    th = np.concatenate([2 * np.pi * np.arange(m) / m for m in nd])
+    u_true = torch.tensor(np.stack([np.cos(th), 0.5 * np.sin(2 * th)], 1).ravel(), device="cuda")
+    rhs = sw.apply(u_true)
+    gb = gm.GMRESBatch(sw.unknown_offsets, 100)
+    soln, za, pk, it, err, ierr = gm.sweep_solve(sw, gb, rhs)
+    off = np.concatenate([[0], np.cumsum(nd)])
+    for s in range(sim_off.size - 1):
+        b0, b1 = sim_off[s], sim_off[s + 1]
+        p0, p1 = off[b0], off[b1]
+        Bs = sp.Bodies(nd[b0:b1])
+        g = {k: geo[k][p0:p1] for k in ("x", "y", "dz", "dsda", "rkappa")}
+        mv = sp.MatVec(Bs, g["x"], g["y"], g["dz"], g["dsda"], g["rkappa"], geo["zk"][b0:b1], mode=2)
+        solver = gm.GMRES(2 * (p1 - p0), 100)
+        xs, zas, pks, its, errs, ierrs = gm.stokes_solve(mv, solver, rhs[2 * p0:2 * p1].contiguous())
+        assert ierr[s] == ierrs == 0 and abs(int(it[s]) - its) <= 1, (s, it[s], its)
+        a, b = soln[2 * p0:2 * p1].cpu().numpy(), xs.cpu().numpy()
+        assert np.linalg.norm(a - b) <= 1e-9 * np.linalg.norm(b), s
+        assert np.abs(pk.cpu().numpy()[b0:b1] - pks.cpu().numpy()).max() <= 1e-9 * max(1.0, np.abs(pks.cpu().numpy()).max())
+        solver.close()
+        mv.close()
+    res = torch.linalg.norm(sw.apply(soln) - rhs) / torch.linalg.norm(rhs)
+    assert float(res) <= 1.5e-10
+    assert float(torch.linalg.norm(soln - u_true) / torch.linalg.norm(u_true)) <= 1e-8
+    gb.close()
+
+
+def test_sweep_512_single_drops(torch, mods):
+    """The batched-sweep shapes: 512 single drops r = 1 + 0.05 cos 2 theta, N log-uniform in
+    128..2048 rounded to even; every simulation converges, its residual below the tolerance."""
+    gm, sp = mods
+    rng = np.random.default_rng(512)
+    nd = (np.round(np.exp(rng.uniform(np.log(128), np.log(2048), 512)) / 2) * 2).astype(np.int32)
+    rku = np.concatenate([perturbed_circle(int(n), [(2, 0.05)])[0] for n in nd]).view(np.float64).copy()
+    print("sweep 512 drops, points", int(nd.sum()), flush=True)
+    B, geo, sw = _sweep(torch, sp, nd, rku, np.arange(nd.size + 1))
+    th = np.concatenate([2 * np.pi * np.arange(m) / m for m in nd])
+    u_true = torch.tensor(np.stack([np.cos(th), 0.5 * np.sin(2 * th)], 1).ravel(), device="cuda")
+    rhs = sw.apply(u_true)
+    gb = gm.GMRESBatch(sw.unknown_offsets, 100)
+    soln, za, pk, it, err, ierr = gm.sweep_solve(sw, gb, rhs)
+    assert np.all(ierr == 0) and np.all(err <= 1e-10) and it.min() >= 2
+    ax = sw.apply(soln).cpu().numpy()
+    r = rhs.cpu().numpy()
+    uo = sw.unknown_offsets
+    for s in range(nd.size):
+        seg = slice(uo[s], uo[s + 1])
+        assert np.linalg.norm(ax[seg] - r[seg]) <= 1.5e-10 * np.linalg.norm(r[seg]), s
+    gb.close()
