@@ -760,36 +760,51 @@ __global__ void k_sw_charges(int n, const double2* __restrict__ dens, const doub
 }
 
 // GW of every point from the other points of its own simulation (SLDIRDR / SLSELFDR,
-// cadapdrop2.f:564-620, 724-769).  One 256-thread block per tile = (simulation, 256
-// targets); the simulation's sources pass through LDS in chunks of 256.  Tiles come in order
-// of decreasing simulation size.
-constexpr int SW_T = 256;
+// cadapdrop2.f:564-620, 724-769).  One 128-thread block per tile = (simulation, 256
+// targets), two targets per thread; the simulation's sources pass through LDS in chunks of
+// 256 aligned with the tiles, so only the tile's own chunk holds the targets themselves and
+// runs the masked pair.  Tiles come in order of decreasing simulation size.
+constexpr int SW_T = 128, SW_TILE = 2 * SW_T;
 __global__ __launch_bounds__(SW_T) void k_sw_gw(const int2* __restrict__ tiles, const int* __restrict__ soff,
                                                 const double* __restrict__ x, const double* __restrict__ y,
                                                 const double2* __restrict__ qa, const double2* __restrict__ qb,
                                                 double2* __restrict__ gw) {
-  __shared__ double2 sz[SW_T], sa[SW_T], sq[SW_T];
+  __shared__ double2 sz[SW_TILE], sa[SW_TILE], sq[SW_TILE];
   const int2 tl = tiles[blockIdx.x];  // (simulation, first target within it)
   const int o = soff[tl.x], n = soff[tl.x + 1] - o;
-  const int it = tl.y + (int)threadIdx.x;
-  const bool act = it < n;
-  const int i = o + (act ? it : 0);
-  const double2 zi = make_double2(x[i], y[i]);
-  double2 acc1 = make_double2(0, 0), acc2 = make_double2(0, 0);
-  for (int c0 = 0; c0 < n; c0 += SW_T) {
-    const int j = c0 + (int)threadIdx.x;
+  const int it0 = tl.y + (int)threadIdx.x, it1 = it0 + SW_T;
+  const bool act0 = it0 < n, act1 = it1 < n;
+  const double2 z0 = act0 ? make_double2(x[o + it0], y[o + it0]) : make_double2(0, 0);
+  const double2 z1 = act1 ? make_double2(x[o + it1], y[o + it1]) : z0;
+  double2 a10 = make_double2(0, 0), a20 = make_double2(0, 0), a11 = make_double2(0, 0), a21 = make_double2(0, 0);
+  for (int c0 = 0; c0 < n; c0 += SW_TILE) {
     __syncthreads();
-    if (j < n) {
-      sz[threadIdx.x] = make_double2(x[o + j], y[o + j]);
-      sa[threadIdx.x] = qa[o + j];
-      sq[threadIdx.x] = qb[o + j];
+    for (int f = threadIdx.x; f < SW_TILE; f += SW_T) {
+      const int j = c0 + f;
+      if (j < n) {
+        sz[f] = make_double2(x[o + j], y[o + j]);
+        sa[f] = qa[o + j];
+        sq[f] = qb[o + j];
+      }
     }
     __syncthreads();
-    const int m = min(SW_T, n - c0);
-    if (act)
-      for (int jj = 0; jj < m; ++jj) p2p_pair<true, false>(zi, sz[jj], sa[jj], sq[jj], c0 + jj == it, acc1, acc2);
+    const int m = min(SW_TILE, n - c0);
+    if (c0 == tl.y) {
+      const int s0 = threadIdx.x, s1 = s0 + SW_T;  // the targets' own positions in this chunk
+      for (int jj = 0; jj < m; ++jj) {
+        p2p_pair<true, true>(z0, sz[jj], sa[jj], sq[jj], jj == s0, a10, a20);
+        p2p_pair<true, true>(z1, sz[jj], sa[jj], sq[jj], jj == s1 || !act1, a11, a21);
+      }
+    } else {
+      for (int jj = 0; jj < m; ++jj) {
+        p2p_pair<false, true>(z0, sz[jj], sa[jj], sq[jj], false, a10, a20);
+        p2p_pair<false, true>(z1, sz[jj], sa[jj], sq[jj], false, a11, a21);
+      }
+    }
   }
-  if (act) gw[i] = make_double2(acc1.x - acc2.x, acc1.y - acc2.y);  // DR: GW = phi - z conj(phi') - conj(psi)
+  // DR: GW = phi - z conj(phi') - conj(psi)
+  if (act0) gw[o + it0] = make_double2(a10.x - a20.x, a10.y - a20.y);
+  if (act1) gw[o + it1] = make_double2(a11.x - a21.x, a11.y - a21.y);
 }
 
 // FASMVP's pointwise part (stokesbubb.f:585-611) with pinf = -pk of the simulation's first body
@@ -875,7 +890,7 @@ extern "C" int eb_sweep_create(eb_sweep* out, eb_bodies B, int32_t nsim, const i
   std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return soff[a + 1] - soff[a] > soff[b + 1] - soff[b]; });
   std::vector<int2> tiles;
   for (int s : ord)
-    for (int t0 = 0; t0 < soff[s + 1] - soff[s]; t0 += SW_T) tiles.push_back(make_int2(s, t0));
+    for (int t0 = 0; t0 < soff[s + 1] - soff[s]; t0 += SW_TILE) tiles.push_back(make_int2(s, t0));
   S->ntile = (int)tiles.size();
   std::vector<double> h(nb);
   for (int k = 0; k < nb; ++k) h[k] = 2.0 * 3.14159265358979323846 / B->nd[k];  // stokesbubb.f:446-448
