@@ -1,0 +1,549 @@
+// eb_field.hip -- F on the device (include/eb_field.h): the Laplace solve, the
+// Dirichlet-to-Neumann map, SETVELF and INTERFACENEW around the Stokes solve.
+//
+// Every layer potential runs on the GPU FMM (eb_fmm_plan): the Laplace operator's Hilbert
+// product (DAPIF2, IFLAG7 = 3) and PVINTEV's on the Hilbert variant, INTERFACENEW's CADAPMOD on
+// the MOD variant, SOLVEI's CADAPDR inside eb_matvec.  Per-body spectral work (interpolation
+// to twice the points, FDIFFF, FOURINT) runs one workgroup per body with the in-LDS FFT.
+#include <math.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "../../include/eb_field.h"
+#include "../../include/eb_gmres.h"
+#include "eb_bodies.h"
+
+using namespace eb;
+
+namespace {
+
+constexpr double PI = 3.14159265358979323846;
+constexpr int EW = 256;  // threads of the pointwise kernels
+
+inline int ew_blocks(int n) { return (n + EW - 1) / EW; }
+
+// NTERMS = -NINT(ln TOL / ln 2) (cadapdrop.f:126)
+inline int nterms_for(double tol) { return -(int)lround(log(tol) / log(2.0)); }
+
+// ------------------------------------------------------------------------------------------
+// Laplace operator FASMVP_LAP (laplace.f:159-262)
+// ------------------------------------------------------------------------------------------
+
+__global__ void k_real_dens(int n, const double* __restrict__ u, double2* __restrict__ d) {
+  const int i = blockIdx.x * EW + threadIdx.x;
+  if (i < n) d[i] = make_double2(u[i], 0.0);
+}
+
+// w = u + 2 (self u - Re(CF / (2 pi i))) - 2 h u dsdth, self = h kappa dsdth / (4 pi)
+// (laplace.f:225-237); CF = the Hilbert product of the charges u dz h
+__global__ void k_lap_point(int n, const int* __restrict__ body_of, const double* __restrict__ h,
+                            const double* __restrict__ u, const double2* __restrict__ cf,
+                            const double* __restrict__ rkappa, const double* __restrict__ dsdth,
+                            double* __restrict__ w) {
+  const int i = blockIdx.x * EW + threadIdx.x;
+  if (i >= n) return;
+  const double hs = h[body_of[i]];
+  const double self = 0.25 * hs * rkappa[i] * dsdth[i] / PI;
+  // Re(z2pii * cf) with z2pii = 1 / (2 pi i) = -i / (2 pi)
+  const double zc = self * u[i] - cf[i].y / (2.0 * PI);
+  const double wi = u[i] + 2.0 * zc;
+  const double du = hs * u[i] * dsdth[i];
+  w[i] = wi - 2.0 * du;
+}
+
+// GETBCs (laplace.f:14-18): rhs = -2 Re z
+__global__ void k_getbcs(int n, const double* __restrict__ x, double* __restrict__ rhs, double* __restrict__ soln) {
+  const int i = blockIdx.x * EW + threadIdx.x;
+  if (i >= n) return;
+  const double r = -2.0 * x[i];
+  rhs[i] = r;
+  soln[i] = r;  // SOLVE_LAPL's initial guess soln = rhs (laplace.f:375-378)
+}
+
+// ------------------------------------------------------------------------------------------
+// PVINTEV (pv_integral.f:53-212) and DIRICH_NEU_MAP (laplace.f:266-332)
+// ------------------------------------------------------------------------------------------
+
+// The spectrum of a length-N sequence in a (after the forward transform) becomes FINTERC's
+// interpolant on 2N points (pv_integral.f:268-312): coefficients scaled by 1/N, the one at
+// N/2 set to zero, the rest zero-padded.  Its values at the even fine points are the inverse
+// length-N transform of those coefficients; at the odd fine points (the midpoints) the same
+// with coefficient k' multiplied by exp(i pi k' / N), k' the signed wavenumber.
+__device__ __forceinline__ void interp_coeffs(double2* a, double2* c, int N) {
+  const double invn = 1.0 / N;
+  const int nyk = N / 2;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    double2 v = make_double2(a[i].x * invn, a[i].y * invn);
+    if (i == nyk) v = make_double2(0.0, 0.0);
+    a[i] = v;
+    const int ks = (i <= nyk) ? i : i - N;  // signed wavenumber
+    double sn, cs;
+    sincospi((double)ks / (double)N, &sn, &cs);
+    c[i] = make_double2(v.x * cs - v.y * sn, v.x * sn + v.y * cs);
+  }
+  __syncthreads();
+}
+
+// Per body: the fine coordinates x2, y2 (2N points: even = the interpolated original points,
+// odd = the midpoints) and the fine density: ucmplx = u dz interpolated at the midpoints, 0 at
+// the original points (QA of pv_integral.f:160-163 is U2 h there, with h the FMM weight).
+__global__ __launch_bounds__(SP_THREADS) void k_pv_interp(BodyView B, const double* __restrict__ x,
+                                                           const double* __restrict__ y,
+                                                           const double* __restrict__ ulap,
+                                                           const double2* __restrict__ dz, double* __restrict__ x2,
+                                                           double* __restrict__ y2, double2* __restrict__ uf) {
+  extern __shared__ double2 lds[];
+  const BodyCtx bc = body_ctx(B);
+  const int N = bc.N, o0 = bc.o0, f0 = 2 * bc.o0;
+  double2* a = lds;
+  double2* c = lds + B.nmax;
+  double2* ws = lds + 2 * B.nmax;
+  // the curve, as one complex signal x + i y (FINTER of x and of y, pv_integral.f:156-159)
+  for (int i = threadIdx.x; i < N; i += blockDim.x) a[i] = make_double2(x[o0 + i], y[o0 + i]);
+  __syncthreads();
+  fft(a, ws, bc, B, -1);
+  interp_coeffs(a, c, N);
+  fft(a, ws, bc, B, +1);
+  fft(c, ws, bc, B, +1);
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    x2[f0 + 2 * i] = a[i].x;
+    y2[f0 + 2 * i] = a[i].y;
+    x2[f0 + 2 * i + 1] = c[i].x;
+    y2[f0 + 2 * i + 1] = c[i].y;
+  }
+  __syncthreads();
+  // the density u dz (FINTERC, pv_integral.f:154-155), needed at the midpoints only
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    const double2 d = dz[o0 + i];
+    const double u = ulap[o0 + i];
+    a[i] = make_double2(u * d.x, u * d.y);
+  }
+  __syncthreads();
+  fft(a, ws, bc, B, -1);
+  interp_coeffs(a, c, N);
+  fft(c, ws, bc, B, +1);
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    uf[f0 + 2 * i] = make_double2(0.0, 0.0);
+    uf[f0 + 2 * i + 1] = c[i];
+  }
+}
+
+// fine weights: H(body) at the midpoints, 0 at the original points
+__global__ void k_pv_weights(BodyView B, double2* __restrict__ wf) {
+  const int k = blockIdx.x;
+  const int N = B.nd[k], f0 = 2 * B.off[k];
+  const double h = 2.0 * PI / N;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    wf[f0 + 2 * i] = make_double2(0.0, 0.0);
+    wf[f0 + 2 * i + 1] = make_double2(h, 0.0);
+  }
+}
+
+// DIRICH_NEU_MAP (laplace.f:297-325): PHICMPLX = -z2pii CFIELD at the original points,
+// zphi = -u/2 - PHICMPLX - z, phi = Re, psi = Im, dphi_dn = -Re(FDIFFF(psi)) / dsdth
+__global__ __launch_bounds__(SP_THREADS) void k_dn_post(BodyView B, const double* __restrict__ x,
+                                                         const double* __restrict__ y,
+                                                         const double* __restrict__ ulap,
+                                                         const double2* __restrict__ cf2,
+                                                         const double* __restrict__ dsdth, double* __restrict__ phi,
+                                                         double* __restrict__ psi, double* __restrict__ dphi) {
+  extern __shared__ double2 lds[];
+  const BodyCtx bc = body_ctx(B);
+  const int N = bc.N, o0 = bc.o0, f0 = 2 * bc.o0;
+  double2* a = lds;
+  double2* ws = lds + B.nmax;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    const double2 cf = cf2[f0 + 2 * i];
+    // PHICMPLX = -z2pii CF = (i / (2 pi)) CF
+    const double2 pc = make_double2(-cf.y / (2.0 * PI), cf.x / (2.0 * PI));
+    const double zr = -0.5 * ulap[o0 + i] - pc.x - x[o0 + i];
+    const double zi = -pc.y - y[o0 + i];
+    if (phi) phi[o0 + i] = zr;
+    if (psi) psi[o0 + i] = zi;
+    a[i] = make_double2(zi, 0.0);
+  }
+  __syncthreads();
+  fft(a, ws, bc, B, -1);
+  const double invn = 1.0 / N;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    const double w = fdiff_k(i, N) * invn;
+    const double2 v = a[i];
+    a[i] = make_double2(-w * v.y, w * v.x);
+  }
+  __syncthreads();
+  fft(a, ws, bc, B, +1);
+  for (int i = threadIdx.x; i < N; i += blockDim.x) dphi[o0 + i] = -a[i].x / dsdth[o0 + i];
+}
+
+// ------------------------------------------------------------------------------------------
+// SETVELF (stokesbubb.f:4-135) with zbond = 0 and iwall = 0: the bond-number and C_k terms
+// are exactly zero then (zfi = 0, zc = 0), which leaves
+//   rhs = -dz / (2 dsda) - zstr + (i E0 / 4) (alpha zf1 + zf(i) - zf(1)),
+// zstr = -conj(zpsi), zpsi = -i cap z / 2 (stokesbubb.f:63-67), zf = the FOURINT integral of
+// FFT(dz dphi_dn^2) / N, and alpha the value the C_k loop left behind (stokesbubb.f:94):
+// 2 pi (N_k - 1) / N_k of the last body.
+// ------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(SP_THREADS) void k_setvelf(BodyView B, const double* __restrict__ x,
+                                                         const double* __restrict__ y, const double2* __restrict__ dz,
+                                                         const double* __restrict__ dsda,
+                                                         const double* __restrict__ dphi, double cap, double e0,
+                                                         double alpha_last, double2* __restrict__ rhs) {
+  extern __shared__ double2 lds[];
+  const BodyCtx bc = body_ctx(B);
+  const int N = bc.N, o0 = bc.o0, n = N / 2;
+  double2* a = lds;
+  double2* c = lds + B.nmax;
+  double2* ws = lds + 2 * B.nmax;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    const double d2 = dphi[o0 + i] * dphi[o0 + i];
+    const double2 d = dz[o0 + i];
+    a[i] = make_double2(d.x * d2, d.y * d2);
+  }
+  __syncthreads();
+  fft(a, ws, bc, B, -1);
+  const double invn = 1.0 / N;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    const double2 v = make_double2(a[i].x * invn, a[i].y * invn);
+    a[i] = v;
+    // FOURINT (electric.f:1307-1331)
+    double2 r = make_double2(0, 0);
+    if (i >= 1 && i <= n - 1) {
+      const double ii = (double)i;
+      r = make_double2(v.y / ii, -v.x / ii);
+    } else if (i >= N - n + 1 && i <= N - 1) {
+      const double ii = (double)(N - i);
+      r = make_double2(-v.y / ii, v.x / ii);
+    }
+    c[i] = r;
+  }
+  __syncthreads();
+  fft(c, ws, bc, B, +1);
+  const double2 zf1 = a[0], c0 = c[0];
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    const int g = o0 + i;
+    const double2 d = dz[g];
+    const double s = dsda[g];
+    // zn = -0.5 dz / dsda; rhs = zn + zgrav (= 0) - zstr, zstr = -conj(zpsi)
+    const double2 zn = make_double2(-0.5 * d.x / s, -0.5 * d.y / s);
+    const double hc = 0.5 * cap;
+    double rx = zn.x + hc * y[g];
+    double ry = zn.y + hc * x[g];
+    // electric term: (i E0 / 4) (alpha zf1 + zf(i) - zf(1))
+    const double2 zs = make_double2(alpha_last * zf1.x + c[i].x - c0.x, alpha_last * zf1.y + c[i].y - c0.y);
+    const double q = 0.25 * e0;
+    rx += -q * zs.y;
+    ry += q * zs.x;
+    rhs[g] = make_double2(rx, ry);
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// INTERFACENEW (electric.f:851-878) with zc = 0 and iwall = 0
+// ------------------------------------------------------------------------------------------
+__global__ void k_interface_point(int n, const int* __restrict__ body_of, const double* __restrict__ h,
+                                  const double* __restrict__ x, const double* __restrict__ y,
+                                  const double2* __restrict__ dz, const double* __restrict__ dsda,
+                                  const double* __restrict__ rkappa, const double2* __restrict__ ckern,
+                                  const double2* __restrict__ u, const double2* __restrict__ udot,
+                                  const double2* __restrict__ gw, double cap, double* __restrict__ un) {
+  const int i = blockIdx.x * EW + threadIdx.x;
+  if (i >= n) return;
+  const double hk = h[body_of[i]];
+  const double2 d = dz[i];
+  const double s = dsda[i];
+  const double2 zn = make_double2(d.y / s, -d.x / s);  // -i dz / dsda
+  const double2 ui = u[i], ud = udot[i];
+  // zcorr = (h Re(ckern) u - 2 h udot) / (2 pi i)
+  const double r1 = 0.5 * hk * 2.0 * ckern[i].x;
+  const double2 t = make_double2(r1 * ui.x - 2.0 * hk * ud.x, r1 * ui.y - 2.0 * hk * ud.y);
+  const double2 zcorr = make_double2(t.y / (2.0 * PI), -t.x / (2.0 * PI));
+  // zself = conj(u) h |dz| kappa zn^2 / (2 pi)
+  const double f = hk * hypot(d.x, d.y) * 0.5 * rkappa[i];
+  const double2 cu = make_double2(ui.x * f, -ui.y * f);
+  const double2 zn2 = cmul(zn, zn);
+  const double2 zs = cmul(cu, zn2);
+  const double2 zself = make_double2(zs.x / PI, zs.y / PI);
+  double2 g = gw[i];
+  g = make_double2(g.x + zself.x + zcorr.x, g.y + zself.y + zcorr.y);
+  // gradwr = conj(zpsi), zpsi = -i cap z / 2
+  const double hc = 0.5 * cap;
+  g = make_double2(g.x + hc * y[i], g.y + hc * x[i]);
+  // zvel = -i gradw; Un = Re(zvel conj(zn))
+  const double2 zv = make_double2(g.y, -g.x);
+  un[i] = zv.x * zn.x + zv.y * zn.y;
+}
+
+__global__ void k_body_of_f(const int* __restrict__ off, int nbody, int* __restrict__ body_of) {
+  const int k = blockIdx.x;
+  for (int i = off[k] + threadIdx.x; i < off[k + 1]; i += blockDim.x) body_of[i] = k;
+}
+
+__global__ void k_wdz_f(int n, const double2* __restrict__ dz, const double* __restrict__ h,
+                        const int* __restrict__ body_of, double2* __restrict__ wdz) {
+  const int i = blockIdx.x * EW + threadIdx.x;
+  if (i >= n) return;
+  const double hk = h[body_of[i]];
+  wdz[i] = make_double2(dz[i].x * hk, dz[i].y * hk);
+}
+
+}  // namespace
+
+struct eb_field_s {
+  eb_bodies B = nullptr;
+  int k = 0, n = 0;
+  double cap = 0.0, e0 = 0.0, alpha_last = 0.0;
+  hipStream_t st = nullptr;
+  bool have_geom = false;
+  int nt_lap = 37, nt_pv = 33, nt_mod = 43;  // TOL 1e-11 (laplace.f:212), 1e-10 (pv_integral.f:175), 1e-13
+  // geometry
+  double *x = nullptr, *y = nullptr, *tha = nullptr, *dsda = nullptr, *rkappa = nullptr, *area = nullptr,
+         *xls = nullptr, *h = nullptr;
+  double2 *dz = nullptr, *zk = nullptr, *ckern = nullptr, *wdz = nullptr;
+  int* body_of = nullptr;
+  // Laplace
+  eb_fmm_plan lap = nullptr, pv = nullptr;
+  eb_gmres glap = nullptr;
+  double2 *dens = nullptr, *cf = nullptr, *uf = nullptr, *wf = nullptr, *cf2 = nullptr;
+  double *x2 = nullptr, *y2 = nullptr, *rhs_lap = nullptr, *ulap = nullptr, *phi = nullptr, *psi = nullptr,
+         *dphi = nullptr;
+  // Stokes and the interface velocity
+  eb_matvec mv = nullptr;
+  eb_gmres gst = nullptr;
+  eb_fmm_plan itf = nullptr;
+  double *rhs_st = nullptr, *pk = nullptr, *un = nullptr;
+  double2 *u = nullptr, *za = nullptr, *udot = nullptr, *gw = nullptr;
+};
+
+static void free_field(eb_field F) {
+  if (!F) return;
+  if (F->st) (void)hipStreamSynchronize(F->st);
+  if (F->lap) eb_fmm_plan_destroy(F->lap);
+  if (F->pv) eb_fmm_plan_destroy(F->pv);
+  if (F->itf) eb_fmm_plan_destroy(F->itf);
+  if (F->mv) eb_matvec_destroy(F->mv);
+  if (F->glap) eb_gmres_destroy(F->glap);
+  if (F->gst) eb_gmres_destroy(F->gst);
+  void* p[] = {F->x,    F->y,   F->tha,  F->dsda, F->rkappa, F->area,   F->xls,  F->h,   F->dz,   F->zk,
+               F->ckern, F->wdz, F->body_of, F->dens, F->cf,  F->uf,     F->wf,   F->cf2, F->x2,   F->y2,
+               F->rhs_lap, F->ulap, F->phi, F->psi, F->dphi, F->rhs_st, F->pk, F->un, F->u, F->za, F->udot, F->gw};
+  for (void* q : p)
+    if (q) (void)hipFree(q);
+  delete F;
+}
+
+extern "C" int eb_field_create(eb_field* out, eb_bodies B, double cap, double e0, void* stream) {
+  if (!out || !B) return (eb_set_error("eb_field_create: bad arguments"), EB_ERR_ARG);
+  *out = nullptr;
+  for (int k = 0; k < B->nbody; ++k)
+    if (B->nd[k] > 2048)
+      return (eb_set_error("eb_field_create: body %d has %d points (PVINTEV's interpolation takes up to 2048)", k,
+                           B->nd[k]),
+              EB_ERR_ARG);
+  eb_field F = new eb_field_s();
+  F->B = B;
+  F->k = B->nbody;
+  F->n = B->n;
+  F->cap = cap;
+  F->e0 = e0;
+  F->st = (hipStream_t)stream;
+  const int nl = B->nd[B->nbody - 1];
+  F->alpha_last = 2.0 * PI * (nl - 1.0) / nl;  // stokesbubb.f:94, last body, last point
+  F->nt_lap = nterms_for(1.0e-11);
+  F->nt_pv = nterms_for(1.0e-10);
+  F->nt_mod = nterms_for(1.0e-13);
+  const int n = F->n, k = F->k;
+  std::vector<double> h(k);
+  for (int b = 0; b < k; ++b) h[b] = 2.0 * PI / B->nd[b];
+  int rc;
+  if ((rc = dalloc(&F->x, n)) || (rc = dalloc(&F->y, n)) || (rc = dalloc(&F->tha, n)) || (rc = dalloc(&F->dsda, n)) ||
+      (rc = dalloc(&F->rkappa, n)) || (rc = dalloc(&F->area, k)) || (rc = dalloc(&F->xls, 2)) ||
+      (rc = dalloc(&F->h, k)) || (rc = dalloc(&F->dz, n)) || (rc = dalloc(&F->zk, k)) || (rc = dalloc(&F->ckern, n)) ||
+      (rc = dalloc(&F->wdz, n)) || (rc = dalloc(&F->body_of, n)) || (rc = dalloc(&F->dens, n)) ||
+      (rc = dalloc(&F->cf, n)) || (rc = dalloc(&F->uf, 2 * (size_t)n)) || (rc = dalloc(&F->wf, 2 * (size_t)n)) ||
+      (rc = dalloc(&F->cf2, 2 * (size_t)n)) || (rc = dalloc(&F->x2, 2 * (size_t)n)) ||
+      (rc = dalloc(&F->y2, 2 * (size_t)n)) || (rc = dalloc(&F->rhs_lap, n)) || (rc = dalloc(&F->ulap, n)) ||
+      (rc = dalloc(&F->phi, n)) || (rc = dalloc(&F->psi, n)) || (rc = dalloc(&F->dphi, n)) ||
+      (rc = dalloc(&F->rhs_st, 2 * (size_t)n)) || (rc = dalloc(&F->pk, k)) || (rc = dalloc(&F->un, n)) ||
+      (rc = dalloc(&F->u, n)) || (rc = dalloc(&F->za, k)) || (rc = dalloc(&F->udot, n)) || (rc = dalloc(&F->gw, n))) {
+    free_field(F);
+    return rc;
+  }
+  if (!upload(F->h, h, F->st)) {
+    free_field(F);
+    return (eb_set_error("eb_field_create: upload failed"), EB_ERR_HIP);
+  }
+  hipLaunchKernelGGL(k_body_of_f, dim3(k), dim3(256), 0, F->st, B->d_off, k, F->body_of);
+  hipLaunchKernelGGL(k_pv_weights, dim3(k), dim3(256), 0, F->st, view(B), F->wf);
+  if ((rc = eb_gmres_create(&F->glap, n, 100, stream)) != EB_OK ||
+      (rc = eb_gmres_create(&F->gst, 2 * n, 100, stream)) != EB_OK) {
+    free_field(F);
+    return rc;
+  }
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(F->st) != hipSuccess) {
+    free_field(F);
+    return (eb_set_error("eb_field_create: kernel failed"), EB_ERR_HIP);
+  }
+  *out = F;
+  return EB_OK;
+}
+
+extern "C" int eb_field_destroy(eb_field F) {
+  free_field(F);
+  return EB_OK;
+}
+
+// EB_MODE_AUTO's rule: the O(N^2) sum up to 2048 points, the FMM with LEVMAX 7 above
+static int plan_mode(int n) { return n <= 2048 ? EB_MODE_DIRECT : EB_MODE_FMM; }
+
+extern "C" int eb_field_set_geometry(eb_field F, const double* rku) {
+  if (!F || !rku) return (eb_set_error("eb_field_set_geometry: bad arguments"), EB_ERR_ARG);
+  const int n = F->n;
+  void* s = (void*)F->st;
+  int rc = eb_extractdat(F->B, rku, F->x, F->y, (double*)F->zk, (double*)F->dz, F->tha, F->dsda, F->rkappa,
+                         (double*)F->ckern, F->area, F->xls, s);
+  if (rc != EB_OK) return rc;
+  hipLaunchKernelGGL(k_wdz_f, dim3(ew_blocks(n)), dim3(EW), 0, F->st, n, F->dz, F->h, F->body_of, F->wdz);
+  EB_HIP_CHECK(hipGetLastError());
+  const double* wd = (const double*)F->wdz;
+  if (!F->have_geom) {
+    if ((rc = eb_fmm_plan_create(&F->lap, n, F->x, F->y, wd, 7, F->nt_lap, EB_VARIANT_HILBERT, plan_mode(n), s)) ||
+        (rc = eb_fmm_plan_create(&F->itf, n, F->x, F->y, wd, 7, F->nt_mod, EB_VARIANT_MOD, plan_mode(n), s)) ||
+        (rc = eb_matvec_create(&F->mv, F->B, F->x, F->y, (const double*)F->dz, F->dsda, F->rkappa,
+                               (const double*)F->zk, 7, F->nt_mod, EB_MODE_AUTO, s)))
+      return rc;
+  } else {
+    if ((rc = eb_fmm_plan_rebuild(F->lap, F->x, F->y, wd, s)) || (rc = eb_fmm_plan_rebuild(F->itf, F->x, F->y, wd, s)) ||
+        (rc = eb_matvec_rebuild(F->mv, F->x, F->y, (const double*)F->dz, F->dsda, F->rkappa, (const double*)F->zk, s)))
+      return rc;
+  }
+  F->have_geom = true;
+  return EB_OK;
+}
+
+static int need_geom(eb_field F, const char* what) {
+  if (!F || !F->have_geom) {
+    eb_set_error("%s: no geometry (call eb_field_set_geometry or eb_field_rhs first)", what);
+    return EB_ERR_ARG;
+  }
+  return EB_OK;
+}
+
+extern "C" int eb_field_laplace_apply(eb_field F, const double* u, double* w) {
+  int rc = need_geom(F, "eb_field_laplace_apply");
+  if (rc != EB_OK) return rc;
+  const int n = F->n;
+  hipLaunchKernelGGL(k_real_dens, dim3(ew_blocks(n)), dim3(EW), 0, F->st, n, u, F->dens);
+  if ((rc = eb_fmm_apply(F->lap, (const double*)F->dens, (double*)F->cf, (void*)F->st)) != EB_OK) return rc;
+  hipLaunchKernelGGL(k_lap_point, dim3(ew_blocks(n)), dim3(EW), 0, F->st, n, F->body_of, F->h, u, F->cf, F->rkappa,
+                     F->dsda, w);
+  EB_HIP_CHECK(hipGetLastError());
+  return EB_OK;
+}
+
+static int lap_gmres_apply(void* ctx, const double* x, double* y, void* stream) {
+  (void)stream;
+  return eb_field_laplace_apply((eb_field)ctx, x, y);
+}
+
+extern "C" int eb_field_laplace_solve(eb_field F, double* ulap, int32_t* iter, int32_t* ierr) {
+  int rc = need_geom(F, "eb_field_laplace_solve");
+  if (rc != EB_OK || !ulap) return rc != EB_OK ? rc : (eb_set_error("eb_field_laplace_solve: bad arguments"), EB_ERR_ARG);
+  const int n = F->n;
+  hipLaunchKernelGGL(k_getbcs, dim3(ew_blocks(n)), dim3(EW), 0, F->st, n, F->x, F->rhs_lap, ulap);
+  EB_HIP_CHECK(hipGetLastError());
+  int32_t it = 0, ie = 0;
+  double err = 0.0;
+  // SOLVE_LAPL: ITOL 0, TOL 1e-10, IGWK(5) = -1, MAXL 100 (laplace.f:355-384; electric.f:1035)
+  rc = eb_gmres_solve(F->glap, lap_gmres_apply, F, F->rhs_lap, ulap, 1.0e-10, -1, &it, &err, &ie);
+  if (iter) *iter = it;
+  if (ierr) *ierr = ie;
+  return rc;
+}
+
+extern "C" int eb_field_dirich_neu(eb_field F, const double* ulap, double* phi, double* psi, double* dphi_dn) {
+  int rc = need_geom(F, "eb_field_dirich_neu");
+  if (rc != EB_OK) return rc;
+  if (!ulap || !dphi_dn) return (eb_set_error("eb_field_dirich_neu: bad arguments"), EB_ERR_ARG);
+  eb_bodies B = F->B;
+  const int n2 = 2 * F->n;
+  void* s = (void*)F->st;
+  hipLaunchKernelGGL(k_pv_interp, dim3(B->nbody), dim3(SP_THREADS), lds_bytes(B, 2), F->st, view(B), F->x, F->y, ulap,
+                     F->dz, F->x2, F->y2, F->uf);
+  EB_HIP_CHECK(hipGetLastError());
+  if (!F->pv)
+    rc = eb_fmm_plan_create(&F->pv, n2, F->x2, F->y2, (const double*)F->wf, 7, F->nt_pv, EB_VARIANT_HILBERT,
+                            plan_mode(n2), s);
+  else
+    rc = eb_fmm_plan_rebuild(F->pv, F->x2, F->y2, (const double*)F->wf, s);
+  if (rc != EB_OK) return rc;
+  if ((rc = eb_fmm_apply(F->pv, (const double*)F->uf, (double*)F->cf2, s)) != EB_OK) return rc;
+  hipLaunchKernelGGL(k_dn_post, dim3(B->nbody), dim3(SP_THREADS), lds_bytes(B, 1), F->st, view(B), F->x, F->y, ulap,
+                     F->cf2, F->dsda, phi, psi, dphi_dn);
+  EB_HIP_CHECK(hipGetLastError());
+  return EB_OK;
+}
+
+extern "C" int eb_field_setvelf(eb_field F, const double* dphi_dn, double* rhs) {
+  int rc = need_geom(F, "eb_field_setvelf");
+  if (rc != EB_OK) return rc;
+  if (!dphi_dn || !rhs) return (eb_set_error("eb_field_setvelf: bad arguments"), EB_ERR_ARG);
+  eb_bodies B = F->B;
+  hipLaunchKernelGGL(k_setvelf, dim3(B->nbody), dim3(SP_THREADS), lds_bytes(B, 2), F->st, view(B), F->x, F->y, F->dz,
+                     F->dsda, dphi_dn, F->cap, F->e0, F->alpha_last, (double2*)rhs);
+  EB_HIP_CHECK(hipGetLastError());
+  return EB_OK;
+}
+
+extern "C" int eb_field_stokes_solve(eb_field F, const double* rhs, double* u, double* za, double* pk, int32_t* iter,
+                                     int32_t* ierr) {
+  int rc = need_geom(F, "eb_field_stokes_solve");
+  if (rc != EB_OK) return rc;
+  int32_t it = 0, ie = 0;
+  double err = 0.0;
+  rc = eb_stokes_solve(F->mv, F->gst, rhs, u, za, pk, &it, &err, &ie, (void*)F->st);
+  if (iter) *iter = it;
+  if (ierr) *ierr = ie;
+  return rc;
+}
+
+extern "C" int eb_field_interface(eb_field F, const double* u, double* un) {
+  int rc = need_geom(F, "eb_field_interface");
+  if (rc != EB_OK) return rc;
+  if (!u || !un) return (eb_set_error("eb_field_interface: bad arguments"), EB_ERR_ARG);
+  const int n = F->n;
+  void* s = (void*)F->st;
+  // udot = FDIFFF(u) per body (electric.f:820-825), GW = CADAPMOD (electric.f:835-837)
+  if ((rc = eb_fdifff(F->B, u, (double*)F->udot, s)) != EB_OK) return rc;
+  if ((rc = eb_fmm_apply(F->itf, u, (double*)F->gw, s)) != EB_OK) return rc;
+  hipLaunchKernelGGL(k_interface_point, dim3(ew_blocks(n)), dim3(EW), 0, F->st, n, F->body_of, F->h, F->x, F->y,
+                     F->dz, F->dsda, F->rkappa, F->ckern, (const double2*)u, F->udot, F->gw, F->cap, un);
+  EB_HIP_CHECK(hipGetLastError());
+  return EB_OK;
+}
+
+extern "C" int eb_field_rhs(eb_field F, const double* rku, double* rkup) {
+  if (!F || !rku || !rkup) return (eb_set_error("eb_field_rhs: bad arguments"), EB_ERR_ARG);
+  int rc;
+  int32_t it, ie;
+  if ((rc = eb_field_set_geometry(F, rku)) != EB_OK) return rc;
+  if ((rc = eb_field_laplace_solve(F, F->ulap, &it, &ie)) != EB_OK) return rc;
+  if ((rc = eb_field_dirich_neu(F, F->ulap, F->phi, F->psi, F->dphi)) != EB_OK) return rc;
+  if ((rc = eb_field_setvelf(F, F->dphi, F->rhs_st)) != EB_OK) return rc;
+  if ((rc = eb_field_stokes_solve(F, F->rhs_st, (double*)F->u, (double*)F->za, F->pk, &it, &ie)) != EB_OK) return rc;
+  if ((rc = eb_field_interface(F, (const double*)F->u, F->un)) != EB_OK) return rc;
+  return eb_ftail(F->B, F->tha, F->un, (const double*)F->dz, F->dsda, rkup, (void*)F->st);
+}
+
+extern "C" int eb_field_get_geometry(eb_field F, eb_field_geometry* g) {
+  if (!F || !g) return (eb_set_error("eb_field_get_geometry: bad arguments"), EB_ERR_ARG);
+  g->x = F->x;
+  g->y = F->y;
+  g->dz = (const double*)F->dz;
+  g->tha = F->tha;
+  g->dsda = F->dsda;
+  g->rkappa = F->rkappa;
+  g->ckern = (const double*)F->ckern;
+  g->zk = (const double*)F->zk;
+  return EB_OK;
+}

@@ -210,15 +210,21 @@ __global__ __launch_bounds__(256) void k_ntbox(const int* __restrict__ leaf_star
 
 // QA = DENS*DZDT*H/(2 pi i), QB = (sqb*conj(DENS)*DZDT + DENS*conj(DZDT))*H/(2 pi i)
 // (cadapdrop.f:218-220, cadap.f:215), in sorted order.  QC = -conj(z) QA is formed where used.
+// The Hilbert product takes QA = -DENS*WDZ (its sum runs over 1/(z_i - z_j)) and no QB.
 __global__ void k_charges(int n, const int32_t* __restrict__ perm, const double2* __restrict__ dens,
-                          const double2* __restrict__ wdzs, double sqb, double2* __restrict__ qa,
+                          const double2* __restrict__ wdzs, double sqb, int hilbert, double2* __restrict__ qa,
                           double2* __restrict__ qb) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double2 d = dens[perm[i]];
   const double2 w = wdzs[i];
-  const double c = -0.15915494309189533577;  // -1/(2 pi): 1/(2 pi i) = -i/(2 pi)
   const double2 dw = cmul(d, w);
+  if (hilbert) {
+    qa[i] = make_double2(-dw.x, -dw.y);
+    qb[i] = make_double2(0.0, 0.0);
+    return;
+  }
+  const double c = -0.15915494309189533577;  // -1/(2 pi): 1/(2 pi i) = -i/(2 pi)
   qa[i] = make_double2(-c * dw.y, c * dw.x);
   const double2 t = cadd(cscale(cmul(cconj(d), w), sqb), cmul(d, cconj(w)));
   qb[i] = make_double2(-c * t.y, c * t.x);
@@ -928,7 +934,7 @@ extern "C" int eb_fmm_plan_create(eb_fmm_plan* out, int32_t n, const double* x, 
     eb_set_error("eb_fmm_plan_create: nterms=%d outside 1..60", nterms);
     return EB_ERR_TERMS;
   }
-  if (variant < 0 || variant > 2 || (mode != EB_MODE_FMM && mode != EB_MODE_DIRECT)) {
+  if (variant < 0 || variant > 3 || (mode != EB_MODE_FMM && mode != EB_MODE_DIRECT)) {
     eb_set_error("eb_fmm_plan_create: bad variant %d or mode %d", variant, mode);
     return EB_ERR_ARG;
   }
@@ -939,7 +945,8 @@ extern "C" int eb_fmm_plan_create(eb_fmm_plan* out, int32_t n, const double* x, 
   P->kmax = pick_kmax(nterms);
   P->variant = variant;
   P->mode = mode;
-  P->sgn = (variant == EB_VARIANT_DR) ? -1.0 : 1.0;
+  // GW = phi + sgn (z conj(phi') + conj(psi)); the Hilbert product is phi alone
+  P->sgn = (variant == EB_VARIANT_DR) ? -1.0 : (variant == EB_VARIANT_HILBERT ? 0.0 : 1.0);
   P->sqb = (variant == EB_VARIANT_PLUS) ? 1.0 : -1.0;
   P->st = (hipStream_t)stream;
   hipStream_t st = P->st;
@@ -1105,7 +1112,7 @@ extern "C" int eb_fmm_apply(eb_fmm_plan P, const double* dens, double* gradw, vo
     EB_HIP_CHECK(hipEventRecord(P->ev[0], st));
   }
   hipLaunchKernelGGL(k_charges, dim3((n + 255) / 256), dim3(256), 0, st, n, P->perm, (const double2*)dens, P->wdzs,
-                     P->sqb, P->qa, P->qb);
+                     P->sqb, P->variant == EB_VARIANT_HILBERT ? 1 : 0, P->qa, P->qb);
   if (P->mode == EB_MODE_DIRECT) {
     if (P->timing)
       for (int i = 1; i <= 6; ++i) (void)hipEventRecord(P->ev[i], st);

@@ -18,121 +18,10 @@
 #include <vector>
 
 #include "../../include/eb_spectral.h"
+#include "eb_bodies.h"
 #include "eb_fft_plan.h"
-#include "eb_internal.h"
 
 using namespace eb;
-
-constexpr int SP_THREADS = ebfft::FFT_THREADS;
-
-struct eb_bodies_s {
-  int nbody = 0, n = 0, nmax = 0, ext = 0;
-  std::vector<int> nd, off;
-  int* d_nd = nullptr;       // [nbody]
-  int* d_off = nullptr;      // [nbody + 1]
-  int* d_plan_of = nullptr;  // [nbody] plan of the body's length
-  int* d_order = nullptr;    // [nbody] bodies by decreasing transform cost
-  ebfft::Plan* d_plans = nullptr;
-  ebfft::Stage* d_stages = nullptr;
-  double2* d_tab = nullptr;  // twiddles, roots, chirps, filter transforms
-  double* d_red = nullptr;   // [nbody][4] xmin, xmax, ymin, ymax
-};
-
-// ------------------------------------------------------------------------------------------
-// per-body helpers in LDS
-// ------------------------------------------------------------------------------------------
-
-// wavenumber of coefficient i under FDIFFF's rules (dcfft.f:1404-1422): 0 for the mean and,
-// for even N, the Nyquist coefficient
-__device__ __forceinline__ double fdiff_k(int i, int N) {
-  if (i == 0) return 0.0;
-  if ((N & 1) == 0) {
-    const int h = N / 2;
-    if (i < h) return (double)i;
-    if (i == h) return 0.0;
-    return (double)(i - N);
-  }
-  return (i <= (N - 1) / 2) ? (double)i : (double)(i - N);
-}
-
-// FILTER (electric.f:1152-1187) on coefficients c[0..N) in LDS: exp(-10 (i/n)^25) with the
-// ratio formed in single precision, components below 1e-13 zeroed, Nyquist zeroed.
-__device__ __forceinline__ double filt_factor(int i, int n) {
-  const double ratio = (double)((float)i / (float)n);
-  double r = 1.0, b = ratio;  // ratio**25 by repeated squaring
-  for (int e = 25; e > 0; e >>= 1) {
-    if (e & 1) r *= b;
-    b *= b;
-  }
-  return exp(-10.0 * r);
-}
-
-__device__ __forceinline__ void filter_block(double2* c, int N) {
-  const int n = N / 2;
-  const double eps = 1e-13;
-  for (int i = threadIdx.x; i < N; i += blockDim.x) {
-    int m;
-    if (i <= n) m = i;
-    else m = N - i;  // i = N - m, m = 1..n-1
-    double2 v = c[i];
-    if (i == n) {
-      v = make_double2(0, 0);
-    } else {
-      const double f = filt_factor(m, n);
-      v.x *= f;
-      v.y *= f;
-      if (fabs(v.x) < eps) v.x = 0.0;
-      if (fabs(v.y) < eps) v.y = 0.0;
-    }
-    c[i] = v;
-  }
-  __syncthreads();
-}
-
-__device__ __forceinline__ double block_sum(double v, double* red) {
-  red[threadIdx.x] = v;
-  __syncthreads();
-  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
-    if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
-    __syncthreads();
-  }
-  const double s = red[0];
-  __syncthreads();
-  return s;
-}
-
-struct BodyView {
-  const int* nd;
-  const int* off;
-  const int* plan_of;
-  const int* order;
-  const ebfft::Plan* plans;
-  const ebfft::Stage* stages;
-  const double2* tab;
-  int nmax;
-};
-
-static BodyView view(eb_bodies B) {
-  return BodyView{B->d_nd, B->d_off, B->d_plan_of, B->d_order, B->d_plans, B->d_stages, B->d_tab, B->nmax};
-}
-
-// the body of this workgroup and its transform plan
-struct BodyCtx {
-  int k, N, o0;
-  ebfft::Plan p;
-};
-__device__ __forceinline__ BodyCtx body_ctx(const BodyView& B) {
-  BodyCtx c;
-  c.k = B.order[blockIdx.x];
-  c.N = B.nd[c.k];
-  c.o0 = B.off[c.k];
-  c.p = B.plans[B.plan_of[c.k]];
-  return c;
-}
-
-__device__ __forceinline__ void fft(double2* x, double2* ws, const BodyCtx& c, const BodyView& B, int sign) {
-  ebfft::fft_inplace(x, ws, c.p, B.stages, B.tab, sign);
-}
 
 // ------------------------------------------------------------------------------------------
 // kernels (one block per body; dynamic LDS holds nseq sequences of nmax entries followed by
@@ -462,15 +351,6 @@ __global__ void k_shift_pk(int nbody, double* __restrict__ pk_out, const double*
 // ------------------------------------------------------------------------------------------
 // host side
 // ------------------------------------------------------------------------------------------
-template <typename T>
-static int dalloc(T** p, size_t count) {
-  *p = nullptr;
-  if (hipMalloc((void**)p, sizeof(T) * (count ? count : 1)) != hipSuccess) {
-    eb_set_error("hipMalloc(%zu bytes) failed", sizeof(T) * count);
-    return EB_ERR_NOMEM;
-  }
-  return EB_OK;
-}
 
 static void free_bodies(eb_bodies B) {
   if (!B) return;
@@ -480,10 +360,6 @@ static void free_bodies(eb_bodies B) {
   delete B;
 }
 
-template <typename T>
-static bool upload(T* dst, const std::vector<T>& src, hipStream_t st) {
-  return src.empty() || hipMemcpyAsync(dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice, st) == hipSuccess;
-}
 
 extern "C" int eb_bodies_create(eb_bodies* out, int32_t nbody, const int32_t* nd, void* stream) {
   if (!out || nbody < 1 || !nd) {
@@ -559,8 +435,6 @@ extern "C" int eb_bodies_destroy(eb_bodies B) {
 
 extern "C" int eb_bodies_npoints(eb_bodies B) { return B ? B->n : 0; }
 
-// dynamic LDS of a spectral kernel: nseq sequences of nmax entries and the Bluestein workspace
-static size_t lds_bytes(eb_bodies B, int nseq) { return sizeof(double2) * ((size_t)B->nmax * nseq + (size_t)B->ext); }
 
 extern "C" int eb_dcfft(eb_bodies B, double* c, int32_t sign, void* stream) {
   if (!B || !c || (sign != 1 && sign != -1)) return (eb_set_error("eb_dcfft: bad arguments"), EB_ERR_ARG);
@@ -702,6 +576,23 @@ extern "C" int eb_matvec_create(eb_matvec* out, eb_bodies B, const double* x, co
   }
   *out = M;
   return EB_OK;
+}
+
+extern "C" int eb_matvec_rebuild(eb_matvec M, const double* x, const double* y, const double* dz, const double* dsdth,
+                                 const double* rkappa, const double* zk, void* stream) {
+  if (!M || !x || !y || !dz || !dsdth || !rkappa || !zk) return (eb_set_error("eb_matvec_rebuild: bad arguments"), EB_ERR_ARG);
+  hipStream_t st = (hipStream_t)stream;
+  const int n = M->n, nb = M->B->nbody;
+  EB_HIP_CHECK(hipMemcpyAsync(M->x, x, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+  EB_HIP_CHECK(hipMemcpyAsync(M->y, y, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+  EB_HIP_CHECK(hipMemcpyAsync(M->dsdth, dsdth, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+  EB_HIP_CHECK(hipMemcpyAsync(M->rkappa, rkappa, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+  EB_HIP_CHECK(hipMemcpyAsync(M->dz, dz, sizeof(double2) * n, hipMemcpyDeviceToDevice, st));
+  EB_HIP_CHECK(hipMemcpyAsync(M->zk, zk, sizeof(double2) * nb, hipMemcpyDeviceToDevice, st));
+  hipLaunchKernelGGL(k_wdz, dim3((n + 255) / 256), dim3(256), 0, st, n, (const double2*)M->dz, M->h, M->body_of,
+                     M->wdz);
+  EB_HIP_CHECK(hipGetLastError());
+  return eb_fmm_plan_rebuild(M->fmm, M->x, M->y, (const double*)M->wdz, stream);
 }
 
 extern "C" int eb_matvec_apply(eb_matvec M, const double* xx, double* yy, double* stress, double* za, double* pk,

@@ -41,6 +41,10 @@ extern "C" {
 #define EB_VARIANT_DR   0  /* CADAPDR  (cadapdrop.f): GW = phi - z conj(phi') - conj(psi) */
 #define EB_VARIANT_MOD  1  /* CADAPMOD (cadapmod.f):  GW = phi + z conj(phi') + conj(psi) */
 #define EB_VARIANT_PLUS 2  /* CADAP    (cadap.f):     as MOD, with +conj(dens) dz in QB (cadap.f:215) */
+/* DAPIF2 with IFLAG7 = 3 (dapif1.f:458-560): the Hilbert-matrix product
+ * GW_i = sum_{j != i} DENS_j WDZ_j / (z_i - z_j), WDZ = the plan's complex weights.  Plans only
+ * (eb_fmm_plan_create); used by the Laplace side. */
+#define EB_VARIANT_HILBERT 3
 
 /* Evaluation modes of the drop-in layer. */
 #define EB_MODE_AUTO   0   /* direct O(N^2) sum below EB_DIRECT_MAX points, FMM above */

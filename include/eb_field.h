@@ -1,0 +1,68 @@
+/*
+ * eb_field.h -- C-ABI of F, the right-hand side of the drop evolution, on the device.
+ *
+ * F (electric.f:968-1109) for one simulation of k bodies, with no gravity (zbond = 0) and no
+ * wall (iwall = 0), the settings of the reference's drivers for these runs:
+ *
+ *   EXTRACTDAT (electric.f:636-716)
+ *   GETBCs + SOLVE_LAPL (laplace.f:5-25, 335-425): DGMRES on the Laplace operator FASMVP_LAP
+ *     (laplace.f:159-262), whose Hilbert product (DAPIF2 with IFLAG7 = 3) runs on the GPU FMM
+ *   DIRICH_NEU_MAP (laplace.f:266-332) with PVINTEV (pv_integral.f:53-212): spectral
+ *     interpolation to twice the points, alternating-point Hilbert product, FDIFFF
+ *   SETVELF (stokesbubb.f:4-135), SOLVEI (stokesbubb.f:294-415),
+ *   INTERFACENEW (electric.f:791-912) with CADAPMOD on the GPU FMM
+ *   the F tail: EQUIT, frame velocity, FFT, FILTER, inverse FFT (electric.f:1068-1103)
+ *
+ * All arrays are device pointers; rku / rkup are interleaved (x, y) per point, bodies back to
+ * back.  cap and e0 are COMMON /flowinfo/'s CAP and E0.  Every call is stream-ordered except
+ * the GMRES solves inside, which synchronise their stream once per iteration.
+ */
+#ifndef EB_FIELD_H
+#define EB_FIELD_H
+
+#include <stdint.h>
+
+#include "eb_spectral.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct eb_field_s *eb_field;
+
+/* Bodies up to 2048 points each (PVINTEV works on twice the points). */
+int eb_field_create(eb_field *f, eb_bodies bodies, double cap, double e0, void *stream);
+int eb_field_destroy(eb_field f);
+
+/* rkup = F(rku): the whole chain above, on the field's stream. */
+int eb_field_rhs(eb_field f, const double *rku, double *rkup);
+
+/* The stages one at a time, for tests and for callers that replace part of the chain.
+ * eb_field_set_geometry runs EXTRACTDAT on rku and rebuilds every FMM tree; the others act on
+ * that geometry. */
+int eb_field_set_geometry(eb_field f, const double *rku);
+/* w = FASMVP_LAP(u) for the nbk unknowns of the Laplace system (the reference also writes k
+ * constraint rows past the system's end, laplace.f:251-259; they are not part of it). */
+int eb_field_laplace_apply(eb_field f, const double *u, double *w);
+/* GETBCs + SOLVE_LAPL: ulap (nbk); *iter, *ierr as DGMRES reports them (may be NULL). */
+int eb_field_laplace_solve(eb_field f, double *ulap, int32_t *iter, int32_t *ierr);
+/* DIRICH_NEU_MAP: phi, psi, dphi_dn (nbk each; phi and psi may be NULL). */
+int eb_field_dirich_neu(eb_field f, const double *ulap, double *phi, double *psi, double *dphi_dn);
+/* SETVELF: rhs (2 nbk) from dphi_dn. */
+int eb_field_setvelf(eb_field f, const double *dphi_dn, double *rhs);
+/* SOLVEI: u (nbk complex = 2 nbk doubles), za (k complex), pk (k); *iter, *ierr may be NULL. */
+int eb_field_stokes_solve(eb_field f, const double *rhs, double *u, double *za, double *pk, int32_t *iter,
+                          int32_t *ierr);
+/* INTERFACENEW: the normal velocity un (nbk) from the density u. */
+int eb_field_interface(eb_field f, const double *u, double *un);
+
+/* Geometry of the current call (device pointers owned by the field). */
+typedef struct {
+  const double *x, *y, *dz, *tha, *dsda, *rkappa, *ckern, *zk;
+} eb_field_geometry;
+int eb_field_get_geometry(eb_field f, eb_field_geometry *g);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -76,6 +76,10 @@ int eb_matvec_create(eb_matvec *mv, eb_bodies bodies, const double *x, const dou
  * (per body) the reference's za and pk (after the pinf shift). */
 int eb_matvec_apply(eb_matvec mv, const double *xx, double *yy, double *stress, double *za, double *pk,
                     void *stream);
+/* New geometry for the same bodies (the RK stages of a time step): copies the arrays and
+ * rebuilds the FMM tree. */
+int eb_matvec_rebuild(eb_matvec mv, const double *x, const double *y, const double *dz, const double *dsdth,
+                      const double *rkappa, const double *zk, void *stream);
 int eb_matvec_destroy(eb_matvec mv);
 
 /* ---------------------------------------------------------------------------------------

@@ -742,3 +742,98 @@ class RefSim:
                          _d(np.ascontiguousarray(u, np.float64)), _d(w), _d(h), _d(za), _d(pk),
                          _d(stress), _d(dens))
         return w, stress.view(np.complex128)
+
+
+# ------------------------------------------------------------------------------------
+# The reference's F, stage by stage (electric.f:968-1109 with iwall = 0)
+# ------------------------------------------------------------------------------------
+
+def ref_field_chain(nd, rku, cap=0.0, e0=0.0, maxl=100):
+    """Run the stages of the reference's F on one geometry and return every intermediate:
+    EXTRACTDAT, GETBCs, SOLVE_LAPL (laplace.f:335-425), DIRICH_NEU_MAP (laplace.f:266-332),
+    SETVELF (stokesbubb.f:4-135), SOLVEI (stokesbubb.f:294-415) and INTERFACENEW
+    (electric.f:791-912), in F's order, with COMMON /problem/, /inteqn/, /flowinfo/ and
+    /wallinfo/ set as F sees them (zbond = 0, iwall = 0).  Run it from a scratch directory:
+    the reference writes fort.13."""
+    r = ref()
+    nd = np.ascontiguousarray(nd, np.int32)
+    k = nd.size
+    nbk = int(nd.sum())
+    kmax, nmax, ndmax = 150, 450000, 3000
+    sim = RefSim(LIBREF)
+    sim.lib = r
+    sim.setup(nd, cap=cap, zbond=0j, e0=e0)
+    inteqn = (ctypes.c_char * (48 * nmax)).from_address(ctypes.addressof(ctypes.c_char.in_dll(r, "inteqn_")))
+    base = ctypes.addressof(inteqn)
+    c_dsda = ctypes.c_void_p(base)
+    c_rkappa = ctypes.c_void_p(base + 8 * nmax)
+    c_z = ctypes.c_void_p(base + 16 * nmax)
+    c_dz = ctypes.c_void_p(base + 32 * nmax)
+    prob = (ctypes.c_char * (16 * kmax)).from_address(ctypes.addressof(ctypes.c_char.in_dll(r, "problem_")))
+    c_zk = ctypes.c_void_p(ctypes.addressof(prob))
+    ifwork = 10 * ndmax + 50
+    x, y = np.zeros(nbk), np.zeros(nbk)
+    tha, ckern = np.zeros(nbk), np.zeros(2 * nbk)
+    zf, zfi = np.zeros(2 * nbk + 64), np.zeros(2 * nbk + 64)
+    wsave = np.zeros(2 * ifwork * k)
+    zc = np.zeros(2 * k)
+    xl, sl = ctypes.c_double(), ctypes.c_double()
+    zbond = np.zeros(2)
+    rku = np.ascontiguousarray(rku, np.float64)
+    r.extractdat_(_c_int(k), _i(nd), _c_int(nbk), _d(x), _d(y), c_z, c_zk, c_dz, _d(tha), c_dsda, c_rkappa,
+                  _d(ckern), _d(rku), _d(zf), _d(zfi), _d(wsave), _c_int(ifwork), _d(zc), ctypes.byref(xl),
+                  ctypes.byref(sl), _d(zbond), _c_int(0))
+    view = lambda off, cnt, dt: np.frombuffer(inteqn, dtype=dt, count=cnt, offset=off).copy()  # noqa: E731
+    dsda = view(0, nbk, np.float64)
+    rkappa = view(8 * nmax, nbk, np.float64)
+    z = view(16 * nmax, nbk, np.complex128)
+    dz = view(32 * nmax, nbk, np.complex128)
+    zk = np.frombuffer(prob, dtype=np.complex128, count=k).copy()
+    # Laplace: GETBCs and SOLVE_LAPL (N = nbk, MAXL 100, TOL 1e-10, no restart)
+    rhs = np.zeros(2 * nbk + kmax)
+    r.getbcs_(_c_int(k), _i(nd), _c_int(nbk), c_zk, c_z, _d(rhs))
+    rhs_lap = rhs[:nbk].copy()
+    n_l = nbk
+    lrw = 10 + (2 * nbk + k) * (maxl + 6) + maxl * (maxl + 3) + 4096  # FASMVP_LAP writes k rows past N
+    rwork = np.zeros(lrw)
+    iwork = np.zeros(64, np.int32)
+    iwork[0] = maxl
+    soln = np.zeros(2 * nbk + kmax)
+    ulap = np.zeros(nbk + kmax)
+    ak = np.zeros(kmax)
+    r.solve_lapl_(_i(nd), _c_int(k), _c_int(nbk), _d(rhs), _d(soln), _d(ulap), _d(ak), _d(rwork), _c_int(lrw),
+                  _i(iwork), _c_int(30), c_dsda)
+    ulap = ulap[:nbk].copy()
+    # DIRICH_NEU_MAP
+    phi, psi, dphi = np.zeros(nbk), np.zeros(nbk), np.zeros(nbk)
+    r.dirich_neu_map_(_c_int(k), _i(nd), _c_int(nbk), c_z, _d(ulap), c_dz, c_zk, _d(ak), _d(phi), _d(psi),
+                      _d(dphi), c_dsda)
+    # SETVELF
+    arg = np.zeros(nbk + 64)
+    zb = np.zeros(2 * k)
+    rhs_s = np.zeros(2 * nbk + kmax)
+    r.setvelf_(_c_int(k), _i(nd), _c_int(nbk), c_z, c_zk, c_dz, _d(arg), _d(zb), _d(zc), _d(zf), _d(zfi),
+               _d(wsave), _c_int(ifwork), _d(rhs_s), _c_int(0), _d(zbond), _c_double(cap), c_dsda, _d(dphi),
+               _c_double(e0))
+    rhs_stokes = rhs_s[:2 * nbk].copy()
+    # SOLVEI (N = 2 nbk)
+    rwork[:] = 0.0
+    iwork[:] = 0
+    iwork[0] = maxl
+    soln[:] = 0.0
+    pk = np.zeros(k)
+    pinf = ctypes.c_double()
+    za = np.zeros(2 * k)
+    u = np.zeros(2 * nbk)
+    r.solvei_(_i(nd), _c_int(k), _c_int(nbk), _d(rhs_s), _d(soln), _d(pk), ctypes.byref(pinf), _d(za), c_zk, c_z,
+              c_dz, _d(u), _d(rwork), _c_int(lrw), _i(iwork), _c_int(30))
+    # INTERFACENEW
+    h = (2 * np.pi / nd).astype(np.float64)
+    un = np.zeros(nbk)
+    r.interfacenew_(_i(nd), _c_int(k), _c_int(nbk), _d(h), _d(x), _d(y), c_dz, _d(u), _d(un), c_rkappa,
+                    _d(ckern), _d(wsave), c_zk, _c_int(ifwork), _d(zc), _c_int(0), ctypes.byref(xl),
+                    ctypes.byref(sl), _c_double(cap), c_dsda, _d(zb))
+    r.prini_(_c_int(0), _c_int(0))
+    return dict(x=x, y=y, z=z, zk=zk, dz=dz, dsda=dsda, rkappa=rkappa, tha=tha, ckern=ckern.view(np.complex128),
+                rhs_lap=rhs_lap, ulap=ulap, phi=phi, psi=psi, dphi_dn=dphi, rhs_stokes=rhs_stokes,
+                u=u.view(np.complex128), pk=pk, za=za.view(np.complex128), un=un)

@@ -2,8 +2,8 @@
 
 Runs only where the reference sources exist (they are compiled in place by oracle/Makefile);
 the fixtures it writes are data -- inputs and the reference's outputs -- and are committed so
-the tests run without the reference.  Usage:  python tests/golden/make_golden.py [atree|gmres|sweep]
-("atree" / "gmres" / "sweep" rewrite only the adaptive-tree / GMRES / batched-sweep fixtures).
+the tests run without the reference.  Usage:  python tests/golden/make_golden.py [atree|gmres|sweep|field]
+("atree" / "gmres" / "sweep" / "field" rewrite only those fixtures).
 """
 from __future__ import annotations
 
@@ -266,6 +266,51 @@ def main_sweep():
         os.chdir(cwd)
 
 
+def main_field():
+    """The reference's F stage by stage (oracle.ref_field_chain), its whole F (RefSim.rhs) and a
+    few RK2 steps, on three geometries: the single drop of the plain method (r = 1 + 0.05 cos
+    2 theta, N = 256), two drops of different N, and the four unit circles of bubbin."""
+    import tempfile
+    from electricbubbles_amd.workloads import perturbed_circle
+    cwd = os.getcwd()
+    os.chdir(tempfile.mkdtemp())  # the reference writes fort.13
+    try:
+        cases = []
+        cases.append(("single_drop", [256], [0j], [(2, 0.05)], 0.1, 1.0))
+        cases.append(("two_drops", [128, 160], [0j, 3.0 + 0.5j], [(2, 0.05), (3, 0.03)], 0.0, 0.64))
+        cases.append(("four_circles", [128] * 4, [1.5 + 1.5j, -1.5 + 1.5j, -1.5 - 1.5j, 1.5 - 1.5j], [(2, 0.0)], 0.0,
+                      0.64))
+        for tag, nds, centres, modes, cap, e0 in cases:
+            nd = np.array(nds, np.int32)
+            rku = np.concatenate([perturbed_circle(int(n), modes, center=c)[0] for n, c in zip(nds, centres)])
+            rku = rku.view(np.float64).copy()
+            ch = oracle.ref_field_chain(nd, rku, cap=cap, e0=e0)
+            sim = oracle.RefSim(oracle.LIBREF)
+            sim.setup(nd, cap=cap, e0=e0)
+            rkup = sim.rhs(rku)
+            # FASMVP_LAP on a smooth density
+            nbk = int(nd.sum())
+            th = np.concatenate([2 * np.pi * np.arange(m) / m for m in nd])
+            ut = np.cos(th) + 0.3 * np.sin(3 * th)
+            w = np.zeros(nbk + 150)
+            ak = np.zeros(150)
+            r = oracle.ref()
+            r.fasmvp_lap_(oracle._c_int(nd.size), oracle._i(nd), oracle._c_int(nbk), oracle._d(oracle._cplx(ch["zk"])),
+                          oracle._d(oracle._cplx(ch["z"])), oracle._d(oracle._cplx(ch["dz"])),
+                          oracle._d(ch["rkappa"]), oracle._d(ch["dsda"]), oracle._d(ak), oracle._d(ut), oracle._d(w))
+            r.prini_(oracle._c_int(0), oracle._c_int(0))
+            dt, nsteps = 1e-3, 3
+            sim.setup(nd, cap=cap, e0=e0)
+            traj = sim.rk2(rku, dt, nsteps)
+            print(tag, "F norm", np.linalg.norm(rkup), "Un max", np.abs(ch["un"]).max())
+            save("field_" + tag, nd=nd, rku=rku, cap=np.float64(cap), e0=np.float64(e0), rkup=rkup, lap_u=ut,
+                 lap_w=w[:nbk].copy(), rk2=traj, rk2_dt=np.float64(dt), rk2_steps=np.int32(nsteps),
+                 **{k: v for k, v in ch.items() if k in ("rhs_lap", "ulap", "phi", "psi", "dphi_dn", "rhs_stokes",
+                                                         "u", "za", "pk", "un")})
+    finally:
+        os.chdir(cwd)
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["atree"]:
         main_atree()
@@ -273,8 +318,11 @@ if __name__ == "__main__":
         main_gmres()
     elif sys.argv[1:] == ["sweep"]:
         main_sweep()
+    elif sys.argv[1:] == ["field"]:
+        main_field()
     else:
         main()
         main_atree()
         main_gmres()
         main_sweep()
+        main_field()
