@@ -187,10 +187,13 @@ __global__ __launch_bounds__(SP_THREADS) void k_dn_post(BodyView B, const double
 __global__ __launch_bounds__(SP_THREADS) void k_setvelf(BodyView B, const double* __restrict__ x,
                                                          const double* __restrict__ y, const double2* __restrict__ dz,
                                                          const double* __restrict__ dsda,
-                                                         const double* __restrict__ dphi, double cap, double e0,
-                                                         double alpha_last, double2* __restrict__ rhs) {
+                                                         const double* __restrict__ dphi,
+                                                         const double* __restrict__ cap_b,
+                                                         const double* __restrict__ e0_b,
+                                                         const double* __restrict__ alpha_b, double2* __restrict__ rhs) {
   extern __shared__ double2 lds[];
   const BodyCtx bc = body_ctx(B);
+  const double cap = cap_b[bc.k], e0 = e0_b[bc.k], alpha_last = alpha_b[bc.k];
   const int N = bc.N, o0 = bc.o0, n = N / 2;
   double2* a = lds;
   double2* c = lds + B.nmax;
@@ -246,10 +249,12 @@ __global__ void k_interface_point(int n, const int* __restrict__ body_of, const 
                                   const double2* __restrict__ dz, const double* __restrict__ dsda,
                                   const double* __restrict__ rkappa, const double2* __restrict__ ckern,
                                   const double2* __restrict__ u, const double2* __restrict__ udot,
-                                  const double2* __restrict__ gw, double cap, double* __restrict__ un) {
+                                  const double2* __restrict__ gw, const double* __restrict__ cap_b,
+                                  double* __restrict__ un) {
   const int i = blockIdx.x * EW + threadIdx.x;
   if (i >= n) return;
-  const double hk = h[body_of[i]];
+  const int kb = body_of[i];
+  const double hk = h[kb], cap = cap_b[kb];
   const double2 d = dz[i];
   const double s = dsda[i];
   const double2 zn = make_double2(d.y / s, -d.x / s);  // -i dz / dsda
@@ -298,7 +303,7 @@ struct eb_field_s {
   int nt_lap = 37, nt_pv = 33, nt_mod = 43;  // TOL 1e-11 (laplace.f:212), 1e-10 (pv_integral.f:175), 1e-13
   // geometry
   double *x = nullptr, *y = nullptr, *tha = nullptr, *dsda = nullptr, *rkappa = nullptr, *area = nullptr,
-         *xls = nullptr, *h = nullptr;
+         *xls = nullptr, *h = nullptr, *cap_b = nullptr, *e0_b = nullptr, *alpha_b = nullptr;
   double2 *dz = nullptr, *zk = nullptr, *ckern = nullptr, *wdz = nullptr;
   int* body_of = nullptr;
   // Laplace
@@ -324,7 +329,8 @@ static void free_field(eb_field F) {
   if (F->mv) eb_matvec_destroy(F->mv);
   if (F->glap) eb_gmres_destroy(F->glap);
   if (F->gst) eb_gmres_destroy(F->gst);
-  void* p[] = {F->x,    F->y,   F->tha,  F->dsda, F->rkappa, F->area,   F->xls,  F->h,   F->dz,   F->zk,
+  void* p[] = {F->cap_b, F->e0_b, F->alpha_b,
+               F->x,    F->y,   F->tha,  F->dsda, F->rkappa, F->area,   F->xls,  F->h,   F->dz,   F->zk,
                F->ckern, F->wdz, F->body_of, F->dens, F->cf,  F->uf,     F->wf,   F->cf2, F->x2,   F->y2,
                F->rhs_lap, F->ulap, F->phi, F->psi, F->dphi, F->rhs_st, F->pk, F->un, F->u, F->za, F->udot, F->gw};
   for (void* q : p)
@@ -353,10 +359,11 @@ extern "C" int eb_field_create(eb_field* out, eb_bodies B, double cap, double e0
   F->nt_pv = nterms_for(1.0e-10);
   F->nt_mod = nterms_for(1.0e-13);
   const int n = F->n, k = F->k;
-  std::vector<double> h(k);
+  std::vector<double> h(k), capv(k, cap), e0v(k, e0), alv(k, F->alpha_last);
   for (int b = 0; b < k; ++b) h[b] = 2.0 * PI / B->nd[b];
   int rc;
-  if ((rc = dalloc(&F->x, n)) || (rc = dalloc(&F->y, n)) || (rc = dalloc(&F->tha, n)) || (rc = dalloc(&F->dsda, n)) ||
+  if ((rc = dalloc(&F->cap_b, k)) || (rc = dalloc(&F->e0_b, k)) || (rc = dalloc(&F->alpha_b, k)) ||
+      (rc = dalloc(&F->x, n)) || (rc = dalloc(&F->y, n)) || (rc = dalloc(&F->tha, n)) || (rc = dalloc(&F->dsda, n)) ||
       (rc = dalloc(&F->rkappa, n)) || (rc = dalloc(&F->area, k)) || (rc = dalloc(&F->xls, 2)) ||
       (rc = dalloc(&F->h, k)) || (rc = dalloc(&F->dz, n)) || (rc = dalloc(&F->zk, k)) || (rc = dalloc(&F->ckern, n)) ||
       (rc = dalloc(&F->wdz, n)) || (rc = dalloc(&F->body_of, n)) || (rc = dalloc(&F->dens, n)) ||
@@ -369,7 +376,8 @@ extern "C" int eb_field_create(eb_field* out, eb_bodies B, double cap, double e0
     free_field(F);
     return rc;
   }
-  if (!upload(F->h, h, F->st)) {
+  if (!upload(F->h, h, F->st) || !upload(F->cap_b, capv, F->st) || !upload(F->e0_b, e0v, F->st) ||
+      !upload(F->alpha_b, alv, F->st)) {
     free_field(F);
     return (eb_set_error("eb_field_create: upload failed"), EB_ERR_HIP);
   }
@@ -490,7 +498,7 @@ extern "C" int eb_field_setvelf(eb_field F, const double* dphi_dn, double* rhs) 
   if (!dphi_dn || !rhs) return (eb_set_error("eb_field_setvelf: bad arguments"), EB_ERR_ARG);
   eb_bodies B = F->B;
   hipLaunchKernelGGL(k_setvelf, dim3(B->nbody), dim3(SP_THREADS), lds_bytes(B, 2), F->st, view(B), F->x, F->y, F->dz,
-                     F->dsda, dphi_dn, F->cap, F->e0, F->alpha_last, (double2*)rhs);
+                     F->dsda, dphi_dn, F->cap_b, F->e0_b, F->alpha_b, (double2*)rhs);
   EB_HIP_CHECK(hipGetLastError());
   return EB_OK;
 }
@@ -517,7 +525,7 @@ extern "C" int eb_field_interface(eb_field F, const double* u, double* un) {
   if ((rc = eb_fdifff(F->B, u, (double*)F->udot, s)) != EB_OK) return rc;
   if ((rc = eb_fmm_apply(F->itf, u, (double*)F->gw, s)) != EB_OK) return rc;
   hipLaunchKernelGGL(k_interface_point, dim3(ew_blocks(n)), dim3(EW), 0, F->st, n, F->body_of, F->h, F->x, F->y,
-                     F->dz, F->dsda, F->rkappa, F->ckern, (const double2*)u, F->udot, F->gw, F->cap, un);
+                     F->dz, F->dsda, F->rkappa, F->ckern, (const double2*)u, F->udot, F->gw, F->cap_b, un);
   EB_HIP_CHECK(hipGetLastError());
   return EB_OK;
 }
@@ -545,5 +553,264 @@ extern "C" int eb_field_get_geometry(eb_field F, eb_field_geometry* g) {
   g->rkappa = F->rkappa;
   g->ckern = (const double*)F->ckern;
   g->zk = (const double*)F->zk;
+  return EB_OK;
+}
+
+// ==========================================================================================
+// F of many independent simulations at once (eb_sweep_field)
+// ==========================================================================================
+//
+// The same chain for every simulation of a sweep, each stage one launch for all of them:
+// the Laplace and Stokes solves run in the batched GMRES (one system per simulation), the
+// layer potentials sum over each simulation's own points (the O(N^2) sum that the plain path
+// also takes up to 2048 points), the per-body spectral kernels already cover every body.
+namespace {
+
+constexpr int HB_T = 256;
+
+// CF_t = sum_s q_s / (z_t - z_s), q = dens * w, over one simulation: targets t and sources s
+// are points base + stride * i + toff / soff of the simulation's segment [o, o + n) (the
+// coarse points, stride 1 with the self pair dropped, or the fine points, stride 2: targets
+// at the original points, sources at the midpoints).  One 256-thread block per (simulation,
+// 256 targets) tile; sources staged through LDS.
+template <bool SELF>
+__global__ __launch_bounds__(HB_T) void k_hilbert_seg(const int2* __restrict__ tiles, const int* __restrict__ soff,
+                                                      int stride, int toff, int srcoff, const double* __restrict__ x,
+                                                      const double* __restrict__ y, const double2* __restrict__ dens,
+                                                      const double2* __restrict__ w, double2* __restrict__ cf) {
+  __shared__ double2 sz[HB_T], sq[HB_T];
+  const int2 tl = tiles[blockIdx.x];
+  const int o = soff[tl.x], n = soff[tl.x + 1] - o;
+  const int base = stride * o;
+  const int it = tl.y + (int)threadIdx.x;
+  const bool act = it < n;
+  const int ti = base + stride * (act ? it : 0) + toff;
+  const double zx = x[ti], zy = y[ti];
+  double ax = 0.0, ay = 0.0;
+  for (int c0 = 0; c0 < n; c0 += HB_T) {
+    __syncthreads();
+    const int js = c0 + (int)threadIdx.x;
+    if (js < n) {
+      const int sj = base + stride * js + srcoff;
+      sz[threadIdx.x] = make_double2(x[sj], y[sj]);
+      sq[threadIdx.x] = cmul(dens[sj], w[sj]);
+    }
+    __syncthreads();
+    const int m = min(HB_T, n - c0);
+    if (act)
+      for (int jj = 0; jj < m; ++jj) {
+        // q / (z_t - z_s) = q conj(d) / |d|^2, d = z_t - z_s
+        const double dx = zx - sz[jj].x, dy = zy - sz[jj].y;
+        const bool self = SELF && (c0 + jj == it);
+        const double r2 = self ? 1.0 : fma(dx, dx, dy * dy);
+        const double ir = self ? 0.0 : rcp_nr(r2);
+        const double2 q = sq[jj];
+        ax = fma(fma(q.x, dx, q.y * dy), ir, ax);
+        ay = fma(fma(q.y, dx, -q.x * dy), ir, ay);
+      }
+  }
+  if (act) cf[base + stride * it + toff] = make_double2(ax, ay);
+}
+
+}  // namespace
+
+struct eb_sweep_field_s {
+  eb_bodies B = nullptr;
+  int nsim = 0, k = 0, n = 0, ntile = 0;
+  hipStream_t st = nullptr;
+  bool have_geom = false;
+  std::vector<int> soff_h, uoff_h, sim_off;
+  std::vector<int32_t> it_lap, ie_lap, it_st, ie_st;
+  int *soff = nullptr, *body_of = nullptr;
+  int2* tiles = nullptr;
+  double *h = nullptr, *cap_b = nullptr, *e0_b = nullptr, *alpha_b = nullptr;
+  double *x = nullptr, *y = nullptr, *tha = nullptr, *dsda = nullptr, *rkappa = nullptr, *area = nullptr,
+         *xls = nullptr;
+  double2 *dz = nullptr, *zk = nullptr, *ckern = nullptr, *wdz = nullptr;
+  double2 *dens = nullptr, *cf = nullptr, *uf = nullptr, *wf = nullptr, *cf2 = nullptr;
+  double *x2 = nullptr, *y2 = nullptr, *rhs_lap = nullptr, *ulap = nullptr, *dphi = nullptr;
+  eb_gmres_batch glap = nullptr, gst = nullptr;
+  eb_sweep sw = nullptr;
+  double *rhs_st = nullptr, *pk = nullptr, *un = nullptr;
+  double2 *u = nullptr, *za = nullptr, *udot = nullptr, *gw = nullptr;
+};
+
+static void free_sf(eb_sweep_field F) {
+  if (!F) return;
+  if (F->st) (void)hipStreamSynchronize(F->st);
+  if (F->glap) eb_gmres_batch_destroy(F->glap);
+  if (F->gst) eb_gmres_batch_destroy(F->gst);
+  if (F->sw) eb_sweep_destroy(F->sw);
+  void* p[] = {F->soff, F->body_of, F->tiles, F->h, F->cap_b, F->e0_b, F->alpha_b, F->x, F->y, F->tha, F->dsda,
+               F->rkappa, F->area, F->xls, F->dz, F->zk, F->ckern, F->wdz, F->dens, F->cf, F->uf, F->wf, F->cf2,
+               F->x2, F->y2, F->rhs_lap, F->ulap, F->dphi, F->rhs_st, F->pk, F->un, F->u, F->za, F->udot, F->gw};
+  for (void* q : p)
+    if (q) (void)hipFree(q);
+  delete F;
+}
+
+extern "C" int eb_sweep_field_create(eb_sweep_field* out, eb_bodies B, int32_t nsim, const int32_t* sim_off,
+                                     const double* cap, const double* e0, void* stream) {
+  if (!out || !B || nsim < 1 || !sim_off || !cap || !e0)
+    return (eb_set_error("eb_sweep_field_create: bad arguments"), EB_ERR_ARG);
+  *out = nullptr;
+  if (sim_off[0] != 0 || sim_off[nsim] != B->nbody)
+    return (eb_set_error("eb_sweep_field_create: simulation offsets must run from 0 to the body count"), EB_ERR_ARG);
+  for (int s = 0; s < nsim; ++s)
+    if (sim_off[s + 1] <= sim_off[s])
+      return (eb_set_error("eb_sweep_field_create: simulation %d has no body", s), EB_ERR_ARG);
+  for (int b = 0; b < B->nbody; ++b)
+    if (B->nd[b] > 2048)
+      return (eb_set_error("eb_sweep_field_create: body %d has %d points (up to 2048)", b, B->nd[b]), EB_ERR_ARG);
+  eb_sweep_field F = new eb_sweep_field_s();
+  F->B = B;
+  F->nsim = nsim;
+  F->k = B->nbody;
+  F->n = B->n;
+  F->st = (hipStream_t)stream;
+  const int n = F->n, k = F->k;
+  F->sim_off.assign(sim_off, sim_off + nsim + 1);
+  F->soff_h.resize(nsim + 1);
+  F->uoff_h.resize(nsim + 1);
+  for (int s = 0; s <= nsim; ++s) {
+    F->soff_h[s] = B->off[sim_off[s]];
+    F->uoff_h[s] = 2 * F->soff_h[s];
+  }
+  std::vector<double> h(k), capv(k), e0v(k), alv(k);
+  for (int s = 0; s < nsim; ++s) {
+    const int nl = B->nd[sim_off[s + 1] - 1];
+    for (int b = sim_off[s]; b < sim_off[s + 1]; ++b) {
+      h[b] = 2.0 * PI / B->nd[b];
+      capv[b] = cap[s];
+      e0v[b] = e0[s];
+      alv[b] = 2.0 * PI * (nl - 1.0) / nl;  // stokesbubb.f:94, the simulation's last body
+    }
+  }
+  std::vector<int> ord(nsim);
+  for (int s = 0; s < nsim; ++s) ord[s] = s;
+  std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) {
+    return F->soff_h[a + 1] - F->soff_h[a] > F->soff_h[b + 1] - F->soff_h[b];
+  });
+  std::vector<int2> tiles;
+  for (int s : ord)
+    for (int t0 = 0; t0 < F->soff_h[s + 1] - F->soff_h[s]; t0 += HB_T) tiles.push_back(make_int2(s, t0));
+  F->ntile = (int)tiles.size();
+  F->it_lap.assign(nsim, 0);
+  F->ie_lap.assign(nsim, 0);
+  F->it_st.assign(nsim, 0);
+  F->ie_st.assign(nsim, 0);
+  int rc;
+  const size_t n2 = 2 * (size_t)n;
+  if ((rc = dalloc(&F->soff, nsim + 1)) || (rc = dalloc(&F->body_of, n)) || (rc = dalloc(&F->tiles, tiles.size())) ||
+      (rc = dalloc(&F->h, k)) || (rc = dalloc(&F->cap_b, k)) || (rc = dalloc(&F->e0_b, k)) ||
+      (rc = dalloc(&F->alpha_b, k)) || (rc = dalloc(&F->x, n)) || (rc = dalloc(&F->y, n)) ||
+      (rc = dalloc(&F->tha, n)) || (rc = dalloc(&F->dsda, n)) || (rc = dalloc(&F->rkappa, n)) ||
+      (rc = dalloc(&F->area, k)) || (rc = dalloc(&F->xls, 2)) || (rc = dalloc(&F->dz, n)) || (rc = dalloc(&F->zk, k)) ||
+      (rc = dalloc(&F->ckern, n)) || (rc = dalloc(&F->wdz, n)) || (rc = dalloc(&F->dens, n)) ||
+      (rc = dalloc(&F->cf, n)) || (rc = dalloc(&F->uf, n2)) || (rc = dalloc(&F->wf, n2)) || (rc = dalloc(&F->cf2, n2)) ||
+      (rc = dalloc(&F->x2, n2)) || (rc = dalloc(&F->y2, n2)) || (rc = dalloc(&F->rhs_lap, n)) ||
+      (rc = dalloc(&F->ulap, n)) || (rc = dalloc(&F->dphi, n)) || (rc = dalloc(&F->rhs_st, n2)) ||
+      (rc = dalloc(&F->pk, k)) || (rc = dalloc(&F->un, n)) || (rc = dalloc(&F->u, n)) || (rc = dalloc(&F->za, k)) ||
+      (rc = dalloc(&F->udot, n)) || (rc = dalloc(&F->gw, n))) {
+    free_sf(F);
+    return rc;
+  }
+  if (!upload(F->soff, F->soff_h, F->st) || !upload(F->tiles, tiles, F->st) || !upload(F->h, h, F->st) ||
+      !upload(F->cap_b, capv, F->st) || !upload(F->e0_b, e0v, F->st) || !upload(F->alpha_b, alv, F->st)) {
+    free_sf(F);
+    return (eb_set_error("eb_sweep_field_create: upload failed"), EB_ERR_HIP);
+  }
+  hipLaunchKernelGGL(k_body_of_f, dim3(k), dim3(256), 0, F->st, B->d_off, k, F->body_of);
+  hipLaunchKernelGGL(k_pv_weights, dim3(k), dim3(256), 0, F->st, view(B), F->wf);
+  if ((rc = eb_gmres_batch_create(&F->glap, nsim, F->soff_h.data(), 100, stream)) != EB_OK ||
+      (rc = eb_gmres_batch_create(&F->gst, nsim, F->uoff_h.data(), 100, stream)) != EB_OK) {
+    free_sf(F);
+    return rc;
+  }
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(F->st) != hipSuccess) {
+    free_sf(F);
+    return (eb_set_error("eb_sweep_field_create: kernel failed"), EB_ERR_HIP);
+  }
+  *out = F;
+  return EB_OK;
+}
+
+extern "C" int eb_sweep_field_destroy(eb_sweep_field F) {
+  free_sf(F);
+  return EB_OK;
+}
+
+static int sf_lap_apply(void* ctx, const double* u, double* w, void* stream) {
+  (void)stream;
+  eb_sweep_field F = (eb_sweep_field)ctx;
+  const int n = F->n;
+  hipLaunchKernelGGL(k_real_dens, dim3(ew_blocks(n)), dim3(EW), 0, F->st, n, u, F->dens);
+  hipLaunchKernelGGL(k_hilbert_seg<true>, dim3(F->ntile), dim3(HB_T), 0, F->st, F->tiles, F->soff, 1, 0, 0, F->x, F->y,
+                     F->dens, F->wdz, F->cf);
+  hipLaunchKernelGGL(k_lap_point, dim3(ew_blocks(n)), dim3(EW), 0, F->st, n, F->body_of, F->h, u, F->cf, F->rkappa,
+                     F->dsda, w);
+  EB_HIP_CHECK(hipGetLastError());
+  return EB_OK;
+}
+
+extern "C" int eb_sweep_field_rhs(eb_sweep_field F, const double* rku, double* rkup) {
+  if (!F || !rku || !rkup) return (eb_set_error("eb_sweep_field_rhs: bad arguments"), EB_ERR_ARG);
+  eb_bodies B = F->B;
+  const int n = F->n;
+  void* s = (void*)F->st;
+  int rc = eb_extractdat(B, rku, F->x, F->y, (double*)F->zk, (double*)F->dz, F->tha, F->dsda, F->rkappa,
+                         (double*)F->ckern, F->area, nullptr, s);
+  if (rc != EB_OK) return rc;
+  hipLaunchKernelGGL(k_wdz_f, dim3(ew_blocks(n)), dim3(EW), 0, F->st, n, F->dz, F->h, F->body_of, F->wdz);
+  EB_HIP_CHECK(hipGetLastError());
+  if (!F->have_geom) {
+    if ((rc = eb_sweep_create(&F->sw, B, F->nsim, F->sim_off.data(), F->x, F->y, (const double*)F->dz, F->dsda,
+                              F->rkappa, (const double*)F->zk, s)) != EB_OK)
+      return rc;
+    F->have_geom = true;
+  } else if ((rc = eb_sweep_rebuild(F->sw, F->x, F->y, (const double*)F->dz, F->dsda, F->rkappa, (const double*)F->zk,
+                                    s)) != EB_OK) {
+    return rc;
+  }
+  // GETBCs + SOLVE_LAPL of every simulation
+  hipLaunchKernelGGL(k_getbcs, dim3(ew_blocks(n)), dim3(EW), 0, F->st, n, F->x, F->rhs_lap, F->ulap);
+  EB_HIP_CHECK(hipGetLastError());
+  std::vector<double> err(F->nsim);
+  if ((rc = eb_gmres_batch_solve(F->glap, sf_lap_apply, F, F->rhs_lap, F->ulap, 1.0e-10, F->it_lap.data(), err.data(),
+                                 F->ie_lap.data())) != EB_OK)
+    return rc;
+  // PVINTEV + DIRICH_NEU_MAP
+  hipLaunchKernelGGL(k_pv_interp, dim3(B->nbody), dim3(SP_THREADS), lds_bytes(B, 2), F->st, view(B), F->x, F->y,
+                     F->ulap, F->dz, F->x2, F->y2, F->uf);
+  hipLaunchKernelGGL(k_hilbert_seg<false>, dim3(F->ntile), dim3(HB_T), 0, F->st, F->tiles, F->soff, 2, 0, 1, F->x2,
+                     F->y2, F->uf, F->wf, F->cf2);
+  hipLaunchKernelGGL(k_dn_post, dim3(B->nbody), dim3(SP_THREADS), lds_bytes(B, 1), F->st, view(B), F->x, F->y, F->ulap,
+                     F->cf2, F->dsda, nullptr, nullptr, F->dphi);
+  // SETVELF
+  hipLaunchKernelGGL(k_setvelf, dim3(B->nbody), dim3(SP_THREADS), lds_bytes(B, 2), F->st, view(B), F->x, F->y, F->dz,
+                     F->dsda, F->dphi, F->cap_b, F->e0_b, F->alpha_b, (double2*)F->rhs_st);
+  EB_HIP_CHECK(hipGetLastError());
+  // SOLVEI of every simulation
+  if ((rc = eb_sweep_solve(F->sw, F->gst, F->rhs_st, (double*)F->u, nullptr, nullptr, F->it_st.data(), err.data(),
+                           F->ie_st.data(), s)) != EB_OK)
+    return rc;
+  // INTERFACENEW: FDIFFF of u, CADAPMOD of every simulation, corrections
+  if ((rc = eb_fdifff(B, (const double*)F->u, (double*)F->udot, s)) != EB_OK) return rc;
+  if ((rc = eb_sweep_layer(F->sw, (const double*)F->u, (double*)F->gw, EB_VARIANT_MOD, s)) != EB_OK) return rc;
+  hipLaunchKernelGGL(k_interface_point, dim3(ew_blocks(n)), dim3(EW), 0, F->st, n, F->body_of, F->h, F->x, F->y, F->dz,
+                     F->dsda, F->rkappa, F->ckern, F->u, F->udot, F->gw, F->cap_b, F->un);
+  EB_HIP_CHECK(hipGetLastError());
+  return eb_ftail(B, F->tha, F->un, (const double*)F->dz, F->dsda, rkup, s);
+}
+
+extern "C" int eb_sweep_field_iterations(eb_sweep_field F, int32_t* lap_iter, int32_t* lap_ierr, int32_t* stokes_iter,
+                                         int32_t* stokes_ierr) {
+  if (!F) return (eb_set_error("eb_sweep_field_iterations: bad arguments"), EB_ERR_ARG);
+  for (int s = 0; s < F->nsim; ++s) {
+    if (lap_iter) lap_iter[s] = F->it_lap[s];
+    if (lap_ierr) lap_ierr[s] = F->ie_lap[s];
+    if (stokes_iter) stokes_iter[s] = F->it_st[s];
+    if (stokes_ierr) stokes_ierr[s] = F->ie_st[s];
+  }
   return EB_OK;
 }

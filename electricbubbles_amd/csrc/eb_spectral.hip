@@ -659,7 +659,7 @@ constexpr int SW_T = 128, SW_TILE = 2 * SW_T;
 __global__ __launch_bounds__(SW_T) void k_sw_gw(const int2* __restrict__ tiles, const int* __restrict__ soff,
                                                 const double* __restrict__ x, const double* __restrict__ y,
                                                 const double2* __restrict__ qa, const double2* __restrict__ qb,
-                                                double2* __restrict__ gw) {
+                                                double sgn, double2* __restrict__ gw) {
   __shared__ double2 sz[SW_TILE], sa[SW_TILE], sq[SW_TILE];
   const int2 tl = tiles[blockIdx.x];  // (simulation, first target within it)
   const int o = soff[tl.x], n = soff[tl.x + 1] - o;
@@ -693,9 +693,9 @@ __global__ __launch_bounds__(SW_T) void k_sw_gw(const int2* __restrict__ tiles, 
       }
     }
   }
-  // DR: GW = phi - z conj(phi') - conj(psi)
-  if (act0) gw[o + it0] = make_double2(a10.x - a20.x, a10.y - a20.y);
-  if (act1) gw[o + it1] = make_double2(a11.x - a21.x, a11.y - a21.y);
+  // GW = phi + sgn (z conj(phi') + conj(psi)): sgn = -1 for CADAPDR, +1 for CADAPMOD
+  if (act0) gw[o + it0] = make_double2(a10.x + sgn * a20.x, a10.y + sgn * a20.y);
+  if (act1) gw[o + it1] = make_double2(a11.x + sgn * a21.x, a11.y + sgn * a21.y);
 }
 
 // FASMVP's pointwise part (stokesbubb.f:585-611) with pinf = -pk of the simulation's first body
@@ -831,10 +831,41 @@ extern "C" int eb_sweep_apply(eb_sweep S, const double* xx, double* yy, void* st
   const int n = S->n;
   const double2* dens = (const double2*)xx;  // dens(i) = dcmplx(u(2i-1), u(2i)) (stokesbubb.f:527-529)
   hipLaunchKernelGGL(k_sw_charges, dim3((n + 255) / 256), dim3(256), 0, st, n, dens, S->wdz, S->qa, S->qb);
-  hipLaunchKernelGGL(k_sw_gw, dim3(S->ntile), dim3(SW_T), 0, st, S->tiles, S->soff, S->x, S->y, S->qa, S->qb, S->gw);
+  hipLaunchKernelGGL(k_sw_gw, dim3(S->ntile), dim3(SW_T), 0, st, S->tiles, S->soff, S->x, S->y, S->qa, S->qb, -1.0,
+                     S->gw);
   sweep_sums(S, dens, st);
   hipLaunchKernelGGL(k_sw_point, dim3((n + 255) / 256), dim3(256), 0, st, n, S->x, S->y, S->dz, S->dsdth, S->rkappa,
                      S->h, dens, S->za, S->pk, S->gw, (double2*)yy, S->body_of, S->first);
+  EB_HIP_CHECK(hipGetLastError());
+  return EB_OK;
+}
+
+extern "C" int eb_sweep_layer(eb_sweep S, const double* dens, double* gw, int32_t variant, void* stream) {
+  if (!S || !dens || !gw || (variant != EB_VARIANT_DR && variant != EB_VARIANT_MOD))
+    return (eb_set_error("eb_sweep_layer: bad arguments"), EB_ERR_ARG);
+  hipStream_t st = (hipStream_t)stream;
+  const int n = S->n;
+  hipLaunchKernelGGL(k_sw_charges, dim3((n + 255) / 256), dim3(256), 0, st, n, (const double2*)dens, S->wdz, S->qa,
+                     S->qb);
+  hipLaunchKernelGGL(k_sw_gw, dim3(S->ntile), dim3(SW_T), 0, st, S->tiles, S->soff, S->x, S->y, S->qa, S->qb,
+                     variant == EB_VARIANT_DR ? -1.0 : 1.0, (double2*)gw);
+  EB_HIP_CHECK(hipGetLastError());
+  return EB_OK;
+}
+
+extern "C" int eb_sweep_rebuild(eb_sweep S, const double* x, const double* y, const double* dz, const double* dsdth,
+                                const double* rkappa, const double* zk, void* stream) {
+  if (!S || !x || !y || !dz || !dsdth || !rkappa || !zk) return (eb_set_error("eb_sweep_rebuild: bad arguments"), EB_ERR_ARG);
+  hipStream_t st = (hipStream_t)stream;
+  const int n = S->n, nb = S->B->nbody;
+  EB_HIP_CHECK(hipMemcpyAsync(S->x, x, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+  EB_HIP_CHECK(hipMemcpyAsync(S->y, y, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+  EB_HIP_CHECK(hipMemcpyAsync(S->dsdth, dsdth, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+  EB_HIP_CHECK(hipMemcpyAsync(S->rkappa, rkappa, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+  EB_HIP_CHECK(hipMemcpyAsync(S->dz, dz, sizeof(double2) * n, hipMemcpyDeviceToDevice, st));
+  EB_HIP_CHECK(hipMemcpyAsync(S->zk, zk, sizeof(double2) * nb, hipMemcpyDeviceToDevice, st));
+  hipLaunchKernelGGL(k_wdz, dim3((n + 255) / 256), dim3(256), 0, st, n, (const double2*)S->dz, S->h, S->body_of,
+                     S->wdz);
   EB_HIP_CHECK(hipGetLastError());
   return EB_OK;
 }

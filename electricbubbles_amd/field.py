@@ -8,11 +8,14 @@ from __future__ import annotations
 
 import ctypes
 
+import numpy as np
+
 from . import _check, _ptr, _stream_ptr, _torch, lib
 
 FIELD_EXPORTS = ("eb_field_create", "eb_field_destroy", "eb_field_rhs", "eb_field_set_geometry",
                  "eb_field_laplace_apply", "eb_field_laplace_solve", "eb_field_dirich_neu", "eb_field_setvelf",
-                 "eb_field_stokes_solve", "eb_field_interface", "eb_field_get_geometry")
+                 "eb_field_stokes_solve", "eb_field_interface", "eb_field_get_geometry", "eb_sweep_field_create",
+                 "eb_sweep_field_rhs", "eb_sweep_field_iterations", "eb_sweep_field_destroy")
 
 _vp = ctypes.c_void_p
 _i32 = ctypes.c_int32
@@ -33,6 +36,10 @@ def _bind():
         L.eb_field_setvelf.argtypes = [_vp, _vp, _vp]
         L.eb_field_stokes_solve.argtypes = [_vp, _vp, _vp, _vp, _vp, ctypes.POINTER(_i32), ctypes.POINTER(_i32)]
         L.eb_field_interface.argtypes = [_vp, _vp, _vp]
+        L.eb_sweep_field_create.argtypes = [ctypes.POINTER(_vp), _vp, _i32, _vp, _vp, _vp, _vp]
+        L.eb_sweep_field_rhs.argtypes = [_vp, _vp, _vp]
+        L.eb_sweep_field_iterations.argtypes = [_vp, _vp, _vp, _vp, _vp]
+        L.eb_sweep_field_destroy.argtypes = [_vp]
         _bound = True
     return L
 
@@ -102,6 +109,46 @@ class Field:
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
             _bind().eb_field_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class SweepField:
+    """F of many independent simulations at once: simulation s owns bodies
+    sim_off[s]..sim_off[s+1]-1 of `bodies` and has its own CAP and E0."""
+
+    def __init__(self, bodies, sim_off, cap, e0, stream=None):
+        self._torch = _torch()
+        self.bodies = bodies
+        self.stream = stream
+        self.sim_off = np.ascontiguousarray(sim_off, np.int32)
+        self.nsim = self.sim_off.size - 1
+        cap = np.ascontiguousarray(np.broadcast_to(np.asarray(cap, np.float64), (self.nsim,)))
+        e0 = np.ascontiguousarray(np.broadcast_to(np.asarray(e0, np.float64), (self.nsim,)))
+        h = _vp()
+        _check(_bind().eb_sweep_field_create(ctypes.byref(h), bodies._h, self.nsim, self.sim_off.ctypes.data,
+                                             cap.ctypes.data, e0.ctypes.data, _stream_ptr(stream)))
+        self._h = h
+
+    def rhs(self, rku):
+        out = self._torch.empty(rku.numel(), dtype=self._torch.float64, device=rku.device)
+        _check(_bind().eb_sweep_field_rhs(self._h, _ptr(rku.contiguous()), _ptr(out)))
+        return out
+
+    def iterations(self):
+        """(Laplace iterations, Laplace IERR, Stokes iterations, Stokes IERR) of the last call."""
+        a = [np.zeros(self.nsim, np.int32) for _ in range(4)]
+        _check(_bind().eb_sweep_field_iterations(self._h, *(v.ctypes.data for v in a)))
+        return tuple(a)
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _bind().eb_sweep_field_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -62,6 +62,26 @@ typedef struct {
 } eb_field_geometry;
 int eb_field_get_geometry(eb_field f, eb_field_geometry *g);
 
+/* ---------------------------------------------------------------------------------------
+ * F of many independent simulations at once (a parameter sweep).  Simulation s owns bodies
+ * [sim_off[s], sim_off[s+1]) of `bodies` (nsim + 1 host ints) and has its own CAP and E0
+ * (cap, e0: nsim host doubles).  Every stage is one launch for all simulations; the Laplace
+ * and Stokes solves run in the batched GMRES, one system per simulation; every layer
+ * potential sums over the simulation's own points.  The reference runs one simulation per
+ * process: there is no counterpart in it.
+ * --------------------------------------------------------------------------------------- */
+typedef struct eb_sweep_field_s *eb_sweep_field;
+
+int eb_sweep_field_create(eb_sweep_field *f, eb_bodies bodies, int32_t nsim, const int32_t *sim_off,
+                          const double *cap, const double *e0, void *stream);
+/* rkup = F(rku) of every simulation (all bodies back to back, interleaved (x, y)). */
+int eb_sweep_field_rhs(eb_sweep_field f, const double *rku, double *rkup);
+/* GMRES iteration counts and IERR of the last call's Laplace and Stokes solves (nsim each;
+ * any may be NULL). */
+int eb_sweep_field_iterations(eb_sweep_field f, int32_t *lap_iter, int32_t *lap_ierr, int32_t *stokes_iter,
+                              int32_t *stokes_ierr);
+int eb_sweep_field_destroy(eb_sweep_field f);
+
 #ifdef __cplusplus
 }
 #endif

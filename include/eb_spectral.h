@@ -101,6 +101,12 @@ int eb_sweep_apply(eb_sweep sw, const double *xx, double *yy, void *stream);
 /* za (nbody complex) and pk (nbody, after each simulation's pinf shift) of a density;
  * either output may be NULL. */
 int eb_sweep_parts(eb_sweep sw, const double *xx, double *za, double *pk, void *stream);
+/* The layer potential alone, GW of CADAPDR (EB_VARIANT_DR) or CADAPMOD (EB_VARIANT_MOD), of every
+ * simulation over its own points: dens and gw are nbk complex. */
+int eb_sweep_layer(eb_sweep sw, const double *dens, double *gw, int32_t variant, void *stream);
+/* New geometry for the same bodies (the RK stages of a time step). */
+int eb_sweep_rebuild(eb_sweep sw, const double *x, const double *y, const double *dz, const double *dsdth,
+                     const double *rkappa, const double *zk, void *stream);
 /* unknowns of each simulation: offsets into the 2*nbk vector (nsim + 1 ints, host memory) */
 const int32_t *eb_sweep_unknown_offsets(eb_sweep sw);
 int eb_sweep_destroy(eb_sweep sw);

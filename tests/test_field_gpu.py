@@ -112,3 +112,84 @@ def test_rk2_steps(torch, tag):
         spectral.rk2_stage2(dt, phi0, F.rhs(y), ybar)
     assert rel(ybar.cpu().numpy(), f["rk2"]) < 1e-10
     assert np.abs(ybar.cpu().numpy() - f["rk2"]).max() < 1e-8
+
+
+def test_sweep_field_matches_reference(torch):
+    """The three geometries as three simulations of one sweep (each with its own CAP and E0):
+    F of each against the reference's F, and three RK2 steps against the reference's RK2."""
+    from electricbubbles_amd import field, spectral
+    fs = [np.load(os.path.join(GOLDEN, f"field_{t}.npz")) for t in CASES]
+    nd = np.concatenate([f["nd"] for f in fs]).astype(np.int32)
+    sim_off = np.concatenate([[0], np.cumsum([f["nd"].size for f in fs])]).astype(np.int32)
+    B = spectral.Bodies(nd)
+    SF = field.SweepField(B, sim_off, [float(f["cap"]) for f in fs], [float(f["e0"]) for f in fs])
+    rku = torch.tensor(np.concatenate([f["rku"] for f in fs]), device="cuda")
+    rkup = SF.rhs(rku).cpu().numpy()
+    o = 0
+    for f in fs:
+        m = f["rku"].size
+        assert rel(rkup[o:o + m], f["rkup"]) < 1e-8
+        o += m
+    lap_it, lap_ie, st_it, st_ie = SF.iterations()
+    assert np.all(lap_ie == 0) and np.all(st_ie == 0)
+    dt = float(fs[0]["rk2_dt"])
+    ybar = rku.clone()
+    for _ in range(int(fs[0]["rk2_steps"])):
+        phi0, y = spectral.rk2_stage1(dt, ybar, SF.rhs(ybar))
+        spectral.rk2_stage2(dt, phi0, SF.rhs(y), ybar)
+    yb = ybar.cpu().numpy()
+    o = 0
+    for f in fs:
+        m = f["rku"].size
+        assert np.abs(yb[o:o + m] - f["rk2"]).max() < 1e-8
+        o += m
+
+
+def test_sweep_field_matches_plain_fields(torch):
+    """Simulations of one to three drops with uneven lengths and parameters: the sweep's F of
+    each equals the plain per-simulation F (both take the O(N^2) sums at these sizes)."""
+    from electricbubbles_amd import field, spectral
+    from electricbubbles_amd.workloads import perturbed_circle
+    rng = np.random.default_rng(21)
+    nds, zs, sim_off, caps, e0s = [], [], [0], [], []
+    for nb in [1, 2, 1, 3, 1]:
+        for b in range(nb):
+            n = int(rng.choice([128, 130, 194, 256, 322]))
+            zs.append(perturbed_circle(n, [(2, 0.05), (int(rng.integers(3, 7)), rng.uniform(-0.03, 0.03))],
+                                       center=3.0 * b + 0.4j * b)[0])
+            nds.append(n)
+        sim_off.append(sim_off[-1] + nb)
+        caps.append(rng.uniform(0.0, 0.2))
+        e0s.append(rng.uniform(0.1, 2.0))
+    nd = np.array(nds, np.int32)
+    B = spectral.Bodies(nd)
+    SF = field.SweepField(B, sim_off, caps, e0s)
+    rku = torch.tensor(np.concatenate(zs).view(np.float64).copy(), device="cuda")
+    got = SF.rhs(rku).cpu().numpy()
+    off = np.concatenate([[0], np.cumsum(nd)])
+    for s in range(len(caps)):
+        b0, b1 = sim_off[s], sim_off[s + 1]
+        Bs = spectral.Bodies(nd[b0:b1])
+        F = field.Field(Bs, cap=caps[s], e0=e0s[s])
+        ref = F.rhs(rku[2 * off[b0]:2 * off[b1]].contiguous()).cpu().numpy()
+        assert rel(got[2 * off[b0]:2 * off[b1]], ref) < 1e-10, s
+        F.close()
+
+
+def test_sweep_field_512_single_drops(torch):
+    """The batched-sweep shapes (512 single drops, N log-uniform 128..2048 rounded to even,
+    E0 = |E|^2 with |E| uniform in 0.1..2 per simulation, no background flow): every solve
+    converges and F is finite."""
+    from electricbubbles_amd import field, spectral
+    from electricbubbles_amd.workloads import perturbed_circle
+    rng = np.random.default_rng(5120)
+    nd = (np.round(np.exp(rng.uniform(np.log(128), np.log(2048), 512)) / 2) * 2).astype(np.int32)
+    print("sweep field, 512 drops, points", int(nd.sum()), flush=True)
+    B = spectral.Bodies(nd)
+    SF = field.SweepField(B, np.arange(513), np.zeros(512), rng.uniform(0.1, 2.0, 512) ** 2)
+    rku = torch.tensor(np.concatenate([perturbed_circle(int(n), [(2, 0.05)])[0] for n in nd]).view(np.float64).copy(),
+                       device="cuda")
+    out = SF.rhs(rku)
+    assert bool(torch.isfinite(out).all())
+    lap_it, lap_ie, st_it, st_ie = SF.iterations()
+    assert np.all(lap_ie == 0) and np.all(st_ie == 0)
