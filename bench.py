@@ -201,7 +201,9 @@ def sweep_leg(torch, dev, with_cpu):
     per-simulation GPU path (one MatVec, one GMRES workspace, one solve per simulation, the
     direct sum that EB_MODE_AUTO picks at N <= 2048), timed on every 16th simulation and
     reported per solve.  The reference leg runs SOLVEI's solve of a few simulations on one
-    host core."""
+    host core.  The "F" part does the same for whole F evaluations (electric.f:968-1109: the
+    Laplace solve, the Dirichlet-to-Neumann map, SETVELF, SOLVEI, INTERFACENEW and the tail),
+    with E0 = |E|^2, |E| uniform in 0.1..2 per simulation and no background flow."""
     from electricbubbles_amd import gmres, spectral
     from electricbubbles_amd.workloads import perturbed_circle
     rng = np.random.default_rng(20260512)
@@ -262,6 +264,41 @@ def sweep_leg(torch, dev, with_cpu):
            "plain_vs_batched_solution_rel_diff": worst,
            "sweep_apply_ms": round(t_apply * 1e3, 4),
            "sweep_apply_TFLOPs": round(50.0 * pairs / t_apply / 1e12, 2), "rel_residual": res}
+    # whole F evaluations (electric.f:968-1109) of every simulation: batched and plain
+    from electricbubbles_amd import field
+    e0 = rng.uniform(0.1, 2.0, nsim) ** 2  # E0 = |E|^2, |E| uniform in 0.1..2
+    rku_t = torch.tensor(rku, device=dev)
+    SF = field.SweepField(B, np.arange(nsim + 1), np.zeros(nsim), e0)
+    SF.rhs(rku_t)  # warm-up (first geometry)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fb = SF.rhs(rku_t)
+    torch.cuda.synchronize()
+    t_fb = (time.perf_counter() - t0) / reps
+    lap_it, _, st_it, _ = SF.iterations()
+    t_fp, worst_f = 0.0, 0.0
+    for s in sample:
+        p0, p1 = off[s], off[s + 1]
+        Fs = field.Field(spectral.Bodies(nd[s:s + 1]), cap=0.0, e0=float(e0[s]))
+        r = rku_t[2 * p0:2 * p1].contiguous()
+        Fs.rhs(r)  # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fp = Fs.rhs(r)
+        torch.cuda.synchronize()
+        t_fp += time.perf_counter() - t0
+        worst_f = max(worst_f, float(torch.linalg.norm(fp - fb[2 * p0:2 * p1]) / torch.linalg.norm(fp)))
+        Fs.close()
+    per_fp = t_fp / len(sample)
+    out["F"] = {"what": "whole F per simulation: EXTRACTDAT, Laplace solve, Dirichlet-to-Neumann map, SETVELF, "
+                        "SOLVEI, INTERFACENEW, tail",
+                "laplace_iterations_mean": round(float(lap_it.mean()), 2),
+                "stokes_iterations_mean": round(float(st_it.mean()), 2),
+                "batched_ms": round(t_fb * 1e3, 3), "batched_F_per_s": round(nsim / t_fb, 1),
+                "plain_ms_per_F": round(per_fp * 1e3, 3), "plain_F_per_s": round(1.0 / per_fp, 1),
+                "batched_vs_plain": round((nsim / t_fb) * per_fp, 2),
+                "plain_vs_batched_rel_diff": worst_f}
     if with_cpu:
         import tempfile
         import oracle
@@ -270,7 +307,15 @@ def sweep_leg(torch, dev, with_cpu):
             os.chdir(tempfile.mkdtemp())
             sim = oracle.RefSim(oracle.LIBREF)
             t_ref, n_ref = 0.0, 0
+            t_rf, n_rf = 0.0, 0
             try:
+                for s in sample[:4]:
+                    p0, p1 = off[s], off[s + 1]
+                    sim.setup(nd[s:s + 1], cap=0.0, e0=float(e0[s]))
+                    t0 = time.perf_counter()
+                    sim.rhs(rku[2 * p0:2 * p1])
+                    t_rf += time.perf_counter() - t0
+                    n_rf += 1
                 for s in sample[:8]:
                     p0, p1 = off[s], off[s + 1]
                     g = {k: geo[k][p0:p1].cpu().numpy() for k in ("x", "y", "dz", "dsda", "rkappa")}
@@ -284,7 +329,66 @@ def sweep_leg(torch, dev, with_cpu):
                 os.chdir(cwd)
             out["cpu_reference"] = {"solves_per_s": round(n_ref / t_ref, 2), "cores": 1, "kind": "reference",
                                     "sample": f"SOLVEI's solve of {n_ref} of the sampled simulations"}
+            out["F"]["cpu_reference"] = {"F_per_s": round(n_rf / t_rf, 2), "cores": 1, "kind": "reference",
+                                         "sample": f"the reference's F of {n_rf} of the sampled simulations"}
     gb.close()
+    return out
+
+
+def field_leg(torch, dev, with_cpu):
+    """F and the RK2 step (dt = 5e-4) of one simulation on the device, for the many-drop
+    configuration (64 drops, about 28K points, E = (1.5, 0): E0 = 2.25, no background flow): every layer
+    potential on the FMM (LEVMAX 7), the Laplace and Stokes solves in the device GMRES.  The
+    reference's F cannot run this size as written (PVINTEV stops above 12,500 points,
+    pv_integral.f:131-135), so the reference leg times its F on a 16-drop configuration of
+    the same construction, next to the device F on that same configuration."""
+    from electricbubbles_amd import field, spectral
+    from electricbubbles_amd.workloads import many_drops_cfg
+    out = {}
+    for tag, ndrops, seed in (("many_drop_64", 64, 20260506), ("many_drop_16", 16, 20260516)):
+        nd, rku = many_drops_cfg(ndrops, np.random.default_rng(seed))
+        B = spectral.Bodies(nd)
+        F = field.Field(B, cap=0.0, e0=2.25)
+        y0 = torch.tensor(rku, device=dev)
+        F.rhs(y0)  # warm-up: plans and trees
+        torch.cuda.synchronize()
+        reps = 3
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            dy = F.rhs(y0)
+        torch.cuda.synchronize()
+        t_f = (time.perf_counter() - t0) / reps
+        lap_it, _, st_it, _ = F.iterations()
+        ybar = y0.clone()
+        dt = 5e-4  # the many-drop configuration's step
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            phi0, y = spectral.rk2_stage1(dt, ybar, F.rhs(ybar))
+            spectral.rk2_stage2(dt, phi0, F.rhs(y), ybar)
+        torch.cuda.synchronize()
+        t_step = (time.perf_counter() - t0) / reps
+        o = {"workload": f"{ndrops} drops, {int(nd.sum())} points", "ms_per_F": round(t_f * 1e3, 3),
+             "laplace_iterations": lap_it, "stokes_iterations": st_it, "rk2_dt": dt,
+             "ms_per_rk2_step": round(t_step * 1e3, 3), "steps_per_s": round(1.0 / t_step, 2),
+             "iterations_after_steps": list(F.iterations()[::2])}
+        if with_cpu and ndrops == 16:
+            import tempfile
+            import oracle
+            if oracle.ref_available():
+                cwd = os.getcwd()
+                os.chdir(tempfile.mkdtemp())
+                try:
+                    sim = oracle.RefSim(oracle.LIBREF)
+                    sim.setup(nd, cap=0.0, e0=2.25)
+                    t0 = time.perf_counter()
+                    ref = sim.rhs(rku)
+                    t_ref = time.perf_counter() - t0
+                finally:
+                    os.chdir(cwd)
+                o["cpu_reference"] = {"s_per_F": round(t_ref, 3), "cores": 1, "kind": "reference",
+                                      "F_rel_diff": float(np.linalg.norm(dy.cpu().numpy() - ref) / np.linalg.norm(ref))}
+        out[tag] = o
+        F.close()
     return out
 
 
@@ -302,6 +406,7 @@ def main():
     ap.add_argument("--no-solve", action="store_true", help="skip the Stokes GMRES solve leg")
     ap.add_argument("--no-spectral", action="store_true", help="skip the spectral geometry leg")
     ap.add_argument("--no-sweep", action="store_true", help="skip the batched-versus-plain sweep leg")
+    ap.add_argument("--no-field", action="store_true", help="skip the F and RK2 step leg")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 256 MiB device copy first (the known-size kernel that scales HBM counters)")
     args = ap.parse_args()
@@ -309,6 +414,12 @@ def main():
         print("bench.py: this build runs on one GPU only (the FMM is measured on one MI355X); "
               "--gpus above 1 is not supported", file=sys.stderr)
         sys.exit(2)
+    # The one JSON line goes to the original stdout; everything else written to fd 1 (the
+    # reference Fortran's unit-6 printing in the CPU legs) is discarded.
+    json_fd = os.dup(1)
+    devnull = os.open(os.devnull, os.O_WRONLY)
+    os.dup2(devnull, 1)
+    os.close(devnull)
 
     import torch
     import electricbubbles_amd as eb
@@ -406,6 +517,7 @@ def main():
     solve = None if args.no_solve else stokes_solve_leg(torch, dev, not args.no_cpu)
     spec = None if args.no_spectral else spectral_leg(torch, dev, not args.no_cpu)
     sweep = None if args.no_sweep else sweep_leg(torch, dev, not args.no_cpu)
+    fld = None if args.no_field else field_leg(torch, dev, not args.no_cpu)
 
     cpu = None
     if not args.no_cpu:
@@ -435,6 +547,7 @@ def main():
         "stokes_solve": solve,
         "spectral": spec,
         "sweep": sweep,
+        "field": fld,
         "fp64_fraction_total": round(total_fl / (t_apply) / 1e12 / PEAK_FP64_TFLOPS, 4),
         "roofline": {"bound": "mfma", "kernel": dom, "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
@@ -442,7 +555,9 @@ def main():
         "accuracy_vs_direct": check,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(line), flush=True)
+    sys.stdout.flush()
+    with os.fdopen(json_fd, "w") as jf:
+        jf.write(json.dumps(line) + "\n")
     plan.close()
 
 

@@ -300,6 +300,7 @@ struct eb_field_s {
   double cap = 0.0, e0 = 0.0, alpha_last = 0.0;
   hipStream_t st = nullptr;
   bool have_geom = false;
+  int32_t it_lap = 0, ie_lap = 0, it_st = 0, ie_st = 0;  // the last call's GMRES counts
   int nt_lap = 37, nt_pv = 33, nt_mod = 43;  // TOL 1e-11 (laplace.f:212), 1e-10 (pv_integral.f:175), 1e-13
   // geometry
   double *x = nullptr, *y = nullptr, *tha = nullptr, *dsda = nullptr, *rkappa = nullptr, *area = nullptr,
@@ -464,6 +465,8 @@ extern "C" int eb_field_laplace_solve(eb_field F, double* ulap, int32_t* iter, i
   double err = 0.0;
   // SOLVE_LAPL: ITOL 0, TOL 1e-10, IGWK(5) = -1, MAXL 100 (laplace.f:355-384; electric.f:1035)
   rc = eb_gmres_solve(F->glap, lap_gmres_apply, F, F->rhs_lap, ulap, 1.0e-10, -1, &it, &err, &ie);
+  F->it_lap = it;
+  F->ie_lap = ie;
   if (iter) *iter = it;
   if (ierr) *ierr = ie;
   return rc;
@@ -510,6 +513,8 @@ extern "C" int eb_field_stokes_solve(eb_field F, const double* rhs, double* u, d
   int32_t it = 0, ie = 0;
   double err = 0.0;
   rc = eb_stokes_solve(F->mv, F->gst, rhs, u, za, pk, &it, &err, &ie, (void*)F->st);
+  F->it_st = it;
+  F->ie_st = ie;
   if (iter) *iter = it;
   if (ierr) *ierr = ie;
   return rc;
@@ -541,6 +546,16 @@ extern "C" int eb_field_rhs(eb_field F, const double* rku, double* rkup) {
   if ((rc = eb_field_stokes_solve(F, F->rhs_st, (double*)F->u, (double*)F->za, F->pk, &it, &ie)) != EB_OK) return rc;
   if ((rc = eb_field_interface(F, (const double*)F->u, F->un)) != EB_OK) return rc;
   return eb_ftail(F->B, F->tha, F->un, (const double*)F->dz, F->dsda, rkup, (void*)F->st);
+}
+
+extern "C" int eb_field_iterations(eb_field F, int32_t* lap_iter, int32_t* lap_ierr, int32_t* stokes_iter,
+                                   int32_t* stokes_ierr) {
+  if (!F) return (eb_set_error("eb_field_iterations: bad arguments"), EB_ERR_ARG);
+  if (lap_iter) *lap_iter = F->it_lap;
+  if (lap_ierr) *lap_ierr = F->ie_lap;
+  if (stokes_iter) *stokes_iter = F->it_st;
+  if (stokes_ierr) *stokes_ierr = F->ie_st;
+  return EB_OK;
 }
 
 extern "C" int eb_field_get_geometry(eb_field F, eb_field_geometry* g) {

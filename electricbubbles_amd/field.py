@@ -14,7 +14,7 @@ from . import _check, _ptr, _stream_ptr, _torch, lib
 
 FIELD_EXPORTS = ("eb_field_create", "eb_field_destroy", "eb_field_rhs", "eb_field_set_geometry",
                  "eb_field_laplace_apply", "eb_field_laplace_solve", "eb_field_dirich_neu", "eb_field_setvelf",
-                 "eb_field_stokes_solve", "eb_field_interface", "eb_field_get_geometry", "eb_sweep_field_create",
+                 "eb_field_stokes_solve", "eb_field_interface", "eb_field_get_geometry", "eb_field_iterations", "eb_sweep_field_create",
                  "eb_sweep_field_rhs", "eb_sweep_field_iterations", "eb_sweep_field_destroy")
 
 _vp = ctypes.c_void_p
@@ -36,6 +36,7 @@ def _bind():
         L.eb_field_setvelf.argtypes = [_vp, _vp, _vp]
         L.eb_field_stokes_solve.argtypes = [_vp, _vp, _vp, _vp, _vp, ctypes.POINTER(_i32), ctypes.POINTER(_i32)]
         L.eb_field_interface.argtypes = [_vp, _vp, _vp]
+        L.eb_field_iterations.argtypes = [_vp] + [ctypes.POINTER(_i32)] * 4
         L.eb_sweep_field_create.argtypes = [ctypes.POINTER(_vp), _vp, _i32, _vp, _vp, _vp, _vp]
         L.eb_sweep_field_rhs.argtypes = [_vp, _vp, _vp]
         L.eb_sweep_field_iterations.argtypes = [_vp, _vp, _vp, _vp, _vp]
@@ -105,6 +106,12 @@ class Field:
         un = self._f64(self.bodies.n, u)
         _check(_bind().eb_field_interface(self._h, _ptr(u.contiguous()), _ptr(un)))
         return un
+
+    def iterations(self):
+        """(Laplace iterations, Laplace IERR, Stokes iterations, Stokes IERR) of the last solves."""
+        v = [_i32() for _ in range(4)]
+        _check(_bind().eb_field_iterations(self._h, *(ctypes.byref(a) for a in v)))
+        return tuple(a.value for a in v)
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:

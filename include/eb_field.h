@@ -56,6 +56,9 @@ int eb_field_stokes_solve(eb_field f, const double *rhs, double *u, double *za, 
 /* INTERFACENEW: the normal velocity un (nbk) from the density u. */
 int eb_field_interface(eb_field f, const double *u, double *un);
 
+/* GMRES iteration counts and IERR of the last Laplace and Stokes solves (any may be NULL). */
+int eb_field_iterations(eb_field f, int32_t *lap_iter, int32_t *lap_ierr, int32_t *stokes_iter, int32_t *stokes_ierr);
+
 /* Geometry of the current call (device pointers owned by the field). */
 typedef struct {
   const double *x, *y, *dz, *tha, *dsda, *rkappa, *ckern, *zk;

@@ -307,6 +307,18 @@ def main_field():
                  lap_w=w[:nbk].copy(), rk2=traj, rk2_dt=np.float64(dt), rk2_steps=np.int32(nsteps),
                  **{k: v for k, v in ch.items() if k in ("rhs_lap", "ulap", "phi", "psi", "dphi_dn", "rhs_stokes",
                                                          "u", "za", "pk", "un")})
+        # a many-drop configuration (8 drops of 256..1024 points, E = (1.5, 0)): F and two RK2
+        # steps at the configuration's dt
+        from electricbubbles_amd.workloads import many_drops_cfg
+        nd, rku = many_drops_cfg(8, np.random.default_rng(20260508))
+        sim = oracle.RefSim(oracle.LIBREF)
+        sim.setup(nd, cap=0.0, e0=2.25)
+        rkup = sim.rhs(rku)
+        sim.setup(nd, cap=0.0, e0=2.25)
+        traj = sim.rk2(rku, 5e-4, 2)
+        print("many_drops_8 points", int(nd.sum()), "F norm", np.linalg.norm(rkup))
+        save("field_many_drops_8", nd=nd, rku=rku, cap=np.float64(0.0), e0=np.float64(2.25), rkup=rkup, rk2=traj,
+             rk2_dt=np.float64(5e-4), rk2_steps=np.int32(2))
     finally:
         os.chdir(cwd)
 

@@ -193,3 +193,21 @@ def test_sweep_field_512_single_drops(torch):
     assert bool(torch.isfinite(out).all())
     lap_it, lap_ie, st_it, st_ie = SF.iterations()
     assert np.all(lap_ie == 0) and np.all(st_ie == 0)
+
+
+def test_many_drops_rhs_and_rk2(torch):
+    """Eight drops of the many-drop construction (256..1024 points each, E0 = 2.25): the whole
+    F and two RK2 steps at the configuration's dt against the reference's."""
+    from electricbubbles_amd import field, spectral
+    f = np.load(os.path.join(GOLDEN, "field_many_drops_8.npz"))
+    print("many drops", list(f["nd"]), flush=True)
+    B = spectral.Bodies(f["nd"])
+    F = field.Field(B, cap=float(f["cap"]), e0=float(f["e0"]))
+    rku = torch.tensor(f["rku"], device="cuda")
+    assert rel(F.rhs(rku).cpu().numpy(), f["rkup"]) < 1e-8
+    dt = float(f["rk2_dt"])
+    ybar = rku.clone()
+    for _ in range(int(f["rk2_steps"])):
+        phi0, y = spectral.rk2_stage1(dt, ybar, F.rhs(ybar))
+        spectral.rk2_stage2(dt, phi0, F.rhs(y), ybar)
+    assert np.abs(ybar.cpu().numpy() - f["rk2"]).max() < 1e-8
