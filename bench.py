@@ -407,6 +407,8 @@ def main():
     ap.add_argument("--no-spectral", action="store_true", help="skip the spectral geometry leg")
     ap.add_argument("--no-sweep", action="store_true", help="skip the batched-versus-plain sweep leg")
     ap.add_argument("--no-field", action="store_true", help="skip the F and RK2 step leg")
+    ap.add_argument("--only", choices=["solve", "spectral", "sweep", "field"],
+                    help="run one secondary leg alone and print its JSON object (for profiling runs)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 256 MiB device copy first (the known-size kernel that scales HBM counters)")
     args = ap.parse_args()
@@ -429,6 +431,13 @@ def main():
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
     dev = torch.device("cuda:0")
+    if args.only:
+        legs = {"solve": stokes_solve_leg, "spectral": spectral_leg, "sweep": sweep_leg, "field": field_leg}
+        res = {"leg": args.only, args.only: legs[args.only](torch, dev, not args.no_cpu)}
+        sys.stdout.flush()
+        with os.fdopen(json_fd, "w") as jf:
+            jf.write(json.dumps(res) + "\n")
+        return
     rng = np.random.default_rng(20260303)
     n = args.npts
     if args.workload == "curves":
