@@ -301,6 +301,7 @@ def lists(ilev, device="cuda", stream=None):
     return ([intl[b, : nintl[b]].copy() for b in range(nb)], [nbor[b, : nnbor[b]].copy() for b in range(nb)])
 
 
-from . import atree, field, gmres, spectral  # noqa: E402
+from . import atree, field, gmres, sim, spectral  # noqa: E402
 
-EXPORTS = EXPORTS + spectral.SPECTRAL_EXPORTS + atree.ATREE_EXPORTS + gmres.GMRES_EXPORTS + field.FIELD_EXPORTS
+EXPORTS = (EXPORTS + spectral.SPECTRAL_EXPORTS + atree.ATREE_EXPORTS + gmres.GMRES_EXPORTS + field.FIELD_EXPORTS
+           + sim.SIM_EXPORTS)

@@ -398,6 +398,10 @@ __global__ __launch_bounds__(256) void k_xlat(XlatLevels LS, const int* __restri
     box_range(leaf_start, LS.L, l, ix, iy, bs, be);
     live = be > bs;
   }
+  // A wave whose four target boxes are all empty has nothing to write (their local
+  // expansions are never read: L2L and L2P only visit non-empty boxes).  No barrier follows,
+  // so the whole wave may leave; sparse geometries leave most waves here.
+  if (MODE != XL_M2M && __ballot(live) == 0) return;
   const int pt = pidx(ix, iy, Wt);
   auto src_box = [&](int o) -> int {
     if (MODE == XL_M2L) return pt + offs[(cls * 27 + o) * 2 + 1] * Ws + offs[(cls * 27 + o) * 2 + 0];

@@ -837,3 +837,101 @@ def ref_field_chain(nd, rku, cap=0.0, e0=0.0, maxl=100):
     return dict(x=x, y=y, z=z, zk=zk, dz=dz, dsda=dsda, rkappa=rkappa, tha=tha, ckern=ckern.view(np.complex128),
                 rhs_lap=rhs_lap, ulap=ulap, phi=phi, psi=psi, dphi_dn=dphi, rhs_stokes=rhs_stokes,
                 u=u.view(np.complex128), pk=pk, za=za.view(np.complex128), un=un)
+
+
+def ref_manydrop(p, nsteps):
+    """MANYDROP's run (electric.f:62-245) with the reference's own routines, for a run
+    description p (dict: k, nd, dt, tfinal, cap, e0, ai, bi, thetax, cx, cy, ncyc; no gravity):
+    GEOINIT, CLUSTPARAM, GEOSHIFT, INITCOND, then nsteps passes of the time loop (RK2,
+    EXTRACTSOL, GETFOURIER, DOUBLE, the step law).  Plotting is left out.  Run it from a
+    scratch directory: the reference writes fort.13.  Returns the state after the steps and
+    the per-step (time, dt, total area)."""
+    r = ref()
+    kmax, ndmax = 150, 3000
+    nmax = ndmax * kmax
+    k = int(p["k"])
+    sim = RefSim(LIBREF)
+    sim.lib = r
+    nd0 = np.asarray(p["nd"], np.int32)
+    sim.setup(nd0, cap=float(p["cap"]), zbond=0j, e0=float(p["e0"]))
+    prob = (ctypes.c_char * (16 * kmax + 4 * kmax + 12)).from_address(
+        ctypes.addressof(ctypes.c_char.in_dll(r, "problem_")))
+    c_nd = ctypes.c_void_p(ctypes.addressof(prob) + 16 * kmax)  # COMMON /problem/ nd, as MANYDROP's nd
+    ndv = np.frombuffer(prob, dtype=np.int32, count=kmax, offset=16 * kmax)
+    ints = np.frombuffer(prob, dtype=np.int32, count=3, offset=20 * kmax)
+    f64 = lambda key: np.ascontiguousarray(np.asarray(p[key], np.float64)[:k])  # noqa: E731
+    ai, bi, th, cx, cy = f64("ai"), f64("bi"), f64("thetax"), f64("cx"), f64("cy")
+    ncyc = np.ascontiguousarray(np.asarray(p["ncyc"], np.int32)[:k])
+    zk = (cx + 1j * cy).astype(np.complex128)
+    nbk = int(nd0.sum())
+    z = np.zeros((k, ndmax), np.complex128)  # Fortran z(ndmax, k)
+    dsdth, rkappa = np.zeros(nmax), np.zeros(nmax)
+    area0, arcl0 = np.zeros(k), np.zeros(k)
+    r.geoinit_(_c_int(k), c_nd, _c_int(nbk), _d(ai), _d(bi), _d(th), _i(ncyc), _d(_cplx(zk)), _d(z), _d(dsdth),
+               _d(rkappa), _c_int(10), _d(area0), _c_int(ndmax))
+    ifwork = 10 * ndmax + 50
+    wsave = np.zeros(2 * ifwork * kmax)
+    za, zai = np.zeros(2 * nmax), np.zeros(2 * nmax)
+    beta = np.zeros(nmax)
+    r.clustparam_(_c_int(k), c_nd, _c_int(nbk), _d(dsdth), _d(arcl0), _d(beta), _d(za), _d(zai), _d(wsave),
+                  _c_int(ifwork))
+    r.geoshift_(_c_int(k), c_nd, _c_int(nbk), _d(ai), _d(bi), _d(th), _i(ncyc), _d(_cplx(zk)), _d(beta), _d(z),
+                _d(rkappa), _c_int(10), _c_int(ndmax))
+    dt = float(p["dt"])
+    cmin = max(dt * ndv[b] / arcl0[b] for b in range(k))
+    neqmax = 2 * nmax
+    ybar, y, dy = np.zeros(neqmax), np.zeros(neqmax), np.zeros(neqmax)
+    phi = np.zeros(4 * neqmax)
+    nbk_c, neq_c = ctypes.c_int(0), ctypes.c_int(0)
+    r.initcond_(_c_int(k), _c_int(ndmax), c_nd, ctypes.byref(nbk_c), ctypes.byref(neq_c), _d(z), _d(ybar))
+    ints[:] = [k, nbk_c.value, neq_c.value]
+    areatot0 = float(area0.sum())
+    time, tfinal = 0.0, float(p["tfinal"])
+    dz = np.zeros(2 * nmax)
+    amode, fmode = np.zeros(nmax), np.zeros(nmax)
+    z2 = np.zeros(2 * nmax)
+    nzwork = 32 * nmax + 100
+    zwork = np.zeros(2 * nzwork)
+    area, arcl = np.zeros(k), np.zeros(k)
+    hist = []
+    for _ in range(nsteps):
+        time += dt
+        if time > tfinal:
+            dt = tfinal - time + dt
+            time = tfinal
+        neq = int(ints[2])
+        r.rk2_(_c_int(k), _c_int(neq), _c_double(time), _c_double(dt), _d(phi), _d(ybar), _d(y), _d(dy))
+        icut = 0
+        ist, ist2 = 0, 0
+        for b in range(k):
+            a_c, l_c, e_c, km_c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+            zb = z[b]
+            nb = ctypes.c_int(int(ndv[b]))
+            r.extractsol_(ctypes.byref(nb), _c_int(neq), ctypes.byref(a_c), ctypes.byref(l_c), _d(zb),
+                          ctypes.c_void_p(dz.ctypes.data + 16 * ist), ctypes.c_void_p(ybar.ctypes.data + 8 * ist2),
+                          _d(za), _d(zai), ctypes.c_void_p(wsave.ctypes.data + 16 * ifwork * b),
+                          ctypes.byref(e_c), ctypes.byref(km_c), _d(rkappa))
+            area[b], arcl[b] = a_c.value, l_c.value
+            idoub = ctypes.c_int(0)
+            r.getfourier_(ctypes.byref(nb), _d(amode), _d(fmode), _d(zb), _d(za),
+                          ctypes.c_void_p(wsave.ctypes.data + 16 * ifwork * b), ctypes.byref(idoub))
+            if idoub.value == 1:
+                icut = 1
+                istop = ctypes.c_int(0)
+                ndp = ctypes.c_void_p(ctypes.addressof(prob) + 16 * kmax + 4 * b)
+                r.double_(ndp, _d(zb), _d(z2), _d(za), ctypes.c_void_p(wsave.ctypes.data + 16 * ifwork * b),
+                          _d(zwork), _c_int(nzwork), _c_int(ndmax), ctypes.byref(istop))
+                ist += int(ndv[b]) // 2
+                ist2 += int(ndv[b])
+            else:
+                ist += int(ndv[b])
+                ist2 += 2 * int(ndv[b])
+        dt = min(cmin * arcl[b] / ndv[b] for b in range(k))
+        if icut:
+            r.initcond_(_c_int(k), _c_int(ndmax), c_nd, ctypes.byref(nbk_c), ctypes.byref(neq_c), _d(z), _d(ybar))
+            ints[:] = [k, nbk_c.value, neq_c.value]
+        hist.append((time, dt, float(area.sum())))
+    r.prini_(_c_int(0), _c_int(0))
+    neq = int(ints[2])
+    return dict(ybar=ybar[:neq].copy(), nd=ndv[:k].copy(), time=time, dt=dt, hist=np.array(hist),
+                area0=area0, arcl0=arcl0, areatot0=areatot0, cmin=cmin)

@@ -2,8 +2,8 @@
 
 Runs only where the reference sources exist (they are compiled in place by oracle/Makefile);
 the fixtures it writes are data -- inputs and the reference's outputs -- and are committed so
-the tests run without the reference.  Usage:  python tests/golden/make_golden.py [atree|gmres|sweep|field]
-("atree" / "gmres" / "sweep" / "field" rewrite only those fixtures).
+the tests run without the reference.  Usage:  python tests/golden/make_golden.py [atree|gmres|sweep|field|sim]
+("atree" / "gmres" / "sweep" / "field" / "sim" rewrite only those fixtures).
 """
 from __future__ import annotations
 
@@ -323,6 +323,34 @@ def main_field():
         os.chdir(cwd)
 
 
+def main_sim():
+    """MANYDROP's run with the reference's routines (oracle.ref_manydrop): the initial shapes
+    after CLUSTPARAM and GEOSHIFT, and the state, step sizes and areas after a few passes of
+    the time loop, for the four circles of bubbin, an ellipse and a starfish."""
+    import tempfile
+    cwd = os.getcwd()
+    os.chdir(tempfile.mkdtemp())  # the reference writes fort.13
+    try:
+        cases = {
+            "four_circles": (dict(k=4, nd=[128] * 4, dt=0.01, tfinal=0.5, nplot=10, cap=0.0, e0=0.64, ai=[1.0] * 4,
+                                  bi=[1.0] * 4, thetax=[0.0] * 4, cx=[-1.5, -1.5, 1.5, 1.5],
+                                  cy=[1.5, -1.5, 1.5, -1.5], ncyc=[0] * 4), 5),
+            "ellipse": (dict(k=1, nd=[256], dt=1e-3, tfinal=1.0, nplot=10, cap=0.0, e0=1.0, ai=[1.1],
+                             bi=[1.0 / 1.1], thetax=[0.3], cx=[0.2], cy=[-0.1], ncyc=[0]), 10),
+            "starfish": (dict(k=1, nd=[256], dt=5e-4, tfinal=1.0, nplot=10, cap=0.0, e0=0.5, ai=[1.0], bi=[0.1],
+                              thetax=[0.0], cx=[0.0], cy=[0.0], ncyc=[3]), 4),
+        }
+        for tag, (p, nsteps) in cases.items():
+            r0 = oracle.ref_manydrop(p, 0)
+            r1 = oracle.ref_manydrop(p, nsteps)
+            print(tag, "steps", nsteps, "dt", r1["dt"], "area", r1["hist"][-1])
+            save("sim_" + tag, nsteps=np.int32(nsteps), ybar0=r0["ybar"], ybar=r1["ybar"], nd_after=r1["nd"],
+                 hist=r1["hist"], arcl0=r1["arcl0"], area0=r1["area0"], cmin=np.float64(r1["cmin"]),
+                 **{k: np.asarray(v) for k, v in p.items()})
+    finally:
+        os.chdir(cwd)
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["atree"]:
         main_atree()
@@ -332,9 +360,12 @@ if __name__ == "__main__":
         main_sweep()
     elif sys.argv[1:] == ["field"]:
         main_field()
+    elif sys.argv[1:] == ["sim"]:
+        main_sim()
     else:
         main()
         main_atree()
         main_gmres()
         main_sweep()
         main_field()
+        main_sim()

@@ -1,0 +1,88 @@
+"""GPU parity of the drop evolution (include/eb_sim.h) with the reference's MANYDROP run:
+tests/golden/sim_*.npz hold the reference's initial shapes after GEOINIT, CLUSTPARAM and
+GEOSHIFT, and its state, step sizes and total areas after a few passes of the time loop
+(oracle.ref_manydrop, written by tests/golden/make_golden.py sim)."""
+import os
+
+import numpy as np
+import pytest
+
+from tests.conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+CASES = ["four_circles", "ellipse", "starfish"]
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as _t
+    if not _t.cuda.is_available():
+        pytest.skip("no GPU")
+    return _t
+
+
+def load(tag):
+    f = np.load(os.path.join(GOLDEN, f"sim_{tag}.npz"))
+    keys = ("k", "nd", "dt", "tfinal", "nplot", "cap", "e0", "ai", "bi", "thetax", "cx", "cy", "ncyc")
+    p = {key: f[key].tolist() for key in keys}
+    return f, p
+
+
+@pytest.mark.parametrize("tag", CASES)
+def test_initial_shapes(torch, tag):
+    """GEOINIT + CLUSTPARAM (equal arclength by Newton) + GEOSHIFT."""
+    from electricbubbles_amd import sim
+    f, p = load(tag)
+    print("sim case", tag, flush=True)
+    S = sim.Simulation(p)
+    st = S.state()
+    assert np.abs(st["ybar"] - f["ybar0"]).max() < 1e-12
+    assert np.abs(st["arcl0"] - f["arcl0"]).max() < 1e-13 * f["arcl0"].max()
+    S.close()
+
+
+@pytest.mark.parametrize("tag", CASES)
+def test_time_loop(torch, tag):
+    """RK2 steps with the step law: state, step sizes and total areas after the steps."""
+    from electricbubbles_amd import sim
+    f, p = load(tag)
+    S = sim.Simulation(p)
+    hist = []
+    for _ in range(int(f["nsteps"])):
+        S.step()
+        st = S.state()
+        hist.append((st["time"], st["dt"], st["area_total"]))
+    hist = np.array(hist)
+    assert np.abs(st["ybar"] - f["ybar"]).max() < 1e-8
+    assert np.array_equal(st["nd"], f["nd_after"])
+    assert np.abs(hist[:, 0] - f["hist"][:, 0]).max() < 1e-12
+    assert np.abs(hist[:, 1] / f["hist"][:, 1] - 1).max() < 1e-10
+    assert np.abs(hist[:, 2] / f["hist"][:, 2] - 1).max() < 1e-12
+    S.close()
+
+
+def test_dump_and_restore_continue_the_run(torch, tmp_path):
+    """DUMPDATA then READDATA: the restored run's next step equals the original's.  As in the
+    reference's READDATA branch (electric.f:96-103), the step law's cmin is formed again from
+    the dumped DT, so the step after that follows dt = cmin' arcl / nd with the new cmin'."""
+    from electricbubbles_amd import sim
+    f, p = load("four_circles")
+    S = sim.Simulation(p)
+    for _ in range(2):
+        S.step()
+    path = tmp_path / "geometry.data"
+    S.dump(path)
+    st = S.state()
+    R = sim.Simulation(dict(p, tfinal=p["tfinal"] - st["time"]), restore=path)
+    rs = R.state()
+    assert np.abs(rs["ybar"] - st["ybar"]).max() < 1e-15 * 10
+    assert rs["time"] == pytest.approx(st["time"], rel=1e-15)
+    S.step()
+    R.step()
+    a, b = S.state(), R.state()
+    assert np.abs(a["ybar"] - b["ybar"]).max() < 1e-13
+    cmin_r = max(st["dt"] * n / l0 for n, l0 in zip(st["nd"], st["arcl0"]))
+    assert b["dt"] == pytest.approx(min(cmin_r * l / n for n, l in zip(b["nd"], b["arcl"])), rel=1e-14)
+    S.close()
+    R.close()
