@@ -20,7 +20,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["eb_sort.hip", "eb_fmm.hip", "eb_spectral.hip", "eb_atree.hip", "eb_gmres.hip", "eb_field.hip", "eb_sim.hip", "eb_dropin.cpp"]
+HIP_SOURCES = ["eb_sort.hip", "eb_fmm.hip", "eb_spectral.hip", "eb_atree.hip", "eb_gmres.hip", "eb_field.hip", "eb_sim.hip",
+               "eb_adjoint.hip", "eb_dropin.cpp"]
 HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
     # packed FP32 instructions stay off in all device code

@@ -2,8 +2,8 @@
 
 Runs only where the reference sources exist (they are compiled in place by oracle/Makefile);
 the fixtures it writes are data -- inputs and the reference's outputs -- and are committed so
-the tests run without the reference.  Usage:  python tests/golden/make_golden.py [atree|gmres|sweep|field|sim]
-("atree" / "gmres" / "sweep" / "field" / "sim" rewrite only those fixtures).
+the tests run without the reference.  Usage:  python tests/golden/make_golden.py [atree|gmres|sweep|field|sim|adjoint]
+("atree" / "gmres" / "sweep" / "field" / "sim" / "adjoint" rewrite only those fixtures).
 """
 from __future__ import annotations
 
@@ -351,8 +351,100 @@ def main_sim():
         os.chdir(cwd)
 
 
+def _np_fdifff(z):
+    """FDIFFF (dcfft.f:1371-1427) per row: i k FFT / N with k = 0 and Nyquist zeroed."""
+    N = z.shape[1]
+    k = np.fft.fftfreq(N, 1.0 / N)
+    k[N // 2] = 0.0
+    return np.fft.ifft(1j * k * np.fft.fft(z, axis=1), axis=1)
+
+
+def np_deformation(z):
+    """D = (L - B) / (L + B) per body from the area and second moments of area (the definition
+    the adjoint's loss uses)."""
+    N = z.shape[1]
+    h = 2 * np.pi / N
+    za = _np_fdifff(z)
+    x, y, xa, ya = z.real, z.imag, za.real, za.imag
+    A = 0.5 * h * (x * ya - y * xa).sum(1)
+    cx = 0.5 * h * (x * x * ya).sum(1) / A
+    cy = -0.5 * h * (y * y * xa).sum(1) / A
+    a = h * (x ** 3 * ya).sum(1) / 3 - A * cx * cx
+    c = -h * (y ** 3 * xa).sum(1) / 3 - A * cy * cy
+    b = 0.5 * h * (x * x * y * ya).sum(1) - A * cx * cy
+    rt = np.sqrt(0.25 * (a - c) ** 2 + b * b)
+    sp, sm = np.sqrt(0.5 * (a + c) + rt), np.sqrt(0.5 * (a + c) - rt)
+    return (sp - sm) / (sp + sm)
+
+
+def np_shapes(centres, modes, N):
+    M = modes.shape[1] // 2
+    th = 2 * np.pi * np.arange(N) / N
+    m = np.arange(1, M + 1)
+    r = 1.0 + modes[:, :M] @ np.cos(np.outer(m, th)) + modes[:, M:] @ np.sin(np.outer(m, th))
+    return centres[:, None] + r * np.exp(1j * th)[None, :]
+
+
+def main_adjoint():
+    """Central differences (Richardson-extrapolated) of the summed deformation after a few RK4
+    steps whose F is the reference's own (RefSim.rhs): four drops of 64 points on a 2 x 2 grid, eight shape modes
+    each; E0 = |E|^2, CAP = Ca.  The adjoint's gradient is checked against these."""
+    import tempfile
+    cwd = os.getcwd()
+    os.chdir(tempfile.mkdtemp())  # the reference writes fort.13
+    try:
+        k_side, N, nm, spacing, amp = 2, 64, 8, 3.0, 0.03
+        rng = np.random.default_rng(2)
+        g = (np.arange(k_side) - 0.5 * (k_side - 1)) * spacing
+        cx, cy = np.meshgrid(g, g, indexing="xy")
+        centres = (cx + 1j * cy).ravel()
+        modes = rng.uniform(0.0, amp, (k_side * k_side, nm))
+        E, Ca, dt, steps = 1.0, 0.1, 1e-3, 4
+        nd = np.full(k_side * k_side, N, np.int32)
+        sim = oracle.RefSim(oracle.LIBREF)
+
+        def loss(E_, Ca_, modes_):
+            sim.setup(nd, cap=Ca_, e0=E_ * E_)
+            z = np_shapes(centres, modes_, N)
+
+            def F(zz):
+                y = np.stack([zz.real.ravel(), zz.imag.ravel()], 1).ravel()
+                d = sim.rhs(y)
+                return (d[0::2] + 1j * d[1::2]).reshape(zz.shape)
+            for _ in range(steps):
+                k1 = F(z)
+                k2 = F(z + 0.5 * dt * k1)
+                k3 = F(z + 0.5 * dt * k2)
+                k4 = F(z + dt * k3)
+                z = z + dt / 6 * (k1 + 2 * k2 + 2 * k3 + k4)
+            return float(np_deformation(z).sum())
+
+        h = 1e-4
+
+        def rich(fn):
+            # central differences at h and h/2, Richardson-extrapolated (error O(h^4))
+            d1 = (fn(h) - fn(-h)) / (2 * h)
+            d2 = (fn(h / 2) - fn(-h / 2)) / h
+            return (4 * d2 - d1) / 3
+
+        L0 = loss(E, Ca, modes)
+        fd_E = rich(lambda e: loss(E + e, Ca, modes))
+        fd_Ca = rich(lambda e: loss(E, Ca + e, modes))
+        dirs = rng.standard_normal((3,) + modes.shape)
+        dirs /= np.linalg.norm(dirs.reshape(3, -1), axis=1)[:, None, None]
+        fd_dirs = np.array([rich(lambda e: loss(E, Ca, modes + e * v)) for v in dirs])
+        print("adjoint fixture: loss", L0, "dE", fd_E, "dCa", fd_Ca, "dirs", fd_dirs)
+        save("adjoint_fd_four_drops", centres=centres, modes=modes, E=np.float64(E), Ca=np.float64(Ca),
+             dt=np.float64(dt), steps=np.int32(steps), N=np.int32(N), h=np.float64(h), loss=np.float64(L0),
+             fd_E=np.float64(fd_E), fd_Ca=np.float64(fd_Ca), dirs=dirs, fd_dirs=fd_dirs)
+    finally:
+        os.chdir(cwd)
+
+
 if __name__ == "__main__":
-    if sys.argv[1:] == ["atree"]:
+    if sys.argv[1:] == ["adjoint"]:
+        main_adjoint()
+    elif sys.argv[1:] == ["atree"]:
         main_atree()
     elif sys.argv[1:] == ["gmres"]:
         main_gmres()
