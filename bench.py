@@ -39,6 +39,23 @@ PROFILED_TRAFFIC_BYTES = {
 }
 
 
+_T0 = time.perf_counter()
+
+
+def log(msg):
+    """Progress on stderr (the one JSON line goes to stdout)."""
+    sys.stderr.write(f"bench [{time.perf_counter() - _T0:7.1f} s] {msg}\n")
+    sys.stderr.flush()
+
+
+def leg(name, fn, *a):
+    log(f"{name}: start")
+    t = time.perf_counter()
+    r = fn(*a)
+    log(f"{name}: done in {time.perf_counter() - t:.1f} s")
+    return r
+
+
 def flops_model(info, n, p):
     """Algorithmic flop convention: complex add 2, complex*real 2, complex*complex 6,
     reciprocal 6, divide 12; per stage counts from the tree."""
@@ -299,6 +316,40 @@ def sweep_leg(torch, dev, with_cpu):
                 "plain_ms_per_F": round(per_fp * 1e3, 3), "plain_F_per_s": round(1.0 / per_fp, 1),
                 "batched_vs_plain": round((nsim / t_fb) * per_fp, 2),
                 "plain_vs_batched_rel_diff": worst_f}
+    # one Heun step of every simulation: eager (host-controlled solves) and as a replayed
+    # hipGraph (device-controlled solves, at most 12 Arnoldi steps each)
+    gst = torch.cuda.Stream()
+    Bg = spectral.Bodies(nd, stream=gst)
+    SFg = field.SweepField(Bg, np.arange(nsim + 1), np.zeros(nsim), e0, stream=gst)
+    torch.cuda.synchronize()
+    dt = 1e-3
+    with torch.cuda.stream(gst):
+        ye = rku_t.clone()
+        yg = rku_t.clone()
+        phi0, y = spectral.rk2_stage1(dt, ye, SFg.rhs(ye), stream=gst)  # warm-up
+        spectral.rk2_stage2(dt, phi0, SFg.rhs(y), ye, stream=gst)
+        SFg.rk2_graph(yg, dt)  # capture
+        gst.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            phi0, y = spectral.rk2_stage1(dt, ye, SFg.rhs(ye), stream=gst)
+            spectral.rk2_stage2(dt, phi0, SFg.rhs(y), ye, stream=gst)
+        gst.synchronize()
+        t_eager = (time.perf_counter() - t0) / reps
+        redone = 0
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            redone += SFg.rk2_graph(yg, dt)
+        gst.synchronize()
+        t_graph = (time.perf_counter() - t0) / reps
+        diff = float((yg - ye).norm() / ye.norm())
+    torch.cuda.synchronize()
+    out["rk2_step"] = {"what": "one Heun step (2 F) of all simulations", "dt": dt, "eager_ms": round(t_eager * 1e3, 3),
+                       "graph_ms": round(t_graph * 1e3, 3), "graph_budget": 12, "graph_steps_redone": int(redone),
+                       "steps_per_s_graph": round(1.0 / t_graph, 1),
+                       "simulation_steps_per_s_graph": round(nsim / t_graph, 1),
+                       "graph_vs_eager_state_rel_diff": diff}
+    SFg.close()
     if with_cpu:
         import tempfile
         import oracle
@@ -396,10 +447,12 @@ def adjoint_leg(torch, dev, with_cpu, steps=200):
     """Gradient of the summed deformation D = (L - B)/(L + B) of 16 drops after `steps` RK4
     steps at dt = 1e-3 with respect to |E| (E0 = |E|^2 = 1), Ca (the imposed strain's CAP =
     0.1) and the 64 real Fourier modes of each initial shape (1024 modes: cos and sin of
-    m = 1..32, amplitudes U[0, 0.02], mode m entering with weight 1/m); drops of radius 1 and
+    m = 1..32, amplitudes U[0, 0.02], mode m entering with weight 1/m^2); drops of radius 1 and
     N = 512 on a 4 x 4 grid with spacing 4 (8192 points).  With equal weights the shapes carry
     curvatures of order 100, where the reference's own F gives |dz/dt| of 200 to 2600 and the
-    explicit step at dt = 1e-3 diverges at once; with weight 1/m, |dz/dt| is below 3.  GMRES
+    explicit step at dt = 1e-3 diverges at once; with weight 1/m the high-frequency content
+    grows from the first steps and the run diverges by step 40; with 1/m^2 it runs the 200
+    steps with |dz/dt| near 1.1 and flat GMRES counts.  GMRES
     tolerance 1e-12 in both directions.  Forward alone, then the adjoint (forward with
     checkpoints every 20 steps, reverse sweep with recomputation), and a spot check of the
     gradient along a random direction of all 1026 parameters against Richardson-extrapolated
@@ -407,7 +460,7 @@ def adjoint_leg(torch, dev, with_cpu, steps=200):
     from electricbubbles_amd import adjoint, field, spectral
     P = adjoint.Problem(16, 512, tol=1e-12)
     centres, modes = adjoint.grid_problem(4, 512, 4.0, 64, 0.02, seed=5, device=dev)
-    E, Ca, dt, decay = 1.0, 0.1, 1e-3, 1.0
+    E, Ca, dt, decay = 1.0, 0.1, 1e-3, 2.0
     # warm-up (FFT plans, workspace)
     adjoint.forward_loss(P, centres, modes, E, Ca, 1, dt, decay=decay)
     torch.cuda.synchronize()
@@ -418,7 +471,9 @@ def adjoint_leg(torch, dev, with_cpu, steps=200):
     t_fwd = time.perf_counter() - t0
     it_fwd = {k: float(np.mean([a for a, _ in v])) for k, v in P.sys.iters.items() if v}
     P.sys.iters = {k: [] for k in P.sys.iters}
+    log(f"adjoint: forward {steps} steps in {t_fwd:.1f} s")
     loss, gE, gCa, gm, tim = adjoint.gradient(P, centres, modes, E, Ca, steps, dt, checkpoint=20, decay=decay)
+    log(f"adjoint: gradient in {tim['total_s']:.1f} s")
     it_grad = {k: float(np.mean([a for a, _ in v])) for k, v in P.sys.iters.items() if v}
     # spot check over 10 steps along a random unit direction (dE, dCa, dmodes)
     ns = min(10, steps)
@@ -446,8 +501,8 @@ def adjoint_leg(torch, dev, with_cpu, steps=200):
     torch.cuda.synchronize()
     t_lib = (time.perf_counter() - t0) / 3
     Fl.close()
-    out = {"workload": "16 drops x 512 points (4 x 4 grid, spacing 4), 1024 shape modes + |E| + Ca, "
-                       f"{steps} RK4 steps at dt = 1e-3, checkpoints every 20",
+    out = {"workload": "16 drops x 512 points (4 x 4 grid, spacing 4), 1024 shape modes (weight 1/m^2) "
+                       f"+ |E| + Ca, {steps} RK4 steps at dt = 1e-3, checkpoints every 20",
            "s_per_grad": round(tim["total_s"], 3), "forward_s": round(t_fwd, 3),
            "grad_over_forward": round(tim["total_s"] / t_fwd, 3),
            "grad_phases_s": {k: round(val, 3) for k, val in tim.items()},
@@ -498,6 +553,7 @@ def main():
     ap.add_argument("--no-sweep", action="store_true", help="skip the batched-versus-plain sweep leg")
     ap.add_argument("--no-field", action="store_true", help="skip the F and RK2 step leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-gradient leg")
+    ap.add_argument("--no-autotune", action="store_true", help="skip the FMM autotuner")
     ap.add_argument("--adjoint-steps", type=int, default=200, help="RK4 steps of the adjoint leg")
     ap.add_argument("--only", choices=["solve", "spectral", "sweep", "field", "adjoint"],
                     help="run one secondary leg alone and print its JSON object (for profiling runs)")
@@ -556,6 +612,7 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
+    log(f"FMM: {n} points, LEVMAX {L}, p {p}")
     plan = eb.FMMPlan(x, y, w, L, p, eb.VARIANT_DR, eb.MODE_FMM, stream=stream)
     for _ in range(args.warmup):
         plan.rebuild(x, y, w)
@@ -594,6 +651,18 @@ def main():
     traffic = PROFILED_TRAFFIC_BYTES.get((args.workload, n, L, p), {}).get(dom)
     total_fl = sum(fl.values())
 
+    # the autotuner's choice of depth and expansion length for the same points at 1e-10
+    tuned = None
+    if not args.no_autotune:
+        from electricbubbles_amd import autotune
+        t0 = time.perf_counter()
+        log("autotune: start")
+        tr = autotune.tune_fmm(x, y, w, d, target=1e-10)
+        log("autotune: done")
+        tuned = {"target_rel": 1e-10, "levmax": tr["levmax"], "nterms": tr["nterms"], "apply_ms": tr["ms"],
+                 "err_vs_p60": tr["err"], "tuning_s": round(time.perf_counter() - t0, 3),
+                 "candidates": [{k: c[k] for k in ("levmax", "points_per_leaf", "ms")} for c in tr["candidates"]]}
+
     # the adaptive tree of the same points (DAPIF2's rules, at most 40 points per childless box)
     from electricbubbles_amd.atree import AdaptiveTree
     at = AdaptiveTree(x, y, 40)
@@ -616,15 +685,17 @@ def main():
         got = out.cpu().numpy()[tgt]
         check = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
 
-    solve = None if args.no_solve else stokes_solve_leg(torch, dev, not args.no_cpu)
-    spec = None if args.no_spectral else spectral_leg(torch, dev, not args.no_cpu)
-    sweep = None if args.no_sweep else sweep_leg(torch, dev, not args.no_cpu)
-    fld = None if args.no_field else field_leg(torch, dev, not args.no_cpu)
-    adj = None if args.no_adjoint else adjoint_leg(torch, dev, not args.no_cpu, args.adjoint_steps)
+    solve = None if args.no_solve else leg("stokes solve", stokes_solve_leg, torch, dev, not args.no_cpu)
+    spec = None if args.no_spectral else leg("spectral", spectral_leg, torch, dev, not args.no_cpu)
+    sweep = None if args.no_sweep else leg("sweep", sweep_leg, torch, dev, not args.no_cpu)
+    fld = None if args.no_field else leg("field", field_leg, torch, dev, not args.no_cpu)
+    adj = None if args.no_adjoint else leg("adjoint", adjoint_leg, torch, dev, not args.no_cpu, args.adjoint_steps)
 
     cpu = None
     if not args.no_cpu:
+        log("cpu baseline: start")
         cpu = cpu_baseline(z, dz, nd, h, dens, L, p, max(ncurves, 1) if ncurves else 4096)
+        log("cpu baseline: done")
 
     value = n / t_full / 1e6
     line = {
@@ -647,6 +718,7 @@ def main():
         "apply_only": {"value": round(n / t_apply / 1e6, 3), "unit": "Mpts/s", "ms": round(t_apply * 1e3, 4)},
         "stage_ms": {k: round(v, 4) for k, v in stage.items()},
         "adaptive_tree": at_info,
+        "autotune": tuned,
         "stokes_solve": solve,
         "spectral": spec,
         "sweep": sweep,

@@ -591,6 +591,44 @@ def gradient(P, centres, modes, E, Ca, steps, dt, checkpoint=20, method="rk4", d
             {"forward_s": t1 - t0, "reverse_s": t2 - t1, "total_s": t2 - t0})
 
 
+def _trajectory_function():
+    torch = _torch()
+
+    class TrajectoryLoss(torch.autograd.Function):
+        """loss = sum of final deformations as a function of (|E|, Ca, modes); backward runs
+        the checkpointed reverse sweep of `gradient`."""
+
+        @staticmethod
+        def forward(ctx, E, Ca, modes, P, centres, steps, dt, checkpoint, method, decay):
+            with torch.enable_grad():  # the reverse sweep builds its own graphs
+                loss, gE, gCa, gm, _ = gradient(P, centres, modes.detach(), float(E), float(Ca), steps, dt,
+                                                checkpoint, method, decay)
+            f64 = dict(dtype=torch.float64, device=modes.device)
+            ctx.save_for_backward(torch.tensor(gE, **f64), torch.tensor(gCa, **f64), gm)
+            return torch.tensor(loss, dtype=torch.float64, device=modes.device)
+
+        @staticmethod
+        def backward(ctx, g):
+            gE, gCa, gm = ctx.saved_tensors
+            return g * gE, g * gCa, g * gm, None, None, None, None, None, None, None
+
+    return TrajectoryLoss
+
+
+_TL = None
+
+
+def trajectory_loss(E, Ca, modes, P, centres, steps, dt, checkpoint=20, method="rk4", decay=0.0):
+    """Autograd entry point: the summed final deformation as a differentiable function of the
+    0-d tensors E (|E|) and Ca and the (k, 2M) tensor of shape modes.  The forward pass already
+    runs the reverse sweep (one forward and one reverse in total); loss.backward() then hands
+    the stored gradients to E, Ca and modes."""
+    global _TL
+    if _TL is None:
+        _TL = _trajectory_function()
+    return _TL.apply(E, Ca, modes, P, centres, steps, dt, checkpoint, method, decay)
+
+
 def grid_problem(k_side=4, N=512, spacing=4.0, nmodes=64, amp=0.02, seed=0, device="cuda"):
     """The 16-drop adjoint configuration: drops on a k_side x k_side grid with the given
     spacing, radius 1, nmodes real Fourier modes per drop of amplitude U[0, amp]."""

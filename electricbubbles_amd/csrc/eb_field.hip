@@ -648,11 +648,20 @@ struct eb_sweep_field_s {
   eb_sweep sw = nullptr;
   double *rhs_st = nullptr, *pk = nullptr, *un = nullptr;
   double2 *u = nullptr, *za = nullptr, *udot = nullptr, *gw = nullptr;
+  // the captured RK2 step (eb_sweep_field_rk2_graph)
+  hipGraphExec_t gexec = nullptr;
+  double* g_ybar = nullptr;
+  double g_dt = 0.0;
+  int g_budget = 0;
+  double *ysave = nullptr, *yst = nullptr, *phi0 = nullptr, *dy = nullptr;
 };
 
 static void free_sf(eb_sweep_field F) {
   if (!F) return;
   if (F->st) (void)hipStreamSynchronize(F->st);
+  if (F->gexec) (void)hipGraphExecDestroy(F->gexec);
+  for (double* q : {F->ysave, F->yst, F->phi0, F->dy})
+    if (q) (void)hipFree(q);
   if (F->glap) eb_gmres_batch_destroy(F->glap);
   if (F->gst) eb_gmres_batch_destroy(F->gst);
   if (F->sw) eb_sweep_destroy(F->sw);
@@ -768,8 +777,10 @@ static int sf_lap_apply(void* ctx, const double* u, double* w, void* stream) {
   return EB_OK;
 }
 
-extern "C" int eb_sweep_field_rhs(eb_sweep_field F, const double* rku, double* rkup) {
-  if (!F || !rku || !rkup) return (eb_set_error("eb_sweep_field_rhs: bad arguments"), EB_ERR_ARG);
+// F of every simulation.  budget = 0: the solves run the host-controlled batched DGMRES; budget
+// > 0: the device-controlled one with at most `budget` Arnoldi steps and no host
+// synchronisation anywhere (the graph-capturable form; geometry must exist already).
+static int sweep_rhs(eb_sweep_field F, const double* rku, double* rkup, int budget) {
   eb_bodies B = F->B;
   const int n = F->n;
   void* s = (void*)F->st;
@@ -791,9 +802,12 @@ extern "C" int eb_sweep_field_rhs(eb_sweep_field F, const double* rku, double* r
   hipLaunchKernelGGL(k_getbcs, dim3(ew_blocks(n)), dim3(EW), 0, F->st, n, F->x, F->rhs_lap, F->ulap);
   EB_HIP_CHECK(hipGetLastError());
   std::vector<double> err(F->nsim);
-  if ((rc = eb_gmres_batch_solve(F->glap, sf_lap_apply, F, F->rhs_lap, F->ulap, 1.0e-10, F->it_lap.data(), err.data(),
-                                 F->ie_lap.data())) != EB_OK)
-    return rc;
+  if (budget > 0)
+    rc = eb_gmres_batch_solve_async(F->glap, sf_lap_apply, F, F->rhs_lap, F->ulap, 1.0e-10, budget);
+  else
+    rc = eb_gmres_batch_solve(F->glap, sf_lap_apply, F, F->rhs_lap, F->ulap, 1.0e-10, F->it_lap.data(), err.data(),
+                              F->ie_lap.data());
+  if (rc != EB_OK) return rc;
   // PVINTEV + DIRICH_NEU_MAP
   hipLaunchKernelGGL(k_pv_interp, dim3(B->nbody), dim3(SP_THREADS), lds_bytes(B, 2), F->st, view(B), F->x, F->y,
                      F->ulap, F->dz, F->x2, F->y2, F->uf);
@@ -806,9 +820,12 @@ extern "C" int eb_sweep_field_rhs(eb_sweep_field F, const double* rku, double* r
                      F->dsda, F->dphi, F->cap_b, F->e0_b, F->alpha_b, (double2*)F->rhs_st);
   EB_HIP_CHECK(hipGetLastError());
   // SOLVEI of every simulation
-  if ((rc = eb_sweep_solve(F->sw, F->gst, F->rhs_st, (double*)F->u, nullptr, nullptr, F->it_st.data(), err.data(),
-                           F->ie_st.data(), s)) != EB_OK)
-    return rc;
+  if (budget > 0)
+    rc = eb_sweep_solve_async(F->sw, F->gst, F->rhs_st, (double*)F->u, budget, s);
+  else
+    rc = eb_sweep_solve(F->sw, F->gst, F->rhs_st, (double*)F->u, nullptr, nullptr, F->it_st.data(), err.data(),
+                        F->ie_st.data(), s);
+  if (rc != EB_OK) return rc;
   // INTERFACENEW: FDIFFF of u, CADAPMOD of every simulation, corrections
   if ((rc = eb_fdifff(B, (const double*)F->u, (double*)F->udot, s)) != EB_OK) return rc;
   if ((rc = eb_sweep_layer(F->sw, (const double*)F->u, (double*)F->gw, EB_VARIANT_MOD, s)) != EB_OK) return rc;
@@ -816,6 +833,83 @@ extern "C" int eb_sweep_field_rhs(eb_sweep_field F, const double* rku, double* r
                      F->dsda, F->rkappa, F->ckern, F->u, F->udot, F->gw, F->cap_b, F->un);
   EB_HIP_CHECK(hipGetLastError());
   return eb_ftail(B, F->tha, F->un, (const double*)F->dz, F->dsda, rkup, s);
+}
+
+extern "C" int eb_sweep_field_rhs(eb_sweep_field F, const double* rku, double* rkup) {
+  if (!F || !rku || !rkup) return (eb_set_error("eb_sweep_field_rhs: bad arguments"), EB_ERR_ARG);
+  return sweep_rhs(F, rku, rkup, 0);
+}
+
+// One Heun step (electric.f:773-783) on the field's buffers: phi0 = dt F(ybar), y = ybar + phi0,
+// ybar += phi0/2 + dt F(y)/2, with the RK2 kernels' roundings.
+static int sweep_rk2_step(eb_sweep_field F, double* ybar, double dt, int budget) {
+  const int neq = 2 * F->n;
+  void* s = (void*)F->st;
+  int rc;
+  if ((rc = sweep_rhs(F, ybar, F->dy, budget)) != EB_OK) return rc;
+  if ((rc = eb_rk2_stage1(neq, dt, ybar, F->dy, F->phi0, F->yst, s)) != EB_OK) return rc;
+  if ((rc = sweep_rhs(F, F->yst, F->dy, budget)) != EB_OK) return rc;
+  return eb_rk2_stage2(neq, dt, F->phi0, F->dy, ybar, s);
+}
+
+extern "C" int eb_sweep_field_rk2_graph(eb_sweep_field F, double* ybar, double dt, int32_t budget, int32_t* redone) {
+  if (!F || !ybar || budget < 1) return (eb_set_error("eb_sweep_field_rk2_graph: bad arguments"), EB_ERR_ARG);
+  if (F->st == nullptr)
+    return (eb_set_error("eb_sweep_field_rk2_graph: the field must use a non-default stream (graph capture)"),
+            EB_ERR_ARG);
+  const size_t neq = 2 * (size_t)F->n;
+  int rc;
+  if (!F->ysave) {
+    if ((rc = dalloc(&F->ysave, neq)) || (rc = dalloc(&F->yst, neq)) || (rc = dalloc(&F->phi0, neq)) ||
+        (rc = dalloc(&F->dy, neq)))
+      return rc;
+  }
+  if (!F->have_geom && (rc = sweep_rhs(F, ybar, F->dy, 0)) != EB_OK) return rc;  // creates the sweep operator
+  if (!F->gexec || F->g_ybar != ybar || F->g_dt != dt || F->g_budget != budget) {
+    if (F->gexec) {
+      EB_HIP_CHECK(hipGraphExecDestroy(F->gexec));
+      F->gexec = nullptr;
+    }
+    EB_HIP_CHECK(hipStreamSynchronize(F->st));
+    hipGraph_t graph = nullptr;
+    EB_HIP_CHECK(hipStreamBeginCapture(F->st, hipStreamCaptureModeThreadLocal));
+    int crc = hipMemcpyAsync(F->ysave, ybar, sizeof(double) * neq, hipMemcpyDeviceToDevice, F->st) == hipSuccess
+                  ? EB_OK
+                  : EB_ERR_HIP;
+    if (crc == EB_OK) crc = eb_gmres_batch_clear_fail(F->glap);
+    if (crc == EB_OK) crc = eb_gmres_batch_clear_fail(F->gst);
+    if (crc == EB_OK) crc = sweep_rk2_step(F, ybar, dt, budget);
+    const hipError_t e = hipStreamEndCapture(F->st, &graph);
+    if (crc != EB_OK || e != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      if (crc == EB_OK) eb_set_error("eb_sweep_field_rk2_graph: capture failed: %s", hipGetErrorString(e));
+      return crc != EB_OK ? crc : EB_ERR_HIP;
+    }
+    const hipError_t ie = hipGraphInstantiate(&F->gexec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ie != hipSuccess) {
+      F->gexec = nullptr;
+      return (eb_set_error("eb_sweep_field_rk2_graph: instantiate failed: %s", hipGetErrorString(ie)), EB_ERR_HIP);
+    }
+    F->g_ybar = ybar;
+    F->g_dt = dt;
+    F->g_budget = budget;
+  }
+  EB_HIP_CHECK(hipGraphLaunch(F->gexec, F->st));
+  int32_t fl = 0, fs = 0;
+  if ((rc = eb_gmres_batch_results(F->glap, F->it_lap.data(), nullptr, F->ie_lap.data(), &fl)) != EB_OK ||
+      (rc = eb_gmres_batch_results(F->gst, F->it_st.data(), nullptr, F->ie_st.data(), &fs)) != EB_OK)
+    return rc;
+  if (redone) *redone = 0;
+  if (fl + fs > 0) {
+    // some system needed more than `budget` Arnoldi steps: the step again from the saved state
+    // with the host-controlled solves
+    EB_HIP_CHECK(hipMemcpyAsync(ybar, F->ysave, sizeof(double) * neq, hipMemcpyDeviceToDevice, F->st));
+    if ((rc = sweep_rk2_step(F, ybar, dt, 0)) != EB_OK) return rc;
+    EB_HIP_CHECK(hipStreamSynchronize(F->st));
+    if (redone) *redone = 1;
+  }
+  return EB_OK;
 }
 
 extern "C" int eb_sweep_field_iterations(eb_sweep_field F, int32_t* lap_iter, int32_t* lap_ierr, int32_t* stokes_iter,

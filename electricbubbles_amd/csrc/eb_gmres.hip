@@ -177,7 +177,7 @@ int ew_grid(int n) { return std::max(1, std::min(4096, (n + 255) / 256)); }
 // DHEQR column update for column ll (1-based), dheqr.f:100-176 (IJOB > 1, or the full
 // factorisation of a 2x1 matrix for ll = 1 which is the same arithmetic on one column).
 // hes is column-major with leading dimension ldh; q holds (c, s) pairs.  Returns INFO.
-int dheqr_column(double* hes, int ldh, int ll, double* q) {
+__host__ __device__ int dheqr_column(double* hes, int ldh, int ll, double* q) {
   auto A = [&](int i, int j) -> double& { return hes[(size_t)(j - 1) * ldh + (i - 1)]; };
   for (int k = 1; k <= ll - 1; ++k) {
     const int i = 2 * (k - 1);
@@ -206,7 +206,7 @@ int dheqr_column(double* hes, int ldh, int ll, double* q) {
 }
 
 // DHELS (dhels.f): least-squares solution of the factored Hessenberg system, in place on b.
-void dhels(const double* hes, int ldh, int n, const double* q, double* b) {
+__host__ __device__ void dhels(const double* hes, int ldh, int n, const double* q, double* b) {
   auto A = [&](int i, int j) { return hes[(size_t)(j - 1) * ldh + (i - 1)]; };
   for (int k = 1; k <= n; ++k) {
     const double c = q[2 * (k - 1)], s = q[2 * (k - 1) + 1];
@@ -607,6 +607,121 @@ __global__ __launch_bounds__(GB_T) void k_gmb_reorth(int ll, const int* __restri
   if (threadIdx.x == 0) h[maxl + 2] = sumdsq;
 }
 
+// ---- device-side control (no host synchronisation inside a solve) ---------------------------
+// The scalar work of one Arnoldi step of every system -- DORTH's re-orthogonalisation test,
+// DHEQR's rotation, ISDGMR's test and at the end DHELS -- as the host loop below does it, in
+// the same order and arithmetic, one thread per system.  With it a solve is a fixed sequence
+// of launches that a hipGraph can hold.
+
+struct GmbState {
+  double bnrm, r0nrm, prod, rho, err, snormw;
+  int it, ie;
+};
+
+__global__ void k_gmb_dstart(int S, double tol, const double* __restrict__ hs, int hld, int* __restrict__ act,
+                             double* __restrict__ fac, int* __restrict__ lcnt, GmbState* __restrict__ st) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S) return;
+  const double* h = hs + (size_t)s * hld;
+  GmbState g;
+  g.bnrm = sqrt(h[0]);
+  g.r0nrm = sqrt(h[1]);
+  g.err = g.r0nrm / g.bnrm;  // ISDGMR, ITOL = 0, at the start of the cycle
+  g.prod = 1.0;
+  g.rho = 0.0;
+  g.snormw = 0.0;
+  g.it = 0;
+  g.ie = 0;
+  const int a = g.err <= tol ? 0 : 1;
+  act[s] = a;
+  fac[s] = a ? 1.0 / g.r0nrm : 0.0;
+  lcnt[s] = 0;
+  st[s] = g;
+}
+
+// after k_gmb_orth: |w| after MGS, and DORTH's test for a second pass (dorth.f:104-107)
+__global__ void k_gmb_dctl1(int S, const double* __restrict__ hs, int hld, int mx, const int* __restrict__ act,
+                            int* __restrict__ flag, GmbState* __restrict__ st) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S) return;
+  flag[s] = 0;
+  if (!act[s]) return;
+  const double* h = hs + (size_t)s * hld;
+  const double vnrm = sqrt(h[mx]);
+  const double snormw = sqrt(h[mx + 1]);
+  st[s].snormw = snormw;
+  if (!(vnrm + 0.001 * snormw != vnrm)) flag[s] = 1;
+}
+
+// after k_gmb_reorth: the Hessenberg column, DHEQR, the tests of dpigmr.f:366-392 and, for a
+// system that stops with an update, DHELS into cf (stride cld).  A system still iterating
+// after `budget` steps stops with the update of those steps and IE = 3 (counted in *fail).
+__global__ void k_gmb_dctl2(int ll, int S, double tol, int budget, const double* __restrict__ hs, int hld, int mx,
+                            const int* __restrict__ maxl_s, int* __restrict__ act, const int* __restrict__ flag,
+                            double* __restrict__ fac, int* __restrict__ lcnt, double* __restrict__ cf, int cld,
+                            double* __restrict__ hesa, double* __restrict__ qa, GmbState* __restrict__ st,
+                            int* __restrict__ fail) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S) return;
+  fac[s] = 0.0;
+  if (!act[s]) return;
+  const double* h = hs + (size_t)s * hld;
+  const int ldh = mx + 1;
+  GmbState g = st[s];
+  if (flag[s]) {
+    const double sumdsq = h[mx + 2];
+    if (sumdsq != 0.0) g.snormw = sqrt(fmax(0.0, g.snormw * g.snormw - sumdsq));
+  }
+  double* hes = hesa + (size_t)s * ldh * mx;
+  double* q = qa + (size_t)s * 2 * mx;
+  for (int i = 1; i <= ll; ++i) hes[(size_t)(ll - 1) * ldh + (i - 1)] = h[i - 1];
+  hes[(size_t)(ll - 1) * ldh + ll] = g.snormw;
+  const int info = dheqr_column(hes, ldh, ll, q);
+  g.it = ll;
+  int done = 0;
+  if (info == ll) {
+    done = 2;
+    g.ie = 2;
+  } else {
+    g.prod = g.prod * q[2 * ll - 1];
+    g.rho = fabs(g.prod * g.r0nrm);
+    g.err = g.rho / g.bnrm;
+    if (g.err <= tol) {
+      done = 1;
+      g.ie = 0;
+    } else if (ll == maxl_s[s]) {
+      if (!(g.rho < g.r0nrm)) {
+        done = 2;
+        g.ie = 2;
+      } else {
+        done = 1;
+        g.ie = 1;
+      }
+    } else if (ll == budget) {
+      done = 1;
+      g.ie = 3;
+      atomicAdd(fail, 1);
+    }
+  }
+  if (done) {
+    act[s] = 0;
+    if (done == 1) {
+      double* bb = cf + (size_t)s * cld;
+      bb[0] = g.r0nrm;
+      for (int j = 1; j <= ll; ++j) bb[j] = 0.0;
+      dhels(hes, ldh, ll, q, bb);
+      lcnt[s] = ll;
+    }
+  } else {
+    fac[s] = 1.0 / g.snormw;
+  }
+  st[s] = g;
+}
+
+__global__ void k_gmb_zero(int* __restrict__ v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *v = 0;
+}
+
 // x_s += sum_{j < l_s} c_s[j] V_s(:, j) in the order of dpigmr.f:411-413
 __global__ void k_gmb_combine(const int* __restrict__ off, const int* __restrict__ lcnt, const double* __restrict__ coef,
                               int cld, const double* __restrict__ V, size_t ldv, double* __restrict__ x) {
@@ -638,6 +753,14 @@ struct eb_gmres_batch_s {
   // host state
   std::vector<double> hes, q, h, fac, cf;
   std::vector<int> act, flag, lcnt;
+  // device-side control (eb_gmres_batch_solve_async)
+  GmbState* d_state = nullptr;
+  int* d_flag = nullptr;
+  int* d_maxl = nullptr;
+  int* d_fail = nullptr;
+  double* d_hes = nullptr;
+  double* d_q = nullptr;
+  double* d_cf = nullptr;  // nsys x (maxl + 1): DHELS right-hand sides and coefficients
 };
 
 extern "C" int eb_gmres_batch_create(eb_gmres_batch* out, int32_t nsys, const int32_t* off, int32_t maxl,
@@ -669,7 +792,17 @@ extern "C" int eb_gmres_batch_create(eb_gmres_batch* out, int32_t nsys, const in
                   hipMalloc((void**)&g->r, sizeof(double) * (size_t)g->ntot) == hipSuccess &&
                   hipMalloc((void**)&g->hs, sizeof(double) * ns * g->hld) == hipSuccess &&
                   hipMalloc((void**)&g->coef, sizeof(double) * ns * mx) == hipSuccess &&
+                  hipMalloc((void**)&g->d_state, sizeof(GmbState) * ns) == hipSuccess &&
+                  hipMalloc((void**)&g->d_flag, sizeof(int) * ns) == hipSuccess &&
+                  hipMalloc((void**)&g->d_maxl, sizeof(int) * ns) == hipSuccess &&
+                  hipMalloc((void**)&g->d_fail, sizeof(int)) == hipSuccess &&
+                  hipMalloc((void**)&g->d_hes, sizeof(double) * ns * (mx + 1) * mx) == hipSuccess &&
+                  hipMalloc((void**)&g->d_q, sizeof(double) * ns * 2 * mx) == hipSuccess &&
+                  hipMalloc((void**)&g->d_cf, sizeof(double) * ns * (mx + 1)) == hipSuccess &&
                   hipMemcpyAsync(g->d_off, off, sizeof(int) * (ns + 1), hipMemcpyHostToDevice, g->st) == hipSuccess &&
+                  hipMemcpyAsync(g->d_maxl, g->maxl_s.data(), sizeof(int) * ns, hipMemcpyHostToDevice, g->st) ==
+                      hipSuccess &&
+                  hipMemsetAsync(g->d_fail, 0, sizeof(int), g->st) == hipSuccess &&
                   hipStreamSynchronize(g->st) == hipSuccess;
   if (!ok) {
     eb_gmres_batch_destroy(g);
@@ -690,7 +823,8 @@ extern "C" int eb_gmres_batch_create(eb_gmres_batch* out, int32_t nsys, const in
 extern "C" int eb_gmres_batch_destroy(eb_gmres_batch g) {
   if (!g) return EB_OK;
   if (g->st) (void)hipStreamSynchronize(g->st);
-  void* p[] = {g->d_off, g->d_act, g->d_lcnt, g->d_fac, g->V, g->r, g->hs, g->coef};
+  void* p[] = {g->d_off, g->d_act, g->d_lcnt, g->d_fac, g->V,      g->r,    g->hs,   g->coef,
+               g->d_state, g->d_flag, g->d_maxl, g->d_fail, g->d_hes, g->d_q, g->d_cf};
   for (void* q : p)
     if (q) (void)hipFree(q);
   delete g;
@@ -825,6 +959,69 @@ extern "C" int eb_gmres_batch_solve(eb_gmres_batch g, eb_apply_fn apply, void* c
     if (ierr) ierr[s] = ie[s];
   }
   return EB_OK;
+}
+
+extern "C" int eb_gmres_batch_solve_async(eb_gmres_batch g, eb_apply_fn apply, void* ctx, const double* b, double* x,
+                                          double tol, int32_t budget) {
+  if (!g || !apply || !b || !x || budget < 1)
+    return (eb_set_error("eb_gmres_batch_solve_async: bad arguments"), EB_ERR_ARG);
+  if (tol == 0.0) tol = 500.0 * 1.1102230246251565e-16;  // 500 * D1MACH(3)
+  const int S = g->nsys, hld = g->hld, mx = g->maxl;
+  const int nl = std::min<int>(budget, mx);
+  const size_t ldv = (size_t)g->ntot;
+  hipStream_t st = g->st;
+  const dim3 cb((S + 127) / 128), ct(128);
+  int rc = apply(ctx, x, g->r, st);
+  if (rc != EB_OK) return rc;
+  hipLaunchKernelGGL(k_gmb_residual, dim3(S), dim3(GB_T), 0, st, g->d_off, b, g->r, g->hs, hld);
+  hipLaunchKernelGGL(k_gmb_dstart, cb, ct, 0, st, S, tol, g->hs, hld, g->d_act, g->d_fac, g->d_lcnt, g->d_state);
+  hipLaunchKernelGGL(k_gmb_scale, dim3(S), dim3(256), 0, st, g->d_off, g->d_act, g->d_fac, g->r, g->V);
+  for (int ll = 1; ll <= nl; ++ll) {
+    double* w = g->V + (size_t)ll * ldv;
+    if ((rc = apply(ctx, g->V + (size_t)(ll - 1) * ldv, w, st)) != EB_OK) return rc;
+    hipLaunchKernelGGL(k_gmb_orth, dim3(S), dim3(GB_T), 0, st, ll, g->d_off, g->d_act, g->V, ldv, g->hs, hld, mx);
+    hipLaunchKernelGGL(k_gmb_dctl1, cb, ct, 0, st, S, g->hs, hld, mx, g->d_act, g->d_flag, g->d_state);
+    hipLaunchKernelGGL(k_gmb_reorth, dim3(S), dim3(GB_T), 0, st, ll, g->d_off, g->d_flag, g->V, ldv, g->hs, hld, mx);
+    hipLaunchKernelGGL(k_gmb_dctl2, cb, ct, 0, st, ll, S, tol, nl, g->hs, hld, mx, g->d_maxl, g->d_act, g->d_flag,
+                       g->d_fac, g->d_lcnt, g->d_cf, mx + 1, g->d_hes, g->d_q, g->d_state, g->d_fail);
+    hipLaunchKernelGGL(k_gmb_scale, dim3(S), dim3(256), 0, st, g->d_off, g->d_act, g->d_fac, w, w);
+  }
+  hipLaunchKernelGGL(k_gmb_combine, dim3(S), dim3(256), 0, st, g->d_off, g->d_lcnt, g->d_cf, mx + 1, g->V, ldv, x);
+  EB_HIP_CHECK(hipGetLastError());
+  return EB_OK;
+}
+
+extern "C" int eb_gmres_batch_clear_fail(eb_gmres_batch g) {
+  if (!g) return (eb_set_error("eb_gmres_batch_clear_fail: bad arguments"), EB_ERR_ARG);
+  hipLaunchKernelGGL(k_gmb_zero, dim3(1), dim3(64), 0, g->st, g->d_fail);
+  EB_HIP_CHECK(hipGetLastError());
+  return EB_OK;
+}
+
+extern "C" int eb_gmres_batch_results(eb_gmres_batch g, int32_t* iter, double* err, int32_t* ierr, int32_t* nfail) {
+  if (!g) return (eb_set_error("eb_gmres_batch_results: bad arguments"), EB_ERR_ARG);
+  std::vector<GmbState> h(g->nsys);
+  int f = 0;
+  EB_HIP_CHECK(hipMemcpyAsync(h.data(), g->d_state, sizeof(GmbState) * g->nsys, hipMemcpyDeviceToHost, g->st));
+  EB_HIP_CHECK(hipMemcpyAsync(&f, g->d_fail, sizeof(int), hipMemcpyDeviceToHost, g->st));
+  EB_HIP_CHECK(hipStreamSynchronize(g->st));
+  for (int s = 0; s < g->nsys; ++s) {
+    if (iter) iter[s] = h[s].it;
+    if (err) err[s] = h[s].err;
+    if (ierr) ierr[s] = h[s].ie;
+  }
+  if (nfail) *nfail = f;
+  return EB_OK;
+}
+
+extern "C" int eb_sweep_solve_async(eb_sweep sw, eb_gmres_batch g, const double* rhs, double* soln, int32_t budget,
+                                    void* stream) {
+  if (!sw || !g || !rhs || !soln) return (eb_set_error("eb_sweep_solve_async: bad arguments"), EB_ERR_ARG);
+  hipStream_t st = (hipStream_t)stream;
+  if (st != g->st)
+    return (eb_set_error("eb_sweep_solve_async: the GMRES workspace was created on another stream"), EB_ERR_ARG);
+  hipLaunchKernelGGL(k_gm_copy, dim3(ew_grid(g->ntot)), dim3(256), 0, st, g->ntot, rhs, soln);
+  return eb_gmres_batch_solve_async(g, eb_sweep_gmres_apply, sw, rhs, soln, 1.0e-10, budget);
 }
 
 extern "C" int eb_sweep_gmres_apply(void* sw, const double* x, double* y, void* stream) {

@@ -15,7 +15,8 @@ from . import _check, _ptr, _stream_ptr, _torch, lib
 FIELD_EXPORTS = ("eb_field_create", "eb_field_destroy", "eb_field_rhs", "eb_field_set_geometry",
                  "eb_field_laplace_apply", "eb_field_laplace_solve", "eb_field_dirich_neu", "eb_field_setvelf",
                  "eb_field_stokes_solve", "eb_field_interface", "eb_field_get_geometry", "eb_field_iterations", "eb_sweep_field_create",
-                 "eb_sweep_field_rhs", "eb_sweep_field_iterations", "eb_sweep_field_destroy")
+                 "eb_sweep_field_rhs", "eb_sweep_field_iterations", "eb_sweep_field_destroy",
+                 "eb_sweep_field_rk2_graph")
 
 _vp = ctypes.c_void_p
 _i32 = ctypes.c_int32
@@ -41,6 +42,7 @@ def _bind():
         L.eb_sweep_field_rhs.argtypes = [_vp, _vp, _vp]
         L.eb_sweep_field_iterations.argtypes = [_vp, _vp, _vp, _vp, _vp]
         L.eb_sweep_field_destroy.argtypes = [_vp]
+        L.eb_sweep_field_rk2_graph.argtypes = [_vp, _vp, ctypes.c_double, _i32, ctypes.POINTER(_i32)]
         _bound = True
     return L
 
@@ -152,6 +154,16 @@ class SweepField:
         a = [np.zeros(self.nsim, np.int32) for _ in range(4)]
         _check(_bind().eb_sweep_field_iterations(self._h, *(v.ctypes.data for v in a)))
         return tuple(a)
+
+    def rk2_graph(self, ybar, dt, budget=12):
+        """One Heun step of every simulation in place on ybar (a contiguous device tensor),
+        captured in a hipGraph on the first call and replayed after (eb_sweep_field_rk2_graph).
+        The field must have been created with a non-default stream; returns True when the step
+        was redone with the host-controlled solves (a system needed more than `budget` steps)."""
+        assert ybar.is_contiguous() and ybar.dtype == self._torch.float64
+        red = _i32()
+        _check(_bind().eb_sweep_field_rk2_graph(self._h, _ptr(ybar), float(dt), int(budget), ctypes.byref(red)))
+        return bool(red.value)
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:

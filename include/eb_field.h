@@ -85,6 +85,16 @@ int eb_sweep_field_iterations(eb_sweep_field f, int32_t *lap_iter, int32_t *lap_
                               int32_t *stokes_ierr);
 int eb_sweep_field_destroy(eb_sweep_field f);
 
+/* One Heun RK2 step (electric.f:773-783) of every simulation, ybar += (dt F(ybar) + dt F(y)) / 2,
+ * captured once into a hipGraph and replayed on later calls: the two F of the step with their
+ * four batched solves run the device-controlled DGMRES (eb_gmres_batch_solve_async) with at
+ * most `budget` Arnoldi steps each, so the step has no host synchronisation.  After the replay
+ * the solves' failure counters are read; if any system needed more than `budget` steps, ybar
+ * is restored and the step is run again with the host-controlled solves (*redone = 1).
+ * ybar: 2 nbk doubles on the device, updated in place.  The graph is rebuilt when ybar, dt or
+ * budget change.  The field must have been created on a non-default stream. */
+int eb_sweep_field_rk2_graph(eb_sweep_field f, double *ybar, double dt, int32_t budget, int32_t *redone);
+
 #ifdef __cplusplus
 }
 #endif

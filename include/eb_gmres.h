@@ -78,8 +78,23 @@ int eb_gmres_batch_solve(eb_gmres_batch g, eb_apply_fn apply, void *ctx, const d
                          int32_t *iter, double *err, int32_t *ierr);
 int eb_gmres_batch_destroy(eb_gmres_batch g);
 
+/* The same solve with the per-system scalar work (DORTH's test, DHEQR, ISDGMR's test, DHELS)
+ * in device kernels, one thread per system, in the same order and arithmetic: nothing is
+ * synchronised or copied to the host, so the launches can be captured in a hipGraph.  It
+ * always runs min(budget, MAXL) Arnoldi steps (systems that are done drop out of the
+ * kernels); a system still iterating after `budget` steps stops there with the update of
+ * those steps, IERR = 3, and is counted in the workspace's failure counter.
+ * eb_gmres_batch_results fetches ITER, ERR, IERR of the last such solve and the counter
+ * (synchronises); eb_gmres_batch_clear_fail zeroes the counter with a kernel. */
+int eb_gmres_batch_solve_async(eb_gmres_batch g, eb_apply_fn apply, void *ctx, const double *b, double *x,
+                               double tol, int32_t budget);
+int eb_gmres_batch_clear_fail(eb_gmres_batch g);
+int eb_gmres_batch_results(eb_gmres_batch g, int32_t *iter, double *err, int32_t *ierr, int32_t *nfail);
+
 /* eb_apply_fn for the sweep operator (ctx = an eb_sweep) */
 int eb_sweep_gmres_apply(void *sw, const double *x, double *y, void *stream);
+/* eb_sweep_solve's solve through eb_gmres_batch_solve_async (no za / pk). */
+int eb_sweep_solve_async(eb_sweep sw, eb_gmres_batch g, const double *rhs, double *soln, int32_t budget, void *stream);
 
 /* SOLVEI of every simulation of the sweep at once: soln = rhs as the initial guesses,
  * TOL = 1e-10, the workspace's MAXL, then za and pk of each solution.  The workspace must

@@ -1,0 +1,39 @@
+"""GPU checks of the FMM autotuner (electricbubbles_amd/autotune.py): the chosen expansion
+length and depth meet the accuracy target against the library's O(N^2) sum, the choice is the
+fastest measured candidate that meets it, and a second call comes from the cache."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as _t
+    if not _t.cuda.is_available():
+        pytest.skip("no GPU")
+    return _t
+
+
+def test_tuned_plan_meets_target(torch):
+    import electricbubbles_amd as eb
+    from electricbubbles_amd import autotune
+    from electricbubbles_amd.workloads import complex_normal, curves_cfg, weights
+    rng = np.random.default_rng(4)
+    z, dz, nd, h = curves_cfg(256, 256, rng)
+    n = z.size
+    print("autotune case: curves", n, "points", flush=True)
+    x = torch.tensor(z.real.copy(), device="cuda")
+    y = torch.tensor(z.imag.copy(), device="cuda")
+    w = torch.tensor(weights(dz, nd, h), device="cuda")
+    d = torch.tensor(complex_normal(n, rng), device="cuda")
+    res = autotune.tune_fmm(x, y, w, d, target=1e-10)
+    assert len(res["candidates"]) >= 2
+    ok = [c for c in res["candidates"] if c["err"] <= 1e-10]
+    assert ok and res["levmax"] == min(ok, key=lambda c: c["ms"])["levmax"]
+    direct = eb.FMMPlan(x, y, w, 5, 43, mode=eb.MODE_DIRECT).apply(d)
+    tuned = eb.FMMPlan(x, y, w, res["levmax"], res["nterms"], mode=eb.MODE_FMM).apply(d)
+    err = float((tuned - direct).norm() / direct.norm())
+    print("tuned", res["levmax"], res["nterms"], "err vs direct", err, flush=True)
+    assert err < 3e-10
+    assert autotune.tune_fmm(x, y, w, d, target=1e-10) is res

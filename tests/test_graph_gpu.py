@@ -1,0 +1,84 @@
+"""GPU checks of the captured RK2 step of a sweep (eb_sweep_field_rk2_graph): the device-
+controlled batched DGMRES inside it gives the host-controlled solves' results and iteration
+counts, every replay of the graph matches the eager step (with eager work on the same stream
+between replays), and a budget too small for some system falls back to the eager step."""
+import numpy as np
+import pytest
+
+from electricbubbles_amd.workloads import perturbed_circle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as _t
+    if not _t.cuda.is_available():
+        pytest.skip("no GPU")
+    return _t
+
+
+def _case(nsim=12, seed=21):
+    rng = np.random.default_rng(seed)
+    nd = (np.round(np.exp(rng.uniform(np.log(128), np.log(512), nsim)) / 2) * 2).astype(np.int32)
+    rku = np.concatenate([perturbed_circle(int(n), [(2, 0.05)])[0] for n in nd]).view(np.float64).copy()
+    e0 = rng.uniform(0.1, 2.0, nsim) ** 2
+    cap = rng.uniform(0.0, 0.2, nsim)
+    return nd, rku, e0, cap
+
+
+def _field(torch, nd, e0, cap, stream):
+    from electricbubbles_amd import field, spectral
+    B = spectral.Bodies(nd, stream=stream)
+    return B, field.SweepField(B, np.arange(nd.size + 1), cap, e0, stream=stream)
+
+
+def _eager_step(SF, spectral, ybar, dt, stream):
+    phi0, y = spectral.rk2_stage1(dt, ybar, SF.rhs(ybar), stream=stream)
+    spectral.rk2_stage2(dt, phi0, SF.rhs(y), ybar, stream=stream)
+
+
+def test_graph_step_matches_eager(torch):
+    from electricbubbles_amd import spectral
+    nd, rku, e0, cap = _case()
+    print("graph step: sweep of", nd.size, "simulations", flush=True)
+    st = torch.cuda.Stream()
+    B, SF = _field(torch, nd, e0, cap, st)
+    B2, SF2 = _field(torch, nd, e0, cap, st)
+    y0 = torch.tensor(rku, device="cuda")
+    torch.cuda.synchronize()
+    yg, ye = y0.clone(), y0.clone()
+    dt = 1e-3
+    with torch.cuda.stream(st):
+        for step in range(4):
+            redone = SF.rk2_graph(yg, dt, budget=12)
+            it_g = SF.iterations()
+            _eager_step(SF2, spectral, ye, dt, st)
+            it_e = SF2.iterations()
+            SF.rhs(ye)  # eager work of the graph's own field between replays
+            st.synchronize()
+            assert not redone
+            for a, b in zip(it_g, it_e):
+                assert np.array_equal(a, b), (step, a, b)
+            d = float((yg - ye).norm() / ye.norm())
+            print("step", step, "rel diff", d, flush=True)
+            assert d < 1e-13
+    torch.cuda.synchronize()
+
+
+def test_graph_budget_fallback(torch):
+    from electricbubbles_amd import spectral
+    nd, rku, e0, cap = _case(nsim=6, seed=5)
+    print("graph budget fallback:", nd.size, "simulations", flush=True)
+    st = torch.cuda.Stream()
+    B, SF = _field(torch, nd, e0, cap, st)
+    B2, SF2 = _field(torch, nd, e0, cap, st)
+    y0 = torch.tensor(rku, device="cuda")
+    torch.cuda.synchronize()
+    yg, ye = y0.clone(), y0.clone()
+    with torch.cuda.stream(st):
+        redone = SF.rk2_graph(yg, 1e-3, budget=2)
+        _eager_step(SF2, spectral, ye, 1e-3, st)
+        st.synchronize()
+    assert redone
+    assert float((yg - ye).norm() / ye.norm()) < 1e-13
