@@ -592,9 +592,11 @@ template <bool SELF>
 __global__ __launch_bounds__(HB_T) void k_hilbert_seg(const int2* __restrict__ tiles, const int* __restrict__ soff,
                                                       int stride, int toff, int srcoff, const double* __restrict__ x,
                                                       const double* __restrict__ y, const double2* __restrict__ dens,
-                                                      const double2* __restrict__ w, double2* __restrict__ cf) {
+                                                      const double2* __restrict__ w, const int* __restrict__ mask,
+                                                      double2* __restrict__ cf) {
   __shared__ double2 sz[HB_T], sq[HB_T];
   const int2 tl = tiles[blockIdx.x];
+  if (mask && !mask[tl.x]) return;  // a system the solver no longer iterates: output unused
   const int o = soff[tl.x], n = soff[tl.x + 1] - o;
   const int base = stride * o;
   const int it = tl.y + (int)threadIdx.x;
@@ -770,7 +772,7 @@ static int sf_lap_apply(void* ctx, const double* u, double* w, void* stream) {
   const int n = F->n;
   hipLaunchKernelGGL(k_real_dens, dim3(ew_blocks(n)), dim3(EW), 0, F->st, n, u, F->dens);
   hipLaunchKernelGGL(k_hilbert_seg<true>, dim3(F->ntile), dim3(HB_T), 0, F->st, F->tiles, F->soff, 1, 0, 0, F->x, F->y,
-                     F->dens, F->wdz, F->cf);
+                     F->dens, F->wdz, eb_gmres_batch_active(F->glap), F->cf);
   hipLaunchKernelGGL(k_lap_point, dim3(ew_blocks(n)), dim3(EW), 0, F->st, n, F->body_of, F->h, u, F->cf, F->rkappa,
                      F->dsda, w);
   EB_HIP_CHECK(hipGetLastError());
@@ -812,7 +814,7 @@ static int sweep_rhs(eb_sweep_field F, const double* rku, double* rkup, int budg
   hipLaunchKernelGGL(k_pv_interp, dim3(B->nbody), dim3(SP_THREADS), lds_bytes(B, 2), F->st, view(B), F->x, F->y,
                      F->ulap, F->dz, F->x2, F->y2, F->uf);
   hipLaunchKernelGGL(k_hilbert_seg<false>, dim3(F->ntile), dim3(HB_T), 0, F->st, F->tiles, F->soff, 2, 0, 1, F->x2,
-                     F->y2, F->uf, F->wf, F->cf2);
+                     F->y2, F->uf, F->wf, (const int*)nullptr, F->cf2);
   hipLaunchKernelGGL(k_dn_post, dim3(B->nbody), dim3(SP_THREADS), lds_bytes(B, 1), F->st, view(B), F->x, F->y, F->ulap,
                      F->cf2, F->dsda, nullptr, nullptr, F->dphi);
   // SETVELF

@@ -117,6 +117,88 @@ __global__ __launch_bounds__(GM_T) void k_gm_dot(int n, const double* __restrict
   finish2(s, 0.0, part, count, out, nullptr);
 }
 
+// DORTH's first pass (dorth.f:90-107) in one workgroup for n <= 1024 * PER: |w|^2 and h_0 =
+// v_0 . w, then w -= h_{i-1} v_{i-1} and h_i = v_i . w for i = 1 .. ll (|w|^2 after the last).
+// w stays in registers for the whole sweep and each basis vector is read once (v_i is kept
+// for the next pass; with KEEPV false it is read again), so one launch replaces ll + 1
+// launches of the multi-block kernels.  Reductions are fixed-order (wave, then block).
+// Used up to n = 16 * 1024 (beyond that the register arrays spill).
+constexpr int MG1_T = 1024;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// block-wide sum over MG1_T threads, returned to every thread
+__device__ __forceinline__ double block_sum1k(double v, double* s) {
+  v = wave_sum(v);
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0) s[wv] = v;
+  __syncthreads();
+  double t = lane < MG1_T / 64 ? s[lane] : 0.0;
+  t = wave_sum(t);
+  __syncthreads();  // s is reused by the next call
+  return t;
+}
+
+template <int PER, bool KEEPV>
+__global__ __launch_bounds__(MG1_T) void k_gm_mgs1(int n, int ll, double* __restrict__ V, double* __restrict__ scal,
+                                                   int sh, int svn, int ssn) {
+  __shared__ double s[MG1_T / 64];
+  const double* v0 = V;
+  double* w = V + (size_t)ll * n;
+  double wr[PER], vp[KEEPV ? PER : 1];
+  double a = 0.0, b = 0.0;
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int i = threadIdx.x + k * MG1_T;
+    const double wi = i < n ? w[i] : 0.0;
+    const double vi = i < n ? v0[i] : 0.0;
+    wr[k] = wi;
+    if (KEEPV) vp[k] = vi;
+    a += wi * wi;
+    b += vi * wi;
+  }
+  a = block_sum1k(a, s);
+  b = block_sum1k(b, s);
+  if (threadIdx.x == 0) {
+    scal[svn] = a;
+    scal[sh] = b;
+  }
+  double h = b;
+  for (int j = 1; j <= ll; ++j) {
+    const double c = -h;
+    const double* vprev = V + (size_t)(j - 1) * n;
+    const double* vn = j < ll ? V + (size_t)j * n : nullptr;
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int i = threadIdx.x + k * MG1_T;
+      if (i < n) {
+        const double pv = KEEPV ? vp[k] : vprev[i];
+        const double wi = wr[k] + c * pv;
+        wr[k] = wi;
+        if (vn) {
+          const double vi = vn[i];
+          if (KEEPV) vp[k] = vi;
+          acc += vi * wi;
+        } else {
+          acc += wi * wi;
+        }
+      }
+    }
+    h = block_sum1k(acc, s);
+    if (threadIdx.x == 0) scal[j < ll ? sh + j : ssn] = h;
+  }
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int i = threadIdx.x + k * MG1_T;
+    if (i < n) w[i] = wr[k];
+  }
+}
+
 // w += c v (c from the host)
 __global__ void k_gm_axpy(int n, double c, const double* __restrict__ v, double* __restrict__ w) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) w[i] += c * v[i];
@@ -256,13 +338,21 @@ int dpigmr_cycle(eb_gmres_s* g, eb_apply_fn apply, void* ctx, double* x, double 
     int rc = apply(ctx, col(g, ll - 1), w, st);
     if (rc != EB_OK) return rc;
     // DORTH (dorth.f:90-121): VNRM, then modified Gram-Schmidt against v_1 .. v_ll
-    hipLaunchKernelGGL(k_gm_norm_dot, dim3(nb), dim3(GM_T), 0, st, n, w, col(g, 0), g->red.part, g->red.count,
-                       g->scal + s_vn(g), g->scal + S_H + 0);
-    for (int i = 1; i <= ll; ++i) {
-      const double* vnext = i < ll ? col(g, i) : nullptr;
-      double* out = i < ll ? g->scal + S_H + i : g->scal + s_sn(g);
-      hipLaunchKernelGGL(k_gm_axpy_dot, dim3(nb), dim3(GM_T), 0, st, n, w, col(g, i - 1), g->scal + S_H + (i - 1),
-                         vnext, g->red.part, g->red.count, out);
+    if (n <= 8 * MG1_T) {
+      hipLaunchKernelGGL((k_gm_mgs1<8, true>), dim3(1), dim3(MG1_T), 0, st, n, ll, g->V, g->scal, S_H, s_vn(g),
+                         s_sn(g));
+    } else if (n <= 16 * MG1_T) {
+      hipLaunchKernelGGL((k_gm_mgs1<16, true>), dim3(1), dim3(MG1_T), 0, st, n, ll, g->V, g->scal, S_H, s_vn(g),
+                         s_sn(g));
+    } else {
+      hipLaunchKernelGGL(k_gm_norm_dot, dim3(nb), dim3(GM_T), 0, st, n, w, col(g, 0), g->red.part, g->red.count,
+                         g->scal + s_vn(g), g->scal + S_H + 0);
+      for (int i = 1; i <= ll; ++i) {
+        const double* vnext = i < ll ? col(g, i) : nullptr;
+        double* out = i < ll ? g->scal + S_H + i : g->scal + s_sn(g);
+        hipLaunchKernelGGL(k_gm_axpy_dot, dim3(nb), dim3(GM_T), 0, st, n, w, col(g, i - 1), g->scal + S_H + (i - 1),
+                           vnext, g->red.part, g->red.count, out);
+      }
     }
     EB_HIP_CHECK(hipGetLastError());
     if ((rc = fetch(g, 0, maxl + 3, h.data())) != EB_OK) return rc;
@@ -722,6 +812,12 @@ __global__ void k_gmb_zero(int* __restrict__ v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *v = 0;
 }
 
+// every system active: the operator must form the initial residual of all of them
+__global__ void k_gmb_all_active(int S, int* __restrict__ act) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < S) act[s] = 1;
+}
+
 // x_s += sum_{j < l_s} c_s[j] V_s(:, j) in the order of dpigmr.f:411-413
 __global__ void k_gmb_combine(const int* __restrict__ off, const int* __restrict__ lcnt, const double* __restrict__ coef,
                               int cld, const double* __restrict__ V, size_t ldv, double* __restrict__ x) {
@@ -852,7 +948,8 @@ extern "C" int eb_gmres_batch_solve(eb_gmres_batch g, eb_apply_fn apply, void* c
   std::fill(g->hes.begin(), g->hes.end(), 0.0);
   std::fill(g->q.begin(), g->q.end(), 0.0);
   std::fill(g->lcnt.begin(), g->lcnt.end(), 0);
-  // r = b - A x, BNRM = |b| per system
+  // r = b - A x, BNRM = |b| per system (every system active for the operator)
+  hipLaunchKernelGGL(k_gmb_all_active, dim3((S + 127) / 128), dim3(128), 0, st, S, g->d_act);
   int rc = apply(ctx, x, g->r, st);
   if (rc != EB_OK) return rc;
   hipLaunchKernelGGL(k_gmb_residual, dim3(S), dim3(GB_T), 0, st, g->d_off, b, g->r, g->hs, hld);
@@ -971,6 +1068,7 @@ extern "C" int eb_gmres_batch_solve_async(eb_gmres_batch g, eb_apply_fn apply, v
   const size_t ldv = (size_t)g->ntot;
   hipStream_t st = g->st;
   const dim3 cb((S + 127) / 128), ct(128);
+  hipLaunchKernelGGL(k_gmb_all_active, cb, ct, 0, st, S, g->d_act);
   int rc = apply(ctx, x, g->r, st);
   if (rc != EB_OK) return rc;
   hipLaunchKernelGGL(k_gmb_residual, dim3(S), dim3(GB_T), 0, st, g->d_off, b, g->r, g->hs, hld);
@@ -990,6 +1088,8 @@ extern "C" int eb_gmres_batch_solve_async(eb_gmres_batch g, eb_apply_fn apply, v
   EB_HIP_CHECK(hipGetLastError());
   return EB_OK;
 }
+
+extern "C" const int32_t* eb_gmres_batch_active(eb_gmres_batch g) { return g ? g->d_act : nullptr; }
 
 extern "C" int eb_gmres_batch_clear_fail(eb_gmres_batch g) {
   if (!g) return (eb_set_error("eb_gmres_batch_clear_fail: bad arguments"), EB_ERR_ARG);
@@ -1021,7 +1121,10 @@ extern "C" int eb_sweep_solve_async(eb_sweep sw, eb_gmres_batch g, const double*
   if (st != g->st)
     return (eb_set_error("eb_sweep_solve_async: the GMRES workspace was created on another stream"), EB_ERR_ARG);
   hipLaunchKernelGGL(k_gm_copy, dim3(ew_grid(g->ntot)), dim3(256), 0, st, g->ntot, rhs, soln);
-  return eb_gmres_batch_solve_async(g, eb_sweep_gmres_apply, sw, rhs, soln, 1.0e-10, budget);
+  eb_sweep_set_mask(sw, g->d_act);
+  const int rc = eb_gmres_batch_solve_async(g, eb_sweep_gmres_apply, sw, rhs, soln, 1.0e-10, budget);
+  eb_sweep_set_mask(sw, nullptr);
+  return rc;
 }
 
 extern "C" int eb_sweep_gmres_apply(void* sw, const double* x, double* y, void* stream) {
@@ -1038,7 +1141,9 @@ extern "C" int eb_sweep_solve(eb_sweep sw, eb_gmres_batch g, const double* rhs, 
     return (eb_set_error("eb_sweep_solve: the GMRES workspace does not match the sweep's systems"), EB_ERR_ARG);
   // soln = rhs as the initial guesses (stokesbubb.f:343-346), ITOL 0, TOL 1e-10, no restart
   hipLaunchKernelGGL(k_gm_copy, dim3(ew_grid(g->ntot)), dim3(256), 0, st, g->ntot, rhs, soln);
+  eb_sweep_set_mask(sw, g->d_act);
   int rc = eb_gmres_batch_solve(g, eb_sweep_gmres_apply, sw, rhs, soln, 1.0e-10, iter, err, ierr);
+  eb_sweep_set_mask(sw, nullptr);
   if (rc != EB_OK) return rc;
   if (za || pk) rc = eb_sweep_parts(sw, soln, za, pk, stream);
   return rc;

@@ -659,9 +659,10 @@ constexpr int SW_T = 128, SW_TILE = 2 * SW_T;
 __global__ __launch_bounds__(SW_T) void k_sw_gw(const int2* __restrict__ tiles, const int* __restrict__ soff,
                                                 const double* __restrict__ x, const double* __restrict__ y,
                                                 const double2* __restrict__ qa, const double2* __restrict__ qb,
-                                                double sgn, double2* __restrict__ gw) {
+                                                double sgn, const int* __restrict__ mask, double2* __restrict__ gw) {
   __shared__ double2 sz[SW_TILE], sa[SW_TILE], sq[SW_TILE];
   const int2 tl = tiles[blockIdx.x];  // (simulation, first target within it)
+  if (mask && !mask[tl.x]) return;    // a system the solver no longer iterates: output unused
   const int o = soff[tl.x], n = soff[tl.x + 1] - o;
   const int it0 = tl.y + (int)threadIdx.x, it1 = it0 + SW_T;
   const bool act0 = it0 < n, act1 = it1 < n;
@@ -740,6 +741,7 @@ struct eb_sweep_s {
   double2 *dz = nullptr, *zk = nullptr, *wdz = nullptr, *gw = nullptr, *za = nullptr, *qa = nullptr, *qb = nullptr;
   int *body_of = nullptr, *first = nullptr, *soff = nullptr;
   int2* tiles = nullptr;
+  const int* mask = nullptr;  // per simulation: 0 = skip in eb_sweep_apply (set around solves)
 };
 
 static void free_sw(eb_sweep S) {
@@ -832,7 +834,7 @@ extern "C" int eb_sweep_apply(eb_sweep S, const double* xx, double* yy, void* st
   const double2* dens = (const double2*)xx;  // dens(i) = dcmplx(u(2i-1), u(2i)) (stokesbubb.f:527-529)
   hipLaunchKernelGGL(k_sw_charges, dim3((n + 255) / 256), dim3(256), 0, st, n, dens, S->wdz, S->qa, S->qb);
   hipLaunchKernelGGL(k_sw_gw, dim3(S->ntile), dim3(SW_T), 0, st, S->tiles, S->soff, S->x, S->y, S->qa, S->qb, -1.0,
-                     S->gw);
+                     S->mask, S->gw);
   sweep_sums(S, dens, st);
   hipLaunchKernelGGL(k_sw_point, dim3((n + 255) / 256), dim3(256), 0, st, n, S->x, S->y, S->dz, S->dsdth, S->rkappa,
                      S->h, dens, S->za, S->pk, S->gw, (double2*)yy, S->body_of, S->first);
@@ -848,7 +850,7 @@ extern "C" int eb_sweep_layer(eb_sweep S, const double* dens, double* gw, int32_
   hipLaunchKernelGGL(k_sw_charges, dim3((n + 255) / 256), dim3(256), 0, st, n, (const double2*)dens, S->wdz, S->qa,
                      S->qb);
   hipLaunchKernelGGL(k_sw_gw, dim3(S->ntile), dim3(SW_T), 0, st, S->tiles, S->soff, S->x, S->y, S->qa, S->qb,
-                     variant == EB_VARIANT_DR ? -1.0 : 1.0, (double2*)gw);
+                     variant == EB_VARIANT_DR ? -1.0 : 1.0, (const int*)nullptr, (double2*)gw);
   EB_HIP_CHECK(hipGetLastError());
   return EB_OK;
 }
@@ -882,6 +884,12 @@ extern "C" int eb_sweep_parts(eb_sweep S, const double* xx, double* za, double* 
 }
 
 extern "C" const int32_t* eb_sweep_unknown_offsets(eb_sweep S) { return S ? S->uoff.data() : nullptr; }
+
+extern "C" int eb_sweep_set_mask(eb_sweep S, const int32_t* mask) {
+  if (!S) return (eb_set_error("eb_sweep_set_mask: bad arguments"), EB_ERR_ARG);
+  S->mask = mask;
+  return EB_OK;
+}
 
 extern "C" int eb_sweep_destroy(eb_sweep S) {
   free_sw(S);

@@ -89,6 +89,9 @@ int eb_gmres_batch_destroy(eb_gmres_batch g);
 int eb_gmres_batch_solve_async(eb_gmres_batch g, eb_apply_fn apply, void *ctx, const double *b, double *x,
                                double tol, int32_t budget);
 int eb_gmres_batch_clear_fail(eb_gmres_batch g);
+/* The workspace's per-system active flags (device, nsys ints): 1 while a system iterates (all
+ * 1 for the initial residual).  An operator may skip the systems whose flag is 0. */
+const int32_t *eb_gmres_batch_active(eb_gmres_batch g);
 int eb_gmres_batch_results(eb_gmres_batch g, int32_t *iter, double *err, int32_t *ierr, int32_t *nfail);
 
 /* eb_apply_fn for the sweep operator (ctx = an eb_sweep) */

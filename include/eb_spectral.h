@@ -109,6 +109,10 @@ int eb_sweep_rebuild(eb_sweep sw, const double *x, const double *y, const double
                      const double *rkappa, const double *zk, void *stream);
 /* unknowns of each simulation: offsets into the 2*nbk vector (nsim + 1 ints, host memory) */
 const int32_t *eb_sweep_unknown_offsets(eb_sweep sw);
+/* Per-simulation flags (device, nsim ints, may be NULL): while set, eb_sweep_apply skips the
+ * layer potential of simulations whose flag is 0 (their output is left unspecified).  The
+ * batched solves set it to the solver's active flags around their operator calls. */
+int eb_sweep_set_mask(eb_sweep sw, const int32_t *mask);
 int eb_sweep_destroy(eb_sweep sw);
 
 /* Drop-in for FASMVP (stokesbubb.f:496-498) on host arrays, Fortran convention (all by
