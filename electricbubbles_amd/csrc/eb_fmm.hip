@@ -305,6 +305,56 @@ __global__ __launch_bounds__(64) void k_p2m(int L, int p, const int* __restrict_
 }
 
 // ------------------------------------------------------------------------------------------
+// Compacted M2L target lists.  For parity class c (block c): the non-empty boxes of levels
+// 2..L, entry ((l - 2) << 24) | q with q the box's index within the class at level l, in level
+// then q order; each level's run is padded with -1 to a multiple of 4 so that every group of
+// four entries (one wave of k_xlat) lies within one level.  count[c] = the padded length.
+// The list of class c starts at c * cap, cap = sum over levels of (boxes of the class + 3).
+// ------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k_m2l_compact(int L, const int* __restrict__ leaf_start, int cap,
+                                                      int* __restrict__ list, int* __restrict__ count) {
+  __shared__ int wsum[16], wpre[16], tot_s;
+  const int cls = blockIdx.x;
+  int* out = list + (size_t)cls * cap;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int pos = 0;
+  for (int l = 2; l <= L; ++l) {
+    const int half = 1 << (l - 1), nq = half * half;
+    const int seg = pos;
+    for (int c0 = 0; c0 < nq; c0 += 1024) {
+      const int q = c0 + (int)threadIdx.x;
+      bool live = false;
+      if (q < nq) {
+        const int ix = 2 * (q % half) + (cls & 1), iy = 2 * (q / half) + (cls >> 1);
+        int bs, be;
+        box_range(leaf_start, L, l, ix, iy, bs, be);
+        live = be > bs;
+      }
+      const unsigned long long m = __ballot(live);
+      const int wrank = __popcll(m & ((1ull << lane) - 1ull));
+      if (lane == 0) wsum[wv] = __popcll(m);
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int k = 0; k < 16; ++k) {
+          wpre[k] = acc;
+          acc += wsum[k];
+        }
+        tot_s = acc;
+      }
+      __syncthreads();
+      if (live) out[pos + wpre[wv] + wrank] = ((l - 2) << 24) | q;
+      pos += tot_s;
+      __syncthreads();  // wsum, wpre, tot_s are rewritten by the next chunk
+    }
+    const int pad = (4 - ((pos - seg) & 3)) & 3;
+    if ((int)threadIdx.x < pad) out[pos + threadIdx.x] = -1;
+    pos += pad;
+  }
+  if (threadIdx.x == 0) count[cls] = pos;
+}
+
+// ------------------------------------------------------------------------------------------
 // Translations on FP64 matrix cores: M2M (upward), M2L (interaction lists), L2L (downward).
 //
 // With expansions scaled by the half-width of their box, each translation is
@@ -344,11 +394,17 @@ __device__ __forceinline__ double swap_adjacent(double v) {
   return __hiloint2double(hi, lo);
 }
 
+// With a compacted target list (M2L only: m2l_list / m2l_count from k_m2l_compact) block b
+// takes entries 16 (b / 4) .. 16 (b / 4) + 15 of parity class b % 4, four per wave: only
+// non-empty target boxes are translated, which matters where most boxes are empty (drops
+// scattered over a large square).  Blocks past the class's count leave at once.
 template <int MODE, int KMAX, int KS, int MT>
 __global__ __launch_bounds__(256) void k_xlat(XlatLevels LS, const int* __restrict__ leaf_start,
                                               const double2* __restrict__ src, double2* __restrict__ dst,
                                               const double* __restrict__ Mat, const double2* __restrict__ pre,
-                                              const double2* __restrict__ post, const int* __restrict__ offs) {
+                                              const double2* __restrict__ post, const int* __restrict__ offs,
+                                              const int* __restrict__ m2l_list, const int* __restrict__ m2l_count,
+                                              int m2l_cap) {
   static_assert(KS * 4 <= KMAX && MT * 16 <= KMAX, "tile exceeds the padded expansion length");
   constexpr int NS = MODE == XL_M2L ? 27 : (MODE == XL_M2M ? 4 : 1);
   constexpr int NK = KS * 4, NM = MT * 16, CS = NM + 4;
@@ -357,9 +413,20 @@ __global__ __launch_bounds__(256) void k_xlat(XlatLevels LS, const int* __restri
   __shared__ double2 Qs[NS * NM];
   // XCD-aware order: consecutive logical blocks (neighbouring rows, all four parity classes)
   // are dispatched to the same XCD and share its L2.
-  const int nphys = gridDim.x;
+  const bool compact = MODE == XL_M2L && m2l_list != nullptr;
+  int nphys = gridDim.x;
+  if (compact) {
+    // logical blocks that can hold entries: 4 classes x the longest class's blocks, so that
+    // the XCD bands below spread them over all eight XCDs
+    int mx = 0;
+    for (int c = 0; c < 4; ++c) mx = max(mx, (m2l_count[c] + 15) / 16);
+    nphys = (4 * mx + 7) / 8 * 8;
+    if ((int)blockIdx.x >= nphys) return;  // the whole block: before any barrier
+  }
   const int b = (blockIdx.x % 8) * (nphys / 8) + blockIdx.x / 8;
   const int cls = b & 3;
+  const int ccount = compact ? m2l_count[cls] : 0;
+  if (compact && 16 * (b >> 2) >= ccount) return;
   for (int i = threadIdx.x; i < NK * NM; i += blockDim.x) {
     const int k = i / NM, m = i % NM;
     Cs[k * CS + m] = Mat[k * KMAX + m];
@@ -380,16 +447,30 @@ __global__ __launch_bounds__(256) void k_xlat(XlatLevels LS, const int* __restri
   }
   __syncthreads();
   if (b >= LS.total) return;
-  int e = 0;
-  while (e + 1 < LS.nent && b >= LS.blk0[e + 1]) ++e;
-  const int l = LS.lev[e], Wt = LS.Wt[e], NBPt = LS.NBPt[e], Ws = LS.Ws[e], NBPs = LS.NBPs[e];
-  const int g = (b - LS.blk0[e]) >> 2;
-  const int half = 1 << (l - 1);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int col = lane & 15, kr = lane >> 4;
   const int t = col >> 2, s = (col >> 1) & 1, comp = col & 1;
-  const int q = (g * 4 + wv) * 4 + t;
-  const bool valid = q < half * half;
+  int e = 0, q;
+  bool valid;
+  if (compact) {
+    const int idx = 16 * (b >> 2) + 4 * wv + t;
+    const int ent = idx < ccount ? m2l_list[(size_t)cls * m2l_cap + idx] : -1;
+    valid = ent >= 0;
+    // every valid entry of a wave has the same level (runs are padded to groups of 4)
+    int ev = valid ? (ent >> 24) : -1;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) ev = max(ev, __shfl_xor(ev, o, 64));
+    if (ev < 0) return;  // a wave of padding only; no barrier follows
+    e = __builtin_amdgcn_readfirstlane(ev);
+    q = valid ? (ent & 0xFFFFFF) : 0;
+  } else {
+    while (e + 1 < LS.nent && b >= LS.blk0[e + 1]) ++e;
+    const int g = (b - LS.blk0[e]) >> 2;
+    q = (g * 4 + wv) * 4 + t;
+    valid = q < (1 << (2 * (LS.lev[e] - 1)));
+  }
+  const int l = LS.lev[e], Wt = LS.Wt[e], NBPt = LS.NBPt[e], Ws = LS.Ws[e], NBPs = LS.NBPs[e];
+  const int half = 1 << (l - 1);
   const int qq = valid ? q : 0;
   const int ix = 2 * (qq % half) + (cls & 1), iy = 2 * (qq / half) + (cls >> 1);
   bool live = valid;
@@ -831,6 +912,9 @@ struct eb_fmm_plan_s {
   int* p2p_off = nullptr;
   int* p2p_tmp = nullptr;
   int* ntbox = nullptr;
+  int* m2l_list = nullptr;   // compacted M2L targets (k_m2l_compact), 4 x m2l_cap
+  int* m2l_count = nullptr;  // 4
+  int m2l_cap = 0;
   double2* mp = nullptr;
   double2* lo = nullptr;
   size_t lvl_off[EB_MAXLEV + 2] = {0};
@@ -849,7 +933,7 @@ static void plan_free(eb_fmm_plan P) {
   if (!P) return;
   void* ptrs[] = {P->geom, P->bbpart, P->zs0, P->zs, P->wdzs, P->qa, P->qb, P->gwf, P->keys, P->keys2,
                   P->perm, P->vals2, P->sorttmp, P->leaf_start, P->p2p_items, P->p2p_cnt, P->p2p_off,
-                  P->p2p_tmp, P->ntbox, P->mp, P->lo, P->dMat,
+                  P->p2p_tmp, P->ntbox, P->m2l_list, P->m2l_count, P->mp, P->lo, P->dMat,
                   P->dU, P->dNU, P->dPM, P->dQM, P->dPL, P->dQL, P->doffs};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
@@ -914,6 +998,9 @@ static int build_tree(eb_fmm_plan P, const double* x, const double* y, const dou
   if (rc) return rc;
   hipLaunchKernelGGL(k_p2p_items, dim3((nleaf + 255) / 256), dim3(256), 0, st, nleaf, P->leaf_start, P->p2p_off,
                      P->p2p_items);
+  if (P->m2l_list)
+    hipLaunchKernelGGL(k_m2l_compact, dim3(4), dim3(1024), 0, st, L, P->leaf_start, P->m2l_cap, P->m2l_list,
+                       P->m2l_count);
   EB_HIP_CHECK(hipGetLastError());
   return EB_OK;
 }
@@ -977,6 +1064,13 @@ extern "C" int eb_fmm_plan_create(eb_fmm_plan* out, int32_t n, const double* x, 
   EB_TRY(dmalloc(&P->p2p_off, nleaf + 1));
   EB_TRY(dmalloc(&P->p2p_tmp, scan_temp_ints(nleaf)));
   EB_TRY(dmalloc(&P->ntbox, EB_MAXLEV + 2));
+  if (mode == EB_MODE_FMM && L >= 2) {
+    size_t cap = 0;
+    for (int l = 2; l <= L; ++l) cap += ((size_t)1 << (2 * (l - 1))) + 3;
+    P->m2l_cap = (int)cap;
+    EB_TRY(dmalloc(&P->m2l_list, 4 * cap));
+    EB_TRY(dmalloc(&P->m2l_count, 4));
+  }
 
   EB_TRY(build_tree(P, x, y, wdz));
 
@@ -1055,8 +1149,9 @@ static void launch_xlat_ks(eb_fmm_plan P, const XlatLevels& LS, const double2* s
   constexpr int MT = K / 16;
   const int grid = (LS.total + 7) / 8 * 8;
   const double* mat = P->dMat + (size_t)MODE * K * K;
+  const bool c = MODE == XL_M2L && P->m2l_list != nullptr;
   hipLaunchKernelGGL((k_xlat<MODE, K, KS, MT>), dim3(grid), dim3(256), 0, P->st, LS, P->leaf_start, src, dst, mat,
-                     pre, post, P->doffs);
+                     pre, post, P->doffs, c ? P->m2l_list : nullptr, c ? P->m2l_count : nullptr, P->m2l_cap);
 }
 
 // (KS, MT) = (ceil(p/4), K/16): the four k-step counts that round up to K
