@@ -8,7 +8,9 @@ fixed seed; kernel = CADAPDR's GW (DR signs), FP64, evaluated at every point.
 One step = one complete FMM evaluation from raw device-resident coordinates: tree build
 (binning, sort, leaf table) + P2M, M2M, M2L+L2L, L2P, P2P.  `value` is points per second of
 that whole evaluation (Mpts/s).  The apply-only rate (tree kept, as inside one GMRES solve)
-is reported beside it.
+is reported beside it.  LEVMAX and p are the FMM autotuner's choice for 1e-10 relative
+accuracy (2-norm against the same tree at p = 60) unless given; the accuracy of the timed
+configuration and the fixed LEVMAX 8 / p = 32 point are reported beside it.
 
 Prints ONE JSON line.  Single GPU only.
 """
@@ -32,10 +34,11 @@ PEAK_HBM_GBS = 8000.0
 
 # HBM bytes per launch of the dominant kernels, from two rocprofv3 --pmc passes (FETCH_SIZE,
 # WRITE_SIZE) of this script, scaled by the same counters on a 256 MiB copy (fetch x 1.99976,
-# write x 1.000); profiles/fmm_curves_n1048576_L8_p32_hbm_traffic.md has the commands and
-# the per-kernel table.  Keyed by (workload, points, levmax, nterms).
+# write x 1.000); profiles/fmm_curves_n1048576_L8_p{32,36}_hbm_traffic.md have the commands
+# and the per-kernel tables.  Keyed by (workload, points, levmax, nterms).
 PROFILED_TRAFFIC_BYTES = {
-    ("curves", 1 << 20, 8, 32): {"m2l": 208822278, "p2p": 417546006},
+    ("curves", 1 << 20, 8, 32): {"m2l": 221885419, "p2p": 417531852},
+    ("curves", 1 << 20, 8, 36): {"m2l": 294264137, "p2p": 417535436},
 }
 
 
@@ -317,7 +320,7 @@ def sweep_leg(torch, dev, with_cpu):
                 "batched_vs_plain": round((nsim / t_fb) * per_fp, 2),
                 "plain_vs_batched_rel_diff": worst_f}
     # one Heun step of every simulation: eager (host-controlled solves) and as a replayed
-    # hipGraph (device-controlled solves, at most 12 Arnoldi steps each)
+    # hipGraph (device-controlled solves, at most 8 Arnoldi steps each)
     gst = torch.cuda.Stream()
     Bg = spectral.Bodies(nd, stream=gst)
     SFg = field.SweepField(Bg, np.arange(nsim + 1), np.zeros(nsim), e0, stream=gst)
@@ -345,7 +348,7 @@ def sweep_leg(torch, dev, with_cpu):
         diff = float((yg - ye).norm() / ye.norm())
     torch.cuda.synchronize()
     out["rk2_step"] = {"what": "one Heun step (2 F) of all simulations", "dt": dt, "eager_ms": round(t_eager * 1e3, 3),
-                       "graph_ms": round(t_graph * 1e3, 3), "graph_budget": 12, "graph_steps_redone": int(redone),
+                       "graph_ms": round(t_graph * 1e3, 3), "graph_budget": 8, "graph_steps_redone": int(redone),
                        "steps_per_s_graph": round(1.0 / t_graph, 1),
                        "simulation_steps_per_s_graph": round(nsim / t_graph, 1),
                        "graph_vs_eager_state_rel_diff": diff}
@@ -543,11 +546,11 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", choices=["curves", "uniform"], default="curves")
-    ap.add_argument("--levmax", type=int, default=8)
-    ap.add_argument("--nterms", type=int, default=32)
+    ap.add_argument("--levmax", type=int, default=0, help="0: the autotuner's choice")
+    ap.add_argument("--nterms", type=int, default=0, help="0: the autotuner's choice (1e-10 relative)")
     ap.add_argument("--npts", type=int, default=1 << 20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-check", action="store_true", help="skip the accuracy spot check")
+    ap.add_argument("--no-check", action="store_true", help="skip the accuracy check against p = 60")
     ap.add_argument("--no-solve", action="store_true", help="skip the Stokes GMRES solve leg")
     ap.add_argument("--no-spectral", action="store_true", help="skip the spectral geometry leg")
     ap.add_argument("--no-sweep", action="store_true", help="skip the batched-versus-plain sweep leg")
@@ -612,6 +615,24 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
+    # the autotuner picks the depth and the expansion length for these points at 1e-10
+    # relative (2-norm, against the same FMM at p = 60) unless both are given
+    tuned = None
+    if L <= 0 or p <= 0 or not args.no_autotune:
+        from electricbubbles_amd import autotune
+        t0 = time.perf_counter()
+        log("autotune: start")
+        tr = autotune.tune_fmm(x, y, w, d, target=1e-10)
+        log(f"autotune: LEVMAX {tr['levmax']}, p {tr['nterms']}")
+        tuned = {"target_rel": 1e-10, "levmax": tr["levmax"], "nterms": tr["nterms"], "apply_ms": tr["ms"],
+                 "err_vs_p60": tr["err"], "tuning_s": round(time.perf_counter() - t0, 3),
+                 "p_scan": tr["p_scan"], "p_scan_levmax": tr["p_scan_levmax"],
+                 "candidates": [{k: c[k] for k in ("levmax", "points_per_leaf", "ms")} for c in tr["candidates"]]}
+        if L <= 0:
+            L = tr["levmax"]
+        if p <= 0:
+            p = tr["nterms"]
+
     log(f"FMM: {n} points, LEVMAX {L}, p {p}")
     plan = eb.FMMPlan(x, y, w, L, p, eb.VARIANT_DR, eb.MODE_FMM, stream=stream)
     for _ in range(args.warmup):
@@ -651,18 +672,6 @@ def main():
     traffic = PROFILED_TRAFFIC_BYTES.get((args.workload, n, L, p), {}).get(dom)
     total_fl = sum(fl.values())
 
-    # the autotuner's choice of depth and expansion length for the same points at 1e-10
-    tuned = None
-    if not args.no_autotune:
-        from electricbubbles_amd import autotune
-        t0 = time.perf_counter()
-        log("autotune: start")
-        tr = autotune.tune_fmm(x, y, w, d, target=1e-10)
-        log("autotune: done")
-        tuned = {"target_rel": 1e-10, "levmax": tr["levmax"], "nterms": tr["nterms"], "apply_ms": tr["ms"],
-                 "err_vs_p60": tr["err"], "tuning_s": round(time.perf_counter() - t0, 3),
-                 "candidates": [{k: c[k] for k in ("levmax", "points_per_leaf", "ms")} for c in tr["candidates"]]}
-
     # the adaptive tree of the same points (DAPIF2's rules, at most 40 points per childless box)
     from electricbubbles_amd.atree import AdaptiveTree
     at = AdaptiveTree(x, y, 40)
@@ -677,13 +686,37 @@ def main():
     torch.cuda.synchronize()
     at_info["build_ms"] = round((time.perf_counter() - t0) / reps * 1e3, 3)
 
-    check = None
+    # accuracy of the timed configuration: relative 2-norm difference from the same tree at
+    # p = 60 (the reference's longest expansion), whose own truncation is below 1e-15; and the
+    # fixed LEVMAX 8 / p = 32 point for comparison
+    check, fixed32 = None, None
     if not args.no_check:
-        import oracle
-        tgt = np.sort(rng.choice(n, 64, replace=False)).astype(np.int32)
-        ref = oracle.direct(oracle.DR, nd, z.real, z.imag, dz, dens, h, tgt)
-        got = out.cpu().numpy()[tgt]
-        check = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+        ref_plan = eb.FMMPlan(x, y, w, L, 60, eb.VARIANT_DR, eb.MODE_FMM, stream=stream)
+        ref60 = ref_plan.apply(d)
+        check = float((out - ref60).norm() / ref60.norm())
+        ref_plan.close()
+        if (L, p) != (8, 32):
+            p32 = eb.FMMPlan(x, y, w, 8, 32, eb.VARIANT_DR, eb.MODE_FMM, stream=stream)
+            o32 = p32.apply(d)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                p32.rebuild(x, y, w)
+                p32.apply(d, o32)
+            torch.cuda.synchronize()
+            t32f = (time.perf_counter() - t0) / args.steps
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                p32.apply(d, o32)
+            torch.cuda.synchronize()
+            t32a = (time.perf_counter() - t0) / args.steps
+            r60 = eb.FMMPlan(x, y, w, 8, 60, eb.VARIANT_DR, eb.MODE_FMM, stream=stream)
+            rr = r60.apply(d)
+            fixed32 = {"levmax": 8, "nterms": 32, "value": round(n / t32f / 1e6, 3),
+                       "apply_only_value": round(n / t32a / 1e6, 3),
+                       "err2_vs_p60": float((o32 - rr).norm() / rr.norm())}
+            r60.close()
+            p32.close()
 
     solve = None if args.no_solve else leg("stokes solve", stokes_solve_leg, torch, dev, not args.no_cpu)
     spec = None if args.no_spectral else leg("spectral", spectral_leg, torch, dev, not args.no_cpu)
@@ -728,7 +761,8 @@ def main():
         "roofline": {"bound": "mfma", "kernel": dom, "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
                      "flops_per_launch": fl[dom]},
-        "accuracy_vs_direct": check,
+        "accuracy_err2_vs_p60": check,
+        "fixed_levmax8_p32": fixed32,
         "cpu_baseline": cpu,
     }
     sys.stdout.flush()

@@ -480,8 +480,8 @@ def rhs(P, z, e0, cap):
 
 def initial_shapes(centres, modes, N, decay=0.0):
     """z_j = c + (1 + sum_m m^-decay (a_m cos(m th_j) + b_m sin(m th_j))) e^{i th_j},
-    th_j = 2 pi j / N; modes (k, 2M): a_1..a_M, b_1..b_M per body."""
-    torch = _torch()
+    th_j = 2 pi j / N; modes (k, 2M): a_1..a_M, b_1..b_M per body.  Device-agnostic."""
+    import torch
     k, twoM = modes.shape
     M = twoM // 2
     th = torch.arange(N, dtype=torch.float64, device=modes.device) * (2.0 * PI / N)
@@ -494,11 +494,17 @@ def initial_shapes(centres, modes, N, decay=0.0):
 
 def deformation(P, z):
     """D = (L - B) / (L + B) per body, L and B the semi-axes of the ellipse with the body's area
-    and second moments of area (trapezoid sums of Green's-theorem integrands)."""
-    torch = _torch()
-    za = _fdifff(P, z)
+    and second moments of area (trapezoid sums of Green's-theorem integrands).  z: (k, N)
+    complex, counterclockwise; P is unused apart from its type (any object works), so the
+    loss can be evaluated on any device."""
+    import torch
+    N = z.shape[1]
+    kw = torch.fft.fftfreq(N, 1.0 / N, dtype=torch.float64, device=z.device)
+    if N % 2 == 0:
+        kw[N // 2] = 0.0
+    za = torch.fft.ifft(1j * kw * torch.fft.fft(z, dim=1), dim=1)
     x, y, xa, ya = z.real, z.imag, za.real, za.imag
-    h = P.h
+    h = 2.0 * PI / N
     A = 0.5 * h * (x * ya - y * xa).sum(1)
     mx = 0.5 * h * (x * x * ya).sum(1)
     my = -0.5 * h * (y * y * xa).sum(1)

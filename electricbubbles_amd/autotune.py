@@ -47,7 +47,7 @@ def _rel(a, b):
     return float((a - b).norm() / b.norm())
 
 
-def tune_fmm(x, y, wdz, dens=None, target=1e-10, variant=VARIANT_DR, nterms=(16, 24, 28, 32, 40, 48),
+def tune_fmm(x, y, wdz, dens=None, target=1e-10, variant=VARIANT_DR, nterms=(16, 24, 28, 32, 36, 40, 48),
              occupancy=(4.0, 64.0), reps=3, cache_file=None, seed=0):
     """Returns {"levmax", "nterms", "ms", "err", "candidates": [...]} for the point set (x, y)
     with weights wdz (device tensors as FMMPlan takes them).  dens: the density to time and
@@ -84,10 +84,12 @@ def tune_fmm(x, y, wdz, dens=None, target=1e-10, variant=VARIANT_DR, nterms=(16,
     ref = ref_plan.apply(dens)
     ref_plan.close()
     p_sel = P_MAX
+    p_scan = []
     for p in sorted(nterms):
         plan = FMMPlan(x, y, wdz, Lm, p, variant=variant, mode=MODE_FMM)
         err = _rel(plan.apply(dens), ref)
         plan.close()
+        p_scan.append({"nterms": p, "err": err})
         if err <= target:
             p_sel = p
             break
@@ -102,7 +104,7 @@ def tune_fmm(x, y, wdz, dens=None, target=1e-10, variant=VARIANT_DR, nterms=(16,
     ok = [c for c in cands if c["err"] <= target] or cands
     best = min(ok, key=lambda c: c["ms"])
     res = {"levmax": best["levmax"], "nterms": p_sel, "ms": best["ms"], "err": best["err"], "target": target,
-           "candidates": cands}
+           "candidates": cands, "p_scan": p_scan, "p_scan_levmax": Lm}
     _CACHE[key] = res
     if cache_file:
         disk = {}

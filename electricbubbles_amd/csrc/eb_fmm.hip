@@ -305,53 +305,99 @@ __global__ __launch_bounds__(64) void k_p2m(int L, int p, const int* __restrict_
 }
 
 // ------------------------------------------------------------------------------------------
-// Compacted M2L target lists.  For parity class c (block c): the non-empty boxes of levels
+// Compacted M2L target lists.  For parity class c: the non-empty boxes of levels
 // 2..L, entry ((l - 2) << 24) | q with q the box's index within the class at level l, in level
 // then q order; each level's run is padded with -1 to a multiple of 4 so that every group of
 // four entries (one wave of k_xlat) lies within one level.  count[c] = the padded length.
 // The list of class c starts at c * cap, cap = sum over levels of (boxes of the class + 3).
 // ------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void k_m2l_compact(int L, const int* __restrict__ leaf_start, int cap,
-                                                      int* __restrict__ list, int* __restrict__ count) {
-  __shared__ int wsum[16], wpre[16], tot_s;
-  const int cls = blockIdx.x;
-  int* out = list + (size_t)cls * cap;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  int pos = 0;
-  for (int l = 2; l <= L; ++l) {
-    const int half = 1 << (l - 1), nq = half * half;
-    const int seg = pos;
-    for (int c0 = 0; c0 < nq; c0 += 1024) {
-      const int q = c0 + (int)threadIdx.x;
-      bool live = false;
-      if (q < nq) {
-        const int ix = 2 * (q % half) + (cls & 1), iy = 2 * (q / half) + (cls >> 1);
-        int bs, be;
-        box_range(leaf_start, L, l, ix, iy, bs, be);
-        live = be > bs;
-      }
-      const unsigned long long m = __ballot(live);
-      const int wrank = __popcll(m & ((1ull << lane) - 1ull));
-      if (lane == 0) wsum[wv] = __popcll(m);
-      __syncthreads();
-      if (threadIdx.x == 0) {
-        int acc = 0;
-        for (int k = 0; k < 16; ++k) {
-          wpre[k] = acc;
-          acc += wsum[k];
-        }
-        tot_s = acc;
-      }
-      __syncthreads();
-      if (live) out[pos + wpre[wv] + wrank] = ((l - 2) << 24) | q;
-      pos += tot_s;
-      __syncthreads();  // wsum, wpre, tot_s are rewritten by the next chunk
-    }
-    const int pad = (4 - ((pos - seg) & 3)) & 3;
-    if ((int)threadIdx.x < pad) out[pos + threadIdx.x] = -1;
-    pos += pad;
+// Two passes over blocks of 1024 boxes, blockIdx.x = cls * nb + j, j enumerating the chunks of
+// levels 2..L in order: k_m2l_count counts each chunk's non-empty boxes, k_m2l_write places
+// them after the chunks (and padded levels) before it.
+__device__ __forceinline__ void m2l_chunk(int L, int nb, int& cls, int& l, int& c0) {
+  cls = blockIdx.x / nb;
+  int j = blockIdx.x - cls * nb;
+  for (l = 2; l <= L; ++l) {
+    const int nc = ((1 << (2 * (l - 1))) + 1023) >> 10;
+    if (j < nc) break;
+    j -= nc;
   }
-  if (threadIdx.x == 0) count[cls] = pos;
+  c0 = j << 10;
+}
+
+__device__ __forceinline__ bool m2l_live(const int* __restrict__ leaf_start, int L, int l, int cls, int q) {
+  const int half = 1 << (l - 1);
+  if (q >= half * half) return false;
+  const int ix = 2 * (q % half) + (cls & 1), iy = 2 * (q / half) + (cls >> 1);
+  int bs, be;
+  box_range(leaf_start, L, l, ix, iy, bs, be);
+  return be > bs;
+}
+
+__global__ __launch_bounds__(1024) void k_m2l_count(int L, int nb, const int* __restrict__ leaf_start,
+                                                    int* __restrict__ ccnt) {
+  __shared__ int wsum[16];
+  int cls, l, c0;
+  m2l_chunk(L, nb, cls, l, c0);
+  const bool live = m2l_live(leaf_start, L, l, cls, c0 + (int)threadIdx.x);
+  const unsigned long long m = __ballot(live);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = __popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int acc = 0;
+    for (int k = 0; k < 16; ++k) acc += wsum[k];
+    ccnt[blockIdx.x] = acc;
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_m2l_write(int L, int nb, const int* __restrict__ leaf_start,
+                                                    const int* __restrict__ ccnt, int cap, int* __restrict__ list,
+                                                    int* __restrict__ count) {
+  __shared__ int wsum[16], wpre[16], base_s, last_s, lvl_s;
+  int cls, l, c0;
+  m2l_chunk(L, nb, cls, l, c0);
+  if (threadIdx.x == 0) {
+    // entries before this chunk: whole padded levels below l, then this level's earlier chunks
+    const int* cc = ccnt + cls * nb;
+    int pos = 0, j = 0;
+    for (int ll = 2; ll < l; ++ll) {
+      const int nc = ((1 << (2 * (ll - 1))) + 1023) >> 10;
+      int t = 0;
+      for (int k = 0; k < nc; ++k) t += cc[j + k];
+      pos += (t + 3) & ~3;
+      j += nc;
+    }
+    const int nc = ((1 << (2 * (l - 1))) + 1023) >> 10;
+    const int mine = c0 >> 10;
+    int lvl = 0;
+    for (int k = 0; k < nc; ++k) {
+      if (k == mine) base_s = pos + lvl;
+      lvl += cc[j + k];
+    }
+    last_s = mine == nc - 1;
+    lvl_s = pos + lvl;  // end of this level's entries before padding
+  }
+  const int q = c0 + (int)threadIdx.x;
+  const bool live = m2l_live(leaf_start, L, l, cls, q);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const unsigned long long m = __ballot(live);
+  if (lane == 0) wsum[wv] = __popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int acc = 0;
+    for (int k = 0; k < 16; ++k) {
+      wpre[k] = acc;
+      acc += wsum[k];
+    }
+  }
+  __syncthreads();
+  int* out = list + (size_t)cls * cap;
+  if (live) out[base_s + wpre[wv] + __popcll(m & ((1ull << lane) - 1ull))] = ((l - 2) << 24) | q;
+  if (last_s) {
+    const int end = lvl_s, pad = (4 - ((end) & 3)) & 3;
+    if ((int)threadIdx.x < pad) out[end + threadIdx.x] = -1;
+    if (l == L && threadIdx.x == 0) count[cls] = end + pad;
+  }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -394,7 +440,7 @@ __device__ __forceinline__ double swap_adjacent(double v) {
   return __hiloint2double(hi, lo);
 }
 
-// With a compacted target list (M2L only: m2l_list / m2l_count from k_m2l_compact) block b
+// With a compacted target list (M2L only: m2l_list / m2l_count from k_m2l_write) block b
 // takes entries 16 (b / 4) .. 16 (b / 4) + 15 of parity class b % 4, four per wave: only
 // non-empty target boxes are translated, which matters where most boxes are empty (drops
 // scattered over a large square).  Blocks past the class's count leave at once.
@@ -912,9 +958,10 @@ struct eb_fmm_plan_s {
   int* p2p_off = nullptr;
   int* p2p_tmp = nullptr;
   int* ntbox = nullptr;
-  int* m2l_list = nullptr;   // compacted M2L targets (k_m2l_compact), 4 x m2l_cap
+  int* m2l_list = nullptr;   // compacted M2L targets (k_m2l_count, k_m2l_write), 4 x m2l_cap
   int* m2l_count = nullptr;  // 4
-  int m2l_cap = 0;
+  int* m2l_ccnt = nullptr;   // per (class, chunk of 1024 boxes) counts, 4 x m2l_nb
+  int m2l_cap = 0, m2l_nb = 0;
   double2* mp = nullptr;
   double2* lo = nullptr;
   size_t lvl_off[EB_MAXLEV + 2] = {0};
@@ -933,7 +980,7 @@ static void plan_free(eb_fmm_plan P) {
   if (!P) return;
   void* ptrs[] = {P->geom, P->bbpart, P->zs0, P->zs, P->wdzs, P->qa, P->qb, P->gwf, P->keys, P->keys2,
                   P->perm, P->vals2, P->sorttmp, P->leaf_start, P->p2p_items, P->p2p_cnt, P->p2p_off,
-                  P->p2p_tmp, P->ntbox, P->m2l_list, P->m2l_count, P->mp, P->lo, P->dMat,
+                  P->p2p_tmp, P->ntbox, P->m2l_list, P->m2l_count, P->m2l_ccnt, P->mp, P->lo, P->dMat,
                   P->dU, P->dNU, P->dPM, P->dQM, P->dPL, P->dQL, P->doffs};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
@@ -998,9 +1045,12 @@ static int build_tree(eb_fmm_plan P, const double* x, const double* y, const dou
   if (rc) return rc;
   hipLaunchKernelGGL(k_p2p_items, dim3((nleaf + 255) / 256), dim3(256), 0, st, nleaf, P->leaf_start, P->p2p_off,
                      P->p2p_items);
-  if (P->m2l_list)
-    hipLaunchKernelGGL(k_m2l_compact, dim3(4), dim3(1024), 0, st, L, P->leaf_start, P->m2l_cap, P->m2l_list,
-                       P->m2l_count);
+  if (P->m2l_list) {
+    hipLaunchKernelGGL(k_m2l_count, dim3(4 * P->m2l_nb), dim3(1024), 0, st, L, P->m2l_nb, P->leaf_start,
+                       P->m2l_ccnt);
+    hipLaunchKernelGGL(k_m2l_write, dim3(4 * P->m2l_nb), dim3(1024), 0, st, L, P->m2l_nb, P->leaf_start,
+                       P->m2l_ccnt, P->m2l_cap, P->m2l_list, P->m2l_count);
+  }
   EB_HIP_CHECK(hipGetLastError());
   return EB_OK;
 }
@@ -1068,8 +1118,12 @@ extern "C" int eb_fmm_plan_create(eb_fmm_plan* out, int32_t n, const double* x, 
     size_t cap = 0;
     for (int l = 2; l <= L; ++l) cap += ((size_t)1 << (2 * (l - 1))) + 3;
     P->m2l_cap = (int)cap;
+    int nb = 0;
+    for (int l = 2; l <= L; ++l) nb += ((1 << (2 * (l - 1))) + 1023) >> 10;
+    P->m2l_nb = nb;
     EB_TRY(dmalloc(&P->m2l_list, 4 * cap));
     EB_TRY(dmalloc(&P->m2l_count, 4));
+    EB_TRY(dmalloc(&P->m2l_ccnt, 4 * (size_t)nb));
   }
 
   EB_TRY(build_tree(P, x, y, wdz));

@@ -155,7 +155,7 @@ class SweepField:
         _check(_bind().eb_sweep_field_iterations(self._h, *(v.ctypes.data for v in a)))
         return tuple(a)
 
-    def rk2_graph(self, ybar, dt, budget=12):
+    def rk2_graph(self, ybar, dt, budget=8):
         """One Heun step of every simulation in place on ybar (a contiguous device tensor),
         captured in a hipGraph on the first call and replayed after (eb_sweep_field_rk2_graph).
         The field must have been created with a non-default stream; returns True when the step
