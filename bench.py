@@ -624,17 +624,19 @@ def main():
         log("autotune: start")
         tr = autotune.tune_fmm(x, y, w, d, target=1e-10)
         log(f"autotune: LEVMAX {tr['levmax']}, p {tr['nterms']}")
-        tuned = {"target_rel": 1e-10, "levmax": tr["levmax"], "nterms": tr["nterms"], "apply_ms": tr["ms"],
-                 "err_vs_p60": tr["err"], "tuning_s": round(time.perf_counter() - t0, 3),
-                 "p_scan": tr["p_scan"], "p_scan_levmax": tr["p_scan_levmax"],
+        tuned = {"target_rel": 1e-10, "levmax": tr["levmax"], "nterms": tr["nterms"], "p2p_tile": tr["p2p_tile"],
+                 "apply_ms": tr["ms"], "err_vs_p60": tr["err"], "tuning_s": round(time.perf_counter() - t0, 3),
+                 "p_scan": tr["p_scan"], "p_scan_levmax": tr["p_scan_levmax"], "tile_scan": tr["tile_scan"],
                  "candidates": [{k: c[k] for k in ("levmax", "points_per_leaf", "ms")} for c in tr["candidates"]]}
         if L <= 0:
             L = tr["levmax"]
         if p <= 0:
             p = tr["nterms"]
 
-    log(f"FMM: {n} points, LEVMAX {L}, p {p}")
+    tile = tuned["p2p_tile"] if tuned and (L, p) == (tuned["levmax"], tuned["nterms"]) else 42
+    log(f"FMM: {n} points, LEVMAX {L}, p {p}, near-field tile {tile}")
     plan = eb.FMMPlan(x, y, w, L, p, eb.VARIANT_DR, eb.MODE_FMM, stream=stream)
+    plan.set_p2p_tile(tile)
     for _ in range(args.warmup):
         plan.rebuild(x, y, w)
         plan.apply(d, out)
@@ -747,7 +749,8 @@ def main():
         "config": {"workload": f"FMM alone, {n} points on {ncurves} circles x 256 (r U[0.01,0.05], "
                                f"centres U[0,1]^2)" if ncurves else f"FMM alone, {n} uniform points",
                    "kernel": "CADAPDR GW (DR signs), sources = targets",
-                   "levmax": L, "nterms": p, "leaves_nonempty": info["leaves"], "max_leaf": info["max_leaf"]},
+                   "levmax": L, "nterms": p, "p2p_tile": tile, "leaves_nonempty": info["leaves"],
+                   "max_leaf": info["max_leaf"]},
         "apply_only": {"value": round(n / t_apply / 1e6, 3), "unit": "Mpts/s", "ms": round(t_apply * 1e3, 4)},
         "stage_ms": {k: round(v, 4) for k, v in stage.items()},
         "adaptive_tree": at_info,

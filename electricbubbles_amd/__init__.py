@@ -46,7 +46,7 @@ _i64p = ctypes.POINTER(ctypes.c_int64)
 EXPORTS = (
     "eb_last_error", "eb_version", "eb_cadapdr", "eb_cadapmod", "eb_cadap", "eb_set_mode",
     "eb_fmm_plan_create", "eb_fmm_plan_rebuild", "eb_fmm_apply", "eb_fmm_plan_destroy", "eb_fmm_plan_info",
-    "eb_fmm_plan_ntbox", "eb_fmm_enable_timing", "eb_fmm_stage_ms", "eb_cadap_tables",
+    "eb_fmm_plan_ntbox", "eb_fmm_enable_timing", "eb_fmm_stage_ms", "eb_fmm_set_p2p_tile", "eb_cadap_tables",
     "eb_cadap_lists",
 )
 
@@ -82,6 +82,7 @@ def lib():
     L.eb_fmm_plan_info.argtypes = [_vp, _dp, _i64p]
     L.eb_fmm_plan_ntbox.argtypes = [_vp, _ip]
     L.eb_fmm_enable_timing.argtypes = [_vp, _i32]
+    L.eb_fmm_set_p2p_tile.argtypes = [_vp, _i32]
     L.eb_fmm_stage_ms.argtypes = [_vp, _dp]
     L.eb_cadap_tables.argtypes = [_i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _ip, _ip, _vp]
     L.eb_cadap_lists.argtypes = [_i32, _vp, _vp, _vp, _vp, _vp]
@@ -246,6 +247,11 @@ class FMMPlan:
 
     def enable_timing(self, on=True):
         _check(lib().eb_fmm_enable_timing(self._h, 1 if on else 0))
+
+    def set_p2p_tile(self, tmax):
+        """Near-field work items of at most tmax targets (16..256; default 42)."""
+        _check(lib().eb_fmm_set_p2p_tile(self._h, int(tmax)))
+        self.p2p_tile = int(tmax)
 
     def stage_ms(self):
         a = np.zeros(7)

@@ -7,7 +7,9 @@ from its caller (stokesbubb.f:564).  For a given point set and accuracy target t
   plan at p = 60, the longest the reference allows (cadapdrop.f:129-133);
 * LEVMAX: the depth with the shortest measured apply (tree kept, as inside a GMRES solve)
   among depths whose mean occupancy of the non-empty leaves lies in [lo, hi] points, each
-  checked against the same accuracy target.
+  checked against the same accuracy target;
+* the near-field tile: the number of targets per P2P work item (eb_fmm_set_p2p_tile) with
+  the shortest measured apply at that depth and length.
 
 Every candidate is timed on the device with the real data; results are cached per
 (n, variant, target, geometry fingerprint), in memory and optionally in a JSON file.
@@ -48,8 +50,8 @@ def _rel(a, b):
 
 
 def tune_fmm(x, y, wdz, dens=None, target=1e-10, variant=VARIANT_DR, nterms=(16, 24, 28, 32, 36, 40, 48),
-             occupancy=(4.0, 64.0), reps=3, cache_file=None, seed=0):
-    """Returns {"levmax", "nterms", "ms", "err", "candidates": [...]} for the point set (x, y)
+             occupancy=(4.0, 64.0), tiles=(16, 24, 32, 42, 64, 96, 128), reps=3, cache_file=None, seed=0):
+    """Returns {"levmax", "nterms", "p2p_tile", "ms", "err", "candidates": [...]} for the point set (x, y)
     with weights wdz (device tensors as FMMPlan takes them).  dens: the density to time and
     check with (complex, default: a fixed pseudo-random one)."""
     torch = _torch()
@@ -103,8 +105,17 @@ def tune_fmm(x, y, wdz, dens=None, target=1e-10, variant=VARIANT_DR, nterms=(16,
         cands.append({"levmax": L, "points_per_leaf": round(occ, 2), "ms": round(t * 1e3, 4), "err": err})
     ok = [c for c in cands if c["err"] <= target] or cands
     best = min(ok, key=lambda c: c["ms"])
-    res = {"levmax": best["levmax"], "nterms": p_sel, "ms": best["ms"], "err": best["err"], "target": target,
-           "candidates": cands, "p_scan": p_scan, "p_scan_levmax": Lm}
+    # near-field tile at the chosen depth and length
+    plan = FMMPlan(x, y, wdz, best["levmax"], p_sel, variant=variant, mode=MODE_FMM)
+    tile_scan = []
+    for tm in tiles:
+        plan.set_p2p_tile(tm)
+        t, _ = _time_apply(plan, dens, reps)
+        tile_scan.append({"p2p_tile": tm, "ms": round(t * 1e3, 4)})
+    plan.close()
+    bt = min(tile_scan, key=lambda c: c["ms"])
+    res = {"levmax": best["levmax"], "nterms": p_sel, "p2p_tile": bt["p2p_tile"], "ms": bt["ms"], "err": best["err"],
+           "target": target, "candidates": cands, "p_scan": p_scan, "p_scan_levmax": Lm, "tile_scan": tile_scan}
     _CACHE[key] = res
     if cache_file:
         disk = {}

@@ -674,16 +674,17 @@ __global__ __launch_bounds__(256) void k_l2p(int n, int L, const uint32_t* __res
 }
 
 
-// Near-field work items: a leaf with T points becomes ceil(T / P2P_TMAX) items of nearly
-// equal target ranges (all with the leaf's whole 3x3 neighbourhood as sources), so that the
-// lanes of a block stay busy on large leaves.  item = (Morton key, first target, end target).
-constexpr int P2P_TMAX = 42;
+// Near-field work items: a leaf with T points becomes ceil(T / tmax) items of nearly equal
+// target ranges (all with the leaf's whole 3x3 neighbourhood as sources), so that the lanes of
+// a block stay busy on large leaves.  item = (Morton key, first target, end target).  tmax is
+// the plan's tile (eb_fmm_set_p2p_tile), P2P_TMAX_DEF by default.
+constexpr int P2P_TMAX_DEF = 42, P2P_TMAX_MIN = 16, P2P_TMAX_MAX = 256;
 
-__global__ void k_p2p_count(int nleaf, const int* __restrict__ leaf_start, int* __restrict__ cnt) {
+__global__ void k_p2p_count(int nleaf, const int* __restrict__ leaf_start, int tmax, int* __restrict__ cnt) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= nleaf) return;
   const int T = leaf_start[m + 1] - leaf_start[m];
-  cnt[m] = (T + P2P_TMAX - 1) / P2P_TMAX;
+  cnt[m] = (T + tmax - 1) / tmax;
 }
 
 __global__ void k_p2p_items(int nleaf, const int* __restrict__ leaf_start, const int* __restrict__ off,
@@ -954,6 +955,7 @@ struct eb_fmm_plan_s {
   int* sorttmp = nullptr;
   int* leaf_start = nullptr;
   int4* p2p_items = nullptr;  // near-field work items and their count (after the scan total)
+  int p2p_tmax = P2P_TMAX_DEF;
   int* p2p_cnt = nullptr;
   int* p2p_off = nullptr;
   int* p2p_tmp = nullptr;
@@ -1019,9 +1021,24 @@ static int pick_kmax(int p) {
 
 // Bin the points (tree build): bounding box and RSCAL, scaled coordinates and finest-level
 // keys, stable sort by key, leaf table, gather into sorted order, non-empty counts per level.
-// most near-field work items: one per non-empty leaf plus one per extra P2P_TMAX points
-// (sum over leaves of ceil(T/P2P_TMAX) <= leaves + sum (T-1)/P2P_TMAX <= leaves + n/P2P_TMAX)
-static size_t p2p_item_cap(int n, int L) { return ((size_t)1 << (2 * L)) + (size_t)n / P2P_TMAX + 1; }
+// most near-field work items for tile tmax: one per non-empty leaf plus one per extra tmax
+// points (sum over leaves of ceil(T/tmax) <= leaves + sum (T-1)/tmax <= leaves + n/tmax)
+static size_t p2p_item_cap(int n, int L, int tmax) { return ((size_t)1 << (2 * L)) + (size_t)n / tmax + 1; }
+
+// near-field work items of the current leaves and tile; their count stays on the device
+// (p2p_off[nleaf])
+static int build_p2p_items(eb_fmm_plan P) {
+  hipStream_t st = P->st;
+  const int nleaf = 1 << (2 * P->L);
+  hipLaunchKernelGGL(k_p2p_count, dim3((nleaf + 255) / 256), dim3(256), 0, st, nleaf, P->leaf_start, P->p2p_tmax,
+                     P->p2p_cnt);
+  const int rc = scan_exclusive(P->p2p_cnt, P->p2p_off, nleaf, true, P->p2p_tmp, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_p2p_items, dim3((nleaf + 255) / 256), dim3(256), 0, st, nleaf, P->leaf_start, P->p2p_off,
+                     P->p2p_items);
+  EB_HIP_CHECK(hipGetLastError());
+  return EB_OK;
+}
 
 static int build_tree(eb_fmm_plan P, const double* x, const double* y, const double* wdz) {
   hipStream_t st = P->st;
@@ -1039,12 +1056,7 @@ static int build_tree(eb_fmm_plan P, const double* x, const double* y, const dou
                      P->leaf_start);
   hipLaunchKernelGGL(k_gather, dim3((n + 255) / 256), dim3(256), 0, st, n, P->perm, P->zs0,
                      (const double2*)wdz, P->zs, P->wdzs);
-  // near-field work items; their count stays on the device (p2p_off[nleaf])
-  hipLaunchKernelGGL(k_p2p_count, dim3((nleaf + 255) / 256), dim3(256), 0, st, nleaf, P->leaf_start, P->p2p_cnt);
-  rc = scan_exclusive(P->p2p_cnt, P->p2p_off, nleaf, true, P->p2p_tmp, st);
-  if (rc) return rc;
-  hipLaunchKernelGGL(k_p2p_items, dim3((nleaf + 255) / 256), dim3(256), 0, st, nleaf, P->leaf_start, P->p2p_off,
-                     P->p2p_items);
+  if ((rc = build_p2p_items(P)) != EB_OK) return rc;
   if (P->m2l_list) {
     hipLaunchKernelGGL(k_m2l_count, dim3(4 * P->m2l_nb), dim3(1024), 0, st, L, P->m2l_nb, P->leaf_start,
                        P->m2l_ccnt);
@@ -1109,7 +1121,7 @@ extern "C" int eb_fmm_plan_create(eb_fmm_plan* out, int32_t n, const double* x, 
   EB_TRY(dmalloc(&P->vals2, n));
   EB_TRY(dmalloc(&P->sorttmp, radix_sort_temp_ints(n)));
   EB_TRY(dmalloc(&P->leaf_start, nleaf + 1));
-  EB_TRY(dmalloc(&P->p2p_items, p2p_item_cap(n, L)));
+  EB_TRY(dmalloc(&P->p2p_items, p2p_item_cap(n, L, P2P_TMAX_MIN)));
   EB_TRY(dmalloc(&P->p2p_cnt, nleaf));
   EB_TRY(dmalloc(&P->p2p_off, nleaf + 1));
   EB_TRY(dmalloc(&P->p2p_tmp, scan_temp_ints(nleaf)));
@@ -1283,7 +1295,8 @@ extern "C" int eb_fmm_apply(eb_fmm_plan P, const double* dens, double* gradw, vo
     } else if (P->timing) {
       for (int i = 1; i <= 6; ++i) (void)hipEventRecord(P->ev[i], st);
     }
-    hipLaunchKernelGGL(k_p2p, dim3((unsigned)p2p_item_cap(P->n, P->L)), dim3(64), 0, st, P->L, P->leaf_start,
+    hipLaunchKernelGGL(k_p2p, dim3((unsigned)p2p_item_cap(P->n, P->L, P->p2p_tmax)), dim3(64), 0, st, P->L,
+                       P->leaf_start,
                        P->p2p_items, P->p2p_off + (1 << (2 * P->L)), P->zs, P->qa, P->qb, far ? P->gwf : nullptr,
                        P->perm, P->geom, P->sgn, (double2*)gradw);
   }
@@ -1308,6 +1321,15 @@ extern "C" int eb_fmm_plan_rebuild(eb_fmm_plan P, const double* x, const double*
   }
   if (stream) P->st = (hipStream_t)stream;
   return build_tree(P, x, y, wdz);
+}
+
+extern "C" int eb_fmm_set_p2p_tile(eb_fmm_plan P, int32_t tmax) {
+  if (!P || tmax < P2P_TMAX_MIN || tmax > P2P_TMAX_MAX) {
+    eb_set_error("eb_fmm_set_p2p_tile: tile %d outside %d..%d", tmax, P2P_TMAX_MIN, P2P_TMAX_MAX);
+    return EB_ERR_ARG;
+  }
+  P->p2p_tmax = tmax;
+  return build_p2p_items(P);
 }
 
 extern "C" int eb_fmm_plan_destroy(eb_fmm_plan P) {

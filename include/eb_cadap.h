@@ -127,6 +127,10 @@ int eb_fmm_plan_ntbox(eb_fmm_plan plan, int32_t *ntbox);
 /* Per-stage timing of the last apply on the plan's stream: milliseconds of
  * [charges, P2M, M2M, M2L, L2L, L2P, P2P] measured with HIP events when enabled. */
 int eb_fmm_enable_timing(eb_fmm_plan plan, int32_t on);
+/* Near-field tile: a leaf of T points becomes ceil(T / tmax) work items of the P2P kernel
+ * (tmax in 16..256, 42 by default); the items are rebuilt for the current tree.  A tuning
+ * knob of this build (no counterpart in the reference): results change only by rounding. */
+int eb_fmm_set_p2p_tile(eb_fmm_plan plan, int32_t tmax);
 int eb_fmm_stage_ms(eb_fmm_plan plan, double *ms7);
 
 /* ---------------------------------------------------------------------------------------

@@ -37,3 +37,28 @@ def test_tuned_plan_meets_target(torch):
     print("tuned", res["levmax"], res["nterms"], "err vs direct", err, flush=True)
     assert err < 3e-10
     assert autotune.tune_fmm(x, y, w, d, target=1e-10) is res
+
+
+def test_p2p_tile_changes_only_rounding(torch):
+    """The near-field tile (targets per P2P work item) splits the same pair sums differently."""
+    import electricbubbles_amd as eb
+    from electricbubbles_amd.workloads import complex_normal, curves_cfg, weights
+    rng = np.random.default_rng(8)
+    z, dz, nd, h = curves_cfg(64, 256, rng)
+    print("p2p tile case: curves", z.size, "points", flush=True)
+    x = torch.tensor(z.real.copy(), device="cuda")
+    y = torch.tensor(z.imag.copy(), device="cuda")
+    w = torch.tensor(weights(dz, nd, h), device="cuda")
+    d = torch.tensor(complex_normal(z.size, rng), device="cuda")
+    for L in (4, 6):  # dense leaves (about 64 points) and sparse ones
+        plan = eb.FMMPlan(x, y, w, L, 32)
+        ref = plan.apply(d).clone()
+        for tm in (16, 24, 64, 128, 256):
+            plan.set_p2p_tile(tm)
+            got = plan.apply(d)
+            assert float((got - ref).norm() / ref.norm()) < 1e-14, (L, tm)
+        plan.rebuild(x, y, w)  # the tile survives a rebuild
+        assert float((plan.apply(d) - ref).norm() / ref.norm()) < 1e-14
+        plan.close()
+    with pytest.raises(eb.EBError):
+        eb.FMMPlan(x, y, w, 4, 32).set_p2p_tile(8)
