@@ -122,8 +122,12 @@ __global__ __launch_bounds__(GM_T) void k_gm_dot(int n, const double* __restrict
 // w stays in registers for the whole sweep and each basis vector is read once (v_i is kept
 // for the next pass; with KEEPV false it is read again), so one launch replaces ll + 1
 // launches of the multi-block kernels.  Reductions are fixed-order (wave, then block).
-// Used up to n = 16 * 1024 (beyond that the register arrays spill).
+// Used up to MG1_NMAX unknowns (the register arrays would spill beyond 16 * 1024).
 constexpr int MG1_T = 1024;
+// up to this many unknowns DORTH's modified Gram-Schmidt runs in one workgroup; above it the
+// classical two-pass sweep takes fewer, wider launches (about 36 us per Arnoldi step whatever
+// ll, against about 3.5 us per pass in one workgroup)
+constexpr int MG1_NMAX = 4 * MG1_T;
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -199,6 +203,44 @@ __global__ __launch_bounds__(MG1_T) void k_gm_mgs1(int n, int ll, double* __rest
   }
 }
 
+// Above the one-workgroup limit, one Arnoldi step orthogonalises by classical Gram-Schmidt
+// with one re-orthogonalisation (two passes, each: the ll dots in chunks, their fixed-order
+// sums, the update), eight launches per step instead of ll + 1.
+constexpr int CGS_CH = 2048;
+
+__global__ __launch_bounds__(GM_T) void k_gm_cgs_dot(int n, int nch, const double* __restrict__ V,
+                                                     const double* __restrict__ w, double* __restrict__ part) {
+  __shared__ double s[GM_T];
+  const int c = blockIdx.x, j = blockIdx.y;
+  const double* v = V + (size_t)j * n;
+  const int i0 = c * CGS_CH, i1 = min(n, i0 + CGS_CH);
+  double a = 0.0;
+  for (int i = i0 + (int)threadIdx.x; i < i1; i += GM_T) a += v[i] * w[i];
+  a = block_tree(a, s);
+  if (threadIdx.x == 0) part[(size_t)j * nch + c] = a;
+}
+
+// h_j = sum over chunks (in order); scal[sh + j] = h_j on the first pass, += h_j on the second
+__global__ void k_gm_cgs_sum(int ll, int nch, const double* __restrict__ part, double* __restrict__ scal, int sh,
+                             int second, double* __restrict__ hc) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ll) return;
+  double h = 0.0;
+  for (int c = 0; c < nch; ++c) h += part[(size_t)j * nch + c];
+  hc[j] = h;
+  scal[sh + j] = second ? scal[sh + j] + h : h;
+}
+
+// w -= sum_j hc[j] V(:, j), j in order
+__global__ void k_gm_cgs_update(int n, int ll, const double* __restrict__ V, const double* __restrict__ hc,
+                                double* __restrict__ w) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double t = w[i];
+    for (int j = 0; j < ll; ++j) t -= hc[j] * V[(size_t)j * n + i];
+    w[i] = t;
+  }
+}
+
 // w += c v (c from the host)
 __global__ void k_gm_axpy(int n, double c, const double* __restrict__ v, double* __restrict__ w) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) w[i] += c * v[i];
@@ -244,6 +286,9 @@ struct eb_gmres_s {
   double* coef = nullptr;     // combination coefficients (maxl + 1)
   RedBuf red;
   int nblk = 1;
+  int nch = 0;                // chunks of the classical Gram-Schmidt passes (n > MG1 limit)
+  double* cgs_part = nullptr; // maxl x nch partial dots
+  double* cgs_h = nullptr;    // maxl coefficients of one pass
 };
 
 namespace {
@@ -338,21 +383,21 @@ int dpigmr_cycle(eb_gmres_s* g, eb_apply_fn apply, void* ctx, double* x, double 
     int rc = apply(ctx, col(g, ll - 1), w, st);
     if (rc != EB_OK) return rc;
     // DORTH (dorth.f:90-121): VNRM, then modified Gram-Schmidt against v_1 .. v_ll
-    if (n <= 8 * MG1_T) {
-      hipLaunchKernelGGL((k_gm_mgs1<8, true>), dim3(1), dim3(MG1_T), 0, st, n, ll, g->V, g->scal, S_H, s_vn(g),
-                         s_sn(g));
-    } else if (n <= 16 * MG1_T) {
-      hipLaunchKernelGGL((k_gm_mgs1<16, true>), dim3(1), dim3(MG1_T), 0, st, n, ll, g->V, g->scal, S_H, s_vn(g),
-                         s_sn(g));
+    if (n <= MG1_NMAX) {
+      hipLaunchKernelGGL((k_gm_mgs1<MG1_NMAX / MG1_T, true>), dim3(1), dim3(MG1_T), 0, st, n, ll, g->V, g->scal, S_H,
+                         s_vn(g), s_sn(g));
     } else {
+      // |w| before, two classical passes against v_0 .. v_{ll-1}, |w| after
       hipLaunchKernelGGL(k_gm_norm_dot, dim3(nb), dim3(GM_T), 0, st, n, w, col(g, 0), g->red.part, g->red.count,
-                         g->scal + s_vn(g), g->scal + S_H + 0);
-      for (int i = 1; i <= ll; ++i) {
-        const double* vnext = i < ll ? col(g, i) : nullptr;
-        double* out = i < ll ? g->scal + S_H + i : g->scal + s_sn(g);
-        hipLaunchKernelGGL(k_gm_axpy_dot, dim3(nb), dim3(GM_T), 0, st, n, w, col(g, i - 1), g->scal + S_H + (i - 1),
-                           vnext, g->red.part, g->red.count, out);
+                         g->scal + s_vn(g), g->scal + s_aux(g));
+      for (int pass = 0; pass < 2; ++pass) {
+        hipLaunchKernelGGL(k_gm_cgs_dot, dim3(g->nch, ll), dim3(GM_T), 0, st, n, g->nch, g->V, w, g->cgs_part);
+        hipLaunchKernelGGL(k_gm_cgs_sum, dim3((ll + 63) / 64), dim3(64), 0, st, ll, g->nch, g->cgs_part, g->scal,
+                           S_H, pass, g->cgs_h);
+        hipLaunchKernelGGL(k_gm_cgs_update, dim3(ew_grid(n)), dim3(256), 0, st, n, ll, g->V, g->cgs_h, w);
       }
+      hipLaunchKernelGGL(k_gm_norm_dot, dim3(nb), dim3(GM_T), 0, st, n, w, w, g->red.part, g->red.count,
+                         g->scal + s_sn(g), g->scal + s_aux(g));
     }
     EB_HIP_CHECK(hipGetLastError());
     if ((rc = fetch(g, 0, maxl + 3, h.data())) != EB_OK) return rc;
@@ -455,6 +500,11 @@ extern "C" int eb_gmres_create(eb_gmres* out, int32_t n, int32_t maxl, void* str
             hipMalloc((void**)&g->coef, sizeof(double) * (maxl + 2)) == hipSuccess &&
             hipMalloc((void**)&g->red.part, sizeof(double) * 2 * GM_MAXBLK) == hipSuccess &&
             hipMalloc((void**)&g->red.count, sizeof(unsigned)) == hipSuccess;
+  if (ok && n > MG1_NMAX) {
+    g->nch = (n + CGS_CH - 1) / CGS_CH;
+    ok = hipMalloc((void**)&g->cgs_part, sizeof(double) * (size_t)maxl * g->nch) == hipSuccess &&
+         hipMalloc((void**)&g->cgs_h, sizeof(double) * (size_t)maxl) == hipSuccess;
+  }
   if (!ok || hipMemsetAsync(g->red.count, 0, sizeof(unsigned), g->st) != hipSuccess ||
       hipMemsetAsync(g->scal, 0, sizeof(double) * (maxl + 8), g->st) != hipSuccess) {
     eb_gmres_destroy(g);
@@ -467,7 +517,7 @@ extern "C" int eb_gmres_create(eb_gmres* out, int32_t n, int32_t maxl, void* str
 extern "C" int eb_gmres_destroy(eb_gmres g) {
   if (!g) return EB_OK;
   if (g->st) (void)hipStreamSynchronize(g->st);
-  void* p[] = {g->V, g->r, g->scal, g->coef, g->red.part, g->red.count};
+  void* p[] = {g->V, g->r, g->scal, g->coef, g->red.part, g->red.count, g->cgs_part, g->cgs_h};
   for (void* q : p)
     if (q) (void)hipFree(q);
   delete g;
