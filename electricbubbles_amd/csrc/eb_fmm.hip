@@ -304,6 +304,80 @@ __global__ __launch_bounds__(64) void k_p2m(int L, int p, const int* __restrict_
   }
 }
 
+// value of lane l - 1 (DPP wave_shr:1); lane 0 reads 0
+__device__ __forceinline__ double2 from_lane_below(double2 v) {
+  const int a = __builtin_amdgcn_update_dpp(0, __double2loint(v.x), 0x138, 0xF, 0xF, false);
+  const int b = __builtin_amdgcn_update_dpp(0, __double2hiint(v.x), 0x138, 0xF, 0xF, false);
+  const int c = __builtin_amdgcn_update_dpp(0, __double2loint(v.y), 0x138, 0xF, 0xF, false);
+  const int d = __builtin_amdgcn_update_dpp(0, __double2hiint(v.y), 0x138, 0xF, 0xF, false);
+  return make_double2(__hiloint2double(b, a), __hiloint2double(d, c));
+}
+
+// P2M for KMAX = 48 and 64: the same sums with lane k forming both series' coefficient k, so
+// that one LDS read of t^k serves two outputs and t^(k-1) comes from lane k - 1 through DPP
+// instead of a second read.  Lanes k >= KMAX (KMAX = 48) follow along without writing.
+template <int KMAX>
+__global__ __launch_bounds__(64) void k_p2m_wide(int L, int p, const int* __restrict__ leaf_start,
+                                                 const double2* __restrict__ zs, const double2* __restrict__ qa,
+                                                 const double2* __restrict__ qb, double2* __restrict__ mp, int W,
+                                                 int NBP) {
+  static_assert(KMAX >= 48 && KMAX <= 64, "one lane per exponent");
+  constexpr int CH = 16, QK = KMAX / 4;
+  __shared__ double2 P[CH][KMAX + 1];
+  __shared__ double2 AB[CH][3];  // -(1/r) qa, -(1/r) qb, -(1/r^2) conj(z) qa of each source
+  const int nside = 1 << L;
+  const int ix = blockIdx.x % nside, iy = blockIdx.x / nside;
+  const uint32_t mk = morton((uint32_t)ix, (uint32_t)iy);
+  const int s = leaf_start[mk], e = leaf_start[mk + 1];
+  const double h = 64.0 / nside, ir = 2.0 / h;
+  const double2 c = make_double2((ix + 0.5) * h, (iy + 0.5) * h);
+  const int lane = threadIdx.x;
+  const int js = lane & (CH - 1), qk = lane / CH;
+  const int k = lane < KMAX ? lane : KMAX - 1;
+  const double fk = (double)lane;
+  double2 a0 = make_double2(0, 0), a1 = make_double2(0, 0);
+  for (int c0 = s; c0 < e; c0 += CH) {
+    const int nj = min(CH, e - c0);
+    if (js < nj) {
+      const int j = c0 + js;
+      const double2 z = zs[j];
+      const double2 t = make_double2((z.x - c.x) * ir, (z.y - c.y) * ir);
+      if (qk == 0) {
+        const double2 q1 = qa[j], q2 = qb[j];
+        AB[js][0] = make_double2(-q1.x * ir, -q1.y * ir);
+        AB[js][1] = make_double2(-q2.x * ir, -q2.y * ir);
+        const double2 qc = cmul(make_double2(-z.x, z.y), q1);  // -conj(z) qa
+        AB[js][2] = make_double2(qc.x * ir * ir, qc.y * ir * ir);
+      }
+      double2 pw = make_double2(1.0, 0.0), bb = t;
+      for (int ee = qk * QK; ee > 0; ee >>= 1) {
+        if (ee & 1) pw = cmul(pw, bb);
+        bb = cmul(bb, bb);
+      }
+#pragma unroll
+      for (int kk = 0; kk < QK; ++kk) {
+        P[js][qk * QK + kk] = pw;
+        pw = cmul(pw, t);
+      }
+    }
+    __syncthreads();
+    for (int jj = 0; jj < nj; ++jj) {
+      const double2 pk = P[jj][k];
+      const double2 pkm1 = from_lane_below(pk);  // t^(k-1); lane 0 takes 0 (and fk = 0)
+      cfma(a0, AB[jj][0], pk);
+      cfma(a1, AB[jj][1], pk);
+      rfma(a1, fk, cmul(AB[jj][2], pkm1));
+    }
+    __syncthreads();
+  }
+  if (lane < KMAX) {
+    const int pb = pidx(ix, iy, W);
+    const bool in = lane < p;
+    mp[(size_t)lane * NBP + pb] = in ? a0 : make_double2(0, 0);
+    mp[(size_t)(KMAX + lane) * NBP + pb] = in ? a1 : make_double2(0, 0);
+  }
+}
+
 // ------------------------------------------------------------------------------------------
 // Compacted M2L target lists.  For parity class c: the non-empty boxes of levels
 // 2..L, entry ((l - 2) << 24) | q with q the box's index within the class at level l, in level
@@ -444,17 +518,27 @@ __device__ __forceinline__ double swap_adjacent(double v) {
 // takes entries 16 (b / 4) .. 16 (b / 4) + 15 of parity class b % 4, four per wave: only
 // non-empty target boxes are translated, which matters where most boxes are empty (drops
 // scattered over a large square).  Blocks past the class's count leave at once.
-template <int MODE, int KMAX, int KS, int MT>
-__global__ __launch_bounds__(256) void k_xlat(XlatLevels LS, const int* __restrict__ leaf_start,
-                                              const double2* __restrict__ src, double2* __restrict__ dst,
-                                              const double* __restrict__ Mat, const double2* __restrict__ pre,
-                                              const double2* __restrict__ post, const int* __restrict__ offs,
-                                              const int* __restrict__ m2l_list, const int* __restrict__ m2l_count,
-                                              int m2l_cap) {
-  static_assert(KS * 4 <= KMAX && MT * 16 <= KMAX, "tile exceeds the padded expansion length");
+//
+// Output rows: MT tiles of 16 on the matrix cores, then TR (0, 4 or 8) tail rows on the
+// vector ALUs.  A 16x16x4 FP64 MFMA costs as much as 64 vector FMAs per lane; the tail costs
+// KS + 2 per row and source, so p = 33..40 pays for 32 + TR rows instead of 48.  Rows past
+// 16 MT + TR are not written: they are zero for every p <= 16 MT + TR (the tables vanish
+// there), and no reader goes past row 4 KS - 1 (translations) or p - 1 (k_l2p).
+template <int MODE, int KMAX, int KS, int MT, int TR>
+__device__ __forceinline__ void xlat_body(const XlatLevels& LS, const int* __restrict__ leaf_start,
+                                          const double2* __restrict__ src, double2* __restrict__ dst,
+                                          const double* __restrict__ Mat, const double2* __restrict__ pre,
+                                          const double2* __restrict__ post, const int* __restrict__ offs,
+                                          const int* __restrict__ m2l_list, const int* __restrict__ m2l_count,
+                                          int m2l_cap) {
+  static_assert(KS * 4 <= KMAX && MT * 16 + TR <= KMAX, "tile exceeds the padded expansion length");
+  static_assert(TR % 4 == 0 && TR <= 8 && KS * 4 <= MT * 16 + TR, "tail rows must cover the expansion");
   constexpr int NS = MODE == XL_M2L ? 27 : (MODE == XL_M2M ? 4 : 1);
-  constexpr int NK = KS * 4, NM = MT * 16, CS = NM + 4;
-  __shared__ double Cs[NK * CS];
+  constexpr int NM = MT * 16 + TR, NK = KS * 4;
+  // row stride of the Pascal tile: 2 CS = 32 (mod 64) dwords puts the four k rows that one
+  // MFMA operand read touches on alternate halves of the LDS banks
+  constexpr int CS = TR ? (NM + 16) / 32 * 32 + 16 : NM + 4;
+  __shared__ __align__(16) double Cs[NK * CS];
   __shared__ double2 Ps[NS * NK * 2];  // per (source, k, component): the weights of this lane's column
   __shared__ double2 Qs[NS * NM];
   // XCD-aware order: consecutive logical blocks (neighbouring rows, all four parity classes)
@@ -556,6 +640,21 @@ __global__ __launch_bounds__(256) void k_xlat(XlatLevels LS, const int* __restri
   for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
     for (int r = 0; r < 4; ++r) sa[mt][r] = sb[mt][r] = 0.0;
+  // tail rows: this lane's partial sums over its k rows (kr + 4 ks), reduced over the four
+  // k-row lanes of its column after the last source
+  double ta[TR ? TR : 1], tb[TR ? TR : 1];
+#pragma unroll
+  for (int r = 0; r < TR; ++r) ta[r] = tb[r] = 0.0;
+  // the tail's Pascal entries of this lane's k rows, the same for every source: held in
+  // registers for a 4-row tail, read from LDS per source for an 8-row one
+  constexpr bool CT_REG = TR == 4;
+  double ct[CT_REG ? KS : 1][CT_REG ? TR : 1];
+  if constexpr (CT_REG) {
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+      for (int r = 0; r < TR; ++r) ct[ks][r] = Cs[(4 * ks + kr) * CS + 16 * MT + r];
+  }
   // One source: prescale the loaded expansion (cur), start loading the next source into the
   // other register set (nxt; in flight during the products), the Pascal product on the
   // matrix cores, then the post-scaling.  The empty asm statement is a compiler barrier for
@@ -579,6 +678,29 @@ __global__ __launch_bounds__(256) void k_xlat(XlatLevels LS, const int* __restri
       for (int mt = 0; mt < MT; ++mt)
         acc[mt] = __builtin_amdgcn_mfma_f64_16x16x4f64(Cs[(4 * ks + kr) * CS + 16 * mt + col], xb[ks], acc[mt],
                                                        0, 0, 0);
+    if constexpr (TR > 0) {
+      double part[TR];
+#pragma unroll
+      for (int r = 0; r < TR; ++r) part[r] = 0.0;
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+        for (int r = 0; r < TR; r += 2) {
+          double2 cc;
+          if constexpr (CT_REG)
+            cc = make_double2(ct[ks][r], ct[ks][r + 1]);
+          else
+            cc = *(const double2*)&Cs[(4 * ks + kr) * CS + 16 * MT + r];
+          part[r] = fma(cc.x, xb[ks], part[r]);
+          part[r + 1] = fma(cc.y, xb[ks], part[r + 1]);
+        }
+#pragma unroll
+      for (int r = 0; r < TR; ++r) {
+        const double2 w = Qs[o * NM + 16 * MT + r];
+        ta[r] = fma(w.x, part[r], ta[r]);
+        tb[r] = fma(w.y, part[r], tb[r]);
+      }
+    }
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
@@ -606,6 +728,28 @@ __global__ __launch_bounds__(256) void k_xlat(XlatLevels LS, const int* __restri
       const double bo = swap_adjacent(sb[mt][r]);
       lam[mt][r] = sa[mt][r] + __hiloint2double(__double2hiint(bo) ^ (int)sgn_hi, __double2loint(bo));
     }
+  // tail: sum over the k-row lanes (lanes col + 16 kr), then lane kr keeps rows kr + 4 r'
+  double tlam[TR ? TR / 4 : 1];
+  if constexpr (TR > 0) {
+#pragma unroll
+    for (int r = 0; r < TR; ++r) {
+      ta[r] += __shfl_xor(ta[r], 16, 64);
+      tb[r] += __shfl_xor(tb[r], 16, 64);
+      ta[r] += __shfl_xor(ta[r], 32, 64);
+      tb[r] += __shfl_xor(tb[r], 32, 64);
+    }
+#pragma unroll
+    for (int q = 0; q < TR / 4; ++q) {
+      double a = ta[4 * q], bq = tb[4 * q];
+#pragma unroll
+      for (int j = 1; j < 4; ++j) {
+        a = kr == j ? ta[4 * q + j] : a;
+        bq = kr == j ? tb[4 * q + j] : bq;
+      }
+      const double bo = swap_adjacent(bq);
+      tlam[q] = a + __hiloint2double(__double2hiint(bo) ^ (int)sgn_hi, __double2loint(bo));
+    }
+  }
   if (live) {
     double* out = (double*)(dst + LS.offt[e]);
 #pragma unroll
@@ -616,8 +760,31 @@ __global__ __launch_bounds__(256) void k_xlat(XlatLevels LS, const int* __restri
         const size_t at = 2 * ((size_t)(s * KMAX + m) * NBPt + pt) + comp;
         out[at] = MODE == XL_L2L ? out[at] + lam[mt][r] : lam[mt][r];
       }
+#pragma unroll
+    for (int q = 0; q < TR / 4; ++q) {
+      const int m = 16 * MT + kr + 4 * q;
+      const size_t at = 2 * ((size_t)(s * KMAX + m) * NBPt + pt) + comp;
+      out[at] = MODE == XL_L2L ? out[at] + tlam[q] : tlam[q];
+    }
   }
 }
+
+#define EB_XLAT_ARGS                                                                                     \
+  XlatLevels LS, const int* __restrict__ leaf_start, const double2* __restrict__ src,                  \
+      double2* __restrict__ dst, const double* __restrict__ Mat, const double2* __restrict__ pre,      \
+      const double2* __restrict__ post, const int* __restrict__ offs, const int* __restrict__ m2l_list, \
+      const int* __restrict__ m2l_count, int m2l_cap
+template <int MODE, int KMAX, int KS, int MT, int TR>
+__global__ __launch_bounds__(256) void k_xlat(EB_XLAT_ARGS) {
+  xlat_body<MODE, KMAX, KS, MT, TR>(LS, leaf_start, src, dst, Mat, pre, post, offs, m2l_list, m2l_count, m2l_cap);
+}
+// The M2L with a vector tail: its LDS admits two workgroups per CU, so the registers are held
+// to what two waves per SIMD allow.
+template <int MODE, int KMAX, int KS, int MT, int TR>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_xlat2(EB_XLAT_ARGS) {
+  xlat_body<MODE, KMAX, KS, MT, TR>(LS, leaf_start, src, dst, Mat, pre, post, offs, m2l_list, m2l_count, m2l_cap);
+}
+#undef EB_XLAT_ARGS
 
 // L2P at the finest level (SLLVALDR, cadapdrop2.f:331-348): one thread per point.
 //   GW_far = phi + sgn (z conj(phi') + conj(psi))
@@ -626,7 +793,7 @@ __global__ __launch_bounds__(256) void k_xlat(XlatLevels LS, const int* __restri
 // level-L L2L followed by one evaluation (the L2L shift is exact), without writing the
 // shifted expansions of every leaf.
 template <int KMAX, bool PARENT>
-__global__ __launch_bounds__(256) void k_l2p(int n, int L, const uint32_t* __restrict__ keys,
+__global__ __launch_bounds__(256) void k_l2p(int n, int L, int p, const uint32_t* __restrict__ keys,
                                              const double2* __restrict__ zs, const double2* __restrict__ lo, int W,
                                              int NBP, const double2* __restrict__ lop, int Wp, int NBPp, double sgn,
                                              double2* __restrict__ gwf) {
@@ -640,8 +807,9 @@ __global__ __launch_bounds__(256) void k_l2p(int n, int L, const uint32_t* __res
   const double2 u = make_double2((z.x - (ix + 0.5) * h) * ir, (z.y - (iy + 0.5) * h) * ir);
   const int pb = pidx(ix, iy, W);
   double2 phi = make_double2(0, 0), dphi = make_double2(0, 0), psi = make_double2(0, 0);
+  // rows p..KMAX-1 of a local expansion are zero (and not written by every translation)
 #pragma unroll 4
-  for (int m = KMAX - 1; m >= 0; --m) {
+  for (int m = p - 1; m >= 0; --m) {
     const double2 l1 = lo[(size_t)m * NBP + pb];
     const double2 l2 = lo[(size_t)(KMAX + m) * NBP + pb];
     dphi = cadd(cmul(dphi, u), phi);
@@ -656,7 +824,7 @@ __global__ __launch_bounds__(256) void k_l2p(int n, int L, const uint32_t* __res
     const int pp = pidx(px, py, Wp);
     double2 f = make_double2(0, 0), df = make_double2(0, 0), g = make_double2(0, 0);
 #pragma unroll 4
-    for (int m = KMAX - 1; m >= 0; --m) {
+    for (int m = p - 1; m >= 0; --m) {
       const double2 l1 = lop[(size_t)m * NBPp + pp];
       const double2 l2 = lop[(size_t)(KMAX + m) * NBPp + pp];
       df = cadd(cmul(df, up), f);
@@ -1212,12 +1380,18 @@ static XlatLevels xlat_levels(eb_fmm_plan P, int l0, int l1, int dl) {
 template <int MODE, int K, int KS>
 static void launch_xlat_ks(eb_fmm_plan P, const XlatLevels& LS, const double2* src, double2* dst,
                            const double2* pre, const double2* post) {
-  constexpr int MT = K / 16;
+  // p = 33..40 (K = 48): two MFMA tiles and a tail of 4 or 8 rows on the vector ALUs
+  constexpr int TR = (K == 48 && KS == 9) ? 4 : ((K == 48 && KS == 10) ? 8 : 0);
+  constexpr int MT = TR ? 2 : K / 16;
   const int grid = (LS.total + 7) / 8 * 8;
   const double* mat = P->dMat + (size_t)MODE * K * K;
   const bool c = MODE == XL_M2L && P->m2l_list != nullptr;
-  hipLaunchKernelGGL((k_xlat<MODE, K, KS, MT>), dim3(grid), dim3(256), 0, P->st, LS, P->leaf_start, src, dst, mat,
-                     pre, post, P->doffs, c ? P->m2l_list : nullptr, c ? P->m2l_count : nullptr, P->m2l_cap);
+  if constexpr (MODE == XL_M2L && TR == 4)
+    hipLaunchKernelGGL((k_xlat2<MODE, K, KS, MT, TR>), dim3(grid), dim3(256), 0, P->st, LS, P->leaf_start, src, dst,
+                       mat, pre, post, P->doffs, c ? P->m2l_list : nullptr, c ? P->m2l_count : nullptr, P->m2l_cap);
+  else
+    hipLaunchKernelGGL((k_xlat<MODE, K, KS, MT, TR>), dim3(grid), dim3(256), 0, P->st, LS, P->leaf_start, src, dst,
+                       mat, pre, post, P->doffs, c ? P->m2l_list : nullptr, c ? P->m2l_count : nullptr, P->m2l_cap);
 }
 
 // (KS, MT) = (ceil(p/4), K/16): the four k-step counts that round up to K
@@ -1238,7 +1412,11 @@ static void launch_far(eb_fmm_plan P) {
   const int L = P->L;
   if (P->timing) (void)hipEventRecord(P->ev[1], st);
   // P2M at the finest level
-  hipLaunchKernelGGL(k_p2m<K>, dim3(1 << (2 * L)), dim3(64), 0, st, L, P->p, P->leaf_start, P->zs, P->qa, P->qb,
+  if constexpr (K >= 48)
+    hipLaunchKernelGGL(k_p2m_wide<K>, dim3(1 << (2 * L)), dim3(64), 0, st, L, P->p, P->leaf_start, P->zs, P->qa,
+                       P->qb, P->mp + P->lvl_off[L], P->lvl_W[L], P->lvl_NBP[L]);
+  else
+    hipLaunchKernelGGL(k_p2m<K>, dim3(1 << (2 * L)), dim3(64), 0, st, L, P->p, P->leaf_start, P->zs, P->qa, P->qb,
                      P->mp + P->lvl_off[L], P->lvl_W[L], P->lvl_NBP[L]);
   if (P->timing) (void)hipEventRecord(P->ev[2], st);
   // upward pass, level by level
@@ -1253,11 +1431,13 @@ static void launch_far(eb_fmm_plan P) {
     launch_xlat<XL_L2L, K>(P, xlat_levels(P, l, l, -1), P->lo, P->lo, P->dPL, P->dQL);
   if (P->timing) (void)hipEventRecord(P->ev[5], st);
   if (L >= 3)
-    hipLaunchKernelGGL((k_l2p<K, true>), dim3(xcd_grid((P->n + 255) / 256)), dim3(256), 0, st, P->n, L, P->keys,
+    hipLaunchKernelGGL((k_l2p<K, true>), dim3(xcd_grid((P->n + 255) / 256)), dim3(256), 0, st, P->n, L, P->p,
+                       P->keys,
                        P->zs, P->lo + P->lvl_off[L], P->lvl_W[L], P->lvl_NBP[L], P->lo + P->lvl_off[L - 1],
                        P->lvl_W[L - 1], P->lvl_NBP[L - 1], P->sgn, P->gwf);
   else
-    hipLaunchKernelGGL((k_l2p<K, false>), dim3(xcd_grid((P->n + 255) / 256)), dim3(256), 0, st, P->n, L, P->keys,
+    hipLaunchKernelGGL((k_l2p<K, false>), dim3(xcd_grid((P->n + 255) / 256)), dim3(256), 0, st, P->n, L, P->p,
+                       P->keys,
                        P->zs, P->lo + P->lvl_off[L], P->lvl_W[L], P->lvl_NBP[L], P->lo + P->lvl_off[L],
                        P->lvl_W[L], P->lvl_NBP[L], P->sgn, P->gwf);
   if (P->timing) (void)hipEventRecord(P->ev[6], st);
