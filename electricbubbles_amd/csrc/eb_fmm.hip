@@ -322,7 +322,9 @@ __global__ __launch_bounds__(64) void k_p2m_wide(int L, int p, const int* __rest
                                                  const double2* __restrict__ qb, double2* __restrict__ mp, int W,
                                                  int NBP) {
   static_assert(KMAX >= 48 && KMAX <= 64, "one lane per exponent");
-  constexpr int CH = 16, QK = KMAX / 4;
+  // chunks of 8 sources, eight lanes per source filling the powers: 7 KB of LDS per block,
+  // so that six waves per SIMD fit (16-source chunks held them to three)
+  constexpr int CH = 8, QK = KMAX / 8;
   __shared__ double2 P[CH][KMAX + 1];
   __shared__ double2 AB[CH][3];  // -(1/r) qa, -(1/r) qb, -(1/r^2) conj(z) qa of each source
   const int nside = 1 << L;
@@ -871,7 +873,7 @@ __global__ void k_p2p_items(int nleaf, const int* __restrict__ leaf_start, const
 // the lanes split into G = 64/T source slices per target, summed through LDS at the end.
 constexpr int P2P_CH = 128;  // 6 KB of LDS per block: LDS does not limit the waves per CU
 
-__global__ __launch_bounds__(64) void k_p2p(int L, const int* __restrict__ leaf_start,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_p2p(int L, const int* __restrict__ leaf_start,
                                             const int4* __restrict__ items, const int* __restrict__ nitems,
                                             const double2* __restrict__ zs, const double2* __restrict__ qa,
                                             const double2* __restrict__ qb, const double2* __restrict__ gwf,
