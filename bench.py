@@ -37,8 +37,8 @@ PEAK_HBM_GBS = 8000.0
 # write x 1.000); profiles/fmm_curves_n1048576_L8_p{32,36}_hbm_traffic.md have the commands
 # and the per-kernel tables.  Keyed by (workload, points, levmax, nterms).
 PROFILED_TRAFFIC_BYTES = {
-    ("curves", 1 << 20, 8, 32): {"m2l": 221885419, "p2p": 417531852},
-    ("curves", 1 << 20, 8, 36): {"m2l": 294264137, "p2p": 417535436},
+    ("curves", 1 << 20, 8, 32): {"m2l": 223790708, "p2p": 421251112},
+    ("curves", 1 << 20, 8, 36): {"m2l": 242718987, "p2p": 421227921},
 }
 
 
