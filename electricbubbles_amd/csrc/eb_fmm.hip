@@ -523,7 +523,7 @@ __device__ __forceinline__ double swap_adjacent(double v) {
 //
 // Output rows: MT tiles of 16 on the matrix cores, then TR (0, 4 or 8) tail rows on the
 // vector ALUs.  A 16x16x4 FP64 MFMA costs as much as 64 vector FMAs per lane; the tail costs
-// KS + 2 per row and source, so p = 33..40 pays for 32 + TR rows instead of 48.  Rows past
+// KS + 2 per row and source, so p = 33..36 pays for 36 rows instead of 48.  Rows past
 // 16 MT + TR are not written: they are zero for every p <= 16 MT + TR (the tables vanish
 // there), and no reader goes past row 4 KS - 1 (translations) or p - 1 (k_l2p).
 template <int MODE, int KMAX, int KS, int MT, int TR>
@@ -1382,8 +1382,10 @@ static XlatLevels xlat_levels(eb_fmm_plan P, int l0, int l1, int dl) {
 template <int MODE, int K, int KS>
 static void launch_xlat_ks(eb_fmm_plan P, const XlatLevels& LS, const double2* src, double2* dst,
                            const double2* pre, const double2* post) {
-  // p = 33..40 (K = 48): two MFMA tiles and a tail of 4 or 8 rows on the vector ALUs
-  constexpr int TR = (K == 48 && KS == 9) ? 4 : ((K == 48 && KS == 10) ? 8 : 0);
+  // p = 33..36 (K = 48): two MFMA tiles and a 4-row tail on the vector ALUs.  An 8-row tail
+  // for p = 37..40 reads its Pascal entries from LDS per source (registers do not hold
+  // them), and measured slower than three tiles: 0.84 against 0.68 ms at p = 37.
+  constexpr int TR = (K == 48 && KS == 9) ? 4 : 0;
   constexpr int MT = TR ? 2 : K / 16;
   const int grid = (LS.total + 7) / 8 * 8;
   const double* mat = P->dMat + (size_t)MODE * K * K;
