@@ -95,20 +95,6 @@ __global__ __launch_bounds__(GM_T) void k_gm_norm_dot(int n, const double* __res
   finish2(s0, s1, part, count, out_norm2, out_dot);
 }
 
-// w -= h v_i (h read from device memory); then vnext . w, or |w|^2 when vnext is NULL
-__global__ __launch_bounds__(GM_T) void k_gm_axpy_dot(int n, double* __restrict__ w, const double* __restrict__ vi,
-                                                      const double* __restrict__ h, const double* __restrict__ vnext,
-                                                      double* part, unsigned* count, double* out) {
-  const double c = -*h;
-  double s = 0.0;
-  for (int i = blockIdx.x * GM_T + threadIdx.x; i < n; i += gridDim.x * GM_T) {
-    const double wi = w[i] + c * vi[i];
-    w[i] = wi;
-    s += vnext ? vnext[i] * wi : wi * wi;
-  }
-  finish2(s, 0.0, part, count, out, nullptr);
-}
-
 // v . w
 __global__ __launch_bounds__(GM_T) void k_gm_dot(int n, const double* __restrict__ v, const double* __restrict__ w,
                                                  double* part, unsigned* count, double* out) {
