@@ -21,7 +21,7 @@ extern "C" int eb_fmm_plan_ntbox(eb_fmm_plan P, int32_t* ntbox);
 
 namespace {
 
-constexpr int kDirectMax = 2048;  // EB_MODE_AUTO: O(N^2) sum at or below this many points
+constexpr int kDirectMax = EB_DIRECT_MAX;  // EB_MODE_AUTO: O(N^2) sum at or below this many points
 
 struct Ctx {
   std::mutex mu;

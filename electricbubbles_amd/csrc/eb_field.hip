@@ -402,8 +402,8 @@ extern "C" int eb_field_destroy(eb_field F) {
   return EB_OK;
 }
 
-// EB_MODE_AUTO's rule: the O(N^2) sum up to 2048 points, the FMM with LEVMAX 7 above
-static int plan_mode(int n) { return n <= 2048 ? EB_MODE_DIRECT : EB_MODE_FMM; }
+// EB_MODE_AUTO's rule: the O(N^2) sum up to EB_DIRECT_MAX points, the FMM with LEVMAX 7 above
+static int plan_mode(int n) { return n <= EB_DIRECT_MAX ? EB_MODE_DIRECT : EB_MODE_FMM; }
 
 extern "C" int eb_field_set_geometry(eb_field F, const double* rku) {
   if (!F || !rku) return (eb_set_error("eb_field_set_geometry: bad arguments"), EB_ERR_ARG);

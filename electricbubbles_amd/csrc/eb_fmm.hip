@@ -982,33 +982,67 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
   }
 }
 
-// O(N^2) direct evaluation of the same kernel (small N), sources staged through LDS.
-__global__ __launch_bounds__(256) void k_direct(int n, const double2* __restrict__ zs, const double2* __restrict__ qa,
-                                                const double2* __restrict__ qb, const int32_t* __restrict__ perm,
-                                                const Geom* __restrict__ g, double sgn, double2* __restrict__ out) {
-  __shared__ double2 sz[256], sa[256], sbq[256];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// O(N^2) direct evaluation of the same kernel (small N).  Grid (target tiles of 64, source
+// chunks): each one-wave block sums one chunk of sources for 64 targets, staged through LDS 64
+// at a time, and writes its partial (qa/d and the bracket, separately); k_direct_sum adds the
+// chunks in a fixed order.  A few thousand points thus fill the chip instead of n / 256
+// workgroups of the one-pass form.  The reciprocal keeps two Newton steps.
+constexpr int DIR_T = 64;
+constexpr int DIR_MAXBLK = 4096;  // target tiles x chunks at most (fewer, longer chunks above)
+
+__global__ __launch_bounds__(DIR_T) void k_direct_part(int n, int sch, const double2* __restrict__ zs,
+                                                       const double2* __restrict__ qa, const double2* __restrict__ qb,
+                                                       double2* __restrict__ part) {
+  __shared__ double2 sz[DIR_T], sa[DIR_T], sbq[DIR_T];
+  const int i = blockIdx.x * DIR_T + threadIdx.x;
+  const int c = blockIdx.y;
+  const int j0 = c * sch, j1 = min(n, j0 + sch);
   const double2 zi = i < n ? zs[i] : make_double2(0, 0);
   double2 acc1 = make_double2(0, 0), acc2 = make_double2(0, 0);
-  for (int t0 = 0; t0 < n; t0 += 256) {
+  for (int t0 = j0; t0 < j1; t0 += DIR_T) {
     const int j = t0 + threadIdx.x;
-    if (j < n) {
+    if (j < j1) {
       sz[threadIdx.x] = zs[j];
       sa[threadIdx.x] = qa[j];
       sbq[threadIdx.x] = qb[j];
     }
     __syncthreads();
-    const int m = min(256, n - t0);
-    if (i < n)
-      for (int jj = 0; jj < m; ++jj) {
-        p2p_pair<true, false>(zi, sz[jj], sa[jj], sbq[jj], t0 + jj == i, acc1, acc2);
+    const int m = min(DIR_T, j1 - t0);
+    if (i < n) {
+      if (i >= t0 && i < t0 + m) {  // this tile holds the target itself
+        for (int jj = 0; jj < m; ++jj) p2p_pair<true, false>(zi, sz[jj], sa[jj], sbq[jj], t0 + jj == i, acc1, acc2);
+      } else {
+        for (int jj = 0; jj < m; ++jj) p2p_pair<false, false>(zi, sz[jj], sa[jj], sbq[jj], false, acc1, acc2);
       }
+    }
     __syncthreads();
   }
   if (i < n) {
-    const double rs = g->rscal;
-    out[perm[i]] = make_double2((acc1.x + sgn * acc2.x) * rs, (acc1.y + sgn * acc2.y) * rs);
+    part[2 * ((size_t)c * n + i)] = acc1;
+    part[2 * ((size_t)c * n + i) + 1] = acc2;
   }
+}
+
+__global__ __launch_bounds__(256) void k_direct_sum(int n, int nch, const double2* __restrict__ part,
+                                                    const int32_t* __restrict__ perm, const Geom* __restrict__ g,
+                                                    double sgn, double2* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double2 acc1 = make_double2(0, 0), acc2 = make_double2(0, 0);
+  for (int c = 0; c < nch; ++c) {
+    acc1 = cadd(acc1, part[2 * ((size_t)c * n + i)]);
+    acc2 = cadd(acc2, part[2 * ((size_t)c * n + i) + 1]);
+  }
+  const double rs = g->rscal;
+  out[perm[i]] = make_double2((acc1.x + sgn * acc2.x) * rs, (acc1.y + sgn * acc2.y) * rs);
+}
+
+// chunk length (a multiple of DIR_T) and count for n points: about DIR_MAXBLK blocks in all
+static void direct_chunks(int n, int& sch, int& nch) {
+  const int tiles = (n + DIR_T - 1) / DIR_T;
+  int want = std::max(1, std::min(tiles, DIR_MAXBLK / tiles));
+  sch = ((n + want - 1) / want + DIR_T - 1) / DIR_T * DIR_T;
+  nch = (n + sch - 1) / sch;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1130,6 +1164,8 @@ struct eb_fmm_plan_s {
   int* p2p_off = nullptr;
   int* p2p_tmp = nullptr;
   int* ntbox = nullptr;
+  double2* dir_part = nullptr;  // direct mode: 2 x n partial sums per source chunk
+  int dir_sch = 0, dir_nch = 0;
   int* m2l_list = nullptr;   // compacted M2L targets (k_m2l_count, k_m2l_write), 4 x m2l_cap
   int* m2l_count = nullptr;  // 4
   int* m2l_ccnt = nullptr;   // per (class, chunk of 1024 boxes) counts, 4 x m2l_nb
@@ -1153,7 +1189,7 @@ static void plan_free(eb_fmm_plan P) {
   void* ptrs[] = {P->geom, P->bbpart, P->zs0, P->zs, P->wdzs, P->qa, P->qb, P->gwf, P->keys, P->keys2,
                   P->perm, P->vals2, P->sorttmp, P->leaf_start, P->p2p_items, P->p2p_cnt, P->p2p_off,
                   P->p2p_tmp, P->ntbox, P->m2l_list, P->m2l_count, P->m2l_ccnt, P->mp, P->lo, P->dMat,
-                  P->dU, P->dNU, P->dPM, P->dQM, P->dPL, P->dQL, P->doffs};
+                  P->dU, P->dNU, P->dPM, P->dQM, P->dPL, P->dQL, P->doffs, P->dir_part};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   for (auto& e : P->ev)
@@ -1296,6 +1332,10 @@ extern "C" int eb_fmm_plan_create(eb_fmm_plan* out, int32_t n, const double* x, 
   EB_TRY(dmalloc(&P->p2p_off, nleaf + 1));
   EB_TRY(dmalloc(&P->p2p_tmp, scan_temp_ints(nleaf)));
   EB_TRY(dmalloc(&P->ntbox, EB_MAXLEV + 2));
+  if (mode == EB_MODE_DIRECT) {
+    direct_chunks(n, P->dir_sch, P->dir_nch);
+    EB_TRY(dmalloc(&P->dir_part, (size_t)2 * P->dir_nch * n));
+  }
   if (mode == EB_MODE_FMM && L >= 2) {
     size_t cap = 0;
     for (int l = 2; l <= L; ++l) cap += ((size_t)1 << (2 * (l - 1))) + 3;
@@ -1465,8 +1505,10 @@ extern "C" int eb_fmm_apply(eb_fmm_plan P, const double* dens, double* gradw, vo
   if (P->mode == EB_MODE_DIRECT) {
     if (P->timing)
       for (int i = 1; i <= 6; ++i) (void)hipEventRecord(P->ev[i], st);
-    hipLaunchKernelGGL(k_direct, dim3((n + 255) / 256), dim3(256), 0, st, n, P->zs, P->qa, P->qb, P->perm, P->geom,
-                       P->sgn, (double2*)gradw);
+    hipLaunchKernelGGL(k_direct_part, dim3((n + DIR_T - 1) / DIR_T, P->dir_nch), dim3(DIR_T), 0, st, n, P->dir_sch,
+                       P->zs, P->qa, P->qb, P->dir_part);
+    hipLaunchKernelGGL(k_direct_sum, dim3((n + 255) / 256), dim3(256), 0, st, n, P->dir_nch, P->dir_part, P->perm,
+                       P->geom, P->sgn, (double2*)gradw);
   } else {
     const bool far = P->L >= 2;
     if (far) {

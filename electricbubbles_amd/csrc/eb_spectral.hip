@@ -567,7 +567,7 @@ extern "C" int eb_matvec_create(eb_matvec* out, eb_bodies B, const double* x, co
     free_mv(M);
     return (eb_set_error("eb_matvec_create: kernel failed"), EB_ERR_HIP);
   }
-  if (mode == EB_MODE_AUTO) mode = n <= 2048 ? EB_MODE_DIRECT : EB_MODE_FMM;  // as the CADAPDR drop-in
+  if (mode == EB_MODE_AUTO) mode = n <= EB_DIRECT_MAX ? EB_MODE_DIRECT : EB_MODE_FMM;  // as the CADAPDR drop-in
   rc = eb_fmm_plan_create(&M->fmm, n, M->x, M->y, (const double*)M->wdz, levmax, nterms, EB_VARIANT_DR, mode, stream);
   if (rc != EB_OK) {
     M->fmm = nullptr;
@@ -952,7 +952,7 @@ extern "C" int32_t eb_fasmvp(const int32_t* nd, const int32_t* nbk, const int32_
     if (hipMemcpyAsync(c.dgeo, key.data(), sizeof(double) * key.size(), hipMemcpyHostToDevice, c.st) != hipSuccess)
       return (eb_set_error("eb_fasmvp: copy failed"), EB_ERR_HIP);
     double* g = c.dgeo;
-    const int mode = N <= 2048 ? EB_MODE_DIRECT : EB_MODE_FMM;
+    const int mode = N <= EB_DIRECT_MAX ? EB_MODE_DIRECT : EB_MODE_FMM;
     rc = eb_matvec_create(&c.M, c.B, g, g + N, g + 2 * N, g + 4 * N, g + 5 * N, g + 6 * N, 7, 43, mode, c.st);
     if (rc != EB_OK) {
       c.M = nullptr;

@@ -47,9 +47,13 @@ extern "C" {
 #define EB_VARIANT_HILBERT 3
 
 /* Evaluation modes of the drop-in layer. */
-#define EB_MODE_AUTO   0   /* direct O(N^2) sum below EB_DIRECT_MAX points, FMM above */
+#define EB_MODE_AUTO   0   /* direct O(N^2) sum up to EB_DIRECT_MAX points, FMM above */
 #define EB_MODE_FMM    1
 #define EB_MODE_DIRECT 2
+/* Below about 10^4 points one FMM apply costs its launch latency (0.17-0.21 ms on one MI355X
+ * at LEVMAX 5..7, p = 43) while the chunked O(N^2) sum costs 0.024 ms at 2048 points and
+ * 0.093 ms at 8192; they cross near 12,000. */
+#define EB_DIRECT_MAX  10240
 
 const char *eb_last_error(void);
 int eb_version(void);

@@ -65,7 +65,7 @@ int eb_rk2_stage2(int32_t neq, double dt, const double *phi0, const double *dy, 
  * (stokesbubb.f:496-614) with the layer potential from the GPU FMM (eb_fmm_plan).
  * Geometry (x, y, dz = DZ, dsdth = |z_alpha|, rkappa, zk per body) is bound at creation;
  * each apply maps the interleaved density xx (2*nbk) to yy (2*nbk).  h_k = 2 pi / nd_k.
- * mode: EB_MODE_FMM, EB_MODE_DIRECT, or EB_MODE_AUTO (direct sum up to 2048 points).
+ * mode: EB_MODE_FMM, EB_MODE_DIRECT, or EB_MODE_AUTO (direct sum up to EB_DIRECT_MAX points).
  * --------------------------------------------------------------------------------------- */
 typedef struct eb_matvec_s *eb_matvec;
 
@@ -87,7 +87,7 @@ int eb_matvec_destroy(eb_matvec mv);
  * bodies [sim_off[s], sim_off[s+1]) of `bodies` (sim_off: nsim + 1 host ints, sim_off[0] = 0,
  * sim_off[nsim] = the body count).  eb_sweep_apply is MATVEC + FASMVP (stokesbubb.f:419-614)
  * of every simulation at once: each simulation's layer potential sums over its own points
- * only (the O(N^2) sum that EB_MODE_AUTO also picks for up to 2048 points) and its pressure
+ * only (the O(N^2) sum that EB_MODE_AUTO also picks for small simulations) and its pressure
  * term uses its own first body's pk (pinf = -pk(1), stokesbubb.f:546).  Geometry as for
  * eb_matvec_create, all simulations back to back.  Replaces the reference's one process per
  * simulation; it has no Fortran counterpart.
