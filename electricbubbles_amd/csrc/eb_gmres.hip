@@ -190,15 +190,17 @@ __global__ __launch_bounds__(MG1_T) void k_gm_mgs1(int n, int ll, double* __rest
 }
 
 // Above the one-workgroup limit, one Arnoldi step orthogonalises by classical Gram-Schmidt
-// with one re-orthogonalisation (two passes, each: the ll dots in chunks, their fixed-order
-// sums, the update), eight launches per step instead of ll + 1.
+// with one re-orthogonalisation: two passes of (the ll dots in chunks; their fixed-order sums
+// and the update), four launches per step instead of ll + 1.  The first pass's dots also
+// give |w|^2 before, the second pass's update |w|^2 after.
 constexpr int CGS_CH = 2048;
 
-__global__ __launch_bounds__(GM_T) void k_gm_cgs_dot(int n, int nch, const double* __restrict__ V,
+__global__ __launch_bounds__(GM_T) void k_gm_cgs_dot(int n, int nch, int ll, const double* __restrict__ V,
                                                      const double* __restrict__ w, double* __restrict__ part) {
   __shared__ double s[GM_T];
   const int c = blockIdx.x, j = blockIdx.y;
-  const double* v = V + (size_t)j * n;
+  // grid row j < ll: v_j . w; row ll (launched on the first pass only): |w|^2
+  const double* v = j < ll ? V + (size_t)j * n : w;
   const int i0 = c * CGS_CH, i1 = min(n, i0 + CGS_CH);
   double a = 0.0;
   for (int i = i0 + (int)threadIdx.x; i < i1; i += GM_T) a += v[i] * w[i];
@@ -206,25 +208,38 @@ __global__ __launch_bounds__(GM_T) void k_gm_cgs_dot(int n, int nch, const doubl
   if (threadIdx.x == 0) part[(size_t)j * nch + c] = a;
 }
 
-// h_j = sum over chunks (in order); scal[sh + j] = h_j on the first pass, += h_j on the second
-__global__ void k_gm_cgs_sum(int ll, int nch, const double* __restrict__ part, double* __restrict__ scal, int sh,
-                             int second, double* __restrict__ hc) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= ll) return;
-  double h = 0.0;
-  for (int c = 0; c < nch; ++c) h += part[(size_t)j * nch + c];
-  hc[j] = h;
-  scal[sh + j] = second ? scal[sh + j] + h : h;
-}
-
-// w -= sum_j hc[j] V(:, j), j in order
-__global__ void k_gm_cgs_update(int n, int ll, const double* __restrict__ V, const double* __restrict__ hc,
-                                double* __restrict__ w) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    double t = w[i];
-    for (int j = 0; j < ll; ++j) t -= hc[j] * V[(size_t)j * n + i];
-    w[i] = t;
+// One classical pass's update: every block first forms h_j = sum over chunks of part (in
+// chunk order, so every block holds the same h) in LDS, then w -= sum_j h_j V(:, j), j in
+// order.  Block 0 stores scal[sh + j] = h_j on the first pass (+= on the second) and, when vn
+// is given, |w|^2 before the pass from the partials' row ll.  With NORM the blocks then sum
+// |w|^2 after the update (finish2, fixed order) into *sn.  Dynamic LDS: ll doubles.
+template <bool NORM>
+__global__ __launch_bounds__(GM_T) void k_gm_cgs_update(int n, int ll, int nch, const double* __restrict__ V,
+                                                        const double* __restrict__ part, double* __restrict__ scal,
+                                                        int sh, int second, double* __restrict__ vn,
+                                                        double* __restrict__ w, double* red_part, unsigned* red_count,
+                                                        double* __restrict__ sn) {
+  extern __shared__ double hs[];
+  for (int j = threadIdx.x; j <= ll; j += GM_T) {
+    if (j == ll && !vn) break;
+    double h = 0.0;
+    for (int c = 0; c < nch; ++c) h += part[(size_t)j * nch + c];
+    if (j < ll) {
+      hs[j] = h;
+      if (blockIdx.x == 0) scal[sh + j] = second ? scal[sh + j] + h : h;
+    } else if (blockIdx.x == 0) {
+      *vn = h;
+    }
   }
+  __syncthreads();
+  double s2 = 0.0;
+  for (int i = blockIdx.x * GM_T + threadIdx.x; i < n; i += gridDim.x * GM_T) {
+    double t = w[i];
+    for (int j = 0; j < ll; ++j) t -= hs[j] * V[(size_t)j * n + i];
+    w[i] = t;
+    s2 += t * t;
+  }
+  if (NORM) finish2(s2, 0.0, red_part, red_count, sn, nullptr);
 }
 
 // w += c v (c from the host)
@@ -273,8 +288,7 @@ struct eb_gmres_s {
   RedBuf red;
   int nblk = 1;
   int nch = 0;                // chunks of the classical Gram-Schmidt passes (n > MG1 limit)
-  double* cgs_part = nullptr; // maxl x nch partial dots
-  double* cgs_h = nullptr;    // maxl coefficients of one pass
+  double* cgs_part = nullptr; // (maxl + 1) x nch partial dots (the last row: |w|^2)
 };
 
 namespace {
@@ -373,17 +387,16 @@ int dpigmr_cycle(eb_gmres_s* g, eb_apply_fn apply, void* ctx, double* x, double 
       hipLaunchKernelGGL((k_gm_mgs1<MG1_NMAX / MG1_T, true>), dim3(1), dim3(MG1_T), 0, st, n, ll, g->V, g->scal, S_H,
                          s_vn(g), s_sn(g));
     } else {
-      // |w| before, two classical passes against v_0 .. v_{ll-1}, |w| after
-      hipLaunchKernelGGL(k_gm_norm_dot, dim3(nb), dim3(GM_T), 0, st, n, w, col(g, 0), g->red.part, g->red.count,
-                         g->scal + s_vn(g), g->scal + s_aux(g));
-      for (int pass = 0; pass < 2; ++pass) {
-        hipLaunchKernelGGL(k_gm_cgs_dot, dim3(g->nch, ll), dim3(GM_T), 0, st, n, g->nch, g->V, w, g->cgs_part);
-        hipLaunchKernelGGL(k_gm_cgs_sum, dim3((ll + 63) / 64), dim3(64), 0, st, ll, g->nch, g->cgs_part, g->scal,
-                           S_H, pass, g->cgs_h);
-        hipLaunchKernelGGL(k_gm_cgs_update, dim3(ew_grid(n)), dim3(256), 0, st, n, ll, g->V, g->cgs_h, w);
-      }
-      hipLaunchKernelGGL(k_gm_norm_dot, dim3(nb), dim3(GM_T), 0, st, n, w, w, g->red.part, g->red.count,
-                         g->scal + s_sn(g), g->scal + s_aux(g));
+      // two classical passes against v_0 .. v_{ll-1} in four launches: the first pass's dots
+      // carry |w|^2 before (grid row ll), the second pass's update |w|^2 after
+      const size_t lds = sizeof(double) * (size_t)(ll + 1);
+      hipLaunchKernelGGL(k_gm_cgs_dot, dim3(g->nch, ll + 1), dim3(GM_T), 0, st, n, g->nch, ll, g->V, w,
+                         g->cgs_part);
+      hipLaunchKernelGGL(k_gm_cgs_update<false>, dim3(nb), dim3(GM_T), lds, st, n, ll, g->nch, g->V, g->cgs_part,
+                         g->scal, S_H, 0, g->scal + s_vn(g), w, nullptr, nullptr, nullptr);
+      hipLaunchKernelGGL(k_gm_cgs_dot, dim3(g->nch, ll), dim3(GM_T), 0, st, n, g->nch, ll, g->V, w, g->cgs_part);
+      hipLaunchKernelGGL(k_gm_cgs_update<true>, dim3(nb), dim3(GM_T), lds, st, n, ll, g->nch, g->V, g->cgs_part,
+                         g->scal, S_H, 1, nullptr, w, g->red.part, g->red.count, g->scal + s_sn(g));
     }
     EB_HIP_CHECK(hipGetLastError());
     if ((rc = fetch(g, 0, maxl + 3, h.data())) != EB_OK) return rc;
@@ -488,8 +501,7 @@ extern "C" int eb_gmres_create(eb_gmres* out, int32_t n, int32_t maxl, void* str
             hipMalloc((void**)&g->red.count, sizeof(unsigned)) == hipSuccess;
   if (ok && n > MG1_NMAX) {
     g->nch = (n + CGS_CH - 1) / CGS_CH;
-    ok = hipMalloc((void**)&g->cgs_part, sizeof(double) * (size_t)maxl * g->nch) == hipSuccess &&
-         hipMalloc((void**)&g->cgs_h, sizeof(double) * (size_t)maxl) == hipSuccess;
+    ok = hipMalloc((void**)&g->cgs_part, sizeof(double) * (size_t)(maxl + 1) * g->nch) == hipSuccess;
   }
   if (!ok || hipMemsetAsync(g->red.count, 0, sizeof(unsigned), g->st) != hipSuccess ||
       hipMemsetAsync(g->scal, 0, sizeof(double) * (maxl + 8), g->st) != hipSuccess) {
@@ -503,7 +515,7 @@ extern "C" int eb_gmres_create(eb_gmres* out, int32_t n, int32_t maxl, void* str
 extern "C" int eb_gmres_destroy(eb_gmres g) {
   if (!g) return EB_OK;
   if (g->st) (void)hipStreamSynchronize(g->st);
-  void* p[] = {g->V, g->r, g->scal, g->coef, g->red.part, g->red.count, g->cgs_part, g->cgs_h};
+  void* p[] = {g->V, g->r, g->scal, g->coef, g->red.part, g->red.count, g->cgs_part};
   for (void* q : p)
     if (q) (void)hipFree(q);
   delete g;
