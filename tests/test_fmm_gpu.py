@@ -189,6 +189,46 @@ def test_plan_matches_direct_sum(torch, kind, levmax, p, variant):
     assert e < bound, (kind, levmax, p, variant, e)
 
 
+def test_expansion_length_sweep_converges(torch):
+    """Every kernel variant the expansion length selects (K = 16, 32, 48, 64; the M2L with a
+    vector tail at p = 33..36, three MFMA tiles at 37..48; the wide P2M from K = 48) against
+    the long-double direct sum.  A wrong row or coefficient in one variant leaves its error
+    stuck at that row's size, so besides a bound per p the error must keep falling by the
+    series' rate from one p to the next (about 2.2 per term on these curves)."""
+    rng = np.random.default_rng(77)
+    z, dz, nd, h = curves_cfg(64, 256, rng)
+    dens = complex_normal(z.size, rng)
+    tgt = np.sort(rng.choice(z.size, 2048, replace=False)).astype(np.int32)
+    ref = oracle.direct(oracle.DR, nd, z.real, z.imag, dz, dens, h, tgt)
+    errs = {}
+    for p in (16, 17, 24, 30, 31, 32, 33, 34, 35, 36, 37, 38, 40, 41, 44, 48, 49, 52, 60):
+        print(f"p sweep: p={p}", flush=True)
+        plan, out = _plan_case(torch, z, dz, nd, h, dens, 6, p)
+        plan.close()
+        errs[p] = rel(out[tgt], ref)
+    print({p: f"{e:.2e}" for p, e in errs.items()}, flush=True)
+    for p, e in errs.items():
+        assert e < max(5e-13, 1e-3 * 2.19 ** -(p - 16)), (p, e)
+    for p0, p1 in ((30, 32), (31, 33), (32, 34), (33, 35), (34, 36), (35, 37), (36, 38)):
+        assert errs[p1] < 0.6 * errs[p0], (p0, p1, errs[p0], errs[p1])
+
+
+@pytest.mark.parametrize("n", [2, 3, 63, 64, 65, 1000, 4097, 10240])
+def test_direct_mode_chunk_boundaries(torch, n):
+    """The direct sum runs as target tiles of 64 x source chunks with a fixed-order chunk
+    sum; sizes at and around the tile and chunk boundaries against the long-double sum."""
+    rng = np.random.default_rng(n)
+    z = rng.uniform(size=n) + 1j * rng.uniform(size=n)
+    dzv = np.exp(1j * rng.uniform(0, 6.28, n))
+    nd = np.array([n], np.int32)
+    h = np.array([0.01])
+    dens = complex_normal(n, rng)
+    tgt = np.arange(n, dtype=np.int32) if n <= 4097 else np.sort(rng.choice(n, 1024, replace=False)).astype(np.int32)
+    ref = oracle.direct(oracle.DR, nd, z.real, z.imag, dzv, dens, h, tgt)
+    _, out = _plan_case(torch, z, dzv, nd, h, dens, 4, 43, mode=eb.MODE_DIRECT)
+    assert rel(out[tgt], ref) < 1e-12, n
+
+
 def test_plan_is_real_linear(torch):
     rng = np.random.default_rng(5)
     z, dz, nd, h = curves_cfg(32, 128, rng)
