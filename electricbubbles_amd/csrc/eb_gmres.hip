@@ -9,6 +9,7 @@
 // the reference's order, and scales v_{l+1} on the device.  DHELS and DRLCAL produce
 // coefficient vectors on the host that one linear-combination kernel applies.
 #include <math.h>
+#include <string.h>
 
 #include <algorithm>
 #include <vector>
@@ -289,6 +290,7 @@ struct eb_gmres_s {
   int nblk = 1;
   int nch = 0;                // chunks of the classical Gram-Schmidt passes (n > MG1 limit)
   double* cgs_part = nullptr; // (maxl + 1) x nch partial dots (the last row: |w|^2)
+  double* hpin = nullptr;     // pinned host copy of scal (the per-iteration read-back)
 };
 
 namespace {
@@ -349,9 +351,12 @@ __host__ __device__ void dhels(const double* hes, int ldh, int n, const double* 
   }
 }
 
+// The scalars go through a pinned buffer: a copy into pageable memory is staged by the
+// runtime and costs several microseconds more per Arnoldi step.
 int fetch(eb_gmres_s* g, int first, int count, double* host) {
-  EB_HIP_CHECK(hipMemcpyAsync(host, g->scal + first, sizeof(double) * count, hipMemcpyDeviceToHost, g->st));
+  EB_HIP_CHECK(hipMemcpyAsync(g->hpin, g->scal + first, sizeof(double) * count, hipMemcpyDeviceToHost, g->st));
   EB_HIP_CHECK(hipStreamSynchronize(g->st));
+  memcpy(host, g->hpin, sizeof(double) * count);
   return EB_OK;
 }
 
@@ -498,7 +503,8 @@ extern "C" int eb_gmres_create(eb_gmres* out, int32_t n, int32_t maxl, void* str
             hipMalloc((void**)&g->scal, sizeof(double) * (maxl + 8)) == hipSuccess &&
             hipMalloc((void**)&g->coef, sizeof(double) * (maxl + 2)) == hipSuccess &&
             hipMalloc((void**)&g->red.part, sizeof(double) * 2 * GM_MAXBLK) == hipSuccess &&
-            hipMalloc((void**)&g->red.count, sizeof(unsigned)) == hipSuccess;
+            hipMalloc((void**)&g->red.count, sizeof(unsigned)) == hipSuccess &&
+            hipHostMalloc((void**)&g->hpin, sizeof(double) * (maxl + 8), hipHostMallocDefault) == hipSuccess;
   if (ok && n > MG1_NMAX) {
     g->nch = (n + CGS_CH - 1) / CGS_CH;
     ok = hipMalloc((void**)&g->cgs_part, sizeof(double) * (size_t)(maxl + 1) * g->nch) == hipSuccess;
@@ -518,6 +524,7 @@ extern "C" int eb_gmres_destroy(eb_gmres g) {
   void* p[] = {g->V, g->r, g->scal, g->coef, g->red.part, g->red.count, g->cgs_part};
   for (void* q : p)
     if (q) (void)hipFree(q);
+  if (g->hpin) (void)hipHostFree(g->hpin);
   delete g;
   return EB_OK;
 }
