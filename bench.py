@@ -346,12 +346,23 @@ def sweep_leg(torch, dev, with_cpu):
         gst.synchronize()
         t_graph = (time.perf_counter() - t0) / reps
         diff = float((yg - ye).norm() / ye.norm())
+        # the sweep's run: 200 replayed Heun steps of all simulations at dt = 1e-3
+        nrun, redone_run = 200, 0
+        t0 = time.perf_counter()
+        for _ in range(nrun):
+            redone_run += SFg.rk2_graph(yg, dt)
+        gst.synchronize()
+        t_run = time.perf_counter() - t0
+        run_finite = bool(torch.isfinite(yg).all())
     torch.cuda.synchronize()
     out["rk2_step"] = {"what": "one Heun step (2 F) of all simulations", "dt": dt, "eager_ms": round(t_eager * 1e3, 3),
                        "graph_ms": round(t_graph * 1e3, 3), "graph_budget": 8, "graph_steps_redone": int(redone),
                        "steps_per_s_graph": round(1.0 / t_graph, 1),
                        "simulation_steps_per_s_graph": round(nsim / t_graph, 1),
-                       "graph_vs_eager_state_rel_diff": diff}
+                       "graph_vs_eager_state_rel_diff": diff,
+                       "run": {"steps": nrun, "s": round(t_run, 3), "steps_per_s": round(nrun / t_run, 1),
+                               "simulation_steps_per_s": round(nsim * nrun / t_run, 1),
+                               "steps_redone": int(redone_run), "state_finite": run_finite}}
     SFg.close()
     if with_cpu:
         import tempfile
