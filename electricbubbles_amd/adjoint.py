@@ -221,8 +221,8 @@ def _functions():
             sys = P.sys
             sys.set_geometry(z, dz, s, kap, zk)
             key = ("lap", ctx.slot)
-            lam = sys.laplace_solve(ubar.contiguous(), P.tol, transpose=True, x0=P.cache.adj.get(key))
-            P.cache.adj[key] = lam
+            lam = sys.laplace_solve(ubar.contiguous(), P.tol, transpose=True, x0=P.cache.adj_guess(key))
+            P.cache.adj_store(key, lam)
             with torch.enable_grad():
                 g = [x.detach().requires_grad_() for x in (z, dz, s, kap)]
                 val = (lam * laplace_operator(sys, u, *g)).sum()
@@ -252,8 +252,9 @@ def _functions():
             sys.set_geometry(z, dz, s, kap, zk)
             b = torch.view_as_real(sbar.contiguous()).reshape(-1)
             key = ("st", ctx.slot)
-            lam = torch.view_as_complex(sys.stokes_solve(b, P.tol, transpose=True, x0=P.cache.adj.get(key)).view(-1, 2))
-            P.cache.adj[key] = torch.view_as_real(lam).reshape(-1)
+            lam = torch.view_as_complex(sys.stokes_solve(b, P.tol, transpose=True,
+                                                         x0=P.cache.adj_guess(key)).view(-1, 2))
+            P.cache.adj_store(key, torch.view_as_real(lam).reshape(-1))
             with torch.enable_grad():
                 g = [x.detach().requires_grad_() for x in (z, dz, s, kap, zk)]
                 av = stokes_operator(sys, sg, *g)
@@ -332,8 +333,9 @@ class SolveCache:
     * The forward pass with checkpoints records every Laplace and Stokes solution; when a
       segment is recomputed with a graph, each solve starts from the solution it found the
       first time, so DGMRES stops at its initial residual test with that same solution.
-    * In the reverse sweep, each transposed solve starts from the adjoint solution of the
-      same stage of the previous step (the adjoint state changes little from step to step).
+    * In the reverse sweep, each transposed solve starts from the linear extrapolation of the
+      adjoint solutions of the same stage in the two steps after it (the adjoint state is
+      smooth in time), or from the one solution when only one exists.
     The convergence test (relative residual below the tolerance) is the same in every case.
     """
 
@@ -366,6 +368,17 @@ class SolveCache:
     def after_solve(self, kind, x):
         if self.mode == "record":
             self.store[kind].append(x.detach())
+
+    def adj_guess(self, key):
+        h = self.adj.get(key)
+        if not h:
+            return None
+        return h[-1] if len(h) == 1 else 2.0 * h[-1] - h[-2]
+
+    def adj_store(self, key, lam):
+        h = self.adj.setdefault(key, [])
+        h.append(lam.detach())
+        del h[:-2]
 
 
 class Problem:
