@@ -333,9 +333,9 @@ class SolveCache:
     * The forward pass with checkpoints records every Laplace and Stokes solution; when a
       segment is recomputed with a graph, each solve starts from the solution it found the
       first time, so DGMRES stops at its initial residual test with that same solution.
-    * In the reverse sweep, each transposed solve starts from the linear extrapolation of the
-      adjoint solutions of the same stage in the two steps after it (the adjoint state is
-      smooth in time), or from the one solution when only one exists.
+    * In the reverse sweep, each transposed solve starts from the quadratic extrapolation of
+      the adjoint solutions of the same stage in the three steps after it (the adjoint state
+      is smooth in time); linear, or the one solution, while fewer exist.
     The convergence test (relative residual below the tolerance) is the same in every case.
     """
 
@@ -373,12 +373,16 @@ class SolveCache:
         h = self.adj.get(key)
         if not h:
             return None
-        return h[-1] if len(h) == 1 else 2.0 * h[-1] - h[-2]
+        if len(h) == 1:
+            return h[-1]
+        if len(h) == 2:
+            return 2.0 * h[-1] - h[-2]
+        return 3.0 * (h[-1] - h[-2]) + h[-3]
 
     def adj_store(self, key, lam):
         h = self.adj.setdefault(key, [])
         h.append(lam.detach())
-        del h[:-2]
+        del h[:-3]
 
 
 class Problem:
