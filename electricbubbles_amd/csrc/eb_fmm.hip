@@ -314,8 +314,10 @@ __device__ __forceinline__ double2 from_lane_below(double2 v) {
 }
 
 // P2M for KMAX = 48 and 64: the same sums with lane k forming both series' coefficient k, so
-// that one LDS read of t^k serves two outputs and t^(k-1) comes from lane k - 1 through DPP
-// instead of a second read.  Lanes k >= KMAX (KMAX = 48) follow along without writing.
+// that one LDS read of t^k serves two outputs.  The QC term's sum over sources of
+// E_j t_j^(k-1) is lane k - 1's sum of E_j t_j^k, taken through DPP once after the last
+// source and multiplied by k then: three complex FMAs per source and lane.  Lanes k >= KMAX
+// (KMAX = 48) follow along without writing.
 template <int KMAX>
 __global__ __launch_bounds__(64) void k_p2m_wide(int L, int p, const int* __restrict__ leaf_start,
                                                  const double2* __restrict__ zs, const double2* __restrict__ qa,
@@ -337,7 +339,7 @@ __global__ __launch_bounds__(64) void k_p2m_wide(int L, int p, const int* __rest
   const int js = lane & (CH - 1), qk = lane / CH;
   const int k = lane < KMAX ? lane : KMAX - 1;
   const double fk = (double)lane;
-  double2 a0 = make_double2(0, 0), a1 = make_double2(0, 0);
+  double2 a0 = make_double2(0, 0), a1 = make_double2(0, 0), a2 = make_double2(0, 0);
   for (int c0 = s; c0 < e; c0 += CH) {
     const int nj = min(CH, e - c0);
     if (js < nj) {
@@ -365,13 +367,13 @@ __global__ __launch_bounds__(64) void k_p2m_wide(int L, int p, const int* __rest
     __syncthreads();
     for (int jj = 0; jj < nj; ++jj) {
       const double2 pk = P[jj][k];
-      const double2 pkm1 = from_lane_below(pk);  // t^(k-1); lane 0 takes 0 (and fk = 0)
       cfma(a0, AB[jj][0], pk);
       cfma(a1, AB[jj][1], pk);
-      rfma(a1, fk, cmul(AB[jj][2], pkm1));
+      cfma(a2, AB[jj][2], pk);
     }
     __syncthreads();
   }
+  rfma(a1, fk, from_lane_below(a2));  // lane 0 takes 0 (and fk = 0)
   if (lane < KMAX) {
     const int pb = pidx(ix, iy, W);
     const bool in = lane < p;
