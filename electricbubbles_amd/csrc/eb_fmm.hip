@@ -353,15 +353,15 @@ __global__ __launch_bounds__(64) void k_p2m_wide(int L, int p, const int* __rest
         const double2 qc = cmul(make_double2(-z.x, z.y), q1);  // -conj(z) qa
         AB[js][2] = make_double2(qc.x * ir * ir, qc.y * ir * ir);
       }
-      double2 pw = make_double2(1.0, 0.0), bb = t;
-      for (int ee = qk * QK; ee > 0; ee >>= 1) {
-        if (ee & 1) pw = cmul(pw, bb);
-        bb = cmul(bb, bb);
-      }
+      // lane qk (0..7) writes t^(qk + 8 m), m = 0..QK-1: t^qk from t, t^2, t^4, then steps of t^8
+      const double2 t2 = cmul(t, t), t4 = cmul(t2, t2), t8 = cmul(t4, t4);
+      double2 pw = (qk & 1) ? t : make_double2(1.0, 0.0);
+      if (qk & 2) pw = cmul(pw, t2);
+      if (qk & 4) pw = cmul(pw, t4);
 #pragma unroll
-      for (int kk = 0; kk < QK; ++kk) {
-        P[js][qk * QK + kk] = pw;
-        pw = cmul(pw, t);
+      for (int m = 0; m < QK; ++m) {
+        P[js][qk + 8 * m] = pw;
+        if (m + 1 < QK) pw = cmul(pw, t8);
       }
     }
     __syncthreads();
