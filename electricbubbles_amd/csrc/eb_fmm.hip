@@ -267,16 +267,14 @@ __global__ __launch_bounds__(64) void k_p2m(int L, int p, const int* __restrict_
         const double2 qc = cmul(make_double2(-z.x, z.y), q1);  // -conj(z) qa
         E[js] = make_double2(qc.x * ir * ir, qc.y * ir * ir);
       }
-      // pw = t^(qk*QK) by squaring, then QK consecutive powers
-      double2 pw = make_double2(1.0, 0.0), bb = t;
-      for (int ee = qk * QK; ee > 0; ee >>= 1) {
-        if (ee & 1) pw = cmul(pw, bb);
-        bb = cmul(bb, bb);
-      }
+      // lane qk (0..3) writes t^(qk + 4 m), m = 0..QK-1: t^qk from t and t^2, then steps of t^4
+      const double2 t2 = cmul(t, t), t4 = cmul(t2, t2);
+      double2 pw = (qk & 1) ? t : make_double2(1.0, 0.0);
+      if (qk & 2) pw = cmul(pw, t2);
 #pragma unroll
-      for (int k = 0; k < QK; ++k) {
-        P[js][qk * QK + k] = pw;
-        pw = cmul(pw, t);
+      for (int m = 0; m < QK; ++m) {
+        P[js][qk + 4 * m] = pw;
+        if (m + 1 < QK) pw = cmul(pw, t4);
       }
     }
     __syncthreads();
