@@ -33,7 +33,7 @@ EB_ERR_ARG = 17
 EB_ERR_HIP = 18
 EB_ERR_NOMEM = 19
 
-VARIANT_DR, VARIANT_MOD, VARIANT_PLUS = 0, 1, 2
+VARIANT_DR, VARIANT_MOD, VARIANT_PLUS, VARIANT_HILBERT = 0, 1, 2, 3
 MODE_AUTO, MODE_FMM, MODE_DIRECT = 0, 1, 2
 
 _vp = ctypes.c_void_p
@@ -44,7 +44,7 @@ _i64p = ctypes.POINTER(ctypes.c_int64)
 
 # exported symbols of include/eb_cadap.h and include/eb_spectral.h
 EXPORTS = (
-    "eb_last_error", "eb_version", "eb_cadapdr", "eb_cadapmod", "eb_cadap", "eb_set_mode",
+    "eb_last_error", "eb_version", "eb_cadapdr", "eb_cadapmod", "eb_cadap", "eb_set_mode", "eb_get_mode",
     "eb_fmm_plan_create", "eb_fmm_plan_rebuild", "eb_fmm_apply", "eb_fmm_plan_destroy", "eb_fmm_plan_info",
     "eb_fmm_plan_ntbox", "eb_fmm_enable_timing", "eb_fmm_stage_ms", "eb_fmm_set_p2p_tile", "eb_cadap_tables",
     "eb_cadap_lists",
@@ -75,6 +75,7 @@ def lib():
         f.argtypes = dropin_args
         f.restype = _i32
     L.eb_set_mode.argtypes = [ctypes.c_int]
+    L.eb_get_mode.restype = ctypes.c_int
     L.eb_fmm_plan_create.argtypes = [ctypes.POINTER(_vp), _i32, _vp, _vp, _vp, _i32, _i32, _i32, _i32, _vp]
     L.eb_fmm_apply.argtypes = [_vp, _vp, _vp, _vp]
     L.eb_fmm_plan_rebuild.argtypes = [_vp, _vp, _vp, _vp, _vp]
@@ -170,7 +171,12 @@ def cadap(levmax, xat, yat, dzdt, nd, dens, h, tol, **kw):
 
 
 def set_mode(mode: int) -> None:
+    """Evaluation mode of the drop-in layer (CADAP*, FASMVP, DAPIF2 entry points)."""
     _check(lib().eb_set_mode(mode))
+
+
+def get_mode() -> int:
+    return int(lib().eb_get_mode())
 
 
 # ----------------------------------------------------------------------------------------
@@ -216,11 +222,32 @@ class FMMPlan:
                                         _stream_ptr(stream)))
         self._h = h
 
+    def _enter(self):
+        """The plan's own stream when it differs from torch's current one (made to wait for the
+        current stream's work first), else None."""
+        cur = self._torch.cuda.current_stream()
+        s = self.stream
+        if s is None or s == cur:
+            return None
+        s.wait_stream(cur)
+        return s
+
+    def _leave(self, s, *tensors):
+        """Order the current stream after the plan's stream and keep the tensors' memory
+        from being reused before the plan's stream is done with it."""
+        if s is None:
+            return
+        self._torch.cuda.current_stream().wait_stream(s)
+        for t in tensors:
+            t.record_stream(s)
+
     def rebuild(self, x, y, wdz):
         """Re-bin moved points (same n) in place."""
         self._keep = (x.contiguous(), y.contiguous(), wdz.contiguous())
+        s = self._enter()
         _check(lib().eb_fmm_plan_rebuild(self._h, _ptr(self._keep[0]), _ptr(self._keep[1]),
                                          _ptr(self._keep[2]), _stream_ptr(self.stream)))
+        self._leave(s, *self._keep)
 
     def apply(self, dens, out=None):
         torch = self._torch
@@ -228,7 +255,9 @@ class FMMPlan:
         dens = dens.contiguous()
         if out is None:
             out = torch.empty(self.n, dtype=torch.complex128, device=dens.device)
+        s = self._enter()
         _check(lib().eb_fmm_apply(self._h, _ptr(dens), _ptr(out), _stream_ptr(self.stream)))
+        self._leave(s, dens, out)
         return out
 
     def info(self):

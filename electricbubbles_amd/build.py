@@ -43,7 +43,7 @@ def _run(cmd, cwd=None):
 
 def build_hip(verbose: bool = False) -> str:
     out = os.path.join(HERE, "libeb.so")
-    objs = []
+    objs, cmds = [], []
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
     for src in HIP_SOURCES:
@@ -52,8 +52,14 @@ def build_hip(verbose: bool = False) -> str:
             continue
         obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
         lang = ["-x", "hip"] if src.endswith(".hip") else []
-        _run([HIPCC, *HIP_FLAGS, *HIP_EXTRA.get(src, []), *lang, "-c", path, "-o", obj])
+        cmds.append([HIPCC, *HIP_FLAGS, *HIP_EXTRA.get(src, []), *lang, "-c", path, "-o", obj])
         objs.append(obj)
+    # the translation units are independent: compile them side by side (MAX_JOBS caps it)
+    from concurrent.futures import ThreadPoolExecutor
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with ThreadPoolExecutor(jobs) as ex:
+        for f in [ex.submit(_run, c) for c in cmds]:
+            f.result()
     tmp = out + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", tmp])
     os.replace(tmp, out)

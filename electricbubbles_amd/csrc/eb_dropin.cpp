@@ -23,15 +23,21 @@ namespace {
 
 constexpr int kDirectMax = EB_DIRECT_MAX;  // EB_MODE_AUTO: O(N^2) sum at or below this many points
 
+// One cached plan per variant: within one F the reference calls CADAPDR (SOLVEI's matvecs)
+// and then CADAPMOD (INTERFACENEW, electric.f:835-837) on the same geometry, and a single slot
+// would rebuild both plans on every F.
+struct Slot {
+  eb_fmm_plan plan = nullptr;
+  int n = 0, levmax = 0, nterms = 0, pmode = -1;
+  std::vector<double> kx, ky, kw;
+  std::vector<int32_t> ntbox;
+};
+
 struct Ctx {
   std::mutex mu;
   int mode = EB_MODE_AUTO;
-  eb_fmm_plan plan = nullptr;
-  // key of the cached plan
-  int n = 0, levmax = 0, nterms = 0, variant = -1, pmode = -1;
-  std::vector<double> kx, ky, kw;
-  std::vector<int32_t> ntbox;
-  // device buffers
+  Slot slot[4];  // indexed by EB_VARIANT_*
+  // device buffers shared by the slots (a plan reads x, y, wdz only while it bins them)
   int cap = 0;
   double *dx = nullptr, *dy = nullptr, *dw = nullptr, *ddens = nullptr, *dgw = nullptr;
   hipStream_t st = nullptr;
@@ -44,6 +50,7 @@ Ctx& ctx() {
 
 int grow(Ctx& c, int n) {
   if (n <= c.cap) return EB_OK;
+  if (c.st) (void)hipStreamSynchronize(c.st);
   double** bufs[] = {&c.dx, &c.dy, &c.dw, &c.ddens, &c.dgw};
   const size_t sz[] = {1, 1, 2, 2, 2};
   for (int i = 0; i < 5; ++i) {
@@ -56,6 +63,84 @@ int grow(Ctx& c, int n) {
     }
   }
   c.cap = n;
+  return EB_OK;
+}
+
+int ensure_stream(Ctx& c) {
+  if (c.st) return EB_OK;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    eb_set_error("drop-in: no HIP device available");
+    return EB_ERR_HIP;
+  }
+  if (hipStreamCreate(&c.st) != hipSuccess) {
+    eb_set_error("drop-in: hipStreamCreate failed");
+    return EB_ERR_HIP;
+  }
+  return EB_OK;
+}
+
+// Make slot `variant` hold a plan of n points at (x, y, w) (host arrays; w = weighted tangent,
+// interleaved complex) with LEVMAX L, nterms terms and plan mode pmode.  A plan whose sizes
+// match is re-binned in place (eb_fmm_plan_rebuild: no allocation); otherwise it is replaced.
+// Called with the context lock held.
+int slot_plan(Ctx& c, int variant, int n, const double* x, const double* y, const double* w, int L, int nterms,
+              int pmode) {
+  Slot& S = c.slot[variant];
+  const bool sizes = S.plan && S.n == n && S.levmax == L && S.nterms == nterms && S.pmode == pmode;
+  if (sizes && memcmp(S.kx.data(), x, sizeof(double) * n) == 0 && memcmp(S.ky.data(), y, sizeof(double) * n) == 0 &&
+      memcmp(S.kw.data(), w, sizeof(double) * 2 * n) == 0)
+    return EB_OK;
+  int rc;
+  if ((rc = grow(c, n)) != EB_OK) return rc;
+  if (hipMemcpyAsync(c.dx, x, sizeof(double) * n, hipMemcpyHostToDevice, c.st) != hipSuccess ||
+      hipMemcpyAsync(c.dy, y, sizeof(double) * n, hipMemcpyHostToDevice, c.st) != hipSuccess ||
+      hipMemcpyAsync(c.dw, w, sizeof(double) * 2 * n, hipMemcpyHostToDevice, c.st) != hipSuccess) {
+    eb_set_error("drop-in: host-to-device copy failed");
+    return EB_ERR_HIP;
+  }
+  if (sizes) {
+    rc = eb_fmm_plan_rebuild(S.plan, c.dx, c.dy, c.dw, c.st);
+  } else {
+    if (S.plan) eb_fmm_plan_destroy(S.plan);
+    S.plan = nullptr;
+    rc = eb_fmm_plan_create(&S.plan, n, c.dx, c.dy, c.dw, L, nterms, variant, pmode, c.st);
+    if (rc != EB_OK) S.plan = nullptr;
+  }
+  if (rc != EB_OK) {
+    if (S.plan) eb_fmm_plan_destroy(S.plan);
+    S.plan = nullptr;
+    return rc;
+  }
+  S.ntbox.assign(L + 1, 0);
+  if ((rc = eb_fmm_plan_ntbox(S.plan, S.ntbox.data())) != EB_OK) {
+    eb_fmm_plan_destroy(S.plan);
+    S.plan = nullptr;
+    return rc;
+  }
+  S.n = n;
+  S.levmax = L;
+  S.nterms = nterms;
+  S.pmode = pmode;
+  S.kx.assign(x, x + n);
+  S.ky.assign(y, y + n);
+  S.kw.assign(w, w + 2 * (size_t)n);
+  return EB_OK;
+}
+
+// GW of slot `variant` for the density dens (host, interleaved complex) into gw (host).
+int slot_apply(Ctx& c, int variant, int n, const double* dens, double* gw) {
+  if (hipMemcpyAsync(c.ddens, dens, sizeof(double) * 2 * n, hipMemcpyHostToDevice, c.st) != hipSuccess) {
+    eb_set_error("drop-in: density copy failed");
+    return EB_ERR_HIP;
+  }
+  int rc = eb_fmm_apply(c.slot[variant].plan, c.ddens, c.dgw, c.st);
+  if (rc != EB_OK) return rc;
+  if (hipMemcpyAsync(gw, c.dgw, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, c.st) != hipSuccess ||
+      hipStreamSynchronize(c.st) != hipSuccess) {
+    eb_set_error("drop-in: device-to-host copy failed");
+    return EB_ERR_HIP;
+  }
   return EB_OK;
 }
 
@@ -102,18 +187,10 @@ int32_t cadap_common(int variant, const int32_t* iflag1, const int32_t* iflag2, 
 
   Ctx& c = ctx();
   std::lock_guard<std::mutex> lk(c.mu);
-  if (!c.st) {
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
-      eb_set_error("CADAP drop-in: no HIP device available");
-      ier[0] = EB_ERR_HIP;
-      return EB_ERR_HIP;
-    }
-    if (hipStreamCreate(&c.st) != hipSuccess) {
-      eb_set_error("CADAP drop-in: hipStreamCreate failed");
-      ier[0] = EB_ERR_HIP;
-      return EB_ERR_HIP;
-    }
+  int rc;
+  if ((rc = ensure_stream(c)) != EB_OK) {
+    ier[0] = rc;
+    return rc;
   }
   const int pmode = (c.mode == EB_MODE_AUTO) ? (n <= kDirectMax ? EB_MODE_DIRECT : EB_MODE_FMM) : c.mode;
 
@@ -127,47 +204,11 @@ int32_t cadap_common(int variant, const int32_t* iflag1, const int32_t* iflag2, 
         w[2 * j + 1] = dzdt[2 * j + 1] * h[k];
       }
   }
-  const bool same = c.plan && c.n == n && c.levmax == L && c.nterms == nterms && c.variant == variant &&
-                    c.pmode == pmode && memcmp(c.kx.data(), xat, sizeof(double) * n) == 0 &&
-                    memcmp(c.ky.data(), yat, sizeof(double) * n) == 0 &&
-                    memcmp(c.kw.data(), w.data(), sizeof(double) * 2 * n) == 0;
-  int rc;
-  if (!same) {
-    if (c.plan) {
-      eb_fmm_plan_destroy(c.plan);
-      c.plan = nullptr;
-    }
-    if ((rc = grow(c, n)) != EB_OK) {
-      ier[0] = rc;
-      return rc;
-    }
-    if (hipMemcpyAsync(c.dx, xat, sizeof(double) * n, hipMemcpyHostToDevice, c.st) != hipSuccess ||
-        hipMemcpyAsync(c.dy, yat, sizeof(double) * n, hipMemcpyHostToDevice, c.st) != hipSuccess ||
-        hipMemcpyAsync(c.dw, w.data(), sizeof(double) * 2 * n, hipMemcpyHostToDevice, c.st) != hipSuccess) {
-      eb_set_error("CADAP drop-in: host-to-device copy failed");
-      ier[0] = EB_ERR_HIP;
-      return EB_ERR_HIP;
-    }
-    rc = eb_fmm_plan_create(&c.plan, n, c.dx, c.dy, c.dw, L, nterms, variant, pmode, c.st);
-    if (rc != EB_OK) {
-      c.plan = nullptr;
-      ier[0] = rc;
-      return rc;
-    }
-    c.ntbox.assign(L + 1, 0);
-    if ((rc = eb_fmm_plan_ntbox(c.plan, c.ntbox.data())) != EB_OK) {
-      ier[0] = rc;
-      return rc;
-    }
-    c.n = n;
-    c.levmax = L;
-    c.nterms = nterms;
-    c.variant = variant;
-    c.pmode = pmode;
-    c.kx.assign(xat, xat + n);
-    c.ky.assign(yat, yat + n);
-    c.kw = w;
+  if ((rc = slot_plan(c, variant, n, xat, yat, w.data(), L, nterms, pmode)) != EB_OK) {
+    ier[0] = rc;
+    return rc;
   }
+  const Slot& S = c.slot[variant];
   // per-level workspace checks in the reference's order (cadapdrop.f:261-319): integer
   // tables first, then the local-expansion storage 2 (NTERMS+1) NTBOX.
   for (int l = 1; l <= L; ++l) {
@@ -180,7 +221,7 @@ int32_t cadap_common(int variant, const int32_t* iflag1, const int32_t* iflag2, 
       ier[4] = *leniw;
       return 8;
     }
-    const long long itotc = itotp + 2LL * (nterms + 1) * c.ntbox[l];
+    const long long itotc = itotp + 2LL * (nterms + 1) * S.ntbox[l];
     if (itotc > *lenw) {
       ier[0] = 8;
       ier[1] = (int32_t)itotc;
@@ -188,20 +229,9 @@ int32_t cadap_common(int variant, const int32_t* iflag1, const int32_t* iflag2, 
       return 8;
     }
   }
-  if (hipMemcpyAsync(c.ddens, dens, sizeof(double) * 2 * n, hipMemcpyHostToDevice, c.st) != hipSuccess) {
-    eb_set_error("CADAP drop-in: density copy failed");
-    ier[0] = EB_ERR_HIP;
-    return EB_ERR_HIP;
-  }
-  if ((rc = eb_fmm_apply(c.plan, c.ddens, c.dgw, c.st)) != EB_OK) {
+  if ((rc = slot_apply(c, variant, n, dens, gradw)) != EB_OK) {
     ier[0] = rc;
     return rc;
-  }
-  if (hipMemcpyAsync(gradw, c.dgw, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, c.st) != hipSuccess ||
-      hipStreamSynchronize(c.st) != hipSuccess) {
-    eb_set_error("CADAP drop-in: device-to-host copy failed");
-    ier[0] = EB_ERR_HIP;
-    return EB_ERR_HIP;
   }
   inform[0] = L;
   return EB_OK;
@@ -236,4 +266,10 @@ extern "C" int eb_set_mode(int mode) {
   std::lock_guard<std::mutex> lk(c.mu);
   c.mode = mode;
   return EB_OK;
+}
+
+extern "C" int eb_get_mode(void) {
+  Ctx& c = ctx();
+  std::lock_guard<std::mutex> lk(c.mu);
+  return c.mode;
 }

@@ -300,6 +300,7 @@ struct eb_field_s {
   double cap = 0.0, e0 = 0.0, alpha_last = 0.0;
   hipStream_t st = nullptr;
   bool have_geom = false;
+  int mode = EB_MODE_AUTO;  // evaluation mode of every layer potential (eb_field_set_mode)
   int32_t it_lap = 0, ie_lap = 0, it_st = 0, ie_st = 0;  // the last call's GMRES counts
   int nt_lap = 37, nt_pv = 33, nt_mod = 43;  // TOL 1e-11 (laplace.f:212), 1e-10 (pv_integral.f:175), 1e-13
   // geometry
@@ -402,8 +403,31 @@ extern "C" int eb_field_destroy(eb_field F) {
   return EB_OK;
 }
 
-// EB_MODE_AUTO's rule: the O(N^2) sum up to EB_DIRECT_MAX points, the FMM with LEVMAX 7 above
-static int plan_mode(int n) { return n <= EB_DIRECT_MAX ? EB_MODE_DIRECT : EB_MODE_FMM; }
+// EB_MODE_AUTO's rule: the O(N^2) sum up to EB_DIRECT_MAX points, the FMM with LEVMAX 7 above;
+// a mode set by eb_field_set_mode overrides it
+static int plan_mode(eb_field F, int n) {
+  if (F->mode != EB_MODE_AUTO) return F->mode;
+  return n <= EB_DIRECT_MAX ? EB_MODE_DIRECT : EB_MODE_FMM;
+}
+
+extern "C" int eb_field_set_mode(eb_field F, int32_t mode) {
+  if (!F || (mode != EB_MODE_AUTO && mode != EB_MODE_FMM && mode != EB_MODE_DIRECT))
+    return (eb_set_error("eb_field_set_mode: bad arguments"), EB_ERR_ARG);
+  if (mode == F->mode) return EB_OK;
+  if (F->st) (void)hipStreamSynchronize(F->st);
+  // the plans are made again, in the new mode, by the next eb_field_set_geometry
+  eb_fmm_plan* plans[] = {&F->lap, &F->pv, &F->itf};
+  for (eb_fmm_plan* q : plans)
+    if (*q) {
+      eb_fmm_plan_destroy(*q);
+      *q = nullptr;
+    }
+  if (F->mv) eb_matvec_destroy(F->mv);
+  F->mv = nullptr;
+  F->have_geom = false;
+  F->mode = mode;
+  return EB_OK;
+}
 
 extern "C" int eb_field_set_geometry(eb_field F, const double* rku) {
   if (!F || !rku) return (eb_set_error("eb_field_set_geometry: bad arguments"), EB_ERR_ARG);
@@ -416,10 +440,10 @@ extern "C" int eb_field_set_geometry(eb_field F, const double* rku) {
   EB_HIP_CHECK(hipGetLastError());
   const double* wd = (const double*)F->wdz;
   if (!F->have_geom) {
-    if ((rc = eb_fmm_plan_create(&F->lap, n, F->x, F->y, wd, 7, F->nt_lap, EB_VARIANT_HILBERT, plan_mode(n), s)) ||
-        (rc = eb_fmm_plan_create(&F->itf, n, F->x, F->y, wd, 7, F->nt_mod, EB_VARIANT_MOD, plan_mode(n), s)) ||
+    if ((rc = eb_fmm_plan_create(&F->lap, n, F->x, F->y, wd, 7, F->nt_lap, EB_VARIANT_HILBERT, plan_mode(F, n), s)) ||
+        (rc = eb_fmm_plan_create(&F->itf, n, F->x, F->y, wd, 7, F->nt_mod, EB_VARIANT_MOD, plan_mode(F, n), s)) ||
         (rc = eb_matvec_create(&F->mv, F->B, F->x, F->y, (const double*)F->dz, F->dsda, F->rkappa,
-                               (const double*)F->zk, 7, F->nt_mod, EB_MODE_AUTO, s)))
+                               (const double*)F->zk, 7, F->nt_mod, plan_mode(F, n), s)))
       return rc;
   } else {
     if ((rc = eb_fmm_plan_rebuild(F->lap, F->x, F->y, wd, s)) || (rc = eb_fmm_plan_rebuild(F->itf, F->x, F->y, wd, s)) ||
@@ -484,7 +508,7 @@ extern "C" int eb_field_dirich_neu(eb_field F, const double* ulap, double* phi, 
   EB_HIP_CHECK(hipGetLastError());
   if (!F->pv)
     rc = eb_fmm_plan_create(&F->pv, n2, F->x2, F->y2, (const double*)F->wf, 7, F->nt_pv, EB_VARIANT_HILBERT,
-                            plan_mode(n2), s);
+                            plan_mode(F, n2), s);
   else
     rc = eb_fmm_plan_rebuild(F->pv, F->x2, F->y2, (const double*)F->wf, s);
   if (rc != EB_OK) return rc;

@@ -1182,6 +1182,7 @@ struct eb_fmm_plan_s {
   bool timing = false;
   hipEvent_t ev[8] = {};
   double stage_ms[7] = {0};
+  hipEvent_t ev_switch = nullptr;  // orders a new stream after the plan's previous one
 };
 
 static void plan_free(eb_fmm_plan P) {
@@ -1194,6 +1195,7 @@ static void plan_free(eb_fmm_plan P) {
     if (q) (void)hipFree(q);
   for (auto& e : P->ev)
     if (e) (void)hipEventDestroy(e);
+  if (P->ev_switch) (void)hipEventDestroy(P->ev_switch);
   delete P;
 }
 
@@ -1214,6 +1216,18 @@ static int pick_kmax(int p) {
   if (p <= 32) return 32;
   if (p <= 48) return 48;
   return 64;
+}
+
+// Move the plan to `stream` (NULL keeps the current one).  Work already queued on the old stream
+// (a tree build, the last apply) is ordered before anything the new stream runs next.
+static int use_stream(eb_fmm_plan P, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  if (!s || s == P->st) return EB_OK;
+  if (!P->ev_switch) EB_HIP_CHECK(hipEventCreateWithFlags(&P->ev_switch, hipEventDisableTiming));
+  EB_HIP_CHECK(hipEventRecord(P->ev_switch, P->st));
+  EB_HIP_CHECK(hipStreamWaitEvent(s, P->ev_switch, 0));
+  P->st = s;
+  return EB_OK;
 }
 
 #define EB_TRY(x)            \
@@ -1492,7 +1506,7 @@ extern "C" int eb_fmm_apply(eb_fmm_plan P, const double* dens, double* gradw, vo
     eb_set_error("eb_fmm_apply: null argument");
     return EB_ERR_ARG;
   }
-  if (stream) P->st = (hipStream_t)stream;
+  if (int rc = use_stream(P, stream)) return rc;
   hipStream_t st = P->st;
   const int n = P->n;
   if (P->timing) {
@@ -1545,7 +1559,7 @@ extern "C" int eb_fmm_plan_rebuild(eb_fmm_plan P, const double* x, const double*
     eb_set_error("eb_fmm_plan_rebuild: null argument");
     return EB_ERR_ARG;
   }
-  if (stream) P->st = (hipStream_t)stream;
+  if (int rc = use_stream(P, stream)) return rc;
   return build_tree(P, x, y, wdz);
 }
 

@@ -910,6 +910,7 @@ struct FasCtx {
   eb_matvec M = nullptr;
   std::vector<int> nd;
   std::vector<double> key;
+  int mode = -1;  // the drop-in mode the cached matvec was built for
   double *dgeo = nullptr, *du = nullptr, *dw = nullptr, *dst = nullptr, *dza = nullptr, *dpk = nullptr;
 };
 FasCtx& fctx() {
@@ -937,7 +938,8 @@ extern "C" int32_t eb_fasmvp(const int32_t* nd, const int32_t* nbk, const int32_
   key.insert(key.end(), zk, zk + 2 * K);
   std::vector<int> ndv(nd, nd + K);
   int rc;
-  if (!c.M || ndv != c.nd || key != c.key) {
+  const int mode = eb_get_mode();
+  if (!c.M || ndv != c.nd || key != c.key || mode != c.mode) {
     if (c.M) eb_matvec_destroy(c.M);
     if (c.B) eb_bodies_destroy(c.B);
     c.M = nullptr;
@@ -952,7 +954,7 @@ extern "C" int32_t eb_fasmvp(const int32_t* nd, const int32_t* nbk, const int32_
     if (hipMemcpyAsync(c.dgeo, key.data(), sizeof(double) * key.size(), hipMemcpyHostToDevice, c.st) != hipSuccess)
       return (eb_set_error("eb_fasmvp: copy failed"), EB_ERR_HIP);
     double* g = c.dgeo;
-    const int mode = N <= EB_DIRECT_MAX ? EB_MODE_DIRECT : EB_MODE_FMM;
+    // the drop-in layer's mode (eb_set_mode): EB_MODE_AUTO is resolved by eb_matvec_create
     rc = eb_matvec_create(&c.M, c.B, g, g + N, g + 2 * N, g + 4 * N, g + 5 * N, g + 6 * N, 7, 43, mode, c.st);
     if (rc != EB_OK) {
       c.M = nullptr;
@@ -960,6 +962,7 @@ extern "C" int32_t eb_fasmvp(const int32_t* nd, const int32_t* nbk, const int32_
     }
     c.nd = ndv;
     c.key.swap(key);
+    c.mode = mode;
   }
   if (hipMemcpyAsync(c.du, u, sizeof(double) * 2 * N, hipMemcpyHostToDevice, c.st) != hipSuccess)
     return (eb_set_error("eb_fasmvp: copy failed"), EB_ERR_HIP);

@@ -12,7 +12,7 @@ import numpy as np
 
 from . import _check, _ptr, _stream_ptr, _torch, lib
 
-FIELD_EXPORTS = ("eb_field_create", "eb_field_destroy", "eb_field_rhs", "eb_field_set_geometry",
+FIELD_EXPORTS = ("eb_field_create", "eb_field_destroy", "eb_field_set_mode", "eb_field_rhs", "eb_field_set_geometry",
                  "eb_field_laplace_apply", "eb_field_laplace_solve", "eb_field_dirich_neu", "eb_field_setvelf",
                  "eb_field_stokes_solve", "eb_field_interface", "eb_field_get_geometry", "eb_field_iterations", "eb_sweep_field_create",
                  "eb_sweep_field_rhs", "eb_sweep_field_iterations", "eb_sweep_field_destroy",
@@ -29,6 +29,7 @@ def _bind():
     if not _bound:
         L.eb_field_create.argtypes = [ctypes.POINTER(_vp), _vp, ctypes.c_double, ctypes.c_double, _vp]
         L.eb_field_destroy.argtypes = [_vp]
+        L.eb_field_set_mode.argtypes = [_vp, _i32]
         L.eb_field_rhs.argtypes = [_vp, _vp, _vp]
         L.eb_field_set_geometry.argtypes = [_vp, _vp]
         L.eb_field_laplace_apply.argtypes = [_vp, _vp, _vp]
@@ -58,6 +59,11 @@ class Field:
         h = _vp()
         _check(_bind().eb_field_create(ctypes.byref(h), bodies._h, float(cap), float(e0), _stream_ptr(stream)))
         self._h = h
+
+    def set_mode(self, mode):
+        """Evaluation mode of every layer potential (MODE_AUTO, MODE_FMM or MODE_DIRECT); takes
+        effect at the next set_geometry / rhs."""
+        _check(_bind().eb_field_set_mode(self._h, int(mode)))
 
     def _f64(self, n, like):
         return self._torch.empty(n, dtype=self._torch.float64, device=like.device)

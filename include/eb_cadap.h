@@ -95,8 +95,11 @@ int32_t eb_cadap(const int32_t *iflag1, const int32_t *iflag2, const int32_t *le
                  const double *tol, int32_t *inform, int32_t *ier, const double *xleft,
                  const double *slength);
 
-/* Evaluation mode used by the drop-in layer (process-wide; default EB_MODE_AUTO). */
+/* Evaluation mode used by the drop-in layer (process-wide; default EB_MODE_AUTO): the CADAP
+ * entry points, eb_fasmvp and eb_dapif2 take the O(N^2) sum or the FMM by it.  No counterpart
+ * in the reference (its CADAPDR is always the FMM); a knob of this build for parity runs. */
 int eb_set_mode(int mode);
+int eb_get_mode(void);
 
 /* ---------------------------------------------------------------------------------------
  * Layer 2: device-resident plan / apply.  All pointers are device pointers; `stream` is a

@@ -33,6 +33,12 @@ typedef struct eb_field_s *eb_field;
 /* Bodies up to 2048 points each (PVINTEV works on twice the points). */
 int eb_field_create(eb_field *f, eb_bodies bodies, double cap, double e0, void *stream);
 int eb_field_destroy(eb_field f);
+/* Evaluation mode of every layer potential of F (the Laplace and PVINTEV Hilbert products, the
+ * Stokes matvec, CADAPMOD): EB_MODE_AUTO (default: the O(N^2) sum up to EB_DIRECT_MAX points,
+ * the FMM above), EB_MODE_FMM or EB_MODE_DIRECT.  Drops the current plans; the next
+ * eb_field_set_geometry (or eb_field_rhs) makes them in the new mode.  No counterpart in the
+ * reference (its products are always FMMs); a knob for parity runs of either path. */
+int eb_field_set_mode(eb_field f, int32_t mode);
 
 /* rkup = F(rku): the whole chain above, on the field's stream. */
 int eb_field_rhs(eb_field f, const double *rku, double *rkup);

@@ -178,6 +178,21 @@ def direct(variant, nd, x, y, dzdt, dens, h, targets=None):
     return gw.view(np.complex128)
 
 
+def cauchy(x, y, qa, targets=None):
+    """DAPIF2's Hilbert-matrix product (IFLAG7 = 3, dapif1.f:540-558) as an O(N^2) sum in long
+    double: sum_{j != i} qa_j / (z_i - z_j) at the 0-based targets (default all)."""
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    q = _cplx(qa)
+    n = x.size
+    if targets is None:
+        targets = np.arange(n, dtype=np.int32)
+    targets = np.ascontiguousarray(targets, np.int32)
+    out = np.zeros(2 * targets.size)
+    lib().ora_cauchy(n, _d(x), _d(y), _d(q), targets.size, _i(targets), _d(out))
+    return out.view(np.complex128)
+
+
 def fasmvp_epilogue(nd, x, y, dz, dsdth, rkappa, zk, u, stress):
     nd = np.ascontiguousarray(nd, np.int32)
     k = nd.size
@@ -514,6 +529,28 @@ def ref_dapif2_scaled(x, y, napb, tol=1e-11, mex=300, eps7=1e-16, seed=0):
               _c_int(nsp), _CLOSE_STUB)
     return dict(ierr=int(ierr[0]), xs=wksp[:nat].copy(), ys=wksp[nat:2 * nat].copy(), nbox=int(inform[1]),
                 nlev=int(inform[2]))
+
+
+def ref_dapif2(x, y, qa, napb=30, tol=1e-11, mex=300, eps7=1e-16, nsp=None):
+    """The reference DAPIF2 with IFLAG7 = 3 (the Hilbert-matrix product, as laplace.f:204-215
+    and pv_integral.f:171-182 call it).  Returns (field, ierr[0:4], inform[0:4])."""
+    r = ref()
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    nat = x.size
+    q = _cplx(qa).copy()
+    poten = np.zeros(nat)
+    field = np.zeros(2 * nat)
+    if nsp is None:
+        nsp = 4_000_000 + 400 * nat
+    wksp = np.zeros(nsp)
+    iout = np.zeros(2, np.int32)
+    ierr = np.zeros(8, np.int32)
+    inform = np.zeros(8, np.int32)
+    r.dapif2_(_i(iout), _c_int(3), _c_int(nat), _c_int(napb), _c_int(2), _c_int(mex), _i(ierr), _i(inform),
+              _c_double(tol), _c_double(eps7), _d(x), _d(y), _d(q), _d(poten), _d(field), _d(wksp),
+              _c_int(nsp), _CLOSE_STUB)
+    return field.view(np.complex128), ierr[:4].copy(), inform[:4].copy()
 
 
 def ref_atree(xs, ys, napb, lmx=None, eps7=1e-16):

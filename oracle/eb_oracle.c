@@ -14,8 +14,10 @@
 
 #include <complex.h>
 #include <math.h>
+#include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
+#include <unistd.h>
 
 typedef double complex cplx;
 
@@ -391,6 +393,71 @@ int ora_cadap(int variant, int levmax, int kbod, const int *nd, const double *x,
 /* ------------------------------------------------------------------------------------- */
 typedef long double complex lcplx;
 
+/* The target loop runs on up to ORA_THREADS host threads (environment variable, default the
+ * online cores, at most 64); each target's sum over the sources runs in source order on one
+ * thread, so the result does not depend on the thread count. */
+typedef struct {
+    const lcplx *z, *qa, *qb;
+    int n, ntgt, kind;       /* kind 0: cadap kernel with sign sgn; 1: Cauchy sum */
+    long double sgn;
+    const int *tgt;
+    double *out;
+    int t0, t1;
+} ora_sum_job;
+
+static void *ora_sum_run(void *arg)
+{
+    ora_sum_job *J = (ora_sum_job *)arg;
+    for (int t = J->t0; t < J->t1; ++t) {
+        const int i = J->tgt[t];
+        lcplx acc = 0;
+        for (int s = 0; s < J->n; ++s) {
+            if (s == i) continue;
+            lcplx dd = J->z[s] - J->z[i];
+            long double r2 = creall(dd) * creall(dd) + cimagl(dd) * cimagl(dd);
+            lcplx inv = conjl(dd) / r2;           /* 1/d */
+            if (J->kind == 1) {
+                acc -= J->qa[s] * inv;            /* qa_s / (z_i - z_s) */
+                continue;
+            }
+            lcplx cinv = dd / r2;                 /* 1/conj(d) */
+            acc += J->qa[s] * inv + J->sgn * (conjl(J->qb[s]) * cinv - conjl(J->qa[s]) * dd * cinv * cinv);
+        }
+        J->out[2 * t] = (double)creall(acc);
+        J->out[2 * t + 1] = (double)cimagl(acc);
+    }
+    return NULL;
+}
+
+static void ora_sum_parallel(ora_sum_job base)
+{
+    int nth = 1;
+    const char *env = getenv("ORA_THREADS");
+    if (env && atoi(env) > 0) nth = atoi(env);
+    else {
+        long c = sysconf(_SC_NPROCESSORS_ONLN);
+        nth = c > 0 ? (int)c : 1;
+    }
+    if (nth > 64) nth = 64;
+    if (nth > base.ntgt) nth = base.ntgt > 0 ? base.ntgt : 1;
+    /* below some 10^6 pairs the threads cost more than they save */
+    if ((long long)base.ntgt * base.n < 4000000LL) nth = 1;
+    ora_sum_job jobs[64];
+    pthread_t th[64];
+    int started[64] = {0};
+    for (int k = 0; k < nth; ++k) {
+        jobs[k] = base;
+        jobs[k].t0 = (int)((long long)base.ntgt * k / nth);
+        jobs[k].t1 = (int)((long long)base.ntgt * (k + 1) / nth);
+        if (k > 0 && pthread_create(&th[k], NULL, ora_sum_run, &jobs[k]) == 0) started[k] = 1;
+    }
+    ora_sum_run(&jobs[0]);
+    for (int k = 1; k < nth; ++k) {
+        if (started[k]) pthread_join(th[k], NULL);
+        else ora_sum_run(&jobs[k]);
+    }
+}
+
 void ora_direct(int variant, int kbod, const int *nd, const double *x, const double *y,
                 const double *dzdt, const double *dens, const double *h,
                 int ntgt, const int *tgt, double *gradw)
@@ -412,21 +479,25 @@ void ora_direct(int variant, int kbod, const int *nd, const double *x, const dou
             qa[j] = z2pi * d * w * (long double)h[k];
             qb[j] = (sqb * conjl(d) * w + d * conjl(w)) * z2pi * (long double)h[k];
         }
-    for (int t = 0; t < ntgt; ++t) {
-        int i = tgt[t];
-        lcplx acc = 0;
-        for (int s = 0; s < n; ++s) {
-            if (s == i) continue;
-            lcplx dd = z[s] - z[i];
-            long double r2 = creall(dd) * creall(dd) + cimagl(dd) * cimagl(dd);
-            lcplx inv = conjl(dd) / r2;           /* 1/d */
-            lcplx cinv = dd / r2;                 /* 1/conj(d) */
-            acc += qa[s] * inv + sgn * (conjl(qb[s]) * cinv - conjl(qa[s]) * dd * cinv * cinv);
-        }
-        gradw[2 * t] = (double)creall(acc);
-        gradw[2 * t + 1] = (double)cimagl(acc);
-    }
+    ora_sum_job J = {z, qa, qb, n, ntgt, 0, sgn, tgt, gradw, 0, 0};
+    ora_sum_parallel(J);
     free(qa); free(qb); free(z);
+}
+
+/* ------------------------------------------------------------------------------------- */
+/* Hilbert-matrix product of DAPIF2 with IFLAG7 = 3 (dapif1.f:540-558, the direct form    */
+/* DADIRE dapif1.f:104-311): FIELD_i = sum_{j != i} qa_j / (z_i - z_j), long double.      */
+/* ------------------------------------------------------------------------------------- */
+void ora_cauchy(int n, const double *x, const double *y, const double *qa, int ntgt, const int *tgt, double *field)
+{
+    lcplx *q = (lcplx *)malloc(sizeof(lcplx) * n), *z = (lcplx *)malloc(sizeof(lcplx) * n);
+    for (int j = 0; j < n; ++j) {
+        z[j] = (long double)x[j] + I * (long double)y[j];
+        q[j] = (long double)qa[2 * j] + I * (long double)qa[2 * j + 1];
+    }
+    ora_sum_job J = {z, q, NULL, n, ntgt, 1, 0.0L, tgt, field, 0, 0};
+    ora_sum_parallel(J);
+    free(q); free(z);
 }
 
 /* ------------------------------------------------------------------------------------- */

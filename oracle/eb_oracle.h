@@ -41,10 +41,17 @@ int ora_cadap(int variant, int levmax, int kbod, const int *nd, const double *x,
               double *gradw, int *inform, int *ier, long long *counts);
 
 /* O(N^2) direct sum of the same kernel in the original coordinates, long double accumulation.
- * Evaluates at the ntgt target indices tgt[] (0-based); sources are all points except the target. */
+ * Evaluates at the ntgt target indices tgt[] (0-based); sources are all points except the target.
+ * Targets are split over ORA_THREADS host threads (default: the online cores); each target's
+ * sum runs in source order, so the result does not depend on the thread count. */
 void ora_direct(int variant, int kbod, const int *nd, const double *x, const double *y,
                 const double *dzdt, const double *dens, const double *h,
                 int ntgt, const int *tgt, double *gradw);
+
+/* The Hilbert-matrix product of DAPIF2 with IFLAG7 = 3 (dapif1.f:540-558):
+ * field[t] = sum_{j != tgt[t]} qa_j / (z_tgt[t] - z_j), long double, at the ntgt targets.
+ * qa and field are interleaved complex. */
+void ora_cauchy(int n, const double *x, const double *y, const double *qa, int ntgt, const int *tgt, double *field);
 
 /* stokesbubb.f:527-549,585-611: the FASMVP epilogue around a given GW (``stress``).
  * u: 2*nbk interleaved density, w: 2*nbk output; za (complex, k) and pk (k) are outputs. */

@@ -2,8 +2,8 @@
 
 Runs only where the reference sources exist (they are compiled in place by oracle/Makefile);
 the fixtures it writes are data -- inputs and the reference's outputs -- and are committed so
-the tests run without the reference.  Usage:  python tests/golden/make_golden.py [atree|gmres|sweep|field|sim|adjoint]
-("atree" / "gmres" / "sweep" / "field" / "sim" / "adjoint" rewrite only those fixtures).
+the tests run without the reference.  Usage:  python tests/golden/make_golden.py [atree|gmres|sweep|field|sim|adjoint|dapif2]
+("atree" / "gmres" / "sweep" / "field" / "sim" / "adjoint" / "dapif2" rewrite only those fixtures).
 """
 from __future__ import annotations
 
@@ -441,9 +441,52 @@ def main_adjoint():
         os.chdir(cwd)
 
 
+def main_dapif2():
+    """DAPIF2's Hilbert-matrix product (IFLAG7 = 3) as laplace.f:204-215 and pv_integral.f:171-182
+    call it (NAPB 30, NINIRE 2, MEX 300, EPS7 1e-16): FIELD, IERR and INFORM of the reference on
+    three point sets -- closed curves carrying u dz h, as FASMVP_LAP forms them (TOL 1e-11), the
+    alternating-weight points of PVINTEV (TOL 1e-10), and uniform points with complex charges --
+    plus its error returns (TOL too small, NAT <= NAPB, workspace too short)."""
+    rng = np.random.default_rng(20261204)
+    cwd = os.getcwd()
+    os.chdir(os.path.join(ROOT, "gpurun_out") if os.path.isdir(os.path.join(ROOT, "gpurun_out")) else "/tmp")
+    try:
+        out = {}
+        z, dz, nd, h = curves_cfg(24, 128, rng)
+        u = rng.normal(size=z.size)
+        cases = [("lap", z, u * dz * np.repeat(h, nd), 1e-11)]
+        z2, dz2, nd2, h2 = curves_cfg(12, 256, rng)
+        q2 = np.where(np.arange(z2.size) % 2 == 1, rng.normal(size=z2.size) * np.repeat(h2, nd2), 0.0)
+        cases.append(("pv", z2, q2.astype(complex), 1e-10))
+        z3 = rng.uniform(-1, 1, 3000) + 1j * rng.uniform(-1, 1, 3000)
+        q3 = (rng.normal(size=3000) + 1j * rng.normal(size=3000)) / np.sqrt(2)
+        cases.append(("uniform", z3, q3, 1e-11))
+        for tag, zz, qq, tol in cases:
+            field, ierr, inform = oracle.ref_dapif2(zz.real, zz.imag, qq, napb=30, tol=tol)
+            assert ierr[0] == 0, (tag, ierr)
+            out.update({f"x_{tag}": zz.real, f"y_{tag}": zz.imag, f"qa_{tag}": qq, f"tol_{tag}": np.float64(tol),
+                        f"field_{tag}": field, f"ierr_{tag}": ierr, f"inform_{tag}": inform})
+        # error returns, on the "lap" points
+        z, q = cases[0][1], cases[0][2]
+        errs = {"tol": dict(tol=1e-18), "few": dict(napb=5000), "nsp": dict(nsp=2 * z.size - 1)}
+        for tag, kw in errs.items():
+            args = dict(napb=30, tol=1e-11)
+            args.update(kw)
+            sel = slice(None)
+            _, ierr, inform = oracle.ref_dapif2(z[sel].real, z[sel].imag, q[sel], **args)
+            assert ierr[0] != 0, tag
+            out[f"err_{tag}_ierr"] = ierr
+            out[f"err_{tag}_inform"] = inform
+        save("dapif2_hilbert", **out)
+    finally:
+        os.chdir(cwd)
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["adjoint"]:
         main_adjoint()
+    elif sys.argv[1:] == ["dapif2"]:
+        main_dapif2()
     elif sys.argv[1:] == ["atree"]:
         main_atree()
     elif sys.argv[1:] == ["gmres"]:
@@ -461,3 +504,4 @@ if __name__ == "__main__":
         main_sweep()
         main_field()
         main_sim()
+        main_dapif2()

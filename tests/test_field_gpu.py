@@ -16,6 +16,10 @@ from tests.conftest import GOLDEN
 pytestmark = pytest.mark.gpu
 
 CASES = ["single_drop", "two_drops", "four_circles"]
+# "auto" takes the O(N^2) sums at these sizes (EB_DIRECT_MAX); "fmm" forces every layer potential
+# of F (the Laplace and PVINTEV Hilbert products, the Stokes matvec, CADAPMOD) onto the FMM at
+# the reference's LEVMAX 7, the path F takes above EB_DIRECT_MAX points
+MODES = ["auto", "fmm"]
 
 
 @pytest.fixture(scope="module")
@@ -31,12 +35,14 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-def setup(torch, tag):
+def setup(torch, tag, mode="auto"):
+    import electricbubbles_amd as eb
     from electricbubbles_amd import field, spectral
     f = np.load(os.path.join(GOLDEN, f"field_{tag}.npz"))
-    print("field case", tag, list(f["nd"]), flush=True)
+    print("field case", tag, mode, list(f["nd"]), flush=True)
     B = spectral.Bodies(f["nd"])
     F = field.Field(B, cap=float(f["cap"]), e0=float(f["e0"]))
+    F.set_mode({"auto": eb.MODE_AUTO, "fmm": eb.MODE_FMM}[mode])
     rku = torch.tensor(f["rku"], device="cuda")
     F.set_geometry(rku)
     return f, B, F, rku
@@ -46,65 +52,73 @@ def dev(torch, a):
     return torch.tensor(np.ascontiguousarray(a), device="cuda")
 
 
+@pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("tag", CASES)
-def test_laplace_operator(torch, tag):
-    f, B, F, rku = setup(torch, tag)
+def test_laplace_operator(torch, tag, mode):
+    f, B, F, rku = setup(torch, tag, mode)
     w = F.laplace_apply(dev(torch, f["lap_u"])).cpu().numpy()
     assert rel(w, f["lap_w"]) < 1e-10
 
 
+@pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("tag", CASES)
-def test_laplace_solve(torch, tag):
-    f, B, F, rku = setup(torch, tag)
+def test_laplace_solve(torch, tag, mode):
+    f, B, F, rku = setup(torch, tag, mode)
     u, it, ierr = F.laplace_solve(rku)
     assert ierr == 0
     assert rel(u.cpu().numpy(), f["ulap"]) < 1e-8
 
 
+@pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("tag", CASES)
-def test_dirichlet_neumann_map(torch, tag):
-    f, B, F, rku = setup(torch, tag)
+def test_dirichlet_neumann_map(torch, tag, mode):
+    f, B, F, rku = setup(torch, tag, mode)
     phi, psi, dphi = F.dirich_neu(dev(torch, f["ulap"]))
     assert rel(phi.cpu().numpy(), f["phi"]) < 1e-10
     assert rel(psi.cpu().numpy(), f["psi"]) < 1e-10
     assert rel(dphi.cpu().numpy(), f["dphi_dn"]) < 1e-9
 
 
+@pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("tag", CASES)
-def test_setvelf(torch, tag):
-    f, B, F, rku = setup(torch, tag)
+def test_setvelf(torch, tag, mode):
+    f, B, F, rku = setup(torch, tag, mode)
     rhs = F.setvelf(dev(torch, f["dphi_dn"])).cpu().numpy()
     assert rel(rhs, f["rhs_stokes"]) < 1e-12
 
 
+@pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("tag", CASES)
-def test_stokes_solve(torch, tag):
-    f, B, F, rku = setup(torch, tag)
+def test_stokes_solve(torch, tag, mode):
+    f, B, F, rku = setup(torch, tag, mode)
     u, za, pk, it, ierr = F.stokes_solve(dev(torch, f["rhs_stokes"]))
     assert ierr == 0
     assert rel(u.cpu().numpy(), f["u"]) < 1e-9
     assert np.abs(pk.cpu().numpy() - f["pk"]).max() <= 1e-9 * max(1.0, np.abs(f["pk"]).max())
 
 
+@pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("tag", CASES)
-def test_interface_velocity(torch, tag):
-    f, B, F, rku = setup(torch, tag)
+def test_interface_velocity(torch, tag, mode):
+    f, B, F, rku = setup(torch, tag, mode)
     un = F.interface(dev(torch, f["u"])).cpu().numpy()
     assert rel(un, f["un"]) < 1e-10
 
 
+@pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("tag", CASES)
-def test_whole_rhs(torch, tag):
-    f, B, F, rku = setup(torch, tag)
+def test_whole_rhs(torch, tag, mode):
+    f, B, F, rku = setup(torch, tag, mode)
     rkup = F.rhs(rku).cpu().numpy()
     assert rel(rkup, f["rkup"]) < 1e-8
 
 
+@pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("tag", CASES)
-def test_rk2_steps(torch, tag):
+def test_rk2_steps(torch, tag, mode):
     """Three Heun steps (electric.f:762-786) with F on the GPU against the reference's."""
     from electricbubbles_amd import spectral
-    f, B, F, rku = setup(torch, tag)
+    f, B, F, rku = setup(torch, tag, mode)
     dt = float(f["rk2_dt"])
     ybar = rku.clone()
     for _ in range(int(f["rk2_steps"])):
@@ -195,14 +209,17 @@ def test_sweep_field_512_single_drops(torch):
     assert np.all(lap_ie == 0) and np.all(st_ie == 0)
 
 
-def test_many_drops_rhs_and_rk2(torch):
+@pytest.mark.parametrize("mode", MODES)
+def test_many_drops_rhs_and_rk2(torch, mode):
     """Eight drops of the many-drop construction (256..1024 points each, E0 = 2.25): the whole
     F and two RK2 steps at the configuration's dt against the reference's."""
+    import electricbubbles_amd as eb
     from electricbubbles_amd import field, spectral
     f = np.load(os.path.join(GOLDEN, "field_many_drops_8.npz"))
-    print("many drops", list(f["nd"]), flush=True)
+    print("many drops", mode, list(f["nd"]), flush=True)
     B = spectral.Bodies(f["nd"])
     F = field.Field(B, cap=float(f["cap"]), e0=float(f["e0"]))
+    F.set_mode({"auto": eb.MODE_AUTO, "fmm": eb.MODE_FMM}[mode])
     rku = torch.tensor(f["rku"], device="cuda")
     assert rel(F.rhs(rku).cpu().numpy(), f["rkup"]) < 1e-8
     dt = float(f["rk2_dt"])

@@ -288,3 +288,67 @@ def test_large_curves_against_direct_subset(torch):
     tgt = np.sort(rng.choice(z.size, 256, replace=False)).astype(np.int32)
     ref = oracle.direct(oracle.DR, nd, z.real, z.imag, dz, dens, h, tgt)
     assert rel(out[tgt], ref) < TOL_REL
+
+
+# ------------------------------------------------------------------------------------------
+# The Hilbert-matrix product (EB_VARIANT_HILBERT) on the FMM: the product DAPIF2 forms with
+# IFLAG7 = 3 for the Laplace side (laplace.f:204-215, pv_integral.f:171-182), against the
+# long-double sum of oracle.cauchy (itself pinned to the reference DAPIF2's output and to a
+# brute-force numpy sum in tests/test_oracle_cpu.py).
+# ------------------------------------------------------------------------------------------
+
+@pytest.mark.parametrize("kind,n,levmax,p", [("curves", 16384, 7, 37), ("curves", 65536, 7, 37),
+                                             ("uniform", 16384, 6, 37), ("curves", 65536, 8, 33)])
+def test_hilbert_fmm_against_cauchy_sum(torch, kind, n, levmax, p):
+    rng = np.random.default_rng(n + levmax + p)
+    if kind == "curves":
+        z, _, _, _ = curves_cfg(n // 128, 128, rng)
+    else:
+        z, _, _, _ = uniform_cfg(n, rng)
+    qa = complex_normal(n, rng)
+    print("hilbert fmm", kind, n, levmax, p, flush=True)
+    x = torch.tensor(z.real.copy(), device="cuda")
+    y = torch.tensor(z.imag.copy(), device="cuda")
+    ones = torch.ones(n, dtype=torch.complex128, device="cuda")
+    plan = eb.FMMPlan(x, y, ones, levmax, p, eb.VARIANT_HILBERT, eb.MODE_FMM)
+    out = plan.apply(torch.tensor(qa, device="cuda")).cpu().numpy()
+    tgt = np.sort(rng.choice(n, 2048, replace=False)).astype(np.int32)
+    ref = oracle.cauchy(z.real, z.imag, qa, tgt)
+    # p = 37 is the term count of TOL 1e-11 (laplace.f:212); p = 33 that of 1e-10 (pv_integral.f:175)
+    assert rel(out[tgt], ref) < (1e-10 if p >= 37 else 1e-9)
+    # the direct mode of the same plan type at the same targets
+    dplan = eb.FMMPlan(x, y, ones, levmax, p, eb.VARIANT_HILBERT, eb.MODE_DIRECT) if n <= 16384 else None
+    if dplan is not None:
+        dout = dplan.apply(torch.tensor(qa, device="cuda")).cpu().numpy()
+        assert rel(dout[tgt], ref) < 1e-13
+
+
+# ------------------------------------------------------------------------------------------
+# The 2^20-point workloads of the headline (circles and uniform points, the bench's seeds) at
+# the tuned depth and expansion length, against the long-double O(N^2) sum over all 2^20
+# sources at 1024 random targets.
+# ------------------------------------------------------------------------------------------
+
+@pytest.mark.parametrize("kind", ["curves", "uniform"])
+def test_two_to_the_20_against_direct_sum(torch, kind):
+    rng = np.random.default_rng(20260303)
+    n = 1 << 20
+    if kind == "curves":
+        z, dz, nd, h = curves_cfg(n // 256, 256, rng)
+    else:
+        z, dz, nd, h = uniform_cfg(n, rng)
+    dens = complex_normal(n, rng)
+    print("2^20", kind, "LEVMAX 8, p 36", flush=True)
+    plan, out = _plan_case(torch, z, dz, nd, h, dens, 8, 36)
+    info = plan.info()
+    assert info["levmax"] == 8 and info["nterms"] == 36
+    tgt = np.sort(np.random.default_rng(7).choice(n, 1024, replace=False)).astype(np.int32)
+    ref = oracle.direct(oracle.DR, nd, z.real, z.imag, dz, dens, h, tgt)
+    err = rel(out[tgt], ref)
+    print(f"2^20 {kind}: rel err {err:.2e} at 1024 targets, max leaf {info['max_leaf']}", flush=True)
+    assert err < TOL_REL
+    # the same targets from a plan re-binned in place (the bench's timed loop)
+    plan.rebuild(plan._keep[0], plan._keep[1], plan._keep[2])
+    out2 = plan.apply(torch.tensor(dens, device="cuda")).cpu().numpy()
+    assert rel(out2[tgt], ref) < TOL_REL
+    plan.close()

@@ -42,7 +42,22 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-def test_fasmvp_through_shim_matches_reference(sims):
+@pytest.fixture(params=["auto", "fmm"])
+def dropin_mode(request):
+    """The drop-in layer's mode: "auto" takes the O(N^2) sum at these sizes (N <= EB_DIRECT_MAX);
+    "fmm" sends every CADAPDR / CADAPMOD call of the unchanged drivers to the GPU FMM at the
+    LEVMAX and TOL they pass (stokesbubb.f:564-571, electric.f:835-837)."""
+    import electricbubbles_amd as eb
+    mode = {"auto": eb.MODE_AUTO, "fmm": eb.MODE_FMM}[request.param]
+    eb.set_mode(mode)
+    print("drop-in mode", request.param, flush=True)
+    try:
+        yield mode
+    finally:
+        eb.set_mode(eb.MODE_AUTO)
+
+
+def test_fasmvp_through_shim_matches_reference(sims, dropin_mode):
     ref, gpu = sims
     g = np.load(os.path.join(GOLDEN, "geometry_two_drops.npz"))
     args = (g["nd"], g["x"], g["y"], g["dz"], g["dsda"], g["rkappa"], g["zk"], g["u"])
@@ -52,7 +67,7 @@ def test_fasmvp_through_shim_matches_reference(sims):
     assert rel(w_gpu, g["w"]) < 1e-10
 
 
-def test_single_drop_trajectory_matches_reference(sims):
+def test_single_drop_trajectory_matches_reference(sims, dropin_mode):
     """Config-1 drop r = 1 + 0.05 cos 2 theta, 256 points, 10 Heun steps of dt 1e-3: the
     reference's F and RK2 with the GPU FMM against the reference as written (shapes 1e-8)."""
     ref, gpu = sims

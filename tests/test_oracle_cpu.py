@@ -152,6 +152,43 @@ def test_direct_sum_two_points_closed_form():
     assert abs(got[0] - expect0) < 1e-14 * abs(expect0)
 
 
+@pytest.mark.parametrize("tag", ["lap", "pv", "uniform"])
+def test_cauchy_sum_matches_reference_dapif2_and_numpy(tag):
+    """oracle.cauchy (DAPIF2's IFLAG7 = 3 product as a long-double sum) against the reference
+    DAPIF2's FIELD at its TOL and against a brute-force numpy sum; a flipped sign (z_j - z_i)
+    and a dropped self-exclusion both fail the numpy pin."""
+    f = g("dapif2_hilbert")
+    x, y, qa = f[f"x_{tag}"], f[f"y_{tag}"], f[f"qa_{tag}"]
+    got = oracle.cauchy(x, y, qa)
+    assert rel(got, f[f"field_{tag}"]) < 10 * float(f[f"tol_{tag}"])
+    z = x + 1j * y
+    d = z[:, None] - z[None, :]
+    np.fill_diagonal(d, np.inf)
+    ref = (qa[None, :] / d).sum(1)
+    assert rel(got, ref) < 1e-14
+    assert rel(-got, ref) > 1.0                      # the sign of z_i - z_j
+    dd = d.copy()
+    np.fill_diagonal(dd, 1.0)                        # a self term 1 * qa_i left in
+    assert rel(got, (qa[None, :] / dd).sum(1)) > 1e-6
+    # a subset of targets equals the same rows of the whole product
+    tgt = np.array([0, 5, x.size - 1], np.int32)
+    np.testing.assert_array_equal(oracle.cauchy(x, y, qa, tgt), got[tgt])
+
+
+def test_direct_sum_is_independent_of_thread_count(monkeypatch):
+    """The threaded long-double sums give the same bits on one and on several threads."""
+    rng = np.random.default_rng(4)
+    z, dz, nd, h = curves_cfg(16, 256, rng)
+    dens = complex_normal(z.size, rng)
+    outs = []
+    for nth in ("1", "5"):
+        monkeypatch.setenv("ORA_THREADS", nth)
+        outs.append((oracle.direct(oracle.DR, nd, z.real, z.imag, dz, dens, h),
+                     oracle.cauchy(z.real, z.imag, dens)))
+    np.testing.assert_array_equal(outs[0][0], outs[1][0])
+    np.testing.assert_array_equal(outs[0][1], outs[1][1])
+
+
 def test_cadap_error_codes_match_reference():
     t = g("cadap_errors")
     for tag in ("tol",):
