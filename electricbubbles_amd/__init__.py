@@ -47,7 +47,7 @@ EXPORTS = (
     "eb_last_error", "eb_version", "eb_cadapdr", "eb_cadapmod", "eb_cadap", "eb_set_mode", "eb_get_mode",
     "eb_fmm_plan_create", "eb_fmm_plan_rebuild", "eb_fmm_apply", "eb_fmm_plan_destroy", "eb_fmm_plan_info",
     "eb_fmm_plan_ntbox", "eb_fmm_enable_timing", "eb_fmm_stage_ms", "eb_fmm_set_p2p_tile", "eb_cadap_tables",
-    "eb_cadap_lists",
+    "eb_cadap_lists", "eb_direct_apply_noalloc",
 )
 
 _lib = None
@@ -87,6 +87,7 @@ def lib():
     L.eb_fmm_stage_ms.argtypes = [_vp, _dp]
     L.eb_cadap_tables.argtypes = [_i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _ip, _ip, _vp]
     L.eb_cadap_lists.argtypes = [_i32, _vp, _vp, _vp, _vp, _vp]
+    L.eb_direct_apply_noalloc.argtypes = [_i32, _vp, _vp, _vp, _vp, _i32, _vp, _vp]
     _lib = L
     return L
 
@@ -297,6 +298,18 @@ class FMMPlan:
             self.close()
         except Exception:
             pass
+
+
+def direct_apply_noalloc(x, y, wdz, dens, out, variant=VARIANT_DR, stream=None):
+    """GW of the O(N^2) sum (as an FMMPlan in MODE_DIRECT) in one call that allocates no device
+    memory: every tensor, out included, is the caller's (n <= 10,240 points).  Calls that
+    overlap on one GPU share the library's fixed working array and must not run concurrently."""
+    n = int(x.numel())
+    for t in (x, y, wdz, dens, out):
+        assert t.is_cuda and t.is_contiguous() and t.numel() == n
+    _check(lib().eb_direct_apply_noalloc(n, _ptr(x), _ptr(y), _ptr(wdz), _ptr(dens), int(variant), _ptr(out),
+                                         _stream_ptr(stream)))
+    return out
 
 
 def tables(ilev, x, y, stream=None):

@@ -1034,7 +1034,8 @@ __global__ __launch_bounds__(256) void k_direct_sum(int n, int nch, const double
     acc2 = cadd(acc2, part[2 * ((size_t)c * n + i) + 1]);
   }
   const double rs = g->rscal;
-  out[perm[i]] = make_double2((acc1.x + sgn * acc2.x) * rs, (acc1.y + sgn * acc2.y) * rs);
+  // perm == nullptr: the points are in the caller's order
+  out[perm ? perm[i] : i] = make_double2((acc1.x + sgn * acc2.x) * rs, (acc1.y + sgn * acc2.y) * rs);
 }
 
 // chunk length (a multiple of DIR_T) and count for n points: about DIR_MAXBLK blocks in all
@@ -1043,6 +1044,114 @@ static void direct_chunks(int n, int& sch, int& nch) {
   int want = std::max(1, std::min(tiles, DIR_MAXBLK / tiles));
   sch = ((n + want - 1) / want + DIR_T - 1) / DIR_T * DIR_T;
   nch = (n + sch - 1) / sch;
+}
+
+// ------------------------------------------------------------------------------------------
+// The direct (O(N^2)) evaluation without allocation: eb_direct_apply_noalloc.  The plan path
+// (eb_fmm_plan_create in EB_MODE_DIRECT + eb_fmm_apply) allocates its bounding-box partials,
+// sorted points, charges and per-chunk partial sums with hipMalloc; this entry point runs the
+// same kernels in the caller's point order with those buffers carved from `scratch`, a
+// file-scope device array of fixed size that each device holds until the process ends.
+// ------------------------------------------------------------------------------------------
+// Largest part of the scratch: 2 x nch x n partial sums.  direct_chunks keeps (target tiles of
+// DIR_T) x chunks <= DIR_MAXBLK, so nch x n <= DIR_MAXBLK x DIR_T for every n.
+constexpr size_t NA_PART = (size_t)2 * DIR_MAXBLK * DIR_T;            // double2 elements
+constexpr size_t NA_PTS = (size_t)EB_DIRECT_MAX;                        // points
+constexpr size_t NA_BB = 1024;                                          // bounding-box partials
+constexpr size_t NA_BYTES = 256 + NA_BB * sizeof(double4) + 3 * NA_PTS * sizeof(double2) + NA_PART * sizeof(double2);
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+// scaled coordinates and charges in the caller's order (k_keys' scaling, k_charges' charges)
+__global__ void k_na_prep(int n, const double* __restrict__ x, const double* __restrict__ y,
+                          const double2* __restrict__ wdz, const double2* __restrict__ dens, const Geom* __restrict__ g,
+                          double sqb, int hilbert, double2* __restrict__ zs, double2* __restrict__ qa,
+                          double2* __restrict__ qb) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double rs = g->rscal;
+  zs[i] = make_double2((x[i] - g->rminx) * rs, (y[i] - g->rminy) * rs);
+  const double2 d = dens[i];
+  const double2 dw = cmul(d, wdz[i]);
+  if (hilbert) {
+    qa[i] = make_double2(-dw.x, -dw.y);
+    qb[i] = make_double2(0.0, 0.0);
+    return;
+  }
+  const double c = -0.15915494309189533577;  // -1/(2 pi): 1/(2 pi i) = -i/(2 pi)
+  qa[i] = make_double2(-c * dw.y, c * dw.x);
+  const double2 w = wdz[i];
+  const double2 t = cadd(cscale(cmul(cconj(d), w), sqb), cmul(d, cconj(w)));
+  qb[i] = make_double2(-c * t.y, c * t.x);
+}
+
+namespace {
+std::mutex na_mu;
+constexpr int NA_MAXDEV = 64;
+void* na_ptr[NA_MAXDEV] = {};  // device address of `scratch` on each device, once looked up
+}  // namespace
+
+// hipGetSymbolAddress once per device, with that device current (the caller's current device)
+static int na_scratch(unsigned char** out) {
+  int dev = 0;
+  EB_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= NA_MAXDEV) {
+    eb_set_error("eb_direct_apply_noalloc: device %d beyond the %d the library tracks", dev, NA_MAXDEV);
+    return EB_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lk(na_mu);
+  if (!na_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      eb_set_error("eb_direct_apply_noalloc: hipGetSymbolAddress(scratch) failed on device %d: %s", dev,
+                   hipGetErrorString(e));
+      return EB_ERR_HIP;
+    }
+    na_ptr[dev] = p;
+  }
+  *out = (unsigned char*)na_ptr[dev];
+  return EB_OK;
+}
+
+extern "C" int eb_direct_apply_noalloc(int32_t n, const double* x, const double* y, const double* wdz,
+                                       const double* dens, int32_t variant, double* gradw, void* stream) {
+  if (n < 2 || n > EB_DIRECT_MAX || !x || !y || !wdz || !dens || !gradw) {
+    eb_set_error("eb_direct_apply_noalloc: need 2 <= n <= %d points (its fixed scratch) and non-null arrays (n=%d)",
+                 EB_DIRECT_MAX, n);
+    return EB_ERR_ARG;
+  }
+  if (variant < 0 || variant > 3) {
+    eb_set_error("eb_direct_apply_noalloc: bad variant %d", variant);
+    return EB_ERR_ARG;
+  }
+  unsigned char* base = nullptr;
+  if (int rc = na_scratch(&base)) return rc;
+  int sch = 0, nch = 0;
+  direct_chunks(n, sch, nch);
+  const int bbblocks = std::min((int)NA_BB, (n + BB_THREADS - 1) / BB_THREADS);
+  if ((size_t)2 * nch * n > NA_PART) {  // cannot happen by direct_chunks' bound; kept as a guard
+    eb_set_error("eb_direct_apply_noalloc: %d chunks of %d points exceed the scratch", nch, n);
+    return EB_ERR_ARG;
+  }
+  // scratch layout: Geom | bounding-box partials | zs, qa, qb (n each) | partial sums
+  Geom* g = (Geom*)base;
+  double4* bb = (double4*)(base + 256);
+  double2* zs = (double2*)(base + 256 + NA_BB * sizeof(double4));
+  double2* qa = zs + NA_PTS;
+  double2* qb = qa + NA_PTS;
+  double2* part = qb + NA_PTS;
+  const double sgn = (variant == EB_VARIANT_DR) ? -1.0 : (variant == EB_VARIANT_HILBERT ? 0.0 : 1.0);
+  const double sqb = (variant == EB_VARIANT_PLUS) ? 1.0 : -1.0;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_bbox_part, dim3(bbblocks), dim3(BB_THREADS), 0, st, x, y, n, bb);
+  hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(BB_THREADS), 0, st, bb, bbblocks, g);
+  hipLaunchKernelGGL(k_na_prep, dim3((n + 255) / 256), dim3(256), 0, st, n, x, y, (const double2*)wdz,
+                     (const double2*)dens, g, sqb, variant == EB_VARIANT_HILBERT ? 1 : 0, zs, qa, qb);
+  hipLaunchKernelGGL(k_direct_part, dim3((n + DIR_T - 1) / DIR_T, nch), dim3(DIR_T), 0, st, n, sch, zs, qa, qb, part);
+  hipLaunchKernelGGL(k_direct_sum, dim3((n + 255) / 256), dim3(256), 0, st, n, nch, part, (const int32_t*)nullptr, g,
+                     sgn, (double2*)gradw);
+  EB_HIP_CHECK(hipGetLastError());
+  return EB_OK;
 }
 
 // ------------------------------------------------------------------------------------------

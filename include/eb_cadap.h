@@ -122,6 +122,18 @@ int eb_fmm_plan_rebuild(eb_fmm_plan plan, const double *x, const double *y, cons
 int eb_fmm_apply(eb_fmm_plan plan, const double *dens, double *gradw, void *stream);
 int eb_fmm_plan_destroy(eb_fmm_plan plan);
 
+/* The O(N^2) evaluation of eb_fmm_apply in EB_MODE_DIRECT, in one call that allocates nothing:
+ * gradw[i] = GW at point i (caller's order) of the kernel `variant` for n points (x, y),
+ * weighted tangents wdz and density dens; every pointer is a device pointer, stream-ordered on
+ * `stream`.  No counterpart in the reference: it is for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops).  Its working memory is a fixed
+ * device array of about 8.5 MiB inside the library, held by each GPU it runs on until the
+ * process ends, so n is limited to EB_DIRECT_MAX points (larger n: EB_ERR_ARG).  Every call on
+ * one GPU shares that array: calls that overlap on one GPU (other streams, other host threads)
+ * race and must be serialised by the caller. */
+int eb_direct_apply_noalloc(int32_t n, const double *x, const double *y, const double *wdz, const double *dens,
+                            int32_t variant, double *gradw, void *stream);
+
 /* Plan facts: [0] rscal, [1] rminx, [2] rminy; counts: [0] levmax, [1] nterms, [2] kmax
  * (padded p), [3] #ordered near-field pairs, [4] #M2L pairs (both boxes non-empty),
  * [5] #non-empty boxes over levels 2..levmax, [6] #non-empty leaf boxes, [7] max leaf count.
