@@ -21,6 +21,10 @@ __device__ __forceinline__ void cfma(double2& acc, double2 a, double2 b) {
   acc.x = fma(a.x, b.x, fma(-a.y, b.y, acc.x));
   acc.y = fma(a.x, b.y, fma(a.y, b.x, acc.y));
 }
+// a*b + c in four FMAs (cadd(cmul(a, b), c) compiles to two multiplies, two FMAs and two adds)
+__device__ __forceinline__ double2 cmuladd(double2 a, double2 b, double2 c) {
+  return make_double2(fma(a.x, b.x, fma(-a.y, b.y, c.x)), fma(a.x, b.y, fma(a.y, b.x, c.y)));
+}
 // acc += s*a  (s real)
 __device__ __forceinline__ void rfma(double2& acc, double s, double2 a) {
   acc.x = fma(s, a.x, acc.x);

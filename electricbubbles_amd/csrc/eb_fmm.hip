@@ -50,18 +50,6 @@ extern "C" int eb_version(void) { return 1; }
 // ------------------------------------------------------------------------------------------
 __device__ __forceinline__ int pidx(int ix, int iy, int W) { return (iy + 3) * W + (ix + 3); }
 
-// Logical block index in an XCD-aware order.  The dispatcher hands physical block i to XCD
-// i % 8; this mapping gives each XCD one contiguous range of logical blocks (neighbouring
-// boxes or points), so lines shared by neighbours are fetched into one L2 instead of eight.
-// The grid must be a multiple of 8 (xcd_grid); blocks past the logical count return.
-// Used where the work per block is uniform (L2P): with uneven leaves (P2P, P2M) a
-// contiguous range per XCD unbalances the XCDs, which costs more than the L2 misses.
-__device__ __forceinline__ int xcd_block() {
-  const int n = (int)gridDim.x;
-  return (int)(blockIdx.x % 8) * (n / 8) + (int)(blockIdx.x / 8);
-}
-static inline int xcd_grid(int nblk) { return (nblk + 7) & ~7; }
-
 // [start, end) of the sorted points inside box (ix, iy) of level l, from the leaf table.
 __device__ __forceinline__ void box_range(const int* __restrict__ leaf_start, int L, int l, int ix, int iy,
                                           int& s, int& e) {
@@ -298,7 +286,7 @@ __global__ __launch_bounds__(64) void k_p2m(int L, int p, const int* __restrict_
 #pragma unroll
   for (int o = 0; o < NO; ++o) {
     const int t = lane + 64 * o;
-    if (t < 2 * KMAX) mp[(size_t)t * NBP + pb] = (t % KMAX) < p ? acc[o] : make_double2(0, 0);
+    if (t < 2 * KMAX) mp[(size_t)pb * 2 * KMAX + t] = (t % KMAX) < p ? acc[o] : make_double2(0, 0);
   }
 }
 
@@ -375,8 +363,9 @@ __global__ __launch_bounds__(64) void k_p2m_wide(int L, int p, const int* __rest
   if (lane < KMAX) {
     const int pb = pidx(ix, iy, W);
     const bool in = lane < p;
-    mp[(size_t)lane * NBP + pb] = in ? a0 : make_double2(0, 0);
-    mp[(size_t)(KMAX + lane) * NBP + pb] = in ? a1 : make_double2(0, 0);
+    double2* box = mp + (size_t)pb * 2 * KMAX;
+    box[lane] = in ? a0 : make_double2(0, 0);
+    box[KMAX + lane] = in ? a1 : make_double2(0, 0);
   }
 }
 
@@ -626,10 +615,10 @@ __device__ __forceinline__ void xlat_body(const XlatLevels& LS, const int* __res
   // offset outside the level reads 0 instead of faulting.
   const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
       (void*)(src + LS.offs[e]), (short)0, (int)(uint32_t)((size_t)2 * KMAX * NBPs * 16), 0x00020000);
-  const int row_bytes = 4 * NBPs * 16;  // k -> k + 4
-  const int lane_row = (s * KMAX + kr) * NBPs * 16;
+  const int row_bytes = 4 * 16;  // k -> k + 4 within the box's coefficients
+  const int lane_row = (s * KMAX + kr) * 16;
   auto load_src = [&](int o, double2* a) {
-    const int off = lane_row + src_box(o) * 16;
+    const int off = lane_row + src_box(o) * (2 * KMAX * 16);
 #pragma unroll
     for (int ks = 0; ks < KS; ++ks)
       a[ks] = __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(rsrc, off, ks * row_bytes, 0));
@@ -759,13 +748,13 @@ __device__ __forceinline__ void xlat_body(const XlatLevels& LS, const int* __res
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int m = 16 * mt + kr + 4 * r;
-        const size_t at = 2 * ((size_t)(s * KMAX + m) * NBPt + pt) + comp;
+        const size_t at = 2 * ((size_t)pt * 2 * KMAX + s * KMAX + m) + comp;
         out[at] = MODE == XL_L2L ? out[at] + lam[mt][r] : lam[mt][r];
       }
 #pragma unroll
     for (int q = 0; q < TR / 4; ++q) {
       const int m = 16 * MT + kr + 4 * q;
-      const size_t at = 2 * ((size_t)(s * KMAX + m) * NBPt + pt) + comp;
+      const size_t at = 2 * ((size_t)pt * 2 * KMAX + s * KMAX + m) + comp;
       out[at] = MODE == XL_L2L ? out[at] + tlam[q] : tlam[q];
     }
   }
@@ -788,59 +777,98 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 }
 #undef EB_XLAT_ARGS
 
-// L2P at the finest level (SLLVALDR, cadapdrop2.f:331-348): one thread per point.
+// L2P at the finest level (SLLVALDR, cadapdrop2.f:331-348):
 //   GW_far = phi + sgn (z conj(phi') + conj(psi))
 // With PARENT the leaf's own expansion holds only its interaction-list terms, and the
 // parent's complete local expansion is evaluated at the point as well: the same sum as a
 // level-L L2L followed by one evaluation (the L2L shift is exact), without writing the
 // shifted expansions of every leaf.
+//
+// Work items: the points under one level-(L-1) box (four sibling leaves and their parent),
+// ceil(P / L2P_T) items of nearly equal ranges of sorted points; item = (parent Morton key,
+// first point, end point).  One wave per item stages the five local expansions it needs into
+// LDS with coalesced loads, then lane i evaluates point i from LDS: the ~20 points of a leaf
+// share one set of coefficient loads instead of each lane loading all 4p rows itself.
+// The near field (k_p2p) has already written RSCAL * (its sum) to out[perm[i]]; this kernel
+// adds RSCAL * (the far field) there, so the near field can run beside the far-field passes.
+constexpr int L2P_T = 64;
+
+static size_t l2p_item_cap(int n, int L) { return ((size_t)1 << (2 * (L - 1))) + (size_t)n / L2P_T + 1; }
+
+__global__ void k_l2p_count(int npar, const int* __restrict__ leaf_start, int* __restrict__ cnt) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= npar) return;
+  const int P = leaf_start[(m + 1) << 2] - leaf_start[m << 2];
+  cnt[m] = (P + L2P_T - 1) / L2P_T;
+}
+
+__global__ void k_l2p_items(int npar, const int* __restrict__ leaf_start, const int* __restrict__ off,
+                            int4* __restrict__ items) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= npar) return;
+  const int s = leaf_start[m << 2], P = leaf_start[(m + 1) << 2] - s;
+  const int o = off[m], np = off[m + 1] - o;
+  for (int q = 0; q < np; ++q)
+    items[o + q] = make_int4(m, s + (int)((long)P * q / np), s + (int)((long)P * (q + 1) / np), 0);
+}
+
 template <int KMAX, bool PARENT>
-__global__ __launch_bounds__(256) void k_l2p(int n, int L, int p, const uint32_t* __restrict__ keys,
-                                             const double2* __restrict__ zs, const double2* __restrict__ lo, int W,
-                                             int NBP, const double2* __restrict__ lop, int Wp, int NBPp, double sgn,
-                                             double2* __restrict__ gwf) {
-  const int i = xcd_block() * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+__global__ __launch_bounds__(L2P_T) void k_l2p(int L, int p, const int4* __restrict__ items,
+                                               const int* __restrict__ nitems, const uint32_t* __restrict__ keys,
+                                               const double2* __restrict__ zs, const double2* __restrict__ lo, int W,
+                                               const double2* __restrict__ lop, int Wp, double sgn,
+                                               const int32_t* __restrict__ perm, const Geom* __restrict__ g,
+                                               double2* __restrict__ out) {
+  __shared__ double2 C[5][2][KMAX];  // the four leaves' locals (child c = x bit | y bit << 1), the parent's
+  if ((int)blockIdx.x >= *nitems) return;
+  const int4 it = items[blockIdx.x];
+  const int px = (int)compact2((uint32_t)it.x), py = (int)compact2((uint32_t)it.x >> 1);
+  const int nb = PARENT ? 5 : 4;
+  for (int t = threadIdx.x; t < nb * 2 * p; t += L2P_T) {
+    const int b = t / (2 * p), r = t - b * 2 * p, sr = r / p, m = r - sr * p;
+    const double2* src = b < 4 ? lo + (size_t)pidx(2 * px + (b & 1), 2 * py + (b >> 1), W) * 2 * KMAX
+                               : lop + (size_t)pidx(px, py, Wp) * 2 * KMAX;
+    C[b][sr][m] = src[sr * KMAX + m];
+  }
+  __syncthreads();
+  const int i = it.y + (int)threadIdx.x;
+  if (i >= it.z) return;
   const uint32_t key = keys[i];
   const int ix = (int)compact2(key), iy = (int)compact2(key >> 1);
+  const int c = (int)(key & 3u);
   const int nside = 1 << L;
   const double h = 64.0 / nside, ir = 2.0 / h;
   const double2 z = zs[i];
   const double2 u = make_double2((z.x - (ix + 0.5) * h) * ir, (z.y - (iy + 0.5) * h) * ir);
-  const int pb = pidx(ix, iy, W);
+  const double hp = 2.0 * h, irp = 2.0 / hp;
+  const double2 up = make_double2((z.x - (px + 0.5) * hp) * irp, (z.y - (py + 0.5) * hp) * irp);
   double2 phi = make_double2(0, 0), dphi = make_double2(0, 0), psi = make_double2(0, 0);
-  // rows p..KMAX-1 of a local expansion are zero (and not written by every translation)
+  double2 f = make_double2(0, 0), df = make_double2(0, 0), gq = make_double2(0, 0);
+  // both Horner chains in one loop; rows p..KMAX-1 of a local expansion are zero
 #pragma unroll 4
   for (int m = p - 1; m >= 0; --m) {
-    const double2 l1 = lo[(size_t)m * NBP + pb];
-    const double2 l2 = lo[(size_t)(KMAX + m) * NBP + pb];
-    dphi = cadd(cmul(dphi, u), phi);
-    phi = cadd(cmul(phi, u), l1);
-    psi = cadd(cmul(psi, u), l2);
+    dphi = cmuladd(dphi, u, phi);
+    phi = cmuladd(phi, u, C[c][0][m]);
+    psi = cmuladd(psi, u, C[c][1][m]);
+    if (PARENT) {
+      df = cmuladd(df, up, f);
+      f = cmuladd(f, up, C[4][0][m]);
+      gq = cmuladd(gq, up, C[4][1][m]);
+    }
   }
   double2 phip = cscale(dphi, ir);
   if (PARENT) {
-    const int px = ix >> 1, py = iy >> 1;
-    const double hp = 2.0 * h, irp = 2.0 / hp;
-    const double2 up = make_double2((z.x - (px + 0.5) * hp) * irp, (z.y - (py + 0.5) * hp) * irp);
-    const int pp = pidx(px, py, Wp);
-    double2 f = make_double2(0, 0), df = make_double2(0, 0), g = make_double2(0, 0);
-#pragma unroll 4
-    for (int m = p - 1; m >= 0; --m) {
-      const double2 l1 = lop[(size_t)m * NBPp + pp];
-      const double2 l2 = lop[(size_t)(KMAX + m) * NBPp + pp];
-      df = cadd(cmul(df, up), f);
-      f = cadd(cmul(f, up), l1);
-      g = cadd(cmul(g, up), l2);
-    }
     phi = cadd(phi, f);
-    psi = cadd(psi, g);
+    psi = cadd(psi, gq);
     phip = cadd(phip, cscale(df, irp));
   }
   // z conj(phi') + conj(psi)
   const double2 zc = cmul(z, cconj(phip));
   const double2 t = make_double2(zc.x + psi.x, zc.y - psi.y);
-  gwf[i] = make_double2(phi.x + sgn * t.x, phi.y + sgn * t.y);
+  const double rs = g->rscal;
+  const int o = perm[i];
+  const double2 nearv = out[o];
+  out[o] = make_double2(nearv.x + (phi.x + sgn * t.x) * rs, nearv.y + (phi.y + sgn * t.y) * rs);
 }
 
 
@@ -1272,6 +1300,10 @@ struct eb_fmm_plan_s {
   int* p2p_cnt = nullptr;
   int* p2p_off = nullptr;
   int* p2p_tmp = nullptr;
+  int4* l2p_items = nullptr;  // far-field evaluation items and their count (after the scan total)
+  int* l2p_cnt = nullptr;
+  int* l2p_off = nullptr;
+  int* l2p_tmp = nullptr;
   int* ntbox = nullptr;
   double2* dir_part = nullptr;  // direct mode: 2 x n partial sums per source chunk
   int dir_sch = 0, dir_nch = 0;
@@ -1292,19 +1324,29 @@ struct eb_fmm_plan_s {
   hipEvent_t ev[8] = {};
   double stage_ms[7] = {0};
   hipEvent_t ev_switch = nullptr;  // orders a new stream after the plan's previous one
+  // the near field runs on st2 beside the far-field passes on st (fork after the charges,
+  // join before L2P, which adds the far field to the near field's output)
+  hipStream_t st2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
 
 static void plan_free(eb_fmm_plan P) {
   if (!P) return;
   void* ptrs[] = {P->geom, P->bbpart, P->zs0, P->zs, P->wdzs, P->qa, P->qb, P->gwf, P->keys, P->keys2,
                   P->perm, P->vals2, P->sorttmp, P->leaf_start, P->p2p_items, P->p2p_cnt, P->p2p_off,
-                  P->p2p_tmp, P->ntbox, P->m2l_list, P->m2l_count, P->m2l_ccnt, P->mp, P->lo, P->dMat,
+                  P->p2p_tmp, P->l2p_items, P->l2p_cnt, P->l2p_off, P->l2p_tmp, P->ntbox, P->m2l_list, P->m2l_count, P->m2l_ccnt, P->mp, P->lo, P->dMat,
                   P->dU, P->dNU, P->dPM, P->dQM, P->dPL, P->dQL, P->doffs, P->dir_part};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   for (auto& e : P->ev)
     if (e) (void)hipEventDestroy(e);
   if (P->ev_switch) (void)hipEventDestroy(P->ev_switch);
+  if (P->ev_fork) (void)hipEventDestroy(P->ev_fork);
+  if (P->ev_join) (void)hipEventDestroy(P->ev_join);
+  if (P->st2) {
+    (void)hipStreamSynchronize(P->st2);
+    (void)hipStreamDestroy(P->st2);
+  }
   delete P;
 }
 
@@ -1369,6 +1411,21 @@ static int build_p2p_items(eb_fmm_plan P) {
   return EB_OK;
 }
 
+// far-field evaluation items of the current leaves; their count stays on the device
+// (l2p_off[npar])
+static int build_l2p_items(eb_fmm_plan P) {
+  if (!P->l2p_items) return EB_OK;
+  hipStream_t st = P->st;
+  const int npar = 1 << (2 * (P->L - 1));
+  hipLaunchKernelGGL(k_l2p_count, dim3((npar + 255) / 256), dim3(256), 0, st, npar, P->leaf_start, P->l2p_cnt);
+  const int rc = scan_exclusive(P->l2p_cnt, P->l2p_off, npar, true, P->l2p_tmp, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_l2p_items, dim3((npar + 255) / 256), dim3(256), 0, st, npar, P->leaf_start, P->l2p_off,
+                     P->l2p_items);
+  EB_HIP_CHECK(hipGetLastError());
+  return EB_OK;
+}
+
 static int build_tree(eb_fmm_plan P, const double* x, const double* y, const double* wdz) {
   hipStream_t st = P->st;
   const int n = P->n, L = P->L;
@@ -1386,6 +1443,7 @@ static int build_tree(eb_fmm_plan P, const double* x, const double* y, const dou
   hipLaunchKernelGGL(k_gather, dim3((n + 255) / 256), dim3(256), 0, st, n, P->perm, P->zs0,
                      (const double2*)wdz, P->zs, P->wdzs);
   if ((rc = build_p2p_items(P)) != EB_OK) return rc;
+  if ((rc = build_l2p_items(P)) != EB_OK) return rc;
   if (P->m2l_list) {
     hipLaunchKernelGGL(k_m2l_count, dim3(4 * P->m2l_nb), dim3(1024), 0, st, L, P->m2l_nb, P->leaf_start,
                        P->m2l_ccnt);
@@ -1443,7 +1501,6 @@ extern "C" int eb_fmm_plan_create(eb_fmm_plan* out, int32_t n, const double* x, 
   EB_TRY(dmalloc(&P->wdzs, n));
   EB_TRY(dmalloc(&P->qa, n));
   EB_TRY(dmalloc(&P->qb, n));
-  EB_TRY(dmalloc(&P->gwf, n));
   EB_TRY(dmalloc(&P->keys, n));
   EB_TRY(dmalloc(&P->keys2, n));
   EB_TRY(dmalloc(&P->perm, n));
@@ -1460,6 +1517,11 @@ extern "C" int eb_fmm_plan_create(eb_fmm_plan* out, int32_t n, const double* x, 
     EB_TRY(dmalloc(&P->dir_part, (size_t)2 * P->dir_nch * n));
   }
   if (mode == EB_MODE_FMM && L >= 2) {
+    const int npar = 1 << (2 * (L - 1));
+    EB_TRY(dmalloc(&P->l2p_items, l2p_item_cap(n, L)));
+    EB_TRY(dmalloc(&P->l2p_cnt, npar));
+    EB_TRY(dmalloc(&P->l2p_off, npar + 1));
+    EB_TRY(dmalloc(&P->l2p_tmp, scan_temp_ints(npar)));
     size_t cap = 0;
     for (int l = 2; l <= L; ++l) cap += ((size_t)1 << (2 * (l - 1))) + 3;
     P->m2l_cap = (int)cap;
@@ -1474,6 +1536,11 @@ extern "C" int eb_fmm_plan_create(eb_fmm_plan* out, int32_t n, const double* x, 
   EB_TRY(build_tree(P, x, y, wdz));
 
   if (mode == EB_MODE_FMM && L >= 2) {
+    EB_TRY(hipStreamCreateWithFlags(&P->st2, hipStreamNonBlocking) == hipSuccess &&
+                   hipEventCreateWithFlags(&P->ev_fork, hipEventDisableTiming) == hipSuccess &&
+                   hipEventCreateWithFlags(&P->ev_join, hipEventDisableTiming) == hipSuccess
+               ? EB_OK
+               : (eb_set_error("eb_fmm_plan_create: stream or event creation failed"), EB_ERR_HIP));
     size_t off = 0;
     for (int l = 2; l <= L; ++l) {
       const int W = (1 << l) + 6;
@@ -1573,8 +1640,18 @@ static void launch_xlat(eb_fmm_plan P, const XlatLevels& LS, const double2* src,
   }
 }
 
+// the near field: RSCAL * (the near sum) into out[perm[i]], on stream s
+static void launch_p2p(eb_fmm_plan P, hipStream_t s, double2* out) {
+  hipLaunchKernelGGL(k_p2p, dim3((unsigned)p2p_item_cap(P->n, P->L, P->p2p_tmax)), dim3(64), 0, s, P->L, P->leaf_start,
+                     P->p2p_items, P->p2p_off + (1 << (2 * P->L)), P->zs, P->qa, P->qb, (const double2*)nullptr, P->perm,
+                     P->geom, P->sgn, out);
+}
+
+// The far-field passes on the plan's stream, then L2P, which adds RSCAL * (the far field) to the
+// near field's output: after the join with the near field's stream, or (stage timing) after
+// running the near field in place on the plan's stream.
 template <int K>
-static void launch_far(eb_fmm_plan P) {
+static void launch_far(eb_fmm_plan P, double2* out) {
   hipStream_t st = P->st;
   const int L = P->L;
   if (P->timing) (void)hipEventRecord(P->ev[1], st);
@@ -1596,18 +1673,23 @@ static void launch_far(eb_fmm_plan P) {
   // the shift into the leaves is left to L2P, which evaluates the parents' expansions directly
   for (int l = 3; l <= L - 1; ++l)
     launch_xlat<XL_L2L, K>(P, xlat_levels(P, l, l, -1), P->lo, P->lo, P->dPL, P->dQL);
-  if (P->timing) (void)hipEventRecord(P->ev[5], st);
+  if (P->timing) {
+    (void)hipEventRecord(P->ev[5], st);
+    launch_p2p(P, st, out);
+    (void)hipEventRecord(P->ev[6], st);
+  } else {
+    (void)hipStreamWaitEvent(st, P->ev_join, 0);
+  }
+  const dim3 lgrid((unsigned)l2p_item_cap(P->n, L));
+  const int* nl2p = P->l2p_off + (1 << (2 * (L - 1)));
   if (L >= 3)
-    hipLaunchKernelGGL((k_l2p<K, true>), dim3(xcd_grid((P->n + 255) / 256)), dim3(256), 0, st, P->n, L, P->p,
-                       P->keys,
-                       P->zs, P->lo + P->lvl_off[L], P->lvl_W[L], P->lvl_NBP[L], P->lo + P->lvl_off[L - 1],
-                       P->lvl_W[L - 1], P->lvl_NBP[L - 1], P->sgn, P->gwf);
+    hipLaunchKernelGGL((k_l2p<K, true>), lgrid, dim3(L2P_T), 0, st, L, P->p, P->l2p_items, nl2p, P->keys, P->zs,
+                       P->lo + P->lvl_off[L], P->lvl_W[L], P->lo + P->lvl_off[L - 1], P->lvl_W[L - 1], P->sgn, P->perm,
+                       P->geom, out);
   else
-    hipLaunchKernelGGL((k_l2p<K, false>), dim3(xcd_grid((P->n + 255) / 256)), dim3(256), 0, st, P->n, L, P->p,
-                       P->keys,
-                       P->zs, P->lo + P->lvl_off[L], P->lvl_W[L], P->lvl_NBP[L], P->lo + P->lvl_off[L],
-                       P->lvl_W[L], P->lvl_NBP[L], P->sgn, P->gwf);
-  if (P->timing) (void)hipEventRecord(P->ev[6], st);
+    hipLaunchKernelGGL((k_l2p<K, false>), lgrid, dim3(L2P_T), 0, st, L, P->p, P->l2p_items, nl2p, P->keys, P->zs,
+                       P->lo + P->lvl_off[L], P->lvl_W[L], P->lo + P->lvl_off[L], P->lvl_W[L], P->sgn, P->perm,
+                       P->geom, out);
 }
 
 extern "C" int eb_fmm_apply(eb_fmm_plan P, const double* dens, double* gradw, void* stream) {
@@ -1618,6 +1700,7 @@ extern "C" int eb_fmm_apply(eb_fmm_plan P, const double* dens, double* gradw, vo
   if (int rc = use_stream(P, stream)) return rc;
   hipStream_t st = P->st;
   const int n = P->n;
+  double2* out = (double2*)gradw;
   if (P->timing) {
     for (auto& e : P->ev)
       if (!e) EB_HIP_CHECK(hipEventCreate(&e));
@@ -1631,33 +1714,36 @@ extern "C" int eb_fmm_apply(eb_fmm_plan P, const double* dens, double* gradw, vo
     hipLaunchKernelGGL(k_direct_part, dim3((n + DIR_T - 1) / DIR_T, P->dir_nch), dim3(DIR_T), 0, st, n, P->dir_sch,
                        P->zs, P->qa, P->qb, P->dir_part);
     hipLaunchKernelGGL(k_direct_sum, dim3((n + 255) / 256), dim3(256), 0, st, n, P->dir_nch, P->dir_part, P->perm,
-                       P->geom, P->sgn, (double2*)gradw);
-  } else {
-    const bool far = P->L >= 2;
-    if (far) {
-      switch (P->kmax) {
-        case 16: launch_far<16>(P); break;
-        case 32: launch_far<32>(P); break;
-        case 48: launch_far<48>(P); break;
-        default: launch_far<64>(P); break;
-      }
-    } else if (P->timing) {
-      for (int i = 1; i <= 6; ++i) (void)hipEventRecord(P->ev[i], st);
+                       P->geom, P->sgn, out);
+  } else if (P->L >= 2) {
+    if (!P->timing) {
+      // the near field beside the far-field passes: fork after the charges
+      EB_HIP_CHECK(hipEventRecord(P->ev_fork, st));
+      EB_HIP_CHECK(hipStreamWaitEvent(P->st2, P->ev_fork, 0));
+      launch_p2p(P, P->st2, out);
+      EB_HIP_CHECK(hipEventRecord(P->ev_join, P->st2));
     }
-    hipLaunchKernelGGL(k_p2p, dim3((unsigned)p2p_item_cap(P->n, P->L, P->p2p_tmax)), dim3(64), 0, st, P->L,
-                       P->leaf_start,
-                       P->p2p_items, P->p2p_off + (1 << (2 * P->L)), P->zs, P->qa, P->qb, far ? P->gwf : nullptr,
-                       P->perm, P->geom, P->sgn, (double2*)gradw);
+    switch (P->kmax) {
+      case 16: launch_far<16>(P, out); break;
+      case 32: launch_far<32>(P, out); break;
+      case 48: launch_far<48>(P, out); break;
+      default: launch_far<64>(P, out); break;
+    }
+  } else {
+    if (P->timing)
+      for (int i = 1; i <= 5; ++i) (void)hipEventRecord(P->ev[i], st);
+    launch_p2p(P, st, out);
+    if (P->timing) (void)hipEventRecord(P->ev[6], st);
   }
   EB_HIP_CHECK(hipGetLastError());
   if (P->timing) {
     EB_HIP_CHECK(hipEventRecord(P->ev[7], st));
     EB_HIP_CHECK(hipEventSynchronize(P->ev[7]));
-    for (int i = 0; i < 7; ++i) {
-      float ms = 0;
-      EB_HIP_CHECK(hipEventElapsedTime(&ms, P->ev[i], P->ev[i + 1]));
-      P->stage_ms[i] = ms;
-    }
+    float ms[7] = {0};
+    for (int i = 0; i < 7; ++i) EB_HIP_CHECK(hipEventElapsedTime(&ms[i], P->ev[i], P->ev[i + 1]));
+    // events: charges, P2M, M2M, M2L, L2L, P2P, L2P; reported as charges, P2M, M2M, M2L, L2L, L2P, P2P
+    const int order[7] = {0, 1, 2, 3, 4, 6, 5};
+    for (int i = 0; i < 7; ++i) P->stage_ms[order[i]] = ms[i];
   }
   return EB_OK;
 }
