@@ -8,9 +8,12 @@ fixed seed; kernel = CADAPDR's GW (DR signs), FP64, evaluated at every point.
 One step = one complete FMM evaluation from raw device-resident coordinates: tree build
 (binning, sort, leaf table) + P2M, M2M, M2L+L2L, L2P, P2P.  `value` is points per second of
 that whole evaluation (Mpts/s).  The apply-only rate (tree kept, as inside one GMRES solve)
-is reported beside it.  LEVMAX and p are the FMM autotuner's choice for 1e-10 relative
-accuracy (2-norm against the same tree at p = 60) unless given; the accuracy of the timed
-configuration and the fixed LEVMAX 8 / p = 32 point are reported beside it.
+is reported beside it, with every stage's time, FP64 fraction and GB/s.  LEVMAX and p are the
+FMM autotuner's choice for 1e-10 relative accuracy (2-norm against the O(N^2) sum at 1024
+targets) unless given.  The accuracy of the timed configuration is measured against the
+tree-free O(N^2) sum on the GPU at 1024 seeded targets, and in the CPU leg against the
+long-double sum of the C restatement at the same targets.  The same measurement of the second
+FMM workload (2^20 uniform points) and the fixed LEVMAX 8 / p = 32 point are reported beside it.
 
 Prints ONE JSON line.  Single GPU only.
 """
@@ -59,10 +62,14 @@ def leg(name, fn, *a):
     return r
 
 
+STAGES = ("charges", "p2m", "m2m", "m2l", "l2l", "l2p", "p2p")
+
+
 def flops_model(info, n, p):
     """Algorithmic flop convention: complex add 2, complex*real 2, complex*complex 6,
     reciprocal 6, divide 12; per stage counts from the tree."""
     return {
+        "charges": 0.0,
         "p2p": 50.0 * info["near_pairs"],
         "m2l": (8.0 * p * p + 34.0 * p) * info["m2l_pairs"],
         "p2m": 26.0 * p * n,
@@ -70,6 +77,55 @@ def flops_model(info, n, p):
         "m2m": (8.0 * p * p + 30.0 * p) * info["boxes"],
         "l2l": (8.0 * p * p + 30.0 * p) * info["boxes"],
     }
+
+
+def bytes_model(info, n, p):
+    """Algorithmic HBM bytes per stage (each datum read or written once): points carry x, y (16 B),
+    the weighted tangent (16 B), the density (16 B) and GW (16 B); an expansion of one box is
+    2 series x p complex coefficients (32 p B).
+      charges: density, weight, permutation in; two charges out            68 B per point
+      P2M:     sorted points and charges in; one multipole per leaf out    48 B per point + 32 p per leaf
+      M2M:     every child multipole in, every parent out                 64 p per box
+      M2L:     every source multipole in, every target local out          64 p per box
+      L2L:     parent local in, child local read and written               96 p per box
+      L2P:     own and parent locals per leaf, point in, GW read and written 56 B per point + 64 p per leaf
+      P2P:     points and charges in, GW out                               64 B per point"""
+    leaves, boxes = info["leaves"], info["boxes"]
+    return {
+        "charges": 68.0 * n,
+        "p2m": 48.0 * n + 32.0 * p * leaves,
+        "m2m": 64.0 * p * boxes,
+        "m2l": 64.0 * p * boxes,
+        "l2l": 96.0 * p * boxes,
+        "l2p": 56.0 * n + 64.0 * p * leaves,
+        "p2p": 64.0 * n,
+    }
+
+
+def stage_table(stage_ms, fl, by):
+    """Per stage: time, algorithmic GFLOP and TFLOP/s with the fraction of the FP64 peak, and
+    algorithmic GB and GB/s with the fraction of the HBM peak."""
+    out = {}
+    for k in STAGES:
+        t = stage_ms[k] * 1e-3
+        out[k] = {"ms": round(stage_ms[k], 4), "gflop": round(fl[k] / 1e9, 4),
+                  "tflops": round(fl[k] / t / 1e12, 3) if t > 0 else None,
+                  "fp64_frac": round(fl[k] / t / 1e12 / PEAK_FP64_TFLOPS, 4) if t > 0 else None,
+                  "gbytes": round(by[k] / 1e9, 4), "gbps": round(by[k] / t / 1e9, 1) if t > 0 else None,
+                  "hbm_frac": round(by[k] / t / 1e9 / PEAK_HBM_GBS, 4) if t > 0 else None}
+    return out
+
+
+def direct_check(z, dz, nd, h, dens, gw, ntgt=1024, seed=7):
+    """Part of the CPU leg: the relative 2-norm difference between the GPU's GW and the
+    long-double O(N^2) sum of the C restatement over all sources, at ntgt random targets."""
+    import oracle
+    tgt = np.sort(np.random.default_rng(seed).choice(z.size, ntgt, replace=False)).astype(np.int32)
+    t0 = time.perf_counter()
+    ref = oracle.direct(oracle.DR, nd, z.real, z.imag, dz, dens, h, tgt)
+    return {"rel_err2": float(np.linalg.norm(gw[tgt] - ref) / np.linalg.norm(ref)), "targets": int(ntgt),
+            "sources": int(z.size), "checker": "long-double O(N^2) sum (oracle.direct)",
+            "s": round(time.perf_counter() - t0, 2)}
 
 
 def cpu_baseline(z, dz, nd, h, dens, levmax, p, ncurves_total, budget_s=20.0):
@@ -561,7 +617,8 @@ def main():
     ap.add_argument("--nterms", type=int, default=0, help="0: the autotuner's choice (1e-10 relative)")
     ap.add_argument("--npts", type=int, default=1 << 20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-check", action="store_true", help="skip the accuracy check against p = 60")
+    ap.add_argument("--no-check", action="store_true", help="skip the fixed LEVMAX 8 / p = 32 comparison point")
+    ap.add_argument("--no-second", action="store_true", help="skip the second FMM workload (uniform points)")
     ap.add_argument("--no-solve", action="store_true", help="skip the Stokes GMRES solve leg")
     ap.add_argument("--no-spectral", action="store_true", help="skip the spectral geometry leg")
     ap.add_argument("--no-sweep", action="store_true", help="skip the batched-versus-plain sweep leg")
@@ -587,7 +644,6 @@ def main():
 
     import torch
     import electricbubbles_amd as eb
-    from electricbubbles_amd.workloads import complex_normal, curves_cfg, uniform_cfg, weights
 
     if not torch.cuda.is_available():
         print("bench.py: no GPU visible", file=sys.stderr)
@@ -601,24 +657,8 @@ def main():
         with os.fdopen(json_fd, "w") as jf:
             jf.write(json.dumps(res) + "\n")
         return
-    rng = np.random.default_rng(20260303)
     n = args.npts
-    if args.workload == "curves":
-        ncurves = n // 256
-        z, dz, nd, h = curves_cfg(ncurves, 256, rng)
-    else:
-        ncurves = 0
-        z, dz, nd, h = uniform_cfg(n, rng)
-    dens = complex_normal(n, rng)
-    p, L = args.nterms, args.levmax
-
-    x = torch.tensor(z.real.copy(), device=dev)
-    y = torch.tensor(z.imag.copy(), device=dev)
-    w = torch.tensor(weights(dz, nd, h), device=dev)
-    d = torch.tensor(dens, device=dev)
-    out = torch.empty(n, dtype=torch.complex128, device=dev)
     stream = torch.cuda.current_stream()
-
     if args.calibrate_copy:
         src = torch.empty(32 << 20, dtype=torch.float64, device=dev).uniform_()
         dst = torch.empty_like(src)
@@ -626,110 +666,11 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    # the autotuner picks the depth and the expansion length for these points at 1e-10
-    # relative (2-norm, against the same FMM at p = 60) unless both are given
-    tuned = None
-    if L <= 0 or p <= 0 or not args.no_autotune:
-        from electricbubbles_amd import autotune
-        t0 = time.perf_counter()
-        log("autotune: start")
-        tr = autotune.tune_fmm(x, y, w, d, target=1e-10)
-        log(f"autotune: LEVMAX {tr['levmax']}, p {tr['nterms']}")
-        tuned = {"target_rel": 1e-10, "levmax": tr["levmax"], "nterms": tr["nterms"], "p2p_tile": tr["p2p_tile"],
-                 "apply_ms": tr["ms"], "err_vs_p60": tr["err"], "tuning_s": round(time.perf_counter() - t0, 3),
-                 "p_scan": tr["p_scan"], "p_scan_levmax": tr["p_scan_levmax"], "tile_scan": tr["tile_scan"],
-                 "candidates": [{k: c[k] for k in ("levmax", "points_per_leaf", "ms")} for c in tr["candidates"]]}
-        if L <= 0:
-            L = tr["levmax"]
-        if p <= 0:
-            p = tr["nterms"]
-
-    tile = tuned["p2p_tile"] if tuned and (L, p) == (tuned["levmax"], tuned["nterms"]) else 42
-    log(f"FMM: {n} points, LEVMAX {L}, p {p}, near-field tile {tile}")
-    plan = eb.FMMPlan(x, y, w, L, p, eb.VARIANT_DR, eb.MODE_FMM, stream=stream)
-    plan.set_p2p_tile(tile)
-    for _ in range(args.warmup):
-        plan.rebuild(x, y, w)
-        plan.apply(d, out)
-    torch.cuda.synchronize()
-
-    # timed: K complete evaluations (tree build + far + near)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        plan.rebuild(x, y, w)
-        plan.apply(d, out)
-    torch.cuda.synchronize()
-    t_full = (time.perf_counter() - t0) / args.steps
-
-    # apply only (tree kept: the GMRES inner loop)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        plan.apply(d, out)
-    torch.cuda.synchronize()
-    t_apply = (time.perf_counter() - t0) / args.steps
-
-    # per-stage times with HIP events on the plan's stream (outside the timed loops)
-    plan.enable_timing(True)
-    reps = 5
-    acc = None
-    for _ in range(reps):
-        plan.apply(d, out)
-        s = plan.stage_ms()
-        acc = s if acc is None else {k: acc[k] + s[k] for k in s}
-    stage = {k: v / reps for k, v in acc.items()}
-    plan.enable_timing(False)
-    info = plan.info()
-    fl = flops_model(info, n, p)
-    dom = max(("p2p", "m2l"), key=lambda k: stage[k])
-    achieved = fl[dom] / (stage[dom] * 1e-3) / 1e12
-    traffic = PROFILED_TRAFFIC_BYTES.get((args.workload, n, L, p), {}).get(dom)
-    total_fl = sum(fl.values())
-
-    # the adaptive tree of the same points (DAPIF2's rules, at most 40 points per childless box)
-    from electricbubbles_amd.atree import AdaptiveTree
-    at = AdaptiveTree(x, y, 40)
-    at.close()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    reps = 3
-    for _ in range(reps):
-        at = AdaptiveTree(x, y, 40)
-        at_info = {"napb": 40, "boxes": at.nbox, "levels": at.nlev, "max_leaf": at.maxp}
-        at.close()
-    torch.cuda.synchronize()
-    at_info["build_ms"] = round((time.perf_counter() - t0) / reps * 1e3, 3)
-
-    # accuracy of the timed configuration: relative 2-norm difference from the same tree at
-    # p = 60 (the reference's longest expansion), whose own truncation is below 1e-15; and the
-    # fixed LEVMAX 8 / p = 32 point for comparison
-    check, fixed32 = None, None
-    if not args.no_check:
-        ref_plan = eb.FMMPlan(x, y, w, L, 60, eb.VARIANT_DR, eb.MODE_FMM, stream=stream)
-        ref60 = ref_plan.apply(d)
-        check = float((out - ref60).norm() / ref60.norm())
-        ref_plan.close()
-        if (L, p) != (8, 32):
-            p32 = eb.FMMPlan(x, y, w, 8, 32, eb.VARIANT_DR, eb.MODE_FMM, stream=stream)
-            o32 = p32.apply(d)
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            for _ in range(args.steps):
-                p32.rebuild(x, y, w)
-                p32.apply(d, o32)
-            torch.cuda.synchronize()
-            t32f = (time.perf_counter() - t0) / args.steps
-            t0 = time.perf_counter()
-            for _ in range(args.steps):
-                p32.apply(d, o32)
-            torch.cuda.synchronize()
-            t32a = (time.perf_counter() - t0) / args.steps
-            r60 = eb.FMMPlan(x, y, w, 8, 60, eb.VARIANT_DR, eb.MODE_FMM, stream=stream)
-            rr = r60.apply(d)
-            fixed32 = {"levmax": 8, "nterms": 32, "value": round(n / t32f / 1e6, 3),
-                       "apply_only_value": round(n / t32a / 1e6, 3),
-                       "err2_vs_p60": float((o32 - rr).norm() / rr.norm())}
-            r60.close()
-            p32.close()
+    head, head_host = leg(f"FMM {args.workload}", fmm_workload, torch, eb, dev, args.workload, n, args, stream)
+    other = None
+    if not args.no_second:
+        kind2 = "uniform" if args.workload == "curves" else "curves"
+        other, other_host = leg(f"FMM {kind2}", fmm_workload, torch, eb, dev, kind2, n, args, stream)
 
     solve = None if args.no_solve else leg("stokes solve", stokes_solve_leg, torch, dev, not args.no_cpu)
     spec = None if args.no_spectral else leg("spectral", spectral_leg, torch, dev, not args.no_cpu)
@@ -740,49 +681,181 @@ def main():
     cpu = None
     if not args.no_cpu:
         log("cpu baseline: start")
-        cpu = cpu_baseline(z, dz, nd, h, dens, L, p, max(ncurves, 1) if ncurves else 4096)
+        z, dz, nd, h, dens, _, _ = head_host
+        cpu = cpu_baseline(z, dz, nd, h, dens, head["levmax"], head["nterms"],
+                           nd.size if args.workload == "curves" else 4096)
+        # the GPU results against the long-double O(N^2) sum of the C restatement (the checker)
+        cpu["accuracy_check"] = {args.workload: direct_check(*head_host[:6])}
+        if other is not None:
+            cpu["accuracy_check"][other["kind"]] = direct_check(*other_host[:6])
         log("cpu baseline: done")
 
-    value = n / t_full / 1e6
+    L, p = head["levmax"], head["nterms"]
+    fl, stage = head["_flops"], head["stage_ms"]
+    dom = max(("p2p", "m2l"), key=lambda k: stage[k])
+    achieved = fl[dom] / (stage[dom] * 1e-3) / 1e12
+    traffic = PROFILED_TRAFFIC_BYTES.get((args.workload, n, L, p), {}).get(dom)
     line = {
         "metric": "FMM Mpts/s @N=2^20",
-        "value": round(value, 3),
+        "value": head["value"],
         "unit": "Mpts/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(t_full * 1e3, 4),
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"FMM alone, {n} points on {ncurves} circles x 256 (r U[0.01,0.05], "
-                               f"centres U[0,1]^2)" if ncurves else f"FMM alone, {n} uniform points",
-                   "kernel": "CADAPDR GW (DR signs), sources = targets",
-                   "levmax": L, "nterms": p, "p2p_tile": tile, "leaves_nonempty": info["leaves"],
-                   "max_leaf": info["max_leaf"]},
-        "apply_only": {"value": round(n / t_apply / 1e6, 3), "unit": "Mpts/s", "ms": round(t_apply * 1e3, 4)},
+        "config": head["config"],
+        "apply_only": head["apply_only"],
         "stage_ms": {k: round(v, 4) for k, v in stage.items()},
-        "adaptive_tree": at_info,
-        "autotune": tuned,
+        "stages": head["stages"],
+        "fp64_fraction_total": head["fp64_fraction_total"],
+        "accuracy": head["accuracy"],
+        "adaptive_tree": head["adaptive_tree"],
+        "autotune": head["autotune"],
+        "fixed_levmax8_p32": head.get("fixed_levmax8_p32"),
+        "second_workload": {k: v for k, v in other.items() if not k.startswith("_")} if other else None,
         "stokes_solve": solve,
         "spectral": spec,
         "sweep": sweep,
         "field": fld,
         "adjoint": adj,
-        "fp64_fraction_total": round(total_fl / (t_apply) / 1e12 / PEAK_FP64_TFLOPS, 4),
         "roofline": {"bound": "mfma", "kernel": dom, "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
-                     "flops_per_launch": fl[dom]},
-        "accuracy_err2_vs_p60": check,
-        "fixed_levmax8_p32": fixed32,
+                     "flops_per_launch": fl[dom],
+                     "timing": "HIP events on the plan's stream, mean of 5 applies after the timed loops"},
         "cpu_baseline": cpu,
     }
     sys.stdout.flush()
     with os.fdopen(json_fd, "w") as jf:
         jf.write(json.dumps(line) + "\n")
+
+
+def fmm_workload(torch, eb, dev, kind, n, args, stream):
+    """One FMM-alone workload (2^20 points by default): the circles (4096 x 256 points, radii
+    U[0.01, 0.05], centres U[0,1]^2) or uniform points in the unit square, CN(0,1) densities,
+    CADAPDR's kernel.  The autotuner's depth and length for 1e-10 unless given; K complete
+    evaluations (tree build + apply) timed, then K applies alone, then per-stage HIP events;
+    accuracy against the tree-free O(N^2) sum at 1024 seeded targets (outside the timed loops).
+    Returns the result object and the host copies the CPU leg needs."""
+    from electricbubbles_amd.workloads import complex_normal, curves_cfg, uniform_cfg, weights
+    rng = np.random.default_rng(20260303)
+    if kind == "curves":
+        ncurves = n // 256
+        z, dz, nd, h = curves_cfg(ncurves, 256, rng)
+    else:
+        ncurves = 0
+        z, dz, nd, h = uniform_cfg(n, rng)
+    dens = complex_normal(n, rng)
+    x = torch.tensor(z.real.copy(), device=dev)
+    y = torch.tensor(z.imag.copy(), device=dev)
+    w = torch.tensor(weights(dz, nd, h), device=dev)
+    d = torch.tensor(dens, device=dev)
+    out = torch.empty(n, dtype=torch.complex128, device=dev)
+    p, L = args.nterms, args.levmax
+    tuned = None
+    if L <= 0 or p <= 0 or not args.no_autotune:
+        from electricbubbles_amd import autotune
+        t0 = time.perf_counter()
+        tr = autotune.tune_fmm(x, y, w, d, target=1e-10)
+        log(f"autotune {kind}: LEVMAX {tr['levmax']}, p {tr['nterms']}")
+        tuned = {"target_rel": 1e-10, "check": tr["check"], "levmax": tr["levmax"], "nterms": tr["nterms"],
+                 "p2p_tile": tr["p2p_tile"], "apply_ms": tr["ms"], "err_vs_direct": tr["err"],
+                 "tuning_s": round(time.perf_counter() - t0, 3), "p_scan": tr["p_scan"],
+                 "p_scan_levmax": tr["p_scan_levmax"], "tile_scan": tr["tile_scan"],
+                 "candidates": [{k: c[k] for k in ("levmax", "points_per_leaf", "ms", "err")} for c in tr["candidates"]]}
+        L = tr["levmax"] if L <= 0 else L
+        p = tr["nterms"] if p <= 0 else p
+    tile = tuned["p2p_tile"] if tuned and (L, p) == (tuned["levmax"], tuned["nterms"]) else 42
+    log(f"FMM {kind}: {n} points, LEVMAX {L}, p {p}, near-field tile {tile}")
+    plan = eb.FMMPlan(x, y, w, L, p, eb.VARIANT_DR, eb.MODE_FMM, stream=stream)
+    plan.set_p2p_tile(tile)
+
+    def timed(full, steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            if full:
+                plan.rebuild(x, y, w)
+            plan.apply(d, out)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps
+
+    timed(True, args.warmup)
+    t_full = timed(True, args.steps)  # K complete evaluations (tree build + far + near)
+    t_apply = timed(False, args.steps)  # apply only (tree kept: the GMRES inner loop)
+    # per-stage times with HIP events on the plan's stream (stages run one after another here)
+    plan.enable_timing(True)
+    acc = None
+    for _ in range(5):
+        plan.apply(d, out)
+        st = plan.stage_ms()
+        acc = st if acc is None else {k: acc[k] + st[k] for k in st}
+    stage = {k: v / 5 for k, v in acc.items()}
+    plan.enable_timing(False)
+    plan.apply(d, out)
+    info = plan.info()
+    fl, by = flops_model(info, n, p), bytes_model(info, n, p)
+    # accuracy: the O(N^2) sum over all sources at 1024 seeded targets (tree-free, on the GPU)
+    tgt = torch.tensor(np.sort(np.random.default_rng(7).choice(n, 1024, replace=False)).astype(np.int32), device=dev)
+    ref = eb.direct_subset(x, y, w, d, tgt)
+    err = float((out[tgt.long()] - ref).norm() / ref.norm())
+    gw = out.cpu().numpy()
+    # the adaptive tree of the same points (DAPIF2's rules, at most 40 points per childless box)
+    from electricbubbles_amd.atree import AdaptiveTree
+    at = AdaptiveTree(x, y, 40)
+    at.close()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        at = AdaptiveTree(x, y, 40)
+        at_info = {"napb": 40, "boxes": at.nbox, "levels": at.nlev, "max_leaf": at.maxp}
+        at.close()
+    torch.cuda.synchronize()
+    at_info["build_ms"] = round((time.perf_counter() - t0) / 3 * 1e3, 3)
+    res = {
+        "kind": kind,
+        "value": round(n / t_full / 1e6, 3), "ms_per_step": round(t_full * 1e3, 4),
+        "apply_only": {"value": round(n / t_apply / 1e6, 3), "unit": "Mpts/s", "ms": round(t_apply * 1e3, 4)},
+        "levmax": L, "nterms": p,
+        "config": {"workload": f"FMM alone, {n} points on {ncurves} circles x 256 (r U[0.01,0.05], "
+                               f"centres U[0,1]^2)" if ncurves else f"FMM alone, {n} uniform points in the unit square",
+                   "kernel": "CADAPDR GW (DR signs), sources = targets",
+                   "levmax": L, "nterms": p, "p2p_tile": tile, "leaves_nonempty": info["leaves"],
+                   "max_leaf": info["max_leaf"]},
+        "stage_ms": stage,
+        "stages": stage_table(stage, fl, by),
+        "fp64_fraction_total": round(sum(fl.values()) / t_apply / 1e12 / PEAK_FP64_TFLOPS, 4),
+        "accuracy": {"rel_err2_vs_direct": err, "targets": 1024, "sources": n,
+                     "checker": "O(N^2) sum over all sources on the GPU (eb_direct_subset: no tree, lists or expansions)"},
+        "adaptive_tree": at_info,
+        "autotune": tuned,
+        "_flops": fl,
+    }
     plan.close()
+    if not args.no_check and kind == "curves" and (L, p) != (8, 32):
+        p32 = eb.FMMPlan(x, y, w, 8, 32, eb.VARIANT_DR, eb.MODE_FMM, stream=stream)
+        o32 = p32.apply(d)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            p32.rebuild(x, y, w)
+            p32.apply(d, o32)
+        torch.cuda.synchronize()
+        t32f = (time.perf_counter() - t0) / args.steps
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            p32.apply(d, o32)
+        torch.cuda.synchronize()
+        t32a = (time.perf_counter() - t0) / args.steps
+        res["fixed_levmax8_p32"] = {"levmax": 8, "nterms": 32, "value": round(n / t32f / 1e6, 3),
+                                    "apply_only_value": round(n / t32a / 1e6, 3),
+                                    "rel_err2_vs_direct": float((o32[tgt.long()] - ref).norm() / ref.norm())}
+        p32.close()
+    return res, (z, dz, nd, h, dens, gw, tgt.cpu().numpy())
 
 
 if __name__ == "__main__":

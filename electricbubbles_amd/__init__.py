@@ -47,7 +47,7 @@ EXPORTS = (
     "eb_last_error", "eb_version", "eb_cadapdr", "eb_cadapmod", "eb_cadap", "eb_set_mode", "eb_get_mode",
     "eb_fmm_plan_create", "eb_fmm_plan_rebuild", "eb_fmm_apply", "eb_fmm_plan_destroy", "eb_fmm_plan_info",
     "eb_fmm_plan_ntbox", "eb_fmm_enable_timing", "eb_fmm_stage_ms", "eb_fmm_set_p2p_tile", "eb_cadap_tables",
-    "eb_cadap_lists", "eb_direct_apply_noalloc",
+    "eb_cadap_lists", "eb_direct_apply_noalloc", "eb_direct_subset",
 )
 
 _lib = None
@@ -88,6 +88,7 @@ def lib():
     L.eb_cadap_tables.argtypes = [_i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _ip, _ip, _vp]
     L.eb_cadap_lists.argtypes = [_i32, _vp, _vp, _vp, _vp, _vp]
     L.eb_direct_apply_noalloc.argtypes = [_i32, _vp, _vp, _vp, _vp, _i32, _vp, _vp]
+    L.eb_direct_subset.argtypes = [_i32, _vp, _vp, _vp, _vp, _i32, _i32, _vp, _vp, _vp]
     _lib = L
     return L
 
@@ -309,6 +310,19 @@ def direct_apply_noalloc(x, y, wdz, dens, out, variant=VARIANT_DR, stream=None):
         assert t.is_cuda and t.is_contiguous() and t.numel() == n
     _check(lib().eb_direct_apply_noalloc(n, _ptr(x), _ptr(y), _ptr(wdz), _ptr(dens), int(variant), _ptr(out),
                                          _stream_ptr(stream)))
+    return out
+
+
+def direct_subset(x, y, wdz, dens, targets, variant=VARIANT_DR, stream=None):
+    """GW of the O(N^2) sum over all sources at the listed targets (int32 device tensor of
+    0-based indices): the tree-free checker of FMM accuracy.  Returns complex128 (ntgt,)."""
+    torch = _torch()
+    n = int(x.numel())
+    x, y, wdz, dens = (t.contiguous() for t in (x, y, wdz, dens))
+    tg = targets.to(device=x.device, dtype=torch.int32).contiguous()
+    out = torch.empty(tg.numel(), dtype=torch.complex128, device=x.device)
+    _check(lib().eb_direct_subset(n, _ptr(x), _ptr(y), _ptr(wdz), _ptr(dens), int(variant), int(tg.numel()), _ptr(tg),
+                                  _ptr(out), _stream_ptr(stream)))
     return out
 
 

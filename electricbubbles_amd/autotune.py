@@ -3,8 +3,9 @@
 The reference fixes both by hand: NTERMS = -NINT(ln TOL / ln 2) (cadapdrop.f:126) and LEVMAX
 from its caller (stokesbubb.f:564).  For a given point set and accuracy target this picks
 
-* p: the shortest expansion whose result is within `target` (relative, 2-norm) of the same
-  plan at p = 60, the longest the reference allows (cadapdrop.f:129-133);
+* p: the shortest expansion whose result is within `target` (relative, 2-norm over up to 1024
+  seeded random targets) of the O(N^2) sum over all sources at those targets (eb_direct_subset,
+  which shares nothing with the FMM's tree, lists or expansions);
 * LEVMAX: the depth with the shortest measured apply (tree kept, as inside a GMRES solve)
   among depths whose mean occupancy of the non-empty leaves lies in [lo, hi] points, each
   checked against the same accuracy target;
@@ -21,9 +22,10 @@ import math
 import os
 import time
 
-from . import FMMPlan, MODE_FMM, VARIANT_DR, _torch
+from . import FMMPlan, MODE_FMM, VARIANT_DR, _torch, direct_subset
 
-P_MAX = 60
+P_MAX = 60  # the longest expansion the reference allows (cadapdrop.f:129-133)
+N_CHECK = 1024  # targets of the accuracy check
 _CACHE: dict = {}
 
 
@@ -53,7 +55,8 @@ def tune_fmm(x, y, wdz, dens=None, target=1e-10, variant=VARIANT_DR, nterms=(16,
              occupancy=(4.0, 64.0), tiles=(16, 24, 32, 42, 64, 96, 128), reps=3, cache_file=None, seed=0):
     """Returns {"levmax", "nterms", "p2p_tile", "ms", "err", "candidates": [...]} for the point set (x, y)
     with weights wdz (device tensors as FMMPlan takes them).  dens: the density to time and
-    check with (complex, default: a fixed pseudo-random one)."""
+    check with (complex, default: a fixed pseudo-random one).  err is the relative 2-norm
+    difference from the O(N^2) sum at the check targets."""
     torch = _torch()
     n = int(x.numel())
     key = f"{n}|{variant}|{target:.3e}|{_fingerprint(x, y)}"
@@ -80,16 +83,18 @@ def tune_fmm(x, y, wdz, dens=None, target=1e-10, variant=VARIANT_DR, nterms=(16,
     if not levels:
         L = max(2, min(10, int(round(math.log(max(n, 4) / 16.0, 4)))))
         levels = [(L, float("nan"))]
-    # p: shortest expansion within target of p = P_MAX, at the middle candidate depth
+    # the reference values: the O(N^2) sum at seeded random targets
+    g = torch.Generator(device="cpu").manual_seed(seed + 1)
+    tgt = torch.randperm(n, generator=g)[:min(n, N_CHECK)].sort().values.to(torch.int32).to(x.device)
+    ref = direct_subset(x, y, wdz, dens, tgt, variant)
+    tl = tgt.long()
+    # p: shortest expansion within target of the direct sum, at the middle candidate depth
     Lm = levels[len(levels) // 2][0]
-    ref_plan = FMMPlan(x, y, wdz, Lm, P_MAX, variant=variant, mode=MODE_FMM)
-    ref = ref_plan.apply(dens)
-    ref_plan.close()
     p_sel = P_MAX
     p_scan = []
     for p in sorted(nterms):
         plan = FMMPlan(x, y, wdz, Lm, p, variant=variant, mode=MODE_FMM)
-        err = _rel(plan.apply(dens), ref)
+        err = _rel(plan.apply(dens)[tl], ref)
         plan.close()
         p_scan.append({"nterms": p, "err": err})
         if err <= target:
@@ -100,7 +105,7 @@ def tune_fmm(x, y, wdz, dens=None, target=1e-10, variant=VARIANT_DR, nterms=(16,
     for L, occ in levels:
         plan = FMMPlan(x, y, wdz, L, p_sel, variant=variant, mode=MODE_FMM)
         t, out = _time_apply(plan, dens, reps)
-        err = _rel(out, ref)
+        err = _rel(out[tl], ref)
         plan.close()
         cands.append({"levmax": L, "points_per_leaf": round(occ, 2), "ms": round(t * 1e3, 4), "err": err})
     ok = [c for c in cands if c["err"] <= target] or cands
@@ -115,7 +120,7 @@ def tune_fmm(x, y, wdz, dens=None, target=1e-10, variant=VARIANT_DR, nterms=(16,
     plan.close()
     bt = min(tile_scan, key=lambda c: c["ms"])
     res = {"levmax": best["levmax"], "nterms": p_sel, "p2p_tile": bt["p2p_tile"], "ms": bt["ms"], "err": best["err"],
-           "target": target, "candidates": cands, "p_scan": p_scan, "p_scan_levmax": Lm, "tile_scan": tile_scan}
+           "target": target, "check": f"O(N^2) sum at {int(tgt.numel())} targets", "candidates": cands, "p_scan": p_scan, "p_scan_levmax": Lm, "tile_scan": tile_scan}
     _CACHE[key] = res
     if cache_file:
         disk = {}

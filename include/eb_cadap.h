@@ -134,6 +134,15 @@ int eb_fmm_plan_destroy(eb_fmm_plan plan);
 int eb_direct_apply_noalloc(int32_t n, const double *x, const double *y, const double *wdz, const double *dens,
                             int32_t variant, double *gradw, void *stream);
 
+/* The closed-form O(N^2) sum of the kernel `variant` over all n sources at the ntgt target
+ * indices tgt[] (0-based, device int32): gradw[t] = GW at point tgt[t], the point itself
+ * excluded.  Device pointers; synchronises `stream` (it allocates and frees its chunk
+ * partials).  Independent of the tree, lists and expansions: the checker of accuracy
+ * measurements (the autotuner and the benchmark).  No counterpart in the reference, whose
+ * direct routines (SLDIRDR / SLSELFDR, cadapdrop2.f:564-769) sum only near boxes. */
+int eb_direct_subset(int32_t n, const double *x, const double *y, const double *wdz, const double *dens,
+                     int32_t variant, int32_t ntgt, const int32_t *tgt, double *gradw, void *stream);
+
 /* Plan facts: [0] rscal, [1] rminx, [2] rminy; counts: [0] levmax, [1] nterms, [2] kmax
  * (padded p), [3] #ordered near-field pairs, [4] #M2L pairs (both boxes non-empty),
  * [5] #non-empty boxes over levels 2..levmax, [6] #non-empty leaf boxes, [7] max leaf count.

@@ -352,3 +352,31 @@ def test_two_to_the_20_against_direct_sum(torch, kind):
     out2 = plan.apply(torch.tensor(dens, device="cuda")).cpu().numpy()
     assert rel(out2[tgt], ref) < TOL_REL
     plan.close()
+
+
+# ------------------------------------------------------------------------------------------
+# The tree-free O(N^2) checker at a subset of targets (eb_direct_subset), used by the
+# autotuner and the benchmark to measure accuracy: against the long-double sums.
+# ------------------------------------------------------------------------------------------
+
+@pytest.mark.parametrize("variant", [eb.VARIANT_DR, eb.VARIANT_MOD, eb.VARIANT_PLUS, eb.VARIANT_HILBERT])
+def test_direct_subset_against_long_double_sums(torch, variant):
+    rng = np.random.default_rng(40 + variant)
+    z, dz, nd, h = curves_cfg(40, 256, rng)
+    z = np.concatenate([z, z[:3] + 1e-7])  # near-coincident points
+    dz = np.concatenate([dz, dz[:3]])
+    nd = np.append(nd, 3).astype(np.int32)
+    h = np.append(h, h[0])
+    dens = complex_normal(z.size, rng)
+    tgt = np.sort(rng.choice(z.size, 300, replace=False)).astype(np.int32)
+    tgt[-1] = z.size - 1
+    x = torch.tensor(z.real.copy(), device="cuda")
+    y = torch.tensor(z.imag.copy(), device="cuda")
+    w = torch.tensor(weights(dz, nd, h), device="cuda")
+    got = eb.direct_subset(x, y, w, torch.tensor(dens, device="cuda"), torch.tensor(tgt, device="cuda"),
+                           variant).cpu().numpy()
+    if variant == eb.VARIANT_HILBERT:
+        ref = oracle.cauchy(z.real, z.imag, dens * weights(dz, nd, h), tgt)
+    else:
+        ref = oracle.direct(variant, nd, z.real, z.imag, dz, dens, h, tgt)
+    assert rel(got, ref) < 1e-13
