@@ -440,6 +440,25 @@ def geometry(P, z):
     return dict(Z=Z, dz=-za, s=s, kap=kap, ckern=-zaa / za, tha=-kap * s, zk=Z[:, 0] / P.N)
 
 
+def _finter_curve(Zm):
+    """The curve's interpolant as PVINTEV forms it with FINTER for x and for y
+    (pv_integral.f:216-252): FINTER clears the imaginary part of coefficient N/2 - 1 (and the
+    real part of the Nyquist one) of each real series, which halves the imaginary part of the
+    +-(N/2 - 1) pair of x's and y's interpolants; in zhat = FFT(x + i y) only entries N/2 - 1 and
+    N/2 + 1 change (Zm has its Nyquist entry cleared already)."""
+    torch = _torch()
+    nyk = Zm.shape[1] // 2
+    zp, zm = Zm[:, nyk - 1], Zm[:, nyk + 1]
+    xh = 0.5 * (zp + zm.conj())
+    yh = -0.5j * (zp - zm.conj())
+    xq = torch.complex(xh.real, 0.5 * xh.imag)
+    yq = torch.complex(yh.real, 0.5 * yh.imag)
+    out = Zm.clone()
+    out[:, nyk - 1] = xq + 1j * yq
+    out[:, nyk + 1] = xq.conj() + 1j * yq.conj()
+    return out
+
+
 def rhs(P, z, e0, cap):
     """dz/dt = F(z) for z of shape (k, N) complex; e0, cap real scalars or 0-d tensors."""
     torch = _torch()
@@ -452,7 +471,7 @@ def rhs(P, z, e0, cap):
     u = LaplaceSolve.apply(P, -2.0 * fz.real, fz, fdz, fs, fk, zk).view(P.k, N)
     # DIRICH_NEU_MAP with PVINTEV (laplace.f:266-332, pv_integral.f:53-212): the curve and u dz
     # interpolated to the midpoints, the alternating-point Hilbert product at the original points
-    Zm = g["Z"] * P.mask
+    Zm = _finter_curve(g["Z"] * P.mask)
     tgt = torch.fft.ifft(Zm, dim=1)
     mid = torch.fft.ifft(Zm * P.shift, dim=1)
     umid = torch.fft.ifft(torch.fft.fft(u * dz, dim=1) * P.shift, dim=1)

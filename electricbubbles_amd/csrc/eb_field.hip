@@ -85,6 +85,44 @@ __device__ __forceinline__ void interp_coeffs(double2* a, double2* c, int N) {
   __syncthreads();
 }
 
+// The same for the curve z = x + i y, with the interpolant that PVINTEV forms for x and for y
+// with FINTER, the real-data routine (pv_integral.f:216-252).  FINTER clears the imaginary part
+// of coefficient N/2 - 1 and the real part of coefficient N/2 (the two words `work(ipt+2*nyk-1)`
+// and `work(ipt+2*nyk)`, pv_integral.f:236-237) where FINTERC clears the Nyquist entry, and
+// keeps the real part of the inverse transform.  For real data that halves the imaginary part of
+// the +-(N/2 - 1) pair of x's interpolant (and of y's); the Nyquist entry is zero up to rounding.
+// In terms of zhat = FFT(x + i y) only entries N/2 - 1 and N/2 + 1 change.  Resolved curves
+// carry nothing at mode N/2 - 1; curves that need DOUBLE do.
+__device__ __forceinline__ void interp_coeffs_curve(double2* a, double2* c, int N) {
+  const double invn = 1.0 / N;
+  const int nyk = N / 2;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    const double2 v = make_double2(a[i].x * invn, a[i].y * invn);
+    a[i] = i == nyk ? make_double2(0.0, 0.0) : v;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && nyk >= 2) {
+    const double2 zp = a[nyk - 1], zm = a[nyk + 1];  // modes +(N/2 - 1) and -(N/2 - 1)
+    // xhat = (zp + conj(zm)) / 2, yhat = (zp - conj(zm)) / (2 i), at mode +(N/2 - 1)
+    const double2 xh = make_double2(0.5 * (zp.x + zm.x), 0.5 * (zp.y - zm.y));
+    const double2 yh = make_double2(0.5 * (zp.y + zm.y), -0.5 * (zp.x - zm.x));
+    // FINTER's interpolant: the imaginary parts halved; then zhat = xhat + i yhat at +(N/2 - 1)
+    // and conj(xhat) + i conj(yhat) at -(N/2 - 1)
+    const double2 xq = make_double2(xh.x, 0.5 * xh.y), yq = make_double2(yh.x, 0.5 * yh.y);
+    a[nyk - 1] = make_double2(xq.x - yq.y, xq.y + yq.x);
+    a[nyk + 1] = make_double2(xq.x + yq.y, -xq.y + yq.x);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    const double2 v = a[i];
+    const int ks = (i <= nyk) ? i : i - N;  // signed wavenumber
+    double sn, cs;
+    sincospi((double)ks / (double)N, &sn, &cs);
+    c[i] = make_double2(v.x * cs - v.y * sn, v.x * sn + v.y * cs);
+  }
+  __syncthreads();
+}
+
 // Per body: the fine coordinates x2, y2 (2N points: even = the interpolated original points,
 // odd = the midpoints) and the fine density: ucmplx = u dz interpolated at the midpoints, 0 at
 // the original points (QA of pv_integral.f:160-163 is U2 h there, with h the FMM weight).
@@ -103,7 +141,7 @@ __global__ __launch_bounds__(SP_THREADS) void k_pv_interp(BodyView B, const doub
   for (int i = threadIdx.x; i < N; i += blockDim.x) a[i] = make_double2(x[o0 + i], y[o0 + i]);
   __syncthreads();
   fft(a, ws, bc, B, -1);
-  interp_coeffs(a, c, N);
+  interp_coeffs_curve(a, c, N);
   fft(a, ws, bc, B, +1);
   fft(c, ws, bc, B, +1);
   for (int i = threadIdx.x; i < N; i += blockDim.x) {

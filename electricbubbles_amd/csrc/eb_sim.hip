@@ -362,8 +362,8 @@ static int sim_alloc(eb_sim S) {
 static int check_params(const eb_sim_params* p) {
   if (!p || p->k < 1 || p->k > EB_SIM_KMAX) return (eb_set_error("eb_sim: K outside 1..%d", EB_SIM_KMAX), EB_ERR_ARG);
   for (int b = 0; b < p->k; ++b)
-    if (p->nd[b] < 16 || p->nd[b] > 2048 || (p->nd[b] & 1))
-      return (eb_set_error("eb_sim: body %d has ND = %d (even, 16..2048)", b, p->nd[b]), EB_ERR_ARG);
+    if (p->nd[b] < 16 || p->nd[b] > EB_SIM_NDMAX || (p->nd[b] & 1))
+      return (eb_set_error("eb_sim: body %d has ND = %d (even, 16..%d)", b, p->nd[b], EB_SIM_NDMAX), EB_ERR_ARG);
   if (p->bondx != 0.0 || p->bondy != 0.0)
     return (eb_set_error("eb_sim: gravity (BOND not 0) runs with a wall in the reference and is not built"), EB_ERR_ARG);
   if (!(p->dt > 0.0)) return (eb_set_error("eb_sim: DT must be positive"), EB_ERR_ARG);
@@ -487,9 +487,18 @@ extern "C" int eb_sim_step(eb_sim S, int32_t* done) {
     S->area[b] = S->post[5 * b];
     S->arcl[b] = S->post[5 * b + 1];
     S->areatot += S->area[b];
-    dbl[b] = S->post[5 * b + 4] != 0.0 && 2 * S->nd[b] <= 2048;
+    dbl[b] = S->post[5 * b + 4] != 0.0;
     icut = icut || dbl[b];
   }
+  // A body that needs doubling past F's limit ends the run, as the reference STOPs when DOUBLE
+  // would pass NDMAX (electric.f:190-199); the step itself has been taken.
+  for (int b = 0; b < S->k; ++b)
+    if (dbl[b] && 2 * S->nd[b] > EB_SIM_NDMAX) {
+      ++S->steps;
+      return (eb_set_error("eb_sim_step: body %d needs doubling to %d points, past the limit of %d (the reference "
+                           "stops the same way at its NDMAX)", b, 2 * S->nd[b], EB_SIM_NDMAX),
+              EB_ERR_UNSUPPORTED);
+    }
   if (icut) {
     // DOUBLE every flagged body, rebuild the bodies and F for the new lengths (electric.f:179-199)
     std::vector<int> nd_new(S->nd), off_new(S->k + 1, 0);

@@ -25,6 +25,7 @@ extern "C" {
 #endif
 
 #define EB_SIM_KMAX 150 /* bodies (the reference's KMAX, electric.f:10) */
+#define EB_SIM_NDMAX 2048 /* points per body (F's PVINTEV interpolation; the reference's NDMAX is 3000) */
 
 /* The run description that READINI reads from `bubbin`. */
 typedef struct {
@@ -48,7 +49,9 @@ typedef struct eb_sim_s *eb_sim;
  * which the reference runs with a wall) is rejected. */
 int eb_sim_create(eb_sim *s, const eb_sim_params *p, void *stream);
 /* One pass of the time loop: the RK2 step, EXTRACTSOL and GETFOURIER of every body, DOUBLE
- * where the spectrum asks for it, the new dt.  *done = 1 once time reaches TFINAL. */
+ * where the spectrum asks for it, the new dt.  *done = 1 once time reaches TFINAL.  A body
+ * whose spectrum asks for doubling past EB_SIM_NDMAX points ends the run with
+ * EB_ERR_UNSUPPORTED after the RK2 step (the reference STOPs there, electric.f:190-199). */
 int eb_sim_step(eb_sim s, int32_t *done);
 
 typedef struct {

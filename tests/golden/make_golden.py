@@ -280,6 +280,12 @@ def main_field():
         cases.append(("two_drops", [128, 160], [0j, 3.0 + 0.5j], [(2, 0.05), (3, 0.03)], 0.0, 0.64))
         cases.append(("four_circles", [128] * 4, [1.5 + 1.5j, -1.5 + 1.5j, -1.5 - 1.5j, 1.5 - 1.5j], [(2, 0.0)], 0.0,
                       0.64))
+        # under-resolved bodies (content at mode N/2 - 1, where PVINTEV's FINTER differs from
+        # FINTERC, pv_integral.f:236-237), off the origin, with the straining flow on
+        cases.append(("underresolved", [32, 48], [0.3 - 0.2j, 2.8 + 0.4j], [(5, 0.1), (14, 0.02)], 0.05, 0.5))
+        only = os.environ.get("EB_GOLDEN_FIELD")
+        if only:
+            cases = [c for c in cases if c[0] == only]
         for tag, nds, centres, modes, cap, e0 in cases:
             nd = np.array(nds, np.int32)
             rku = np.concatenate([perturbed_circle(int(n), modes, center=c)[0] for n, c in zip(nds, centres)])
@@ -307,6 +313,8 @@ def main_field():
                  lap_w=w[:nbk].copy(), rk2=traj, rk2_dt=np.float64(dt), rk2_steps=np.int32(nsteps),
                  **{k: v for k, v in ch.items() if k in ("rhs_lap", "ulap", "phi", "psi", "dphi_dn", "rhs_stokes",
                                                          "u", "za", "pk", "un")})
+        if only:
+            return
         # a many-drop configuration (8 drops of 256..1024 points, E = (1.5, 0)): F and two RK2
         # steps at the configuration's dt
         from electricbubbles_amd.workloads import many_drops_cfg
@@ -339,13 +347,23 @@ def main_sim():
                              bi=[1.0 / 1.1], thetax=[0.3], cx=[0.2], cy=[-0.1], ncyc=[0]), 10),
             "starfish": (dict(k=1, nd=[256], dt=5e-4, tfinal=1.0, nplot=10, cap=0.0, e0=0.5, ai=[1.0], bi=[0.1],
                               thetax=[0.0], cx=[0.0], cy=[0.0], ncyc=[3]), 4),
+            # a five-lobed star on 32 points beside an ellipse: GETFOURIER's test (modes n-10..n-1
+            # above 1e-11, electric.f:560-566) fires on the star in each of the first three steps,
+            # and DOUBLE (electric.f:721-757) takes it to 64, 128 and 256 points
+            "doubling": (dict(k=2, nd=[32, 64], dt=1e-3, tfinal=1.0, nplot=10, cap=0.05, e0=0.5, ai=[1.0, 0.8],
+                              bi=[0.1, 0.8], thetax=[0.0, 0.2], cx=[0.0, 3.0], cy=[0.0, 0.5], ncyc=[5, 0]), 4),
         }
+        only = os.environ.get("EB_GOLDEN_SIM")
+        if only:
+            cases = {only: cases[only]}
         for tag, (p, nsteps) in cases.items():
             r0 = oracle.ref_manydrop(p, 0)
             r1 = oracle.ref_manydrop(p, nsteps)
-            print(tag, "steps", nsteps, "dt", r1["dt"], "area", r1["hist"][-1])
+            # body lengths after each step (the doubling history)
+            nds = np.array([oracle.ref_manydrop(p, s)["nd"] for s in range(1, nsteps + 1)], np.int32)
+            print(tag, "steps", nsteps, "dt", r1["dt"], "area", r1["hist"][-1], "nd", nds.tolist())
             save("sim_" + tag, nsteps=np.int32(nsteps), ybar0=r0["ybar"], ybar=r1["ybar"], nd_after=r1["nd"],
-                 hist=r1["hist"], arcl0=r1["arcl0"], area0=r1["area0"], cmin=np.float64(r1["cmin"]),
+                 nd_steps=nds, hist=r1["hist"], arcl0=r1["arcl0"], area0=r1["area0"], cmin=np.float64(r1["cmin"]),
                  **{k: np.asarray(v) for k, v in p.items()})
     finally:
         os.chdir(cwd)

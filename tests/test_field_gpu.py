@@ -15,7 +15,9 @@ from tests.conftest import GOLDEN
 
 pytestmark = pytest.mark.gpu
 
-CASES = ["single_drop", "two_drops", "four_circles"]
+# "underresolved": bodies of 32 and 48 points with content at mode N/2 - 1, where PVINTEV's
+# real-data interpolation (FINTER, pv_integral.f:216-252) differs from the complex one
+CASES = ["single_drop", "two_drops", "four_circles", "underresolved"]
 # "auto" takes the O(N^2) sums at these sizes (EB_DIRECT_MAX); "fmm" forces every layer potential
 # of F (the Laplace and PVINTEV Hilbert products, the Stokes matvec, CADAPMOD) onto the FMM at
 # the reference's LEVMAX 7, the path F takes above EB_DIRECT_MAX points

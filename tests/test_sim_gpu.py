@@ -11,7 +11,9 @@ from tests.conftest import GOLDEN
 
 pytestmark = pytest.mark.gpu
 
-CASES = ["four_circles", "ellipse", "starfish"]
+# "doubling": GETFOURIER's test fires on a five-lobed star of 32 points in each of the first three
+# steps and DOUBLE takes it to 64, 128 and 256 points (electric.f:179-237, 531-570, 721-757)
+CASES = ["four_circles", "ellipse", "starfish", "doubling"]
 
 
 @pytest.fixture(scope="module")
@@ -53,9 +55,13 @@ def test_time_loop(torch, tag):
         S.step()
         st = S.state()
         hist.append((st["time"], st["dt"], st["area_total"]))
+        if "nd_steps" in f.files:  # the body lengths after every step
+            assert np.array_equal(st["nd"], f["nd_steps"][len(hist) - 1]), (len(hist), st["nd"])
     hist = np.array(hist)
     assert np.abs(st["ybar"] - f["ybar"]).max() < 1e-8
     assert np.array_equal(st["nd"], f["nd_after"])
+    if "nd_steps" in f.files:
+        assert st["doublings"] == int((np.diff(np.vstack([f["nd"], f["nd_steps"]]), axis=0) != 0).any(1).sum())
     assert np.abs(hist[:, 0] - f["hist"][:, 0]).max() < 1e-12
     assert np.abs(hist[:, 1] / f["hist"][:, 1] - 1).max() < 1e-10
     assert np.abs(hist[:, 2] / f["hist"][:, 2] - 1).max() < 1e-12
@@ -86,3 +92,21 @@ def test_dump_and_restore_continue_the_run(torch, tmp_path):
     assert b["dt"] == pytest.approx(min(cmin_r * l / n for n, l in zip(b["nd"], b["arcl"])), rel=1e-14)
     S.close()
     R.close()
+
+
+def test_doubling_past_the_body_limit_stops_the_run(torch):
+    """A 2048-point star whose top ten Fourier modes hold about 4e-9 (above GETFOURIER's 1e-11,
+    electric.f:560-566) asks for 4096 points: the step is taken and the run ends with
+    EB_ERR_UNSUPPORTED, as the reference STOPs when DOUBLE would pass its NDMAX
+    (electric.f:190-199)."""
+    import electricbubbles_amd as eb
+    from electricbubbles_amd import sim
+    p = dict(k=1, nd=[2048], dt=1e-4, tfinal=1.0, nplot=10, cap=0.0, e0=0.5, ai=[1.0], bi=[0.2], thetax=[0.0],
+             cx=[0.0], cy=[0.0], ncyc=[7])
+    S = sim.Simulation(p)
+    with pytest.raises(eb.EBError) as ei:
+        S.step()
+    assert ei.value.code == eb.EB_ERR_UNSUPPORTED
+    st = S.state()
+    assert st["steps"] == 1 and list(st["nd"]) == [2048] and st["doublings"] == 0
+    S.close()
