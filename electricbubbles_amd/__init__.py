@@ -47,7 +47,7 @@ EXPORTS = (
     "eb_last_error", "eb_version", "eb_cadapdr", "eb_cadapmod", "eb_cadap", "eb_set_mode", "eb_get_mode",
     "eb_fmm_plan_create", "eb_fmm_plan_rebuild", "eb_fmm_apply", "eb_fmm_plan_destroy", "eb_fmm_plan_info",
     "eb_fmm_plan_ntbox", "eb_fmm_enable_timing", "eb_fmm_stage_ms", "eb_fmm_set_p2p_tile", "eb_cadap_tables",
-    "eb_cadap_lists", "eb_direct_apply_noalloc", "eb_direct_subset",
+    "eb_cadap_lists", "eb_direct_apply_noalloc", "eb_direct_subset", "eb_dapif2",
 )
 
 _lib = None
@@ -74,6 +74,9 @@ def lib():
         f = getattr(L, name)
         f.argtypes = dropin_args
         f.restype = _i32
+    L.eb_dapif2.argtypes = [_ip, _ip, _ip, _ip, _ip, _ip, _ip, _ip, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _ip,
+                            _vp]
+    L.eb_dapif2.restype = _i32
     L.eb_set_mode.argtypes = [ctypes.c_int]
     L.eb_get_mode.restype = ctypes.c_int
     L.eb_fmm_plan_create.argtypes = [ctypes.POINTER(_vp), _i32, _vp, _vp, _vp, _i32, _i32, _i32, _i32, _vp]
@@ -170,6 +173,32 @@ def cadapmod(levmax, xat, yat, dzdt, nd, dens, h, tol, **kw):
 def cadap(levmax, xat, yat, dzdt, nd, dens, h, tol, **kw):
     """CADAP (cadap.f:3): as CADAPMOD with +conj(dens) dz in QB."""
     return _dropin("eb_cadap", levmax, xat, yat, dzdt, nd, dens, h, tol, **kw)
+
+
+def dapif2(xa, ya, qa, napb=30, tol=1e-11, mex=300, eps7=1e-16, nsp=None, iflag7=3):
+    """DAPIF2 (dapif1.f:458) with IFLAG7 = 3 on host arrays: FIELD = the Hilbert-matrix product
+    sum_{j != i} QA(j) / (z_i - z_j).  Returns (field, ierr[0:4], inform[0:4]) as the Fortran routine
+    leaves them (NSP defaults to 4,000,000 + 400 NAT words; WKSP itself is not used)."""
+    x = _as_c(xa, np.float64)
+    y = _as_c(ya, np.float64)
+    q = _as_c(qa, np.complex128).view(np.float64)
+    nat = x.size
+    if nsp is None:
+        nsp = 4_000_000 + 400 * nat
+    field = np.zeros(2 * nat)
+    ierr = np.zeros(8, np.int32)
+    inform = np.zeros(8, np.int32)
+    iout = np.zeros(2, np.int32)
+    dummy = np.zeros(2)
+    I = lambda v: ctypes.byref(ctypes.c_int32(int(v)))  # noqa: E731
+    D = lambda v: ctypes.byref(ctypes.c_double(float(v)))  # noqa: E731
+    dp = lambda a: a.ctypes.data_as(_dp)  # noqa: E731
+    rc = lib().eb_dapif2(iout.ctypes.data_as(_ip), I(iflag7), I(nat), I(napb), I(2), I(mex), ierr.ctypes.data_as(_ip),
+                         inform.ctypes.data_as(_ip), D(tol), D(eps7), dp(x), dp(y), dp(q), dp(dummy), dp(field),
+                         dp(dummy), I(nsp), None)
+    if rc >= EB_ERR_UNSUPPORTED and ierr[0] < 0:
+        raise EBError(rc, last_error())
+    return field.view(np.complex128), ierr[:4].copy(), inform[:4].copy()
 
 
 def set_mode(mode: int) -> None:

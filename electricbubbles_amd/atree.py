@@ -13,7 +13,8 @@ import numpy as np
 
 from . import EBError, _check, _ptr, _stream_ptr, _torch, lib
 
-ATREE_EXPORTS = ("eb_atree_build", "eb_atree_counts", "eb_atree_tables", "eb_atree_destroy")
+ATREE_EXPORTS = ("eb_atree_build", "eb_atree_build_bounded", "eb_atree_counts", "eb_atree_tables",
+                 "eb_atree_destroy")
 
 EB_ATREE_EPS_STOP, EB_ATREE_FEW_POINTS, EB_ATREE_TOO_DEEP = 1, 16, 32
 

@@ -567,7 +567,7 @@ int group_by_owner(eb_atree_s* T, int n, const int* owner, const int* node, DBuf
   return EB_OK;
 }
 
-int build(eb_atree_s* T, const double* x, const double* y, int lmx, double eps7, int32_t* ierr) {
+int build(eb_atree_s* T, const double* x, const double* y, int lmx, double eps7, long long nboxmx, int32_t* ierr) {
   const int n = T->nat, napb = T->napb;
   hipStream_t st = T->st;
   // ---- scaling
@@ -631,6 +631,15 @@ int build(eb_atree_s* T, const double* x, const double* y, int lmx, double eps7,
     AT_TRY(scan(T, T->s_a.p, T->s_b.p, m, true));
     int nsplit = 0;
     AT_TRY(read_int(T, T->s_b.p + m, &nsplit));
+    if ((long long)nbox + 4LL * nsplit > nboxmx) {
+      // DAFAMI adds four children per split box and stops at the first count past NBOXMX
+      // (dapif1.f:1436-1441), with IERR(2) = (NBOXMX - NBOX) * 36 + NSP, NSP being an
+      // undeclared local of DAFAMI (zero in the static storage the reference needs)
+      const long long over = (long long)nbox + 4 * ((nboxmx - nbox) / 4 + 1);
+      ierr[0] = EB_ATREE_WORKSPACE;
+      ierr[1] = (int32_t)((nboxmx - over) * 36);
+      return EB_ATREE_WORKSPACE;
+    }
     const size_t need = (size_t)nbox + 4 * (size_t)nsplit + 2;
     AT_TRY(grow(xb, need, nbox, st));
     AT_TRY(grow(yb, need, nbox, st));
@@ -813,6 +822,12 @@ int build(eb_atree_s* T, const double* x, const double* y, int lmx, double eps7,
 
 extern "C" int eb_atree_build(eb_atree* t, int32_t nat, const double* x, const double* y, int32_t napb, int32_t lmx,
                               double eps7, int32_t* ierr, void* stream) {
+  return eb_atree_build_bounded(t, nat, x, y, napb, lmx, eps7, -1, ierr, stream);
+}
+
+extern "C" int eb_atree_build_bounded(eb_atree* t, int32_t nat, const double* x, const double* y, int32_t napb,
+                                      int32_t lmx, double eps7, int64_t nboxmx, int32_t* ierr, void* stream) {
+  if (nboxmx < 0) nboxmx = INT64_MAX;
   int32_t dummy[3];
   if (!ierr) ierr = dummy;
   ierr[0] = ierr[1] = ierr[2] = 0;
@@ -839,7 +854,7 @@ extern "C" int eb_atree_build(eb_atree* t, int32_t nat, const double* x, const d
   T->nat = nat;
   T->napb = napb;
   T->st = (hipStream_t)stream;
-  const int rc = build(T, x, y, std::min<int>(lmx, AT_MAXLEV), eps7, ierr);
+  const int rc = build(T, x, y, std::min<int>(lmx, AT_MAXLEV), eps7, (long long)nboxmx, ierr);
   if (rc != EB_OK) {
     (void)hipStreamSynchronize(T->st);
     delete T;

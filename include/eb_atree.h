@@ -30,6 +30,8 @@ extern "C" {
                                    partly formed level; this build stops and reports it */
 #define EB_ATREE_FEW_POINTS 16  /* NAT <= NAPB */
 #define EB_ATREE_TOO_DEEP   32  /* more than LMX levels needed */
+#define EB_ATREE_WORKSPACE  8   /* more boxes than NBOXMX before compression (DAFAMI,
+                                   dapif1.f:1436-1441; eb_atree_build_bounded only) */
 
 typedef struct eb_atree_s *eb_atree;
 
@@ -48,6 +50,11 @@ typedef struct {
  * IERR(1..3) as DAPIF2 sets them and *t stays NULL), or an EB_ERR_* code. */
 int eb_atree_build(eb_atree *t, int32_t nat, const double *x, const double *y, int32_t napb, int32_t lmx,
                    double eps7, int32_t *ierr, void *stream);
+/* The same with DAPIF2's bound on the boxes before compression (NBOXMX, dapif1.f:784): the
+ * build stops with EB_ATREE_WORKSPACE, IERR(1) = 8 and IERR(2) as DAFAMI sets them, where the
+ * reference's tree would outgrow its part of the workspace.  nboxmx < 0: no bound. */
+int eb_atree_build_bounded(eb_atree *t, int32_t nat, const double *x, const double *y, int32_t napb, int32_t lmx,
+                           double eps7, int64_t nboxmx, int32_t *ierr, void *stream);
 /* counts[8] = {nbox, nlev, len, len3, len4, maxp, jatlen, nat} (host array). */
 int eb_atree_counts(eb_atree t, int32_t *counts);
 int eb_atree_tables(eb_atree t, eb_atree_view *view);

@@ -95,6 +95,20 @@ int32_t eb_cadap(const int32_t *iflag1, const int32_t *iflag2, const int32_t *le
                  const double *tol, int32_t *inform, int32_t *ier, const double *xleft,
                  const double *slength);
 
+/* DAPIF2 (dapif1.f:458-459), the adaptive FMM of the Laplace side, with its 18 arguments by
+ * reference: (IOUT, IFLAG7, NAT, NAPB, NINIRE, MEX, IERR, INFORM, TOL, EPS7, XA, YA, QA, POTEN,
+ * FIELD, WKSP, NSP, CLOSE).  IFLAG7 = 3 only, the Hilbert-matrix product that FASMVP_LAP
+ * (laplace.f:214-215) and PVINTEV (pv_integral.f:181-182) call:
+ * FIELD(i) = sum_{j != i} QA(j) / (z_i - z_j), QA and FIELD COMPLEX*16.  IERR(1..3) and INFORM(1..4)
+ * are set as the reference sets them (IERR 4, 16, 1, 32 and the workspace checks with IERR 8),
+ * from the adaptive tree of DAPIF2's rules built on the GPU; WKSP is not used and CLOSE is never
+ * called.  Other IFLAG7 values return EB_ERR_UNSUPPORTED with IERR(1) = -1.  Returns IERR(1)
+ * or an EB_ERR_* status. */
+int32_t eb_dapif2(int32_t *iout, const int32_t *iflag7, const int32_t *nat, const int32_t *napb,
+                  const int32_t *ninire, const int32_t *mex, int32_t *ierr, int32_t *inform, const double *tol,
+                  const double *eps7, const double *xa, const double *ya, const double *qa, double *poten,
+                  double *field, double *wksp, const int32_t *nsp, const void *close);
+
 /* Evaluation mode used by the drop-in layer (process-wide; default EB_MODE_AUTO): the CADAP
  * entry points, eb_fasmvp and eb_dapif2 take the O(N^2) sum or the FMM by it.  No counterpart
  * in the reference (its CADAPDR is always the FMM); a knob of this build for parity runs. */

@@ -543,7 +543,9 @@ def ref_dapif2(x, y, qa, napb=30, tol=1e-11, mex=300, eps7=1e-16, nsp=None):
     field = np.zeros(2 * nat)
     if nsp is None:
         nsp = 4_000_000 + 400 * nat
-    wksp = np.zeros(nsp)
+    # the buffer is never smaller than the default: with a short NSP, DALI34 can write its lists
+    # past NSP before the next workspace check (dapif1.f:918-929)
+    wksp = np.zeros(max(nsp, 4_000_000 + 400 * nat))
     iout = np.zeros(2, np.int32)
     ierr = np.zeros(8, np.int32)
     inform = np.zeros(8, np.int32)
@@ -761,6 +763,19 @@ class RefSim:
                           _d(y), _d(dy))
             t += dt
         return ybar
+
+    def fasmvp_lap(self, nd, zk, z, dz, rkappa, dsda, u):
+        """FASMVP_LAP (laplace.f:159-262) on a real density u: the nbk rows of the Laplace
+        operator (the routine also writes k constraint rows past them)."""
+        nd = np.ascontiguousarray(nd, np.int32)
+        k = nd.size
+        nbk = int(nd.sum())
+        w = np.zeros(nbk + k + 8)
+        ak = np.zeros(k + 8)
+        self.lib.fasmvp_lap_(_c_int(k), _i(nd), _c_int(nbk), _d(_cplx(zk)), _d(_cplx(z)), _d(_cplx(dz)),
+                             _d(np.ascontiguousarray(rkappa, np.float64)), _d(np.ascontiguousarray(dsda, np.float64)),
+                             _d(ak), _d(np.ascontiguousarray(u, np.float64)), _d(w))
+        return w[:nbk].copy()
 
     def fasmvp(self, nd, x, y, dz, dsdth, rkappa, zk, u):
         nd = np.ascontiguousarray(nd, np.int32)

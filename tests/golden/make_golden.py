@@ -2,8 +2,9 @@
 
 Runs only where the reference sources exist (they are compiled in place by oracle/Makefile);
 the fixtures it writes are data -- inputs and the reference's outputs -- and are committed so
-the tests run without the reference.  Usage:  python tests/golden/make_golden.py [atree|gmres|sweep|field|sim|adjoint|dapif2]
-("atree" / "gmres" / "sweep" / "field" / "sim" / "adjoint" / "dapif2" rewrite only those fixtures).
+the tests run without the reference.  Usage:  python tests/golden/make_golden.py [atree|gmres|sweep|field|sim|adjoint|adjoint16|dapif2]
+("atree" / "gmres" / "sweep" / "field" / "sim" / "adjoint" / "adjoint16" / "dapif2" rewrite only those
+fixtures; "adjoint16" runs the reference's F some 680 times, in parallel worker processes).
 """
 from __future__ import annotations
 
@@ -395,12 +396,71 @@ def np_deformation(z):
     return (sp - sm) / (sp + sm)
 
 
-def np_shapes(centres, modes, N):
+def np_shapes(centres, modes, N, decay=0.0):
+    """z = c + (1 + sum_m m^-decay (a_m cos m th + b_m sin m th)) e^{i th} (the adjoint's shapes)."""
     M = modes.shape[1] // 2
     th = 2 * np.pi * np.arange(N) / N
     m = np.arange(1, M + 1)
-    r = 1.0 + modes[:, :M] @ np.cos(np.outer(m, th)) + modes[:, M:] @ np.sin(np.outer(m, th))
+    w = m.astype(np.float64) ** (-float(decay))
+    r = 1.0 + modes[:, :M] @ (w[:, None] * np.cos(np.outer(m, th))) + modes[:, M:] @ (w[:, None] * np.sin(np.outer(m, th)))
     return centres[:, None] + r * np.exp(1j * th)[None, :]
+
+
+def _ref_rk4_loss(args):
+    """Summed deformation after `steps` RK4 steps whose F is the reference's own (RefSim.rhs), run
+    in a scratch directory of its own (the reference writes fort.13): one worker process."""
+    import tempfile
+    centres, modes, E_, Ca_, N, steps, dt, decay = args
+    os.chdir(tempfile.mkdtemp())
+    nd = np.full(centres.size, N, np.int32)
+    sim = oracle.RefSim(oracle.LIBREF)
+    sim.setup(nd, cap=Ca_, e0=E_ * E_)
+    z = np_shapes(centres, modes, N, decay)
+
+    def F(zz):
+        y = np.stack([zz.real.ravel(), zz.imag.ravel()], 1).ravel()
+        d = sim.rhs(y)
+        return (d[0::2] + 1j * d[1::2]).reshape(zz.shape)
+    for _ in range(steps):
+        k1 = F(z)
+        k2 = F(z + 0.5 * dt * k1)
+        k3 = F(z + 0.5 * dt * k2)
+        k4 = F(z + dt * k3)
+        z = z + dt / 6 * (k1 + 2 * k2 + 2 * k3 + k4)
+    return float(np_deformation(z).sum())
+
+
+def main_adjoint16():
+    """The adjoint configuration at its size: 16 drops of 512 points on a 4 x 4 grid (spacing 4),
+    64 shape modes per drop with amplitudes U[0, 0.02] weighted 1/m^2, |E| = 1, Ca = 0.1, 10 RK4
+    steps at dt = 1e-3 with the reference's own F.  Richardson-extrapolated central differences
+    (h = 1e-4) of the summed deformation in |E|, Ca and two random unit directions of all 1026
+    parameters; the forwards run in parallel worker processes."""
+    import multiprocessing as mp
+    # adjoint.grid_problem(4, 512, 4.0, 64, 0.02, seed=5): the same draws, on the host
+    rng = np.random.default_rng(5)
+    g = (np.arange(4) - 1.5) * 4.0
+    cx, cy = np.meshgrid(g, g, indexing="xy")
+    centres = (cx + 1j * cy).ravel()
+    modes = rng.uniform(0.0, 0.02, (16, 64))
+    E, Ca, dt, steps, N, decay, h = 1.0, 0.1, 1e-3, 10, 512, 2.0, 1e-4
+    rng = np.random.default_rng(16)
+    dirs = rng.standard_normal((2, 2 + modes.size))
+    dirs /= np.linalg.norm(dirs, axis=1)[:, None]
+    pert = [(1.0, 0.0, np.zeros_like(modes)), (0.0, 1.0, np.zeros_like(modes))]
+    pert += [(v[0], v[1], v[2:].reshape(modes.shape)) for v in dirs]
+    jobs = [(centres, modes, E, Ca, N, steps, dt, decay)]
+    for dE, dCa, dm in pert:
+        for e in (h, -h, h / 2, -h / 2):
+            jobs.append((centres, modes + e * dm, E + e * dE, Ca + e * dCa, N, steps, dt, decay))
+    with mp.get_context("spawn").Pool(min(8, os.cpu_count() or 1)) as pool:
+        vals = pool.map(_ref_rk4_loss, jobs)
+    L0, rest = vals[0], np.array(vals[1:]).reshape(len(pert), 4)
+    fd = (4 * (rest[:, 2] - rest[:, 3]) / h - (rest[:, 0] - rest[:, 1]) / (2 * h)) / 3
+    print("adjoint16 fixture: loss", L0, "fd", fd)
+    save("adjoint_fd_16drops", centres=centres, modes=modes, E=np.float64(E), Ca=np.float64(Ca), dt=np.float64(dt),
+         steps=np.int32(steps), N=np.int32(N), decay=np.float64(decay), h=np.float64(h), loss=np.float64(L0),
+         fd_E=np.float64(fd[0]), fd_Ca=np.float64(fd[1]), dirs=dirs, fd_dirs=fd[2:])
 
 
 def main_adjoint():
@@ -486,7 +546,10 @@ def main_dapif2():
                         f"field_{tag}": field, f"ierr_{tag}": ierr, f"inform_{tag}": inform})
         # error returns, on the "lap" points
         z, q = cases[0][1], cases[0][2]
-        errs = {"tol": dict(tol=1e-18), "few": dict(napb=5000), "nsp": dict(nsp=2 * z.size - 1)}
+        need = int(out["inform_lap"][3])  # INFORM(4): the workspace the call uses
+        errs = {"tol": dict(tol=1e-18), "few": dict(napb=5000), "nsp": dict(nsp=2 * z.size - 1),
+                "nsp_last": dict(nsp=need - 2), "nsp_half": dict(nsp=need // 2),
+                "nsp_tables": dict(nsp=need // 8)}
         for tag, kw in errs.items():
             args = dict(napb=30, tol=1e-11)
             args.update(kw)
@@ -495,6 +558,9 @@ def main_dapif2():
             assert ierr[0] != 0, tag
             out[f"err_{tag}_ierr"] = ierr
             out[f"err_{tag}_inform"] = inform
+            out[f"err_{tag}_args"] = np.array([args["napb"], args.get("nsp", 0)], np.int64)
+            out[f"err_{tag}_tol"] = np.float64(args["tol"])
+            print("dapif2 error case", tag, ierr, inform)
         save("dapif2_hilbert", **out)
     finally:
         os.chdir(cwd)
@@ -505,6 +571,8 @@ if __name__ == "__main__":
         main_adjoint()
     elif sys.argv[1:] == ["dapif2"]:
         main_dapif2()
+    elif sys.argv[1:] == ["adjoint16"]:
+        main_adjoint16()
     elif sys.argv[1:] == ["atree"]:
         main_atree()
     elif sys.argv[1:] == ["gmres"]:

@@ -277,3 +277,72 @@ def test_trajectory_loss_autograd_entry(torch):
     assert abs(float(E.grad) - 2 * ref[1]) < 1e-13 * abs(ref[1])
     assert abs(float(Ca.grad) - 2 * ref[2]) < 1e-13 * abs(ref[2])
     assert rel(m.grad.cpu().numpy(), 2 * ref[3].cpu().numpy()) < 1e-13
+
+
+# ------------------------------------------------------------------------------------------
+# The adjoint configuration at its size: 16 drops x 512 points (4 x 4 grid, spacing 4), 64
+# shape modes per drop of amplitude U[0, 0.02] weighted 1/m^2, |E| = 1, Ca = 0.1, RK4 at
+# dt = 1e-3, GMRES 1e-12.
+# ------------------------------------------------------------------------------------------
+
+def _config16(torch):
+    from electricbubbles_amd import adjoint
+    centres, modes = adjoint.grid_problem(4, 512, 4.0, 64, 0.02, seed=5, device="cuda")
+    return adjoint.Problem(16, 512, tol=1e-12), centres, modes
+
+
+def test_gradient_at_size_matches_gpu_central_differences(torch):
+    """20 steps: the gradient along two random unit directions of all 1026 parameters against
+    Richardson-extrapolated central differences (h = 1e-5 and 5e-6) of the same forward."""
+    from electricbubbles_amd import adjoint
+    P, centres, modes = _config16(torch)
+    steps, dt, E, Ca, decay = 20, 1e-3, 1.0, 0.1, 2.0
+    print("adjoint 16 x 512, 20 steps", flush=True)
+    loss, gE, gCa, gm, _ = adjoint.gradient(P, centres, modes, E, Ca, steps, dt, checkpoint=5, decay=decay)
+    rng = np.random.default_rng(20)
+    h = 1e-5
+    for _ in range(2):
+        v = rng.standard_normal(2 + modes.numel())
+        v /= np.linalg.norm(v)
+        vm = torch.tensor(v[2:].reshape(modes.shape), device="cuda")
+
+        def L(e):
+            return adjoint.forward_loss(P, centres, modes + e * vm, E + e * v[0], Ca + e * v[1], steps, dt,
+                                        decay=decay)
+        fd = (4 * (L(h / 2) - L(-h / 2)) / h - (L(h) - L(-h)) / (2 * h)) / 3
+        an = gE * v[0] + gCa * v[1] + float((gm * vm).sum())
+        print("directional derivative: adjoint", an, "central differences", fd, flush=True)
+        assert abs(an - fd) < 1e-6 * abs(fd)
+
+
+def test_gradient_at_size_matches_reference_differences(torch):
+    """10 steps: dL/d|E|, dL/dCa and two directional derivatives against central differences of
+    the same loss whose F is the reference's own (tests/golden/adjoint_fd_16drops.npz)."""
+    from electricbubbles_amd import adjoint
+    f = np.load(os.path.join(GOLDEN, "adjoint_fd_16drops.npz"))
+    P = adjoint.Problem(16, int(f["N"]), tol=1e-12)
+    centres = torch.tensor(f["centres"], device="cuda")
+    modes = torch.tensor(f["modes"], device="cuda")
+    loss, gE, gCa, gm, _ = adjoint.gradient(P, centres, modes, float(f["E"]), float(f["Ca"]), int(f["steps"]),
+                                            float(f["dt"]), checkpoint=5, decay=float(f["decay"]))
+    print("loss", loss, float(f["loss"]), "dE", gE, float(f["fd_E"]), "dCa", gCa, float(f["fd_Ca"]), flush=True)
+    assert abs(loss - float(f["loss"])) < 1e-9 * abs(float(f["loss"]))
+    assert abs(gE - float(f["fd_E"])) < 1e-6 * abs(float(f["fd_E"]))
+    assert abs(gCa - float(f["fd_Ca"])) < 1e-6 * abs(float(f["fd_Ca"]))
+    gmn = gm.cpu().numpy().ravel()
+    for v, fd in zip(f["dirs"], f["fd_dirs"]):
+        an = gE * v[0] + gCa * v[1] + float((gmn * v[2:]).sum())
+        assert abs(an - fd) < 1e-6 * abs(fd), (an, fd)
+
+
+def test_checkpoint_interval_does_not_change_the_200_step_gradient(torch):
+    """The 200-step run: checkpoints every 20 steps and every 25 give the same gradient."""
+    from electricbubbles_amd import adjoint
+    P, centres, modes = _config16(torch)
+    print("adjoint 16 x 512, 200 steps, checkpoints 20 and 25", flush=True)
+    a = adjoint.gradient(P, centres, modes, 1.0, 0.1, 200, 1e-3, checkpoint=20, decay=2.0)
+    b = adjoint.gradient(P, centres, modes, 1.0, 0.1, 200, 1e-3, checkpoint=25, decay=2.0)
+    assert abs(a[0] - b[0]) < 1e-13 * abs(a[0])
+    assert abs(a[1] - b[1]) < 1e-9 * abs(a[1])
+    assert abs(a[2] - b[2]) < 1e-9 * abs(a[2])
+    assert rel(a[3].cpu().numpy(), b[3].cpu().numpy()) < 1e-9

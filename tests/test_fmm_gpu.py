@@ -380,3 +380,34 @@ def test_direct_subset_against_long_double_sums(torch, variant):
     else:
         ref = oracle.direct(variant, nd, z.real, z.imag, dz, dens, h, tgt)
     assert rel(got, ref) < 1e-13
+
+
+# ------------------------------------------------------------------------------------------
+# The DAPIF2 drop-in (IFLAG7 = 3): FIELD, IERR and INFORM against the reference DAPIF2
+# (tests/golden/dapif2_hilbert.npz), in the O(N^2) and the FMM modes.
+# ------------------------------------------------------------------------------------------
+
+@pytest.mark.parametrize("mode", [eb.MODE_AUTO, eb.MODE_FMM])
+@pytest.mark.parametrize("tag", ["lap", "pv", "uniform"])
+def test_dapif2_dropin_matches_reference(torch, tag, mode):
+    g = golden("dapif2_hilbert")
+    eb.set_mode(mode)
+    try:
+        print("dapif2 drop-in", tag, mode, flush=True)
+        field, ierr, inform = eb.dapif2(g[f"x_{tag}"], g[f"y_{tag}"], g[f"qa_{tag}"], napb=30,
+                                        tol=float(g[f"tol_{tag}"]), nsp=int(g[f"ierr_{tag}"][2]))
+    finally:
+        eb.set_mode(eb.MODE_AUTO)
+    np.testing.assert_array_equal(ierr, g[f"ierr_{tag}"])
+    np.testing.assert_array_equal(inform, g[f"inform_{tag}"])  # NTERMS, NBOX, levels, workspace used
+    assert rel(field, g[f"field_{tag}"]) < TOL_REL
+
+
+@pytest.mark.parametrize("tag", ["tol", "few", "nsp", "nsp_last", "nsp_half", "nsp_tables"])
+def test_dapif2_dropin_error_codes_match_reference(torch, tag):
+    g = golden("dapif2_hilbert")
+    napb, nsp = (int(v) for v in g[f"err_{tag}_args"])
+    _, ierr, inform = eb.dapif2(g["x_lap"], g["y_lap"], g["qa_lap"], napb=napb, tol=float(g[f"err_{tag}_tol"]),
+                                nsp=nsp if nsp > 0 else None)
+    np.testing.assert_array_equal(ierr, g[f"err_{tag}_ierr"], err_msg=tag)
+    np.testing.assert_array_equal(inform, g[f"err_{tag}_inform"], err_msg=tag)

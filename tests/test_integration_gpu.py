@@ -2,8 +2,9 @@
 
 oracle/_ref/libref_on_eb.so holds the reference's own objects with cadapdrop.o, cadapmod.o and
 cadap.o left out, linked against the ISO_C_BINDING shim (electricbubbles_amd/libeb_fortran.so):
-FASMVP, INTERFACENEW, F and RK2 run exactly as written and reach the GPU through the
-CADAPDR / CADAPMOD names.  The same calls on oracle/_ref/libref.so (the reference as written)
+FASMVP, INTERFACENEW, FASMVP_LAP, PVINTEV, F and RK2 run exactly as written and reach the GPU
+through the CADAPDR / CADAPMOD names and, with the reference's own DAPIF2 made local to its
+object, through the DAPIF2 name.  The same calls on oracle/_ref/libref.so (the reference as written)
 are the checker.
 """
 import os
@@ -84,3 +85,31 @@ def test_single_drop_trajectory_matches_reference(sims, dropin_mode):
     d_ref = (r_ref.max() - r_ref.min()) / (r_ref.max() + r_ref.min())
     d_gpu = (r_gpu.max() - r_gpu.min()) / (r_gpu.max() + r_gpu.min())
     assert abs(d_gpu - d_ref) < 1e-8
+
+
+@pytest.mark.parametrize("tag", ["two_drops", "four_circles", "underresolved"])
+def test_laplace_operator_through_shim_matches_reference(sims, dropin_mode, tag):
+    """FASMVP_LAP (laplace.f:159-262) as written, its DAPIF2 call (laplace.f:214-215) on the GPU,
+    against the reference's own output on the same smooth density."""
+    ref, gpu = sims
+    f = np.load(os.path.join(GOLDEN, f"field_{tag}.npz"))
+    nd = f["nd"]
+    geo = oracle.extractdat(nd, f["rku"])
+    outs = []
+    for s in (ref, gpu):
+        outs.append(s.fasmvp_lap(nd, geo["zk"], geo["x"] + 1j * geo["y"], geo["dz"], geo["rkappa"], geo["dsda"],
+                                 f["lap_u"]))
+    assert rel(outs[1], outs[0]) < 1e-10
+    assert rel(outs[1], f["lap_w"]) < 1e-10
+
+
+@pytest.mark.parametrize("tag", ["two_drops", "underresolved"])
+def test_whole_rhs_through_shim_matches_reference(sims, dropin_mode, tag):
+    """The reference's F as written, with every CADAPDR, CADAPMOD and DAPIF2 call on the GPU."""
+    ref, gpu = sims
+    f = np.load(os.path.join(GOLDEN, f"field_{tag}.npz"))
+    for s in (ref, gpu):
+        s.setup(f["nd"], cap=float(f["cap"]), e0=float(f["e0"]))
+    r, g = ref.rhs(f["rku"]), gpu.rhs(f["rku"])
+    assert rel(g, r) < 1e-8
+    assert rel(g, f["rkup"]) < 1e-8
