@@ -279,7 +279,8 @@ def sweep_leg(torch, dev, with_cpu):
     reported per solve.  The reference leg runs SOLVEI's solve of a few simulations on one
     host core.  The "F" part does the same for whole F evaluations (electric.f:968-1109: the
     Laplace solve, the Dirichlet-to-Neumann map, SETVELF, SOLVEI, INTERFACENEW and the tail),
-    with E0 = |E|^2, |E| uniform in 0.1..2 per simulation and no background flow."""
+    with E0 = |E|^2, |E| uniform in 0.1..2 and Ca (CAP, the imposed straining flow) log-uniform in
+    0.01..0.5 per simulation."""
     from electricbubbles_amd import gmres, spectral
     from electricbubbles_amd.workloads import perturbed_circle
     rng = np.random.default_rng(20260512)
@@ -343,8 +344,11 @@ def sweep_leg(torch, dev, with_cpu):
     # whole F evaluations (electric.f:968-1109) of every simulation: batched and plain
     from electricbubbles_amd import field
     e0 = rng.uniform(0.1, 2.0, nsim) ** 2  # E0 = |E|^2, |E| uniform in 0.1..2
+    # Ca: the reference's CAP, the strength of the imposed straining flow (stokesbubb.f:63-67,
+    # electric.f:869), log-uniform in 0.01..0.5 per simulation
+    cap = np.exp(rng.uniform(np.log(0.01), np.log(0.5), nsim))
     rku_t = torch.tensor(rku, device=dev)
-    SF = field.SweepField(B, np.arange(nsim + 1), np.zeros(nsim), e0)
+    SF = field.SweepField(B, np.arange(nsim + 1), cap, e0)
     SF.rhs(rku_t)  # warm-up (first geometry)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -356,7 +360,7 @@ def sweep_leg(torch, dev, with_cpu):
     t_fp, worst_f = 0.0, 0.0
     for s in sample:
         p0, p1 = off[s], off[s + 1]
-        Fs = field.Field(spectral.Bodies(nd[s:s + 1]), cap=0.0, e0=float(e0[s]))
+        Fs = field.Field(spectral.Bodies(nd[s:s + 1]), cap=float(cap[s]), e0=float(e0[s]))
         r = rku_t[2 * p0:2 * p1].contiguous()
         Fs.rhs(r)  # warm-up
         torch.cuda.synchronize()
@@ -379,7 +383,7 @@ def sweep_leg(torch, dev, with_cpu):
     # hipGraph (device-controlled solves, at most 8 Arnoldi steps each)
     gst = torch.cuda.Stream()
     Bg = spectral.Bodies(nd, stream=gst)
-    SFg = field.SweepField(Bg, np.arange(nsim + 1), np.zeros(nsim), e0, stream=gst)
+    SFg = field.SweepField(Bg, np.arange(nsim + 1), cap, e0, stream=gst)
     torch.cuda.synchronize()
     dt = 1e-3
     with torch.cuda.stream(gst):
@@ -432,7 +436,7 @@ def sweep_leg(torch, dev, with_cpu):
             try:
                 for s in sample[:4]:
                     p0, p1 = off[s], off[s + 1]
-                    sim.setup(nd[s:s + 1], cap=0.0, e0=float(e0[s]))
+                    sim.setup(nd[s:s + 1], cap=float(cap[s]), e0=float(e0[s]))
                     t0 = time.perf_counter()
                     sim.rhs(rku[2 * p0:2 * p1])
                     t_rf += time.perf_counter() - t0
@@ -526,7 +530,7 @@ def adjoint_leg(torch, dev, with_cpu, steps=200):
     tolerance 1e-12 in both directions.  Forward alone, then the adjoint (forward with
     checkpoints every 20 steps, reverse sweep with recomputation), and a spot check of the
     gradient along a random direction of all 1026 parameters against Richardson-extrapolated
-    central differences of the same forward over the first 10 steps."""
+    central differences of the same forward over the first 20 steps."""
     from electricbubbles_amd import adjoint, field, spectral
     P = adjoint.Problem(16, 512, tol=1e-12)
     centres, modes = adjoint.grid_problem(4, 512, 4.0, 64, 0.02, seed=5, device=dev)
@@ -545,8 +549,8 @@ def adjoint_leg(torch, dev, with_cpu, steps=200):
     loss, gE, gCa, gm, tim = adjoint.gradient(P, centres, modes, E, Ca, steps, dt, checkpoint=20, decay=decay)
     log(f"adjoint: gradient in {tim['total_s']:.1f} s")
     it_grad = {k: float(np.mean([a for a, _ in v])) for k, v in P.sys.iters.items() if v}
-    # spot check over 10 steps along a random unit direction (dE, dCa, dmodes)
-    ns = min(10, steps)
+    # spot check over 20 steps along a random unit direction (dE, dCa, dmodes)
+    ns = min(20, steps)
     rng = np.random.default_rng(9)
     v = rng.standard_normal(2 + modes.numel())
     v /= np.linalg.norm(v)

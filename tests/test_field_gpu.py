@@ -194,15 +194,17 @@ def test_sweep_field_matches_plain_fields(torch):
 
 def test_sweep_field_512_single_drops(torch):
     """The batched-sweep shapes (512 single drops, N log-uniform 128..2048 rounded to even,
-    E0 = |E|^2 with |E| uniform in 0.1..2 per simulation, no background flow): every solve
-    converges and F is finite."""
+    E0 = |E|^2 with |E| uniform in 0.1..2 and CAP log-uniform in 0.01..0.5 per simulation):
+    every solve converges and F is finite."""
     from electricbubbles_amd import field, spectral
     from electricbubbles_amd.workloads import perturbed_circle
     rng = np.random.default_rng(5120)
     nd = (np.round(np.exp(rng.uniform(np.log(128), np.log(2048), 512)) / 2) * 2).astype(np.int32)
     print("sweep field, 512 drops, points", int(nd.sum()), flush=True)
     B = spectral.Bodies(nd)
-    SF = field.SweepField(B, np.arange(513), np.zeros(512), rng.uniform(0.1, 2.0, 512) ** 2)
+    e0 = rng.uniform(0.1, 2.0, 512) ** 2
+    cap = np.exp(rng.uniform(np.log(0.01), np.log(0.5), 512))  # Ca log-uniform in 0.01..0.5
+    SF = field.SweepField(B, np.arange(513), cap, e0)
     rku = torch.tensor(np.concatenate([perturbed_circle(int(n), [(2, 0.05)])[0] for n in nd]).view(np.float64).copy(),
                        device="cuda")
     out = SF.rhs(rku)
