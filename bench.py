@@ -424,6 +424,41 @@ def sweep_leg(torch, dev, with_cpu):
                                "simulation_steps_per_s": round(nsim * nrun / t_run, 1),
                                "steps_redone": int(redone_run), "state_finite": run_finite}}
     SFg.close()
+    # the plain path at its best: one simulation per field, its Heun step captured in a hipGraph
+    # with device-controlled solves of the one system (eb_field_rk2_graph), same budget
+    t_pg, t_pe, red_p, worst_p = 0.0, 0.0, 0, 0.0
+    for s in sample:
+        p0, p1 = off[s], off[s + 1]
+        Fp = field.Field(spectral.Bodies(nd[s:s + 1], stream=gst), cap=float(cap[s]), e0=float(e0[s]), stream=gst)
+        with torch.cuda.stream(gst):
+            ye = rku_t[2 * p0:2 * p1].clone()
+            yg = ye.clone()
+            Fp.rk2_graph(yg, dt)  # eager F (plans), capture, first replay
+            phi0, y = spectral.rk2_stage1(dt, ye, Fp.rhs(ye), stream=gst)
+            spectral.rk2_stage2(dt, phi0, Fp.rhs(y), ye, stream=gst)
+            gst.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                phi0, y = spectral.rk2_stage1(dt, ye, Fp.rhs(ye), stream=gst)
+                spectral.rk2_stage2(dt, phi0, Fp.rhs(y), ye, stream=gst)
+            gst.synchronize()
+            t_pe += (time.perf_counter() - t0) / reps
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                red_p += Fp.rk2_graph(yg, dt)
+            gst.synchronize()
+            t_pg += (time.perf_counter() - t0) / reps
+            worst_p = max(worst_p, float((yg - ye).norm() / ye.norm()))
+        Fp.close()
+    torch.cuda.synchronize()
+    pg = len(sample) / t_pg  # simulation steps per second of the plain graph path
+    out["rk2_step"]["plain"] = {
+        "what": "one simulation per field (eb_field_rk2_graph): its Heun step in a hipGraph, device-controlled "
+                "solves of the one system, budget 8", "sample": f"every 16th simulation ({len(sample)} fields)",
+        "eager_ms_per_step": round(t_pe * 1e3 / len(sample), 3), "graph_ms_per_step": round(t_pg * 1e3 / len(sample), 3),
+        "graph_steps_redone": int(red_p), "simulation_steps_per_s_graph": round(pg, 1),
+        "graph_vs_eager_state_rel_diff": worst_p,
+        "batched_graph_vs_plain_graph": round(out["rk2_step"]["simulation_steps_per_s_graph"] / pg, 2)}
     if with_cpu:
         import tempfile
         import oracle

@@ -358,11 +358,28 @@ struct eb_field_s {
   eb_fmm_plan itf = nullptr;
   double *rhs_st = nullptr, *pk = nullptr, *un = nullptr;
   double2 *u = nullptr, *za = nullptr, *udot = nullptr, *gw = nullptr;
+  // the captured RK2 step (eb_field_rk2_graph): device-controlled solves of the one system
+  eb_gmres_batch glap1 = nullptr, gst1 = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  double* g_ybar = nullptr;
+  double g_dt = 0.0;
+  int g_budget = 0;
+  double *ysave = nullptr, *yst = nullptr, *phi0 = nullptr, *dy = nullptr;
 };
+
+static void drop_graph(eb_field F) {
+  if (F->gexec) (void)hipGraphExecDestroy(F->gexec);
+  F->gexec = nullptr;
+}
 
 static void free_field(eb_field F) {
   if (!F) return;
   if (F->st) (void)hipStreamSynchronize(F->st);
+  drop_graph(F);
+  for (double* q : {F->ysave, F->yst, F->phi0, F->dy})
+    if (q) (void)hipFree(q);
+  if (F->glap1) eb_gmres_batch_destroy(F->glap1);
+  if (F->gst1) eb_gmres_batch_destroy(F->gst1);
   if (F->lap) eb_fmm_plan_destroy(F->lap);
   if (F->pv) eb_fmm_plan_destroy(F->pv);
   if (F->itf) eb_fmm_plan_destroy(F->itf);
@@ -453,6 +470,7 @@ extern "C" int eb_field_set_mode(eb_field F, int32_t mode) {
     return (eb_set_error("eb_field_set_mode: bad arguments"), EB_ERR_ARG);
   if (mode == F->mode) return EB_OK;
   if (F->st) (void)hipStreamSynchronize(F->st);
+  drop_graph(F);  // it holds the old plans
   // the plans are made again, in the new mode, by the next eb_field_set_geometry
   eb_fmm_plan* plans[] = {&F->lap, &F->pv, &F->itf};
   for (eb_fmm_plan* q : plans)
@@ -515,6 +533,20 @@ extern "C" int eb_field_laplace_apply(eb_field F, const double* u, double* w) {
 static int lap_gmres_apply(void* ctx, const double* x, double* y, void* stream) {
   (void)stream;
   return eb_field_laplace_apply((eb_field)ctx, x, y);
+}
+
+// SOLVE_LAPL with the device-controlled DGMRES of one system (at most `budget` Arnoldi steps;
+// no host synchronisation): the same steps and tests as eb_gmres_solve, the iteration count
+// and IERR stay on the device (eb_gmres_batch_results).  The pair sums of the products after
+// convergence are skipped through the plan's guard.
+static int laplace_solve_async(eb_field F, double* ulap, int budget) {
+  const int n = F->n;
+  hipLaunchKernelGGL(k_getbcs, dim3(ew_blocks(n)), dim3(EW), 0, F->st, n, F->x, F->rhs_lap, ulap);
+  EB_HIP_CHECK(hipGetLastError());
+  eb_fmm_set_guard(F->lap, eb_gmres_batch_active(F->glap1));
+  const int rc = eb_gmres_batch_solve_async(F->glap1, lap_gmres_apply, F, F->rhs_lap, ulap, 1.0e-10, budget);
+  eb_fmm_set_guard(F->lap, nullptr);
+  return rc;
 }
 
 extern "C" int eb_field_laplace_solve(eb_field F, double* ulap, int32_t* iter, int32_t* ierr) {
@@ -597,17 +629,116 @@ extern "C" int eb_field_interface(eb_field F, const double* u, double* un) {
   return EB_OK;
 }
 
-extern "C" int eb_field_rhs(eb_field F, const double* rku, double* rkup) {
-  if (!F || !rku || !rkup) return (eb_set_error("eb_field_rhs: bad arguments"), EB_ERR_ARG);
+// SOLVEI with the device-controlled DGMRES of one system: soln = rhs as the initial guess,
+// then za and pk of the solution (stokesbubb.f:343-409), as eb_stokes_solve
+static int stokes_solve_async(eb_field F, int budget) {
+  void* s = (void*)F->st;
+  const size_t nu = 2 * (size_t)F->n;
+  EB_HIP_CHECK(hipMemcpyAsync(F->u, F->rhs_st, sizeof(double) * nu, hipMemcpyDeviceToDevice, F->st));
+  eb_matvec_set_guard(F->mv, eb_gmres_batch_active(F->gst1));
+  int rc = eb_gmres_batch_solve_async(F->gst1, eb_matvec_gmres_apply, F->mv, F->rhs_st, (double*)F->u, 1.0e-10, budget);
+  eb_matvec_set_guard(F->mv, nullptr);
+  if (rc != EB_OK) return rc;
+  return eb_matvec_parts(F->mv, (const double*)F->u, (double*)F->za, F->pk, s);
+}
+
+// F (electric.f:968-1109).  budget = 0: the host-controlled solves; budget > 0: the
+// device-controlled ones with at most `budget` Arnoldi steps each and no host synchronisation
+// (the graph-capturable form; the plans and the workspaces must exist already).
+static int field_rhs(eb_field F, const double* rku, double* rkup, int budget) {
   int rc;
   int32_t it, ie;
   if ((rc = eb_field_set_geometry(F, rku)) != EB_OK) return rc;
-  if ((rc = eb_field_laplace_solve(F, F->ulap, &it, &ie)) != EB_OK) return rc;
+  rc = budget > 0 ? laplace_solve_async(F, F->ulap, budget) : eb_field_laplace_solve(F, F->ulap, &it, &ie);
+  if (rc != EB_OK) return rc;
   if ((rc = eb_field_dirich_neu(F, F->ulap, F->phi, F->psi, F->dphi)) != EB_OK) return rc;
   if ((rc = eb_field_setvelf(F, F->dphi, F->rhs_st)) != EB_OK) return rc;
-  if ((rc = eb_field_stokes_solve(F, F->rhs_st, (double*)F->u, (double*)F->za, F->pk, &it, &ie)) != EB_OK) return rc;
+  rc = budget > 0 ? stokes_solve_async(F, budget)
+                  : eb_field_stokes_solve(F, F->rhs_st, (double*)F->u, (double*)F->za, F->pk, &it, &ie);
+  if (rc != EB_OK) return rc;
   if ((rc = eb_field_interface(F, (const double*)F->u, F->un)) != EB_OK) return rc;
   return eb_ftail(F->B, F->tha, F->un, (const double*)F->dz, F->dsda, rkup, (void*)F->st);
+}
+
+extern "C" int eb_field_rhs(eb_field F, const double* rku, double* rkup) {
+  if (!F || !rku || !rkup) return (eb_set_error("eb_field_rhs: bad arguments"), EB_ERR_ARG);
+  return field_rhs(F, rku, rkup, 0);
+}
+
+// One Heun step (electric.f:773-783) on the field's buffers: phi0 = dt F(ybar), y = ybar + phi0,
+// ybar += phi0/2 + dt F(y)/2, with the RK2 kernels' roundings.
+static int field_rk2_step(eb_field F, double* ybar, double dt, int budget) {
+  const int neq = 2 * F->n;
+  void* s = (void*)F->st;
+  int rc;
+  if ((rc = field_rhs(F, ybar, F->dy, budget)) != EB_OK) return rc;
+  if ((rc = eb_rk2_stage1(neq, dt, ybar, F->dy, F->phi0, F->yst, s)) != EB_OK) return rc;
+  if ((rc = field_rhs(F, F->yst, F->dy, budget)) != EB_OK) return rc;
+  return eb_rk2_stage2(neq, dt, F->phi0, F->dy, ybar, s);
+}
+
+extern "C" int eb_field_rk2_graph(eb_field F, double* ybar, double dt, int32_t budget, int32_t* redone) {
+  if (!F || !ybar || budget < 1) return (eb_set_error("eb_field_rk2_graph: bad arguments"), EB_ERR_ARG);
+  if (F->st == nullptr)
+    return (eb_set_error("eb_field_rk2_graph: the field must use a non-default stream (graph capture)"), EB_ERR_ARG);
+  const size_t neq = 2 * (size_t)F->n;
+  int rc;
+  if (!F->ysave) {
+    if ((rc = dalloc(&F->ysave, neq)) || (rc = dalloc(&F->yst, neq)) || (rc = dalloc(&F->phi0, neq)) ||
+        (rc = dalloc(&F->dy, neq)))
+      return rc;
+  }
+  if (!F->glap1) {
+    const int32_t ol[2] = {0, F->n}, ou[2] = {0, 2 * F->n};
+    // MAXL 100 as SOLVE_LAPL and SOLVEI (laplace.f:355-384, electric.f:1011,1060)
+    if ((rc = eb_gmres_batch_create(&F->glap1, 1, ol, 100, (void*)F->st)) != EB_OK ||
+        (rc = eb_gmres_batch_create(&F->gst1, 1, ou, 100, (void*)F->st)) != EB_OK)
+      return rc;
+  }
+  // the plans are made (with synchronisations) by an eager F before the first capture
+  if ((!F->have_geom || !F->pv) && (rc = field_rhs(F, ybar, F->dy, 0)) != EB_OK) return rc;
+  if (!F->gexec || F->g_ybar != ybar || F->g_dt != dt || F->g_budget != budget) {
+    drop_graph(F);
+    EB_HIP_CHECK(hipStreamSynchronize(F->st));
+    hipGraph_t graph = nullptr;
+    EB_HIP_CHECK(hipStreamBeginCapture(F->st, hipStreamCaptureModeThreadLocal));
+    int crc = hipMemcpyAsync(F->ysave, ybar, sizeof(double) * neq, hipMemcpyDeviceToDevice, F->st) == hipSuccess
+                  ? EB_OK
+                  : EB_ERR_HIP;
+    if (crc == EB_OK) crc = eb_gmres_batch_clear_fail(F->glap1);
+    if (crc == EB_OK) crc = eb_gmres_batch_clear_fail(F->gst1);
+    if (crc == EB_OK) crc = field_rk2_step(F, ybar, dt, budget);
+    const hipError_t e = hipStreamEndCapture(F->st, &graph);
+    if (crc != EB_OK || e != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      if (crc == EB_OK) eb_set_error("eb_field_rk2_graph: capture failed: %s", hipGetErrorString(e));
+      return crc != EB_OK ? crc : EB_ERR_HIP;
+    }
+    const hipError_t ie = hipGraphInstantiate(&F->gexec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ie != hipSuccess) {
+      F->gexec = nullptr;
+      return (eb_set_error("eb_field_rk2_graph: instantiate failed: %s", hipGetErrorString(ie)), EB_ERR_HIP);
+    }
+    F->g_ybar = ybar;
+    F->g_dt = dt;
+    F->g_budget = budget;
+  }
+  EB_HIP_CHECK(hipGraphLaunch(F->gexec, F->st));
+  int32_t fl = 0, fs = 0;
+  if ((rc = eb_gmres_batch_results(F->glap1, &F->it_lap, nullptr, &F->ie_lap, &fl)) != EB_OK ||
+      (rc = eb_gmres_batch_results(F->gst1, &F->it_st, nullptr, &F->ie_st, &fs)) != EB_OK)
+    return rc;
+  if (redone) *redone = 0;
+  if (fl + fs > 0) {
+    // a solve needed more than `budget` Arnoldi steps: the step again from the saved state
+    // with the host-controlled solves
+    EB_HIP_CHECK(hipMemcpyAsync(ybar, F->ysave, sizeof(double) * neq, hipMemcpyDeviceToDevice, F->st));
+    if ((rc = field_rk2_step(F, ybar, dt, 0)) != EB_OK) return rc;
+    EB_HIP_CHECK(hipStreamSynchronize(F->st));
+    if (redone) *redone = 1;
+  }
+  return EB_OK;
 }
 
 extern "C" int eb_field_iterations(eb_field F, int32_t* lap_iter, int32_t* lap_ierr, int32_t* stokes_iter,

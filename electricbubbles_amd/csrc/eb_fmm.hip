@@ -1032,7 +1032,8 @@ constexpr int DIR_MAXBLK = 4096;  // target tiles x chunks at most (fewer, longe
 
 __global__ __launch_bounds__(DIR_T) void k_direct_part(int n, int sch, const double2* __restrict__ zs,
                                                        const double2* __restrict__ qa, const double2* __restrict__ qb,
-                                                       double2* __restrict__ part) {
+                                                       const int* __restrict__ guard, double2* __restrict__ part) {
+  if (guard && !*guard) return;  // the solver has stopped iterating: the product is not used
   __shared__ double2 sz[DIR_T], sa[DIR_T], sbq[DIR_T];
   const int i = blockIdx.x * DIR_T + threadIdx.x;
   const int c = blockIdx.y;
@@ -1187,7 +1188,8 @@ extern "C" int eb_direct_apply_noalloc(int32_t n, const double* x, const double*
   hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(BB_THREADS), 0, st, bb, bbblocks, g);
   hipLaunchKernelGGL(k_na_prep, dim3((n + 255) / 256), dim3(256), 0, st, n, x, y, (const double2*)wdz,
                      (const double2*)dens, g, sqb, variant == EB_VARIANT_HILBERT ? 1 : 0, zs, qa, qb);
-  hipLaunchKernelGGL(k_direct_part, dim3((n + DIR_T - 1) / DIR_T, nch), dim3(DIR_T), 0, st, n, sch, zs, qa, qb, part);
+  hipLaunchKernelGGL(k_direct_part, dim3((n + DIR_T - 1) / DIR_T, nch), dim3(DIR_T), 0, st, n, sch, zs, qa, qb,
+                     (const int*)nullptr, part);
   hipLaunchKernelGGL(k_direct_sum, dim3((n + 255) / 256), dim3(256), 0, st, n, nch, part, (const int32_t*)nullptr, g,
                      sgn, (double2*)gradw);
   EB_HIP_CHECK(hipGetLastError());
@@ -1411,6 +1413,7 @@ struct eb_fmm_plan_s {
   int* ntbox = nullptr;
   double2* dir_part = nullptr;  // direct mode: 2 x n partial sums per source chunk
   int dir_sch = 0, dir_nch = 0;
+  const int* guard = nullptr;  // direct mode: device flag; 0 = skip the pair sums (eb_fmm_set_guard)
   int* m2l_list = nullptr;   // compacted M2L targets (k_m2l_count, k_m2l_write), 4 x m2l_cap
   int* m2l_count = nullptr;  // 4
   int* m2l_ccnt = nullptr;   // per (class, chunk of 1024 boxes) counts, 4 x m2l_nb
@@ -1816,7 +1819,7 @@ extern "C" int eb_fmm_apply(eb_fmm_plan P, const double* dens, double* gradw, vo
     if (P->timing)
       for (int i = 1; i <= 6; ++i) (void)hipEventRecord(P->ev[i], st);
     hipLaunchKernelGGL(k_direct_part, dim3((n + DIR_T - 1) / DIR_T, P->dir_nch), dim3(DIR_T), 0, st, n, P->dir_sch,
-                       P->zs, P->qa, P->qb, P->dir_part);
+                       P->zs, P->qa, P->qb, P->guard, P->dir_part);
     hipLaunchKernelGGL(k_direct_sum, dim3((n + 255) / 256), dim3(256), 0, st, n, P->dir_nch, P->dir_part, P->perm,
                        P->geom, P->sgn, out);
   } else if (P->L >= 2) {
@@ -1871,6 +1874,12 @@ extern "C" int eb_fmm_set_p2p_tile(eb_fmm_plan P, int32_t tmax) {
   }
   P->p2p_tmax = tmax;
   return build_p2p_items(P);
+}
+
+extern "C" int eb_fmm_set_guard(eb_fmm_plan P, const int32_t* flag) {
+  if (!P) return (eb_set_error("eb_fmm_set_guard: null plan"), EB_ERR_ARG);
+  P->guard = flag;
+  return EB_OK;
 }
 
 extern "C" int eb_fmm_plan_destroy(eb_fmm_plan P) {

@@ -627,6 +627,11 @@ extern "C" int eb_matvec_parts(eb_matvec M, const double* xx, double* za, double
   return EB_OK;
 }
 
+extern "C" int eb_matvec_set_guard(eb_matvec M, const int32_t* flag) {
+  if (!M) return (eb_set_error("eb_matvec_set_guard: bad arguments"), EB_ERR_ARG);
+  return eb_fmm_set_guard(M->fmm, flag);
+}
+
 extern "C" int eb_matvec_destroy(eb_matvec M) {
   free_mv(M);
   return EB_OK;

@@ -16,7 +16,7 @@ FIELD_EXPORTS = ("eb_field_create", "eb_field_destroy", "eb_field_set_mode", "eb
                  "eb_field_laplace_apply", "eb_field_laplace_solve", "eb_field_dirich_neu", "eb_field_setvelf",
                  "eb_field_stokes_solve", "eb_field_interface", "eb_field_get_geometry", "eb_field_iterations", "eb_sweep_field_create",
                  "eb_sweep_field_rhs", "eb_sweep_field_iterations", "eb_sweep_field_destroy",
-                 "eb_sweep_field_rk2_graph")
+                 "eb_sweep_field_rk2_graph", "eb_field_rk2_graph")
 
 _vp = ctypes.c_void_p
 _i32 = ctypes.c_int32
@@ -44,6 +44,7 @@ def _bind():
         L.eb_sweep_field_iterations.argtypes = [_vp, _vp, _vp, _vp, _vp]
         L.eb_sweep_field_destroy.argtypes = [_vp]
         L.eb_sweep_field_rk2_graph.argtypes = [_vp, _vp, ctypes.c_double, _i32, ctypes.POINTER(_i32)]
+        L.eb_field_rk2_graph.argtypes = [_vp, _vp, ctypes.c_double, _i32, ctypes.POINTER(_i32)]
         _bound = True
     return L
 
@@ -120,6 +121,16 @@ class Field:
         v = [_i32() for _ in range(4)]
         _check(_bind().eb_field_iterations(self._h, *(ctypes.byref(a) for a in v)))
         return tuple(a.value for a in v)
+
+    def rk2_graph(self, ybar, dt, budget=8):
+        """One Heun step in place on ybar (a contiguous device tensor), captured in a hipGraph on
+        the first call and replayed after, with device-controlled solves (eb_field_rk2_graph).
+        The field must have been created with a non-default stream; returns True when the step
+        was redone with the host-controlled solves (a solve needed more than `budget` steps)."""
+        assert ybar.is_contiguous() and ybar.dtype == self._torch.float64
+        red = _i32()
+        _check(_bind().eb_field_rk2_graph(self._h, _ptr(ybar), float(dt), int(budget), ctypes.byref(red)))
+        return bool(red.value)
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:

@@ -173,6 +173,12 @@ int eb_fmm_enable_timing(eb_fmm_plan plan, int32_t on);
  * (tmax in 16..256, 42 by default); the items are rebuilt for the current tree.  A tuning
  * knob of this build (no counterpart in the reference): results change only by rounding. */
 int eb_fmm_set_p2p_tile(eb_fmm_plan plan, int32_t tmax);
+/* Direct mode: a device int read by the pair-sum kernel of every later apply; where it holds 0
+ * the kernel returns at once and the apply's output is left undefined (NULL: always sum).  For
+ * operators inside a device-controlled solve, whose products after convergence are not used
+ * (the solver's active flag, eb_gmres_batch_active).  No counterpart in the reference, whose
+ * solver stops on the host.  FMM-mode plans ignore it. */
+int eb_fmm_set_guard(eb_fmm_plan plan, const int32_t *flag);
 int eb_fmm_stage_ms(eb_fmm_plan plan, double *ms7);
 
 /* ---------------------------------------------------------------------------------------

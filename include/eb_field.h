@@ -43,6 +43,18 @@ int eb_field_set_mode(eb_field f, int32_t mode);
 /* rkup = F(rku): the whole chain above, on the field's stream. */
 int eb_field_rhs(eb_field f, const double *rku, double *rkup);
 
+/* One Heun RK2 step (electric.f:773-783), ybar += (dt F(ybar) + dt F(y)) / 2, captured once into
+ * a hipGraph and replayed on later calls: the Laplace and Stokes solves of both F run the
+ * device-controlled DGMRES of one system (eb_gmres_batch_solve_async, nsys = 1) with at most
+ * `budget` Arnoldi steps each, so the step has no host synchronisation; in direct mode the
+ * pair sums of the products after convergence are skipped (eb_fmm_set_guard).  After the
+ * replay the solves' failure counters are read; if a solve needed more than `budget` steps,
+ * ybar is restored and the step is run again with the host-controlled solves (*redone = 1).
+ * ybar: 2 nbk doubles on the device, updated in place.  The graph is rebuilt when ybar, dt or
+ * budget change, or after eb_field_set_mode.  The field must have been created on a
+ * non-default stream.  No counterpart in the reference (its solver stops on the host). */
+int eb_field_rk2_graph(eb_field f, double *ybar, double dt, int32_t budget, int32_t *redone);
+
 /* The stages one at a time, for tests and for callers that replace part of the chain.
  * eb_field_set_geometry runs EXTRACTDAT on rku and rebuilds every FMM tree; the others act on
  * that geometry. */

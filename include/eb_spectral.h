@@ -80,6 +80,8 @@ int eb_matvec_apply(eb_matvec mv, const double *xx, double *yy, double *stress, 
  * rebuilds the FMM tree. */
 int eb_matvec_rebuild(eb_matvec mv, const double *x, const double *y, const double *dz, const double *dsdth,
                       const double *rkappa, const double *zk, void *stream);
+/* The layer potential's guard flag (eb_fmm_set_guard) for later applies; NULL clears it. */
+int eb_matvec_set_guard(eb_matvec mv, const int32_t *flag);
 int eb_matvec_destroy(eb_matvec mv);
 
 /* ---------------------------------------------------------------------------------------

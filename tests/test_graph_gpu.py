@@ -82,3 +82,74 @@ def test_graph_budget_fallback(torch):
         st.synchronize()
     assert redone
     assert float((yg - ye).norm() / ye.norm()) < 1e-13
+
+
+def _plain_case(seed=3):
+    """One simulation of three bodies (a plain field): 384, 256 and 512 points."""
+    rng = np.random.default_rng(seed)
+    nd = np.array([384, 256, 512], np.int32)
+    shifts = [0.0, 2.6 + 0.4j, -0.3 + 2.7j]
+    parts = []
+    for n, c in zip(nd, shifts):
+        parts.append(perturbed_circle(int(n), [(2, 0.05), (3, 0.02 * rng.uniform())], center=c)[0])
+    return nd, np.concatenate(parts).view(np.float64).copy()
+
+
+def _plain_eager_step(F, spectral, ybar, dt, stream):
+    phi0, y = spectral.rk2_stage1(dt, ybar, F.rhs(ybar), stream=stream)
+    spectral.rk2_stage2(dt, phi0, F.rhs(y), ybar, stream=stream)
+
+
+@pytest.mark.parametrize("mode", ["auto", "fmm"])
+def test_plain_graph_step_matches_eager(torch, mode):
+    """eb_field_rk2_graph: the device-controlled solves of one system give the host-controlled
+    solves' iteration counts and, up to the reductions' rounding, their state; every replay
+    matches the eager step, with eager work of the same field between replays.  In FMM mode the
+    captured step holds the near field's second stream."""
+    import electricbubbles_amd as eb
+    from electricbubbles_amd import field, spectral
+    nd, rku = _plain_case()
+    print("plain graph step:", mode, nd.tolist(), flush=True)
+    st = torch.cuda.Stream()
+    Fg = field.Field(spectral.Bodies(nd, stream=st), cap=0.1, e0=1.5, stream=st)
+    Fe = field.Field(spectral.Bodies(nd, stream=st), cap=0.1, e0=1.5, stream=st)
+    if mode == "fmm":
+        Fg.set_mode(eb.MODE_FMM)
+        Fe.set_mode(eb.MODE_FMM)
+    y0 = torch.tensor(rku, device="cuda")
+    torch.cuda.synchronize()
+    yg, ye = y0.clone(), y0.clone()
+    dt = 1e-3
+    with torch.cuda.stream(st):
+        for step in range(4):
+            redone = Fg.rk2_graph(yg, dt, budget=40)
+            it_g = Fg.iterations()
+            _plain_eager_step(Fe, spectral, ye, dt, st)
+            it_e = Fe.iterations()
+            Fg.rhs(ye)  # eager work of the graph's own field between replays
+            st.synchronize()
+            print("step", step, "iterations graph", it_g, "eager", it_e, "redone", redone, flush=True)
+            assert not redone
+            assert it_g == it_e, (step, it_g, it_e)
+            d = float((yg - ye).norm() / ye.norm())
+            print("step", step, "iterations", it_g, "rel diff", d, flush=True)
+            assert d < 1e-12
+    torch.cuda.synchronize()
+
+
+def test_plain_graph_budget_fallback(torch):
+    from electricbubbles_amd import field, spectral
+    nd, rku = _plain_case(seed=4)
+    print("plain graph budget fallback:", nd.tolist(), flush=True)
+    st = torch.cuda.Stream()
+    Fg = field.Field(spectral.Bodies(nd, stream=st), cap=0.1, e0=1.5, stream=st)
+    Fe = field.Field(spectral.Bodies(nd, stream=st), cap=0.1, e0=1.5, stream=st)
+    y0 = torch.tensor(rku, device="cuda")
+    torch.cuda.synchronize()
+    yg, ye = y0.clone(), y0.clone()
+    with torch.cuda.stream(st):
+        redone = Fg.rk2_graph(yg, 1e-3, budget=2)
+        _plain_eager_step(Fe, spectral, ye, 1e-3, st)
+        st.synchronize()
+    assert redone
+    assert float((yg - ye).norm() / ye.norm()) < 1e-13
