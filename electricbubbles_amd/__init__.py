@@ -47,7 +47,9 @@ EXPORTS = (
     "eb_last_error", "eb_version", "eb_cadapdr", "eb_cadapmod", "eb_cadap", "eb_set_mode", "eb_get_mode",
     "eb_fmm_plan_create", "eb_fmm_plan_rebuild", "eb_fmm_apply", "eb_fmm_plan_destroy", "eb_fmm_plan_info",
     "eb_fmm_plan_ntbox", "eb_fmm_enable_timing", "eb_fmm_stage_ms", "eb_fmm_set_p2p_tile", "eb_cadap_tables",
-    "eb_cadap_lists", "eb_direct_apply_noalloc", "eb_direct_subset", "eb_dapif2",
+    "eb_cadap_lists", "eb_direct_apply_noalloc", "eb_direct_subset", "eb_dapif2", "eb_afmm_create",
+    "eb_afmm_apply", "eb_afmm_info", "eb_afmm_enable_timing", "eb_afmm_stage_ms", "eb_afmm_check",
+    "eb_afmm_destroy", "eb_fmm_set_guard",
 )
 
 _lib = None
@@ -92,6 +94,14 @@ def lib():
     L.eb_cadap_lists.argtypes = [_i32, _vp, _vp, _vp, _vp, _vp]
     L.eb_direct_apply_noalloc.argtypes = [_i32, _vp, _vp, _vp, _vp, _i32, _vp, _vp]
     L.eb_direct_subset.argtypes = [_i32, _vp, _vp, _vp, _vp, _i32, _i32, _vp, _vp, _vp]
+    L.eb_afmm_create.argtypes = [ctypes.POINTER(_vp), _i32, _vp, _vp, _vp, _i32, _i32, _i32, _vp]
+    L.eb_afmm_apply.argtypes = [_vp, _vp, _vp, _vp]
+    L.eb_afmm_info.argtypes = [_vp, _dp, _i64p]
+    L.eb_afmm_enable_timing.argtypes = [_vp, _i32]
+    L.eb_afmm_stage_ms.argtypes = [_vp, _dp]
+    L.eb_afmm_check.argtypes = [_vp, _i64p]
+    L.eb_afmm_destroy.argtypes = [_vp]
+    L.eb_fmm_set_guard.argtypes = [_vp, _vp]
     _lib = L
     return L
 
@@ -328,6 +338,84 @@ class FMMPlan:
             self.close()
         except Exception:
             pass
+
+
+class AFMMPlan(FMMPlan):
+    """The same sums on the adaptive tree of DAPIF2 (eb_afmm_*): childless boxes of at most
+    napb points (40 by default) in place of a uniform grid.  x, y, wdz as FMMPlan; apply(dens)
+    returns GW.  Creating the plan synchronises (tree build and host-side lists)."""
+
+    def __init__(self, x, y, wdz, nterms, napb=40, variant=VARIANT_DR, stream=None):
+        torch = _torch()
+        self._torch = torch
+        self.n = int(x.numel())
+        assert x.dtype == torch.float64 and y.dtype == torch.float64 and wdz.dtype == torch.complex128
+        assert x.is_cuda and y.is_cuda and wdz.is_cuda and y.numel() == self.n and wdz.numel() == self.n
+        self._keep = (x.contiguous(), y.contiguous(), wdz.contiguous())
+        self.nterms, self.napb, self.variant = nterms, napb, variant
+        self.stream = stream
+        h = ctypes.c_void_p()
+        s = self._enter()
+        _check(lib().eb_afmm_create(ctypes.byref(h), self.n, _ptr(self._keep[0]), _ptr(self._keep[1]),
+                                    _ptr(self._keep[2]), int(napb), int(nterms), int(variant), _stream_ptr(stream)))
+        self._leave(s, *self._keep)
+        self._h = h
+
+    def rebuild(self, x, y, wdz):
+        raise EBError(EB_ERR_UNSUPPORTED, "an adaptive plan is built once per geometry: make a new AFMMPlan")
+
+    def apply(self, dens, out=None):
+        torch = self._torch
+        assert dens.dtype == torch.complex128 and dens.numel() == self.n and dens.is_cuda
+        dens = dens.contiguous()
+        if out is None:
+            out = torch.empty(self.n, dtype=torch.complex128, device=dens.device)
+        s = self._enter()
+        _check(lib().eb_afmm_apply(self._h, _ptr(dens), _ptr(out), _stream_ptr(self.stream)))
+        self._leave(s, dens, out)
+        return out
+
+    def info(self):
+        r = np.zeros(3)
+        c = np.zeros(8, np.int64)
+        _check(lib().eb_afmm_info(self._h, r.ctypes.data_as(_dp), c.ctypes.data_as(_i64p)))
+        keys = ("boxes", "leaves", "levels", "near_pairs", "m2l_pairs", "m2p_evals", "p2l_terms", "shifted_boxes")
+        d = dict(zip(keys, (int(v) for v in c)))
+        d.update(rscal=float(r[0]), xonew=float(r[1]), yonew=float(r[2]), nterms=self.nterms, napb=self.napb)
+        return d
+
+    def scaled_points(self):
+        """The tree's coordinates (x - XONEW) * RSCAL of the plan's points, as the plan forms
+        them (the same two roundings): the sums are RSCAL times the sums there."""
+        i = self.info()
+        x, y = self._keep[0], self._keep[1]
+        return (x - i["xonew"]) * i["rscal"], (y - i["yonew"]) * i["rscal"]
+
+    def ntbox(self):
+        raise EBError(EB_ERR_UNSUPPORTED, "no uniform levels in an adaptive plan")
+
+    def enable_timing(self, on=True):
+        _check(lib().eb_afmm_enable_timing(self._h, 1 if on else 0))
+
+    def set_p2p_tile(self, tmax):
+        raise EBError(EB_ERR_UNSUPPORTED, "the adaptive near field takes one childless box per work item")
+
+    def stage_ms(self):
+        a = np.zeros(8)
+        _check(lib().eb_afmm_stage_ms(self._h, a.ctypes.data_as(_dp)))
+        return dict(zip(("charges", "p2p", "p2m", "m2m", "m2l", "p2l", "l2l", "far"), a.tolist()))
+
+    def check_lists(self):
+        """Ordered pairs of childless boxes whose interaction the passes do not carry exactly
+        once (0 for a correct plan); O(leaves^2) on the host."""
+        bad = ctypes.c_int64()
+        _check(lib().eb_afmm_check(self._h, ctypes.byref(bad)))
+        return int(bad.value)
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().eb_afmm_destroy(self._h)
+            self._h = None
 
 
 def direct_apply_noalloc(x, y, wdz, dens, out, variant=VARIANT_DR, stream=None):

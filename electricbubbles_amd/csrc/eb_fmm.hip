@@ -22,10 +22,13 @@
 #include <string.h>
 
 #include <algorithm>
+#include <array>
 #include <mutex>
 #include <vector>
 
 #include "eb_internal.h"
+#include "../../include/eb_afmm.h"
+#include "../../include/eb_atree.h"
 
 using namespace eb;
 
@@ -304,23 +307,18 @@ __device__ __forceinline__ double2 from_lane_below(double2 v) {
 // E_j t_j^(k-1) is lane k - 1's sum of E_j t_j^k, taken through DPP once after the last
 // source and multiplied by k then: three complex FMAs per source and lane.  Lanes k >= KMAX
 // (KMAX = 48) follow along without writing.
+// The sums of one box (centre c, 1/half-width ir, sorted sources s..e) into box[0..2 KMAX).
+// Any KMAX that is a multiple of 8 up to 64 works; below 48 the upper lanes idle.
 template <int KMAX>
-__global__ __launch_bounds__(64) void k_p2m_wide(int L, int p, const int* __restrict__ leaf_start,
-                                                 const double2* __restrict__ zs, const double2* __restrict__ qa,
-                                                 const double2* __restrict__ qb, double2* __restrict__ mp, int W,
-                                                 int NBP) {
-  static_assert(KMAX >= 48 && KMAX <= 64, "one lane per exponent");
+__device__ __forceinline__ void p2m_box(int p, int s, int e, double2 c, double ir, const double2* __restrict__ zs,
+                                        const double2* __restrict__ qa, const double2* __restrict__ qb,
+                                        double2* __restrict__ box) {
+  static_assert(KMAX % 8 == 0 && KMAX <= 64, "one lane per exponent");
   // chunks of 8 sources, eight lanes per source filling the powers: 7 KB of LDS per block,
   // so that six waves per SIMD fit (16-source chunks held them to three)
   constexpr int CH = 8, QK = KMAX / 8;
   __shared__ double2 P[CH][KMAX + 1];
   __shared__ double2 AB[CH][3];  // -(1/r) qa, -(1/r) qb, -(1/r^2) conj(z) qa of each source
-  const int nside = 1 << L;
-  const int ix = blockIdx.x % nside, iy = blockIdx.x / nside;
-  const uint32_t mk = morton((uint32_t)ix, (uint32_t)iy);
-  const int s = leaf_start[mk], e = leaf_start[mk + 1];
-  const double h = 64.0 / nside, ir = 2.0 / h;
-  const double2 c = make_double2((ix + 0.5) * h, (iy + 0.5) * h);
   const int lane = threadIdx.x;
   const int js = lane & (CH - 1), qk = lane / CH;
   const int k = lane < KMAX ? lane : KMAX - 1;
@@ -361,12 +359,25 @@ __global__ __launch_bounds__(64) void k_p2m_wide(int L, int p, const int* __rest
   }
   rfma(a1, fk, from_lane_below(a2));  // lane 0 takes 0 (and fk = 0)
   if (lane < KMAX) {
-    const int pb = pidx(ix, iy, W);
     const bool in = lane < p;
-    double2* box = mp + (size_t)pb * 2 * KMAX;
     box[lane] = in ? a0 : make_double2(0, 0);
     box[KMAX + lane] = in ? a1 : make_double2(0, 0);
   }
+}
+
+template <int KMAX>
+__global__ __launch_bounds__(64) void k_p2m_wide(int L, int p, const int* __restrict__ leaf_start,
+                                                 const double2* __restrict__ zs, const double2* __restrict__ qa,
+                                                 const double2* __restrict__ qb, double2* __restrict__ mp, int W,
+                                                 int NBP) {
+  static_assert(KMAX >= 48 && KMAX <= 64, "one lane per exponent");
+  const int nside = 1 << L;
+  const int ix = blockIdx.x % nside, iy = blockIdx.x / nside;
+  const uint32_t mk = morton((uint32_t)ix, (uint32_t)iy);
+  const int s = leaf_start[mk], e = leaf_start[mk + 1];
+  const double h = 64.0 / nside, ir = 2.0 / h;
+  const double2 c = make_double2((ix + 0.5) * h, (iy + 0.5) * h);
+  p2m_box<KMAX>(p, s, e, c, ir, zs, qa, qb, mp + (size_t)pidx(ix, iy, W) * 2 * KMAX);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -515,13 +526,19 @@ __device__ __forceinline__ double swap_adjacent(double v) {
 // KS + 2 per row and source, so p = 33..36 pays for 36 rows instead of 48.  Rows past
 // 16 MT + TR are not written: they are zero for every p <= 16 MT + TR (the tables vanish
 // there), and no reader goes past row 4 KS - 1 (translations) or p - 1 (k_l2p).
-template <int MODE, int KMAX, int KS, int MT, int TR>
+//
+// ADAPT: boxes of an adaptive tree (the adaptive plan below), stored box-major by box index.
+// Targets come from per-class lists (m2l_list / m2l_count / m2l_cap, no level runs: the
+// scaled translations do not depend on the level), sources from a slot table stab[target * NS
+// + slot]; a slot without a box holds nsrc, one past the last box, which the buffer resource
+// reads as zeros.
+template <int MODE, int KMAX, int KS, int MT, int TR, bool ADAPT = false>
 __device__ __forceinline__ void xlat_body(const XlatLevels& LS, const int* __restrict__ leaf_start,
                                           const double2* __restrict__ src, double2* __restrict__ dst,
                                           const double* __restrict__ Mat, const double2* __restrict__ pre,
                                           const double2* __restrict__ post, const int* __restrict__ offs,
                                           const int* __restrict__ m2l_list, const int* __restrict__ m2l_count,
-                                          int m2l_cap) {
+                                          int m2l_cap, const int* __restrict__ stab = nullptr, int nsrc = 0) {
   static_assert(KS * 4 <= KMAX && MT * 16 + TR <= KMAX, "tile exceeds the padded expansion length");
   static_assert(TR % 4 == 0 && TR <= 8 && KS * 4 <= MT * 16 + TR, "tail rows must cover the expansion");
   constexpr int NS = MODE == XL_M2L ? 27 : (MODE == XL_M2M ? 4 : 1);
@@ -534,7 +551,7 @@ __device__ __forceinline__ void xlat_body(const XlatLevels& LS, const int* __res
   __shared__ double2 Qs[NS * NM];
   // XCD-aware order: consecutive logical blocks (neighbouring rows, all four parity classes)
   // are dispatched to the same XCD and share its L2.
-  const bool compact = MODE == XL_M2L && m2l_list != nullptr;
+  const bool compact = ADAPT || (MODE == XL_M2L && m2l_list != nullptr);
   int nphys = gridDim.x;
   if (compact) {
     // logical blocks that can hold entries: 4 classes x the longest class's blocks, so that
@@ -577,13 +594,18 @@ __device__ __forceinline__ void xlat_body(const XlatLevels& LS, const int* __res
     const int idx = 16 * (b >> 2) + 4 * wv + t;
     const int ent = idx < ccount ? m2l_list[(size_t)cls * m2l_cap + idx] : -1;
     valid = ent >= 0;
-    // every valid entry of a wave has the same level (runs are padded to groups of 4)
-    int ev = valid ? (ent >> 24) : -1;
+    if constexpr (ADAPT) {
+      if (__ballot(valid) == 0) return;  // past the class's list; no barrier follows
+      q = valid ? ent : 0;
+    } else {
+      // every valid entry of a wave has the same level (runs are padded to groups of 4)
+      int ev = valid ? (ent >> 24) : -1;
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) ev = max(ev, __shfl_xor(ev, o, 64));
-    if (ev < 0) return;  // a wave of padding only; no barrier follows
-    e = __builtin_amdgcn_readfirstlane(ev);
-    q = valid ? (ent & 0xFFFFFF) : 0;
+      for (int o = 32; o > 0; o >>= 1) ev = max(ev, __shfl_xor(ev, o, 64));
+      if (ev < 0) return;  // a wave of padding only; no barrier follows
+      e = __builtin_amdgcn_readfirstlane(ev);
+      q = valid ? (ent & 0xFFFFFF) : 0;
+    }
   } else {
     while (e + 1 < LS.nent && b >= LS.blk0[e + 1]) ++e;
     const int g = (b - LS.blk0[e]) >> 2;
@@ -591,11 +613,11 @@ __device__ __forceinline__ void xlat_body(const XlatLevels& LS, const int* __res
     valid = q < (1 << (2 * (LS.lev[e] - 1)));
   }
   const int l = LS.lev[e], Wt = LS.Wt[e], NBPt = LS.NBPt[e], Ws = LS.Ws[e], NBPs = LS.NBPs[e];
-  const int half = 1 << (l - 1);
+  const int half = ADAPT ? 1 : 1 << (l - 1);  // the level tables are not read in ADAPT
   const int qq = valid ? q : 0;
   const int ix = 2 * (qq % half) + (cls & 1), iy = 2 * (qq / half) + (cls >> 1);
   bool live = valid;
-  if (MODE != XL_M2M && valid) {
+  if (!ADAPT && MODE != XL_M2M && valid) {
     int bs, be;
     box_range(leaf_start, LS.L, l, ix, iy, bs, be);
     live = be > bs;
@@ -603,9 +625,10 @@ __device__ __forceinline__ void xlat_body(const XlatLevels& LS, const int* __res
   // A wave whose four target boxes are all empty has nothing to write (their local
   // expansions are never read: L2L and L2P only visit non-empty boxes).  No barrier follows,
   // so the whole wave may leave; sparse geometries leave most waves here.
-  if (MODE != XL_M2M && __ballot(live) == 0) return;
-  const int pt = pidx(ix, iy, Wt);
+  if (!ADAPT && MODE != XL_M2M && __ballot(live) == 0) return;
+  const int pt = ADAPT ? qq : pidx(ix, iy, Wt);
   auto src_box = [&](int o) -> int {
+    if constexpr (ADAPT) return valid ? stab[(size_t)pt * NS + o] : nsrc;
     if (MODE == XL_M2L) return pt + offs[(cls * 27 + o) * 2 + 1] * Ws + offs[(cls * 27 + o) * 2 + 0];
     if (MODE == XL_M2M) return pidx(2 * ix + (o & 1), 2 * iy + (o >> 1), Ws);
     return pidx(ix >> 1, iy >> 1, Ws);
@@ -614,7 +637,8 @@ __device__ __forceinline__ void xlat_body(const XlatLevels& LS, const int* __res
   // per-k row step is a scalar offset (no per-lane address arithmetic per load), and an
   // offset outside the level reads 0 instead of faulting.
   const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
-      (void*)(src + LS.offs[e]), (short)0, (int)(uint32_t)((size_t)2 * KMAX * NBPs * 16), 0x00020000);
+      (void*)(ADAPT ? src : src + LS.offs[e]), (short)0,
+      (int)(uint32_t)((size_t)2 * KMAX * (ADAPT ? nsrc : NBPs) * 16), 0x00020000);
   const int row_bytes = 4 * 16;  // k -> k + 4 within the box's coefficients
   const int lane_row = (s * KMAX + kr) * 16;
   auto load_src = [&](int o, double2* a) {
@@ -742,7 +766,7 @@ __device__ __forceinline__ void xlat_body(const XlatLevels& LS, const int* __res
     }
   }
   if (live) {
-    double* out = (double*)(dst + LS.offt[e]);
+    double* out = (double*)(ADAPT ? dst : dst + LS.offt[e]);
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
@@ -774,6 +798,12 @@ __global__ __launch_bounds__(256) void k_xlat(EB_XLAT_ARGS) {
 template <int MODE, int KMAX, int KS, int MT, int TR>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_xlat2(EB_XLAT_ARGS) {
   xlat_body<MODE, KMAX, KS, MT, TR>(LS, leaf_start, src, dst, Mat, pre, post, offs, m2l_list, m2l_count, m2l_cap);
+}
+// the same on the boxes of an adaptive tree (ADAPT above)
+template <int MODE, int KMAX, int KS, int MT, int TR>
+__global__ __launch_bounds__(256) void k_axlat(EB_XLAT_ARGS, const int* __restrict__ stab, int nsrc) {
+  xlat_body<MODE, KMAX, KS, MT, TR, true>(LS, leaf_start, src, dst, Mat, pre, post, offs, m2l_list, m2l_count,
+                                          m2l_cap, stab, nsrc);
 }
 #undef EB_XLAT_ARGS
 
@@ -2129,5 +2159,867 @@ extern "C" int eb_cadap_lists(int32_t ilev, int32_t* intl, int32_t* nintl, int32
                      nnbor);
   EB_HIP_CHECK(hipGetLastError());
   EB_HIP_CHECK(hipStreamSynchronize((hipStream_t)stream));
+  return EB_OK;
+}
+
+// ==========================================================================================
+// The FMM on the adaptive tree of DAPIF2 (eb_afmm_*): the same kernel sums on the tree that
+// DAASSI .. DALI34 build (dapif1.f:1165-1680, dapif2.f:3-474; eb_atree), whose childless boxes
+// hold at most NAPB points, in place of the uniform grid of LEVMAX levels.
+//
+// Passes (Greengard-Rokhlin adaptive form):
+//   P2M at every childless box; M2M into every parent, level by level upward;
+//   M2L from the 27 slots of MKINTL's interaction list (cadap2.f:780-828: children of the
+//     parent's colleagues, not adjacent) where such a box exists, every box of level >= 2;
+//   P2L from list X: the points of a childless box c into the local expansion of every box b
+//     of c's list W (b is then one of c's X list's targets);
+//   L2L into every box of level >= 3, level by level downward;
+//   at every childless box b: L2P, the multipole expansions of list W (descendants of b's
+//     colleagues that are not adjacent to b but whose parents are), and the direct sum over
+//     list U (every childless box adjacent to b, b itself included).
+// W and U come from a descent through the colleagues; with the interaction lists of b's
+// ancestors they count every source exactly once (eb_afmm_check verifies it on the host).
+// The translations run on the matrix-core kernel above (xlat_body, ADAPT): the expansions of
+// box k sit at k * 2 KMAX (both series), scaled by the box's half-width, so one table serves
+// every level.  Coordinates are the tree's own: (x - XONEW) * RSCAL in [-64, 64]^2.
+// ==========================================================================================
+struct eb_afmm_s {
+  int n = 0, p = 0, kmax = 0, variant = 0, nbox = 0, nleaf = 0, nlev = 0;
+  double sgn = -1.0, sqb = -1.0, rscal = 1.0, xonew = 0.0, yonew = 0.0;
+  hipStream_t st = nullptr;
+  // host copies of the box data and lists (eb_afmm_check, eb_afmm_info)
+  std::vector<int> lev, bx, by, par, leafv, bstart, bcnt;
+  std::vector<std::vector<int>> U, W, X;
+  std::vector<int> stab27_h;
+  long long cnt[8] = {0};
+  // device
+  Geom* geom = nullptr;
+  int32_t* perm = nullptr;
+  double2 *zs = nullptr, *wdzs = nullptr, *qa = nullptr, *qb = nullptr, *mp = nullptr, *lo = nullptr;
+  double2* bctr = nullptr;  // box centres
+  double* brad = nullptr;   // box half-widths
+  int *dstart = nullptr, *dcnt = nullptr;  // point range of each childless box (sorted order)
+  int *leaves = nullptr, *stab27 = nullptr, *kid = nullptr, *dpar = nullptr;
+  int4* items = nullptr;  // (childless box, first point, end point), at most AF_T points each
+  int nitems = 0;
+  int *u_off = nullptr, *u_box = nullptr, *w_off = nullptr, *w_box = nullptr;  // per childless box
+  int *x_tgt = nullptr, *x_off = nullptr, *x_box = nullptr;                     // per X target
+  int nxt = 0;
+  // per-class target lists of the translation launches: [0] M2L, then M2M per level, L2L per level
+  struct Group {
+    int* list = nullptr;  // 4 x cap
+    int* count = nullptr; // 4
+    int cap = 0, grid = 0;
+  };
+  std::vector<Group> m2m, l2l;
+  Group m2l;
+  double* dMat = nullptr;
+  double2 *dU = nullptr, *dNU = nullptr, *dPM = nullptr, *dQM = nullptr, *dPL = nullptr, *dQL = nullptr;
+  std::vector<void*> owned;  // every device allocation above
+  eb_atree tree = nullptr;   // held until the points are gathered
+};
+
+namespace {
+
+constexpr int AF_T = 64;     // points per childless-box work item
+constexpr int AF_CH = 128;   // sources staged per chunk in the near field
+
+// P2M of every childless box: one wave per box
+template <int KMAX>
+__global__ __launch_bounds__(64) void k_af_p2m(int p, const int* __restrict__ leaves, const int* __restrict__ start,
+                                               const int* __restrict__ cnt, const double2* __restrict__ ctr,
+                                               const double* __restrict__ rad, const double2* __restrict__ zs,
+                                               const double2* __restrict__ qa, const double2* __restrict__ qb,
+                                               double2* __restrict__ mp) {
+  const int b = leaves[blockIdx.x];
+  const int s = start[b];
+  p2m_box<KMAX>(p, s, s + cnt[b], ctr[b], 1.0 / rad[b], zs, qa, qb, mp + (size_t)b * 2 * KMAX);
+}
+
+// P2L of list X: the points of every childless box c in X(t) into the local expansion of t
+// (centre ct, half-width r), added to what M2L wrote.  With s = r / (z_j - ct), point j adds
+//   phi: (qa_j / r) s^(m+1),   psi: (qb_j / r) s^(m+1) + (m + 1) (qc_j / r^2) s^(m+2)
+// to coefficient m (qc = -conj(z) qa), the expansion of qa/(z_j - z) and of
+// qb/(z_j - z) + qc/(z_j - z)^2 in powers of (z - ct) / r.  One wave per target box, lane m
+// owns coefficient m of both series.
+template <int KMAX>
+__global__ __launch_bounds__(64) void k_af_p2l(int p, const int* __restrict__ tgt, const int* __restrict__ off,
+                                               const int* __restrict__ src, const int* __restrict__ start,
+                                               const int* __restrict__ cnt, const double2* __restrict__ ctr,
+                                               const double* __restrict__ rad, const double2* __restrict__ zs,
+                                               const double2* __restrict__ qa, const double2* __restrict__ qb,
+                                               double2* __restrict__ lo) {
+  constexpr int CH = 8, QK = KMAX / 8 + 1;  // powers s^1 .. s^(KMAX+1): indices 0 .. KMAX
+  __shared__ double2 P[CH][KMAX + 8];
+  __shared__ double2 AB[CH][3];
+  const int t = tgt[blockIdx.x];
+  const double2 c = ctr[t];
+  const double r = rad[t], ir = 1.0 / r;
+  const int lane = threadIdx.x;
+  const int js = lane & (CH - 1), qk = lane / CH;
+  const int k = lane < KMAX ? lane : KMAX - 1;
+  double2 a0 = make_double2(0, 0), a1 = make_double2(0, 0), a2 = make_double2(0, 0);
+  for (int q = off[blockIdx.x]; q < off[blockIdx.x + 1]; ++q) {
+    const int sb = src[q], s0 = start[sb], e0 = s0 + cnt[sb];
+    for (int c0 = s0; c0 < e0; c0 += CH) {
+      const int nj = min(CH, e0 - c0);
+      if (js < nj) {
+        const int j = c0 + js;
+        const double2 z = zs[j];
+        const double2 sv = cscale(crcp(make_double2(z.x - c.x, z.y - c.y)), r);  // r / (z - ct)
+        if (qk == 0) {
+          const double2 q1 = qa[j], q2 = qb[j];
+          AB[js][0] = cscale(q1, ir);
+          AB[js][1] = cscale(q2, ir);
+          AB[js][2] = cscale(cmul(make_double2(-z.x, z.y), q1), ir * ir);  // qc / r^2
+        }
+        // lane qk writes s^(1 + qk + 8 m) at index qk + 8 m
+        const double2 s2 = cmul(sv, sv), s4 = cmul(s2, s2), s8 = cmul(s4, s4);
+        double2 pw = sv;
+        if (qk & 1) pw = cmul(pw, sv);
+        if (qk & 2) pw = cmul(pw, s2);
+        if (qk & 4) pw = cmul(pw, s4);
+#pragma unroll
+        for (int m = 0; m < QK; ++m) {
+          const int idx = qk + 8 * m;
+          if (idx <= KMAX) P[js][idx] = pw;
+          if (m + 1 < QK) pw = cmul(pw, s8);
+        }
+      }
+      __syncthreads();
+      for (int jj = 0; jj < nj; ++jj) {
+        const double2 pk = P[jj][k], pk1 = P[jj][k + 1];
+        cfma(a0, AB[jj][0], pk);
+        cfma(a1, AB[jj][1], pk);
+        cfma(a2, AB[jj][2], pk1);
+      }
+      __syncthreads();
+    }
+  }
+  rfma(a1, (double)(lane + 1), a2);
+  if (lane < p) {
+    double2* box = lo + (size_t)t * 2 * KMAX;
+    box[lane] = cadd(box[lane], a0);
+    box[KMAX + lane] = cadd(box[KMAX + lane], a1);
+  }
+}
+
+// At the points of one work item of a childless box b: its local expansion (L2P) and the
+// multipole expansions of list W (M2P: phi = sum_k a_k w^(k+1), w = r_c / (z - c_c), phi' =
+// -(w^2 / r_c) d/dw, psi likewise), combined as GW = phi + sgn (z conj(phi') + conj(psi)) and
+// added, times RSCAL, to the near field already in out[perm[i]].
+template <int KMAX>
+__global__ __launch_bounds__(AF_T) void k_af_far(int p, const int4* __restrict__ items,
+                                                 const int* __restrict__ w_off, const int* __restrict__ w_box,
+                                                 const double2* __restrict__ ctr, const double* __restrict__ rad,
+                                                 const double2* __restrict__ zs, const double2* __restrict__ lo,
+                                                 const double2* __restrict__ mp, double sgn,
+                                                 const int32_t* __restrict__ perm, const Geom* __restrict__ g,
+                                                 double2* __restrict__ out) {
+  __shared__ double2 C[2][KMAX];
+  const int4 it = items[blockIdx.x];
+  const int b = it.x;
+  const int i = it.y + (int)threadIdx.x;
+  const bool act = i < it.z;
+  const double2 z = act ? zs[i] : make_double2(0, 0);
+  for (int t = threadIdx.x; t < 2 * p; t += AF_T) {
+    const int sr = t / p, m = t - sr * p;
+    C[sr][m] = lo[(size_t)b * 2 * KMAX + sr * KMAX + m];
+  }
+  __syncthreads();
+  double2 phi = make_double2(0, 0), dphi = make_double2(0, 0), psi = make_double2(0, 0);
+  {
+    const double2 cb = ctr[b];
+    const double ir = 1.0 / rad[b];
+    const double2 u = make_double2((z.x - cb.x) * ir, (z.y - cb.y) * ir);
+#pragma unroll 4
+    for (int m = p - 1; m >= 0; --m) {
+      dphi = cmuladd(dphi, u, phi);
+      phi = cmuladd(phi, u, C[0][m]);
+      psi = cmuladd(psi, u, C[1][m]);
+    }
+    dphi = cscale(dphi, ir);
+  }
+  for (int q = w_off[b]; q < w_off[b + 1]; ++q) {
+    const int c = w_box[q];
+    __syncthreads();
+    for (int t = threadIdx.x; t < 2 * p; t += AF_T) {
+      const int sr = t / p, m = t - sr * p;
+      C[sr][m] = mp[(size_t)c * 2 * KMAX + sr * KMAX + m];
+    }
+    __syncthreads();
+    const double2 cc = ctr[c];
+    const double rc = rad[c];
+    const double2 w = cscale(crcp(make_double2(z.x - cc.x, z.y - cc.y)), rc);
+    // R(w) = sum_k a_k w^k, R'(w), S(w) = sum_k b_k w^k: phi = w R, dphi/dw = R + w R'
+    double2 R = make_double2(0, 0), dR = make_double2(0, 0), S = make_double2(0, 0);
+#pragma unroll 4
+    for (int m = p - 1; m >= 0; --m) {
+      dR = cmuladd(dR, w, R);
+      R = cmuladd(R, w, C[0][m]);
+      S = cmuladd(S, w, C[1][m]);
+    }
+    const double2 w2 = cmul(w, w);
+    phi = cmuladd(w, R, phi);
+    psi = cmuladd(w, S, psi);
+    // phi'(z) = (R + w R') dw/dz, dw/dz = -w^2 / r_c
+    const double2 dq = cmuladd(w, dR, R);
+    dphi = cmuladd(cscale(w2, -1.0 / rc), dq, dphi);
+  }
+  if (!act) return;
+  const double2 zc = cmul(z, cconj(dphi));
+  const double2 t = make_double2(zc.x + psi.x, zc.y - psi.y);
+  const double rs = g->rscal;
+  const int o = perm[i];
+  const double2 nearv = out[o];
+  out[o] = make_double2(nearv.x + (phi.x + sgn * t.x) * rs, nearv.y + (phi.y + sgn * t.y) * rs);
+}
+
+// The near field of one work item of a childless box b: the sources of every box of U(b)
+// (b itself first), staged through LDS per box and chunk; with T < 64 targets the lanes split
+// into G = 64 / T slices of the sources, summed through LDS at the end.  Writes RSCAL * (the
+// sum) to out[perm[i]].
+__global__ __launch_bounds__(64) void k_af_p2p(const int4* __restrict__ items, const int* __restrict__ u_off,
+                                               const int* __restrict__ u_box, const int* __restrict__ start,
+                                               const int* __restrict__ cnt, const double2* __restrict__ zs,
+                                               const double2* __restrict__ qa, const double2* __restrict__ qb,
+                                               const int32_t* __restrict__ perm, const Geom* __restrict__ g,
+                                               double sgn, double2* __restrict__ out) {
+  __shared__ double2 sz[AF_CH], sa[AF_CH], sq[AF_CH];
+  const int4 it = items[blockIdx.x];
+  const int b = it.x, T = it.z - it.y;
+  const int lane = threadIdx.x;
+  const int G = T < 64 ? 64 / T : 1;
+  const int tl = lane % T, gs = lane / T;
+  const bool act = gs < G;
+  const int i = it.y + tl;
+  const double2 zi = act ? zs[i] : make_double2(0, 0);
+  double2 a1 = make_double2(0, 0), a2 = make_double2(0, 0);
+  for (int q = u_off[b]; q < u_off[b + 1]; ++q) {
+    const int ub = u_box[q], s0 = start[ub], n0 = cnt[ub];
+    const bool self = ub == b;
+    for (int c0 = 0; c0 < n0; c0 += AF_CH) {
+      const int nc = min(AF_CH, n0 - c0);
+      __syncthreads();
+      for (int f = lane; f < nc; f += 64) {
+        const int j = s0 + c0 + f;
+        sz[f] = zs[j];
+        sa[f] = qa[j];
+        sq[f] = qb[j];
+      }
+      __syncthreads();
+      if (act) {
+        if (self) {
+          const int fi = i - s0 - c0;  // the target's own position in this chunk (may be outside)
+          for (int f = gs; f < nc; f += G) p2p_pair<true, true>(zi, sz[f], sa[f], sq[f], f == fi, a1, a2);
+        } else {
+          for (int f = gs; f < nc; f += G) p2p_pair<false, true>(zi, sz[f], sa[f], sq[f], false, a1, a2);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  sz[lane] = make_double2(a1.x + sgn * a2.x, a1.y + sgn * a2.y);
+  __syncthreads();
+  if (lane < T) {
+    double2 s = make_double2(0, 0);
+    for (int q = 0; q < G; ++q) s = cadd(s, sz[lane + q * T]);
+    const double rs = g->rscal;
+    out[perm[it.y + lane]] = make_double2(s.x * rs, s.y * rs);
+  }
+}
+
+// sorted coordinates and weights: zs[i] = (xs, ys)[perm[i]], wdzs[i] = wdz[perm[i]]
+__global__ void k_af_gather(int n, const int32_t* __restrict__ perm, const double* __restrict__ xs,
+                            const double* __restrict__ ys, const double2* __restrict__ wdz, double2* __restrict__ zs,
+                            double2* __restrict__ wdzs) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int j = perm[i];
+  zs[i] = make_double2(xs[j], ys[j]);
+  wdzs[i] = wdz[j];
+}
+
+__global__ void k_af_geom(const double* __restrict__ tgeom, Geom* __restrict__ g) {
+  if (threadIdx.x == 0) *g = Geom{tgeom[0], 0.0, 0.0, 0.0};
+}
+
+}  // namespace
+
+static void afmm_free(eb_afmm P) {
+  if (!P) return;
+  if (P->st) (void)hipStreamSynchronize(P->st);
+  if (P->tree) eb_atree_destroy(P->tree);
+  for (void* q : P->owned)
+    if (q) (void)hipFree(q);
+  delete P;
+}
+
+template <typename T>
+static int af_alloc(eb_afmm P, T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  const hipError_t e = hipMalloc((void**)p, sizeof(T) * count);
+  if (e != hipSuccess) {
+    eb_set_error("eb_afmm_create: hipMalloc(%zu bytes) failed: %s", sizeof(T) * count, hipGetErrorString(e));
+    *p = nullptr;
+    return EB_ERR_NOMEM;
+  }
+  P->owned.push_back((void*)*p);
+  return EB_OK;
+}
+
+template <typename T>
+static int af_upload(eb_afmm P, T** p, const std::vector<T>& v) {
+  int rc = af_alloc(P, p, v.size());
+  if (rc != EB_OK) return rc;
+  if (!v.empty()) EB_HIP_CHECK(hipMemcpyAsync(*p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, P->st));
+  return EB_OK;
+}
+
+// Per-class target lists of the translation kernel (ADAPT).  The kernel runs the four
+// classes' blocks of one list position side by side, so aligned lists share their sources in
+// L2: by_parent puts child c of the i-th parent at position i of class c's list (-1 where
+// the child does not exist); otherwise each class lists its boxes in order.
+static int af_group(eb_afmm P, const std::vector<int>& boxes, eb_afmm_s::Group& G, bool by_parent = false,
+                    const std::vector<std::array<int, 4>>* kids = nullptr) {
+  std::vector<int> per[4];
+  if (by_parent) {
+    for (int q : boxes)
+      for (int c = 0; c < 4; ++c) per[c].push_back((*kids)[q][c]);
+  } else {
+    for (int b : boxes) per[(P->bx[b] & 1) | ((P->by[b] & 1) << 1)].push_back(b);
+  }
+  int cap = 0, mx = 0;
+  for (auto& v : per) cap = std::max(cap, (int)v.size());
+  cap = std::max(cap, 1);
+  std::vector<int> list((size_t)4 * cap, -1), count(4);
+  for (int c = 0; c < 4; ++c) {
+    std::copy(per[c].begin(), per[c].end(), list.begin() + (size_t)c * cap);
+    count[c] = (int)per[c].size();
+    mx = std::max(mx, (count[c] + 15) / 16);
+  }
+  G.cap = cap;
+  G.grid = (4 * mx + 7) / 8 * 8;  // the kernel's own count of logical blocks (xlat_body)
+  int rc;
+  if ((rc = af_upload(P, &G.list, list)) != EB_OK || (rc = af_upload(P, &G.count, count)) != EB_OK) return rc;
+  return EB_OK;
+}
+
+// U and W of every childless box by a descent through the colleagues, X from W, and the M2L
+// slot table; counts for eb_afmm_info
+static void af_lists(eb_afmm P) {
+  const int nb = P->nbox;
+  auto key = [](int l, int ix, int iy) -> uint64_t {
+    return ((uint64_t)l << 58) | ((uint64_t)(uint32_t)ix << 29) | (uint64_t)(uint32_t)iy;
+  };
+  std::vector<std::pair<uint64_t, int>> idx(nb);
+  for (int b = 0; b < nb; ++b) idx[b] = {key(P->lev[b], P->bx[b], P->by[b]), b};
+  std::sort(idx.begin(), idx.end());
+  auto find = [&](int l, int ix, int iy) -> int {
+    if (ix < 0 || iy < 0 || ix >= (1 << l) || iy >= (1 << l)) return -1;
+    const uint64_t k = key(l, ix, iy);
+    auto it = std::lower_bound(idx.begin(), idx.end(), std::make_pair(k, -1));
+    return (it != idx.end() && it->first == k) ? it->second : -1;
+  };
+  // closed boxes touch or overlap, compared at the finer level
+  auto adjacent = [&](int a, int b) {
+    const int f = std::max(P->lev[a], P->lev[b]);
+    const long sa = 1L << (f - P->lev[a]), sb = 1L << (f - P->lev[b]);
+    const long ax0 = P->bx[a] * sa, ax1 = ax0 + sa, ay0 = P->by[a] * sa, ay1 = ay0 + sa;
+    const long bx0 = P->bx[b] * sb, bx1 = bx0 + sb, by0 = P->by[b] * sb, by1 = by0 + sb;
+    return ax0 <= bx1 && bx0 <= ax1 && ay0 <= by1 && by0 <= ay1;
+  };
+  std::vector<std::array<int, 4>> kids(nb);
+  for (auto& k : kids) k = {-1, -1, -1, -1};
+  for (int b = 0; b < nb; ++b)
+    if (P->par[b] >= 0) kids[P->par[b]][(P->bx[b] & 1) | ((P->by[b] & 1) << 1)] = b;
+  P->U.assign(nb, {});
+  P->W.assign(nb, {});
+  P->X.assign(nb, {});
+  std::vector<int> stack;
+  for (int b = 0; b < nb; ++b) {
+    if (!P->leafv[b]) continue;
+    auto& U = P->U[b];
+    auto& W = P->W[b];
+    U.push_back(b);
+    const int lb = P->lev[b];
+    // childless boxes of the same or a coarser level that touch b
+    for (int l = 1; l <= lb; ++l) {
+      const int ax = P->bx[b] >> (lb - l), ay = P->by[b] >> (lb - l);
+      for (int dx = -1; dx <= 1; ++dx)
+        for (int dy = -1; dy <= 1; ++dy) {
+          const int c = find(l, ax + dx, ay + dy);
+          if (c < 0 || c == b || !P->leafv[c]) continue;
+          if (l < lb && dx == 0 && dy == 0) continue;  // b's own ancestor
+          if (adjacent(b, c)) U.push_back(c);
+        }
+    }
+    // finer boxes: the descendants of b's colleagues
+    for (int dx = -1; dx <= 1; ++dx)
+      for (int dy = -1; dy <= 1; ++dy) {
+        if (dx == 0 && dy == 0) continue;
+        const int c = find(lb, P->bx[b] + dx, P->by[b] + dy);
+        if (c < 0 || P->leafv[c]) continue;
+        stack.assign(1, c);
+        while (!stack.empty()) {
+          const int a = stack.back();
+          stack.pop_back();
+          for (int ch : kids[a]) {
+            if (ch < 0) continue;
+            if (!adjacent(b, ch)) {
+              W.push_back(ch);
+            } else if (P->leafv[ch]) {
+              U.push_back(ch);
+            } else {
+              stack.push_back(ch);
+            }
+          }
+        }
+      }
+    for (int c : W) P->X[c].push_back(b);
+  }
+  // M2L slots
+  P->stab27_h.assign((size_t)nb * 27, nb);
+  long long m2l = 0;
+  for (int b = 0; b < nb; ++b) {
+    if (P->lev[b] < 2) continue;
+    int dxs[27], dys[27];
+    const int no = intl_offsets(P->bx[b] & 1, P->by[b] & 1, dxs, dys);
+    for (int o = 0; o < no; ++o) {
+      const int c = find(P->lev[b], P->bx[b] + dxs[o], P->by[b] + dys[o]);
+      if (c >= 0) {
+        P->stab27_h[(size_t)b * 27 + o] = c;
+        ++m2l;
+      }
+    }
+  }
+  long long near = 0, m2p = 0, p2l = 0, xl = 0;
+  for (int b = 0; b < nb; ++b) {
+    if (P->leafv[b]) {
+      for (int c : P->U[b]) near += (long long)P->bcnt[b] * P->bcnt[c];
+      m2p += (long long)P->bcnt[b] * P->W[b].size();
+    }
+    for (int c : P->X[b]) p2l += P->bcnt[c];
+    xl += P->X[b].size();
+  }
+  long long shifted = 0;
+  for (int b = 0; b < nb; ++b) shifted += P->lev[b] >= 3;
+  P->cnt[0] = nb;
+  P->cnt[1] = P->nleaf;
+  P->cnt[2] = P->nlev;
+  P->cnt[3] = near - P->n;  // ordered pairs i != j
+  P->cnt[4] = m2l;
+  P->cnt[5] = m2p;
+  P->cnt[6] = p2l;
+  P->cnt[7] = shifted;  // M2M and L2L translations each
+  (void)xl;
+}
+
+extern "C" int eb_afmm_create(eb_afmm* out, int32_t n, const double* x, const double* y, const double* wdz,
+                              int32_t napb, int32_t nterms, int32_t variant, void* stream) {
+  if (!out || !x || !y || !wdz || n < 2 || napb < 1 || nterms < 4 || nterms > 60 || variant < 0 ||
+      variant > EB_VARIANT_HILBERT) {
+    eb_set_error("eb_afmm_create: bad arguments (n >= 2, napb >= 1, 4 <= nterms <= 60)");
+    return EB_ERR_ARG;
+  }
+  *out = nullptr;
+  eb_afmm P = new eb_afmm_s();
+  P->n = n;
+  P->p = nterms;
+  P->kmax = pick_kmax(nterms);
+  P->variant = variant;
+  P->st = (hipStream_t)stream;
+  P->sgn = (variant == EB_VARIANT_DR) ? -1.0 : (variant == EB_VARIANT_HILBERT ? 0.0 : 1.0);
+  P->sqb = (variant == EB_VARIANT_PLUS) ? 1.0 : -1.0;
+  const int K = P->kmax;
+#define AF_TRY(x)           \
+  do {                      \
+    int _rc = (x);          \
+    if (_rc != EB_OK) {     \
+      afmm_free(P);         \
+      return _rc;           \
+    }                       \
+  } while (0)
+  // the tree (its build synchronises the stream per level)
+  int32_t ierr[3] = {0, 0, 0};
+  // at most 29 levels: box positions then fit the 29-bit fields of the host's box keys
+  int rc = eb_atree_build(&P->tree, n, x, y, napb, 29, 0.0, ierr, stream);
+  eb_atree T = P->tree;
+  if (rc != EB_OK) {
+    if (rc == EB_ATREE_TOO_DEEP)
+      eb_set_error("eb_afmm_create: the tree needs more than 29 levels (more than %d coincident points?)", napb);
+    else if (rc == EB_ATREE_FEW_POINTS)
+      eb_set_error("eb_afmm_create: %d points do not exceed NAPB = %d", n, napb);
+    afmm_free(P);
+    return rc < EB_ERR_UNSUPPORTED ? EB_ERR_ARG : rc;
+  }
+  int32_t c8[8];
+  eb_atree_view V;
+  eb_atree_counts(T, c8);
+  eb_atree_tables(T, &V);
+  const int nb = c8[0], nlev = c8[1], jatlen = c8[6];
+  P->nbox = nb;
+  P->nlev = nlev;
+  std::vector<double> xb(nb), yb(nb), tg(3);
+  std::vector<int32_t> nl(nlev + 2), npar(nb), indb(nb), iat(nb), nbat(nb), jat(jatlen);
+  bool ok = hipMemcpyAsync(xb.data(), V.xb, sizeof(double) * nb, hipMemcpyDeviceToHost, P->st) == hipSuccess &&
+            hipMemcpyAsync(yb.data(), V.yb, sizeof(double) * nb, hipMemcpyDeviceToHost, P->st) == hipSuccess &&
+            hipMemcpyAsync(tg.data(), V.geom, sizeof(double) * 3, hipMemcpyDeviceToHost, P->st) == hipSuccess &&
+            hipMemcpyAsync(nl.data(), V.nl, sizeof(int32_t) * (nlev + 2), hipMemcpyDeviceToHost, P->st) == hipSuccess &&
+            hipMemcpyAsync(npar.data(), V.npar, sizeof(int32_t) * nb, hipMemcpyDeviceToHost, P->st) == hipSuccess &&
+            hipMemcpyAsync(indb.data(), V.indb, sizeof(int32_t) * nb, hipMemcpyDeviceToHost, P->st) == hipSuccess &&
+            hipMemcpyAsync(iat.data(), V.iat, sizeof(int32_t) * nb, hipMemcpyDeviceToHost, P->st) == hipSuccess &&
+            hipMemcpyAsync(nbat.data(), V.nbat, sizeof(int32_t) * nb, hipMemcpyDeviceToHost, P->st) == hipSuccess &&
+            hipMemcpyAsync(jat.data(), V.jat, sizeof(int32_t) * jatlen, hipMemcpyDeviceToHost, P->st) == hipSuccess &&
+            hipStreamSynchronize(P->st) == hipSuccess;
+  if (!ok) {
+    afmm_free(P);
+    eb_set_error("eb_afmm_create: copying the tree failed");
+    return EB_ERR_HIP;
+  }
+  P->rscal = tg[0];
+  P->xonew = tg[1];
+  P->yonew = tg[2];
+  // box levels, integer positions, parents, childless flags; points in childless-box order
+  P->lev.assign(nb, 0);
+  P->bx.assign(nb, 0);
+  P->by.assign(nb, 0);
+  P->par.assign(nb, -1);
+  P->leafv.assign(nb, 0);
+  P->bstart.assign(nb, 0);
+  P->bcnt.assign(nb, 0);
+  for (int l = 1; l <= nlev; ++l)
+    for (int b = nl[l - 1] - 1; b < nl[l] - 1 && b < nb; ++b) P->lev[b] = l;
+  std::vector<int32_t> perm;
+  perm.reserve(n);
+  std::vector<int> leaves;
+  for (int b = 0; b < nb; ++b) {
+    const int l = P->lev[b];
+    const double h = 128.0 / (double)(1 << l);
+    P->bx[b] = (int)lround((xb[b] + 64.0) / h - 0.5);
+    P->by[b] = (int)lround((yb[b] + 64.0) / h - 0.5);
+    P->par[b] = npar[b] - 1;
+    if (indb[b] == 1) {
+      P->leafv[b] = 1;
+      P->bstart[b] = (int)perm.size();
+      P->bcnt[b] = nbat[b];
+      for (int q = 0; q < nbat[b]; ++q) perm.push_back(jat[iat[b] + q] - 1);  // JAT(IAT(b)+1..): 1-based
+      leaves.push_back(b);
+    }
+  }
+  P->nleaf = (int)leaves.size();
+  if ((int)perm.size() != n) {
+    afmm_free(P);
+    eb_set_error("eb_afmm_create: the childless boxes hold %zu of %d points", perm.size(), n);
+    return EB_ERR_ARG;
+  }
+  af_lists(P);
+  if ((size_t)2 * K * 16 * (size_t)(nb + 1) >= ((size_t)1 << 31)) {
+    afmm_free(P);
+    eb_set_error("eb_afmm_create: %d boxes exceed the translation kernel's 2 GiB window", nb);
+    return EB_ERR_ARG;
+  }
+  // device tables
+  std::vector<double2> ctr(nb);
+  std::vector<double> rad(nb);
+  std::vector<int> kid((size_t)nb * 4, nb), par1(nb, nb);
+  for (int b = 0; b < nb; ++b) {
+    ctr[b] = make_double2(xb[b], yb[b]);
+    rad[b] = 64.0 / (double)(1 << P->lev[b]);
+    if (P->par[b] >= 0) {
+      kid[(size_t)P->par[b] * 4 + ((P->bx[b] & 1) | ((P->by[b] & 1) << 1))] = b;
+      par1[b] = P->par[b];
+    }
+  }
+  std::vector<int4> items;
+  std::vector<int> uo(nb + 1, 0), ub, wo(nb + 1, 0), wb, xt, xo(1, 0), xbx;
+  for (int b = 0; b < nb; ++b) {
+    if (P->leafv[b]) {
+      const int s = P->bstart[b], c = P->bcnt[b];
+      for (int q = 0; q < c; q += AF_T) items.push_back(make_int4(b, s + q, s + std::min(c, q + AF_T), 0));
+    }
+    ub.insert(ub.end(), P->U[b].begin(), P->U[b].end());
+    uo[b + 1] = (int)ub.size();
+    wb.insert(wb.end(), P->W[b].begin(), P->W[b].end());
+    wo[b + 1] = (int)wb.size();
+    if (!P->X[b].empty()) {
+      xt.push_back(b);
+      xbx.insert(xbx.end(), P->X[b].begin(), P->X[b].end());
+      xo.push_back((int)xbx.size());
+    }
+  }
+  P->nitems = (int)items.size();
+  P->nxt = (int)xt.size();
+  Tables TB;
+  build_tables(P->p, K, TB);
+  AF_TRY(af_alloc(P, &P->geom, 1));
+  AF_TRY(af_upload(P, &P->perm, perm));
+  AF_TRY(af_alloc(P, &P->zs, n));
+  AF_TRY(af_alloc(P, &P->wdzs, n));
+  AF_TRY(af_alloc(P, &P->qa, n));
+  AF_TRY(af_alloc(P, &P->qb, n));
+  AF_TRY(af_alloc(P, &P->mp, (size_t)nb * 2 * K));
+  AF_TRY(af_alloc(P, &P->lo, (size_t)nb * 2 * K));
+  // level-1 boxes receive no translation: their local expansions stay zero
+  EB_HIP_CHECK(hipMemsetAsync(P->lo, 0, sizeof(double2) * (size_t)nb * 2 * K, P->st));
+  AF_TRY(af_upload(P, &P->bctr, ctr));
+  AF_TRY(af_upload(P, &P->brad, rad));
+  AF_TRY(af_upload(P, &P->dstart, P->bstart));
+  AF_TRY(af_upload(P, &P->dcnt, P->bcnt));
+  AF_TRY(af_upload(P, &P->leaves, leaves));
+  AF_TRY(af_upload(P, &P->stab27, P->stab27_h));
+  AF_TRY(af_upload(P, &P->kid, kid));
+  AF_TRY(af_upload(P, &P->dpar, par1));
+  AF_TRY(af_upload(P, &P->items, items));
+  AF_TRY(af_upload(P, &P->u_off, uo));
+  AF_TRY(af_upload(P, &P->u_box, ub));
+  AF_TRY(af_upload(P, &P->w_off, wo));
+  AF_TRY(af_upload(P, &P->w_box, wb));
+  AF_TRY(af_upload(P, &P->x_tgt, xt));
+  AF_TRY(af_upload(P, &P->x_off, xo));
+  AF_TRY(af_upload(P, &P->x_box, xbx));
+  AF_TRY(af_upload(P, &P->dMat, TB.Mat));
+  AF_TRY(af_upload(P, &P->dU, TB.U));
+  AF_TRY(af_upload(P, &P->dNU, TB.NU));
+  AF_TRY(af_upload(P, &P->dPM, TB.PM));
+  AF_TRY(af_upload(P, &P->dQM, TB.QM));
+  AF_TRY(af_upload(P, &P->dPL, TB.PL));
+  AF_TRY(af_upload(P, &P->dQL, TB.QL));
+  // translation targets: M2L at every box of level >= 2; M2M into the parents of each level
+  // (finest first); L2L into every box of each level >= 3 (coarsest first)
+  {
+    std::vector<std::array<int, 4>> kids(nb);
+    for (auto& k : kids) k = {-1, -1, -1, -1};
+    for (int b = 0; b < nb; ++b)
+      if (P->par[b] >= 0) kids[P->par[b]][(P->bx[b] & 1) | ((P->by[b] & 1) << 1)] = b;
+    // M2L into every box of level >= 2: the children of every parent, aligned by parent
+    std::vector<int> parents;
+    for (int b = 0; b < nb; ++b)
+      if (!P->leafv[b]) parents.push_back(b);
+    AF_TRY(af_group(P, parents, P->m2l, true, &kids));
+    for (int l = nlev - 1; l >= 2; --l) {
+      std::vector<int> t;
+      for (int b = nl[l - 1] - 1; b < nl[l] - 1; ++b)
+        if (!P->leafv[b]) t.push_back(b);
+      if (t.empty()) continue;
+      P->m2m.emplace_back();
+      AF_TRY(af_group(P, t, P->m2m.back()));
+    }
+    for (int l = 3; l <= nlev; ++l) {
+      std::vector<int> t;  // the parents (level l - 1) of the targets
+      for (int b = nl[l - 2] - 1; b < nl[l - 1] - 1; ++b)
+        if (!P->leafv[b]) t.push_back(b);
+      if (t.empty()) continue;
+      P->l2l.emplace_back();
+      AF_TRY(af_group(P, t, P->l2l.back(), true, &kids));
+    }
+  }
+  // the points in childless-box order, in the tree's scaled coordinates (x - XONEW) * RSCAL
+  hipLaunchKernelGGL(k_af_geom, dim3(1), dim3(64), 0, P->st, V.geom, P->geom);
+  hipLaunchKernelGGL(k_af_gather, dim3((n + 255) / 256), dim3(256), 0, P->st, n, P->perm, V.xs, V.ys,
+                     (const double2*)wdz, P->zs, P->wdzs);
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(P->st) != hipSuccess) {
+    afmm_free(P);
+    eb_set_error("eb_afmm_create: setup kernels failed");
+    return EB_ERR_HIP;
+  }
+  eb_atree_destroy(P->tree);
+  P->tree = nullptr;
+#undef AF_TRY
+  *out = P;
+  return EB_OK;
+}
+
+template <int MODE, int K, int KS>
+static void af_xlat_ks(eb_afmm P, const eb_afmm_s::Group& G, const int* stab, const double2* src, double2* dst,
+                       const double2* pre, const double2* post) {
+  constexpr int TR = (K == 48 && KS == 9) ? 4 : 0;  // as launch_xlat_ks
+  constexpr int MT = TR ? 2 : K / 16;
+  XlatLevels LS{};
+  LS.nent = 1;
+  LS.total = G.grid;
+  const double* mat = P->dMat + (size_t)MODE * K * K;
+  // no register cap: the slot-table addressing needs a few registers more than the uniform
+  // M2L's, and under the two-waves cap they spilled (348 B per lane to scratch)
+  hipLaunchKernelGGL((k_axlat<MODE, K, KS, MT, TR>), dim3(G.grid), dim3(256), 0, P->st, LS, (const int*)nullptr, src,
+                     dst, mat, pre, post, (const int*)nullptr, G.list, G.count, G.cap, stab, P->nbox);
+}
+
+template <int MODE, int K>
+static void af_xlat(eb_afmm P, const eb_afmm_s::Group& G, const int* stab, const double2* src, double2* dst,
+                    const double2* pre, const double2* post) {
+  if (G.grid == 0) return;
+  switch ((P->p + 3) / 4 - (K / 4 - 3)) {
+    case 0: af_xlat_ks<MODE, K, K / 4 - 3>(P, G, stab, src, dst, pre, post); break;
+    case 1: af_xlat_ks<MODE, K, K / 4 - 2>(P, G, stab, src, dst, pre, post); break;
+    case 2: af_xlat_ks<MODE, K, K / 4 - 1>(P, G, stab, src, dst, pre, post); break;
+    default: af_xlat_ks<MODE, K, K / 4>(P, G, stab, src, dst, pre, post); break;
+  }
+}
+
+template <int K>
+static void af_passes(eb_afmm P, double2* out, hipEvent_t* ev) {
+  hipStream_t st = P->st;
+  auto mark = [&](int i) {
+    if (ev) (void)hipEventRecord(ev[i], st);
+  };
+  mark(1);
+  hipLaunchKernelGGL(k_af_p2p, dim3(P->nitems), dim3(64), 0, st, P->items, P->u_off, P->u_box, P->dstart, P->dcnt,
+                     P->zs, P->qa, P->qb, P->perm, P->geom, P->sgn, out);
+  mark(2);
+  hipLaunchKernelGGL(k_af_p2m<K>, dim3(P->nleaf), dim3(64), 0, st, P->p, P->leaves, P->dstart, P->dcnt, P->bctr,
+                     P->brad, P->zs, P->qa, P->qb, P->mp);
+  mark(3);
+  for (const auto& g : P->m2m) af_xlat<XL_M2M, K>(P, g, P->kid, P->mp, P->mp, P->dPM, P->dQM);
+  mark(4);
+  af_xlat<XL_M2L, K>(P, P->m2l, P->stab27, P->mp, P->lo, P->dU, P->dNU);
+  mark(5);
+  if (P->nxt)
+    hipLaunchKernelGGL(k_af_p2l<K>, dim3(P->nxt), dim3(64), 0, st, P->p, P->x_tgt, P->x_off, P->x_box, P->dstart,
+                       P->dcnt, P->bctr, P->brad, P->zs, P->qa, P->qb, P->lo);
+  mark(6);
+  for (const auto& g : P->l2l) af_xlat<XL_L2L, K>(P, g, P->dpar, P->lo, P->lo, P->dPL, P->dQL);
+  mark(7);
+  hipLaunchKernelGGL(k_af_far<K>, dim3(P->nitems), dim3(AF_T), 0, st, P->p, P->items, P->w_off, P->w_box, P->bctr,
+                     P->brad, P->zs, P->lo, P->mp, P->sgn, P->perm, P->geom, out);
+  mark(8);
+}
+
+// stage timing (eb_afmm_enable_timing): events around charges, P2P, P2M, M2M, M2L, P2L, L2L, far
+static std::mutex af_timing_mu;
+struct AfTiming {
+  hipEvent_t ev[9] = {};
+  double ms[8] = {0};
+};
+static std::vector<std::pair<eb_afmm, AfTiming*>> af_timing;
+
+static AfTiming* af_timer(eb_afmm P) {
+  std::lock_guard<std::mutex> lk(af_timing_mu);
+  for (auto& e : af_timing)
+    if (e.first == P) return e.second;
+  return nullptr;
+}
+
+extern "C" int eb_afmm_apply(eb_afmm P, const double* dens, double* gradw, void* stream) {
+  if (!P || !dens || !gradw) return (eb_set_error("eb_afmm_apply: null argument"), EB_ERR_ARG);
+  hipStream_t s = (hipStream_t)stream;
+  if (s && s != P->st) {
+    // order the new stream after the work queued on the old one
+    hipEvent_t e;
+    EB_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    EB_HIP_CHECK(hipEventRecord(e, P->st));
+    EB_HIP_CHECK(hipStreamWaitEvent(s, e, 0));
+    EB_HIP_CHECK(hipEventDestroy(e));
+    P->st = s;
+  }
+  AfTiming* tm = af_timer(P);
+  if (tm) {
+    for (auto& e : tm->ev)
+      if (!e) EB_HIP_CHECK(hipEventCreate(&e));
+    EB_HIP_CHECK(hipEventRecord(tm->ev[0], P->st));
+  }
+  hipLaunchKernelGGL(k_charges, dim3((P->n + 255) / 256), dim3(256), 0, P->st, P->n, P->perm, (const double2*)dens,
+                     P->wdzs, P->sqb, P->variant == EB_VARIANT_HILBERT ? 1 : 0, P->qa, P->qb);
+  double2* out = (double2*)gradw;
+  hipEvent_t* ev = tm ? tm->ev : nullptr;
+  switch (P->kmax) {
+    case 16: af_passes<16>(P, out, ev); break;
+    case 32: af_passes<32>(P, out, ev); break;
+    case 48: af_passes<48>(P, out, ev); break;
+    default: af_passes<64>(P, out, ev); break;
+  }
+  EB_HIP_CHECK(hipGetLastError());
+  if (tm) {
+    EB_HIP_CHECK(hipEventSynchronize(tm->ev[8]));
+    for (int i = 0; i < 8; ++i) {
+      float ms = 0;
+      EB_HIP_CHECK(hipEventElapsedTime(&ms, tm->ev[i], tm->ev[i + 1]));
+      tm->ms[i] = ms;
+    }
+  }
+  return EB_OK;
+}
+
+extern "C" int eb_afmm_enable_timing(eb_afmm P, int32_t on) {
+  if (!P) return (eb_set_error("eb_afmm_enable_timing: null plan"), EB_ERR_ARG);
+  std::lock_guard<std::mutex> lk(af_timing_mu);
+  for (size_t i = 0; i < af_timing.size(); ++i)
+    if (af_timing[i].first == P) {
+      if (!on) {
+        for (auto& e : af_timing[i].second->ev)
+          if (e) (void)hipEventDestroy(e);
+        delete af_timing[i].second;
+        af_timing.erase(af_timing.begin() + i);
+      }
+      return EB_OK;
+    }
+  if (on) af_timing.push_back({P, new AfTiming()});
+  return EB_OK;
+}
+
+extern "C" int eb_afmm_stage_ms(eb_afmm P, double* ms8) {
+  AfTiming* tm = P ? af_timer(P) : nullptr;
+  if (!tm || !ms8) return (eb_set_error("eb_afmm_stage_ms: timing is not enabled"), EB_ERR_ARG);
+  for (int i = 0; i < 8; ++i) ms8[i] = tm->ms[i];
+  return EB_OK;
+}
+
+extern "C" int eb_afmm_info(eb_afmm P, double* reals3, int64_t* counts8) {
+  if (!P || !counts8) return (eb_set_error("eb_afmm_info: bad arguments"), EB_ERR_ARG);
+  for (int i = 0; i < 8; ++i) counts8[i] = P->cnt[i];
+  if (reals3) {
+    reals3[0] = P->rscal;
+    reals3[1] = P->xonew;
+    reals3[2] = P->yonew;
+  }
+  return EB_OK;
+}
+
+// For every ordered pair (b, c) of childless boxes, the number of ways the passes carry c's
+// points to b's: c in U(b); an ancestor-or-self of c in W(b); an ancestor-or-self of c in
+// the interaction list of the same-level ancestor-or-self of b; c in X of an ancestor-or-self
+// of b.  Every pair must count exactly once; *bad receives the pairs that do not.
+extern "C" int eb_afmm_check(eb_afmm P, int64_t* bad) {
+  if (!P || !bad) return (eb_set_error("eb_afmm_check: bad arguments"), EB_ERR_ARG);
+  const int nb = P->nbox;
+  std::vector<std::vector<int>> Us(nb), Ws(nb), Xs(nb);
+  for (int b = 0; b < nb; ++b) {
+    Us[b] = P->U[b];
+    Ws[b] = P->W[b];
+    Xs[b] = P->X[b];
+    std::sort(Us[b].begin(), Us[b].end());
+    std::sort(Ws[b].begin(), Ws[b].end());
+    std::sort(Xs[b].begin(), Xs[b].end());
+  }
+  auto has = [](const std::vector<int>& v, int x) { return std::binary_search(v.begin(), v.end(), x); };
+  std::vector<int> leaves;
+  for (int b = 0; b < nb; ++b)
+    if (P->leafv[b]) leaves.push_back(b);
+  long long nbad = 0;
+  std::vector<int> ab, ac;
+  for (int b : leaves) {
+    ab.clear();
+    for (int a = b; a >= 0; a = P->par[a]) ab.push_back(a);
+    for (int c : leaves) {
+      ac.clear();
+      for (int a = c; a >= 0; a = P->par[a]) ac.push_back(a);
+      int k = has(Us[b], c) ? 1 : 0;
+      for (int a : ac) k += has(Ws[b], a);
+      for (int a : ab) k += has(Xs[a], c);
+      for (int x : ab)
+        for (int y : ac)
+          if (P->lev[x] == P->lev[y] && P->lev[x] >= 2)
+            for (int o = 0; o < 27; ++o) k += P->stab27_h[(size_t)x * 27 + o] == y;
+      nbad += k != 1;
+    }
+  }
+  *bad = nbad;
+  return EB_OK;
+}
+
+extern "C" int eb_afmm_destroy(eb_afmm P) {
+  if (P) eb_afmm_enable_timing(P, 0);
+  afmm_free(P);
   return EB_OK;
 }
