@@ -754,6 +754,7 @@ def main():
         "fp64_fraction_total": head["fp64_fraction_total"],
         "accuracy": head["accuracy"],
         "adaptive_tree": head["adaptive_tree"],
+        "adaptive_fmm": head["adaptive_fmm"],
         "autotune": head["autotune"],
         "fixed_levmax8_p32": head.get("fixed_levmax8_p32"),
         "second_workload": {k: v for k, v in other.items() if not k.startswith("_")} if other else None,
@@ -771,6 +772,55 @@ def main():
     sys.stdout.flush()
     with os.fdopen(json_fd, "w") as jf:
         jf.write(json.dumps(line) + "\n")
+
+
+def adaptive_fmm(torch, eb, x, y, w, d, p, tgt, ref, args, stream):
+    """The same sums on DAPIF2's adaptive tree (AFMMPlan: at most 40 points per childless box,
+    the same p): plan creation (tree build, lists on the host, uploads) once, K applies timed,
+    per-stage HIP events, accuracy at the same 1024 targets against the same O(N^2) sum."""
+    n = int(x.numel())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    P = eb.AFMMPlan(x, y, w, p, napb=40, stream=stream)
+    torch.cuda.synchronize()
+    t_create = time.perf_counter() - t0
+    out = torch.empty(n, dtype=torch.complex128, device=x.device)
+    for _ in range(args.warmup):
+        P.apply(d, out)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        P.apply(d, out)
+    torch.cuda.synchronize()
+    t_apply = (time.perf_counter() - t0) / args.steps
+    P.enable_timing(True)
+    acc = None
+    for _ in range(5):
+        P.apply(d, out)
+        st = P.stage_ms()
+        acc = st if acc is None else {k: acc[k] + st[k] for k in st}
+    P.enable_timing(False)
+    stage = {k: v / 5 for k, v in acc.items()}
+    P.apply(d, out)
+    err = float((out[tgt.long()] - ref).norm() / ref.norm())
+    info = P.info()
+    P.close()
+    # algorithmic flops: P2L as P2M per (point, box) term, M2P as 36 p per (point, box)
+    fl = {"charges": 0.0, "p2p": 50.0 * info["near_pairs"], "p2m": 26.0 * p * n,
+          "m2m": (8.0 * p * p + 30.0 * p) * info["shifted_boxes"],
+          "m2l": (8.0 * p * p + 34.0 * p) * info["m2l_pairs"], "p2l": 26.0 * p * info["p2l_terms"],
+          "l2l": (8.0 * p * p + 30.0 * p) * info["shifted_boxes"],
+          "far": 26.0 * p * n + 36.0 * p * info["m2p_evals"]}
+    return {"what": "the same sums on DAPIF2's adaptive tree (eb_afmm: NAPB 40, lists U/V/W/X), apply only",
+            "value": round(n / t_apply / 1e6, 3), "unit": "Mpts/s", "ms": round(t_apply * 1e3, 4),
+            "create_ms": round(t_create * 1e3, 1), "nterms": p, "napb": 40,
+            "stage_ms": {k: round(v, 4) for k, v in stage.items()},
+            "fp64_fraction_by_stage": {k: round(fl[k] / (stage[k] * 1e-3) / 1e12 / PEAK_FP64_TFLOPS, 4)
+                                       for k in fl if fl[k] > 0 and stage[k] > 0},
+            "fp64_fraction_total": round(sum(fl.values()) / t_apply / 1e12 / PEAK_FP64_TFLOPS, 4),
+            "counts": {k: info[k] for k in ("boxes", "leaves", "levels", "near_pairs", "m2l_pairs", "m2p_evals",
+                                            "p2l_terms", "shifted_boxes")},
+            "accuracy": {"rel_err2_vs_direct": err, "targets": 1024}}
 
 
 def fmm_workload(torch, eb, dev, kind, n, args, stream):
@@ -855,6 +905,7 @@ def fmm_workload(torch, eb, dev, kind, n, args, stream):
         at.close()
     torch.cuda.synchronize()
     at_info["build_ms"] = round((time.perf_counter() - t0) / 3 * 1e3, 3)
+    adaptive = adaptive_fmm(torch, eb, x, y, w, d, p, tgt, ref, args, stream)
     res = {
         "kind": kind,
         "value": round(n / t_full / 1e6, 3), "ms_per_step": round(t_full * 1e3, 4),
@@ -871,6 +922,7 @@ def fmm_workload(torch, eb, dev, kind, n, args, stream):
         "accuracy": {"rel_err2_vs_direct": err, "targets": 1024, "sources": n,
                      "checker": "O(N^2) sum over all sources on the GPU (eb_direct_subset: no tree, lists or expansions)"},
         "adaptive_tree": at_info,
+        "adaptive_fmm": adaptive,
         "autotune": tuned,
         "_flops": fl,
     }

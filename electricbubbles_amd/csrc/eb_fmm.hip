@@ -549,6 +549,7 @@ __device__ __forceinline__ void xlat_body(const XlatLevels& LS, const int* __res
   __shared__ __align__(16) double Cs[NK * CS];
   __shared__ double2 Ps[NS * NK * 2];  // per (source, k, component): the weights of this lane's column
   __shared__ double2 Qs[NS * NM];
+  __shared__ int Sx[ADAPT ? 16 * NS : 1];  // ADAPT: the block's 16 targets' source boxes
   // XCD-aware order: consecutive logical blocks (neighbouring rows, all four parity classes)
   // are dispatched to the same XCD and share its L2.
   const bool compact = ADAPT || (MODE == XL_M2L && m2l_list != nullptr);
@@ -565,6 +566,14 @@ __device__ __forceinline__ void xlat_body(const XlatLevels& LS, const int* __res
   const int cls = b & 3;
   const int ccount = compact ? m2l_count[cls] : 0;
   if (compact && 16 * (b >> 2) >= ccount) return;
+  if constexpr (ADAPT) {
+    // the slot table's rows of this block's targets, read once instead of per source step
+    for (int i = threadIdx.x; i < 16 * NS; i += blockDim.x) {
+      const int idx = 16 * (b >> 2) + i / NS;
+      const int ent = idx < ccount ? m2l_list[(size_t)cls * m2l_cap + idx] : -1;
+      Sx[i] = ent >= 0 ? stab[(size_t)ent * NS + i % NS] : nsrc;
+    }
+  }
   for (int i = threadIdx.x; i < NK * NM; i += blockDim.x) {
     const int k = i / NM, m = i % NM;
     Cs[k * CS + m] = Mat[k * KMAX + m];
@@ -627,8 +636,9 @@ __device__ __forceinline__ void xlat_body(const XlatLevels& LS, const int* __res
   // so the whole wave may leave; sparse geometries leave most waves here.
   if (!ADAPT && MODE != XL_M2M && __ballot(live) == 0) return;
   const int pt = ADAPT ? qq : pidx(ix, iy, Wt);
+  const int* sxp = &Sx[ADAPT ? (4 * wv + t) * NS : 0];
   auto src_box = [&](int o) -> int {
-    if constexpr (ADAPT) return valid ? stab[(size_t)pt * NS + o] : nsrc;
+    if constexpr (ADAPT) return sxp[o];
     if (MODE == XL_M2L) return pt + offs[(cls * 27 + o) * 2 + 1] * Ws + offs[(cls * 27 + o) * 2 + 0];
     if (MODE == XL_M2M) return pidx(2 * ix + (o & 1), 2 * iy + (o >> 1), Ws);
     return pidx(ix >> 1, iy >> 1, Ws);
@@ -2223,6 +2233,7 @@ namespace {
 
 constexpr int AF_T = 64;     // points per childless-box work item
 constexpr int AF_CH = 128;   // sources staged per chunk in the near field
+constexpr int AF_WG = 4;     // list-W multipoles staged per round in k_af_far
 
 // P2M of every childless box: one wave per box
 template <int KMAX>
@@ -2317,6 +2328,9 @@ __global__ __launch_bounds__(AF_T) void k_af_far(int p, const int4* __restrict__
                                                  const int32_t* __restrict__ perm, const Geom* __restrict__ g,
                                                  double2* __restrict__ out) {
   __shared__ double2 C[2][KMAX];
+  __shared__ double2 CW[AF_WG][2][KMAX];
+  __shared__ double2 WC[AF_WG];
+  __shared__ double WR[AF_WG];
   const int4 it = items[blockIdx.x];
   const int b = it.x;
   const int i = it.y + (int)threadIdx.x;
@@ -2340,31 +2354,40 @@ __global__ __launch_bounds__(AF_T) void k_af_far(int p, const int4* __restrict__
     }
     dphi = cscale(dphi, ir);
   }
-  for (int q = w_off[b]; q < w_off[b + 1]; ++q) {
-    const int c = w_box[q];
+  // list W, AF_WG boxes' multipoles staged per round
+  const int w0 = w_off[b], w1 = w_off[b + 1];
+  for (int q0 = w0; q0 < w1; q0 += AF_WG) {
+    const int nq = min(AF_WG, w1 - q0);
     __syncthreads();
-    for (int t = threadIdx.x; t < 2 * p; t += AF_T) {
-      const int sr = t / p, m = t - sr * p;
-      C[sr][m] = mp[(size_t)c * 2 * KMAX + sr * KMAX + m];
+    for (int t = threadIdx.x; t < nq * 2 * p; t += AF_T) {
+      const int gq = t / (2 * p), r = t - gq * 2 * p, sr = r / p, m = r - sr * p;
+      CW[gq][sr][m] = mp[(size_t)w_box[q0 + gq] * 2 * KMAX + sr * KMAX + m];
+    }
+    if ((int)threadIdx.x < nq) {
+      const int c = w_box[q0 + threadIdx.x];
+      WC[threadIdx.x] = make_double2(ctr[c].x, ctr[c].y);
+      WR[threadIdx.x] = rad[c];
     }
     __syncthreads();
-    const double2 cc = ctr[c];
-    const double rc = rad[c];
-    const double2 w = cscale(crcp(make_double2(z.x - cc.x, z.y - cc.y)), rc);
-    // R(w) = sum_k a_k w^k, R'(w), S(w) = sum_k b_k w^k: phi = w R, dphi/dw = R + w R'
-    double2 R = make_double2(0, 0), dR = make_double2(0, 0), S = make_double2(0, 0);
+    for (int gq = 0; gq < nq; ++gq) {
+      const double2 cc = WC[gq];
+      const double rc = WR[gq];
+      const double2 w = cscale(crcp(make_double2(z.x - cc.x, z.y - cc.y)), rc);
+      // R(w) = sum_k a_k w^k, R'(w), S(w) = sum_k b_k w^k: phi = w R, dphi/dw = R + w R'
+      double2 R = make_double2(0, 0), dR = make_double2(0, 0), S = make_double2(0, 0);
 #pragma unroll 4
-    for (int m = p - 1; m >= 0; --m) {
-      dR = cmuladd(dR, w, R);
-      R = cmuladd(R, w, C[0][m]);
-      S = cmuladd(S, w, C[1][m]);
+      for (int m = p - 1; m >= 0; --m) {
+        dR = cmuladd(dR, w, R);
+        R = cmuladd(R, w, CW[gq][0][m]);
+        S = cmuladd(S, w, CW[gq][1][m]);
+      }
+      const double2 w2 = cmul(w, w);
+      phi = cmuladd(w, R, phi);
+      psi = cmuladd(w, S, psi);
+      // phi'(z) = (R + w R') dw/dz, dw/dz = -w^2 / r_c
+      const double2 dq = cmuladd(w, dR, R);
+      dphi = cmuladd(cscale(w2, -1.0 / rc), dq, dphi);
     }
-    const double2 w2 = cmul(w, w);
-    phi = cmuladd(w, R, phi);
-    psi = cmuladd(w, S, psi);
-    // phi'(z) = (R + w R') dw/dz, dw/dz = -w^2 / r_c
-    const double2 dq = cmuladd(w, dR, R);
-    dphi = cmuladd(cscale(w2, -1.0 / rc), dq, dphi);
   }
   if (!act) return;
   const double2 zc = cmul(z, cconj(dphi));
@@ -2577,7 +2600,29 @@ static void af_lists(eb_afmm P) {
           }
         }
       }
-    for (int c : W) P->X[c].push_back(b);
+  }
+  // Where a childless box of W(b) holds few points, its points act on b's directly (U) instead
+  // of through its multipole; where b itself holds few points, the reverse interaction (b in
+  // X(c): b's points into c's local expansion) becomes direct too when c is childless.  A
+  // direct pair costs 50 flops, an M2P evaluation about 30 p, a P2L term about 26 p per
+  // source point plus its share of c's L2P.
+  constexpr int W_DIRECT = 24, X_DIRECT = 32;
+  for (int b = 0; b < nb; ++b) {
+    if (!P->leafv[b]) continue;
+    std::vector<int> keep;
+    for (int c : P->W[b]) {
+      if (P->leafv[c] && P->bcnt[c] <= W_DIRECT) {
+        P->U[b].push_back(c);
+      } else {
+        keep.push_back(c);
+      }
+      // the reverse: b's points into c (X(c) gets b) unless c is a small childless box
+      if (P->leafv[c] && P->bcnt[c] <= X_DIRECT)
+        P->U[c].push_back(b);
+      else
+        P->X[c].push_back(b);
+    }
+    P->W[b].swap(keep);
   }
   // M2L slots
   P->stab27_h.assign((size_t)nb * 27, nb);
@@ -2840,8 +2885,9 @@ static void af_xlat_ks(eb_afmm P, const eb_afmm_s::Group& G, const int* stab, co
   LS.nent = 1;
   LS.total = G.grid;
   const double* mat = P->dMat + (size_t)MODE * K * K;
-  // no register cap: the slot-table addressing needs a few registers more than the uniform
-  // M2L's, and under the two-waves cap they spilled (348 B per lane to scratch)
+  // no register cap: the uniform M2L under its two-waves cap holds 246 of 256 VGPRs, and the
+  // slot-table addressing pushed this one over it (404 B per lane of scratch, 46 times the
+  // uniform M2L's HBM reads)
   hipLaunchKernelGGL((k_axlat<MODE, K, KS, MT, TR>), dim3(G.grid), dim3(256), 0, P->st, LS, (const int*)nullptr, src,
                      dst, mat, pre, post, (const int*)nullptr, G.list, G.count, G.cap, stab, P->nbox);
 }
