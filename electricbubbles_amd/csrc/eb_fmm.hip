@@ -148,26 +148,27 @@ __global__ void k_keys(const double* __restrict__ x, const double* __restrict__ 
   vals[i] = i;
 }
 
-// leaf_start[m] = first sorted position whose key is >= m, m = 0..4^L
-__global__ void k_leaf_start(const uint32_t* __restrict__ keys, int n, int nleaf, int* __restrict__ leaf_start) {
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m > nleaf) return;
-  int lo = 0, hi = n;
-  while (lo < hi) {
-    int mid = (lo + hi) >> 1;
-    if (keys[mid] < (uint32_t)m) lo = mid + 1;
-    else hi = mid;
+// Leaf ranges and the gather in one launch: thread t finds leaf t's first sorted point
+// (leaf_start[t] = the first position whose key is >= t, t = 0..4^L; t <= nleaf) and gathers
+// point t into sorted order (t < n)
+__global__ void k_leaf_gather(const uint32_t* __restrict__ keys, int n, int nleaf, int* __restrict__ leaf_start,
+                              const int32_t* __restrict__ perm, const double2* __restrict__ zs0,
+                              const double2* __restrict__ wdz, double2* __restrict__ zs, double2* __restrict__ wdzs) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t <= nleaf) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (keys[mid] < (uint32_t)t) lo = mid + 1;
+      else hi = mid;
+    }
+    leaf_start[t] = lo;
   }
-  leaf_start[m] = lo;
-}
-
-__global__ void k_gather(int n, const int32_t* __restrict__ perm, const double2* __restrict__ zs0,
-                         const double2* __restrict__ wdz, double2* __restrict__ zs, double2* __restrict__ wdzs) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int o = perm[i];
-  zs[i] = zs0[o];
-  wdzs[i] = wdz[o];
+  if (t < n) {
+    const int o = perm[t];
+    zs[t] = zs0[o];
+    wdzs[t] = wdz[o];
+  }
 }
 
 __global__ void k_zero_int(int* __restrict__ a, int n) {
@@ -835,23 +836,6 @@ constexpr int L2P_T = 64;
 
 static size_t l2p_item_cap(int n, int L) { return ((size_t)1 << (2 * (L - 1))) + (size_t)n / L2P_T + 1; }
 
-__global__ void k_l2p_count(int npar, const int* __restrict__ leaf_start, int* __restrict__ cnt) {
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= npar) return;
-  const int P = leaf_start[(m + 1) << 2] - leaf_start[m << 2];
-  cnt[m] = (P + L2P_T - 1) / L2P_T;
-}
-
-__global__ void k_l2p_items(int npar, const int* __restrict__ leaf_start, const int* __restrict__ off,
-                            int4* __restrict__ items) {
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= npar) return;
-  const int s = leaf_start[m << 2], P = leaf_start[(m + 1) << 2] - s;
-  const int o = off[m], np = off[m + 1] - o;
-  for (int q = 0; q < np; ++q)
-    items[o + q] = make_int4(m, s + (int)((long)P * q / np), s + (int)((long)P * (q + 1) / np), 0);
-}
-
 template <int KMAX, bool PARENT>
 __global__ __launch_bounds__(L2P_T) void k_l2p(int L, int p, const int4* __restrict__ items,
                                                const int* __restrict__ nitems, const uint32_t* __restrict__ keys,
@@ -918,20 +902,38 @@ __global__ __launch_bounds__(L2P_T) void k_l2p(int L, int p, const int4* __restr
 // the plan's tile (eb_fmm_set_p2p_tile), P2P_TMAX_DEF by default.
 constexpr int P2P_TMAX_DEF = 42, P2P_TMAX_MIN = 16, P2P_TMAX_MAX = 256;
 
-__global__ void k_p2p_count(int nleaf, const int* __restrict__ leaf_start, int tmax, int* __restrict__ cnt) {
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= nleaf) return;
-  const int T = leaf_start[m + 1] - leaf_start[m];
-  cnt[m] = (T + tmax - 1) / tmax;
+// The near-field and the far-field items in one count and one write launch over one scan:
+// entries [0, nleaf) count the near-field items of each leaf, [nleaf, nleaf + npar) the
+// far-field (L2P) items of each level-(L-1) box; off is their exclusive scan (nleaf + npar + 1).
+__global__ void k_items_count(int nleaf, int npar, const int* __restrict__ leaf_start, int tmax,
+                              int* __restrict__ cnt) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < nleaf) {
+    const int T = leaf_start[t + 1] - leaf_start[t];
+    cnt[t] = (T + tmax - 1) / tmax;
+  } else if (t < nleaf + npar) {
+    const int m = t - nleaf;
+    const int P = leaf_start[(m + 1) << 2] - leaf_start[m << 2];
+    cnt[t] = (P + L2P_T - 1) / L2P_T;
+  }
 }
 
-__global__ void k_p2p_items(int nleaf, const int* __restrict__ leaf_start, const int* __restrict__ off,
-                            int4* __restrict__ items) {
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= nleaf) return;
-  const int T = leaf_start[m + 1] - leaf_start[m];
-  const int o = off[m], np = off[m + 1] - o;
-  for (int q = 0; q < np; ++q) items[o + q] = make_int4(m, (int)((long)T * q / np), (int)((long)T * (q + 1) / np), 0);
+__global__ void k_items_write(int nleaf, int npar, const int* __restrict__ leaf_start, const int* __restrict__ off,
+                              int4* __restrict__ p2p_items, int4* __restrict__ l2p_items, int* __restrict__ l2p_total) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < nleaf) {
+    const int T = leaf_start[t + 1] - leaf_start[t];
+    const int o = off[t], np = off[t + 1] - o;
+    for (int q = 0; q < np; ++q)
+      p2p_items[o + q] = make_int4(t, (int)((long)T * q / np), (int)((long)T * (q + 1) / np), 0);
+  } else if (t < nleaf + npar) {
+    const int m = t - nleaf;
+    const int s = leaf_start[m << 2], P = leaf_start[(m + 1) << 2] - s;
+    const int o = off[t] - off[nleaf], np = off[t + 1] - off[t];
+    for (int q = 0; q < np; ++q)
+      l2p_items[o + q] = make_int4(m, s + (int)((long)P * q / np), s + (int)((long)P * (q + 1) / np), 0);
+    if (m == 0) *l2p_total = off[nleaf + npar] - off[nleaf];
+  }
 }
 
 // Near field at the finest level (DONN0DR, cadapdrop2.f:623-722) plus the final scaling
@@ -1446,10 +1448,12 @@ struct eb_fmm_plan_s {
   int* p2p_cnt = nullptr;
   int* p2p_off = nullptr;
   int* p2p_tmp = nullptr;
-  int4* l2p_items = nullptr;  // far-field evaluation items and their count (after the scan total)
-  int* l2p_cnt = nullptr;
-  int* l2p_off = nullptr;
-  int* l2p_tmp = nullptr;
+  int4* l2p_items = nullptr;  // far-field evaluation items (FMM mode, L >= 2); their count
+  int* l2p_total = nullptr;   // (p2p_cnt / p2p_off hold both item kinds' counts and offsets)
+  // direct mode: the rebuild leaves the points in the caller's order (the O(N^2) sum needs no
+  // tree); the sorted tree is made when eb_fmm_plan_ntbox or eb_fmm_plan_info asks for it
+  bool tree_sorted = true;
+  double2* wdz0 = nullptr;  // direct mode: the weights in the caller's order
   int* ntbox = nullptr;
   double2* dir_part = nullptr;  // direct mode: 2 x n partial sums per source chunk
   int dir_sch = 0, dir_nch = 0;
@@ -1481,7 +1485,7 @@ static void plan_free(eb_fmm_plan P) {
   if (!P) return;
   void* ptrs[] = {P->geom, P->bbpart, P->zs0, P->zs, P->wdzs, P->qa, P->qb, P->gwf, P->keys, P->keys2,
                   P->perm, P->vals2, P->sorttmp, P->leaf_start, P->p2p_items, P->p2p_cnt, P->p2p_off,
-                  P->p2p_tmp, P->l2p_items, P->l2p_cnt, P->l2p_off, P->l2p_tmp, P->ntbox, P->m2l_list, P->m2l_count, P->m2l_ccnt, P->mp, P->lo, P->dMat,
+                  P->p2p_tmp, P->l2p_items, P->l2p_total, P->wdz0, P->ntbox, P->m2l_list, P->m2l_count, P->m2l_ccnt, P->mp, P->lo, P->dMat,
                   P->dU, P->dNU, P->dPM, P->dQM, P->dPL, P->dQL, P->doffs, P->dir_part};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
@@ -1543,34 +1547,64 @@ static int use_stream(eb_fmm_plan P, void* stream) {
 // points (sum over leaves of ceil(T/tmax) <= leaves + sum (T-1)/tmax <= leaves + n/tmax)
 static size_t p2p_item_cap(int n, int L, int tmax) { return ((size_t)1 << (2 * L)) + (size_t)n / tmax + 1; }
 
-// near-field work items of the current leaves and tile; their count stays on the device
-// (p2p_off[nleaf])
-static int build_p2p_items(eb_fmm_plan P) {
+// The near-field work items of the current leaves and tile and, in FMM mode, the far-field
+// (L2P) items: one count launch, one scan over both, one write launch.  Their counts stay on
+// the device (p2p_off[nleaf] and *l2p_total).
+static int build_items(eb_fmm_plan P) {
   hipStream_t st = P->st;
   const int nleaf = 1 << (2 * P->L);
-  hipLaunchKernelGGL(k_p2p_count, dim3((nleaf + 255) / 256), dim3(256), 0, st, nleaf, P->leaf_start, P->p2p_tmax,
-                     P->p2p_cnt);
-  const int rc = scan_exclusive(P->p2p_cnt, P->p2p_off, nleaf, true, P->p2p_tmp, st);
+  const int npar = P->l2p_items ? 1 << (2 * (P->L - 1)) : 0;
+  const int m = nleaf + npar;
+  hipLaunchKernelGGL(k_items_count, dim3((m + 255) / 256), dim3(256), 0, st, nleaf, npar, P->leaf_start,
+                     P->p2p_tmax, P->p2p_cnt);
+  const int rc = scan_exclusive(P->p2p_cnt, P->p2p_off, m, true, P->p2p_tmp, st);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_p2p_items, dim3((nleaf + 255) / 256), dim3(256), 0, st, nleaf, P->leaf_start, P->p2p_off,
-                     P->p2p_items);
+  hipLaunchKernelGGL(k_items_write, dim3((m + 255) / 256), dim3(256), 0, st, nleaf, npar, P->leaf_start, P->p2p_off,
+                     P->p2p_items, P->l2p_items, P->l2p_total);
   EB_HIP_CHECK(hipGetLastError());
   return EB_OK;
 }
 
-// far-field evaluation items of the current leaves; their count stays on the device
-// (l2p_off[npar])
-static int build_l2p_items(eb_fmm_plan P) {
-  if (!P->l2p_items) return EB_OK;
+// The sort of the points by leaf and everything that follows from it (leaf ranges, gathered
+// points, work items, the M2L target lists), from the keys and identity permutation of k_keys
+// and the points and weights in the caller's order (zs0, wsrc).
+static int sort_tree(eb_fmm_plan P, const double2* wsrc) {
   hipStream_t st = P->st;
-  const int npar = 1 << (2 * (P->L - 1));
-  hipLaunchKernelGGL(k_l2p_count, dim3((npar + 255) / 256), dim3(256), 0, st, npar, P->leaf_start, P->l2p_cnt);
-  const int rc = scan_exclusive(P->l2p_cnt, P->l2p_off, npar, true, P->l2p_tmp, st);
+  const int n = P->n, L = P->L;
+  const int nleaf = 1 << (2 * L);
+  int rc = radix_sort_pairs(P->keys, P->perm, P->keys2, P->vals2, n, 2 * L, P->sorttmp, st);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_l2p_items, dim3((npar + 255) / 256), dim3(256), 0, st, npar, P->leaf_start, P->l2p_off,
-                     P->l2p_items);
+  const int t = std::max(n, nleaf + 1);
+  hipLaunchKernelGGL(k_leaf_gather, dim3((t + 255) / 256), dim3(256), 0, st, P->keys, n, nleaf, P->leaf_start,
+                     P->perm, P->zs0, wsrc, P->zs, P->wdzs);
+  if ((rc = build_items(P)) != EB_OK) return rc;
+  if (P->m2l_list) {
+    hipLaunchKernelGGL(k_m2l_count, dim3(4 * P->m2l_nb), dim3(1024), 0, st, L, P->m2l_nb, P->leaf_start,
+                       P->m2l_ccnt);
+    hipLaunchKernelGGL(k_m2l_write, dim3(4 * P->m2l_nb), dim3(1024), 0, st, L, P->m2l_nb, P->leaf_start,
+                       P->m2l_ccnt, P->m2l_cap, P->m2l_list, P->m2l_count);
+  }
   EB_HIP_CHECK(hipGetLastError());
+  P->tree_sorted = true;
   return EB_OK;
+}
+
+// points and weights in the caller's order, identity permutation (direct mode)
+__global__ void k_direct_prep(int n, const double2* __restrict__ zs0, const double2* __restrict__ wdz,
+                              double2* __restrict__ zs, double2* __restrict__ wdz0, double2* __restrict__ wdzs) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  zs[i] = zs0[i];
+  const double2 w = wdz[i];
+  wdz0[i] = w;
+  wdzs[i] = w;
+}
+
+// direct mode: the sorted tree on demand (the tables and counts of eb_fmm_plan_ntbox / _info);
+// the permutation then applies to the points as before
+static int ensure_tree(eb_fmm_plan P) {
+  if (P->tree_sorted) return EB_OK;
+  return sort_tree(P, P->wdz0);
 }
 
 static int build_tree(eb_fmm_plan P, const double* x, const double* y, const double* wdz) {
@@ -1583,22 +1617,17 @@ static int build_tree(eb_fmm_plan P, const double* x, const double* y, const dou
   hipLaunchKernelGGL(k_keys, dim3((n + 255) / 256), dim3(256), 0, st, x, y, n, L, P->geom, P->zs0, P->keys,
                      P->perm);
   EB_HIP_CHECK(hipGetLastError());
-  int rc = radix_sort_pairs(P->keys, P->perm, P->keys2, P->vals2, n, 2 * L, P->sorttmp, st);
-  if (rc) return rc;
-  hipLaunchKernelGGL(k_leaf_start, dim3((nleaf + 1 + 255) / 256), dim3(256), 0, st, P->keys, n, nleaf,
-                     P->leaf_start);
-  hipLaunchKernelGGL(k_gather, dim3((n + 255) / 256), dim3(256), 0, st, n, P->perm, P->zs0,
-                     (const double2*)wdz, P->zs, P->wdzs);
-  if ((rc = build_p2p_items(P)) != EB_OK) return rc;
-  if ((rc = build_l2p_items(P)) != EB_OK) return rc;
-  if (P->m2l_list) {
-    hipLaunchKernelGGL(k_m2l_count, dim3(4 * P->m2l_nb), dim3(1024), 0, st, L, P->m2l_nb, P->leaf_start,
-                       P->m2l_ccnt);
-    hipLaunchKernelGGL(k_m2l_write, dim3(4 * P->m2l_nb), dim3(1024), 0, st, L, P->m2l_nb, P->leaf_start,
-                       P->m2l_ccnt, P->m2l_cap, P->m2l_list, P->m2l_count);
+  if (P->mode == EB_MODE_DIRECT) {
+    // the O(N^2) sum runs on the points in the caller's order: four launches per rebuild
+    // instead of the sort, the leaf ranges and the lists
+    hipLaunchKernelGGL(k_direct_prep, dim3((n + 255) / 256), dim3(256), 0, st, n, P->zs0, (const double2*)wdz, P->zs,
+                       P->wdz0, P->wdzs);
+    EB_HIP_CHECK(hipGetLastError());
+    P->tree_sorted = false;
+    return EB_OK;
   }
-  EB_HIP_CHECK(hipGetLastError());
-  return EB_OK;
+  (void)nleaf;
+  return sort_tree(P, (const double2*)wdz);
 }
 
 extern "C" int eb_fmm_plan_create(eb_fmm_plan* out, int32_t n, const double* x, const double* y,
@@ -1655,20 +1684,23 @@ extern "C" int eb_fmm_plan_create(eb_fmm_plan* out, int32_t n, const double* x, 
   EB_TRY(dmalloc(&P->sorttmp, radix_sort_temp_ints(n)));
   EB_TRY(dmalloc(&P->leaf_start, nleaf + 1));
   EB_TRY(dmalloc(&P->p2p_items, p2p_item_cap(n, L, P2P_TMAX_MIN)));
-  EB_TRY(dmalloc(&P->p2p_cnt, nleaf));
-  EB_TRY(dmalloc(&P->p2p_off, nleaf + 1));
-  EB_TRY(dmalloc(&P->p2p_tmp, scan_temp_ints(nleaf)));
+  {
+    // near-field counts and, in FMM mode, the far-field items' counts after them
+    const int npar = (mode == EB_MODE_FMM && L >= 2) ? 1 << (2 * (L - 1)) : 0;
+    EB_TRY(dmalloc(&P->p2p_cnt, nleaf + npar));
+    EB_TRY(dmalloc(&P->p2p_off, nleaf + npar + 1));
+    EB_TRY(dmalloc(&P->p2p_tmp, scan_temp_ints(nleaf + npar)));
+  }
   EB_TRY(dmalloc(&P->ntbox, EB_MAXLEV + 2));
   if (mode == EB_MODE_DIRECT) {
     direct_chunks(n, P->dir_sch, P->dir_nch);
     EB_TRY(dmalloc(&P->dir_part, (size_t)2 * P->dir_nch * n));
+    EB_TRY(dmalloc(&P->wdz0, n));
   }
   if (mode == EB_MODE_FMM && L >= 2) {
     const int npar = 1 << (2 * (L - 1));
     EB_TRY(dmalloc(&P->l2p_items, l2p_item_cap(n, L)));
-    EB_TRY(dmalloc(&P->l2p_cnt, npar));
-    EB_TRY(dmalloc(&P->l2p_off, npar + 1));
-    EB_TRY(dmalloc(&P->l2p_tmp, scan_temp_ints(npar)));
+    EB_TRY(dmalloc(&P->l2p_total, 1));
     size_t cap = 0;
     for (int l = 2; l <= L; ++l) cap += ((size_t)1 << (2 * (l - 1))) + 3;
     P->m2l_cap = (int)cap;
@@ -1828,7 +1860,7 @@ static void launch_far(eb_fmm_plan P, double2* out) {
     (void)hipStreamWaitEvent(st, P->ev_join, 0);
   }
   const dim3 lgrid((unsigned)l2p_item_cap(P->n, L));
-  const int* nl2p = P->l2p_off + (1 << (2 * (L - 1)));
+  const int* nl2p = P->l2p_total;
   if (L >= 3)
     hipLaunchKernelGGL((k_l2p<K, true>), lgrid, dim3(L2P_T), 0, st, L, P->p, P->l2p_items, nl2p, P->keys, P->zs,
                        P->lo + P->lvl_off[L], P->lvl_W[L], P->lo + P->lvl_off[L - 1], P->lvl_W[L - 1], P->sgn, P->perm,
@@ -1913,7 +1945,7 @@ extern "C" int eb_fmm_set_p2p_tile(eb_fmm_plan P, int32_t tmax) {
     return EB_ERR_ARG;
   }
   P->p2p_tmax = tmax;
-  return build_p2p_items(P);
+  return P->tree_sorted ? build_items(P) : EB_OK;  // an unsorted direct-mode tree builds them when sorted
 }
 
 extern "C" int eb_fmm_set_guard(eb_fmm_plan P, const int32_t* flag) {
@@ -1942,6 +1974,7 @@ extern "C" int eb_fmm_stage_ms(eb_fmm_plan P, double* ms7) {
 
 extern "C" int eb_fmm_plan_info(eb_fmm_plan P, double* reals3, int64_t* counts8) {
   if (!P) return EB_ERR_ARG;
+  if (int rc = ensure_tree(P)) return rc;
   EB_HIP_CHECK(hipStreamSynchronize(P->st));
   Geom g;
   EB_HIP_CHECK(hipMemcpy(&g, P->geom, sizeof(Geom), hipMemcpyDeviceToHost));
@@ -2001,6 +2034,7 @@ extern "C" int eb_fmm_plan_info(eb_fmm_plan P, double* reals3, int64_t* counts8)
 // per-level non-empty box counts (levels 1..levmax) of a plan, host copy
 extern "C" int eb_fmm_plan_ntbox(eb_fmm_plan P, int32_t* ntbox) {
   if (!P || !ntbox) return EB_ERR_ARG;
+  if (int rc = ensure_tree(P)) return rc;
   const int nleaf = 1 << (2 * P->L);
   hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, P->st, P->ntbox, EB_MAXLEV + 2);
   hipLaunchKernelGGL(k_ntbox, dim3((nleaf + 255) / 256), dim3(256), 0, P->st, P->leaf_start, P->L, P->ntbox);
