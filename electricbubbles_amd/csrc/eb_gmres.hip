@@ -679,12 +679,8 @@ __global__ void k_gmb_scale(const int* __restrict__ off, const int* __restrict__
 
 // DORTH's first pass (dorth.f:90-107) on w = V(:, ll) of every active system: |w|^2, then
 // h_i = v_i . w and w -= h_i v_i for i = 0 .. ll-1 (MGS), then |w|^2
-__global__ __launch_bounds__(GB_T) void k_gmb_orth(int ll, const int* __restrict__ off, const int* __restrict__ act,
-                                                   double* __restrict__ V, size_t ldv, double* __restrict__ hs,
-                                                   int hld, int maxl) {
-  __shared__ double2 red[GB_T];
-  const int s = blockIdx.x;
-  if (!act[s]) return;
+__device__ __forceinline__ void gmb_orth_body(int s, int ll, const int* __restrict__ off, double* __restrict__ V,
+                                              size_t ldv, double* __restrict__ hs, int hld, int maxl, double2* red) {
   const int o = off[s], n = off[s + 1] - o;
   double* w = V + (size_t)ll * ldv + o;
   double* h = hs + (size_t)s * hld;
@@ -725,14 +721,19 @@ __global__ __launch_bounds__(GB_T) void k_gmb_orth(int ll, const int* __restrict
   }
 }
 
-// DORTH's re-orthogonalisation pass (dorth.f:108-121) for the flagged systems: the updated
-// Hessenberg column stays in h[0..ll) and SUMDSQ goes to h[maxl + 2]
-__global__ __launch_bounds__(GB_T) void k_gmb_reorth(int ll, const int* __restrict__ off, const int* __restrict__ flag,
-                                                     double* __restrict__ V, size_t ldv, double* __restrict__ hs,
-                                                     int hld, int maxl) {
+__global__ __launch_bounds__(GB_T) void k_gmb_orth(int ll, const int* __restrict__ off, const int* __restrict__ act,
+                                                   double* __restrict__ V, size_t ldv, double* __restrict__ hs,
+                                                   int hld, int maxl) {
   __shared__ double2 red[GB_T];
   const int s = blockIdx.x;
-  if (!flag[s]) return;
+  if (!act[s]) return;
+  gmb_orth_body(s, ll, off, V, ldv, hs, hld, maxl, red);
+}
+
+// DORTH's re-orthogonalisation pass (dorth.f:108-121) for the flagged systems: the updated
+// Hessenberg column stays in h[0..ll) and SUMDSQ goes to h[maxl + 2]
+__device__ __forceinline__ void gmb_reorth_body(int s, int ll, const int* __restrict__ off, double* __restrict__ V,
+                                                size_t ldv, double* __restrict__ hs, int hld, int maxl, double2* red) {
   const int o = off[s], n = off[s + 1] - o;
   double* w = V + (size_t)ll * ldv + o;
   double* h = hs + (size_t)s * hld;
@@ -750,6 +751,15 @@ __global__ __launch_bounds__(GB_T) void k_gmb_reorth(int ll, const int* __restri
     sumdsq = sumdsq + tem * tem;
   }
   if (threadIdx.x == 0) h[maxl + 2] = sumdsq;
+}
+
+__global__ __launch_bounds__(GB_T) void k_gmb_reorth(int ll, const int* __restrict__ off, const int* __restrict__ flag,
+                                                     double* __restrict__ V, size_t ldv, double* __restrict__ hs,
+                                                     int hld, int maxl) {
+  __shared__ double2 red[GB_T];
+  const int s = blockIdx.x;
+  if (!flag[s]) return;
+  gmb_reorth_body(s, ll, off, V, ldv, hs, hld, maxl, red);
 }
 
 // ---- device-side control (no host synchronisation inside a solve) ---------------------------
@@ -784,32 +794,29 @@ __global__ void k_gmb_dstart(int S, double tol, const double* __restrict__ hs, i
   st[s] = g;
 }
 
-// after k_gmb_orth: |w| after MGS, and DORTH's test for a second pass (dorth.f:104-107)
-__global__ void k_gmb_dctl1(int S, const double* __restrict__ hs, int hld, int mx, const int* __restrict__ act,
-                            int* __restrict__ flag, GmbState* __restrict__ st) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= S) return;
-  flag[s] = 0;
-  if (!act[s]) return;
+// after the orthogonalisation pass: |w| after MGS, and DORTH's test for a second pass
+// (dorth.f:104-107)
+// one active system: returns the re-orthogonalisation flag
+__device__ __forceinline__ int gmb_dctl1_one(int s, const double* __restrict__ hs, int hld, int mx,
+                                            GmbState* __restrict__ st) {
   const double* h = hs + (size_t)s * hld;
   const double vnrm = sqrt(h[mx]);
   const double snormw = sqrt(h[mx + 1]);
   st[s].snormw = snormw;
-  if (!(vnrm + 0.001 * snormw != vnrm)) flag[s] = 1;
+  return !(vnrm + 0.001 * snormw != vnrm) ? 1 : 0;
 }
 
-// after k_gmb_reorth: the Hessenberg column, DHEQR, the tests of dpigmr.f:366-392 and, for a
+
+
+// after the re-orthogonalisation pass: the Hessenberg column, DHEQR, the tests of dpigmr.f:366-392 and, for a
 // system that stops with an update, DHELS into cf (stride cld).  A system still iterating
 // after `budget` steps stops with the update of those steps and IE = 3 (counted in *fail).
-__global__ void k_gmb_dctl2(int ll, int S, double tol, int budget, const double* __restrict__ hs, int hld, int mx,
-                            const int* __restrict__ maxl_s, int* __restrict__ act, const int* __restrict__ flag,
-                            double* __restrict__ fac, int* __restrict__ lcnt, double* __restrict__ cf, int cld,
-                            double* __restrict__ hesa, double* __restrict__ qa, GmbState* __restrict__ st,
-                            int* __restrict__ fail) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= S) return;
-  fac[s] = 0.0;
-  if (!act[s]) return;
+// one active system (act[s] = 1 on entry); fac[s] is 0 on entry
+__device__ void gmb_dctl2_one(int s, int ll, double tol, int budget, const double* __restrict__ hs, int hld, int mx,
+                              const int* __restrict__ maxl_s, int* __restrict__ act, const int* __restrict__ flag,
+                              double* __restrict__ fac, int* __restrict__ lcnt, double* __restrict__ cf, int cld,
+                              double* __restrict__ hesa, double* __restrict__ qa, GmbState* __restrict__ st,
+                              int* __restrict__ fail) {
   const double* h = hs + (size_t)s * hld;
   const int ldh = mx + 1;
   GmbState g = st[s];
@@ -861,6 +868,56 @@ __global__ void k_gmb_dctl2(int ll, int S, double tol, int budget, const double*
     fac[s] = 1.0 / g.snormw;
   }
   st[s] = g;
+}
+
+
+
+// One Arnoldi step of every system after its operator call, in one launch: one workgroup per
+// system runs the orthogonalisation pass (gmb_orth_body), DORTH's test (gmb_dctl1_one, thread
+// 0), the re-orthogonalisation pass if flagged, the rotation and stopping tests (gmb_dctl2_one,
+// thread 0) and the scaling of the new basis vector, in that order and arithmetic.
+// Systems are independent, so nothing orders one system's step after another's.
+__global__ __launch_bounds__(GB_T) void k_gmb_step(int ll, double tol, int budget, const int* __restrict__ off,
+                                                   double* __restrict__ V, size_t ldv, double* __restrict__ hs,
+                                                   int hld, int mx, const int* __restrict__ maxl_s,
+                                                   int* __restrict__ act, int* __restrict__ flag,
+                                                   double* __restrict__ fac, int* __restrict__ lcnt,
+                                                   double* __restrict__ cf, int cld, double* __restrict__ hesa,
+                                                   double* __restrict__ qa, GmbState* __restrict__ st,
+                                                   int* __restrict__ fail) {
+  __shared__ double2 red[GB_T];
+  __shared__ int sh_flag, sh_act;
+  __shared__ double sh_fac;
+  const int s = blockIdx.x;
+  if (!act[s]) {  // the whole block: before any barrier
+    if (threadIdx.x == 0) {
+      flag[s] = 0;
+      fac[s] = 0.0;
+    }
+    return;
+  }
+  gmb_orth_body(s, ll, off, V, ldv, hs, hld, mx, red);
+  if (threadIdx.x == 0) {
+    const int f = gmb_dctl1_one(s, hs, hld, mx, st);
+    flag[s] = f;
+    sh_flag = f;
+  }
+  __syncthreads();
+  if (sh_flag) gmb_reorth_body(s, ll, off, V, ldv, hs, hld, mx, red);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    fac[s] = 0.0;
+    gmb_dctl2_one(s, ll, tol, budget, hs, hld, mx, maxl_s, act, flag, fac, lcnt, cf, cld, hesa, qa, st, fail);
+    sh_act = act[s];
+    sh_fac = fac[s];
+  }
+  __syncthreads();
+  if (sh_act) {
+    const int o = off[s], n = off[s + 1] - o;
+    double* w = V + (size_t)ll * ldv + o;
+    const double c = sh_fac;
+    for (int i = threadIdx.x; i < n; i += GB_T) w[i] *= c;
+  }
 }
 
 __global__ void k_gmb_zero(int* __restrict__ v) {
@@ -1132,12 +1189,9 @@ extern "C" int eb_gmres_batch_solve_async(eb_gmres_batch g, eb_apply_fn apply, v
   for (int ll = 1; ll <= nl; ++ll) {
     double* w = g->V + (size_t)ll * ldv;
     if ((rc = apply(ctx, g->V + (size_t)(ll - 1) * ldv, w, st)) != EB_OK) return rc;
-    hipLaunchKernelGGL(k_gmb_orth, dim3(S), dim3(GB_T), 0, st, ll, g->d_off, g->d_act, g->V, ldv, g->hs, hld, mx);
-    hipLaunchKernelGGL(k_gmb_dctl1, cb, ct, 0, st, S, g->hs, hld, mx, g->d_act, g->d_flag, g->d_state);
-    hipLaunchKernelGGL(k_gmb_reorth, dim3(S), dim3(GB_T), 0, st, ll, g->d_off, g->d_flag, g->V, ldv, g->hs, hld, mx);
-    hipLaunchKernelGGL(k_gmb_dctl2, cb, ct, 0, st, ll, S, tol, nl, g->hs, hld, mx, g->d_maxl, g->d_act, g->d_flag,
-                       g->d_fac, g->d_lcnt, g->d_cf, mx + 1, g->d_hes, g->d_q, g->d_state, g->d_fail);
-    hipLaunchKernelGGL(k_gmb_scale, dim3(S), dim3(256), 0, st, g->d_off, g->d_act, g->d_fac, w, w);
+    hipLaunchKernelGGL(k_gmb_step, dim3(S), dim3(GB_T), 0, st, ll, tol, nl, g->d_off, g->V, ldv, g->hs, hld, mx,
+                       g->d_maxl, g->d_act, g->d_flag, g->d_fac, g->d_lcnt, g->d_cf, mx + 1, g->d_hes, g->d_q,
+                       g->d_state, g->d_fail);
   }
   hipLaunchKernelGGL(k_gmb_combine, dim3(S), dim3(256), 0, st, g->d_off, g->d_lcnt, g->d_cf, mx + 1, g->V, ldv, x);
   EB_HIP_CHECK(hipGetLastError());
