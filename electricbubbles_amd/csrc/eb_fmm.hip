@@ -517,139 +517,26 @@ __device__ __forceinline__ double swap_adjacent(double v) {
   return __hiloint2double(hi, lo);
 }
 
-// With a compacted target list (M2L only: m2l_list / m2l_count from k_m2l_write) block b
-// takes entries 16 (b / 4) .. 16 (b / 4) + 15 of parity class b % 4, four per wave: only
-// non-empty target boxes are translated, which matters where most boxes are empty (drops
-// scattered over a large square).  Blocks past the class's count leave at once.
-//
-// Output rows: MT tiles of 16 on the matrix cores, then TR (0, 4 or 8) tail rows on the
-// vector ALUs.  A 16x16x4 FP64 MFMA costs as much as 64 vector FMAs per lane; the tail costs
-// KS + 2 per row and source, so p = 33..36 pays for 36 rows instead of 48.  Rows past
-// 16 MT + TR are not written: they are zero for every p <= 16 MT + TR (the tables vanish
-// there), and no reader goes past row 4 KS - 1 (translations) or p - 1 (k_l2p).
-//
-// ADAPT: boxes of an adaptive tree (the adaptive plan below), stored box-major by box index.
-// Targets come from per-class lists (m2l_list / m2l_count / m2l_cap, no level runs: the
-// scaled translations do not depend on the level), sources from a slot table stab[target * NS
-// + slot]; a slot without a box holds nsrc, one past the last box, which the buffer resource
-// reads as zeros.
-template <int MODE, int KMAX, int KS, int MT, int TR, bool ADAPT = false>
-__device__ __forceinline__ void xlat_body(const XlatLevels& LS, const int* __restrict__ leaf_start,
-                                          const double2* __restrict__ src, double2* __restrict__ dst,
-                                          const double* __restrict__ Mat, const double2* __restrict__ pre,
-                                          const double2* __restrict__ post, const int* __restrict__ offs,
-                                          const int* __restrict__ m2l_list, const int* __restrict__ m2l_count,
-                                          int m2l_cap, const int* __restrict__ stab = nullptr, int nsrc = 0) {
-  static_assert(KS * 4 <= KMAX && MT * 16 + TR <= KMAX, "tile exceeds the padded expansion length");
-  static_assert(TR % 4 == 0 && TR <= 8 && KS * 4 <= MT * 16 + TR, "tail rows must cover the expansion");
+// The per-wave part of a translation: this wave's four targets (target col / 4 of lane col), box
+// pt of the destination level (dst), each from the NS sources that src_box(o) names in the
+// source level (src, nsrc_boxes boxes; an index past it reads zeros); live lanes write.
+// Cs, Ps and Qs are the LDS tables (Ps and Qs at this wave's rows).  No barrier inside: a
+// kernel may call it for several target sets between its own barriers.
+template <int MODE, int KMAX, int KS, int MT, int TR, typename SrcBox>
+__device__ __forceinline__ void xlat_wave(const double* Cs, const double2* Ps, const double2* Qs, int pt, bool live,
+                                          SrcBox src_box, const double2* __restrict__ src, int nsrc_boxes,
+                                          double2* __restrict__ dst) {
   constexpr int NS = MODE == XL_M2L ? 27 : (MODE == XL_M2M ? 4 : 1);
   constexpr int NM = MT * 16 + TR, NK = KS * 4;
-  // row stride of the Pascal tile: 2 CS = 32 (mod 64) dwords puts the four k rows that one
-  // MFMA operand read touches on alternate halves of the LDS banks
   constexpr int CS = TR ? (NM + 16) / 32 * 32 + 16 : NM + 4;
-  __shared__ __align__(16) double Cs[NK * CS];
-  __shared__ double2 Ps[NS * NK * 2];  // per (source, k, component): the weights of this lane's column
-  __shared__ double2 Qs[NS * NM];
-  __shared__ int Sx[ADAPT ? 16 * NS : 1];  // ADAPT: the block's 16 targets' source boxes
-  // XCD-aware order: consecutive logical blocks (neighbouring rows, all four parity classes)
-  // are dispatched to the same XCD and share its L2.
-  const bool compact = ADAPT || (MODE == XL_M2L && m2l_list != nullptr);
-  int nphys = gridDim.x;
-  if (compact) {
-    // logical blocks that can hold entries: 4 classes x the longest class's blocks, so that
-    // the XCD bands below spread them over all eight XCDs
-    int mx = 0;
-    for (int c = 0; c < 4; ++c) mx = max(mx, (m2l_count[c] + 15) / 16);
-    nphys = (4 * mx + 7) / 8 * 8;
-    if ((int)blockIdx.x >= nphys) return;  // the whole block: before any barrier
-  }
-  const int b = (blockIdx.x % 8) * (nphys / 8) + blockIdx.x / 8;
-  const int cls = b & 3;
-  const int ccount = compact ? m2l_count[cls] : 0;
-  if (compact && 16 * (b >> 2) >= ccount) return;
-  if constexpr (ADAPT) {
-    // the slot table's rows of this block's targets, read once instead of per source step
-    for (int i = threadIdx.x; i < 16 * NS; i += blockDim.x) {
-      const int idx = 16 * (b >> 2) + i / NS;
-      const int ent = idx < ccount ? m2l_list[(size_t)cls * m2l_cap + idx] : -1;
-      Sx[i] = ent >= 0 ? stab[(size_t)ent * NS + i % NS] : nsrc;
-    }
-  }
-  for (int i = threadIdx.x; i < NK * NM; i += blockDim.x) {
-    const int k = i / NM, m = i % NM;
-    Cs[k * CS + m] = Mat[k * KMAX + m];
-  }
-  // a lane of component c forms c ? Im(w a) : Re(w a) = W.x a.x + W.y a.y with
-  // W = c ? (w.y, w.x) : (w.x, -w.y): the component choice sits in the table, not in selects
-  for (int i = threadIdx.x; i < NS * NK; i += blockDim.x) {
-    const int o = i / NK, k = i % NK;
-    const int row = MODE == XL_M2L ? cls * 27 + o : (MODE == XL_M2M ? o : cls);
-    const double2 w = pre[(size_t)row * KMAX + k];
-    Ps[2 * i] = make_double2(w.x, -w.y);
-    Ps[2 * i + 1] = make_double2(w.y, w.x);
-  }
-  for (int i = threadIdx.x; i < NS * NM; i += blockDim.x) {
-    const int o = i / NM, m = i % NM;
-    const int row = MODE == XL_M2L ? cls * 27 + o : (MODE == XL_M2M ? o : cls);
-    Qs[i] = post[(size_t)row * KMAX + m];
-  }
-  __syncthreads();
-  if (b >= LS.total) return;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
   const int col = lane & 15, kr = lane >> 4;
-  const int t = col >> 2, s = (col >> 1) & 1, comp = col & 1;
-  int e = 0, q;
-  bool valid;
-  if (compact) {
-    const int idx = 16 * (b >> 2) + 4 * wv + t;
-    const int ent = idx < ccount ? m2l_list[(size_t)cls * m2l_cap + idx] : -1;
-    valid = ent >= 0;
-    if constexpr (ADAPT) {
-      if (__ballot(valid) == 0) return;  // past the class's list; no barrier follows
-      q = valid ? ent : 0;
-    } else {
-      // every valid entry of a wave has the same level (runs are padded to groups of 4)
-      int ev = valid ? (ent >> 24) : -1;
-#pragma unroll
-      for (int o = 32; o > 0; o >>= 1) ev = max(ev, __shfl_xor(ev, o, 64));
-      if (ev < 0) return;  // a wave of padding only; no barrier follows
-      e = __builtin_amdgcn_readfirstlane(ev);
-      q = valid ? (ent & 0xFFFFFF) : 0;
-    }
-  } else {
-    while (e + 1 < LS.nent && b >= LS.blk0[e + 1]) ++e;
-    const int g = (b - LS.blk0[e]) >> 2;
-    q = (g * 4 + wv) * 4 + t;
-    valid = q < (1 << (2 * (LS.lev[e] - 1)));
-  }
-  const int l = LS.lev[e], Wt = LS.Wt[e], NBPt = LS.NBPt[e], Ws = LS.Ws[e], NBPs = LS.NBPs[e];
-  const int half = ADAPT ? 1 : 1 << (l - 1);  // the level tables are not read in ADAPT
-  const int qq = valid ? q : 0;
-  const int ix = 2 * (qq % half) + (cls & 1), iy = 2 * (qq / half) + (cls >> 1);
-  bool live = valid;
-  if (!ADAPT && MODE != XL_M2M && valid) {
-    int bs, be;
-    box_range(leaf_start, LS.L, l, ix, iy, bs, be);
-    live = be > bs;
-  }
-  // A wave whose four target boxes are all empty has nothing to write (their local
-  // expansions are never read: L2L and L2P only visit non-empty boxes).  No barrier follows,
-  // so the whole wave may leave; sparse geometries leave most waves here.
-  if (!ADAPT && MODE != XL_M2M && __ballot(live) == 0) return;
-  const int pt = ADAPT ? qq : pidx(ix, iy, Wt);
-  const int* sxp = &Sx[ADAPT ? (4 * wv + t) * NS : 0];
-  auto src_box = [&](int o) -> int {
-    if constexpr (ADAPT) return sxp[o];
-    if (MODE == XL_M2L) return pt + offs[(cls * 27 + o) * 2 + 1] * Ws + offs[(cls * 27 + o) * 2 + 0];
-    if (MODE == XL_M2M) return pidx(2 * ix + (o & 1), 2 * iy + (o >> 1), Ws);
-    return pidx(ix >> 1, iy >> 1, Ws);
-  };
+  const int s = (col >> 1) & 1, comp = col & 1;
   // Source expansions through a buffer resource covering both series of the level: the
   // per-k row step is a scalar offset (no per-lane address arithmetic per load), and an
   // offset outside the level reads 0 instead of faulting.
   const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
-      (void*)(ADAPT ? src : src + LS.offs[e]), (short)0,
-      (int)(uint32_t)((size_t)2 * KMAX * (ADAPT ? nsrc : NBPs) * 16), 0x00020000);
+      (void*)src, (short)0, (int)(uint32_t)((size_t)2 * KMAX * nsrc_boxes * 16), 0x00020000);
   const int row_bytes = 4 * 16;  // k -> k + 4 within the box's coefficients
   const int lane_row = (s * KMAX + kr) * 16;
   auto load_src = [&](int o, double2* a) {
@@ -777,7 +664,7 @@ __device__ __forceinline__ void xlat_body(const XlatLevels& LS, const int* __res
     }
   }
   if (live) {
-    double* out = (double*)(ADAPT ? dst : dst + LS.offt[e]);
+    double* out = (double*)dst;
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
@@ -793,6 +680,137 @@ __device__ __forceinline__ void xlat_body(const XlatLevels& LS, const int* __res
       out[at] = MODE == XL_L2L ? out[at] + tlam[q] : tlam[q];
     }
   }
+}
+
+// With a compacted target list (M2L only: m2l_list / m2l_count from k_m2l_write) block b
+// takes entries 16 (b / 4) .. 16 (b / 4) + 15 of parity class b % 4, four per wave: only
+// non-empty target boxes are translated, which matters where most boxes are empty (drops
+// scattered over a large square).  Blocks past the class's count leave at once.
+//
+// Output rows: MT tiles of 16 on the matrix cores, then TR (0, 4 or 8) tail rows on the
+// vector ALUs.  A 16x16x4 FP64 MFMA costs as much as 64 vector FMAs per lane; the tail costs
+// KS + 2 per row and source, so p = 33..36 pays for 36 rows instead of 48.  Rows past
+// 16 MT + TR are not written: they are zero for every p <= 16 MT + TR (the tables vanish
+// there), and no reader goes past row 4 KS - 1 (translations) or p - 1 (k_l2p).
+//
+// ADAPT: boxes of an adaptive tree (the adaptive plan below), stored box-major by box index.
+// Targets come from per-class lists (m2l_list / m2l_count / m2l_cap, no level runs: the
+// scaled translations do not depend on the level), sources from a slot table stab[target * NS
+// + slot]; a slot without a box holds nsrc, one past the last box, which the buffer resource
+// reads as zeros.
+template <int MODE, int KMAX, int KS, int MT, int TR, bool ADAPT = false>
+__device__ __forceinline__ void xlat_body(const XlatLevels& LS, const int* __restrict__ leaf_start,
+                                          const double2* __restrict__ src, double2* __restrict__ dst,
+                                          const double* __restrict__ Mat, const double2* __restrict__ pre,
+                                          const double2* __restrict__ post, const int* __restrict__ offs,
+                                          const int* __restrict__ m2l_list, const int* __restrict__ m2l_count,
+                                          int m2l_cap, const int* __restrict__ stab = nullptr, int nsrc = 0) {
+  static_assert(KS * 4 <= KMAX && MT * 16 + TR <= KMAX, "tile exceeds the padded expansion length");
+  static_assert(TR % 4 == 0 && TR <= 8 && KS * 4 <= MT * 16 + TR, "tail rows must cover the expansion");
+  constexpr int NS = MODE == XL_M2L ? 27 : (MODE == XL_M2M ? 4 : 1);
+  constexpr int NM = MT * 16 + TR, NK = KS * 4;
+  // row stride of the Pascal tile: 2 CS = 32 (mod 64) dwords puts the four k rows that one
+  // MFMA operand read touches on alternate halves of the LDS banks
+  constexpr int CS = TR ? (NM + 16) / 32 * 32 + 16 : NM + 4;
+  __shared__ __align__(16) double Cs[NK * CS];
+  __shared__ double2 Ps[NS * NK * 2];  // per (source, k, component): the weights of this lane's column
+  __shared__ double2 Qs[NS * NM];
+  __shared__ int Sx[ADAPT ? 16 * NS : 1];  // ADAPT: the block's 16 targets' source boxes
+  // XCD-aware order: consecutive logical blocks (neighbouring rows, all four parity classes)
+  // are dispatched to the same XCD and share its L2.
+  const bool compact = ADAPT || (MODE == XL_M2L && m2l_list != nullptr);
+  int nphys = gridDim.x;
+  if (compact) {
+    // logical blocks that can hold entries: 4 classes x the longest class's blocks, so that
+    // the XCD bands below spread them over all eight XCDs
+    int mx = 0;
+    for (int c = 0; c < 4; ++c) mx = max(mx, (m2l_count[c] + 15) / 16);
+    nphys = (4 * mx + 7) / 8 * 8;
+    if ((int)blockIdx.x >= nphys) return;  // the whole block: before any barrier
+  }
+  const int b = (blockIdx.x % 8) * (nphys / 8) + blockIdx.x / 8;
+  const int cls = b & 3;
+  const int ccount = compact ? m2l_count[cls] : 0;
+  if (compact && 16 * (b >> 2) >= ccount) return;
+  if constexpr (ADAPT) {
+    // the slot table's rows of this block's targets, read once instead of per source step
+    for (int i = threadIdx.x; i < 16 * NS; i += blockDim.x) {
+      const int idx = 16 * (b >> 2) + i / NS;
+      const int ent = idx < ccount ? m2l_list[(size_t)cls * m2l_cap + idx] : -1;
+      Sx[i] = ent >= 0 ? stab[(size_t)ent * NS + i % NS] : nsrc;
+    }
+  }
+  for (int i = threadIdx.x; i < NK * NM; i += blockDim.x) {
+    const int k = i / NM, m = i % NM;
+    Cs[k * CS + m] = Mat[k * KMAX + m];
+  }
+  // a lane of component c forms c ? Im(w a) : Re(w a) = W.x a.x + W.y a.y with
+  // W = c ? (w.y, w.x) : (w.x, -w.y): the component choice sits in the table, not in selects
+  for (int i = threadIdx.x; i < NS * NK; i += blockDim.x) {
+    const int o = i / NK, k = i % NK;
+    const int row = MODE == XL_M2L ? cls * 27 + o : (MODE == XL_M2M ? o : cls);
+    const double2 w = pre[(size_t)row * KMAX + k];
+    Ps[2 * i] = make_double2(w.x, -w.y);
+    Ps[2 * i + 1] = make_double2(w.y, w.x);
+  }
+  for (int i = threadIdx.x; i < NS * NM; i += blockDim.x) {
+    const int o = i / NM, m = i % NM;
+    const int row = MODE == XL_M2L ? cls * 27 + o : (MODE == XL_M2M ? o : cls);
+    Qs[i] = post[(size_t)row * KMAX + m];
+  }
+  __syncthreads();
+  if (b >= LS.total) return;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int col = lane & 15, kr = lane >> 4;
+  const int t = col >> 2, s = (col >> 1) & 1, comp = col & 1;
+  int e = 0, q;
+  bool valid;
+  if (compact) {
+    const int idx = 16 * (b >> 2) + 4 * wv + t;
+    const int ent = idx < ccount ? m2l_list[(size_t)cls * m2l_cap + idx] : -1;
+    valid = ent >= 0;
+    if constexpr (ADAPT) {
+      if (__ballot(valid) == 0) return;  // past the class's list; no barrier follows
+      q = valid ? ent : 0;
+    } else {
+      // every valid entry of a wave has the same level (runs are padded to groups of 4)
+      int ev = valid ? (ent >> 24) : -1;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) ev = max(ev, __shfl_xor(ev, o, 64));
+      if (ev < 0) return;  // a wave of padding only; no barrier follows
+      e = __builtin_amdgcn_readfirstlane(ev);
+      q = valid ? (ent & 0xFFFFFF) : 0;
+    }
+  } else {
+    while (e + 1 < LS.nent && b >= LS.blk0[e + 1]) ++e;
+    const int g = (b - LS.blk0[e]) >> 2;
+    q = (g * 4 + wv) * 4 + t;
+    valid = q < (1 << (2 * (LS.lev[e] - 1)));
+  }
+  const int l = LS.lev[e], Wt = LS.Wt[e], NBPt = LS.NBPt[e], Ws = LS.Ws[e], NBPs = LS.NBPs[e];
+  const int half = ADAPT ? 1 : 1 << (l - 1);  // the level tables are not read in ADAPT
+  const int qq = valid ? q : 0;
+  const int ix = 2 * (qq % half) + (cls & 1), iy = 2 * (qq / half) + (cls >> 1);
+  bool live = valid;
+  if (!ADAPT && MODE != XL_M2M && valid) {
+    int bs, be;
+    box_range(leaf_start, LS.L, l, ix, iy, bs, be);
+    live = be > bs;
+  }
+  // A wave whose four target boxes are all empty has nothing to write (their local
+  // expansions are never read: L2L and L2P only visit non-empty boxes).  No barrier follows,
+  // so the whole wave may leave; sparse geometries leave most waves here.
+  if (!ADAPT && MODE != XL_M2M && __ballot(live) == 0) return;
+  const int pt = ADAPT ? qq : pidx(ix, iy, Wt);
+  const int* sxp = &Sx[ADAPT ? (4 * wv + t) * NS : 0];
+  auto src_box = [&](int o) -> int {
+    if constexpr (ADAPT) return sxp[o];
+    if (MODE == XL_M2L) return pt + offs[(cls * 27 + o) * 2 + 1] * Ws + offs[(cls * 27 + o) * 2 + 0];
+    if (MODE == XL_M2M) return pidx(2 * ix + (o & 1), 2 * iy + (o >> 1), Ws);
+    return pidx(ix >> 1, iy >> 1, Ws);
+  };
+  xlat_wave<MODE, KMAX, KS, MT, TR>(Cs, Ps, Qs, pt, live, src_box, ADAPT ? src : src + LS.offs[e], ADAPT ? nsrc : NBPs,
+                                    ADAPT ? dst : dst + LS.offt[e]);
 }
 
 #define EB_XLAT_ARGS                                                                                     \
@@ -817,6 +835,93 @@ __global__ __launch_bounds__(256) void k_axlat(EB_XLAT_ARGS, const int* __restri
                                           m2l_cap, stab, nsrc);
 }
 #undef EB_XLAT_ARGS
+
+// Several consecutive levels of M2M (upward) or L2L (downward) in one launch, for the coarse
+// levels, whose single launches are bound by launch latency (about 10 us each for a few dozen
+// to a few thousand boxes): one block per box of level `root`, which runs the translations
+// of its subtree level by level with a barrier between levels.  Each wave takes four targets
+// of one parity class at a time (the L2L tables differ by class; all four classes' rows are
+// in LDS), through the same per-wave product as the one-level kernels (xlat_wave).
+struct XlatSub {
+  int root;            // level of the subtree roots: the grid is 4^root blocks
+  int nlev;            // levels run, in order (M2M: finest first, ending at root; L2L: root+1 on)
+  int L;               // finest level of the plan (box_range)
+  int lev[4];
+  int Wt[4], Ws[4], NBPs[4];
+  long long offt[4], offs[4];
+};
+
+template <int MODE, int KMAX, int KS, int MT, int TR>
+__global__ __launch_bounds__(256) void k_xlat_sub(XlatSub X, const int* __restrict__ leaf_start,
+                                                  const double2* src, double2* dst,
+                                                  const double* __restrict__ Mat, const double2* __restrict__ pre,
+                                                  const double2* __restrict__ post) {
+  static_assert(MODE != XL_M2L, "interaction lists run on k_xlat");
+  // src and dst are one array (a level reads what the level before it wrote, after the
+  // barrier), so neither is declared __restrict__
+  constexpr int NM = MT * 16 + TR, NK = KS * 4;
+  constexpr int CS = TR ? (NM + 16) / 32 * 32 + 16 : NM + 4;
+  __shared__ __align__(16) double Cs[NK * CS];
+  __shared__ double2 Ps[4 * NK * 2];  // rows: M2M the child slot o, L2L the target's class
+  __shared__ double2 Qs[4 * NM];
+  for (int i = threadIdx.x; i < NK * NM; i += blockDim.x) {
+    const int k = i / NM, m = i % NM;
+    Cs[k * CS + m] = Mat[k * KMAX + m];
+  }
+  for (int i = threadIdx.x; i < 4 * NK; i += blockDim.x) {
+    const int row = i / NK, k = i % NK;
+    const double2 w = pre[(size_t)row * KMAX + k];
+    Ps[2 * i] = make_double2(w.x, -w.y);
+    Ps[2 * i + 1] = make_double2(w.y, w.x);
+  }
+  for (int i = threadIdx.x; i < 4 * NM; i += blockDim.x) {
+    const int row = i / NM, m = i % NM;
+    Qs[i] = post[(size_t)row * KMAX + m];
+  }
+  __syncthreads();
+  const int side0 = 1 << X.root;
+  const int rx = blockIdx.x % side0, ry = blockIdx.x / side0;
+  const int wv = threadIdx.x >> 6, t = (threadIdx.x & 15) >> 2;
+  for (int e = 0; e < X.nlev; ++e) {
+    const int l = X.lev[e];
+    const int sd = 1 << (l - X.root);           // the subtree's side at level l, in boxes
+    const int hs = sd >= 2 ? sd / 2 : 1;
+    const int nper = hs * hs;                   // targets per class (the root alone when sd = 1)
+    const int jpc = (nper + 3) / 4;             // wave jobs per class
+    for (int job = wv; job < 4 * jpc; job += 4) {
+      const int c = job / jpc, j = (job % jpc) * 4 + t;
+      int ix, iy;
+      bool valid;
+      if (sd >= 2) {
+        valid = j < nper;
+        const int jj = valid ? j : 0;
+        ix = rx * sd + 2 * (jj % hs) + (c & 1);
+        iy = ry * sd + 2 * (jj / hs) + (c >> 1);
+      } else {  // M2M into the root: the job of its own class
+        valid = j == 0 && c == ((rx & 1) | ((ry & 1) << 1));
+        ix = rx;
+        iy = ry;
+      }
+      bool live = valid;
+      if (MODE == XL_L2L && valid) {  // as the one-level kernel: empty boxes' locals are never read
+        int bs, be;
+        box_range(leaf_start, X.L, l, ix, iy, bs, be);
+        live = be > bs;
+      }
+      if (__ballot(live) == 0) continue;  // no barrier inside the level's loop
+      const int pt = pidx(ix, iy, X.Wt[e]);
+      const int Ws = X.Ws[e];
+      auto src_box = [&](int o) -> int {
+        if (MODE == XL_M2M) return pidx(2 * ix + (o & 1), 2 * iy + (o >> 1), Ws);
+        return pidx(ix >> 1, iy >> 1, Ws);
+      };
+      const double2* Pw = MODE == XL_M2M ? Ps : Ps + (size_t)c * NK * 2;
+      const double2* Qw = MODE == XL_M2M ? Qs : Qs + (size_t)c * NM;
+      xlat_wave<MODE, KMAX, KS, MT, TR>(Cs, Pw, Qw, pt, live, src_box, src + X.offs[e], X.NBPs[e], dst + X.offt[e]);
+    }
+    __syncthreads();  // the level's expansions are the next level's sources
+  }
+}
 
 // L2P at the finest level (SLLVALDR, cadapdrop2.f:331-348):
 //   GW_far = phi + sgn (z conj(phi') + conj(psi))
@@ -1819,6 +1924,69 @@ static void launch_xlat(eb_fmm_plan P, const XlatLevels& LS, const double2* src,
   }
 }
 
+// coarse levels [a, b] of M2M (a > b, finest first) or L2L (a < b) as subtree launches
+static XlatSub xlat_sub(eb_fmm_plan P, int root, int lfirst, int llast, int dl) {
+  XlatSub X{};
+  X.root = root;
+  X.L = P->L;
+  const int step = lfirst <= llast ? 1 : -1;
+  for (int l = lfirst;; l += step) {
+    const int e = X.nlev++;
+    X.lev[e] = l;
+    X.Wt[e] = P->lvl_W[l];
+    X.offt[e] = (long long)P->lvl_off[l];
+    X.Ws[e] = P->lvl_W[l + dl];
+    X.NBPs[e] = P->lvl_NBP[l + dl];
+    X.offs[e] = (long long)P->lvl_off[l + dl];
+    if (l == llast) break;
+  }
+  return X;
+}
+
+template <int MODE, int K, int KS>
+static void launch_xlat_sub_ks(eb_fmm_plan P, const XlatSub& X, const double2* src, double2* dst, const double2* pre,
+                               const double2* post) {
+  constexpr int TR = (K == 48 && KS == 9) ? 4 : 0;
+  constexpr int MT = TR ? 2 : K / 16;
+  hipLaunchKernelGGL((k_xlat_sub<MODE, K, KS, MT, TR>), dim3(1u << (2 * X.root)), dim3(256), 0, P->st, X,
+                     P->leaf_start, src, dst, P->dMat + (size_t)MODE * K * K, pre, post);
+}
+
+template <int MODE, int K>
+static void launch_xlat_sub(eb_fmm_plan P, const XlatSub& X, const double2* src, double2* dst, const double2* pre,
+                            const double2* post) {
+  switch ((P->p + 3) / 4 - (K / 4 - 3)) {
+    case 0: launch_xlat_sub_ks<MODE, K, K / 4 - 3>(P, X, src, dst, pre, post); break;
+    case 1: launch_xlat_sub_ks<MODE, K, K / 4 - 2>(P, X, src, dst, pre, post); break;
+    case 2: launch_xlat_sub_ks<MODE, K, K / 4 - 1>(P, X, src, dst, pre, post); break;
+    default: launch_xlat_sub_ks<MODE, K, K / 4>(P, X, src, dst, pre, post); break;
+  }
+}
+
+// The upward pass: the finest M2M level alone (its targets fill the chip), then the coarser
+// ones three levels per launch (subtrees of 21 targets per block)
+template <int K>
+static void launch_m2m(eb_fmm_plan P) {
+  const int L = P->L;
+  if (L - 1 >= 2) launch_xlat<XL_M2M, K>(P, xlat_levels(P, L - 1, L - 1, +1), P->mp, P->mp, P->dPM, P->dQM);
+  for (int top = L - 2; top >= 2; top -= 3) {
+    const int root = std::max(2, top - 2);
+    launch_xlat_sub<XL_M2M, K>(P, xlat_sub(P, root, top, root, +1), P->mp, P->mp, P->dPM, P->dQM);
+  }
+}
+
+// The downward pass into levels 3 .. L-1: the coarse levels two per launch (subtrees of 20
+// targets per block), the finest (L - 1) alone
+template <int K>
+static void launch_l2l(eb_fmm_plan P) {
+  const int L = P->L;
+  for (int l = 3; l <= L - 2; l += 2) {
+    const int last = std::min(l + 1, L - 2);
+    launch_xlat_sub<XL_L2L, K>(P, xlat_sub(P, l - 1, l, last, -1), P->lo, P->lo, P->dPL, P->dQL);
+  }
+  if (L - 1 >= 3) launch_xlat<XL_L2L, K>(P, xlat_levels(P, L - 1, L - 1, -1), P->lo, P->lo, P->dPL, P->dQL);
+}
+
 // the near field: RSCAL * (the near sum) into out[perm[i]], on stream s
 static void launch_p2p(eb_fmm_plan P, hipStream_t s, double2* out) {
   hipLaunchKernelGGL(k_p2p, dim3((unsigned)p2p_item_cap(P->n, P->L, P->p2p_tmax)), dim3(64), 0, s, P->L, P->leaf_start,
@@ -1842,16 +2010,15 @@ static void launch_far(eb_fmm_plan P, double2* out) {
     hipLaunchKernelGGL(k_p2m<K>, dim3(1 << (2 * L)), dim3(64), 0, st, L, P->p, P->leaf_start, P->zs, P->qa, P->qb,
                      P->mp + P->lvl_off[L], P->lvl_W[L], P->lvl_NBP[L]);
   if (P->timing) (void)hipEventRecord(P->ev[2], st);
-  // upward pass, level by level
-  for (int l = L - 1; l >= 2; --l)
-    launch_xlat<XL_M2M, K>(P, xlat_levels(P, l, l, +1), P->mp, P->mp, P->dPM, P->dQM);
+  // upward pass
+  launch_m2m<K>(P);
   if (P->timing) (void)hipEventRecord(P->ev[3], st);
   // interaction lists of every level in one launch, then the downward pass
   launch_xlat<XL_M2L, K>(P, xlat_levels(P, 2, L, 0), P->mp, P->lo, P->dU, P->dNU);
   if (P->timing) (void)hipEventRecord(P->ev[4], st);
-  // the shift into the leaves is left to L2P, which evaluates the parents' expansions directly
-  for (int l = 3; l <= L - 1; ++l)
-    launch_xlat<XL_L2L, K>(P, xlat_levels(P, l, l, -1), P->lo, P->lo, P->dPL, P->dQL);
+  // the downward pass; the shift into the leaves is left to L2P, which evaluates the parents'
+  // expansions directly
+  launch_l2l<K>(P);
   if (P->timing) {
     (void)hipEventRecord(P->ev[5], st);
     launch_p2p(P, st, out);
