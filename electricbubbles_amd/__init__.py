@@ -49,7 +49,7 @@ EXPORTS = (
     "eb_fmm_plan_ntbox", "eb_fmm_enable_timing", "eb_fmm_stage_ms", "eb_fmm_set_p2p_tile", "eb_cadap_tables",
     "eb_cadap_lists", "eb_direct_apply_noalloc", "eb_direct_subset", "eb_dapif2", "eb_afmm_create",
     "eb_afmm_apply", "eb_afmm_info", "eb_afmm_enable_timing", "eb_afmm_stage_ms", "eb_afmm_check",
-    "eb_afmm_destroy", "eb_fmm_set_guard",
+    "eb_afmm_destroy", "eb_fmm_set_guard", "eb_fmm_apply_transpose",
 )
 
 _lib = None
@@ -83,6 +83,7 @@ def lib():
     L.eb_get_mode.restype = ctypes.c_int
     L.eb_fmm_plan_create.argtypes = [ctypes.POINTER(_vp), _i32, _vp, _vp, _vp, _i32, _i32, _i32, _i32, _vp]
     L.eb_fmm_apply.argtypes = [_vp, _vp, _vp, _vp]
+    L.eb_fmm_apply_transpose.argtypes = [_vp, _vp, _vp, _vp]
     L.eb_fmm_plan_rebuild.argtypes = [_vp, _vp, _vp, _vp, _vp]
     L.eb_fmm_plan_destroy.argtypes = [_vp]
     L.eb_fmm_plan_info.argtypes = [_vp, _dp, _i64p]
@@ -301,6 +302,20 @@ class FMMPlan:
         self._leave(s, dens, out)
         return out
 
+    def apply_transpose(self, lam, out=None):
+        """mu = A^T lam for the real-linear map A: dens -> apply(dens), under the real inner
+        product Re sum conj(a) b: the same sums with sources and targets swapped (one set of
+        the plan's passes on transposed charges)."""
+        torch = self._torch
+        assert lam.dtype == torch.complex128 and lam.numel() == self.n and lam.is_cuda
+        lam = lam.contiguous()
+        if out is None:
+            out = torch.empty(self.n, dtype=torch.complex128, device=lam.device)
+        s = self._enter()
+        _check(lib().eb_fmm_apply_transpose(self._h, _ptr(lam), _ptr(out), _stream_ptr(self.stream)))
+        self._leave(s, lam, out)
+        return out
+
     def info(self):
         r = np.zeros(3)
         c = np.zeros(8, np.int64)
@@ -393,6 +408,9 @@ class AFMMPlan(FMMPlan):
 
     def ntbox(self):
         raise EBError(EB_ERR_UNSUPPORTED, "no uniform levels in an adaptive plan")
+
+    def apply_transpose(self, lam, out=None):
+        raise EBError(EB_ERR_UNSUPPORTED, "the transposed product runs on uniform-grid plans (FMMPlan)")
 
     def enable_timing(self, on=True):
         _check(lib().eb_afmm_enable_timing(self._h, 1 if on else 0))

@@ -222,6 +222,56 @@ __global__ void k_charges(int n, const int32_t* __restrict__ perm, const double2
   qb[i] = make_double2(-c * t.y, c * t.x);
 }
 
+// The transposed products (eb_fmm_apply_transpose).  The forward product is real-linear in the
+// density: GW_i = sum_j (c_ij sigma_j + d_ij conj(sigma_j)), and its transpose under the real
+// inner product Re sum conj(a_i) b_i is mu_j = sum_i (conj(c_ij) lam_i + d_ij conj(lam_i)): the
+// same point set with sources and targets swapped.  Written with the forward's charges
+// (alpha = WDZ/(2 pi i), gamma = conj(WDZ)/(2 pi i), QA = alpha sigma,
+// QB = sqb alpha conj(sigma) + gamma sigma), for the CADAP family this is
+//   mu_j = -sgn (conj(alpha_j) V_j + sqb alpha_j phi_j + conj(gamma_j) conj(phi_j))
+// where phi and V = z conj(phi') + conj(psi) are the expansions' two parts at z_j for charges
+// QA = lam, QB = sgn conj(lam) (and QC = -conj(z) QA as in the forward), and for the Hilbert
+// product (GW_i = sum_j WDZ_j sigma_j / (z_i - z_j))
+//   mu_j = conj(WDZ_j) conj(phi_j)  with QA = conj(lam).
+// So the transposed product runs the forward passes unchanged on these charges, and only the
+// charges kernel and the per-target output form differ.
+constexpr int OUT_FORWARD = 0, OUT_T_CADAP = 1, OUT_T_HILBERT = 2;
+struct OutForm {
+  double sgn, sqb;
+  int kind;                // OUT_*
+  const double2* w;        // transposed forms: the points' WDZ in the plan's (sorted) order
+};
+
+// the output at sorted point i from its two parts (phi, V); w = WDZ of the point (transposed)
+template <bool TRANS>
+__device__ __forceinline__ double2 out_form(const OutForm& f, double2 phi, double2 v, double2 w) {
+  if constexpr (!TRANS) {
+    return make_double2(phi.x + f.sgn * v.x, phi.y + f.sgn * v.y);
+  } else {
+    if (f.kind == OUT_T_HILBERT) return cmul(cconj(w), cconj(phi));
+    const double c = 0.15915494309189533577;           // 1/(2 pi)
+    const double2 al = make_double2(c * w.y, -c * w.x);   // WDZ / (2 pi i)
+    const double2 ga = make_double2(-c * w.y, -c * w.x);  // conj(WDZ) / (2 pi i)
+    const double2 r = cadd(cadd(cmul(cconj(al), v), cscale(cmul(al, phi), f.sqb)), cmul(cconj(ga), cconj(phi)));
+    return make_double2(-f.sgn * r.x, -f.sgn * r.y);
+  }
+}
+
+// the transposed products' charges from lam (caller's order) in the sorted order
+__global__ void k_charges_t(int n, const int32_t* __restrict__ perm, const double2* __restrict__ lam, int kind,
+                            double sgn, double2* __restrict__ qa, double2* __restrict__ qb) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double2 l = lam[perm ? perm[i] : i];
+  if (kind == OUT_T_HILBERT) {
+    qa[i] = cconj(l);
+    qb[i] = make_double2(0.0, 0.0);
+  } else {
+    qa[i] = l;
+    qb[i] = make_double2(sgn * l.x, -sgn * l.y);
+  }
+}
+
 // P2M at the finest level (SLMKXP, cadap2.f:444-466), one 64-lane block per leaf box.
 // A chunk of up to 16 sources writes its powers t^m (t = (z-c)/r) into LDS, four lanes per
 // source each covering a quarter of the exponents; output (series, k) then forms
@@ -941,11 +991,11 @@ constexpr int L2P_T = 64;
 
 static size_t l2p_item_cap(int n, int L) { return ((size_t)1 << (2 * (L - 1))) + (size_t)n / L2P_T + 1; }
 
-template <int KMAX, bool PARENT>
+template <int KMAX, bool PARENT, bool TRANS>
 __global__ __launch_bounds__(L2P_T) void k_l2p(int L, int p, const int4* __restrict__ items,
                                                const int* __restrict__ nitems, const uint32_t* __restrict__ keys,
                                                const double2* __restrict__ zs, const double2* __restrict__ lo, int W,
-                                               const double2* __restrict__ lop, int Wp, double sgn,
+                                               const double2* __restrict__ lop, int Wp, OutForm of,
                                                const int32_t* __restrict__ perm, const Geom* __restrict__ g,
                                                double2* __restrict__ out) {
   __shared__ double2 C[5][2][KMAX];  // the four leaves' locals (child c = x bit | y bit << 1), the parent's
@@ -997,7 +1047,8 @@ __global__ __launch_bounds__(L2P_T) void k_l2p(int L, int p, const int4* __restr
   const double rs = g->rscal;
   const int o = perm[i];
   const double2 nearv = out[o];
-  out[o] = make_double2(nearv.x + (phi.x + sgn * t.x) * rs, nearv.y + (phi.y + sgn * t.y) * rs);
+  const double2 fv = out_form<TRANS>(of, phi, t, TRANS ? of.w[i] : make_double2(0, 0));
+  out[o] = make_double2(nearv.x + fv.x * rs, nearv.y + fv.y * rs);
 }
 
 
@@ -1048,11 +1099,12 @@ __global__ void k_items_write(int nleaf, int npar, const int* __restrict__ leaf_
 // the lanes split into G = 64/T source slices per target, summed through LDS at the end.
 constexpr int P2P_CH = 128;  // 6 KB of LDS per block: LDS does not limit the waves per CU
 
+template <bool TRANS>
 __device__ __forceinline__ void p2p_item(int itemid, int L, const int* __restrict__ leaf_start,
                                          const int4* __restrict__ items, const double2* __restrict__ zs,
                                          const double2* __restrict__ qa, const double2* __restrict__ qb,
                                          const double2* __restrict__ gwf, const int32_t* __restrict__ perm,
-                                         const Geom* __restrict__ g, double sgn, double2* __restrict__ out,
+                                         const Geom* __restrict__ g, const OutForm& of, double2* __restrict__ out,
                                          double2* sz, double2* sa, double2* sq) {
   double2* const red = sz;  // the per-slice partial sums reuse the staging buffers
   double2* const red2 = sa;
@@ -1137,8 +1189,10 @@ __device__ __forceinline__ void p2p_item(int itemid, int L, const int* __restric
       }
     }
     __syncthreads();
-    red[lane] = make_double2(a10.x + sgn * a20.x, a10.y + sgn * a20.y);
-    red2[lane] = make_double2(a11.x + sgn * a21.x, a11.y + sgn * a21.y);
+    // the output form is real-linear: applied per slice before the slices' sum
+    const double2 zero = make_double2(0, 0);
+    red[lane] = out_form<TRANS>(of, a10, a20, TRANS && act ? of.w[i0] : zero);
+    red2[lane] = out_form<TRANS>(of, a11, a21, TRANS && has1 ? of.w[i1] : zero);
     __syncthreads();
     if (lane < TPc && t0 + 2 * lane < T) {
       const int j0 = ts + t0 + 2 * lane, j1 = j0 + 1;
@@ -1158,15 +1212,16 @@ __device__ __forceinline__ void p2p_item(int itemid, int L, const int* __restric
 // One block per item (the grid is the item capacity; blocks past the count leave).  A smaller
 // persistent grid striding over the items, meant to leave room for the far-field passes running
 // beside it, measured slower at every size from 1024 to 8192 blocks.
+template <bool TRANS>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_p2p(
     int L, const int* __restrict__ leaf_start, const int4* __restrict__ items, const int* __restrict__ nitems,
     const double2* __restrict__ zs, const double2* __restrict__ qa, const double2* __restrict__ qb,
-    const double2* __restrict__ gwf, const int32_t* __restrict__ perm, const Geom* __restrict__ g, double sgn,
+    const double2* __restrict__ gwf, const int32_t* __restrict__ perm, const Geom* __restrict__ g, OutForm of,
     double2* __restrict__ out) {
   __shared__ double2 sz[P2P_CH], sa[P2P_CH], sq[P2P_CH];
   const int ni = *nitems;
   for (int it = blockIdx.x; it < ni; it += gridDim.x)
-    p2p_item(it, L, leaf_start, items, zs, qa, qb, gwf, perm, g, sgn, out, sz, sa, sq);
+    p2p_item<TRANS>(it, L, leaf_start, items, zs, qa, qb, gwf, perm, g, of, out, sz, sa, sq);
 }
 
 // O(N^2) direct evaluation of the same kernel (small N).  Grid (target tiles of 64, source
@@ -1211,9 +1266,10 @@ __global__ __launch_bounds__(DIR_T) void k_direct_part(int n, int sch, const dou
   }
 }
 
+template <bool TRANS>
 __global__ __launch_bounds__(256) void k_direct_sum(int n, int nch, const double2* __restrict__ part,
                                                     const int32_t* __restrict__ perm, const Geom* __restrict__ g,
-                                                    double sgn, double2* __restrict__ out) {
+                                                    OutForm of, double2* __restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double2 acc1 = make_double2(0, 0), acc2 = make_double2(0, 0);
@@ -1222,8 +1278,9 @@ __global__ __launch_bounds__(256) void k_direct_sum(int n, int nch, const double
     acc2 = cadd(acc2, part[2 * ((size_t)c * n + i) + 1]);
   }
   const double rs = g->rscal;
+  const double2 v = out_form<TRANS>(of, acc1, acc2, TRANS ? of.w[i] : make_double2(0, 0));
   // perm == nullptr: the points are in the caller's order
-  out[perm ? perm[i] : i] = make_double2((acc1.x + sgn * acc2.x) * rs, (acc1.y + sgn * acc2.y) * rs);
+  out[perm ? perm[i] : i] = make_double2(v.x * rs, v.y * rs);
 }
 
 // chunk length (a multiple of DIR_T) and count for n points: about DIR_MAXBLK blocks in all
@@ -1337,8 +1394,8 @@ extern "C" int eb_direct_apply_noalloc(int32_t n, const double* x, const double*
                      (const double2*)dens, g, sqb, variant == EB_VARIANT_HILBERT ? 1 : 0, zs, qa, qb);
   hipLaunchKernelGGL(k_direct_part, dim3((n + DIR_T - 1) / DIR_T, nch), dim3(DIR_T), 0, st, n, sch, zs, qa, qb,
                      (const int*)nullptr, part);
-  hipLaunchKernelGGL(k_direct_sum, dim3((n + 255) / 256), dim3(256), 0, st, n, nch, part, (const int32_t*)nullptr, g,
-                     sgn, (double2*)gradw);
+  hipLaunchKernelGGL(k_direct_sum<false>, dim3((n + 255) / 256), dim3(256), 0, st, n, nch, part,
+                     (const int32_t*)nullptr, g, OutForm{sgn, sqb, OUT_FORWARD, nullptr}, (double2*)gradw);
   EB_HIP_CHECK(hipGetLastError());
   return EB_OK;
 }
@@ -1987,18 +2044,30 @@ static void launch_l2l(eb_fmm_plan P) {
   if (L - 1 >= 3) launch_xlat<XL_L2L, K>(P, xlat_levels(P, L - 1, L - 1, -1), P->lo, P->lo, P->dPL, P->dQL);
 }
 
+// the output form of a forward (of.kind == OUT_FORWARD) or transposed apply
+static OutForm out_form_of(eb_fmm_plan P, bool trans) {
+  if (!trans) return OutForm{P->sgn, P->sqb, OUT_FORWARD, nullptr};
+  return OutForm{P->sgn, P->sqb, P->variant == EB_VARIANT_HILBERT ? OUT_T_HILBERT : OUT_T_CADAP, P->wdzs};
+}
+
 // the near field: RSCAL * (the near sum) into out[perm[i]], on stream s
-static void launch_p2p(eb_fmm_plan P, hipStream_t s, double2* out) {
-  hipLaunchKernelGGL(k_p2p, dim3((unsigned)p2p_item_cap(P->n, P->L, P->p2p_tmax)), dim3(64), 0, s, P->L, P->leaf_start,
-                     P->p2p_items, P->p2p_off + (1 << (2 * P->L)), P->zs, P->qa, P->qb, (const double2*)nullptr, P->perm,
-                     P->geom, P->sgn, out);
+static void launch_p2p(eb_fmm_plan P, hipStream_t s, double2* out, const OutForm& of) {
+  const dim3 grid((unsigned)p2p_item_cap(P->n, P->L, P->p2p_tmax));
+  if (of.kind == OUT_FORWARD)
+    hipLaunchKernelGGL(k_p2p<false>, grid, dim3(64), 0, s, P->L, P->leaf_start, P->p2p_items,
+                       P->p2p_off + (1 << (2 * P->L)), P->zs, P->qa, P->qb, (const double2*)nullptr, P->perm, P->geom,
+                       of, out);
+  else
+    hipLaunchKernelGGL(k_p2p<true>, grid, dim3(64), 0, s, P->L, P->leaf_start, P->p2p_items,
+                       P->p2p_off + (1 << (2 * P->L)), P->zs, P->qa, P->qb, (const double2*)nullptr, P->perm, P->geom,
+                       of, out);
 }
 
 // The far-field passes on the plan's stream, then L2P, which adds RSCAL * (the far field) to the
 // near field's output: after the join with the near field's stream, or (stage timing) after
 // running the near field in place on the plan's stream.
-template <int K>
-static void launch_far(eb_fmm_plan P, double2* out) {
+template <int K, bool TRANS>
+static void launch_far(eb_fmm_plan P, double2* out, const OutForm& of) {
   hipStream_t st = P->st;
   const int L = P->L;
   if (P->timing) (void)hipEventRecord(P->ev[1], st);
@@ -2021,7 +2090,7 @@ static void launch_far(eb_fmm_plan P, double2* out) {
   launch_l2l<K>(P);
   if (P->timing) {
     (void)hipEventRecord(P->ev[5], st);
-    launch_p2p(P, st, out);
+    launch_p2p(P, st, out, of);
     (void)hipEventRecord(P->ev[6], st);
   } else {
     (void)hipStreamWaitEvent(st, P->ev_join, 0);
@@ -2029,18 +2098,28 @@ static void launch_far(eb_fmm_plan P, double2* out) {
   const dim3 lgrid((unsigned)l2p_item_cap(P->n, L));
   const int* nl2p = P->l2p_total;
   if (L >= 3)
-    hipLaunchKernelGGL((k_l2p<K, true>), lgrid, dim3(L2P_T), 0, st, L, P->p, P->l2p_items, nl2p, P->keys, P->zs,
-                       P->lo + P->lvl_off[L], P->lvl_W[L], P->lo + P->lvl_off[L - 1], P->lvl_W[L - 1], P->sgn, P->perm,
-                       P->geom, out);
+    hipLaunchKernelGGL((k_l2p<K, true, TRANS>), lgrid, dim3(L2P_T), 0, st, L, P->p, P->l2p_items, nl2p, P->keys,
+                       P->zs, P->lo + P->lvl_off[L], P->lvl_W[L], P->lo + P->lvl_off[L - 1], P->lvl_W[L - 1], of,
+                       P->perm, P->geom, out);
   else
-    hipLaunchKernelGGL((k_l2p<K, false>), lgrid, dim3(L2P_T), 0, st, L, P->p, P->l2p_items, nl2p, P->keys, P->zs,
-                       P->lo + P->lvl_off[L], P->lvl_W[L], P->lo + P->lvl_off[L], P->lvl_W[L], P->sgn, P->perm,
+    hipLaunchKernelGGL((k_l2p<K, false, TRANS>), lgrid, dim3(L2P_T), 0, st, L, P->p, P->l2p_items, nl2p, P->keys,
+                       P->zs, P->lo + P->lvl_off[L], P->lvl_W[L], P->lo + P->lvl_off[L], P->lvl_W[L], of, P->perm,
                        P->geom, out);
 }
 
-extern "C" int eb_fmm_apply(eb_fmm_plan P, const double* dens, double* gradw, void* stream) {
+template <int K>
+static void launch_far_of(eb_fmm_plan P, double2* out, const OutForm& of) {
+  if (of.kind == OUT_FORWARD)
+    launch_far<K, false>(P, out, of);
+  else
+    launch_far<K, true>(P, out, of);
+}
+
+// eb_fmm_apply (trans = false) and eb_fmm_apply_transpose (trans = true): the same passes, with
+// the transposed charges and output form for the transpose
+static int fmm_apply(eb_fmm_plan P, const double* dens, double* gradw, void* stream, bool trans) {
   if (!P || !dens || !gradw) {
-    eb_set_error("eb_fmm_apply: null argument");
+    eb_set_error(trans ? "eb_fmm_apply_transpose: null argument" : "eb_fmm_apply: null argument");
     return EB_ERR_ARG;
   }
   if (int rc = use_stream(P, stream)) return rc;
@@ -2052,15 +2131,24 @@ extern "C" int eb_fmm_apply(eb_fmm_plan P, const double* dens, double* gradw, vo
       if (!e) EB_HIP_CHECK(hipEventCreate(&e));
     EB_HIP_CHECK(hipEventRecord(P->ev[0], st));
   }
-  hipLaunchKernelGGL(k_charges, dim3((n + 255) / 256), dim3(256), 0, st, n, P->perm, (const double2*)dens, P->wdzs,
-                     P->sqb, P->variant == EB_VARIANT_HILBERT ? 1 : 0, P->qa, P->qb);
+  const OutForm of = out_form_of(P, trans);
+  if (trans)
+    hipLaunchKernelGGL(k_charges_t, dim3((n + 255) / 256), dim3(256), 0, st, n, P->perm, (const double2*)dens, of.kind,
+                       P->sgn, P->qa, P->qb);
+  else
+    hipLaunchKernelGGL(k_charges, dim3((n + 255) / 256), dim3(256), 0, st, n, P->perm, (const double2*)dens, P->wdzs,
+                       P->sqb, P->variant == EB_VARIANT_HILBERT ? 1 : 0, P->qa, P->qb);
   if (P->mode == EB_MODE_DIRECT) {
     if (P->timing)
       for (int i = 1; i <= 6; ++i) (void)hipEventRecord(P->ev[i], st);
     hipLaunchKernelGGL(k_direct_part, dim3((n + DIR_T - 1) / DIR_T, P->dir_nch), dim3(DIR_T), 0, st, n, P->dir_sch,
                        P->zs, P->qa, P->qb, P->guard, P->dir_part);
-    hipLaunchKernelGGL(k_direct_sum, dim3((n + 255) / 256), dim3(256), 0, st, n, P->dir_nch, P->dir_part, P->perm,
-                       P->geom, P->sgn, out);
+    if (trans)
+      hipLaunchKernelGGL(k_direct_sum<true>, dim3((n + 255) / 256), dim3(256), 0, st, n, P->dir_nch, P->dir_part,
+                         P->perm, P->geom, of, out);
+    else
+      hipLaunchKernelGGL(k_direct_sum<false>, dim3((n + 255) / 256), dim3(256), 0, st, n, P->dir_nch, P->dir_part,
+                         P->perm, P->geom, of, out);
   } else if (P->L >= 2) {
     if (!P->timing) {
       // the near field beside the far-field passes: fork after the charges (measured at 2^20
@@ -2068,19 +2156,19 @@ extern "C" int eb_fmm_apply(eb_fmm_plan P, const double* dens, double* gradw, vo
       // with the fork after P2M)
       EB_HIP_CHECK(hipEventRecord(P->ev_fork, st));
       EB_HIP_CHECK(hipStreamWaitEvent(P->st2, P->ev_fork, 0));
-      launch_p2p(P, P->st2, out);
+      launch_p2p(P, P->st2, out, of);
       EB_HIP_CHECK(hipEventRecord(P->ev_join, P->st2));
     }
     switch (P->kmax) {
-      case 16: launch_far<16>(P, out); break;
-      case 32: launch_far<32>(P, out); break;
-      case 48: launch_far<48>(P, out); break;
-      default: launch_far<64>(P, out); break;
+      case 16: launch_far_of<16>(P, out, of); break;
+      case 32: launch_far_of<32>(P, out, of); break;
+      case 48: launch_far_of<48>(P, out, of); break;
+      default: launch_far_of<64>(P, out, of); break;
     }
   } else {
     if (P->timing)
       for (int i = 1; i <= 5; ++i) (void)hipEventRecord(P->ev[i], st);
-    launch_p2p(P, st, out);
+    launch_p2p(P, st, out, of);
     if (P->timing) (void)hipEventRecord(P->ev[6], st);
   }
   EB_HIP_CHECK(hipGetLastError());
@@ -2094,6 +2182,14 @@ extern "C" int eb_fmm_apply(eb_fmm_plan P, const double* dens, double* gradw, vo
     for (int i = 0; i < 7; ++i) P->stage_ms[order[i]] = ms[i];
   }
   return EB_OK;
+}
+
+extern "C" int eb_fmm_apply(eb_fmm_plan P, const double* dens, double* gradw, void* stream) {
+  return fmm_apply(P, dens, gradw, stream, false);
+}
+
+extern "C" int eb_fmm_apply_transpose(eb_fmm_plan P, const double* lam, double* mu, void* stream) {
+  return fmm_apply(P, lam, mu, stream, true);
 }
 
 extern "C" int eb_fmm_plan_rebuild(eb_fmm_plan P, const double* x, const double* y, const double* wdz,

@@ -134,6 +134,15 @@ int eb_fmm_plan_rebuild(eb_fmm_plan plan, const double *x, const double *y, cons
                         void *stream);
 /* gradw[i] = GW at point i for density dens (the CADAPDR result with IFLAG1 = 0). */
 int eb_fmm_apply(eb_fmm_plan plan, const double *dens, double *gradw, void *stream);
+/* mu = A^T lam: the transpose of the real-linear map dens -> gradw of eb_fmm_apply under the real
+ * inner product Re sum_i conj(a_i) b_i, so that Re <lam, A dens> = Re <A^T lam, dens>.  It is
+ * the same layer potential with sources and targets swapped (for the CADAP family
+ * mu_j = sum_{i != j} [conj(c_ij) lam_i + d_ij conj(lam_i)] with GW_i = sum_j [c_ij dens_j +
+ * d_ij conj(dens_j)]), evaluated by the plan's own passes (FMM or O(N^2) by its mode) on
+ * transposed charges.  No counterpart in the reference, which has no derivatives: it transposes
+ * CADAPDR / CADAPMOD / CADAP (cadapdrop.f:3-387) and the Hilbert product of DAPIF2
+ * (dapif1.f:540-558), for adjoint solves. */
+int eb_fmm_apply_transpose(eb_fmm_plan plan, const double *lam, double *mu, void *stream);
 int eb_fmm_plan_destroy(eb_fmm_plan plan);
 
 /* The O(N^2) evaluation of eb_fmm_apply in EB_MODE_DIRECT, in one call that allocates nothing:

@@ -178,6 +178,24 @@ def direct(variant, nd, x, y, dzdt, dens, h, targets=None):
     return gw.view(np.complex128)
 
 
+def direct_transpose(variant, x, y, wdz, lam, targets=None):
+    """The transpose (real inner product Re sum conj(a) b) of dens -> GW of ``direct`` with
+    weights wdz = DZDT * H per point (variant DR / MOD / PLUS), or of the Hilbert product
+    GW_i = sum_{j != i} wdz_j dens_j / (z_i - z_j) (variant 3), in long double, at the 0-based
+    targets (default all).  Pinned by the dense transposes of ``direct`` and ``cauchy``."""
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    w = _cplx(wdz)
+    l = _cplx(lam)
+    n = x.size
+    if targets is None:
+        targets = np.arange(n, dtype=np.int32)
+    targets = np.ascontiguousarray(targets, np.int32)
+    out = np.zeros(2 * targets.size)
+    lib().ora_direct_transpose(int(variant), n, _d(x), _d(y), _d(w), _d(l), targets.size, _i(targets), _d(out))
+    return out.view(np.complex128)
+
+
 def cauchy(x, y, qa, targets=None):
     """DAPIF2's Hilbert-matrix product (IFLAG7 = 3, dapif1.f:540-558) as an O(N^2) sum in long
     double: sum_{j != i} qa_j / (z_i - z_j) at the 0-based targets (default all)."""

@@ -398,17 +398,56 @@ typedef long double complex lcplx;
  * thread, so the result does not depend on the thread count. */
 typedef struct {
     const lcplx *z, *qa, *qb;
-    int n, ntgt, kind;       /* kind 0: cadap kernel with sign sgn; 1: Cauchy sum */
+    int n, ntgt, kind;       /* kind 0: cadap kernel with sign sgn; 1: Cauchy sum; 2, 3: transposes */
     long double sgn;
     const int *tgt;
     double *out;
     int t0, t1;
+    long double sqb;         /* kind 2 */
 } ora_sum_job;
+
+/* kind 2 (the CADAP family) and 3 (Hilbert): qa holds the weights wdz, qb the multipliers lam.
+ * The forward's coefficients of dens_j and conj(dens_j) in GW_i, with d = z_j - z_i (source j,
+ * target i), alpha = wdz/(2 pi i), gamma = conj(wdz)/(2 pi i):
+ *   c_ij = alpha_j / d + sgn sqb conj(alpha_j) / conj(d)
+ *   d_ij = sgn conj(gamma_j) / conj(d) - sgn conj(alpha_j) d / conj(d)^2
+ * (from qa = alpha dens, qb = sqb alpha conj(dens) + gamma dens in the closed form below), and
+ * for the Hilbert product c_ij = wdz_j / (z_i - z_j), d_ij = 0.  Output j sums over i. */
+static void ora_tsum_target(const ora_sum_job *J, int t)
+{
+    const long double pi = 3.14159265358979323846264338327950288L;
+    const lcplx z2pi = 1.0L / (2.0L * pi * I);
+    const int j = J->tgt[t];
+    const lcplx w = J->qa[j];
+    const lcplx al = w * z2pi, ga = conjl(w) * z2pi;
+    lcplx acc = 0;
+    for (int i = 0; i < J->n; ++i) {
+        if (i == j) continue;
+        const lcplx d = J->z[j] - J->z[i];
+        const lcplx lam = J->qb[i];
+        lcplx c, dc;
+        if (J->kind == 3) {
+            c = w / (J->z[i] - J->z[j]);
+            dc = 0;
+        } else {
+            const lcplx cd = conjl(d);
+            c = al / d + J->sgn * J->sqb * conjl(al) / cd;
+            dc = J->sgn * conjl(ga) / cd - J->sgn * conjl(al) * d / (cd * cd);
+        }
+        acc += conjl(c) * lam + dc * conjl(lam);
+    }
+    J->out[2 * t] = (double)creall(acc);
+    J->out[2 * t + 1] = (double)cimagl(acc);
+}
 
 static void *ora_sum_run(void *arg)
 {
     ora_sum_job *J = (ora_sum_job *)arg;
     for (int t = J->t0; t < J->t1; ++t) {
+        if (J->kind >= 2) {
+            ora_tsum_target(J, t);
+            continue;
+        }
         const int i = J->tgt[t];
         lcplx acc = 0;
         for (int s = 0; s < J->n; ++s) {
@@ -482,6 +521,22 @@ void ora_direct(int variant, int kbod, const int *nd, const double *x, const dou
     ora_sum_job J = {z, qa, qb, n, ntgt, 0, sgn, tgt, gradw, 0, 0};
     ora_sum_parallel(J);
     free(qa); free(qb); free(z);
+}
+
+void ora_direct_transpose(int variant, int n, const double *x, const double *y, const double *wdz,
+                          const double *lam, int ntgt, const int *tgt, double *mu)
+{
+    lcplx *z = (lcplx *)malloc(sizeof(lcplx) * n), *w = (lcplx *)malloc(sizeof(lcplx) * n);
+    lcplx *l = (lcplx *)malloc(sizeof(lcplx) * n);
+    for (int j = 0; j < n; ++j) {
+        z[j] = (long double)x[j] + I * (long double)y[j];
+        w[j] = (long double)wdz[2 * j] + I * (long double)wdz[2 * j + 1];
+        l[j] = (long double)lam[2 * j] + I * (long double)lam[2 * j + 1];
+    }
+    ora_sum_job J = {z, w, l, n, ntgt, variant == 3 ? 3 : 2, (variant == ORA_DR) ? -1.0L : 1.0L, tgt, mu, 0, 0,
+                     (variant == ORA_PLUS) ? 1.0L : -1.0L};
+    ora_sum_parallel(J);
+    free(z); free(w); free(l);
 }
 
 /* ------------------------------------------------------------------------------------- */

@@ -48,6 +48,15 @@ void ora_direct(int variant, int kbod, const int *nd, const double *x, const dou
                 const double *dzdt, const double *dens, const double *h,
                 int ntgt, const int *tgt, double *gradw);
 
+/* The transpose of the real-linear map dens -> GW of ora_direct (variant ORA_DR / ORA_MOD /
+ * ORA_PLUS) or of the Hilbert product GW_i = sum_{j != i} wdz_j dens_j / (z_i - z_j) (variant 3),
+ * under the real inner product Re sum conj(a) b: with GW_i = sum_j [c_ij dens_j + d_ij conj(dens_j)],
+ * mu[t] = sum_{i != j} [conj(c_ij) lam_i + d_ij conj(lam_i)] at j = tgt[t], long double.  wdz =
+ * DZDT * H per point.  The reference has no transposes: this is the checker of the build's
+ * adjoint products, pinned by the dense transpose of ora_direct / ora_cauchy. */
+void ora_direct_transpose(int variant, int n, const double *x, const double *y, const double *wdz,
+                          const double *lam, int ntgt, const int *tgt, double *mu);
+
 /* The Hilbert-matrix product of DAPIF2 with IFLAG7 = 3 (dapif1.f:540-558):
  * field[t] = sum_{j != tgt[t]} qa_j / (z_tgt[t] - z_j), long double, at the ntgt targets.
  * qa and field are interleaved complex. */

@@ -189,6 +189,49 @@ def test_direct_sum_is_independent_of_thread_count(monkeypatch):
     np.testing.assert_array_equal(outs[0][1], outs[1][1])
 
 
+def _real_matrix(apply, n):
+    """The 2n x 2n real matrix of a real-linear map of C^n, on the coordinates
+    (Re v_0..Re v_{n-1}, Im v_0..Im v_{n-1}): under them Re sum conj(a) b is the dot product,
+    so the transpose of the map is this matrix's transpose."""
+    cols = []
+    for u in (1.0, 1j):
+        for k in range(n):
+            e = np.zeros(n, complex)
+            e[k] = u
+            v = apply(e)
+            cols.append(np.concatenate([v.real, v.imag]))
+    return np.array(cols).T
+
+
+@pytest.mark.parametrize("variant", [oracle.DR, oracle.MOD, oracle.PLUS, 3])
+def test_direct_transpose_is_the_dense_transpose(variant):
+    """oracle.direct_transpose against the transposed real matrix of the forward sums
+    (oracle.direct for the CADAP family, oracle.cauchy with QA = WDZ dens for the Hilbert
+    product), built column by column on two small bodies; the untransposed matrix and a
+    conj(lam) slip both fail."""
+    rng = np.random.default_rng(11)
+    z, dz, nd, h = curves_cfg(2, 24, rng)
+    n = z.size
+    hp = np.repeat(h, nd)
+    wdz = dz * hp
+    if variant == 3:
+        fwd = lambda d: oracle.cauchy(z.real, z.imag, wdz * d)
+    else:
+        fwd = lambda d: oracle.direct(variant, nd, z.real, z.imag, dz, d, h)
+    M = _real_matrix(fwd, n)
+    lam = complex_normal(n, rng)
+    lr = np.concatenate([lam.real, lam.imag])
+    ref = M.T @ lr
+    ref = ref[:n] + 1j * ref[n:]
+    got = oracle.direct_transpose(variant, z.real, z.imag, wdz, lam)
+    assert rel(got, ref) < 1e-13
+    wrong = M @ lr
+    assert rel(got, wrong[:n] + 1j * wrong[n:]) > 1e-2          # not the map itself
+    assert rel(oracle.direct_transpose(variant, z.real, z.imag, wdz, np.conj(lam)), ref) > 1e-2
+    tgt = np.array([0, 7, n - 1], np.int32)
+    np.testing.assert_array_equal(oracle.direct_transpose(variant, z.real, z.imag, wdz, lam, tgt), got[tgt])
+
+
 def test_cadap_error_codes_match_reference():
     t = g("cadap_errors")
     for tag in ("tol",):
