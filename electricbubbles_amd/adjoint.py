@@ -40,7 +40,7 @@ from . import _check, _ptr, _stream_ptr, _torch, lib
 
 ADJ_EXPORTS = ("eb_adj_create", "eb_adj_destroy", "eb_adj_set_geometry", "eb_adj_laplace_apply",
                "eb_adj_stokes_apply", "eb_adj_laplace_solve", "eb_adj_stokes_solve", "eb_adj_cauchy",
-               "eb_adj_cauchy_vjp", "eb_adj_stokes_pairs", "eb_adj_stokes_pairs_vjp")
+               "eb_adj_cauchy_vjp", "eb_adj_stokes_pairs", "eb_adj_stokes_pairs_vjp", "eb_adj_set_mode")
 
 _vp = ctypes.c_void_p
 _i32 = ctypes.c_int32
@@ -56,6 +56,7 @@ def _bind():
         L.eb_adj_create.argtypes = [ctypes.POINTER(_vp), _i32, ctypes.POINTER(_i32), _i32, _vp]
         L.eb_adj_destroy.argtypes = [_vp]
         L.eb_adj_set_geometry.argtypes = [_vp] * 6
+        L.eb_adj_set_mode.argtypes = [_vp, _i32]
         L.eb_adj_laplace_apply.argtypes = [_vp, _i32, _vp, _vp]
         L.eb_adj_stokes_apply.argtypes = [_vp, _i32, _vp, _vp]
         solve = [_vp, _i32, _vp, _vp, _f64, ctypes.POINTER(_i32), ctypes.POINTER(_f64), ctypes.POINTER(_i32)]
@@ -100,6 +101,12 @@ class AdjointSystem:
 
     def _c(self, n):
         return self._torch.empty(n, dtype=self._torch.complex128, device="cuda")
+
+    def set_mode(self, mode):
+        """How the operators evaluate their layer potentials (eb_adj_set_mode): 0 auto (the
+        O(N^2) sums up to 10240 points, FMM plans above), 1 FMM, 2 the O(N^2) sums.  In FMM
+        mode the transposed operators run the transposed FMM product."""
+        _check(_bind().eb_adj_set_mode(self._h, int(mode)))
 
     def set_geometry(self, z, dz, s, kap, zk):
         self._geom = [t.detach().contiguous() for t in (z, dz, s, kap, zk)]
@@ -388,12 +395,13 @@ class SolveCache:
 class Problem:
     """Constants of F for k bodies of N points each, and the device operators."""
 
-    def __init__(self, k, N, tol=1e-12, maxl=100, device="cuda"):
+    def __init__(self, k, N, tol=1e-12, maxl=100, device="cuda", mode=0):
         torch = _torch()
         if N % 2 or N < 8:
             raise ValueError("bodies need an even number of points (at least 8)")
         self.k, self.N, self.tol = int(k), int(N), float(tol)
         self.sys = AdjointSystem([N] * k, maxl)
+        self.sys.set_mode(mode)
         n = N // 2
         i = np.arange(N)
         ks = np.where(i <= n, i, i - N).astype(np.float64)       # signed wavenumber, +n at Nyquist

@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "../../include/eb_adjoint.h"
+#include "../../include/eb_cadap.h"
 #include "../../include/eb_gmres.h"
 #include "eb_internal.h"
 
@@ -368,6 +369,16 @@ __global__ void k_st_out(int n, int transpose, const int* __restrict__ body_of, 
   y[i] = cadd(cadd3(v, za[b], gw[i]), cadd(self, last));
 }
 
+// coordinates for the FMM plans, and their unit weights
+__global__ void k_split_xy(int n, const double2* __restrict__ z, double* __restrict__ x, double* __restrict__ y,
+                           double2* __restrict__ ones) {
+  const int i = blockIdx.x * EWT + threadIdx.x;
+  if (i >= n) return;
+  x[i] = z[i].x;
+  y[i] = z[i].y;
+  ones[i] = c2(1.0, 0.0);
+}
+
 }  // namespace
 
 struct eb_adj_s {
@@ -385,6 +396,13 @@ struct eb_adj_s {
   double2* part = nullptr;
   size_t part_cap = 0;
   eb_gmres glap = nullptr, gst = nullptr;
+  // EB_MODE_FMM: the operators' layer potentials on FMM plans over the same points (the
+  // Stokes kernel on CADAPDR's, the Laplace Cauchy sum on the Hilbert product with unit
+  // weights), their transposes by eb_fmm_apply_transpose
+  int mode = EB_MODE_AUTO;
+  eb_fmm_plan pst = nullptr, plap = nullptr;
+  double *px = nullptr, *py = nullptr;
+  double2* ones = nullptr;
 };
 
 namespace {
@@ -404,8 +422,10 @@ void free_adj(eb_adj A) {
   if (A->st) (void)hipStreamSynchronize(A->st);
   if (A->glap) eb_gmres_destroy(A->glap);
   if (A->gst) eb_gmres_destroy(A->gst);
+  if (A->pst) eb_fmm_plan_destroy(A->pst);
+  if (A->plap) eb_fmm_plan_destroy(A->plap);
   void* p[] = {A->d_off, A->body_of, A->hb, A->z, A->dz, A->zk, A->wdz, A->zn2, A->e, A->alpha, A->gamma, A->s, A->kap,
-               A->lapd, A->c0, A->c, A->cf, A->qa, A->qb, A->gw, A->bs1, A->bs2, A->part};
+               A->lapd, A->c0, A->c, A->cf, A->qa, A->qb, A->gw, A->bs1, A->bs2, A->part, A->px, A->py, A->ones};
   for (void* q : p)
     if (q) (void)hipFree(q);
   delete A;
@@ -453,6 +473,43 @@ int run_pairs(eb_adj A, const Op& op, bool self, int no, int nl, double2* o0, do
 }
 
 inline int ewb(int n) { return (n + EWT - 1) / EWT; }
+
+// the operators run on FMM plans: EB_MODE_FMM, or EB_MODE_AUTO above EB_DIRECT_MAX points (the
+// drop-in layer's rule)
+bool use_fmm(eb_adj A) { return A->mode == EB_MODE_FMM || (A->mode == EB_MODE_AUTO && A->n > EB_DIRECT_MAX); }
+
+// LEVMAX of the plans: about 16..64 points per box of the finest uniform grid were the points
+// spread evenly (4^L >= n / 48), between 3 and 10
+int adj_levmax(int n) {
+  int L = 3;
+  while (L < 10 && ((int64_t)1 << (2 * L)) * 48 < (int64_t)n) ++L;
+  return L;
+}
+
+// the expansion length of the plans: CADAPMOD's TOL, 1e-13 (electric.f:803), -NINT(log2 TOL)
+constexpr int ADJ_NTERMS = 43;
+
+// builds or re-bins the plans for the current geometry (FMM mode only)
+int sync_plans(eb_adj A) {
+  if (!use_fmm(A) || !A->have_geom) return EB_OK;
+  const int n = A->n;
+  int rc;
+  if (!A->px) {
+    if ((rc = dalloc_a(&A->px, n)) || (rc = dalloc_a(&A->py, n)) || (rc = dalloc_a(&A->ones, n))) return rc;
+  }
+  hipLaunchKernelGGL(k_split_xy, dim3(ewb(n)), dim3(EWT), 0, A->st, n, A->z, A->px, A->py, A->ones);
+  EB_HIP_CHECK(hipGetLastError());
+  if (!A->pst) {
+    const int L = adj_levmax(n);
+    if ((rc = eb_fmm_plan_create(&A->pst, n, A->px, A->py, (const double*)A->wdz, L, ADJ_NTERMS, EB_VARIANT_DR,
+                                 EB_MODE_FMM, A->st)) != EB_OK)
+      return rc;
+    return eb_fmm_plan_create(&A->plap, n, A->px, A->py, (const double*)A->ones, L, ADJ_NTERMS, EB_VARIANT_HILBERT,
+                              EB_MODE_FMM, A->st);
+  }
+  if ((rc = eb_fmm_plan_rebuild(A->pst, A->px, A->py, (const double*)A->wdz, A->st)) != EB_OK) return rc;
+  return eb_fmm_plan_rebuild(A->plap, A->px, A->py, (const double*)A->ones, A->st);
+}
 
 int need_geom(eb_adj A, const char* what) {
   if (!A || !A->have_geom) {
@@ -536,7 +593,14 @@ extern "C" int eb_adj_set_geometry(eb_adj A, const double* z, const double* dz, 
                      A->wdz, A->lapd, A->c0, A->zn2, A->e, A->alpha, A->gamma);
   EB_HIP_CHECK(hipGetLastError());
   A->have_geom = true;
-  return EB_OK;
+  return sync_plans(A);
+}
+
+extern "C" int eb_adj_set_mode(eb_adj A, int32_t mode) {
+  if (!A || (mode != EB_MODE_AUTO && mode != EB_MODE_FMM && mode != EB_MODE_DIRECT))
+    return (eb_set_error("eb_adj_set_mode: bad arguments"), EB_ERR_ARG);
+  A->mode = mode;
+  return sync_plans(A);
 }
 
 extern "C" int eb_adj_laplace_apply(eb_adj A, int32_t transpose, const double* x, double* y) {
@@ -548,8 +612,13 @@ extern "C" int eb_adj_laplace_apply(eb_adj A, int32_t transpose, const double* x
     hipLaunchKernelGGL(k_real_c, dim3(ewb(n)), dim3(EWT), 0, A->st, n, x, A->c);
   else
     hipLaunchKernelGGL(k_lap_charges, dim3(ewb(n)), dim3(EWT), 0, A->st, n, x, A->wdz, A->c);
-  CauchyFwd op{A->z, A->z, A->c};
-  if ((rc = run_pairs(A, op, true, n, n, A->cf, nullptr, nullptr)) != EB_OK) return rc;
+  if (use_fmm(A)) {
+    // CF_i = sum_j c_j / (z_i - z_j): the Hilbert product with unit weights
+    if ((rc = eb_fmm_apply(A->plap, (const double*)A->c, (double*)A->cf, A->st)) != EB_OK) return rc;
+  } else {
+    CauchyFwd op{A->z, A->z, A->c};
+    if ((rc = run_pairs(A, op, true, n, n, A->cf, nullptr, nullptr)) != EB_OK) return rc;
+  }
   hipLaunchKernelGGL(k_lap_out, dim3(ewb(n)), dim3(EWT), 0, A->st, n, (int)transpose, x, A->lapd, A->wdz, A->cf, y);
   EB_HIP_CHECK(hipGetLastError());
   return EB_OK;
@@ -561,7 +630,12 @@ extern "C" int eb_adj_stokes_apply(eb_adj A, int32_t transpose, const double* x,
   if (!x || !y) return (eb_set_error("eb_adj_stokes_apply: bad arguments"), EB_ERR_ARG);
   const int n = A->n;
   const double2* sg = (const double2*)x;
-  if (transpose) {
+  if (use_fmm(A)) {
+    // GW and its transpose on the CADAPDR plan (the plan forms QA, QB from the density itself)
+    rc = transpose ? eb_fmm_apply_transpose(A->pst, x, (double*)A->gw, A->st)
+                   : eb_fmm_apply(A->pst, x, (double*)A->gw, A->st);
+    if (rc != EB_OK) return rc;
+  } else if (transpose) {
     StokesT op{A->z, sg, A->alpha, A->gamma, -1.0};
     if ((rc = run_pairs(A, op, true, n, n, A->gw, nullptr, nullptr)) != EB_OK) return rc;
   } else {

@@ -46,6 +46,14 @@ int eb_adj_destroy(eb_adj a);
 int eb_adj_set_geometry(eb_adj a, const double *z, const double *dz, const double *dsda, const double *kappa,
                         const double *zk);
 
+/* How the operators below evaluate their layer potentials: EB_MODE_DIRECT (the O(N^2) pair
+ * sums), EB_MODE_FMM (FMM plans over the same points: CADAPDR's for GW, the Hilbert product for
+ * CF; the transposed operators through eb_fmm_apply_transpose, the same sums with sources and
+ * targets swapped), EB_MODE_AUTO (default: direct up to EB_DIRECT_MAX points, FMM above).  The
+ * plans have LEVMAX with 4^LEVMAX >= n / 48 (3..10) and 43 terms (TOL 1e-13).  The pair sums
+ * and their reverses further down stay O(N^2) in every mode. */
+int eb_adj_set_mode(eb_adj a, int32_t mode);
+
 /* y = A x or y = A^T x (transpose = 1) for the Laplace (n doubles) and Stokes (2n doubles)
  * operators above. */
 int eb_adj_laplace_apply(eb_adj a, int32_t transpose, const double *x, double *y);

@@ -95,6 +95,54 @@ def test_transposes(torch, tag):
     assert rel(sys.laplace_apply(lam, transpose=True).cpu().numpy(), b.cpu().numpy()) < 1e-12
 
 
+@pytest.mark.parametrize("tag", ["two_drops", "four_circles"])
+def test_fmm_mode_operators(torch, tag):
+    """Operators on FMM plans (eb_adj_set_mode(FMM): the transposed ones through the transposed
+    FMM product) against the O(N^2) pair sums: both directions, the transposes by
+    <A x, y> = <x, A^T y>, and the transposed solves."""
+    f, nd, z = field_case(torch, tag)
+    dsys = system_on(torch, nd, z)[0]
+    fsys = system_on(torch, nd, z)[0]
+    fsys.set_mode(1)
+    rng = np.random.default_rng(8)
+    n = sum(nd)
+    for m, dap, fap in ((n, dsys.laplace_apply, fsys.laplace_apply), (2 * n, dsys.stokes_apply, fsys.stokes_apply)):
+        x = torch.tensor(rng.standard_normal(m), device="cuda")
+        y = torch.tensor(rng.standard_normal(m), device="cuda")
+        for tr in (False, True):
+            e = rel(fap(x, transpose=tr).cpu().numpy(), dap(x, transpose=tr).cpu().numpy())
+            print(f"{tag} m={m} transpose={tr}: FMM against pairs {e:.2e}", flush=True)
+            assert e < 1e-11
+        lhs = float(torch.dot(fap(x), y))
+        rhs_ = float(torch.dot(x, fap(y, transpose=True)))
+        assert abs(lhs - rhs_) < 1e-11 * float(x.norm() * y.norm()), (lhs, rhs_)
+    b = torch.tensor(rng.standard_normal(2 * n), device="cuda")
+    lam_f = fsys.stokes_solve(b, 1e-13, transpose=True)
+    lam_d = dsys.stokes_solve(b, 1e-13, transpose=True)
+    assert rel(lam_f.cpu().numpy(), lam_d.cpu().numpy()) < 1e-10
+
+
+def test_auto_mode_takes_the_fmm_above_the_direct_limit(torch):
+    """Eight drops of 2048 points (16384 > 10240): the default mode gives the FMM mode's bits,
+    within 1e-11 of the pair sums, in both directions."""
+    nd = [2048] * 8
+    th = 2 * np.pi * np.arange(2048) / 2048
+    z = np.concatenate([3 * (b % 4) + 3j * (b // 4) + (1.0 + 0.05 * np.cos(2 * th)) * np.exp(1j * th)
+                        for b in range(8)])
+    asys = system_on(torch, nd, z)[0]
+    fsys = system_on(torch, nd, z)[0]
+    fsys.set_mode(1)
+    dsys = system_on(torch, nd, z)[0]
+    dsys.set_mode(2)
+    x = torch.tensor(np.random.default_rng(2).standard_normal(2 * z.size), device="cuda")
+    for tr in (False, True):
+        a = asys.stokes_apply(x, transpose=tr).cpu().numpy()
+        np.testing.assert_array_equal(a, fsys.stokes_apply(x, transpose=tr).cpu().numpy())
+        e = rel(a, dsys.stokes_apply(x, transpose=tr).cpu().numpy())
+        print(f"16384 points, transpose={tr}: FMM against pairs {e:.2e}", flush=True)
+        assert e < 1e-11
+
+
 def _cplx(torch, rng, n, scale=1.0):
     return torch.tensor(scale * (rng.standard_normal(n) + 1j * rng.standard_normal(n)), device="cuda")
 
