@@ -413,6 +413,18 @@ def sweep_leg(torch, dev, with_cpu):
         gst.synchronize()
         t_run = time.perf_counter() - t0
         run_finite = bool(torch.isfinite(yg).all())
+        # the same run with classic RK4 (4 F per step): 200 replayed steps from the initial state
+        yk = rku_t.clone()
+        SFg.rk4_graph(yk, dt)  # capture
+        gst.synchronize()
+        redone_rk4 = 0
+        t0 = time.perf_counter()
+        for _ in range(nrun):
+            redone_rk4 += SFg.rk4_graph(yk, dt)
+        gst.synchronize()
+        t_rk4 = time.perf_counter() - t0
+        rk4_finite = bool(torch.isfinite(yk).all())
+        d_rk4 = Bg.deformation(yk).cpu().numpy()
     torch.cuda.synchronize()
     out["rk2_step"] = {"what": "one Heun step (2 F) of all simulations", "dt": dt, "eager_ms": round(t_eager * 1e3, 3),
                        "graph_ms": round(t_graph * 1e3, 3), "graph_budget": 8, "graph_steps_redone": int(redone),
@@ -422,6 +434,14 @@ def sweep_leg(torch, dev, with_cpu):
                        "run": {"steps": nrun, "s": round(t_run, 3), "steps_per_s": round(nrun / t_run, 1),
                                "simulation_steps_per_s": round(nsim * nrun / t_run, 1),
                                "steps_redone": int(redone_run), "state_finite": run_finite}}
+    out["rk4_run"] = {"what": "201 replayed classic RK4 steps (4 F each, eb_sweep_field_rk4_graph) of all "
+                              "simulations at dt = 1e-3: the capture's step and the 200 timed",
+                      "steps": nrun, "s": round(t_rk4, 3), "ms_per_step": round(t_rk4 * 1e3 / nrun, 3),
+                      "simulation_steps_per_s": round(nsim * nrun / t_rk4, 1),
+                      "F_per_s": round(4 * nsim * nrun / t_rk4, 1), "steps_redone": int(redone_rk4),
+                      "state_finite": rk4_finite,
+                      "deformation_after": {"min": round(float(d_rk4.min()), 6), "median": round(float(np.median(d_rk4)), 6),
+                                            "max": round(float(d_rk4.max()), 6)}}
     SFg.close()
     # the plain path at its best: one simulation per field, its Heun step captured in a hipGraph
     # with device-controlled solves of the one system (eb_field_rk2_graph), same budget
@@ -491,6 +511,60 @@ def sweep_leg(torch, dev, with_cpu):
             out["F"]["cpu_reference"] = {"F_per_s": round(n_rf / t_rf, 2), "cores": 1, "kind": "reference",
                                          "sample": f"the reference's F of {n_rf} of the sampled simulations"}
     gb.close()
+    return out
+
+
+def single_drop_leg(torch, dev, with_cpu):
+    """The single drop r = 1 + 0.05 cos 2 theta, N = 256, |E| = 1 (E0 = 1), no straining flow:
+    100 classic RK4 steps at dt = 1e-3 as replayed hipGraphs (eb_field_rk4_graph, 4 F per step)
+    with the deformation D = (L - B) / (L + B) on the device after every step (eb_deformation).
+    The reference leg times the reference's own Heun step (it has no RK4) on one host core."""
+    from electricbubbles_amd import field, spectral
+    from electricbubbles_amd.workloads import perturbed_circle
+    z, _, nd, _ = perturbed_circle(256, [(2, 0.05)])
+    y0 = z.view(np.float64).copy()
+    st = torch.cuda.Stream()
+    B = spectral.Bodies(nd, stream=st)
+    F = field.Field(B, cap=0.0, e0=1.0, stream=st)
+    nstep, dt = 100, 1e-3
+    with torch.cuda.stream(st):
+        y = torch.tensor(y0, device=dev)
+        F.rk4_graph(y.clone(), dt)  # plans, capture (on a copy)
+        st.synchronize()
+        y = torch.tensor(y0, device=dev)
+        F.rk4_graph(y, dt)  # capture again for this buffer, first step
+        D = [B.deformation(torch.tensor(y0, device=dev)), B.deformation(y)]
+        st.synchronize()
+        redone = 0
+        t0 = time.perf_counter()
+        for _ in range(nstep - 1):
+            redone += F.rk4_graph(y, dt)
+            D.append(B.deformation(y))
+        st.synchronize()
+        t_run = time.perf_counter() - t0
+    Ds = torch.cat(D).cpu().numpy()
+    out = {"workload": "one drop r = 1 + 0.05 cos 2 theta, N = 256, E0 = 1, CAP = 0, RK4 at dt = 1e-3",
+           "steps": nstep, "ms_per_step": round(t_run * 1e3 / (nstep - 1), 3),
+           "timing": f"{nstep - 1} replayed steps with D after each (the first step captures the graph)",
+           "steps_redone": int(redone), "D_every_10_steps": [round(float(v), 12) for v in Ds[::10]],
+           "D_final": float(Ds[-1])}
+    F.close()
+    if with_cpu:
+        import tempfile
+        import oracle
+        if oracle.ref_available():
+            cwd = os.getcwd()
+            os.chdir(tempfile.mkdtemp())
+            try:
+                sim = oracle.RefSim(oracle.LIBREF)
+                sim.setup(nd, cap=0.0, e0=1.0)
+                t0 = time.perf_counter()
+                sim.rk2(y0, dt, 10)
+                t_ref = (time.perf_counter() - t0) / 10
+            finally:
+                os.chdir(cwd)
+            out["cpu_reference"] = {"ms_per_heun_step": round(t_ref * 1e3, 2), "cores": 1, "kind": "reference",
+                                    "sample": "10 steps of the reference's RK2 (2 F each) on this drop"}
     return out
 
 
@@ -662,9 +736,10 @@ def main():
     ap.add_argument("--no-sweep", action="store_true", help="skip the batched-versus-plain sweep leg")
     ap.add_argument("--no-field", action="store_true", help="skip the F and RK2 step leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-gradient leg")
+    ap.add_argument("--no-single", action="store_true", help="skip the single-drop RK4 leg")
     ap.add_argument("--no-autotune", action="store_true", help="skip the FMM autotuner")
     ap.add_argument("--adjoint-steps", type=int, default=200, help="RK4 steps of the adjoint leg")
-    ap.add_argument("--only", choices=["solve", "spectral", "sweep", "field", "adjoint"],
+    ap.add_argument("--only", choices=["solve", "spectral", "sweep", "field", "adjoint", "single"],
                     help="run one secondary leg alone and print its JSON object (for profiling runs)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 256 MiB device copy first (the known-size kernel that scales HBM counters)")
@@ -689,6 +764,7 @@ def main():
     dev = torch.device("cuda:0")
     if args.only:
         legs = {"solve": stokes_solve_leg, "spectral": spectral_leg, "sweep": sweep_leg, "field": field_leg,
+                "single": single_drop_leg,
                 "adjoint": lambda t, d, c: adjoint_leg(t, d, c, args.adjoint_steps)}
         res = {"leg": args.only, args.only: legs[args.only](torch, dev, not args.no_cpu)}
         sys.stdout.flush()
@@ -714,6 +790,7 @@ def main():
     spec = None if args.no_spectral else leg("spectral", spectral_leg, torch, dev, not args.no_cpu)
     sweep = None if args.no_sweep else leg("sweep", sweep_leg, torch, dev, not args.no_cpu)
     fld = None if args.no_field else leg("field", field_leg, torch, dev, not args.no_cpu)
+    single = None if args.no_single else leg("single drop", single_drop_leg, torch, dev, not args.no_cpu)
     adj = None if args.no_adjoint else leg("adjoint", adjoint_leg, torch, dev, not args.no_cpu, args.adjoint_steps)
 
     cpu = None
@@ -761,6 +838,7 @@ def main():
         "spectral": spec,
         "sweep": sweep,
         "field": fld,
+        "single_drop": single,
         "adjoint": adj,
         "roofline": {"bound": "mfma", "kernel": dom, "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": traffic,

@@ -363,7 +363,7 @@ struct eb_field_s {
   hipGraphExec_t gexec = nullptr;
   double* g_ybar = nullptr;
   double g_dt = 0.0;
-  int g_budget = 0;
+  int g_budget = 0, g_order = 0;
   double *ysave = nullptr, *yst = nullptr, *phi0 = nullptr, *dy = nullptr;
 };
 
@@ -677,10 +677,29 @@ static int field_rk2_step(eb_field F, double* ybar, double dt, int budget) {
   return eb_rk2_stage2(neq, dt, F->phi0, F->dy, ybar, s);
 }
 
-extern "C" int eb_field_rk2_graph(eb_field F, double* ybar, double dt, int32_t budget, int32_t* redone) {
-  if (!F || !ybar || budget < 1) return (eb_set_error("eb_field_rk2_graph: bad arguments"), EB_ERR_ARG);
+// One classic RK4 step on the same buffers (phi0 holds the slopes' weighted sum, yst the stage
+// states): four evaluations of F, each followed by its stage update.
+static int field_rk4_step(eb_field F, double* ybar, double dt, int budget) {
+  const int neq = 2 * F->n;
+  void* s = (void*)F->st;
+  int rc;
+  for (int q = 0; q < 4; ++q) {
+    if ((rc = field_rhs(F, q == 0 ? ybar : F->yst, F->dy, budget)) != EB_OK) return rc;
+    if ((rc = eb_rk4_stage(neq, q, dt, ybar, F->dy, F->phi0, F->yst, s)) != EB_OK) return rc;
+  }
+  return EB_OK;
+}
+
+static int field_step(eb_field F, double* ybar, double dt, int budget, int order) {
+  return order == 4 ? field_rk4_step(F, ybar, dt, budget) : field_rk2_step(F, ybar, dt, budget);
+}
+
+// eb_field_rk2_graph (order 2) and eb_field_rk4_graph (order 4)
+static int field_step_graph(eb_field F, double* ybar, double dt, int32_t budget, int32_t* redone, int order,
+                            const char* name) {
+  if (!F || !ybar || budget < 1) return (eb_set_error("%s: bad arguments", name), EB_ERR_ARG);
   if (F->st == nullptr)
-    return (eb_set_error("eb_field_rk2_graph: the field must use a non-default stream (graph capture)"), EB_ERR_ARG);
+    return (eb_set_error("%s: the field must use a non-default stream (graph capture)", name), EB_ERR_ARG);
   const size_t neq = 2 * (size_t)F->n;
   int rc;
   if (!F->ysave) {
@@ -697,7 +716,7 @@ extern "C" int eb_field_rk2_graph(eb_field F, double* ybar, double dt, int32_t b
   }
   // the plans are made (with synchronisations) by an eager F before the first capture
   if ((!F->have_geom || !F->pv) && (rc = field_rhs(F, ybar, F->dy, 0)) != EB_OK) return rc;
-  if (!F->gexec || F->g_ybar != ybar || F->g_dt != dt || F->g_budget != budget) {
+  if (!F->gexec || F->g_ybar != ybar || F->g_dt != dt || F->g_budget != budget || F->g_order != order) {
     drop_graph(F);
     EB_HIP_CHECK(hipStreamSynchronize(F->st));
     hipGraph_t graph = nullptr;
@@ -707,22 +726,23 @@ extern "C" int eb_field_rk2_graph(eb_field F, double* ybar, double dt, int32_t b
                   : EB_ERR_HIP;
     if (crc == EB_OK) crc = eb_gmres_batch_clear_fail(F->glap1);
     if (crc == EB_OK) crc = eb_gmres_batch_clear_fail(F->gst1);
-    if (crc == EB_OK) crc = field_rk2_step(F, ybar, dt, budget);
+    if (crc == EB_OK) crc = field_step(F, ybar, dt, budget, order);
     const hipError_t e = hipStreamEndCapture(F->st, &graph);
     if (crc != EB_OK || e != hipSuccess) {
       if (graph) (void)hipGraphDestroy(graph);
-      if (crc == EB_OK) eb_set_error("eb_field_rk2_graph: capture failed: %s", hipGetErrorString(e));
+      if (crc == EB_OK) eb_set_error("%s: capture failed: %s", name, hipGetErrorString(e));
       return crc != EB_OK ? crc : EB_ERR_HIP;
     }
     const hipError_t ie = hipGraphInstantiate(&F->gexec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     if (ie != hipSuccess) {
       F->gexec = nullptr;
-      return (eb_set_error("eb_field_rk2_graph: instantiate failed: %s", hipGetErrorString(ie)), EB_ERR_HIP);
+      return (eb_set_error("%s: instantiate failed: %s", name, hipGetErrorString(ie)), EB_ERR_HIP);
     }
     F->g_ybar = ybar;
     F->g_dt = dt;
     F->g_budget = budget;
+    F->g_order = order;
   }
   EB_HIP_CHECK(hipGraphLaunch(F->gexec, F->st));
   int32_t fl = 0, fs = 0;
@@ -734,11 +754,19 @@ extern "C" int eb_field_rk2_graph(eb_field F, double* ybar, double dt, int32_t b
     // a solve needed more than `budget` Arnoldi steps: the step again from the saved state
     // with the host-controlled solves
     EB_HIP_CHECK(hipMemcpyAsync(ybar, F->ysave, sizeof(double) * neq, hipMemcpyDeviceToDevice, F->st));
-    if ((rc = field_rk2_step(F, ybar, dt, 0)) != EB_OK) return rc;
+    if ((rc = field_step(F, ybar, dt, 0, order)) != EB_OK) return rc;
     EB_HIP_CHECK(hipStreamSynchronize(F->st));
     if (redone) *redone = 1;
   }
   return EB_OK;
+}
+
+extern "C" int eb_field_rk2_graph(eb_field F, double* ybar, double dt, int32_t budget, int32_t* redone) {
+  return field_step_graph(F, ybar, dt, budget, redone, 2, "eb_field_rk2_graph");
+}
+
+extern "C" int eb_field_rk4_graph(eb_field F, double* ybar, double dt, int32_t budget, int32_t* redone) {
+  return field_step_graph(F, ybar, dt, budget, redone, 4, "eb_field_rk4_graph");
 }
 
 extern "C" int eb_field_iterations(eb_field F, int32_t* lap_iter, int32_t* lap_ierr, int32_t* stokes_iter,
@@ -847,7 +875,7 @@ struct eb_sweep_field_s {
   hipGraphExec_t gexec = nullptr;
   double* g_ybar = nullptr;
   double g_dt = 0.0;
-  int g_budget = 0;
+  int g_budget = 0, g_order = 0;
   double *ysave = nullptr, *yst = nullptr, *phi0 = nullptr, *dy = nullptr;
 };
 
@@ -1047,11 +1075,27 @@ static int sweep_rk2_step(eb_sweep_field F, double* ybar, double dt, int budget)
   return eb_rk2_stage2(neq, dt, F->phi0, F->dy, ybar, s);
 }
 
-extern "C" int eb_sweep_field_rk2_graph(eb_sweep_field F, double* ybar, double dt, int32_t budget, int32_t* redone) {
-  if (!F || !ybar || budget < 1) return (eb_set_error("eb_sweep_field_rk2_graph: bad arguments"), EB_ERR_ARG);
+// One classic RK4 step of every simulation (phi0: the slopes' weighted sum, yst: the stage states)
+static int sweep_rk4_step(eb_sweep_field F, double* ybar, double dt, int budget) {
+  const int neq = 2 * F->n;
+  void* s = (void*)F->st;
+  int rc;
+  for (int q = 0; q < 4; ++q) {
+    if ((rc = sweep_rhs(F, q == 0 ? ybar : F->yst, F->dy, budget)) != EB_OK) return rc;
+    if ((rc = eb_rk4_stage(neq, q, dt, ybar, F->dy, F->phi0, F->yst, s)) != EB_OK) return rc;
+  }
+  return EB_OK;
+}
+
+static int sweep_step(eb_sweep_field F, double* ybar, double dt, int budget, int order) {
+  return order == 4 ? sweep_rk4_step(F, ybar, dt, budget) : sweep_rk2_step(F, ybar, dt, budget);
+}
+
+static int sweep_step_graph(eb_sweep_field F, double* ybar, double dt, int32_t budget, int32_t* redone, int order,
+                            const char* name) {
+  if (!F || !ybar || budget < 1) return (eb_set_error("%s: bad arguments", name), EB_ERR_ARG);
   if (F->st == nullptr)
-    return (eb_set_error("eb_sweep_field_rk2_graph: the field must use a non-default stream (graph capture)"),
-            EB_ERR_ARG);
+    return (eb_set_error("%s: the field must use a non-default stream (graph capture)", name), EB_ERR_ARG);
   const size_t neq = 2 * (size_t)F->n;
   int rc;
   if (!F->ysave) {
@@ -1060,7 +1104,7 @@ extern "C" int eb_sweep_field_rk2_graph(eb_sweep_field F, double* ybar, double d
       return rc;
   }
   if (!F->have_geom && (rc = sweep_rhs(F, ybar, F->dy, 0)) != EB_OK) return rc;  // creates the sweep operator
-  if (!F->gexec || F->g_ybar != ybar || F->g_dt != dt || F->g_budget != budget) {
+  if (!F->gexec || F->g_ybar != ybar || F->g_dt != dt || F->g_budget != budget || F->g_order != order) {
     if (F->gexec) {
       EB_HIP_CHECK(hipGraphExecDestroy(F->gexec));
       F->gexec = nullptr;
@@ -1073,22 +1117,23 @@ extern "C" int eb_sweep_field_rk2_graph(eb_sweep_field F, double* ybar, double d
                   : EB_ERR_HIP;
     if (crc == EB_OK) crc = eb_gmres_batch_clear_fail(F->glap);
     if (crc == EB_OK) crc = eb_gmres_batch_clear_fail(F->gst);
-    if (crc == EB_OK) crc = sweep_rk2_step(F, ybar, dt, budget);
+    if (crc == EB_OK) crc = sweep_step(F, ybar, dt, budget, order);
     const hipError_t e = hipStreamEndCapture(F->st, &graph);
     if (crc != EB_OK || e != hipSuccess) {
       if (graph) (void)hipGraphDestroy(graph);
-      if (crc == EB_OK) eb_set_error("eb_sweep_field_rk2_graph: capture failed: %s", hipGetErrorString(e));
+      if (crc == EB_OK) eb_set_error("%s: capture failed: %s", name, hipGetErrorString(e));
       return crc != EB_OK ? crc : EB_ERR_HIP;
     }
     const hipError_t ie = hipGraphInstantiate(&F->gexec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     if (ie != hipSuccess) {
       F->gexec = nullptr;
-      return (eb_set_error("eb_sweep_field_rk2_graph: instantiate failed: %s", hipGetErrorString(ie)), EB_ERR_HIP);
+      return (eb_set_error("%s: instantiate failed: %s", name, hipGetErrorString(ie)), EB_ERR_HIP);
     }
     F->g_ybar = ybar;
     F->g_dt = dt;
     F->g_budget = budget;
+    F->g_order = order;
   }
   EB_HIP_CHECK(hipGraphLaunch(F->gexec, F->st));
   int32_t fl = 0, fs = 0;
@@ -1100,11 +1145,19 @@ extern "C" int eb_sweep_field_rk2_graph(eb_sweep_field F, double* ybar, double d
     // some system needed more than `budget` Arnoldi steps: the step again from the saved state
     // with the host-controlled solves
     EB_HIP_CHECK(hipMemcpyAsync(ybar, F->ysave, sizeof(double) * neq, hipMemcpyDeviceToDevice, F->st));
-    if ((rc = sweep_rk2_step(F, ybar, dt, 0)) != EB_OK) return rc;
+    if ((rc = sweep_step(F, ybar, dt, 0, order)) != EB_OK) return rc;
     EB_HIP_CHECK(hipStreamSynchronize(F->st));
     if (redone) *redone = 1;
   }
   return EB_OK;
+}
+
+extern "C" int eb_sweep_field_rk2_graph(eb_sweep_field F, double* ybar, double dt, int32_t budget, int32_t* redone) {
+  return sweep_step_graph(F, ybar, dt, budget, redone, 2, "eb_sweep_field_rk2_graph");
+}
+
+extern "C" int eb_sweep_field_rk4_graph(eb_sweep_field F, double* ybar, double dt, int32_t budget, int32_t* redone) {
+  return sweep_step_graph(F, ybar, dt, budget, redone, 4, "eb_sweep_field_rk4_graph");
 }
 
 extern "C" int eb_sweep_field_iterations(eb_sweep_field F, int32_t* lap_iter, int32_t* lap_ierr, int32_t* stokes_iter,

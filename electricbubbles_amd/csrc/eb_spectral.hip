@@ -263,6 +263,91 @@ __global__ void k_rk2_2(int neq, double dt, const double* __restrict__ phi0, con
   ybar[i] = ybar[i] + 0.5 * phi0[i] + 0.5 * p1;
 }
 
+// Classic fourth-order Runge-Kutta over neq doubles in four stage updates, each after one
+// evaluation dy = F(stage state): acc sums the weighted slopes, ys is the next stage's state.
+//   stage 0: acc = dy,       ys = y + (dt/2) dy
+//   stage 1: acc += 2 dy,    ys = y + (dt/2) dy
+//   stage 2: acc += 2 dy,    ys = y + dt dy
+//   stage 3: y += (dt/6) (acc + dy)
+__global__ void k_rk4(int neq, int stage, double dt, double* __restrict__ y, const double* __restrict__ dy,
+                      double* __restrict__ acc, double* __restrict__ ys) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= neq) return;
+  const double d = dy[i];
+  switch (stage) {
+    case 0:
+      acc[i] = d;
+      ys[i] = y[i] + (0.5 * dt) * d;
+      break;
+    case 1:
+    case 2:
+      acc[i] = acc[i] + 2.0 * d;
+      ys[i] = y[i] + (stage == 1 ? 0.5 * dt : dt) * d;
+      break;
+    default:
+      y[i] = y[i] + (dt / 6.0) * (acc[i] + d);
+      break;
+  }
+}
+
+// The deformation D = (L - B) / (L + B) of every body: L and B are the semi-axes of the ellipse
+// with the body's area and second moments of area, from trapezoid sums of Green's-theorem
+// integrands with the spectral tangent (FDIFFF's wavenumbers), about the mean of the body's
+// points (the central moments do not depend on the origin; taking it near the body keeps the
+// subtraction A c^2 from cancelling).  One workgroup per body.
+__global__ __launch_bounds__(SP_THREADS) void k_deformation(BodyView B, const double2* __restrict__ z,
+                                                            double* __restrict__ D) {
+  extern __shared__ double2 lds[];
+  __shared__ double red[SP_THREADS];
+  const BodyCtx bc = body_ctx(B);
+  const int N = bc.N;
+  double2* a = lds;                // the tangent
+  double2* zz = lds + B.nmax;      // the points, about their mean
+  double2* ws = lds + 2 * B.nmax;
+  double sx = 0.0, sy = 0.0;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    const double2 v = z[bc.o0 + i];
+    a[i] = v;
+    sx += v.x;
+    sy += v.y;
+  }
+  const double x0 = block_sum(sx, red) / N, y0 = block_sum(sy, red) / N;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) zz[i] = make_double2(a[i].x - x0, a[i].y - y0);
+  __syncthreads();
+  fft(a, ws, bc, B, -1);
+  const double invn = 1.0 / N;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    const double w = fdiff_k(i, N) * invn;
+    const double2 v = a[i];
+    a[i] = make_double2(-w * v.y, w * v.x);
+  }
+  __syncthreads();
+  fft(a, ws, bc, B, +1);
+  double s[6] = {0, 0, 0, 0, 0, 0};
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    const double x = zz[i].x, y = zz[i].y, xa = a[i].x, ya = a[i].y;
+    s[0] += x * ya - y * xa;  // 2 A
+    s[1] += x * x * ya;       // 2 mx
+    s[2] += y * y * xa;       // -2 my
+    s[3] += x * x * x * ya;   // 3 ixx
+    s[4] += y * y * y * xa;   // -3 iyy
+    s[5] += x * x * y * ya;   // 2 ixy
+  }
+  double t[6];
+  for (int q = 0; q < 6; ++q) t[q] = block_sum(s[q], red);
+  if (threadIdx.x == 0) {
+    const double h = 2.0 * 3.14159265358979323846 / N;
+    const double A = 0.5 * h * t[0], mx = 0.5 * h * t[1], my = -0.5 * h * t[2];
+    const double ixx = h * t[3] / 3.0, iyy = -h * t[4] / 3.0, ixy = 0.5 * h * t[5];
+    const double cx = mx / A, cy = my / A;
+    const double p = ixx - A * cx * cx, c = iyy - A * cy * cy, b = ixy - A * cx * cy;
+    const double rt = sqrt(0.25 * (p - c) * (p - c) + b * b);
+    const double sp = sqrt(0.5 * (p + c) + rt), sm = sqrt(0.5 * (p + c) - rt);
+    D[bc.k] = (sp - sm) / (sp + sm);
+  }
+}
+
 // ------------------------------------------------------------------------------------------
 // FASMVP epilogue (stokesbubb.f:527-549, 585-611)
 // ------------------------------------------------------------------------------------------
@@ -503,6 +588,24 @@ extern "C" int eb_rk2_stage2(int32_t neq, double dt, const double* phi0, const d
                              void* stream) {
   if (neq < 1) return EB_OK;
   hipLaunchKernelGGL(k_rk2_2, dim3((neq + 255) / 256), dim3(256), 0, (hipStream_t)stream, neq, dt, phi0, dy, ybar);
+  EB_HIP_CHECK(hipGetLastError());
+  return EB_OK;
+}
+
+extern "C" int eb_rk4_stage(int32_t neq, int32_t stage, double dt, double* y, const double* dy, double* acc, double* ys,
+                            void* stream) {
+  if (stage < 0 || stage > 3 || !y || !dy || !acc || (stage < 3 && !ys))
+    return (eb_set_error("eb_rk4_stage: bad arguments"), EB_ERR_ARG);
+  if (neq < 1) return EB_OK;
+  hipLaunchKernelGGL(k_rk4, dim3((neq + 255) / 256), dim3(256), 0, (hipStream_t)stream, neq, stage, dt, y, dy, acc, ys);
+  EB_HIP_CHECK(hipGetLastError());
+  return EB_OK;
+}
+
+extern "C" int eb_deformation(eb_bodies B, const double* rku, double* D, void* stream) {
+  if (!B || !rku || !D) return (eb_set_error("eb_deformation: bad arguments"), EB_ERR_ARG);
+  hipLaunchKernelGGL(k_deformation, dim3(B->nbody), dim3(SP_THREADS), lds_bytes(B, 2), (hipStream_t)stream, view(B),
+                     (const double2*)rku, D);
   EB_HIP_CHECK(hipGetLastError());
   return EB_OK;
 }

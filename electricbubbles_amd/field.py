@@ -16,7 +16,7 @@ FIELD_EXPORTS = ("eb_field_create", "eb_field_destroy", "eb_field_set_mode", "eb
                  "eb_field_laplace_apply", "eb_field_laplace_solve", "eb_field_dirich_neu", "eb_field_setvelf",
                  "eb_field_stokes_solve", "eb_field_interface", "eb_field_get_geometry", "eb_field_iterations", "eb_sweep_field_create",
                  "eb_sweep_field_rhs", "eb_sweep_field_iterations", "eb_sweep_field_destroy",
-                 "eb_sweep_field_rk2_graph", "eb_field_rk2_graph")
+                 "eb_sweep_field_rk2_graph", "eb_field_rk2_graph", "eb_field_rk4_graph", "eb_sweep_field_rk4_graph")
 
 _vp = ctypes.c_void_p
 _i32 = ctypes.c_int32
@@ -45,6 +45,8 @@ def _bind():
         L.eb_sweep_field_destroy.argtypes = [_vp]
         L.eb_sweep_field_rk2_graph.argtypes = [_vp, _vp, ctypes.c_double, _i32, ctypes.POINTER(_i32)]
         L.eb_field_rk2_graph.argtypes = [_vp, _vp, ctypes.c_double, _i32, ctypes.POINTER(_i32)]
+        L.eb_field_rk4_graph.argtypes = [_vp, _vp, ctypes.c_double, _i32, ctypes.POINTER(_i32)]
+        L.eb_sweep_field_rk4_graph.argtypes = [_vp, _vp, ctypes.c_double, _i32, ctypes.POINTER(_i32)]
         _bound = True
     return L
 
@@ -132,6 +134,14 @@ class Field:
         _check(_bind().eb_field_rk2_graph(self._h, _ptr(ybar), float(dt), int(budget), ctypes.byref(red)))
         return bool(red.value)
 
+    def rk4_graph(self, ybar, dt, budget=8):
+        """One classic RK4 step in place on ybar, captured and replayed as rk2_graph
+        (eb_field_rk4_graph); returns True when the step was redone with host-controlled solves."""
+        assert ybar.is_contiguous() and ybar.dtype == self._torch.float64
+        red = _i32()
+        _check(_bind().eb_field_rk4_graph(self._h, _ptr(ybar), float(dt), int(budget), ctypes.byref(red)))
+        return bool(red.value)
+
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
             _bind().eb_field_destroy(self._h)
@@ -180,6 +190,14 @@ class SweepField:
         assert ybar.is_contiguous() and ybar.dtype == self._torch.float64
         red = _i32()
         _check(_bind().eb_sweep_field_rk2_graph(self._h, _ptr(ybar), float(dt), int(budget), ctypes.byref(red)))
+        return bool(red.value)
+
+    def rk4_graph(self, ybar, dt, budget=8):
+        """One classic RK4 step of every simulation in place on ybar, captured and replayed as
+        rk2_graph (eb_sweep_field_rk4_graph)."""
+        assert ybar.is_contiguous() and ybar.dtype == self._torch.float64
+        red = _i32()
+        _check(_bind().eb_sweep_field_rk4_graph(self._h, _ptr(ybar), float(dt), int(budget), ctypes.byref(red)))
         return bool(red.value)
 
     def close(self):

@@ -20,7 +20,7 @@ SPECTRAL_EXPORTS = (
     "eb_bodies_create", "eb_bodies_destroy", "eb_bodies_npoints", "eb_dcfft", "eb_fdifff", "eb_filter",
     "eb_equit", "eb_extractdat", "eb_ftail", "eb_rk2_stage1", "eb_rk2_stage2", "eb_matvec_create",
     "eb_matvec_apply", "eb_matvec_destroy", "eb_fasmvp", "eb_sweep_create", "eb_sweep_apply", "eb_sweep_parts",
-    "eb_sweep_unknown_offsets", "eb_sweep_destroy",
+    "eb_sweep_unknown_offsets", "eb_sweep_destroy", "eb_rk4_stage", "eb_deformation",
 )
 
 _bound = False
@@ -42,6 +42,8 @@ def _bind():
     L.eb_ftail.argtypes = [_vp] * 7
     L.eb_rk2_stage1.argtypes = [_i32, ctypes.c_double, _vp, _vp, _vp, _vp, _vp]
     L.eb_rk2_stage2.argtypes = [_i32, ctypes.c_double, _vp, _vp, _vp, _vp]
+    L.eb_rk4_stage.argtypes = [_i32, _i32, ctypes.c_double, _vp, _vp, _vp, _vp, _vp]
+    L.eb_deformation.argtypes = [_vp, _vp, _vp, _vp]
     L.eb_matvec_create.argtypes = [ctypes.POINTER(_vp), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _i32, _i32, _vp]
     L.eb_matvec_apply.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp]
     L.eb_matvec_destroy.argtypes = [_vp]
@@ -84,6 +86,12 @@ class Bodies:
         out = c.clone()
         _check(_bind().eb_dcfft(self._h, _ptr(out), sign, self._s()))
         return out
+
+    def deformation(self, rku):
+        """D = (L - B) / (L + B) of every body of the state rku (eb_deformation)."""
+        D = self._torch.empty(self.nbody, dtype=self._torch.float64, device="cuda")
+        _check(_bind().eb_deformation(self._h, _ptr(rku.contiguous()), _ptr(D), self._s()))
+        return D
 
     def fdifff(self, f):
         out = self._torch.empty_like(f)
@@ -144,6 +152,14 @@ def rk2_stage2(dt, phi0, dy, ybar, stream=None):
     """In place on ybar."""
     _check(_bind().eb_rk2_stage2(ybar.numel(), dt, _ptr(phi0), _ptr(dy), _ptr(ybar), _stream_ptr(stream)))
     return ybar
+
+
+def rk4_stage(stage, dt, y, dy, acc, ys, stream=None):
+    """Stage update `stage` (0..3) of classic RK4 (eb_rk4_stage): acc and ys are the caller's
+    buffers; stage 3 updates y in place."""
+    _check(_bind().eb_rk4_stage(y.numel(), int(stage), float(dt), _ptr(y), _ptr(dy), _ptr(acc), _ptr(ys),
+                                _stream_ptr(stream)))
+    return y
 
 
 class MatVec:

@@ -54,6 +54,10 @@ int eb_field_rhs(eb_field f, const double *rku, double *rkup);
  * budget change, or after eb_field_set_mode.  The field must have been created on a
  * non-default stream.  No counterpart in the reference (its solver stops on the host). */
 int eb_field_rk2_graph(eb_field f, double *ybar, double dt, int32_t budget, int32_t *redone);
+/* The same for one classic RK4 step (four F, eb_rk4_stage after each), captured and replayed
+ * the same way; the graph is rebuilt when the call switches between the two integrators.  The
+ * reference integrates with RK2 only: no counterpart there. */
+int eb_field_rk4_graph(eb_field f, double *ybar, double dt, int32_t budget, int32_t *redone);
 
 /* The stages one at a time, for tests and for callers that replace part of the chain.
  * eb_field_set_geometry runs EXTRACTDAT on rku and rebuilds every FMM tree; the others act on
@@ -112,6 +116,9 @@ int eb_sweep_field_destroy(eb_sweep_field f);
  * ybar: 2 nbk doubles on the device, updated in place.  The graph is rebuilt when ybar, dt or
  * budget change.  The field must have been created on a non-default stream. */
 int eb_sweep_field_rk2_graph(eb_sweep_field f, double *ybar, double dt, int32_t budget, int32_t *redone);
+/* One classic RK4 step of every simulation (four F with their eight batched solves), captured
+ * and replayed as eb_sweep_field_rk2_graph.  No counterpart in the reference (RK2 only). */
+int eb_sweep_field_rk4_graph(eb_sweep_field f, double *ybar, double dt, int32_t budget, int32_t *redone);
 
 #ifdef __cplusplus
 }

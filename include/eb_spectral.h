@@ -59,6 +59,18 @@ int eb_rk2_stage1(int32_t neq, double dt, const double *ybar, const double *dy, 
                   void *stream);
 int eb_rk2_stage2(int32_t neq, double dt, const double *phi0, const double *dy, double *ybar,
                   void *stream);
+/* Classic fourth-order Runge-Kutta, one stage update after each of its four evaluations
+ * dy = F(stage state) (stage 0: F(y), 1: F(ys after stage 0), ...): acc accumulates
+ * k1 + 2 k2 + 2 k3, ys receives the next stage's state, and stage 3 sets
+ * y += dt/6 (k1 + 2 k2 + 2 k3 + k4).  The reference integrates with RK2 only (electric.f:762-786);
+ * no counterpart there: RK4 serves the runs that ask for a fourth-order integrator. */
+int eb_rk4_stage(int32_t neq, int32_t stage, double dt, double *y, const double *dy, double *acc, double *ys,
+                 void *stream);
+/* D[k] = (L - B) / (L + B) of every body of the state rku (2*nbk interleaved): L, B the semi-axes
+ * of the ellipse with the body's area and second moments of area (trapezoid sums with the
+ * spectral tangent).  The reference prints no deformation: the measure of drop deformation
+ * used by the single-drop runs and the adjoint's loss. */
+int eb_deformation(eb_bodies bodies, const double *rku, double *D, void *stream);
 
 /* ---------------------------------------------------------------------------------------
  * The Stokes matvec of the GMRES solve: MATVEC (stokesbubb.f:419-465) + FASMVP

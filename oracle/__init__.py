@@ -730,6 +730,59 @@ def ref_dgmres_stokes(nd, z, dz, dsdth, rkappa, zk, rhs, maxl=100, tol=1e-10, nr
 # The reference's own right-hand side and time step, on a chosen library handle
 # ------------------------------------------------------------------------------------
 
+def rk4(rhs, y0, dt, nsteps, record=None):
+    """Classic fourth-order Runge-Kutta (the textbook method: y += dt/6 (k1 + 2 k2 + 2 k3 + k4))
+    around a given right-hand side, in the order of the library's stage updates.  The reference
+    integrates with RK2 only (electric.f:762-786); with RefSim.rhs (the reference's own F) this
+    is the checker of the library's RK4 runs.  record(step, y) is called after every step."""
+    y = np.ascontiguousarray(y0, np.float64).copy()
+    for step in range(nsteps):
+        k = rhs(y)
+        acc = k.copy()
+        ys = y + (0.5 * dt) * k
+        k = rhs(ys)
+        acc = acc + 2.0 * k
+        ys = y + (0.5 * dt) * k
+        k = rhs(ys)
+        acc = acc + 2.0 * k
+        ys = y + dt * k
+        k = rhs(ys)
+        y = y + (dt / 6.0) * (acc + k)
+        if record is not None:
+            record(step + 1, y)
+    return y
+
+
+def deformation(nd, rku):
+    """D = (L - B) / (L + B) of every body of the interleaved state rku: L, B the semi-axes of
+    the ellipse with the body's area and second moments of area, from trapezoid sums of
+    Green's-theorem integrands with the spectral tangent (FDIFFF's wavenumbers: the Nyquist
+    coefficient dropped).  The reference prints no deformation; pinned by ellipses' closed form."""
+    z = np.ascontiguousarray(rku, np.float64).view(np.complex128)
+    out = []
+    o = 0
+    for N in np.asarray(nd, np.int64):
+        zb = z[o:o + N]
+        o += N
+        kw = np.fft.fftfreq(N, 1.0 / N)
+        if N % 2 == 0:
+            kw[N // 2] = 0.0
+        za = np.fft.ifft(1j * kw * np.fft.fft(zb))
+        zb = zb - zb.mean()
+        x, y, xa, ya = zb.real, zb.imag, za.real, za.imag
+        h = 2.0 * np.pi / N
+        A = 0.5 * h * np.sum(x * ya - y * xa)
+        mx, my = 0.5 * h * np.sum(x * x * ya), -0.5 * h * np.sum(y * y * xa)
+        ixx, iyy = h * np.sum(x ** 3 * ya) / 3.0, -h * np.sum(y ** 3 * xa) / 3.0
+        ixy = 0.5 * h * np.sum(x * x * y * ya)
+        cx, cy = mx / A, my / A
+        a, c, b = ixx - A * cx * cx, iyy - A * cy * cy, ixy - A * cx * cy
+        rt = np.sqrt(0.25 * (a - c) ** 2 + b * b)
+        sp, sm = np.sqrt(0.5 * (a + c) + rt), np.sqrt(0.5 * (a + c) - rt)
+        out.append((sp - sm) / (sp + sm))
+    return np.array(out)
+
+
 class RefSim:
     """Drive the reference's F (electric.f:968-1109) and RK2 (electric.f:762-786) in-process.
 

@@ -113,3 +113,65 @@ def test_whole_rhs_through_shim_matches_reference(sims, dropin_mode, tag):
     r, g = ref.rhs(f["rku"]), gpu.rhs(f["rku"])
     assert rel(g, r) < 1e-8
     assert rel(g, f["rkup"]) < 1e-8
+
+
+# ------------------------------------------------------------------------------------------
+# The single drop r = 1 + 0.05 cos 2 theta, 256 points, |E| = 1 (E0 = 1), no straining flow,
+# over 100 steps of dt 1e-3, with the deformation D = (L - B) / (L + B) after every step.
+# ------------------------------------------------------------------------------------------
+
+def _cfg1():
+    z, _, nd, _ = perturbed_circle(256, [(2, 0.05)])
+    return nd, z.view(np.float64).copy()
+
+
+def test_single_drop_100_heun_steps_match_reference(sims):
+    """The reference's RK2 driver as written, its CADAPDR / CADAPMOD / DAPIF2 calls on the GPU,
+    against the reference as written: D after every one of 100 steps within 1e-8, the final
+    shape within 1e-8."""
+    ref, gpu = sims
+    nd, y0 = _cfg1()
+    series = []
+    for s in (ref, gpu):
+        s.setup(nd, cap=0.0, e0=1.0)
+        y = y0.copy()
+        D = [oracle.deformation(nd, y)[0]]
+        for step in range(100):
+            y = s.rk2(y, 1e-3, 1, t0=step * 1e-3)
+            D.append(oracle.deformation(nd, y)[0])
+        series.append((y, np.array(D)))
+        print("D after 100 Heun steps", D[-1], flush=True)
+    (y_ref, d_ref), (y_gpu, d_gpu) = series
+    assert np.abs(d_gpu - d_ref).max() < 1e-8
+    assert np.abs(y_gpu - y_ref).max() < 1e-8
+    assert d_ref[-1] > d_ref[0]  # the field stretches the drop along x
+
+
+def test_single_drop_100_rk4_steps_match_reference_f(sims):
+    """The device field's captured RK4 step (eb_field_rk4_graph) against classic RK4 around the
+    reference's own F: D after every one of 100 steps (eb_deformation on the device) within 1e-8,
+    the final shape within 1e-8; eb_deformation against the oracle's D of the same state."""
+    import torch
+
+    from electricbubbles_amd import field, spectral
+    ref, _ = sims
+    nd, y0 = _cfg1()
+    ref.setup(nd, cap=0.0, e0=1.0)
+    d_ref = [oracle.deformation(nd, y0)[0]]
+    y_ref = oracle.rk4(ref.rhs, y0, 1e-3, 100, record=lambda k, y: d_ref.append(oracle.deformation(nd, y)[0]))
+    st = torch.cuda.Stream()
+    B = spectral.Bodies(nd, stream=st)
+    F = field.Field(B, cap=0.0, e0=1.0, stream=st)
+    y = torch.tensor(y0, device="cuda")
+    d_gpu = [float(B.deformation(y)[0])]
+    with torch.cuda.stream(st):
+        for step in range(100):
+            assert not F.rk4_graph(y, 1e-3, budget=40)
+            d_gpu.append(float(B.deformation(y)[0]))
+    st.synchronize()
+    d_gpu = np.array(d_gpu)
+    yg = y.cpu().numpy()
+    print("D after 100 RK4 steps", d_gpu[-1], "reference F", d_ref[-1], flush=True)
+    assert abs(d_gpu[-1] - oracle.deformation(nd, yg)[0]) < 1e-13
+    assert np.abs(d_gpu - np.array(d_ref)).max() < 1e-8
+    assert np.abs(yg - y_ref).max() < 1e-8

@@ -323,6 +323,37 @@ def test_ftail_matches_oracle_pieces():
     assert np.isfinite(rk).all()
 
 
+def test_rk4_on_linear_ode():
+    """One step of classic RK4 on y' = A y is the degree-4 Taylor polynomial of exp(dt A)
+    applied to y; a dropped factor 2 on k2 or k3 fails it."""
+    rng = np.random.default_rng(3)
+    A = rng.standard_normal((6, 6))
+    y0 = rng.standard_normal(6)
+    dt = 0.05
+    hA = dt * A
+    P = np.eye(6) + hA + hA @ hA / 2 + hA @ hA @ hA / 6 + hA @ hA @ hA @ hA / 24
+    got = oracle.rk4(lambda y: A @ y, y0, dt, 1)
+    assert rel(got, P @ y0) < 1e-14
+    y2 = oracle.rk4(lambda y: A @ y, y0, dt, 3)
+    assert rel(y2, np.linalg.matrix_power(P, 3) @ y0) < 1e-14
+    bad = y0 + dt / 6 * (A @ y0 + A @ (y0 + 0.5 * dt * A @ y0) + 2 * A @ (y0 + 0.5 * dt * A @ (y0 + 0.5 * dt * A @ y0)))
+    assert rel(got, bad) > 1e-6
+
+
+def test_deformation_of_ellipses_is_the_closed_form():
+    """D of an ellipse with semi-axes a > b is (a - b) / (a + b) whatever its centre, rotation
+    and parametrisation origin; two bodies at once; a circle gives 0."""
+    th = 2 * np.pi * np.arange(256) / 256
+    bodies, expect = [], []
+    for a, b, rot, c, ph in ((1.3, 0.8, 0.4, 20 - 7j, 0.3), (1.0, 1.0, 0.0, 0.5, 0.0), (2.0, 0.5, -1.1, 3j, 1.0)):
+        e = (a * np.cos(th + ph) + 1j * b * np.sin(th + ph)) * np.exp(1j * rot) + c
+        bodies.append(e)
+        expect.append((a - b) / (a + b))
+    z = np.concatenate(bodies)
+    D = oracle.deformation([256] * 3, z.view(np.float64))
+    np.testing.assert_allclose(D, expect, atol=1e-12)
+
+
 def test_rk2_heun_on_linear_ode():
     lam = -0.7
     y0 = np.array([1.0, 2.0])
