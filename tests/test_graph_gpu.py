@@ -153,3 +153,43 @@ def test_plain_graph_budget_fallback(torch):
         st.synchronize()
     assert redone
     assert float((yg - ye).norm() / ye.norm()) < 1e-13
+
+
+def _rk4_eager_step(F, spectral, y, dt, stream, torch):
+    acc, ys = torch.empty_like(y), torch.empty_like(y)
+    for q in range(4):
+        spectral.rk4_stage(q, dt, y, F.rhs(y if q == 0 else ys), acc, ys, stream=stream)
+
+
+def test_rk4_graph_steps_match_eager(torch):
+    """eb_sweep_field_rk4_graph and eb_field_rk4_graph against the eager RK4 step (four F, the
+    stage updates of eb_rk4_stage) over three replays; switching a field from RK2 to RK4 rebuilds
+    its graph; a one-step budget falls back to the host-controlled solves."""
+    from electricbubbles_amd import field, spectral
+    nd, rku, e0, cap = _case(nsim=8, seed=9)
+    st = torch.cuda.Stream()
+    B, SF = _field(torch, nd, e0, cap, st)
+    B2, SF2 = _field(torch, nd, e0, cap, st)
+    pnd, prku = _plain_case(seed=6)
+    Fg = field.Field(spectral.Bodies(pnd, stream=st), cap=0.1, e0=1.5, stream=st)
+    Fe = field.Field(spectral.Bodies(pnd, stream=st), cap=0.1, e0=1.5, stream=st)
+    dt = 1e-3
+    with torch.cuda.stream(st):
+        for name, G, E, y0 in (("sweep", SF, SF2, rku), ("plain", Fg, Fe, prku)):
+            yg = torch.tensor(y0, device="cuda")
+            ye = yg.clone()
+            assert not G.rk2_graph(yg, dt, budget=40)  # an RK2 graph first: the RK4 call replaces it
+            _plain_eager_step(E, spectral, ye, dt, st) if name == "plain" else _eager_step(E, spectral, ye, dt, st)
+            for step in range(3):
+                assert not G.rk4_graph(yg, dt, budget=40)
+                _rk4_eager_step(E, spectral, ye, dt, st, torch)
+                st.synchronize()
+                d = float((yg - ye).norm() / ye.norm())
+                print(name, "rk4 step", step, "rel diff", d, flush=True)
+                assert d < 1e-12
+            yb, yc = yg.clone(), yg.clone()
+            assert G.rk4_graph(yb, dt, budget=1)  # redone with host-controlled solves
+            _rk4_eager_step(E, spectral, yc, dt, st, torch)
+            st.synchronize()
+            assert float((yb - yc).norm() / yc.norm()) < 1e-12
+    torch.cuda.synchronize()
