@@ -36,11 +36,11 @@ PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 
 # HBM bytes per launch of the dominant kernels, from two rocprofv3 --pmc passes (FETCH_SIZE,
-# WRITE_SIZE) of this script, scaled by the same counters on a 256 MiB copy (fetch x 1.99969,
+# WRITE_SIZE) of this script, scaled by the same counters on a 256 MiB copy (fetch x 1.99968,
 # write x 1.000); profiles/fmm_curves_n1048576_L8_p36_hbm_traffic.md has the commands and the
 # per-kernel table.  Keyed by (workload, points, levmax, nterms).
 PROFILED_TRAFFIC_BYTES = {
-    ("curves", 1 << 20, 8, 36): {"m2l": 222553483, "p2p": 396065074},
+    ("curves", 1 << 20, 8, 36): {"m2l": 223445669, "p2p": 396217442},
 }
 
 
