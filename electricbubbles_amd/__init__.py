@@ -49,7 +49,7 @@ EXPORTS = (
     "eb_fmm_plan_ntbox", "eb_fmm_enable_timing", "eb_fmm_stage_ms", "eb_fmm_set_p2p_tile", "eb_cadap_tables",
     "eb_cadap_lists", "eb_direct_apply_noalloc", "eb_direct_subset", "eb_dapif2", "eb_afmm_create",
     "eb_afmm_apply", "eb_afmm_info", "eb_afmm_enable_timing", "eb_afmm_stage_ms", "eb_afmm_check",
-    "eb_afmm_destroy", "eb_fmm_set_guard", "eb_fmm_apply_transpose",
+    "eb_afmm_destroy", "eb_fmm_set_guard", "eb_fmm_apply_transpose", "eb_fmm_plan_create_segmented",
 )
 
 _lib = None
@@ -84,6 +84,8 @@ def lib():
     L.eb_fmm_plan_create.argtypes = [ctypes.POINTER(_vp), _i32, _vp, _vp, _vp, _i32, _i32, _i32, _i32, _vp]
     L.eb_fmm_apply.argtypes = [_vp, _vp, _vp, _vp]
     L.eb_fmm_apply_transpose.argtypes = [_vp, _vp, _vp, _vp]
+    L.eb_fmm_plan_create_segmented.argtypes = [ctypes.POINTER(_vp), _i32, _vp, _vp, _vp, _i32,
+                                               ctypes.POINTER(_i32), _i32, _i32, _i32, _vp]
     L.eb_fmm_plan_rebuild.argtypes = [_vp, _vp, _vp, _vp, _vp]
     L.eb_fmm_plan_destroy.argtypes = [_vp]
     L.eb_fmm_plan_info.argtypes = [_vp, _dp, _i64p]
@@ -263,6 +265,30 @@ class FMMPlan:
                                         _ptr(self._keep[2]), levmax, nterms, variant, mode,
                                         _stream_ptr(stream)))
         self._h = h
+
+    @classmethod
+    def segmented(cls, x, y, wdz, seg_off, levmax, nterms, variant=VARIANT_DR, stream=None):
+        """A plan over independent segments (eb_fmm_plan_create_segmented): segment s is points
+        seg_off[s] .. seg_off[s+1]-1, binned on its own (own bounding box and scale) with levmax
+        levels below its box, and apply() sums within each segment only."""
+        torch = _torch()
+        self = cls.__new__(cls)
+        self._torch = torch
+        self.n = int(x.numel())
+        assert x.dtype == torch.float64 and y.dtype == torch.float64 and wdz.dtype == torch.complex128
+        assert x.is_cuda and y.is_cuda and wdz.is_cuda and y.numel() == self.n and wdz.numel() == self.n
+        self._keep = (x.contiguous(), y.contiguous(), wdz.contiguous())
+        off = np.ascontiguousarray(seg_off, np.int32)
+        self.nseg = off.size - 1
+        self.levmax, self.nterms, self.variant, self.mode = levmax, nterms, variant, MODE_FMM
+        self.stream = stream
+        h = ctypes.c_void_p()
+        _check(lib().eb_fmm_plan_create_segmented(ctypes.byref(h), self.n, _ptr(self._keep[0]), _ptr(self._keep[1]),
+                                                  _ptr(self._keep[2]), self.nseg,
+                                                  off.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), levmax, nterms,
+                                                  variant, _stream_ptr(stream)))
+        self._h = h
+        return self
 
     def _enter(self):
         """The plan's own stream when it differs from torch's current one (made to wait for the

@@ -148,6 +148,75 @@ __global__ void k_keys(const double* __restrict__ x, const double* __restrict__ 
   vals[i] = i;
 }
 
+// Segmented plans: each segment's bounding box (cadapdrop.f:147-158 on the segment's points)
+// and the scale that maps its larger side onto the side T of one box of level segk.  One block
+// per segment; an empty or single-point segment gets scale 1.
+__global__ __launch_bounds__(BB_THREADS) void k_seg_bbox(int nseg, const int* __restrict__ seg_off,
+                                                         const double* __restrict__ x, const double* __restrict__ y,
+                                                         double T, double4* __restrict__ seg_geo,
+                                                         double* __restrict__ seg_scale, Geom* __restrict__ g) {
+  const int sg = blockIdx.x;
+  const int a = seg_off[sg], b = seg_off[sg + 1];
+  double mnx = INFINITY, mxx = -INFINITY, mny = INFINITY, mxy = -INFINITY;
+  for (int i = a + (int)threadIdx.x; i < b; i += BB_THREADS) {
+    mnx = fmin(mnx, x[i]);
+    mxx = fmax(mxx, x[i]);
+    mny = fmin(mny, y[i]);
+    mxy = fmax(mxy, y[i]);
+  }
+  __shared__ double4 s[BB_THREADS];
+  s[threadIdx.x] = make_double4(mnx, mxx, mny, mxy);
+  __syncthreads();
+  for (int off = BB_THREADS / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) {
+      const double4 u = s[threadIdx.x], v = s[threadIdx.x + off];
+      s[threadIdx.x] = make_double4(fmin(u.x, v.x), fmax(u.y, v.y), fmin(u.z, v.z), fmax(u.w, v.w));
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double4 u = s[0];
+    const double r = fmax(u.y - u.x, u.w - u.z);
+    const double sc = (b - a >= 2 && r > 0.0) ? T / r : 1.0;
+    seg_geo[sg] = make_double4(b > a ? u.x : 0.0, b > a ? u.z : 0.0, sc, 0.0);
+    seg_scale[sg] = sc;
+    if (sg == 0) {  // the plan-wide facts of eb_fmm_plan_info: no single scaling
+      g->rscal = 1.0;
+      g->rminx = 0.0;
+      g->rminy = 0.0;
+    }
+  }
+}
+
+// Segmented plans: scaled coordinates inside the segment's box of level segk and the
+// finest-level key, the segment's own leaf index clamped to its box as ASSIGN1 clamps to the
+// grid (cadap2.f:312-319).
+__global__ void k_seg_keys(int n, int nseg, const int* __restrict__ seg_off, const double* __restrict__ x,
+                           const double* __restrict__ y, int L, int segk, const double4* __restrict__ seg_geo,
+                           double2* __restrict__ zs0, uint32_t* __restrict__ keys, int32_t* __restrict__ vals) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int lo = 0, hi = nseg - 1;  // the segment of point i: the last s with seg_off[s] <= i
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (seg_off[mid] <= i) lo = mid;
+    else hi = mid - 1;
+  }
+  const int sg = lo;
+  const double4 q = seg_geo[sg];
+  const double T = 64.0 / (1 << segk);
+  const int sub = 1 << (L - segk);
+  const double h = T / sub;
+  const double X = (x[i] - q.x) * q.z, Y = (y[i] - q.y) * q.z;
+  int jx = (int)(X / h), jy = (int)(Y / h);
+  jx = jx >= sub ? sub - 1 : (jx < 0 ? 0 : jx);
+  jy = jy >= sub ? sub - 1 : (jy < 0 ? 0 : jy);
+  const int tx = (int)compact2((uint32_t)sg), ty = (int)compact2((uint32_t)sg >> 1);
+  zs0[i] = make_double2(tx * T + X, ty * T + Y);
+  keys[i] = morton((uint32_t)(tx * sub + jx), (uint32_t)(ty * sub + jy));
+  vals[i] = i;
+}
+
 // Leaf ranges and the gather in one launch: thread t finds leaf t's first sorted point
 // (leaf_start[t] = the first position whose key is >= t, t = 0..4^L; t <= nleaf) and gathers
 // point t into sorted order (t < n)
@@ -240,7 +309,14 @@ struct OutForm {
   double sgn, sqb;
   int kind;                // OUT_*
   const double2* w;        // transposed forms: the points' WDZ in the plan's (sorted) order
+  const double* segs;      // segmented plans: each segment's scale (its RSCAL), by segment
+  int segk;                // segmented plans: the segments' level (segment = leaf key >> 2(L - segk))
 };
+
+// RSCAL of a point in leaf `key` of level L: the plan's, or its segment's in a segmented plan
+__device__ __forceinline__ double out_scale(const OutForm& f, const Geom* g, uint32_t key, int L) {
+  return f.segs ? f.segs[key >> (2 * (L - f.segk))] : g->rscal;
+}
 
 // the output at sorted point i from its two parts (phi, V); w = WDZ of the point (transposed)
 template <bool TRANS>
@@ -461,12 +537,12 @@ __device__ __forceinline__ bool m2l_live(const int* __restrict__ leaf_start, int
   return be > bs;
 }
 
-__global__ __launch_bounds__(1024) void k_m2l_count(int L, int nb, const int* __restrict__ leaf_start,
+__global__ __launch_bounds__(1024) void k_m2l_count(int L, int nb, int lmin, const int* __restrict__ leaf_start,
                                                     int* __restrict__ ccnt) {
   __shared__ int wsum[16];
   int cls, l, c0;
   m2l_chunk(L, nb, cls, l, c0);
-  const bool live = m2l_live(leaf_start, L, l, cls, c0 + (int)threadIdx.x);
+  const bool live = l >= lmin && m2l_live(leaf_start, L, l, cls, c0 + (int)threadIdx.x);
   const unsigned long long m = __ballot(live);
   if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = __popcll(m);
   __syncthreads();
@@ -477,7 +553,7 @@ __global__ __launch_bounds__(1024) void k_m2l_count(int L, int nb, const int* __
   }
 }
 
-__global__ __launch_bounds__(1024) void k_m2l_write(int L, int nb, const int* __restrict__ leaf_start,
+__global__ __launch_bounds__(1024) void k_m2l_write(int L, int nb, int lmin, const int* __restrict__ leaf_start,
                                                     const int* __restrict__ ccnt, int cap, int* __restrict__ list,
                                                     int* __restrict__ count) {
   __shared__ int wsum[16], wpre[16], base_s, last_s, lvl_s;
@@ -505,7 +581,7 @@ __global__ __launch_bounds__(1024) void k_m2l_write(int L, int nb, const int* __
     lvl_s = pos + lvl;  // end of this level's entries before padding
   }
   const int q = c0 + (int)threadIdx.x;
-  const bool live = m2l_live(leaf_start, L, l, cls, q);
+  const bool live = l >= lmin && m2l_live(leaf_start, L, l, cls, q);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const unsigned long long m = __ballot(live);
   if (lane == 0) wsum[wv] = __popcll(m);
@@ -557,6 +633,8 @@ struct XlatLevels {
   int Wt[EB_MAXLEV], NBPt[EB_MAXLEV];
   int Ws[EB_MAXLEV], NBPs[EB_MAXLEV];
   long long offt[EB_MAXLEV], offs[EB_MAXLEV];  // expansion offsets (double2 elements)
+  int segk;                   // segmented plans: the segments' level (M2L sources outside the
+                              // target's segment read as zero); 0 otherwise
 };
 
 // value of lane l^1 (DPP quad_perm [1,0,3,2])
@@ -748,7 +826,7 @@ __device__ __forceinline__ void xlat_wave(const double* Cs, const double2* Ps, c
 // scaled translations do not depend on the level), sources from a slot table stab[target * NS
 // + slot]; a slot without a box holds nsrc, one past the last box, which the buffer resource
 // reads as zeros.
-template <int MODE, int KMAX, int KS, int MT, int TR, bool ADAPT = false>
+template <int MODE, int KMAX, int KS, int MT, int TR, bool ADAPT = false, bool SEG = false>
 __device__ __forceinline__ void xlat_body(const XlatLevels& LS, const int* __restrict__ leaf_start,
                                           const double2* __restrict__ src, double2* __restrict__ dst,
                                           const double* __restrict__ Mat, const double2* __restrict__ pre,
@@ -855,6 +933,13 @@ __device__ __forceinline__ void xlat_body(const XlatLevels& LS, const int* __res
   const int* sxp = &Sx[ADAPT ? (4 * wv + t) * NS : 0];
   auto src_box = [&](int o) -> int {
     if constexpr (ADAPT) return sxp[o];
+    if constexpr (MODE == XL_M2L && SEG) {
+      // a source in another segment's box of level segk reads as zero (an index past the level)
+      const int dx = offs[(cls * 27 + o) * 2 + 0], dy = offs[(cls * 27 + o) * 2 + 1];
+      const int sh = l - LS.segk;
+      const bool same = (((ix + dx) >> sh) == (ix >> sh)) && (((iy + dy) >> sh) == (iy >> sh));
+      return same ? pt + dy * Ws + dx : NBPs;
+    }
     if (MODE == XL_M2L) return pt + offs[(cls * 27 + o) * 2 + 1] * Ws + offs[(cls * 27 + o) * 2 + 0];
     if (MODE == XL_M2M) return pidx(2 * ix + (o & 1), 2 * iy + (o >> 1), Ws);
     return pidx(ix >> 1, iy >> 1, Ws);
@@ -868,15 +953,17 @@ __device__ __forceinline__ void xlat_body(const XlatLevels& LS, const int* __res
       double2* __restrict__ dst, const double* __restrict__ Mat, const double2* __restrict__ pre,      \
       const double2* __restrict__ post, const int* __restrict__ offs, const int* __restrict__ m2l_list, \
       const int* __restrict__ m2l_count, int m2l_cap
-template <int MODE, int KMAX, int KS, int MT, int TR>
+template <int MODE, int KMAX, int KS, int MT, int TR, bool SEG = false>
 __global__ __launch_bounds__(256) void k_xlat(EB_XLAT_ARGS) {
-  xlat_body<MODE, KMAX, KS, MT, TR>(LS, leaf_start, src, dst, Mat, pre, post, offs, m2l_list, m2l_count, m2l_cap);
+  xlat_body<MODE, KMAX, KS, MT, TR, false, SEG>(LS, leaf_start, src, dst, Mat, pre, post, offs, m2l_list, m2l_count,
+                                               m2l_cap);
 }
 // The M2L with a vector tail: its LDS admits two workgroups per CU, so the registers are held
 // to what two waves per SIMD allow.
-template <int MODE, int KMAX, int KS, int MT, int TR>
+template <int MODE, int KMAX, int KS, int MT, int TR, bool SEG = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_xlat2(EB_XLAT_ARGS) {
-  xlat_body<MODE, KMAX, KS, MT, TR>(LS, leaf_start, src, dst, Mat, pre, post, offs, m2l_list, m2l_count, m2l_cap);
+  xlat_body<MODE, KMAX, KS, MT, TR, false, SEG>(LS, leaf_start, src, dst, Mat, pre, post, offs, m2l_list, m2l_count,
+                                               m2l_cap);
 }
 // the same on the boxes of an adaptive tree (ADAPT above)
 template <int MODE, int KMAX, int KS, int MT, int TR>
@@ -1044,7 +1131,7 @@ __global__ __launch_bounds__(L2P_T) void k_l2p(int L, int p, const int4* __restr
   // z conj(phi') + conj(psi)
   const double2 zc = cmul(z, cconj(phip));
   const double2 t = make_double2(zc.x + psi.x, zc.y - psi.y);
-  const double rs = g->rscal;
+  const double rs = out_scale(of, g, key, L);
   const int o = perm[i];
   const double2 nearv = out[o];
   const double2 fv = out_form<TRANS>(of, phi, t, TRANS ? of.w[i] : make_double2(0, 0));
@@ -1125,6 +1212,7 @@ __device__ __forceinline__ void p2p_item(int itemid, int L, const int* __restric
     for (int dy = -1; dy <= 1; ++dy) {
       const int jx = ix + dx, jy = iy + dy;
       if (jx < 0 || jy < 0 || jx >= nside || jy >= nside) continue;
+      if (of.segk && (((jx ^ ix) | (jy ^ iy)) >> (L - of.segk))) continue;  // another segment's leaf
       const uint32_t m = morton((uint32_t)jx, (uint32_t)jy);
       sb[nb] = leaf_start[m];
       cum[nb + 1] = cum[nb] + (leaf_start[m + 1] - sb[nb]);
@@ -1132,7 +1220,7 @@ __device__ __forceinline__ void p2p_item(int itemid, int L, const int* __restric
       ++nb;
     }
   const int S = cum[nb];
-  const double rs = g->rscal;
+  const double rs = out_scale(of, g, mk, L);
   // each lane owns two consecutive targets (register blocking halves the LDS reads per
   // pair); with TP = ceil(T/2) < 64 target pairs the lanes split into G = 64/TP slices
   const int TP = (T + 1) >> 1;
@@ -1641,6 +1729,12 @@ struct eb_fmm_plan_s {
   // join before L2P, which adds the far field to the near field's output)
   hipStream_t st2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // segmented plans (eb_fmm_plan_create_segmented): nseg independent point sets, segment s in
+  // box s (Morton order) of level segk, each with its own bounding box and scale
+  int nseg = 0, segk = 0;
+  int* seg_off = nullptr;       // nseg + 1 point offsets
+  double4* seg_geo = nullptr;   // per segment: (XMIN, YMIN, scale, 0)
+  double* seg_scale = nullptr;  // per segment: the scale (the output factor, as RSCAL)
 };
 
 static void plan_free(eb_fmm_plan P) {
@@ -1648,7 +1742,8 @@ static void plan_free(eb_fmm_plan P) {
   void* ptrs[] = {P->geom, P->bbpart, P->zs0, P->zs, P->wdzs, P->qa, P->qb, P->gwf, P->keys, P->keys2,
                   P->perm, P->vals2, P->sorttmp, P->leaf_start, P->p2p_items, P->p2p_cnt, P->p2p_off,
                   P->p2p_tmp, P->l2p_items, P->l2p_total, P->wdz0, P->ntbox, P->m2l_list, P->m2l_count, P->m2l_ccnt, P->mp, P->lo, P->dMat,
-                  P->dU, P->dNU, P->dPM, P->dQM, P->dPL, P->dQL, P->doffs, P->dir_part};
+                  P->dU, P->dNU, P->dPM, P->dQM, P->dPL, P->dQL, P->doffs, P->dir_part, P->seg_off, P->seg_geo,
+                  P->seg_scale};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   for (auto& e : P->ev)
@@ -1741,9 +1836,12 @@ static int sort_tree(eb_fmm_plan P, const double2* wsrc) {
                      P->perm, P->zs0, wsrc, P->zs, P->wdzs);
   if ((rc = build_items(P)) != EB_OK) return rc;
   if (P->m2l_list) {
-    hipLaunchKernelGGL(k_m2l_count, dim3(4 * P->m2l_nb), dim3(1024), 0, st, L, P->m2l_nb, P->leaf_start,
+    // a segmented plan's boxes interact from two levels below the segments' level: the
+    // interaction lists of the segment boxes' children lie wholly in other segments
+    const int lmin = P->segk ? P->segk + 2 : 2;
+    hipLaunchKernelGGL(k_m2l_count, dim3(4 * P->m2l_nb), dim3(1024), 0, st, L, P->m2l_nb, lmin, P->leaf_start,
                        P->m2l_ccnt);
-    hipLaunchKernelGGL(k_m2l_write, dim3(4 * P->m2l_nb), dim3(1024), 0, st, L, P->m2l_nb, P->leaf_start,
+    hipLaunchKernelGGL(k_m2l_write, dim3(4 * P->m2l_nb), dim3(1024), 0, st, L, P->m2l_nb, lmin, P->leaf_start,
                        P->m2l_ccnt, P->m2l_cap, P->m2l_list, P->m2l_count);
   }
   EB_HIP_CHECK(hipGetLastError());
@@ -1774,10 +1872,17 @@ static int build_tree(eb_fmm_plan P, const double* x, const double* y, const dou
   const int n = P->n, L = P->L;
   const int nleaf = 1 << (2 * L);
   const int bbblocks = std::min(1024, (n + BB_THREADS - 1) / BB_THREADS);
-  hipLaunchKernelGGL(k_bbox_part, dim3(bbblocks), dim3(BB_THREADS), 0, st, x, y, n, P->bbpart);
-  hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(BB_THREADS), 0, st, P->bbpart, bbblocks, P->geom);
-  hipLaunchKernelGGL(k_keys, dim3((n + 255) / 256), dim3(256), 0, st, x, y, n, L, P->geom, P->zs0, P->keys,
-                     P->perm);
+  if (P->nseg) {
+    hipLaunchKernelGGL(k_seg_bbox, dim3(P->nseg), dim3(BB_THREADS), 0, st, P->nseg, P->seg_off, x, y,
+                       64.0 / (1 << P->segk), P->seg_geo, P->seg_scale, P->geom);
+    hipLaunchKernelGGL(k_seg_keys, dim3((n + 255) / 256), dim3(256), 0, st, n, P->nseg, P->seg_off, x, y, L, P->segk,
+                       P->seg_geo, P->zs0, P->keys, P->perm);
+  } else {
+    hipLaunchKernelGGL(k_bbox_part, dim3(bbblocks), dim3(BB_THREADS), 0, st, x, y, n, P->bbpart);
+    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(BB_THREADS), 0, st, P->bbpart, bbblocks, P->geom);
+    hipLaunchKernelGGL(k_keys, dim3((n + 255) / 256), dim3(256), 0, st, x, y, n, L, P->geom, P->zs0, P->keys,
+                       P->perm);
+  }
   EB_HIP_CHECK(hipGetLastError());
   if (P->mode == EB_MODE_DIRECT) {
     // the O(N^2) sum runs on the points in the caller's order: four launches per rebuild
@@ -1792,9 +1897,11 @@ static int build_tree(eb_fmm_plan P, const double* x, const double* y, const dou
   return sort_tree(P, (const double2*)wdz);
 }
 
-extern "C" int eb_fmm_plan_create(eb_fmm_plan* out, int32_t n, const double* x, const double* y,
-                                  const double* wdz, int32_t levmax, int32_t nterms, int32_t variant,
-                                  int32_t mode, void* stream) {
+// eb_fmm_plan_create (nseg = 0) and eb_fmm_plan_create_segmented (nseg >= 1 segments with the
+// host offsets seg_off_h; levmax is then the depth below the segments' level)
+static int plan_create(eb_fmm_plan* out, int32_t n, const double* x, const double* y, const double* wdz,
+                       int32_t levmax, int32_t nterms, int32_t variant, int32_t mode, void* stream, int32_t nseg,
+                       const int32_t* seg_off_h) {
   if (!out) {
     eb_set_error("eb_fmm_plan_create: null plan pointer");
     return EB_ERR_ARG;
@@ -1803,6 +1910,25 @@ extern "C" int eb_fmm_plan_create(eb_fmm_plan* out, int32_t n, const double* x, 
   if (n < 2 || !x || !y || !wdz) {
     eb_set_error("eb_fmm_plan_create: need n >= 2 points and non-null x, y, wdz (n=%d)", n);
     return EB_ERR_ARG;
+  }
+  int segk = 0;
+  if (nseg > 0) {
+    while (((int64_t)1 << (2 * segk)) < nseg) ++segk;
+    if (mode != EB_MODE_FMM || !seg_off_h || seg_off_h[0] != 0 || seg_off_h[nseg] != n) {
+      eb_set_error("eb_fmm_plan_create_segmented: FMM mode and offsets 0 = seg_off[0] <= ... <= seg_off[nseg] = n");
+      return EB_ERR_ARG;
+    }
+    for (int i = 0; i < nseg; ++i)
+      if (seg_off_h[i + 1] < seg_off_h[i]) {
+        eb_set_error("eb_fmm_plan_create_segmented: seg_off decreases at segment %d", i);
+        return EB_ERR_ARG;
+      }
+    if (levmax < 1 || segk + levmax > EB_MAXLEV) {
+      eb_set_error("eb_fmm_plan_create_segmented: %d segments need %d levels above the per-segment levmax=%d; "
+                   "the total must stay within 1..%d", nseg, segk, levmax, EB_MAXLEV);
+      return EB_ERR_ARG;
+    }
+    levmax += segk;
   }
   if (levmax < 1 || levmax > EB_MAXLEV) {
     eb_set_error("eb_fmm_plan_create: levmax=%d outside 1..%d", levmax, EB_MAXLEV);
@@ -1831,6 +1957,17 @@ extern "C" int eb_fmm_plan_create(eb_fmm_plan* out, int32_t n, const double* x, 
   const int L = P->L, K = P->kmax;
   const int nleaf = 1 << (2 * L);
   const int bbblocks = std::min(1024, (n + BB_THREADS - 1) / BB_THREADS);
+  if (nseg > 0) {
+    P->nseg = nseg;
+    P->segk = segk;
+    EB_TRY(dmalloc(&P->seg_off, nseg + 1));
+    EB_TRY(dmalloc(&P->seg_geo, nseg));
+    EB_TRY(dmalloc(&P->seg_scale, nseg));
+    EB_TRY(hipMemcpyAsync(P->seg_off, seg_off_h, sizeof(int) * (nseg + 1), hipMemcpyHostToDevice, st) == hipSuccess &&
+                   hipStreamSynchronize(st) == hipSuccess
+               ? EB_OK
+               : (eb_set_error("eb_fmm_plan_create_segmented: offset upload failed"), EB_ERR_HIP));
+  }
 
   EB_TRY(dmalloc(&P->geom, 1));
   EB_TRY(dmalloc(&P->bbpart, bbblocks));
@@ -1896,6 +2033,12 @@ extern "C" int eb_fmm_plan_create(eb_fmm_plan* out, int32_t n, const double* x, 
     EB_TRY((hipMemsetAsync(P->mp, 0, sizeof(double2) * off, st) == hipSuccess)
                ? EB_OK
                : (eb_set_error("hipMemsetAsync failed"), EB_ERR_HIP));
+    // a segmented plan never writes the locals of levels up to segk + 1 (no interaction lists
+    // there), which the downward pass reads as zero
+    if (P->segk)
+      EB_TRY((hipMemsetAsync(P->lo, 0, sizeof(double2) * off, st) == hipSuccess)
+                 ? EB_OK
+                 : (eb_set_error("hipMemsetAsync failed"), EB_ERR_HIP));
     Tables T;
     build_tables(P->p, K, T);
     EB_TRY(dmalloc(&P->dMat, T.Mat.size()));
@@ -1926,6 +2069,22 @@ extern "C" int eb_fmm_plan_create(eb_fmm_plan* out, int32_t n, const double* x, 
   return EB_OK;
 }
 
+extern "C" int eb_fmm_plan_create(eb_fmm_plan* out, int32_t n, const double* x, const double* y,
+                                  const double* wdz, int32_t levmax, int32_t nterms, int32_t variant,
+                                  int32_t mode, void* stream) {
+  return plan_create(out, n, x, y, wdz, levmax, nterms, variant, mode, stream, 0, nullptr);
+}
+
+extern "C" int eb_fmm_plan_create_segmented(eb_fmm_plan* out, int32_t n, const double* x, const double* y,
+                                            const double* wdz, int32_t nseg, const int32_t* seg_off, int32_t levmax,
+                                            int32_t nterms, int32_t variant, void* stream) {
+  if (nseg < 1) {
+    eb_set_error("eb_fmm_plan_create_segmented: nseg=%d", nseg);
+    return EB_ERR_ARG;
+  }
+  return plan_create(out, n, x, y, wdz, levmax, nterms, variant, EB_MODE_FMM, stream, nseg, seg_off);
+}
+
 // Fill the level table of one translation launch: target levels [l0, l1] (one entry each),
 // sources at level l + dl.
 static XlatLevels xlat_levels(eb_fmm_plan P, int l0, int l1, int dl) {
@@ -1947,6 +2106,7 @@ static XlatLevels xlat_levels(eb_fmm_plan P, int l0, int l1, int dl) {
   LS.nent = e;
   LS.blk0[e] = tot;
   LS.total = tot;
+  LS.segk = P->segk;
   return LS;
 }
 
@@ -1961,6 +2121,17 @@ static void launch_xlat_ks(eb_fmm_plan P, const XlatLevels& LS, const double2* s
   const int grid = (LS.total + 7) / 8 * 8;
   const double* mat = P->dMat + (size_t)MODE * K * K;
   const bool c = MODE == XL_M2L && P->m2l_list != nullptr;
+  if constexpr (MODE == XL_M2L) {
+    if (P->segk) {
+      // a segmented plan: sources outside the target's segment are skipped.  The test takes
+      // registers that the two-wave bound of the vector-tail kernel does not leave (it spilled
+      // 344 B per lane there), so the tail form runs unbounded too.
+      hipLaunchKernelGGL((k_xlat<MODE, K, KS, MT, TR, true>), dim3(grid), dim3(256), 0, P->st, LS, P->leaf_start, src,
+                         dst, mat, pre, post, P->doffs, c ? P->m2l_list : nullptr, c ? P->m2l_count : nullptr,
+                         P->m2l_cap);
+      return;
+    }
+  }
   if constexpr (MODE == XL_M2L && TR == 4)
     hipLaunchKernelGGL((k_xlat2<MODE, K, KS, MT, TR>), dim3(grid), dim3(256), 0, P->st, LS, P->leaf_start, src, dst,
                        mat, pre, post, P->doffs, c ? P->m2l_list : nullptr, c ? P->m2l_count : nullptr, P->m2l_cap);
@@ -2046,8 +2217,9 @@ static void launch_l2l(eb_fmm_plan P) {
 
 // the output form of a forward (of.kind == OUT_FORWARD) or transposed apply
 static OutForm out_form_of(eb_fmm_plan P, bool trans) {
-  if (!trans) return OutForm{P->sgn, P->sqb, OUT_FORWARD, nullptr};
-  return OutForm{P->sgn, P->sqb, P->variant == EB_VARIANT_HILBERT ? OUT_T_HILBERT : OUT_T_CADAP, P->wdzs};
+  if (!trans) return OutForm{P->sgn, P->sqb, OUT_FORWARD, nullptr, P->seg_scale, P->segk};
+  return OutForm{P->sgn, P->sqb, P->variant == EB_VARIANT_HILBERT ? OUT_T_HILBERT : OUT_T_CADAP, P->wdzs, P->seg_scale,
+                 P->segk};
 }
 
 // the near field: RSCAL * (the near sum) into out[perm[i]], on stream s

@@ -128,6 +128,19 @@ typedef struct eb_fmm_plan_s *eb_fmm_plan;
 int eb_fmm_plan_create(eb_fmm_plan *plan, int32_t n, const double *x, const double *y,
                        const double *wdz, int32_t levmax, int32_t nterms, int32_t variant,
                        int32_t mode, void *stream);
+/* nseg independent point sets in one plan (the segments of a batch of simulations): segment s
+ * holds points seg_off[s] .. seg_off[s+1]-1 (host array of nseg + 1 offsets, seg_off[0] = 0,
+ * seg_off[nseg] = n, empty segments allowed).  Each segment is binned as CADAPDR bins one point
+ * set (its own bounding box and scale, cadapdrop.f:145-171) into its own box of level
+ * k = ceil(log4 nseg) of one grid, with levmax levels below that box (total depth k + levmax
+ * <= 10), and the sums run within each segment only: gradw[i] is the GW of point i's segment
+ * alone.  One launch per FMM stage serves all segments.  FMM mode; eb_fmm_apply,
+ * eb_fmm_apply_transpose and eb_fmm_plan_rebuild (same segments) work as for other plans.  No
+ * counterpart in the reference, which evaluates one point set per call: a batch of simulations
+ * each too large for the O(N^2) sum. */
+int eb_fmm_plan_create_segmented(eb_fmm_plan *plan, int32_t n, const double *x, const double *y, const double *wdz,
+                                 int32_t nseg, const int32_t *seg_off, int32_t levmax, int32_t nterms,
+                                 int32_t variant, void *stream);
 /* Re-bin new coordinates / weights of the same n into an existing plan (no allocation);
  * used when the geometry moves (every Runge-Kutta stage). */
 int eb_fmm_plan_rebuild(eb_fmm_plan plan, const double *x, const double *y, const double *wdz,
