@@ -877,12 +877,24 @@ struct eb_sweep_field_s {
   double g_dt = 0.0;
   int g_budget = 0, g_order = 0;
   double *ysave = nullptr, *yst = nullptr, *phi0 = nullptr, *dy = nullptr;
+  // EB_MODE_FMM (or AUTO with a simulation above EB_DIRECT_MAX points): the Hilbert products of
+  // the Laplace operator and of PVINTEV on segmented FMM plans (one segment per simulation), and
+  // the sweep operator's layer potentials likewise (eb_sweep_set_mode)
+  int mode = EB_MODE_AUTO, maxn = 0;
+  eb_fmm_plan slap = nullptr, spv = nullptr;
+  std::vector<int> soff2_h;
 };
+
+static bool sf_fmm(eb_sweep_field F) {
+  return F->mode == EB_MODE_FMM || (F->mode == EB_MODE_AUTO && F->maxn > EB_DIRECT_MAX);
+}
 
 static void free_sf(eb_sweep_field F) {
   if (!F) return;
   if (F->st) (void)hipStreamSynchronize(F->st);
   if (F->gexec) (void)hipGraphExecDestroy(F->gexec);
+  if (F->slap) eb_fmm_plan_destroy(F->slap);
+  if (F->spv) eb_fmm_plan_destroy(F->spv);
   for (double* q : {F->ysave, F->yst, F->phi0, F->dy})
     if (q) (void)hipFree(q);
   if (F->glap) eb_gmres_batch_destroy(F->glap);
@@ -923,6 +935,8 @@ extern "C" int eb_sweep_field_create(eb_sweep_field* out, eb_bodies B, int32_t n
     F->soff_h[s] = B->off[sim_off[s]];
     F->uoff_h[s] = 2 * F->soff_h[s];
   }
+  F->soff2_h = F->uoff_h;  // PVINTEV's points: twice each simulation's
+  for (int s = 0; s < nsim; ++s) F->maxn = std::max(F->maxn, F->soff_h[s + 1] - F->soff_h[s]);
   std::vector<double> h(k), capv(k), e0v(k), alv(k);
   for (int s = 0; s < nsim; ++s) {
     const int nl = B->nd[sim_off[s + 1] - 1];
@@ -987,13 +1001,37 @@ extern "C" int eb_sweep_field_destroy(eb_sweep_field F) {
   return EB_OK;
 }
 
+extern "C" int eb_sweep_field_set_mode(eb_sweep_field F, int32_t mode) {
+  if (!F || (mode != EB_MODE_AUTO && mode != EB_MODE_FMM && mode != EB_MODE_DIRECT))
+    return (eb_set_error("eb_sweep_field_set_mode: bad arguments"), EB_ERR_ARG);
+  F->mode = mode;
+  if (F->gexec) {  // the captured step holds the former mode's launches
+    EB_HIP_CHECK(hipGraphExecDestroy(F->gexec));
+    F->gexec = nullptr;
+  }
+  return F->sw ? eb_sweep_set_mode(F->sw, mode) : EB_OK;
+}
+
+// the segmented plan of one Hilbert product (created on first use, re-binned after)
+static int sf_plan(eb_sweep_field F, eb_fmm_plan* P, int npts, const double* x, const double* y, const double2* w,
+                   const std::vector<int>& off, int nmax, int nterms) {
+  if (!*P)
+    return eb_fmm_plan_create_segmented(P, npts, x, y, (const double*)w, F->nsim, off.data(),
+                                        eb::seg_levmax(nmax, F->nsim), nterms, EB_VARIANT_HILBERT, (void*)F->st);
+  return eb_fmm_plan_rebuild(*P, x, y, (const double*)w, (void*)F->st);
+}
+
 static int sf_lap_apply(void* ctx, const double* u, double* w, void* stream) {
   (void)stream;
   eb_sweep_field F = (eb_sweep_field)ctx;
   const int n = F->n;
   hipLaunchKernelGGL(k_real_dens, dim3(ew_blocks(n)), dim3(EW), 0, F->st, n, u, F->dens);
-  hipLaunchKernelGGL(k_hilbert_seg<true>, dim3(F->ntile), dim3(HB_T), 0, F->st, F->tiles, F->soff, 1, 0, 0, F->x, F->y,
-                     F->dens, F->wdz, eb_gmres_batch_active(F->glap), F->cf);
+  if (sf_fmm(F)) {
+    if (int rc = eb_fmm_apply(F->slap, (const double*)F->dens, (double*)F->cf, (void*)F->st)) return rc;
+  } else {
+    hipLaunchKernelGGL(k_hilbert_seg<true>, dim3(F->ntile), dim3(HB_T), 0, F->st, F->tiles, F->soff, 1, 0, 0, F->x,
+                       F->y, F->dens, F->wdz, eb_gmres_batch_active(F->glap), F->cf);
+  }
   hipLaunchKernelGGL(k_lap_point, dim3(ew_blocks(n)), dim3(EW), 0, F->st, n, F->body_of, F->h, u, F->cf, F->rkappa,
                      F->dsda, w);
   EB_HIP_CHECK(hipGetLastError());
@@ -1014,13 +1052,16 @@ static int sweep_rhs(eb_sweep_field F, const double* rku, double* rkup, int budg
   EB_HIP_CHECK(hipGetLastError());
   if (!F->have_geom) {
     if ((rc = eb_sweep_create(&F->sw, B, F->nsim, F->sim_off.data(), F->x, F->y, (const double*)F->dz, F->dsda,
-                              F->rkappa, (const double*)F->zk, s)) != EB_OK)
+                              F->rkappa, (const double*)F->zk, s)) != EB_OK ||
+        (rc = eb_sweep_set_mode(F->sw, F->mode)) != EB_OK)
       return rc;
     F->have_geom = true;
   } else if ((rc = eb_sweep_rebuild(F->sw, F->x, F->y, (const double*)F->dz, F->dsda, F->rkappa, (const double*)F->zk,
                                     s)) != EB_OK) {
     return rc;
   }
+  // TOL 1e-11 for the Laplace side's DAPIF2 (laplace.f:212): 37 terms
+  if (sf_fmm(F) && (rc = sf_plan(F, &F->slap, n, F->x, F->y, F->wdz, F->soff_h, F->maxn, 37)) != EB_OK) return rc;
   // GETBCs + SOLVE_LAPL of every simulation
   hipLaunchKernelGGL(k_getbcs, dim3(ew_blocks(n)), dim3(EW), 0, F->st, n, F->x, F->rhs_lap, F->ulap);
   EB_HIP_CHECK(hipGetLastError());
@@ -1034,8 +1075,16 @@ static int sweep_rhs(eb_sweep_field F, const double* rku, double* rkup, int budg
   // PVINTEV + DIRICH_NEU_MAP
   hipLaunchKernelGGL(k_pv_interp, dim3(B->nbody), dim3(SP_THREADS), lds_bytes(B, 2), F->st, view(B), F->x, F->y,
                      F->ulap, F->dz, F->x2, F->y2, F->uf);
-  hipLaunchKernelGGL(k_hilbert_seg<false>, dim3(F->ntile), dim3(HB_T), 0, F->st, F->tiles, F->soff, 2, 0, 1, F->x2,
-                     F->y2, F->uf, F->wf, (const int*)nullptr, F->cf2);
+  if (sf_fmm(F)) {
+    // all 2N points of every simulation, the midpoints weighted h and the original points 0; the
+    // original points' values are the ones read (TOL 1e-10, pv_integral.f:175: 33 terms)
+    if ((rc = sf_plan(F, &F->spv, 2 * n, F->x2, F->y2, F->wf, F->soff2_h, 2 * F->maxn, 33)) != EB_OK ||
+        (rc = eb_fmm_apply(F->spv, (const double*)F->uf, (double*)F->cf2, s)) != EB_OK)
+      return rc;
+  } else {
+    hipLaunchKernelGGL(k_hilbert_seg<false>, dim3(F->ntile), dim3(HB_T), 0, F->st, F->tiles, F->soff, 2, 0, 1, F->x2,
+                       F->y2, F->uf, F->wf, (const int*)nullptr, F->cf2);
+  }
   hipLaunchKernelGGL(k_dn_post, dim3(B->nbody), dim3(SP_THREADS), lds_bytes(B, 1), F->st, view(B), F->x, F->y, F->ulap,
                      F->cf2, F->dsda, nullptr, nullptr, F->dphi);
   // SETVELF

@@ -31,4 +31,15 @@ __host__ __device__ inline int intl_offsets(int px, int py, int* dxs, int* dys) 
     }
   return n;
 }
+// Depth below the segments' boxes for a segmented plan whose largest segment has nmax points
+// spread along curves: the number of leaves across a segment (2^d) grows until the curve
+// leaves hold about 40 points (nmax / 2^d <= 128, about pi leaves per box side crossed), within
+// the plan's 10 levels.
+inline int seg_levmax(int nmax, int nseg) {
+  int k = 0;
+  while ((1 << (2 * k)) < nseg) ++k;
+  int d = 2;
+  while (d < 10 - k && (nmax >> d) > 128) ++d;
+  return d;
+}
 }  // namespace eb

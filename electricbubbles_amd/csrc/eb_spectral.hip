@@ -850,10 +850,37 @@ struct eb_sweep_s {
   int *body_of = nullptr, *first = nullptr, *soff = nullptr;
   int2* tiles = nullptr;
   const int* mask = nullptr;  // per simulation: 0 = skip in eb_sweep_apply (set around solves)
+  // EB_MODE_FMM (or AUTO with a simulation above EB_DIRECT_MAX points): the layer potentials
+  // on segmented FMM plans, one segment per simulation (CADAPDR's and CADAPMOD's kernels)
+  int mode = EB_MODE_AUTO, maxn = 0;
+  std::vector<int> soff_h;
+  hipStream_t st = nullptr;
+  eb_fmm_plan fdr = nullptr, fmod = nullptr;
 };
+
+static bool sw_fmm(eb_sweep S) { return S->mode == EB_MODE_FMM || (S->mode == EB_MODE_AUTO && S->maxn > EB_DIRECT_MAX); }
+
+// the segmented plans for the current geometry (created on first use, re-binned after)
+static int sw_plans(eb_sweep S, hipStream_t st) {
+  if (!sw_fmm(S)) return EB_OK;
+  const int d = eb::seg_levmax(S->maxn, S->nsim);
+  int rc;
+  if (!S->fdr) {
+    // 43 terms: TOL 1e-13, as FASMVP passes it (stokesbubb.f:564-566) and INTERFACENEW (electric.f:803)
+    if ((rc = eb_fmm_plan_create_segmented(&S->fdr, S->n, S->x, S->y, (const double*)S->wdz, S->nsim,
+                                           S->soff_h.data(), d, 43, EB_VARIANT_DR, st)) != EB_OK)
+      return rc;
+    return eb_fmm_plan_create_segmented(&S->fmod, S->n, S->x, S->y, (const double*)S->wdz, S->nsim, S->soff_h.data(),
+                                        d, 43, EB_VARIANT_MOD, st);
+  }
+  if ((rc = eb_fmm_plan_rebuild(S->fdr, S->x, S->y, (const double*)S->wdz, st)) != EB_OK) return rc;
+  return eb_fmm_plan_rebuild(S->fmod, S->x, S->y, (const double*)S->wdz, st);
+}
 
 static void free_sw(eb_sweep S) {
   if (!S) return;
+  if (S->fdr) eb_fmm_plan_destroy(S->fdr);
+  if (S->fmod) eb_fmm_plan_destroy(S->fmod);
   void* ptrs[] = {S->x, S->y, S->dsdth, S->rkappa, S->h, S->pk, S->dz, S->zk, S->wdz, S->gw, S->za, S->qa, S->qb,
                   S->body_of, S->first, S->soff, S->tiles};
   for (void* q : ptrs)
@@ -893,6 +920,9 @@ extern "C" int eb_sweep_create(eb_sweep* out, eb_bodies B, int32_t nsim, const i
   for (int s : ord)
     for (int t0 = 0; t0 < soff[s + 1] - soff[s]; t0 += SW_TILE) tiles.push_back(make_int2(s, t0));
   S->ntile = (int)tiles.size();
+  S->soff_h = soff;
+  S->st = st;
+  for (int s2 = 0; s2 < nsim; ++s2) S->maxn = std::max(S->maxn, soff[s2 + 1] - soff[s2]);
   std::vector<double> h(nb);
   for (int k = 0; k < nb; ++k) h[k] = 2.0 * 3.14159265358979323846 / B->nd[k];  // stokesbubb.f:446-448
   int rc;
@@ -924,8 +954,19 @@ extern "C" int eb_sweep_create(eb_sweep* out, eb_bodies B, int32_t nsim, const i
     free_sw(S);
     return (eb_set_error("eb_sweep_create: kernel failed"), EB_ERR_HIP);
   }
+  if ((rc = sw_plans(S, st)) != EB_OK) {
+    free_sw(S);
+    return rc;
+  }
   *out = S;
   return EB_OK;
+}
+
+extern "C" int eb_sweep_set_mode(eb_sweep S, int32_t mode) {
+  if (!S || (mode != EB_MODE_AUTO && mode != EB_MODE_FMM && mode != EB_MODE_DIRECT))
+    return (eb_set_error("eb_sweep_set_mode: bad arguments"), EB_ERR_ARG);
+  S->mode = mode;
+  return sw_plans(S, S->st);
 }
 
 // za, pk (unshifted) of the density into S->za, S->pk
@@ -940,9 +981,14 @@ extern "C" int eb_sweep_apply(eb_sweep S, const double* xx, double* yy, void* st
   hipStream_t st = (hipStream_t)stream;
   const int n = S->n;
   const double2* dens = (const double2*)xx;  // dens(i) = dcmplx(u(2i-1), u(2i)) (stokesbubb.f:527-529)
-  hipLaunchKernelGGL(k_sw_charges, dim3((n + 255) / 256), dim3(256), 0, st, n, dens, S->wdz, S->qa, S->qb);
-  hipLaunchKernelGGL(k_sw_gw, dim3(S->ntile), dim3(SW_T), 0, st, S->tiles, S->soff, S->x, S->y, S->qa, S->qb, -1.0,
-                     S->mask, S->gw);
+  if (sw_fmm(S)) {
+    // every simulation's GW (the mask of converged simulations is not consulted)
+    if (int rc = eb_fmm_apply(S->fdr, xx, (double*)S->gw, st)) return rc;
+  } else {
+    hipLaunchKernelGGL(k_sw_charges, dim3((n + 255) / 256), dim3(256), 0, st, n, dens, S->wdz, S->qa, S->qb);
+    hipLaunchKernelGGL(k_sw_gw, dim3(S->ntile), dim3(SW_T), 0, st, S->tiles, S->soff, S->x, S->y, S->qa, S->qb, -1.0,
+                       S->mask, S->gw);
+  }
   sweep_sums(S, dens, st);
   hipLaunchKernelGGL(k_sw_point, dim3((n + 255) / 256), dim3(256), 0, st, n, S->x, S->y, S->dz, S->dsdth, S->rkappa,
                      S->h, dens, S->za, S->pk, S->gw, (double2*)yy, S->body_of, S->first);
@@ -955,6 +1001,7 @@ extern "C" int eb_sweep_layer(eb_sweep S, const double* dens, double* gw, int32_
     return (eb_set_error("eb_sweep_layer: bad arguments"), EB_ERR_ARG);
   hipStream_t st = (hipStream_t)stream;
   const int n = S->n;
+  if (sw_fmm(S)) return eb_fmm_apply(variant == EB_VARIANT_DR ? S->fdr : S->fmod, dens, gw, st);
   hipLaunchKernelGGL(k_sw_charges, dim3((n + 255) / 256), dim3(256), 0, st, n, (const double2*)dens, S->wdz, S->qa,
                      S->qb);
   hipLaunchKernelGGL(k_sw_gw, dim3(S->ntile), dim3(SW_T), 0, st, S->tiles, S->soff, S->x, S->y, S->qa, S->qb,
@@ -977,7 +1024,7 @@ extern "C" int eb_sweep_rebuild(eb_sweep S, const double* x, const double* y, co
   hipLaunchKernelGGL(k_wdz, dim3((n + 255) / 256), dim3(256), 0, st, n, (const double2*)S->dz, S->h, S->body_of,
                      S->wdz);
   EB_HIP_CHECK(hipGetLastError());
-  return EB_OK;
+  return sw_plans(S, st);
 }
 
 extern "C" int eb_sweep_parts(eb_sweep S, const double* xx, double* za, double* pk, void* stream) {

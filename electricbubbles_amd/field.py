@@ -16,7 +16,8 @@ FIELD_EXPORTS = ("eb_field_create", "eb_field_destroy", "eb_field_set_mode", "eb
                  "eb_field_laplace_apply", "eb_field_laplace_solve", "eb_field_dirich_neu", "eb_field_setvelf",
                  "eb_field_stokes_solve", "eb_field_interface", "eb_field_get_geometry", "eb_field_iterations", "eb_sweep_field_create",
                  "eb_sweep_field_rhs", "eb_sweep_field_iterations", "eb_sweep_field_destroy",
-                 "eb_sweep_field_rk2_graph", "eb_field_rk2_graph", "eb_field_rk4_graph", "eb_sweep_field_rk4_graph")
+                 "eb_sweep_field_rk2_graph", "eb_field_rk2_graph", "eb_field_rk4_graph", "eb_sweep_field_rk4_graph",
+                 "eb_sweep_field_set_mode")
 
 _vp = ctypes.c_void_p
 _i32 = ctypes.c_int32
@@ -47,6 +48,7 @@ def _bind():
         L.eb_field_rk2_graph.argtypes = [_vp, _vp, ctypes.c_double, _i32, ctypes.POINTER(_i32)]
         L.eb_field_rk4_graph.argtypes = [_vp, _vp, ctypes.c_double, _i32, ctypes.POINTER(_i32)]
         L.eb_sweep_field_rk4_graph.argtypes = [_vp, _vp, ctypes.c_double, _i32, ctypes.POINTER(_i32)]
+        L.eb_sweep_field_set_mode.argtypes = [_vp, _i32]
         _bound = True
     return L
 
@@ -175,6 +177,11 @@ class SweepField:
         out = self._torch.empty(rku.numel(), dtype=self._torch.float64, device=rku.device)
         _check(_bind().eb_sweep_field_rhs(self._h, _ptr(rku.contiguous()), _ptr(out)))
         return out
+
+    def set_mode(self, mode):
+        """0 auto (segmented FMM plans when a simulation has more than 10240 points), 1 segmented
+        FMM, 2 each simulation's O(N^2) sums (eb_sweep_field_set_mode)."""
+        _check(_bind().eb_sweep_field_set_mode(self._h, int(mode)))
 
     def iterations(self):
         """(Laplace iterations, Laplace IERR, Stokes iterations, Stokes IERR) of the last call."""

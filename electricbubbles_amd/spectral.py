@@ -20,7 +20,7 @@ SPECTRAL_EXPORTS = (
     "eb_bodies_create", "eb_bodies_destroy", "eb_bodies_npoints", "eb_dcfft", "eb_fdifff", "eb_filter",
     "eb_equit", "eb_extractdat", "eb_ftail", "eb_rk2_stage1", "eb_rk2_stage2", "eb_matvec_create",
     "eb_matvec_apply", "eb_matvec_destroy", "eb_fasmvp", "eb_sweep_create", "eb_sweep_apply", "eb_sweep_parts",
-    "eb_sweep_unknown_offsets", "eb_sweep_destroy", "eb_rk4_stage", "eb_deformation",
+    "eb_sweep_unknown_offsets", "eb_sweep_destroy", "eb_rk4_stage", "eb_deformation", "eb_sweep_set_mode",
 )
 
 _bound = False
@@ -44,6 +44,7 @@ def _bind():
     L.eb_rk2_stage2.argtypes = [_i32, ctypes.c_double, _vp, _vp, _vp, _vp]
     L.eb_rk4_stage.argtypes = [_i32, _i32, ctypes.c_double, _vp, _vp, _vp, _vp, _vp]
     L.eb_deformation.argtypes = [_vp, _vp, _vp, _vp]
+    L.eb_sweep_set_mode.argtypes = [_vp, _i32]
     L.eb_matvec_create.argtypes = [ctypes.POINTER(_vp), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _i32, _i32, _vp]
     L.eb_matvec_apply.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp]
     L.eb_matvec_destroy.argtypes = [_vp]
@@ -219,6 +220,11 @@ class Sweep:
         yy = _torch().empty(2 * self.bodies.n, dtype=_torch().float64, device=xx.device)
         _check(_bind().eb_sweep_apply(self._h, _ptr(xx.contiguous()), _ptr(yy), _stream_ptr(self.stream)))
         return yy
+
+    def set_mode(self, mode):
+        """0 auto (segmented FMM when a simulation has more than 10240 points), 1 segmented FMM,
+        2 each simulation's O(N^2) sum (eb_sweep_set_mode)."""
+        _check(_bind().eb_sweep_set_mode(self._h, int(mode)))
 
     def parts(self, xx):
         torch = _torch()

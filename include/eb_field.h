@@ -101,6 +101,12 @@ int eb_sweep_field_create(eb_sweep_field *f, eb_bodies bodies, int32_t nsim, con
                           const double *cap, const double *e0, void *stream);
 /* rkup = F(rku) of every simulation (all bodies back to back, interleaved (x, y)). */
 int eb_sweep_field_rhs(eb_sweep_field f, const double *rku, double *rkup);
+/* Evaluation mode of every layer potential of the sweep's F: EB_MODE_DIRECT (each simulation's
+ * O(N^2) sums), EB_MODE_FMM (segmented FMM plans, one segment per simulation: the Laplace and
+ * PVINTEV Hilbert products at 37 and 33 terms as their TOL 1e-11 and 1e-10, laplace.f:212,
+ * pv_integral.f:175; the Stokes operator and CADAPMOD at 43, eb_sweep_set_mode), EB_MODE_AUTO
+ * (default: FMM when a simulation has more than EB_DIRECT_MAX points).  Drops a captured step. */
+int eb_sweep_field_set_mode(eb_sweep_field f, int32_t mode);
 /* GMRES iteration counts and IERR of the last call's Laplace and Stokes solves (nsim each;
  * any may be NULL). */
 int eb_sweep_field_iterations(eb_sweep_field f, int32_t *lap_iter, int32_t *lap_ierr, int32_t *stokes_iter,

@@ -127,6 +127,13 @@ const int32_t *eb_sweep_unknown_offsets(eb_sweep sw);
  * layer potential of simulations whose flag is 0 (their output is left unspecified).  The
  * batched solves set it to the solver's active flags around their operator calls. */
 int eb_sweep_set_mask(eb_sweep sw, const int32_t *mask);
+/* How the sweep's layer potentials run: EB_MODE_DIRECT (each simulation's O(N^2) sum, the tiles
+ * above), EB_MODE_FMM (segmented FMM plans, one segment per simulation, eb_fmm_plan_create_segmented:
+ * each simulation binned with its own bounding box and scale, 43 terms), EB_MODE_AUTO (default:
+ * FMM when a simulation has more than EB_DIRECT_MAX points).  In FMM mode every simulation's GW is
+ * formed on each apply: the mask above is not consulted.  No counterpart in the reference, which
+ * runs one simulation per process. */
+int eb_sweep_set_mode(eb_sweep sw, int32_t mode);
 int eb_sweep_destroy(eb_sweep sw);
 
 /* Drop-in for FASMVP (stokesbubb.f:496-498) on host arrays, Fortran convention (all by

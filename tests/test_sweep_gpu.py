@@ -151,3 +151,103 @@ def test_sweep_512_single_drops(torch, mods):
         seg = slice(uo[s], uo[s + 1])
         assert np.linalg.norm(ax[seg] - r[seg]) <= 1.5e-10 * np.linalg.norm(r[seg]), s
     gb.close()
+
+
+# ------------------------------------------------------------------------------------------
+# The sweep on segmented FMM plans (eb_sweep_set_mode, eb_sweep_field_set_mode): one segment
+# per simulation, each binned with its own bounding box and scale.
+# ------------------------------------------------------------------------------------------
+
+def test_sweep_fmm_mode_matches_direct(torch, mods):
+    """The operator (CADAPDR's kernel per simulation) on segmented plans against each
+    simulation's O(N^2) sum, simulations of one and of several drops; the solves agree."""
+    gm, sp = mods
+    nd, rku, sim_off = _mixed()
+    B, geo, sw = _sweep(torch, sp, nd, rku, sim_off)
+    u = torch.tensor(np.random.default_rng(4).normal(size=2 * int(nd.sum())), device="cuda")
+    sw.set_mode(2)
+    yd = sw.apply(u).cpu().numpy()
+    sw.set_mode(1)
+    yf = sw.apply(u).cpu().numpy()
+    e = float(np.linalg.norm(yf - yd) / np.linalg.norm(yd))
+    print(f"sweep operator, segmented FMM against the direct sums: {e:.2e}", flush=True)
+    assert e < 1e-11
+    gb = gm.GMRESBatch(sw.unknown_offsets, 100)
+    rhs = torch.tensor(yd, device="cuda")
+    xf, _, _, itf, _, ierr = gm.sweep_solve(sw, gb, rhs)
+    sw.set_mode(2)
+    xd, _, _, itd, _, _ = gm.sweep_solve(sw, gb, rhs)
+    assert np.all(ierr == 0) and np.all(np.abs(itf - itd) <= 1)
+    assert float((xf - xd).norm() / xd.norm()) < 1e-9
+    gb.close()
+
+
+def _big_sims():
+    """Three simulations of five 2048-point drops (10,240 points each, the direct limit) and one
+    of six (12,288 points): the default mode takes the segmented FMM."""
+    rng = np.random.default_rng(13)
+    nds, zs, sim_off = [], [], [0]
+    for s, nb in enumerate([5, 6, 5, 5]):
+        for b in range(nb):
+            c = 3.0 * b + 3.0j * (b % 2) + 0.1 * s
+            zs.append(perturbed_circle(2048, [(2, 0.05), (3, rng.uniform(-0.02, 0.02))], center=c)[0])
+            nds.append(2048)
+        sim_off.append(sim_off[-1] + nb)
+    return np.array(nds, np.int32), np.concatenate(zs).view(np.float64).copy(), np.array(sim_off, np.int32)
+
+
+@pytest.mark.parametrize("case", ["mixed", "big"])
+def test_sweep_field_fmm_mode_matches_direct(torch, case):
+    """F of every simulation with all its layer potentials (Laplace and PVINTEV Hilbert products,
+    the Stokes operator, CADAPMOD) on segmented plans against the O(N^2) mode (1e-8, the
+    solves' tolerance); for simulations above 10,240 points the default mode is the FMM's."""
+    from electricbubbles_amd import field, spectral
+    nd, rku, sim_off = _mixed() if case == "mixed" else _big_sims()
+    nsim = sim_off.size - 1
+    cap = np.linspace(0.01, 0.3, nsim)
+    e0 = np.linspace(0.5, 2.0, nsim)
+    y = torch.tensor(rku, device="cuda")
+    out = {}
+    for mode in (2, 1, 0):
+        SF = field.SweepField(spectral.Bodies(nd), np.asarray(sim_off), cap, e0)
+        SF.set_mode(mode)
+        out[mode] = SF.rhs(y).cpu().numpy()
+        out[(mode, "it")] = SF.iterations()
+        SF.close()
+    e = float(np.linalg.norm(out[1] - out[2]) / np.linalg.norm(out[2]))
+    print(f"{case}: F on segmented FMM plans against the direct sums {e:.2e}; iterations "
+          f"{out[(1, 'it')][2].tolist()} / {out[(2, 'it')][2].tolist()}", flush=True)
+    assert e < 1e-8
+    for a, b in zip(out[(1, "it")], out[(2, "it")]):
+        assert np.all(np.abs(a - b) <= 1)
+    if case == "big":
+        np.testing.assert_array_equal(out[0], out[1])
+    else:
+        np.testing.assert_array_equal(out[0], out[2])
+
+
+def test_sweep_field_fmm_graph_step_matches_eager(torch):
+    """The captured RK4 step of the sweep in FMM mode (segmented plans re-binned and applied
+    inside the graph) against the eager step."""
+    from electricbubbles_amd import field, spectral
+    nd, rku, sim_off = _mixed()
+    nsim = sim_off.size - 1
+    st = torch.cuda.Stream()
+    SG = field.SweepField(spectral.Bodies(nd, stream=st), np.asarray(sim_off), 0.1, 1.0, stream=st)
+    SE = field.SweepField(spectral.Bodies(nd, stream=st), np.asarray(sim_off), 0.1, 1.0, stream=st)
+    SG.set_mode(1)
+    SE.set_mode(1)
+    dt = 1e-3
+    with torch.cuda.stream(st):
+        yg = torch.tensor(rku, device="cuda")
+        ye = yg.clone()
+        for step in range(3):
+            assert not SG.rk4_graph(yg, dt, budget=40)
+            acc, ys = torch.empty_like(ye), torch.empty_like(ye)
+            for q in range(4):
+                spectral.rk4_stage(q, dt, ye, SE.rhs(ye if q == 0 else ys), acc, ys, stream=st)
+            st.synchronize()
+            d = float((yg - ye).norm() / ye.norm())
+            print("FMM-mode sweep rk4 step", step, "rel diff", d, flush=True)
+            assert d < 1e-12
+    assert nsim == 5
