@@ -511,6 +511,48 @@ def sweep_leg(torch, dev, with_cpu):
             out["F"]["cpu_reference"] = {"F_per_s": round(n_rf / t_rf, 2), "cores": 1, "kind": "reference",
                                          "sample": f"the reference's F of {n_rf} of the sampled simulations"}
     gb.close()
+    out["segmented_fmm"] = sweep_large(torch, dev)
+    return out
+
+
+def sweep_large(torch, dev):
+    """Batched simulations too large for the O(N^2) sums: 16 simulations of eight 2048-point
+    drops (16,384 points each, 262,144 in all), whole F of all of them in one call, with every
+    layer potential on segmented FMM plans (one segment per simulation; eb_sweep_field_set_mode)
+    against the same F with each simulation's O(N^2) sums."""
+    from electricbubbles_amd import field, spectral
+    from electricbubbles_amd.workloads import perturbed_circle
+    rng = np.random.default_rng(20261204)
+    nds, zs, sim_off = [], [], [0]
+    for s in range(16):
+        for b in range(8):
+            c = 3.0 * (b % 4) + 3.0j * (b // 4)
+            zs.append(perturbed_circle(2048, [(2, 0.05), (3, rng.uniform(-0.02, 0.02))], center=c)[0])
+            nds.append(2048)
+        sim_off.append(sim_off[-1] + 8)
+    nd = np.array(nds, np.int32)
+    rku = torch.tensor(np.concatenate(zs).view(np.float64).copy(), device=dev)
+    out = {"workload": "16 simulations x 8 drops x 2048 points (16,384 per simulation), E0 = 1, CAP = 0.1"}
+    res = {}
+    for mode, name in ((1, "fmm"), (2, "direct")):
+        SF = field.SweepField(spectral.Bodies(nd), np.array(sim_off), 0.1, 1.0)
+        SF.set_mode(mode)
+        SF.rhs(rku)  # plans, warm-up
+        torch.cuda.synchronize()
+        reps = 3 if mode == 1 else 1
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            f = SF.rhs(rku)
+        torch.cuda.synchronize()
+        t = (time.perf_counter() - t0) / reps
+        lap_it, _, st_it, _ = SF.iterations()
+        res[name] = f
+        out[name] = {"ms_per_F_of_all": round(t * 1e3, 2), "F_per_s": round(16 / t, 1),
+                     "laplace_iterations_mean": round(float(lap_it.mean()), 2),
+                     "stokes_iterations_mean": round(float(st_it.mean()), 2)}
+        SF.close()
+    out["fmm_vs_direct"] = round(out["direct"]["ms_per_F_of_all"] / out["fmm"]["ms_per_F_of_all"], 2)
+    out["rel_diff"] = float((res["fmm"] - res["direct"]).norm() / res["direct"].norm())
     return out
 
 
