@@ -697,11 +697,14 @@ __device__ __forceinline__ void xlat_wave(const double* Cs, const double2* Ps, c
       for (int r = 0; r < TR; ++r) ct[ks][r] = Cs[(4 * ks + kr) * CS + 16 * MT + r];
   }
   // One source: prescale the loaded expansion (cur), start loading the next source into the
-  // other register set (nxt; in flight during the products), the Pascal product on the
-  // matrix cores, then the post-scaling.  The empty asm statement is a compiler barrier for
-  // memory that keeps the Pascal tile in LDS, re-read per source instead of held in
-  // 2*KS*MT registers.
-  auto step = [&](int o, const double2* cur, double2* nxt) {
+  // same registers (the prescaled copy xb is all the step needs; the loads are in flight
+  // during the products: a second register set measured slower, 0.479 against 0.469 ms at
+  // 2^20 points, p = 36), read all of the step's post-scaling weights at once (read one by
+  // one beside their FMAs, each waited for, they cost 0.457 against 0.469 ms), the Pascal
+  // product on the matrix cores, then the post-scaling.  The empty asm statement is a
+  // compiler barrier for memory that keeps the Pascal tile in LDS, re-read per source
+  // instead of held in 2*KS*MT registers.
+  auto step = [&](int o, double2* cur) {
     __asm__ volatile("" ::: "memory");
     double xb[KS];
 #pragma unroll
@@ -709,7 +712,16 @@ __device__ __forceinline__ void xlat_wave(const double* Cs, const double2* Ps, c
       const double2 w = Ps[2 * (o * NK + 4 * ks + kr) + comp];
       xb[ks] = fma(w.x, cur[ks].x, w.y * cur[ks].y);
     }
-    if (o + 1 < NS) load_src(o + 1, nxt);
+    if (o + 1 < NS) load_src(o + 1, cur);
+    double2 qw[MT][4];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) qw[mt][r] = Qs[o * NM + 16 * mt + kr + 4 * r];
+    double2 tw[TR ? TR : 1];
+#pragma unroll
+    for (int r = 0; r < TR; ++r) tw[r] = Qs[o * NM + 16 * MT + r];
+    __builtin_amdgcn_sched_barrier(0);
     v4d acc[MT];
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt) acc[mt] = v4d{0.0, 0.0, 0.0, 0.0};
@@ -737,7 +749,7 @@ __device__ __forceinline__ void xlat_wave(const double* Cs, const double2* Ps, c
         }
 #pragma unroll
       for (int r = 0; r < TR; ++r) {
-        const double2 w = Qs[o * NM + 16 * MT + r];
+        const double2 w = tw[r];
         ta[r] = fma(w.x, part[r], ta[r]);
         tb[r] = fma(w.y, part[r], tb[r]);
       }
@@ -746,20 +758,15 @@ __device__ __forceinline__ void xlat_wave(const double* Cs, const double2* Ps, c
     for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const double2 w = Qs[o * NM + 16 * mt + kr + 4 * r];
+        const double2 w = qw[mt][r];
         sa[mt][r] = fma(w.x, acc[mt][r], sa[mt][r]);
         sb[mt][r] = fma(w.y, acc[mt][r], sb[mt][r]);
       }
   };
-  double2 ra[KS], rb[KS];  // two register sets in turn: the prefetch needs no copy
+  double2 ra[KS];
   load_src(0, ra);
-  int o = 0;
 #pragma unroll 1
-  for (; o + 1 < NS; o += 2) {
-    step(o, ra, rb);
-    step(o + 1, rb, ra);
-  }
-  if (o < NS) step(o, ra, rb);
+  for (int o = 0; o < NS; ++o) step(o, ra);
   const uint32_t sgn_hi = comp ? 0u : 0x80000000u;  // A - B' on component 0, A + B' on 1
   double lam[MT][4];
 #pragma unroll
@@ -1204,22 +1211,40 @@ __device__ __forceinline__ void p2p_item(int itemid, int L, const int* __restric
   const int ts = ls + item.y, te = ls + item.z;                  // this block's targets
   const int T = te - ts;
   const int lane = threadIdx.x;
-  int sb[9], cum[10];
-  int nb = 0, self0 = 0;
-  cum[0] = 0;
-  // MKNBOR order: column-outer, row-inner (cadap2.f:734-740)
-  for (int dx = -1; dx <= 1; ++dx)
-    for (int dy = -1; dy <= 1; ++dy) {
-      const int jx = ix + dx, jy = iy + dy;
-      if (jx < 0 || jy < 0 || jx >= nside || jy >= nside) continue;
-      if (of.segk && (((jx ^ ix) | (jy ^ iy)) >> (L - of.segk))) continue;  // another segment's leaf
-      const uint32_t m = morton((uint32_t)jx, (uint32_t)jy);
-      sb[nb] = leaf_start[m];
-      cum[nb + 1] = cum[nb] + (leaf_start[m + 1] - sb[nb]);
-      if (dx == 0 && dy == 0) self0 = cum[nb];  // flat position of the leaf's own points
-      ++nb;
-    }
-  const int S = cum[nb];
+  // The 3x3 neighbourhood in MKNBOR order (column-outer, row-inner, cadap2.f:734-740) as nine
+  // fixed slots, wave-uniform (scalar registers): ck = the points of slots 0..k-1 (S: all
+  // nine), dk = slot k's first sorted index minus its first flat position; a slot outside
+  // the grid (or in another segment) holds no points.  Named values, not arrays: an array
+  // indexed per lane became a waterfall loop over the lanes' slots.
+  int c1, c2, c3, c4, c5, c6, c7, c8, S;
+  int d0, d1, d2, d3, d4, d5, d6, d7, d8;
+  {
+    int run = 0;
+    auto slot = [&](int k, int& dk) {
+      const int jx = ix + k / 3 - 1, jy = iy + k % 3 - 1;
+      bool in = jx >= 0 && jy >= 0 && jx < nside && jy < nside;
+      if (of.segk) in = in && ((((jx ^ ix) | (jy ^ iy)) >> (L - of.segk)) == 0);
+      int s0 = 0, n0 = 0;
+      if (in) {
+        const uint32_t m = morton((uint32_t)jx, (uint32_t)jy);
+        s0 = leaf_start[m];
+        n0 = leaf_start[m + 1] - s0;
+      }
+      dk = __builtin_amdgcn_readfirstlane(s0 - run);
+      run += n0;
+      return __builtin_amdgcn_readfirstlane(run);
+    };
+    c1 = slot(0, d0);
+    c2 = slot(1, d1);
+    c3 = slot(2, d2);
+    c4 = slot(3, d3);
+    c5 = slot(4, d4);
+    c6 = slot(5, d5);
+    c7 = slot(6, d6);
+    c8 = slot(7, d7);
+    S = slot(8, d8);
+  }
+  const int self0 = c4;
   const double rs = out_scale(of, g, mk, L);
   // each lane owns two consecutive targets (register blocking halves the LDS reads per
   // pair); with TP = ceil(T/2) < 64 target pairs the lanes split into G = 64/TP slices
@@ -1240,14 +1265,45 @@ __device__ __forceinline__ void p2p_item(int itemid, int L, const int* __restric
     for (int c0 = 0; c0 < S; c0 += P2P_CH) {
       const int n = min(P2P_CH, S - c0);
       __syncthreads();
-      for (int f = lane; f < n; f += 64) {
-        const int flat = c0 + f;
-        int q = 0;
-        while (q + 1 < nb && flat >= cum[q + 1]) ++q;
-        const int j = sb[q] + (flat - cum[q]);
-        sz[f] = zs[j];
-        sa[f] = qa[j];
-        sq[f] = qb[j];
+      // each lane stages flat positions lane and lane + 64 of the chunk; the slot of a flat
+      // position is the last slot starting at or before it (empty slots are passed over)
+      static_assert(P2P_CH == 128, "two staged points per lane");
+      // a sum of steps rather than a select chain, which the compiler would turn into a
+      // table lookup through private memory
+      auto src_of = [&](int flat) {
+        int j = flat + d0;
+        j += flat >= c1 ? d1 - d0 : 0;
+        j += flat >= c2 ? d2 - d1 : 0;
+        j += flat >= c3 ? d3 - d2 : 0;
+        j += flat >= c4 ? d4 - d3 : 0;
+        j += flat >= c5 ? d5 - d4 : 0;
+        j += flat >= c6 ? d6 - d5 : 0;
+        j += flat >= c7 ? d7 - d6 : 0;
+        j += flat >= c8 ? d8 - d7 : 0;
+        return j;
+      };
+      const bool u0 = lane < n, u1 = lane + 64 < n;
+      const int j0 = u0 ? src_of(c0 + lane) : 0, j1 = u1 ? src_of(c0 + lane + 64) : 0;
+      double2 gz0, ga0, gq0, gz1, ga1, gq1;
+      if (u0) {
+        gz0 = zs[j0];
+        ga0 = qa[j0];
+        gq0 = qb[j0];
+      }
+      if (u1) {
+        gz1 = zs[j1];
+        ga1 = qa[j1];
+        gq1 = qb[j1];
+      }
+      if (u0) {
+        sz[lane] = gz0;
+        sa[lane] = ga0;
+        sq[lane] = gq0;
+      }
+      if (u1) {
+        sz[lane + 64] = gz1;
+        sa[lane + 64] = ga1;
+        sq[lane + 64] = gq1;
       }
       __syncthreads();
       if (act) {
@@ -1255,21 +1311,23 @@ __device__ __forceinline__ void p2p_item(int itemid, int L, const int* __restric
         // the target itself, so only that range runs the masked pair
         const int fs0 = fself0 - c0, fs1 = fself1 - c0;
         const int sa0 = min(max(self0 - c0, 0), n), sb0 = min(max(self0 + TL - c0, 0), n);
-        auto first = [&](int lo) { return lo + ((gs - lo) % G + G) % G; };  // first f >= lo in this slice
+        // this slice's positions gs, gs + G, ... in three runs: before, inside and after the
+        // leaf's own points
+        int f = gs;
 #pragma unroll 2
-        for (int f = first(0); f < sa0; f += G) {
+        for (; f < sa0; f += G) {
           const double2 zj = sz[f], qj = sa[f], bj = sq[f];
           p2p_pair<false, true>(z0, zj, qj, bj, false, a10, a20);
           p2p_pair<false, true>(z1, zj, qj, bj, false, a11, a21);
         }
 #pragma unroll 2
-        for (int f = first(sa0); f < sb0; f += G) {
+        for (; f < sb0; f += G) {
           const double2 zj = sz[f], qj = sa[f], bj = sq[f];
           p2p_pair<true, true>(z0, zj, qj, bj, f == fs0, a10, a20);
           p2p_pair<true, true>(z1, zj, qj, bj, f == fs1, a11, a21);
         }
 #pragma unroll 2
-        for (int f = first(sb0); f < n; f += G) {
+        for (; f < n; f += G) {
           const double2 zj = sz[f], qj = sa[f], bj = sq[f];
           p2p_pair<false, true>(z0, zj, qj, bj, false, a10, a20);
           p2p_pair<false, true>(z1, zj, qj, bj, false, a11, a21);
