@@ -650,7 +650,7 @@ __device__ __forceinline__ double swap_adjacent(double v) {
 // source level (src, nsrc_boxes boxes; an index past it reads zeros); live lanes write.
 // Cs, Ps and Qs are the LDS tables (Ps and Qs at this wave's rows).  No barrier inside: a
 // kernel may call it for several target sets between its own barriers.
-template <int MODE, int KMAX, int KS, int MT, int TR, typename SrcBox>
+template <int MODE, int KMAX, int KS, int MT, int TR, bool QE = true, typename SrcBox>
 __device__ __forceinline__ void xlat_wave(const double* Cs, const double2* Ps, const double2* Qs, int pt, bool live,
                                           SrcBox src_box, const double2* __restrict__ src, int nsrc_boxes,
                                           double2* __restrict__ dst) {
@@ -681,20 +681,37 @@ __device__ __forceinline__ void xlat_wave(const double* Cs, const double2* Ps, c
   for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
     for (int r = 0; r < 4; ++r) sa[mt][r] = sb[mt][r] = 0.0;
-  // tail rows: this lane's partial sums over its k rows (kr + 4 ks), reduced over the four
-  // k-row lanes of its column after the last source
+  // Tail rows 0..3 (TR = 4; the tiles then hold rows 4..4 + 16 MT - 1): this lane's partial
+  // sums over its k rows k = 4 ks + kr, reduced over the four k-row lanes of its column after
+  // the last source.  In these rows the entries are polynomials of degree r in ks for fixed
+  // kr: C(r + k, k) (M2L) and C(k, r) (L2L), so sum_ks Mat[k][r] xb[ks] =
+  // sum_d et[r][d] B_d with the binomial moments B_d = sum_ks C(ks, d) xb[ks] (integer
+  // constants) and et[r][d] the d-th forward difference over ks of Mat[4 ks + kr][r] at 0
+  // (exact integers): ten coefficients per lane instead of the 4 KS table entries.  M2M's
+  // rows 0..3 have C(n, k) = 0 for k > n: only ks = 0 contributes.
   double ta[TR ? TR : 1], tb[TR ? TR : 1];
 #pragma unroll
   for (int r = 0; r < TR; ++r) ta[r] = tb[r] = 0.0;
-  // the tail's Pascal entries of this lane's k rows, the same for every source: held in
-  // registers for a 4-row tail, read from LDS per source for an 8-row one
-  constexpr bool CT_REG = TR == 4;
-  double ct[CT_REG ? KS : 1][CT_REG ? TR : 1];
-  if constexpr (CT_REG) {
+  double et[TR ? 10 : 1];
+  if constexpr (TR > 0) {
+    if constexpr (MODE == XL_M2M) {
 #pragma unroll
-    for (int ks = 0; ks < KS; ++ks)
+      for (int r = 0; r < 4; ++r) et[r] = Cs[kr * CS + r];
+    } else {
+      int i = 0;
 #pragma unroll
-      for (int r = 0; r < TR; ++r) ct[ks][r] = Cs[(4 * ks + kr) * CS + 16 * MT + r];
+      for (int r = 0; r < 4; ++r) {
+        double v[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[q] = Cs[(4 * q + kr) * CS + r];
+#pragma unroll
+        for (int d = 0; d <= r; ++d) {
+          et[i++] = v[0];
+#pragma unroll
+          for (int q = 0; q + 1 < 4 - d; ++q) v[q] = v[q + 1] - v[q];
+        }
+      }
+    }
   }
   // One source: prescale the loaded expansion (cur), start loading the next source into the
   // same registers (the prescaled copy xb is all the step needs; the loads are in flight
@@ -714,14 +731,16 @@ __device__ __forceinline__ void xlat_wave(const double* Cs, const double2* Ps, c
     }
     if (o + 1 < NS) load_src(o + 1, cur);
     double2 qw[MT][4];
-#pragma unroll
-    for (int mt = 0; mt < MT; ++mt)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) qw[mt][r] = Qs[o * NM + 16 * mt + kr + 4 * r];
     double2 tw[TR ? TR : 1];
+    if constexpr (QE) {
 #pragma unroll
-    for (int r = 0; r < TR; ++r) tw[r] = Qs[o * NM + 16 * MT + r];
-    __builtin_amdgcn_sched_barrier(0);
+      for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) qw[mt][r] = Qs[o * NM + TR + 16 * mt + kr + 4 * r];
+#pragma unroll
+      for (int r = 0; r < TR; ++r) tw[r] = Qs[o * NM + r];
+      __builtin_amdgcn_sched_barrier(0);
+    }
     v4d acc[MT];
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt) acc[mt] = v4d{0.0, 0.0, 0.0, 0.0};
@@ -729,27 +748,30 @@ __device__ __forceinline__ void xlat_wave(const double* Cs, const double2* Ps, c
     for (int ks = 0; ks < KS; ++ks)
 #pragma unroll
       for (int mt = 0; mt < MT; ++mt)
-        acc[mt] = __builtin_amdgcn_mfma_f64_16x16x4f64(Cs[(4 * ks + kr) * CS + 16 * mt + col], xb[ks], acc[mt],
+        acc[mt] = __builtin_amdgcn_mfma_f64_16x16x4f64(Cs[(4 * ks + kr) * CS + TR + 16 * mt + col], xb[ks], acc[mt],
                                                        0, 0, 0);
     if constexpr (TR > 0) {
-      double part[TR];
+      double part[4];
+      if constexpr (MODE == XL_M2M) {
 #pragma unroll
-      for (int r = 0; r < TR; ++r) part[r] = 0.0;
+        for (int r = 0; r < 4; ++r) part[r] = et[r] * xb[0];
+      } else {
+        double b0 = xb[0], b1 = 0.0, b2 = 0.0, b3 = 0.0;
 #pragma unroll
-      for (int ks = 0; ks < KS; ++ks)
-#pragma unroll
-        for (int r = 0; r < TR; r += 2) {
-          double2 cc;
-          if constexpr (CT_REG)
-            cc = make_double2(ct[ks][r], ct[ks][r + 1]);
-          else
-            cc = *(const double2*)&Cs[(4 * ks + kr) * CS + 16 * MT + r];
-          part[r] = fma(cc.x, xb[ks], part[r]);
-          part[r + 1] = fma(cc.y, xb[ks], part[r + 1]);
+        for (int ks = 1; ks < KS; ++ks) {
+          b0 += xb[ks];
+          b1 = fma((double)ks, xb[ks], b1);
+          if (ks >= 2) b2 = fma((double)(ks * (ks - 1) / 2), xb[ks], b2);
+          if (ks >= 3) b3 = fma((double)(ks * (ks - 1) * (ks - 2) / 6), xb[ks], b3);
         }
+        part[0] = et[0] * b0;
+        part[1] = fma(et[1], b0, et[2] * b1);
+        part[2] = fma(et[3], b0, fma(et[4], b1, et[5] * b2));
+        part[3] = fma(et[6], b0, fma(et[7], b1, fma(et[8], b2, et[9] * b3)));
+      }
 #pragma unroll
       for (int r = 0; r < TR; ++r) {
-        const double2 w = tw[r];
+        const double2 w = QE ? tw[r] : Qs[o * NM + r];
         ta[r] = fma(w.x, part[r], ta[r]);
         tb[r] = fma(w.y, part[r], tb[r]);
       }
@@ -758,7 +780,7 @@ __device__ __forceinline__ void xlat_wave(const double* Cs, const double2* Ps, c
     for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const double2 w = qw[mt][r];
+        const double2 w = QE ? qw[mt][r] : Qs[o * NM + TR + 16 * mt + kr + 4 * r];
         sa[mt][r] = fma(w.x, acc[mt][r], sa[mt][r]);
         sb[mt][r] = fma(w.y, acc[mt][r], sb[mt][r]);
       }
@@ -804,13 +826,13 @@ __device__ __forceinline__ void xlat_wave(const double* Cs, const double2* Ps, c
     for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int m = 16 * mt + kr + 4 * r;
+        const int m = TR + 16 * mt + kr + 4 * r;
         const size_t at = 2 * ((size_t)pt * 2 * KMAX + s * KMAX + m) + comp;
         out[at] = MODE == XL_L2L ? out[at] + lam[mt][r] : lam[mt][r];
       }
 #pragma unroll
     for (int q = 0; q < TR / 4; ++q) {
-      const int m = 16 * MT + kr + 4 * q;
+      const int m = kr + 4 * q;
       const size_t at = 2 * ((size_t)pt * 2 * KMAX + s * KMAX + m) + comp;
       out[at] = MODE == XL_L2L ? out[at] + tlam[q] : tlam[q];
     }
@@ -822,18 +844,20 @@ __device__ __forceinline__ void xlat_wave(const double* Cs, const double2* Ps, c
 // non-empty target boxes are translated, which matters where most boxes are empty (drops
 // scattered over a large square).  Blocks past the class's count leave at once.
 //
-// Output rows: MT tiles of 16 on the matrix cores, then TR (0, 4 or 8) tail rows on the
-// vector ALUs.  A 16x16x4 FP64 MFMA costs as much as 64 vector FMAs per lane; the tail costs
-// KS + 2 per row and source, so p = 33..36 pays for 36 rows instead of 48.  Rows past
-// 16 MT + TR are not written: they are zero for every p <= 16 MT + TR (the tables vanish
-// there), and no reader goes past row 4 KS - 1 (translations) or p - 1 (k_l2p).
+// Output rows: with TR = 4, rows 0..3 on the vector ALUs (from four binomial moments of the
+// lane's inputs, see xlat_wave) and MT tiles of 16 on the matrix cores for rows 4..; with
+// TR = 0, the tiles alone from row 0.  A 16x16x4 FP64 MFMA costs as much as 64 vector FMAs
+// per lane and issues on the same SIMD (they do not overlap); the tail costs about
+// 3 KS + 18 per source, so p = 33..36 pays for 36 rows instead of 48.  Rows past 16 MT + TR
+// are not written: they are zero for every p <= 16 MT + TR (the tables vanish there), and no
+// reader goes past row 4 KS - 1 (translations) or p - 1 (k_l2p).
 //
 // ADAPT: boxes of an adaptive tree (the adaptive plan below), stored box-major by box index.
 // Targets come from per-class lists (m2l_list / m2l_count / m2l_cap, no level runs: the
 // scaled translations do not depend on the level), sources from a slot table stab[target * NS
 // + slot]; a slot without a box holds nsrc, one past the last box, which the buffer resource
 // reads as zeros.
-template <int MODE, int KMAX, int KS, int MT, int TR, bool ADAPT = false, bool SEG = false>
+template <int MODE, int KMAX, int KS, int MT, int TR, bool ADAPT = false, bool SEG = false, bool QE = true>
 __device__ __forceinline__ void xlat_body(const XlatLevels& LS, const int* __restrict__ leaf_start,
                                           const double2* __restrict__ src, double2* __restrict__ dst,
                                           const double* __restrict__ Mat, const double2* __restrict__ pre,
@@ -841,7 +865,7 @@ __device__ __forceinline__ void xlat_body(const XlatLevels& LS, const int* __res
                                           const int* __restrict__ m2l_list, const int* __restrict__ m2l_count,
                                           int m2l_cap, const int* __restrict__ stab = nullptr, int nsrc = 0) {
   static_assert(KS * 4 <= KMAX && MT * 16 + TR <= KMAX, "tile exceeds the padded expansion length");
-  static_assert(TR % 4 == 0 && TR <= 8 && KS * 4 <= MT * 16 + TR, "tail rows must cover the expansion");
+  static_assert((TR == 0 || TR == 4) && KS * 4 <= MT * 16 + TR, "tail rows must cover the expansion");
   constexpr int NS = MODE == XL_M2L ? 27 : (MODE == XL_M2M ? 4 : 1);
   constexpr int NM = MT * 16 + TR, NK = KS * 4;
   // row stride of the Pascal tile: 2 CS = 32 (mod 64) dwords puts the four k rows that one
@@ -854,23 +878,26 @@ __device__ __forceinline__ void xlat_body(const XlatLevels& LS, const int* __res
   // XCD-aware order: consecutive logical blocks (neighbouring rows, all four parity classes)
   // are dispatched to the same XCD and share its L2.
   const bool compact = ADAPT || (MODE == XL_M2L && m2l_list != nullptr);
+  // waves per block: 4 on every path; the compacted M2L also runs with 12 (tpb = 4 wpb
+  // targets of one class per block)
+  const int wpb = blockDim.x >> 6, tpb = 4 * wpb;
   int nphys = gridDim.x;
   if (compact) {
     // logical blocks that can hold entries: 4 classes x the longest class's blocks, so that
     // the XCD bands below spread them over all eight XCDs
     int mx = 0;
-    for (int c = 0; c < 4; ++c) mx = max(mx, (m2l_count[c] + 15) / 16);
+    for (int c = 0; c < 4; ++c) mx = max(mx, (m2l_count[c] + tpb - 1) / tpb);
     nphys = (4 * mx + 7) / 8 * 8;
     if ((int)blockIdx.x >= nphys) return;  // the whole block: before any barrier
   }
   const int b = (blockIdx.x % 8) * (nphys / 8) + blockIdx.x / 8;
   const int cls = b & 3;
   const int ccount = compact ? m2l_count[cls] : 0;
-  if (compact && 16 * (b >> 2) >= ccount) return;
+  if (compact && tpb * (b >> 2) >= ccount) return;
   if constexpr (ADAPT) {
     // the slot table's rows of this block's targets, read once instead of per source step
     for (int i = threadIdx.x; i < 16 * NS; i += blockDim.x) {
-      const int idx = 16 * (b >> 2) + i / NS;
+      const int idx = tpb * (b >> 2) + i / NS;
       const int ent = idx < ccount ? m2l_list[(size_t)cls * m2l_cap + idx] : -1;
       Sx[i] = ent >= 0 ? stab[(size_t)ent * NS + i % NS] : nsrc;
     }
@@ -901,7 +928,7 @@ __device__ __forceinline__ void xlat_body(const XlatLevels& LS, const int* __res
   int e = 0, q;
   bool valid;
   if (compact) {
-    const int idx = 16 * (b >> 2) + 4 * wv + t;
+    const int idx = tpb * (b >> 2) + 4 * wv + t;
     const int ent = idx < ccount ? m2l_list[(size_t)cls * m2l_cap + idx] : -1;
     valid = ent >= 0;
     if constexpr (ADAPT) {
@@ -951,7 +978,7 @@ __device__ __forceinline__ void xlat_body(const XlatLevels& LS, const int* __res
     if (MODE == XL_M2M) return pidx(2 * ix + (o & 1), 2 * iy + (o >> 1), Ws);
     return pidx(ix >> 1, iy >> 1, Ws);
   };
-  xlat_wave<MODE, KMAX, KS, MT, TR>(Cs, Ps, Qs, pt, live, src_box, ADAPT ? src : src + LS.offs[e], ADAPT ? nsrc : NBPs,
+  xlat_wave<MODE, KMAX, KS, MT, TR, QE>(Cs, Ps, Qs, pt, live, src_box, ADAPT ? src : src + LS.offs[e], ADAPT ? nsrc : NBPs,
                                     ADAPT ? dst : dst + LS.offt[e]);
 }
 
