@@ -1119,16 +1119,30 @@ __global__ __launch_bounds__(L2P_T) void k_l2p(int L, int p, const int4* __restr
                                                const double2* __restrict__ lop, int Wp, OutForm of,
                                                const int32_t* __restrict__ perm, const Geom* __restrict__ g,
                                                double2* __restrict__ out) {
-  __shared__ double2 C[5][2][KMAX];  // the four leaves' locals (child c = x bit | y bit << 1), the parent's
+  // the four leaves' locals (child c = x bit | y bit << 1) and the parent's, both series in a
+  // row padded by two entries: the lanes of a wave read four leaves' rows at once, and rows
+  // 2 KMAX entries apart (384 dwords at K = 48) would all start on the same LDS bank
+  __shared__ double2 C[5][2 * KMAX + 2];
   if ((int)blockIdx.x >= *nitems) return;
   const int4 it = items[blockIdx.x];
   const int px = (int)compact2((uint32_t)it.x), py = (int)compact2((uint32_t)it.x >> 1);
-  const int nb = PARENT ? 5 : 4;
-  for (int t = threadIdx.x; t < nb * 2 * p; t += L2P_T) {
-    const int b = t / (2 * p), r = t - b * 2 * p, sr = r / p, m = r - sr * p;
+  // lane `row` stages that row of each box's two series: all loads first, then all stores
+  static_assert(KMAX <= L2P_T, "one lane per expansion row");
+  constexpr int NB = PARENT ? 5 : 4;
+  const int row = threadIdx.x;
+  double2 v[NB][2];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
     const double2* src = b < 4 ? lo + (size_t)pidx(2 * px + (b & 1), 2 * py + (b >> 1), W) * 2 * KMAX
                                : lop + (size_t)pidx(px, py, Wp) * 2 * KMAX;
-    C[b][sr][m] = src[sr * KMAX + m];
+#pragma unroll
+    for (int sr = 0; sr < 2; ++sr) v[b][sr] = row < p ? src[sr * KMAX + row] : make_double2(0, 0);
+  }
+  if (row < p) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+      for (int sr = 0; sr < 2; ++sr) C[b][sr * KMAX + row] = v[b][sr];
   }
   __syncthreads();
   const int i = it.y + (int)threadIdx.x;
@@ -1148,12 +1162,12 @@ __global__ __launch_bounds__(L2P_T) void k_l2p(int L, int p, const int4* __restr
 #pragma unroll 4
   for (int m = p - 1; m >= 0; --m) {
     dphi = cmuladd(dphi, u, phi);
-    phi = cmuladd(phi, u, C[c][0][m]);
-    psi = cmuladd(psi, u, C[c][1][m]);
+    phi = cmuladd(phi, u, C[c][m]);
+    psi = cmuladd(psi, u, C[c][KMAX + m]);
     if (PARENT) {
       df = cmuladd(df, up, f);
-      f = cmuladd(f, up, C[4][0][m]);
-      gq = cmuladd(gq, up, C[4][1][m]);
+      f = cmuladd(f, up, C[4][m]);
+      gq = cmuladd(gq, up, C[4][KMAX + m]);
     }
   }
   double2 phip = cscale(dphi, ir);
