@@ -22,6 +22,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <array>
 #include <mutex>
 #include <vector>
@@ -650,7 +651,7 @@ __device__ __forceinline__ double swap_adjacent(double v) {
 // source level (src, nsrc_boxes boxes; an index past it reads zeros); live lanes write.
 // Cs, Ps and Qs are the LDS tables (Ps and Qs at this wave's rows).  No barrier inside: a
 // kernel may call it for several target sets between its own barriers.
-template <int MODE, int KMAX, int KS, int MT, int TR, bool QE = true, typename SrcBox>
+template <int MODE, int KMAX, int KS, int MT, int TR, typename SrcBox>
 __device__ __forceinline__ void xlat_wave(const double* Cs, const double2* Ps, const double2* Qs, int pt, bool live,
                                           SrcBox src_box, const double2* __restrict__ src, int nsrc_boxes,
                                           double2* __restrict__ dst) {
@@ -667,12 +668,23 @@ __device__ __forceinline__ void xlat_wave(const double* Cs, const double2* Ps, c
       (void*)src, (short)0, (int)(uint32_t)((size_t)2 * KMAX * nsrc_boxes * 16), 0x00020000);
   const int row_bytes = 4 * 16;  // k -> k + 4 within the box's coefficients
   const int lane_row = (s * KMAX + kr) * 16;
-  auto load_src = [&](int o, double2* a) {
+  // the first n k steps of source o
+  auto load_src = [&](int o, double2* a, auto n) {
+    constexpr int N = decltype(n)::value;
     const int off = lane_row + src_box(o) * (2 * KMAX * 16);
 #pragma unroll
-    for (int ks = 0; ks < KS; ++ks)
+    for (int ks = 0; ks < N; ++ks)
       a[ks] = __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(rsrc, off, ks * row_bytes, 0));
   };
+  // M2L: the sources after the first M2L_NEAR (m2l_offsets: convergence ratio at most 1/2,
+  // against sqrt(2)/2 for the first group) take only the first KSF = ceil(2 KS / 3) k steps
+  // of their expansions, with every output row kept: 24 of the 36 terms at p = 33..36, where
+  // the 2^20-point circles stay at 8.2e-12 against 7.1e-12 with all terms (20 terms: 4e-10;
+  // keeping 20 output rows instead, whatever the input: 4e-10).  The M2L's matrix-core work
+  // drops by 28 %: 459 -> 380 us at 2^20 points, p = 36.
+  constexpr bool SPLIT = MODE == XL_M2L;
+  constexpr int NNEAR = SPLIT ? M2L_NEAR : NS;
+  constexpr int KSF = SPLIT ? (2 * KS + 2) / 3 : KS;
   // out_m = Re/Im(w_m y_m) with y_m split over this lane (v) and its partner (v'): summed
   // over the sources it is A -+ B' with A = sum w.x v and B = sum w.y v of the partner lane,
   // so one lane swap after the last source replaces one per source.
@@ -721,33 +733,32 @@ __device__ __forceinline__ void xlat_wave(const double* Cs, const double2* Ps, c
   // product on the matrix cores, then the post-scaling.  The empty asm statement is a
   // compiler barrier for memory that keeps the Pascal tile in LDS, re-read per source
   // instead of held in 2*KS*MT registers.
-  auto step = [&](int o, double2* cur) {
+  auto step = [&](int o, double2* cur, auto ksg, auto mtg, auto ksn) {
+    constexpr int KG = decltype(ksg)::value, MG = decltype(mtg)::value;
     __asm__ volatile("" ::: "memory");
-    double xb[KS];
+    double xb[KG];
 #pragma unroll
-    for (int ks = 0; ks < KS; ++ks) {
+    for (int ks = 0; ks < KG; ++ks) {
       const double2 w = Ps[2 * (o * NK + 4 * ks + kr) + comp];
       xb[ks] = fma(w.x, cur[ks].x, w.y * cur[ks].y);
     }
-    if (o + 1 < NS) load_src(o + 1, cur);
-    double2 qw[MT][4];
+    if constexpr (decltype(ksn)::value > 0) load_src(o + 1, cur, ksn);
+    double2 qw[MG][4];
     double2 tw[TR ? TR : 1];
-    if constexpr (QE) {
 #pragma unroll
-      for (int mt = 0; mt < MT; ++mt)
+    for (int mt = 0; mt < MG; ++mt)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) qw[mt][r] = Qs[o * NM + TR + 16 * mt + kr + 4 * r];
+      for (int r = 0; r < 4; ++r) qw[mt][r] = Qs[o * NM + TR + 16 * mt + kr + 4 * r];
 #pragma unroll
-      for (int r = 0; r < TR; ++r) tw[r] = Qs[o * NM + r];
-      __builtin_amdgcn_sched_barrier(0);
-    }
-    v4d acc[MT];
+    for (int r = 0; r < TR; ++r) tw[r] = Qs[o * NM + r];
+    __builtin_amdgcn_sched_barrier(0);
+    v4d acc[MG];
 #pragma unroll
-    for (int mt = 0; mt < MT; ++mt) acc[mt] = v4d{0.0, 0.0, 0.0, 0.0};
+    for (int mt = 0; mt < MG; ++mt) acc[mt] = v4d{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int ks = 0; ks < KS; ++ks)
+    for (int ks = 0; ks < KG; ++ks)
 #pragma unroll
-      for (int mt = 0; mt < MT; ++mt)
+      for (int mt = 0; mt < MG; ++mt)
         acc[mt] = __builtin_amdgcn_mfma_f64_16x16x4f64(Cs[(4 * ks + kr) * CS + TR + 16 * mt + col], xb[ks], acc[mt],
                                                        0, 0, 0);
     if constexpr (TR > 0) {
@@ -758,7 +769,7 @@ __device__ __forceinline__ void xlat_wave(const double* Cs, const double2* Ps, c
       } else {
         double b0 = xb[0], b1 = 0.0, b2 = 0.0, b3 = 0.0;
 #pragma unroll
-        for (int ks = 1; ks < KS; ++ks) {
+        for (int ks = 1; ks < KG; ++ks) {
           b0 += xb[ks];
           b1 = fma((double)ks, xb[ks], b1);
           if (ks >= 2) b2 = fma((double)(ks * (ks - 1) / 2), xb[ks], b2);
@@ -771,24 +782,36 @@ __device__ __forceinline__ void xlat_wave(const double* Cs, const double2* Ps, c
       }
 #pragma unroll
       for (int r = 0; r < TR; ++r) {
-        const double2 w = QE ? tw[r] : Qs[o * NM + r];
-        ta[r] = fma(w.x, part[r], ta[r]);
-        tb[r] = fma(w.y, part[r], tb[r]);
+        ta[r] = fma(tw[r].x, part[r], ta[r]);
+        tb[r] = fma(tw[r].y, part[r], tb[r]);
       }
     }
 #pragma unroll
-    for (int mt = 0; mt < MT; ++mt)
+    for (int mt = 0; mt < MG; ++mt)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const double2 w = QE ? qw[mt][r] : Qs[o * NM + TR + 16 * mt + kr + 4 * r];
-        sa[mt][r] = fma(w.x, acc[mt][r], sa[mt][r]);
-        sb[mt][r] = fma(w.y, acc[mt][r], sb[mt][r]);
+        sa[mt][r] = fma(qw[mt][r].x, acc[mt][r], sa[mt][r]);
+        sb[mt][r] = fma(qw[mt][r].y, acc[mt][r], sb[mt][r]);
       }
   };
+  using IKS = std::integral_constant<int, KS>;
+  using IMT = std::integral_constant<int, MT>;
+  using IKSF = std::integral_constant<int, KSF>;
+  using INONE = std::integral_constant<int, 0>;
   double2 ra[KS];
-  load_src(0, ra);
+  load_src(0, ra, IKS{});
+  if constexpr (NNEAR < NS) {
 #pragma unroll 1
-  for (int o = 0; o < NS; ++o) step(o, ra);
+    for (int o = 0; o < NNEAR - 1; ++o) step(o, ra, IKS{}, IMT{}, IKS{});
+    step(NNEAR - 1, ra, IKS{}, IMT{}, IKSF{});
+#pragma unroll 1
+    for (int o = NNEAR; o < NS - 1; ++o) step(o, ra, IKSF{}, IMT{}, IKSF{});
+    step(NS - 1, ra, IKSF{}, IMT{}, INONE{});
+  } else {
+#pragma unroll 1
+    for (int o = 0; o < NS - 1; ++o) step(o, ra, IKS{}, IMT{}, IKS{});
+    step(NS - 1, ra, IKS{}, IMT{}, INONE{});
+  }
   const uint32_t sgn_hi = comp ? 0u : 0x80000000u;  // A - B' on component 0, A + B' on 1
   double lam[MT][4];
 #pragma unroll
@@ -857,7 +880,7 @@ __device__ __forceinline__ void xlat_wave(const double* Cs, const double2* Ps, c
 // scaled translations do not depend on the level), sources from a slot table stab[target * NS
 // + slot]; a slot without a box holds nsrc, one past the last box, which the buffer resource
 // reads as zeros.
-template <int MODE, int KMAX, int KS, int MT, int TR, bool ADAPT = false, bool SEG = false, bool QE = true>
+template <int MODE, int KMAX, int KS, int MT, int TR, bool ADAPT = false, bool SEG = false>
 __device__ __forceinline__ void xlat_body(const XlatLevels& LS, const int* __restrict__ leaf_start,
                                           const double2* __restrict__ src, double2* __restrict__ dst,
                                           const double* __restrict__ Mat, const double2* __restrict__ pre,
@@ -978,7 +1001,7 @@ __device__ __forceinline__ void xlat_body(const XlatLevels& LS, const int* __res
     if (MODE == XL_M2M) return pidx(2 * ix + (o & 1), 2 * iy + (o >> 1), Ws);
     return pidx(ix >> 1, iy >> 1, Ws);
   };
-  xlat_wave<MODE, KMAX, KS, MT, TR, QE>(Cs, Ps, Qs, pt, live, src_box, ADAPT ? src : src + LS.offs[e], ADAPT ? nsrc : NBPs,
+  xlat_wave<MODE, KMAX, KS, MT, TR>(Cs, Ps, Qs, pt, live, src_box, ADAPT ? src : src + LS.offs[e], ADAPT ? nsrc : NBPs,
                                     ADAPT ? dst : dst + LS.offt[e]);
 }
 
@@ -1731,7 +1754,7 @@ static void build_tables(int p, int KMAX, Tables& T) {
   T.offs.assign(4 * 27 * 2, 0);
   for (int c = 0; c < 4; ++c) {
     int dxs[27], dys[27];
-    int no = intl_offsets(c & 1, c >> 1, dxs, dys);
+    int no = m2l_offsets(c & 1, c >> 1, dxs, dys);
     for (int o = 0; o < no; ++o) {
       T.offs[(c * 27 + o) * 2 + 0] = dxs[o];
       T.offs[(c * 27 + o) * 2 + 1] = dys[o];
@@ -3198,7 +3221,7 @@ static void af_lists(eb_afmm P) {
   for (int b = 0; b < nb; ++b) {
     if (P->lev[b] < 2) continue;
     int dxs[27], dys[27];
-    const int no = intl_offsets(P->bx[b] & 1, P->by[b] & 1, dxs, dys);
+    const int no = m2l_offsets(P->bx[b] & 1, P->by[b] & 1, dxs, dys);
     for (int o = 0; o < no; ++o) {
       const int c = find(P->lev[b], P->bx[b] + dxs[o], P->by[b] + dys[o]);
       if (c >= 0) {

@@ -31,6 +31,29 @@ __host__ __device__ inline int intl_offsets(int px, int py, int* dxs, int* dys) 
     }
   return n;
 }
+// The same 27 offsets in the order the M2L kernels take them: the 12 of ring 2 with
+// min(|dx|, |dy|) <= 1 first (centre distance |d| = 2 or sqrt 5 box widths), then the 15
+// corners of ring 2 and boxes of ring 3 (|d| >= 2 sqrt 2), each group in MKINTL order.  The
+// convergence ratio of a translation between scaled expansions is at most sqrt(2) / |d|, so
+// the second group reaches the first group's truncation error with fewer input terms; the
+// kernels translate it with about two thirds of them (M2L_NEAR sources first).
+constexpr int M2L_NEAR = 12;
+__host__ __device__ inline int m2l_offsets(int px, int py, int* dxs, int* dys) {
+  int ax[27], ay[27];
+  const int no = intl_offsets(px, py, ax, ay);
+  int n = 0;
+  for (int pass = 0; pass < 2; ++pass)
+    for (int o = 0; o < no; ++o) {
+      const int mx = ax[o] < 0 ? -ax[o] : ax[o], my = ay[o] < 0 ? -ay[o] : ay[o];
+      const bool near = (mx > my ? mx : my) == 2 && (mx < my ? mx : my) <= 1;
+      if (near == (pass == 0)) {
+        dxs[n] = ax[o];
+        dys[n] = ay[o];
+        ++n;
+      }
+    }
+  return n;
+}
 // Depth below the segments' boxes for a segmented plan whose largest segment has nmax points
 // spread along curves: the number of leaves across a segment (2^d) grows until the curve
 // leaves hold about 40 points (nmax / 2^d <= 128, about pi leaves per box side crossed), within
