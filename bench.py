@@ -64,17 +64,27 @@ def leg(name, fn, *a):
 STAGES = ("charges", "p2m", "m2m", "m2l", "l2l", "l2p", "p2p")
 
 
+def m2l_flops(split, p):
+    """M2L flops: 8 p p_in + 17 (p + p_in) per pair (two series; prescaling per input term,
+    post-scaling per output term) with p_in = p for the 12 nearest interaction-list sources and
+    p_in = the shorter input length of the 15 farther ones (m2l_split)."""
+    pf = split["m2l_far_terms"]
+    return ((8.0 * p * p + 34.0 * p) * split["m2l_near"]
+            + (8.0 * p * pf + 17.0 * (p + pf)) * split["m2l_far"])
+
+
 def flops_model(info, n, p):
     """Algorithmic flop convention: complex add 2, complex*real 2, complex*complex 6,
-    reciprocal 6, divide 12; per stage counts from the tree."""
+    reciprocal 6, divide 12; per stage counts from the tree.  M2M and L2L count the nonzero
+    entries of their triangular Pascal matrices, p (p + 1) / 2 per series."""
     return {
         "charges": 0.0,
         "p2p": 50.0 * info["near_pairs"],
-        "m2l": (8.0 * p * p + 34.0 * p) * info["m2l_pairs"],
+        "m2l": m2l_flops(info, p),
         "p2m": 26.0 * p * n,
         "l2p": 26.0 * p * n,
-        "m2m": (8.0 * p * p + 30.0 * p) * info["boxes"],
-        "l2l": (8.0 * p * p + 30.0 * p) * info["boxes"],
+        "m2m": (4.0 * p * (p + 1) + 30.0 * p) * info["boxes"],
+        "l2l": (4.0 * p * (p + 1) + 30.0 * p) * info["boxes"],
     }
 
 
@@ -923,12 +933,13 @@ def adaptive_fmm(torch, eb, x, y, w, d, p, tgt, ref, args, stream):
     P.apply(d, out)
     err = float((out[tgt.long()] - ref).norm() / ref.norm())
     info = P.info()
+    info.update(P.m2l_split())
     P.close()
     # algorithmic flops: P2L as P2M per (point, box) term, M2P as 36 p per (point, box)
     fl = {"charges": 0.0, "p2p": 50.0 * info["near_pairs"], "p2m": 26.0 * p * n,
-          "m2m": (8.0 * p * p + 30.0 * p) * info["shifted_boxes"],
-          "m2l": (8.0 * p * p + 34.0 * p) * info["m2l_pairs"], "p2l": 26.0 * p * info["p2l_terms"],
-          "l2l": (8.0 * p * p + 30.0 * p) * info["shifted_boxes"],
+          "m2m": (4.0 * p * (p + 1) + 30.0 * p) * info["shifted_boxes"],
+          "m2l": m2l_flops(info, p), "p2l": 26.0 * p * info["p2l_terms"],
+          "l2l": (4.0 * p * (p + 1) + 30.0 * p) * info["shifted_boxes"],
           "far": 26.0 * p * n + 36.0 * p * info["m2p_evals"]}
     return {"what": "the same sums on DAPIF2's adaptive tree (eb_afmm: NAPB 40, lists U/V/W/X), apply only",
             "value": round(n / t_apply / 1e6, 3), "unit": "Mpts/s", "ms": round(t_apply * 1e3, 4),
@@ -937,7 +948,7 @@ def adaptive_fmm(torch, eb, x, y, w, d, p, tgt, ref, args, stream):
             "fp64_fraction_by_stage": {k: round(fl[k] / (stage[k] * 1e-3) / 1e12 / PEAK_FP64_TFLOPS, 4)
                                        for k in fl if fl[k] > 0 and stage[k] > 0},
             "fp64_fraction_total": round(sum(fl.values()) / t_apply / 1e12 / PEAK_FP64_TFLOPS, 4),
-            "counts": {k: info[k] for k in ("boxes", "leaves", "levels", "near_pairs", "m2l_pairs", "m2p_evals",
+            "counts": {k: info[k] for k in ("boxes", "leaves", "levels", "near_pairs", "m2l_pairs", "m2l_near", "m2l_far", "m2l_far_terms", "m2p_evals",
                                             "p2l_terms", "shifted_boxes")},
             "accuracy": {"rel_err2_vs_direct": err, "targets": 1024}}
 
@@ -1006,6 +1017,7 @@ def fmm_workload(torch, eb, dev, kind, n, args, stream):
     plan.enable_timing(False)
     plan.apply(d, out)
     info = plan.info()
+    info.update(plan.m2l_split())
     fl, by = flops_model(info, n, p), bytes_model(info, n, p)
     # accuracy: the O(N^2) sum over all sources at 1024 seeded targets (tree-free, on the GPU)
     tgt = torch.tensor(np.sort(np.random.default_rng(7).choice(n, 1024, replace=False)).astype(np.int32), device=dev)

@@ -46,6 +46,7 @@ _i64p = ctypes.POINTER(ctypes.c_int64)
 EXPORTS = (
     "eb_last_error", "eb_version", "eb_cadapdr", "eb_cadapmod", "eb_cadap", "eb_set_mode", "eb_get_mode",
     "eb_fmm_plan_create", "eb_fmm_plan_rebuild", "eb_fmm_apply", "eb_fmm_plan_destroy", "eb_fmm_plan_info",
+    "eb_fmm_plan_m2l_split", "eb_afmm_m2l_split",
     "eb_fmm_plan_ntbox", "eb_fmm_enable_timing", "eb_fmm_stage_ms", "eb_fmm_set_p2p_tile", "eb_cadap_tables",
     "eb_cadap_lists", "eb_direct_apply_noalloc", "eb_direct_subset", "eb_dapif2", "eb_afmm_create",
     "eb_afmm_apply", "eb_afmm_info", "eb_afmm_enable_timing", "eb_afmm_stage_ms", "eb_afmm_check",
@@ -89,6 +90,8 @@ def lib():
     L.eb_fmm_plan_rebuild.argtypes = [_vp, _vp, _vp, _vp, _vp]
     L.eb_fmm_plan_destroy.argtypes = [_vp]
     L.eb_fmm_plan_info.argtypes = [_vp, _dp, _i64p]
+    L.eb_fmm_plan_m2l_split.argtypes = [_vp, _i64p]
+    L.eb_afmm_m2l_split.argtypes = [_vp, _i64p]
     L.eb_fmm_plan_ntbox.argtypes = [_vp, _ip]
     L.eb_fmm_enable_timing.argtypes = [_vp, _i32]
     L.eb_fmm_set_p2p_tile.argtypes = [_vp, _i32]
@@ -351,6 +354,12 @@ class FMMPlan:
         d.update(rscal=float(r[0]), rminx=float(r[1]), rminy=float(r[2]))
         return d
 
+    def m2l_split(self):
+        """M2L pairs by source group: (near pairs at all p input terms, far pairs, far input terms)."""
+        c = np.zeros(3, np.int64)
+        _check(lib().eb_fmm_plan_m2l_split(self._h, c.ctypes.data_as(_i64p)))
+        return {"m2l_near": int(c[0]), "m2l_far": int(c[1]), "m2l_far_terms": int(c[2])}
+
     def ntbox(self):
         a = np.zeros(self.levmax + 1, np.int32)
         _check(lib().eb_fmm_plan_ntbox(self._h, a.ctypes.data_as(_ip)))
@@ -424,6 +433,12 @@ class AFMMPlan(FMMPlan):
         d = dict(zip(keys, (int(v) for v in c)))
         d.update(rscal=float(r[0]), xonew=float(r[1]), yonew=float(r[2]), nterms=self.nterms, napb=self.napb)
         return d
+
+    def m2l_split(self):
+        """List-V pairs by source group: (near, far, far input terms)."""
+        c = np.zeros(3, np.int64)
+        _check(lib().eb_afmm_m2l_split(self._h, c.ctypes.data_as(_i64p)))
+        return {"m2l_near": int(c[0]), "m2l_far": int(c[1]), "m2l_far_terms": int(c[2])}
 
     def scaled_points(self):
         """The tree's coordinates (x - XONEW) * RSCAL of the plan's points, as the plan forms

@@ -1899,6 +1899,13 @@ static int pick_kmax(int p) {
   return 64;
 }
 
+// Input terms of the M2L's second source group (xlat_wave: KSF = ceil(2 KS / 3) k steps of
+// KS = min(K / 4, ceil(p / 4))), at most p
+static int m2l_far_terms(int p, int kmax) {
+  const int ks = std::min(kmax / 4, (p + 3) / 4);
+  return std::min(p, 4 * ((2 * ks + 2) / 3));
+}
+
 // Move the plan to `stream` (NULL keeps the current one).  Work already queued on the old stream
 // (a tree build, the last apply) is ordered before anything the new stream runs next.
 static int use_stream(eb_fmm_plan P, void* stream) {
@@ -2585,6 +2592,38 @@ extern "C" int eb_fmm_plan_info(eb_fmm_plan P, double* reals3, int64_t* counts8)
   counts8[5] = boxes;
   counts8[6] = nonempty_leaf;
   counts8[7] = maxleaf;
+  return EB_OK;
+}
+
+extern "C" int eb_fmm_plan_m2l_split(eb_fmm_plan P, int64_t* out3) {
+  if (!P || !out3) return (eb_set_error("eb_fmm_plan_m2l_split: bad arguments"), EB_ERR_ARG);
+  if (int rc = ensure_tree(P)) return rc;
+  EB_HIP_CHECK(hipStreamSynchronize(P->st));
+  const int L = P->L;
+  std::vector<int> ls((size_t)(1 << (2 * L)) + 1);
+  EB_HIP_CHECK(hipMemcpy(ls.data(), P->leaf_start, sizeof(int) * ls.size(), hipMemcpyDeviceToHost));
+  auto cnt = [&](int l, int ix, int iy) -> long long {
+    if (ix < 0 || iy < 0 || ix >= (1 << l) || iy >= (1 << l)) return 0;
+    uint32_t m = morton((uint32_t)ix, (uint32_t)iy);
+    int sh = 2 * (L - l);
+    return ls[(size_t)(m + 1) << sh] - ls[(size_t)m << sh];
+  };
+  long long grp[2] = {0, 0};
+  if (P->mode == EB_MODE_FMM)
+    for (int l = 2; l <= L; ++l) {
+      const int ns = 1 << l;
+      for (int iy = 0; iy < ns; ++iy)
+        for (int ix = 0; ix < ns; ++ix) {
+          if (!cnt(l, ix, iy)) continue;
+          int dxs[27], dys[27];
+          const int no = m2l_offsets(ix & 1, iy & 1, dxs, dys);
+          for (int o = 0; o < no; ++o)
+            if (cnt(l, ix + dxs[o], iy + dys[o])) ++grp[o < M2L_NEAR ? 0 : 1];
+        }
+    }
+  out3[0] = grp[0];
+  out3[1] = grp[1];
+  out3[2] = m2l_far_terms(P->p, P->kmax);
   return EB_OK;
 }
 
@@ -3610,6 +3649,19 @@ extern "C" int eb_afmm_info(eb_afmm P, double* reals3, int64_t* counts8) {
     reals3[1] = P->xonew;
     reals3[2] = P->yonew;
   }
+  return EB_OK;
+}
+
+extern "C" int eb_afmm_m2l_split(eb_afmm P, int64_t* out3) {
+  if (!P || !out3) return (eb_set_error("eb_afmm_m2l_split: bad arguments"), EB_ERR_ARG);
+  long long grp[2] = {0, 0};
+  const int nb = P->nbox;
+  for (int b = 0; b < nb; ++b)
+    for (int o = 0; o < 27; ++o)
+      if (P->stab27_h[(size_t)b * 27 + o] != nb) ++grp[o < M2L_NEAR ? 0 : 1];
+  out3[0] = grp[0];
+  out3[1] = grp[1];
+  out3[2] = m2l_far_terms(P->p, P->kmax);
   return EB_OK;
 }
 

@@ -39,6 +39,9 @@ int eb_afmm_apply(eb_afmm plan, const double *dens, double *gradw, void *stream)
  * box), P2L terms (point x box), boxes of level >= 3 (the M2M and the L2L translations
  * each)}. */
 int eb_afmm_info(eb_afmm plan, double *reals3, int64_t *counts8);
+/* The list-V (M2L) pairs by source group, as eb_fmm_plan_m2l_split: out3[0] near (all p
+ * input terms), out3[1] far, out3[2] the far group's input terms. */
+int eb_afmm_m2l_split(eb_afmm plan, int64_t *out3);
 /* Milliseconds of the last apply's stages [charges, P2P, P2M, M2M, M2L, P2L, L2L, L2P + M2P]
  * from HIP events, while timing is enabled (the apply then synchronises). */
 int eb_afmm_enable_timing(eb_afmm plan, int32_t on);

@@ -184,6 +184,13 @@ int eb_direct_subset(int32_t n, const double *x, const double *y, const double *
  * [5] #non-empty boxes over levels 2..levmax, [6] #non-empty leaf boxes, [7] max leaf count.
  * Synchronises the plan's stream. */
 int eb_fmm_plan_info(eb_fmm_plan plan, double *reals3, int64_t *counts8);
+/* The M2L pairs of [4] above by source group: out3[0] pairs whose source is one of the 12
+ * ring-2 offsets at centre distance 2 or sqrt 5 box widths (all p input terms), out3[1] pairs
+ * whose source is one of the 15 farther ones, out3[2] the input terms those take (about two
+ * thirds of p).  For the flop count of the translations.  Synchronises the plan's stream.
+ * No counterpart in the reference, whose SLSHIFT (cadap2.f:1149-1215) takes NTERMS terms of
+ * every interaction-list source. */
+int eb_fmm_plan_m2l_split(eb_fmm_plan plan, int64_t *out3);
 /* Non-empty boxes per level, ntbox[0..levmax] (entry l = the reference's NTBOX of level l,
  * cadap2.f:331-336; entry 0 unused).  Synchronises the plan's stream. */
 int eb_fmm_plan_ntbox(eb_fmm_plan plan, int32_t *ntbox);
