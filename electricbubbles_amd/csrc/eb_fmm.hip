@@ -449,7 +449,10 @@ __device__ __forceinline__ void p2m_box(int p, int s, int e, double2 c, double i
   __shared__ double2 AB[CH][3];  // -(1/r) qa, -(1/r) qb, -(1/r^2) conj(z) qa of each source
   const int lane = threadIdx.x;
   const int js = lane & (CH - 1), qk = lane / CH;
-  const int k = lane < KMAX ? lane : KMAX - 1;
+  // lanes at or past p read row p - 1 (one broadcast address): the loop is bound by LDS
+  // bandwidth, 1 KB per power read of a wave with 64 distinct rows
+  const int k = lane < p ? lane : p - 1;
+  const int QP = (p + 7) / 8;  // power steps per filling lane: rows up to 8 QP - 1 >= p - 1
   const double fk = (double)lane;
   double2 a0 = make_double2(0, 0), a1 = make_double2(0, 0), a2 = make_double2(0, 0);
   for (int c0 = s; c0 < e; c0 += CH) {
@@ -472,8 +475,8 @@ __device__ __forceinline__ void p2m_box(int p, int s, int e, double2 c, double i
       if (qk & 4) pw = cmul(pw, t4);
 #pragma unroll
       for (int m = 0; m < QK; ++m) {
-        P[js][qk + 8 * m] = pw;
-        if (m + 1 < QK) pw = cmul(pw, t8);
+        if (m < QP) P[js][qk + 8 * m] = pw;
+        if (m + 1 < QK && m + 1 < QP) pw = cmul(pw, t8);
       }
     }
     __syncthreads();
