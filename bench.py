@@ -40,7 +40,7 @@ PEAK_HBM_GBS = 8000.0
 # write x 1.000); profiles/fmm_curves_n1048576_L8_p36_hbm_traffic.md has the commands and the
 # per-kernel table.  Keyed by (workload, points, levmax, nterms).
 PROFILED_TRAFFIC_BYTES = {
-    ("curves", 1 << 20, 8, 36): {"m2l": 223445669, "p2p": 396217442},
+    ("curves", 1 << 20, 8, 36): {"m2l": 218904752, "p2p": 390217048},
 }
 
 
@@ -793,6 +793,8 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=200, help="RK4 steps of the adjoint leg")
     ap.add_argument("--only", choices=["solve", "spectral", "sweep", "field", "adjoint", "single"],
                     help="run one secondary leg alone and print its JSON object (for profiling runs)")
+    ap.add_argument("--p2p-tile", type=int, default=0,
+                    help="near-field tile of the timed plans (0: the autotuner's, else 42)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 256 MiB device copy first (the known-size kernel that scales HBM counters)")
     args = ap.parse_args()
@@ -894,7 +896,7 @@ def main():
         "adjoint": adj,
         "roofline": {"bound": "mfma", "kernel": dom, "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
-                     "flops_per_launch": fl[dom],
+                     "flops_per_launch": fl[dom], "pairs": head["_pairs"] if dom == "m2l" else None,
                      "timing": "HIP events on the plan's stream, mean of 5 applies after the timed loops"},
         "cpu_baseline": cpu,
     }
@@ -989,6 +991,8 @@ def fmm_workload(torch, eb, dev, kind, n, args, stream):
         L = tr["levmax"] if L <= 0 else L
         p = tr["nterms"] if p <= 0 else p
     tile = tuned["p2p_tile"] if tuned and (L, p) == (tuned["levmax"], tuned["nterms"]) else 42
+    if args.p2p_tile > 0:
+        tile = args.p2p_tile
     log(f"FMM {kind}: {n} points, LEVMAX {L}, p {p}, near-field tile {tile}")
     plan = eb.FMMPlan(x, y, w, L, p, eb.VARIANT_DR, eb.MODE_FMM, stream=stream)
     plan.set_p2p_tile(tile)
@@ -1056,6 +1060,7 @@ def fmm_workload(torch, eb, dev, kind, n, args, stream):
         "adaptive_fmm": adaptive,
         "autotune": tuned,
         "_flops": fl,
+        "_pairs": {k: info[k] for k in ("m2l_pairs", "m2l_near", "m2l_far", "m2l_far_terms")},
     }
     plan.close()
     if not args.no_check and kind == "curves" and (L, p) != (8, 32):
