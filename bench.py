@@ -620,7 +620,7 @@ def single_drop_leg(torch, dev, with_cpu):
     return out
 
 
-def field_leg(torch, dev, with_cpu):
+def field_leg(torch, dev, with_cpu, only_drops=0):
     """F and the RK2 step (dt = 5e-4) of one simulation on the device, for the many-drop
     configuration (64 drops, about 28K points, E = (1.5, 0): E0 = 2.25, no background flow): every layer
     potential on the FMM (LEVMAX 7), the Laplace and Stokes solves in the device GMRES.  The
@@ -631,6 +631,8 @@ def field_leg(torch, dev, with_cpu):
     from electricbubbles_amd.workloads import many_drops_cfg
     out = {}
     for tag, ndrops, seed in (("many_drop_64", 64, 20260506), ("many_drop_16", 16, 20260516)):
+        if only_drops and ndrops != only_drops:
+            continue
         nd, rku = many_drops_cfg(ndrops, np.random.default_rng(seed))
         B = spectral.Bodies(nd)
         F = field.Field(B, cap=0.0, e0=2.25)
@@ -793,6 +795,8 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=200, help="RK4 steps of the adjoint leg")
     ap.add_argument("--only", choices=["solve", "spectral", "sweep", "field", "adjoint", "single"],
                     help="run one secondary leg alone and print its JSON object (for profiling runs)")
+    ap.add_argument("--field-drops", type=int, default=0, choices=[0, 16, 64],
+                    help="with --only field: run only the 64- or the 16-drop configuration")
     ap.add_argument("--p2p-tile", type=int, default=0,
                     help="near-field tile of the timed plans (0: the autotuner's, else 42)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -817,7 +821,7 @@ def main():
         sys.exit(1)
     dev = torch.device("cuda:0")
     if args.only:
-        legs = {"solve": stokes_solve_leg, "spectral": spectral_leg, "sweep": sweep_leg, "field": field_leg,
+        legs = {"solve": stokes_solve_leg, "spectral": spectral_leg, "sweep": sweep_leg, "field": lambda t, d, c: field_leg(t, d, c, args.field_drops),
                 "single": single_drop_leg,
                 "adjoint": lambda t, d, c: adjoint_leg(t, d, c, args.adjoint_steps)}
         res = {"leg": args.only, args.only: legs[args.only](torch, dev, not args.no_cpu)}
