@@ -72,35 +72,11 @@ struct Geom {
 // ------------------------------------------------------------------------------------------
 constexpr int BB_THREADS = 256;
 
-// partial min/max of x and y (cadapdrop.f:147-158)
-__global__ __launch_bounds__(BB_THREADS) void k_bbox_part(const double* __restrict__ x, const double* __restrict__ y,
-                                                          int n, double4* __restrict__ part) {
-  double mnx = x[0], mxx = x[0], mny = y[0], mxy = y[0];
-  for (int i = blockIdx.x * BB_THREADS + threadIdx.x; i < n; i += gridDim.x * BB_THREADS) {
-    double a = x[i], b = y[i];
-    mnx = fmin(mnx, a);
-    mxx = fmax(mxx, a);
-    mny = fmin(mny, b);
-    mxy = fmax(mxy, b);
-  }
-  __shared__ double4 s[BB_THREADS];
-  s[threadIdx.x] = make_double4(mnx, mxx, mny, mxy);
-  __syncthreads();
-  for (int off = BB_THREADS / 2; off > 0; off >>= 1) {
-    if (threadIdx.x < off) {
-      double4 a = s[threadIdx.x], b = s[threadIdx.x + off];
-      s[threadIdx.x] = make_double4(fmin(a.x, b.x), fmax(a.y, b.y), fmin(a.z, b.z), fmax(a.w, b.w));
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) part[blockIdx.x] = s[0];
-}
-
-// RSCAL = 64 / max(RMAXX-RMINX, RMAXY-RMINY)  (cadapdrop.f:165-171).  min/max are exact, so
-// the reduction order does not change the result.  One block of BB_THREADS threads.
-__global__ __launch_bounds__(BB_THREADS) void k_bbox_final(const double4* __restrict__ part, int np,
-                                                           Geom* __restrict__ g) {
-  __shared__ double4 s[BB_THREADS];
+// RSCAL = 64 / max(RMAXX-RMINX, RMAXY-RMINY)  (cadapdrop.f:165-171) from np partial min/max:
+// min/max are exact, so the reduction order does not change the result.  One block of
+// BB_THREADS threads.
+__device__ __forceinline__ void bbox_reduce(const double4* __restrict__ part, int np, Geom* __restrict__ g,
+                                            double4* s) {
   double4 a = part[0];
   for (int i = threadIdx.x; i < np; i += BB_THREADS) {
     const double4 b = part[i];
@@ -124,6 +100,55 @@ __global__ __launch_bounds__(BB_THREADS) void k_bbox_final(const double4* __rest
     g->rminx = a.x;
     g->rminy = a.z;
   }
+}
+
+// partial min/max of x and y (cadapdrop.f:147-158).  With a ticket counter (tick != nullptr,
+// 0 on entry) the block that finishes last reduces the partials into g and resets *tick to 0,
+// so the bounding box takes one launch; without one, k_bbox_final does the reduction.
+__global__ __launch_bounds__(BB_THREADS) void k_bbox_part(const double* __restrict__ x, const double* __restrict__ y,
+                                                          int n, double4* __restrict__ part, Geom* __restrict__ g,
+                                                          unsigned* __restrict__ tick) {
+  double mnx = x[0], mxx = x[0], mny = y[0], mxy = y[0];
+  for (int i = blockIdx.x * BB_THREADS + threadIdx.x; i < n; i += gridDim.x * BB_THREADS) {
+    double a = x[i], b = y[i];
+    mnx = fmin(mnx, a);
+    mxx = fmax(mxx, a);
+    mny = fmin(mny, b);
+    mxy = fmax(mxy, b);
+  }
+  __shared__ double4 s[BB_THREADS];
+  s[threadIdx.x] = make_double4(mnx, mxx, mny, mxy);
+  __syncthreads();
+  for (int off = BB_THREADS / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) {
+      double4 a = s[threadIdx.x], b = s[threadIdx.x + off];
+      s[threadIdx.x] = make_double4(fmin(a.x, b.x), fmax(a.y, b.y), fmin(a.z, b.z), fmax(a.w, b.w));
+    }
+    __syncthreads();
+  }
+  __shared__ bool last;
+  if (tick == nullptr) {
+    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+    return;
+  }
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = s[0];
+    __threadfence();  // the partial is visible device-wide before the ticket is taken
+    last = atomicAdd(tick, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  bbox_reduce(part, gridDim.x, g, s);
+  if (threadIdx.x == 0) *tick = 0u;
+}
+
+// the reduction of k_bbox_part's partials for callers without a ticket counter (the
+// non-allocating direct sum clears no device memory).  One block of BB_THREADS threads.
+__global__ __launch_bounds__(BB_THREADS) void k_bbox_final(const double4* __restrict__ part, int np,
+                                                           Geom* __restrict__ g) {
+  __shared__ double4 s[BB_THREADS];
+  bbox_reduce(part, np, g, s);
 }
 
 // Scaled coordinates (cadapdrop.f:203-205) and the finest-level Morton key of ASSIGN1's box
@@ -518,12 +543,13 @@ __global__ __launch_bounds__(64) void k_p2m_wide(int L, int p, const int* __rest
 // four entries (one wave of k_xlat) lies within one level.  count[c] = the padded length.
 // The list of class c starts at c * cap, cap = sum over levels of (boxes of the class + 3).
 // ------------------------------------------------------------------------------------------
-// Two passes over blocks of 1024 boxes, blockIdx.x = cls * nb + j, j enumerating the chunks of
-// levels 2..L in order: k_m2l_count counts each chunk's non-empty boxes, k_m2l_write places
-// them after the chunks (and padded levels) before it.
-__device__ __forceinline__ void m2l_chunk(int L, int nb, int& cls, int& l, int& c0) {
-  cls = blockIdx.x / nb;
-  int j = blockIdx.x - cls * nb;
+// Two passes over blocks of 1024 boxes, block blk = cls * nb + j, j enumerating the chunks of
+// levels 2..L in order: m2l_count_block counts each chunk's non-empty boxes, m2l_write_block
+// places them after the chunks (and padded levels) before it.  Both run as the first 4 * nb
+// blocks of k_tree_count and k_tree_write.
+__device__ __forceinline__ void m2l_chunk(int blk, int L, int nb, int& cls, int& l, int& c0) {
+  cls = blk / nb;
+  int j = blk - cls * nb;
   for (l = 2; l <= L; ++l) {
     const int nc = ((1 << (2 * (l - 1))) + 1023) >> 10;
     if (j < nc) break;
@@ -541,11 +567,11 @@ __device__ __forceinline__ bool m2l_live(const int* __restrict__ leaf_start, int
   return be > bs;
 }
 
-__global__ __launch_bounds__(1024) void k_m2l_count(int L, int nb, int lmin, const int* __restrict__ leaf_start,
-                                                    int* __restrict__ ccnt) {
+__device__ __forceinline__ void m2l_count_block(int blk, int L, int nb, int lmin, const int* __restrict__ leaf_start,
+                                                int* __restrict__ ccnt) {
   __shared__ int wsum[16];
   int cls, l, c0;
-  m2l_chunk(L, nb, cls, l, c0);
+  m2l_chunk(blk, L, nb, cls, l, c0);
   const bool live = l >= lmin && m2l_live(leaf_start, L, l, cls, c0 + (int)threadIdx.x);
   const unsigned long long m = __ballot(live);
   if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = __popcll(m);
@@ -553,16 +579,16 @@ __global__ __launch_bounds__(1024) void k_m2l_count(int L, int nb, int lmin, con
   if (threadIdx.x == 0) {
     int acc = 0;
     for (int k = 0; k < 16; ++k) acc += wsum[k];
-    ccnt[blockIdx.x] = acc;
+    ccnt[blk] = acc;
   }
 }
 
-__global__ __launch_bounds__(1024) void k_m2l_write(int L, int nb, int lmin, const int* __restrict__ leaf_start,
-                                                    const int* __restrict__ ccnt, int cap, int* __restrict__ list,
-                                                    int* __restrict__ count) {
+__device__ __forceinline__ void m2l_write_block(int blk, int L, int nb, int lmin, const int* __restrict__ leaf_start,
+                                                const int* __restrict__ ccnt, int cap, int* __restrict__ list,
+                                                int* __restrict__ count) {
   __shared__ int wsum[16], wpre[16], base_s, last_s, lvl_s;
   int cls, l, c0;
-  m2l_chunk(L, nb, cls, l, c0);
+  m2l_chunk(blk, L, nb, cls, l, c0);
   if (threadIdx.x == 0) {
     // entries before this chunk: whole padded levels below l, then this level's earlier chunks
     const int* cc = ccnt + cls * nb;
@@ -865,7 +891,7 @@ __device__ __forceinline__ void xlat_wave(const double* Cs, const double2* Ps, c
   }
 }
 
-// With a compacted target list (M2L only: m2l_list / m2l_count from k_m2l_write) block b
+// With a compacted target list (M2L only: m2l_list / m2l_count from k_tree_write) block b
 // takes entries 16 (b / 4) .. 16 (b / 4) + 15 of parity class b % 4, four per wave: only
 // non-empty target boxes are translated, which matters where most boxes are empty (drops
 // scattered over a large square).  Blocks past the class's count leave at once.
@@ -1222,9 +1248,10 @@ constexpr int P2P_TMAX_DEF = 42, P2P_TMAX_MIN = 16, P2P_TMAX_MAX = 256;
 // The near-field and the far-field items in one count and one write launch over one scan:
 // entries [0, nleaf) count the near-field items of each leaf, [nleaf, nleaf + npar) the
 // far-field (L2P) items of each level-(L-1) box; off is their exclusive scan (nleaf + npar + 1).
-__global__ void k_items_count(int nleaf, int npar, const int* __restrict__ leaf_start, int tmax,
-                              int* __restrict__ cnt) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+// Entry t of both is handled by thread t of the blocks after the M2L list's (k_tree_count,
+// k_tree_write).
+__device__ __forceinline__ void items_count_entry(int t, int nleaf, int npar, const int* __restrict__ leaf_start,
+                                                  int tmax, int* __restrict__ cnt) {
   if (t < nleaf) {
     const int T = leaf_start[t + 1] - leaf_start[t];
     cnt[t] = (T + tmax - 1) / tmax;
@@ -1235,9 +1262,9 @@ __global__ void k_items_count(int nleaf, int npar, const int* __restrict__ leaf_
   }
 }
 
-__global__ void k_items_write(int nleaf, int npar, const int* __restrict__ leaf_start, const int* __restrict__ off,
-                              int4* __restrict__ p2p_items, int4* __restrict__ l2p_items, int* __restrict__ l2p_total) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void items_write_entry(int t, int nleaf, int npar, const int* __restrict__ leaf_start,
+                                                  const int* __restrict__ off, int4* __restrict__ p2p_items,
+                                                  int4* __restrict__ l2p_items, int* __restrict__ l2p_total) {
   if (t < nleaf) {
     const int T = leaf_start[t + 1] - leaf_start[t];
     const int o = off[t], np = off[t + 1] - o;
@@ -1251,6 +1278,37 @@ __global__ void k_items_write(int nleaf, int npar, const int* __restrict__ leaf_
       l2p_items[o + q] = make_int4(m, s + (int)((long)P * q / np), s + (int)((long)P * (q + 1) / np), 0);
     if (m == 0) *l2p_total = off[nleaf + npar] - off[nleaf];
   }
+}
+
+// The tree's two list builds in one count and one write launch of 1024-thread blocks: blocks
+// [0, nm2l) (nm2l = 4 * m2l_nb, or 0 without M2L lists) count or write the M2L target lists,
+// the blocks after them the work items.
+struct TreeLists {
+  int nleaf, npar, tmax;  // work items: leaves, level-(L-1) boxes (0 without L2P), tile
+  int L, m2l_nb, lmin, nm2l, m2l_cap;
+  const int* leaf_start;
+  int *cnt, *off;  // item counts and their exclusive scan
+  int4 *p2p_items, *l2p_items;
+  int* l2p_total;
+  int *ccnt, *m2l_list, *m2l_count;
+};
+
+__global__ __launch_bounds__(1024) void k_tree_count(TreeLists a) {
+  if ((int)blockIdx.x < a.nm2l) {
+    m2l_count_block(blockIdx.x, a.L, a.m2l_nb, a.lmin, a.leaf_start, a.ccnt);
+    return;
+  }
+  const int t = ((int)blockIdx.x - a.nm2l) * 1024 + (int)threadIdx.x;
+  items_count_entry(t, a.nleaf, a.npar, a.leaf_start, a.tmax, a.cnt);
+}
+
+__global__ __launch_bounds__(1024) void k_tree_write(TreeLists a) {
+  if ((int)blockIdx.x < a.nm2l) {
+    m2l_write_block(blockIdx.x, a.L, a.m2l_nb, a.lmin, a.leaf_start, a.ccnt, a.m2l_cap, a.m2l_list, a.m2l_count);
+    return;
+  }
+  const int t = ((int)blockIdx.x - a.nm2l) * 1024 + (int)threadIdx.x;
+  items_write_entry(t, a.nleaf, a.npar, a.leaf_start, a.off, a.p2p_items, a.l2p_items, a.l2p_total);
 }
 
 // Near field at the finest level (DONN0DR, cadapdrop2.f:623-722) plus the final scaling
@@ -1601,7 +1659,8 @@ extern "C" int eb_direct_apply_noalloc(int32_t n, const double* x, const double*
   const double sgn = (variant == EB_VARIANT_DR) ? -1.0 : (variant == EB_VARIANT_HILBERT ? 0.0 : 1.0);
   const double sqb = (variant == EB_VARIANT_PLUS) ? 1.0 : -1.0;
   hipStream_t st = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_bbox_part, dim3(bbblocks), dim3(BB_THREADS), 0, st, x, y, n, bb);
+  hipLaunchKernelGGL(k_bbox_part, dim3(bbblocks), dim3(BB_THREADS), 0, st, x, y, n, bb, (Geom*)nullptr,
+                     (unsigned*)nullptr);
   hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(BB_THREADS), 0, st, bb, bbblocks, g);
   hipLaunchKernelGGL(k_na_prep, dim3((n + 255) / 256), dim3(256), 0, st, n, x, y, (const double2*)wdz,
                      (const double2*)dens, g, sqb, variant == EB_VARIANT_HILBERT ? 1 : 0, zs, qa, qb);
@@ -1806,6 +1865,7 @@ struct eb_fmm_plan_s {
   // device
   Geom* geom = nullptr;
   double4* bbpart = nullptr;
+  unsigned* bbtick = nullptr;  // k_bbox_part's ticket; 0 between launches
   double2* zs0 = nullptr;
   double2* zs = nullptr;
   double2* wdzs = nullptr;
@@ -1833,7 +1893,7 @@ struct eb_fmm_plan_s {
   double2* dir_part = nullptr;  // direct mode: 2 x n partial sums per source chunk
   int dir_sch = 0, dir_nch = 0;
   const int* guard = nullptr;  // direct mode: device flag; 0 = skip the pair sums (eb_fmm_set_guard)
-  int* m2l_list = nullptr;   // compacted M2L targets (k_m2l_count, k_m2l_write), 4 x m2l_cap
+  int* m2l_list = nullptr;   // compacted M2L targets (k_tree_count, k_tree_write), 4 x m2l_cap
   int* m2l_count = nullptr;  // 4
   int* m2l_ccnt = nullptr;   // per (class, chunk of 1024 boxes) counts, 4 x m2l_nb
   int m2l_cap = 0, m2l_nb = 0;
@@ -1938,18 +1998,38 @@ static size_t p2p_item_cap(int n, int L, int tmax) { return ((size_t)1 << (2 * L
 
 // The near-field work items of the current leaves and tile and, in FMM mode, the far-field
 // (L2P) items: one count launch, one scan over both, one write launch.  Their counts stay on
-// the device (p2p_off[nleaf] and *l2p_total).
-static int build_items(eb_fmm_plan P) {
+// the device (p2p_off[nleaf] and *l2p_total).  With with_m2l the same two launches also build
+// the M2L target lists.
+static int build_items(eb_fmm_plan P, bool with_m2l) {
   hipStream_t st = P->st;
-  const int nleaf = 1 << (2 * P->L);
-  const int npar = P->l2p_items ? 1 << (2 * (P->L - 1)) : 0;
-  const int m = nleaf + npar;
-  hipLaunchKernelGGL(k_items_count, dim3((m + 255) / 256), dim3(256), 0, st, nleaf, npar, P->leaf_start,
-                     P->p2p_tmax, P->p2p_cnt);
+  TreeLists a{};
+  a.nleaf = 1 << (2 * P->L);
+  a.npar = P->l2p_items ? 1 << (2 * (P->L - 1)) : 0;
+  a.tmax = P->p2p_tmax;
+  a.L = P->L;
+  a.leaf_start = P->leaf_start;
+  a.cnt = P->p2p_cnt;
+  a.off = P->p2p_off;
+  a.p2p_items = P->p2p_items;
+  a.l2p_items = P->l2p_items;
+  a.l2p_total = P->l2p_total;
+  if (with_m2l && P->m2l_list) {
+    // a segmented plan's boxes interact from two levels below the segments' level: the
+    // interaction lists of the segment boxes' children lie wholly in other segments
+    a.lmin = P->segk ? P->segk + 2 : 2;
+    a.m2l_nb = P->m2l_nb;
+    a.nm2l = 4 * P->m2l_nb;
+    a.m2l_cap = P->m2l_cap;
+    a.ccnt = P->m2l_ccnt;
+    a.m2l_list = P->m2l_list;
+    a.m2l_count = P->m2l_count;
+  }
+  const int m = a.nleaf + a.npar;
+  const int blocks = a.nm2l + (m + 1023) / 1024;
+  hipLaunchKernelGGL(k_tree_count, dim3(blocks), dim3(1024), 0, st, a);
   const int rc = scan_exclusive(P->p2p_cnt, P->p2p_off, m, true, P->p2p_tmp, st);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_items_write, dim3((m + 255) / 256), dim3(256), 0, st, nleaf, npar, P->leaf_start, P->p2p_off,
-                     P->p2p_items, P->l2p_items, P->l2p_total);
+  hipLaunchKernelGGL(k_tree_write, dim3(blocks), dim3(1024), 0, st, a);
   EB_HIP_CHECK(hipGetLastError());
   return EB_OK;
 }
@@ -1966,17 +2046,7 @@ static int sort_tree(eb_fmm_plan P, const double2* wsrc) {
   const int t = std::max(n, nleaf + 1);
   hipLaunchKernelGGL(k_leaf_gather, dim3((t + 255) / 256), dim3(256), 0, st, P->keys, n, nleaf, P->leaf_start,
                      P->perm, P->zs0, wsrc, P->zs, P->wdzs);
-  if ((rc = build_items(P)) != EB_OK) return rc;
-  if (P->m2l_list) {
-    // a segmented plan's boxes interact from two levels below the segments' level: the
-    // interaction lists of the segment boxes' children lie wholly in other segments
-    const int lmin = P->segk ? P->segk + 2 : 2;
-    hipLaunchKernelGGL(k_m2l_count, dim3(4 * P->m2l_nb), dim3(1024), 0, st, L, P->m2l_nb, lmin, P->leaf_start,
-                       P->m2l_ccnt);
-    hipLaunchKernelGGL(k_m2l_write, dim3(4 * P->m2l_nb), dim3(1024), 0, st, L, P->m2l_nb, lmin, P->leaf_start,
-                       P->m2l_ccnt, P->m2l_cap, P->m2l_list, P->m2l_count);
-  }
-  EB_HIP_CHECK(hipGetLastError());
+  if ((rc = build_items(P, true)) != EB_OK) return rc;
   P->tree_sorted = true;
   return EB_OK;
 }
@@ -2010,8 +2080,8 @@ static int build_tree(eb_fmm_plan P, const double* x, const double* y, const dou
     hipLaunchKernelGGL(k_seg_keys, dim3((n + 255) / 256), dim3(256), 0, st, n, P->nseg, P->seg_off, x, y, L, P->segk,
                        P->seg_geo, P->zs0, P->keys, P->perm);
   } else {
-    hipLaunchKernelGGL(k_bbox_part, dim3(bbblocks), dim3(BB_THREADS), 0, st, x, y, n, P->bbpart);
-    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(BB_THREADS), 0, st, P->bbpart, bbblocks, P->geom);
+    hipLaunchKernelGGL(k_bbox_part, dim3(bbblocks), dim3(BB_THREADS), 0, st, x, y, n, P->bbpart, P->geom,
+                       P->bbtick);
     hipLaunchKernelGGL(k_keys, dim3((n + 255) / 256), dim3(256), 0, st, x, y, n, L, P->geom, P->zs0, P->keys,
                        P->perm);
   }
@@ -2103,6 +2173,8 @@ static int plan_create(eb_fmm_plan* out, int32_t n, const double* x, const doubl
 
   EB_TRY(dmalloc(&P->geom, 1));
   EB_TRY(dmalloc(&P->bbpart, bbblocks));
+  EB_TRY(dmalloc(&P->bbtick, 1));
+  EB_HIP_CHECK(hipMemsetAsync(P->bbtick, 0, sizeof(unsigned), P->st));
   EB_TRY(dmalloc(&P->zs0, n));
   EB_TRY(dmalloc(&P->zs, n));
   EB_TRY(dmalloc(&P->wdzs, n));
@@ -2512,7 +2584,7 @@ extern "C" int eb_fmm_set_p2p_tile(eb_fmm_plan P, int32_t tmax) {
     return EB_ERR_ARG;
   }
   P->p2p_tmax = tmax;
-  return P->tree_sorted ? build_items(P) : EB_OK;  // an unsorted direct-mode tree builds them when sorted
+  return P->tree_sorted ? build_items(P, false) : EB_OK;  // an unsorted direct-mode tree builds them when sorted
 }
 
 extern "C" int eb_fmm_set_guard(eb_fmm_plan P, const int32_t* flag) {
@@ -2723,7 +2795,8 @@ extern "C" int eb_cadap_tables(int32_t ilev, int32_t n, const double* x, const d
     cleanup();
     return rc;
   }
-  hipLaunchKernelGGL(k_bbox_part, dim3(bbblocks), dim3(BB_THREADS), 0, st, x, y, n, part);
+  hipLaunchKernelGGL(k_bbox_part, dim3(bbblocks), dim3(BB_THREADS), 0, st, x, y, n, part, (Geom*)nullptr,
+                     (unsigned*)nullptr);
   hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(BB_THREADS), 0, st, part, bbblocks, g);
   bool ok = hipMemsetAsync(icnt, 0, sizeof(int) * nb, st) == hipSuccess &&
             hipMemsetAsync(iloc, 0, sizeof(int) * nb, st) == hipSuccess &&
