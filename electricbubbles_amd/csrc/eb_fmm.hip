@@ -73,8 +73,7 @@ struct Geom {
 constexpr int BB_THREADS = 256;
 
 // RSCAL = 64 / max(RMAXX-RMINX, RMAXY-RMINY)  (cadapdrop.f:165-171) from np partial min/max:
-// min/max are exact, so the reduction order does not change the result.  One block of
-// BB_THREADS threads.
+// min/max are exact, so the reduction order does not change the result.
 __device__ __forceinline__ void bbox_reduce(const double4* __restrict__ part, int np, Geom* __restrict__ g,
                                             double4* s) {
   double4 a = part[0];
@@ -102,12 +101,11 @@ __device__ __forceinline__ void bbox_reduce(const double4* __restrict__ part, in
   }
 }
 
-// partial min/max of x and y (cadapdrop.f:147-158).  With a ticket counter (tick != nullptr,
-// 0 on entry) the block that finishes last reduces the partials into g and resets *tick to 0,
-// so the bounding box takes one launch; without one, k_bbox_final does the reduction.
+// partial min/max of x and y (cadapdrop.f:147-158), one per block; k_bbox_final reduces them.
+// (A last-block reduction behind a ticket counter measured slower, 30 us against 7 + 4 us at
+// 2^20 points: every block's device-scope fence costs more than the launch it saves.)
 __global__ __launch_bounds__(BB_THREADS) void k_bbox_part(const double* __restrict__ x, const double* __restrict__ y,
-                                                          int n, double4* __restrict__ part, Geom* __restrict__ g,
-                                                          unsigned* __restrict__ tick) {
+                                                          int n, double4* __restrict__ part) {
   double mnx = x[0], mxx = x[0], mny = y[0], mxy = y[0];
   for (int i = blockIdx.x * BB_THREADS + threadIdx.x; i < n; i += gridDim.x * BB_THREADS) {
     double a = x[i], b = y[i];
@@ -126,25 +124,10 @@ __global__ __launch_bounds__(BB_THREADS) void k_bbox_part(const double* __restri
     }
     __syncthreads();
   }
-  __shared__ bool last;
-  if (tick == nullptr) {
-    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
-    return;
-  }
-  if (threadIdx.x == 0) {
-    part[blockIdx.x] = s[0];
-    __threadfence();  // the partial is visible device-wide before the ticket is taken
-    last = atomicAdd(tick, 1u) == gridDim.x - 1;
-  }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  bbox_reduce(part, gridDim.x, g, s);
-  if (threadIdx.x == 0) *tick = 0u;
+  if (threadIdx.x == 0) part[blockIdx.x] = s[0];
 }
 
-// the reduction of k_bbox_part's partials for callers without a ticket counter (the
-// non-allocating direct sum clears no device memory).  One block of BB_THREADS threads.
+// the reduction of k_bbox_part's np partials.  One block of BB_THREADS threads.
 __global__ __launch_bounds__(BB_THREADS) void k_bbox_final(const double4* __restrict__ part, int np,
                                                            Geom* __restrict__ g) {
   __shared__ double4 s[BB_THREADS];
@@ -1659,8 +1642,7 @@ extern "C" int eb_direct_apply_noalloc(int32_t n, const double* x, const double*
   const double sgn = (variant == EB_VARIANT_DR) ? -1.0 : (variant == EB_VARIANT_HILBERT ? 0.0 : 1.0);
   const double sqb = (variant == EB_VARIANT_PLUS) ? 1.0 : -1.0;
   hipStream_t st = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_bbox_part, dim3(bbblocks), dim3(BB_THREADS), 0, st, x, y, n, bb, (Geom*)nullptr,
-                     (unsigned*)nullptr);
+  hipLaunchKernelGGL(k_bbox_part, dim3(bbblocks), dim3(BB_THREADS), 0, st, x, y, n, bb);
   hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(BB_THREADS), 0, st, bb, bbblocks, g);
   hipLaunchKernelGGL(k_na_prep, dim3((n + 255) / 256), dim3(256), 0, st, n, x, y, (const double2*)wdz,
                      (const double2*)dens, g, sqb, variant == EB_VARIANT_HILBERT ? 1 : 0, zs, qa, qb);
@@ -1865,7 +1847,6 @@ struct eb_fmm_plan_s {
   // device
   Geom* geom = nullptr;
   double4* bbpart = nullptr;
-  unsigned* bbtick = nullptr;  // k_bbox_part's ticket; 0 between launches
   double2* zs0 = nullptr;
   double2* zs = nullptr;
   double2* wdzs = nullptr;
@@ -2080,8 +2061,8 @@ static int build_tree(eb_fmm_plan P, const double* x, const double* y, const dou
     hipLaunchKernelGGL(k_seg_keys, dim3((n + 255) / 256), dim3(256), 0, st, n, P->nseg, P->seg_off, x, y, L, P->segk,
                        P->seg_geo, P->zs0, P->keys, P->perm);
   } else {
-    hipLaunchKernelGGL(k_bbox_part, dim3(bbblocks), dim3(BB_THREADS), 0, st, x, y, n, P->bbpart, P->geom,
-                       P->bbtick);
+    hipLaunchKernelGGL(k_bbox_part, dim3(bbblocks), dim3(BB_THREADS), 0, st, x, y, n, P->bbpart);
+    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(BB_THREADS), 0, st, P->bbpart, bbblocks, P->geom);
     hipLaunchKernelGGL(k_keys, dim3((n + 255) / 256), dim3(256), 0, st, x, y, n, L, P->geom, P->zs0, P->keys,
                        P->perm);
   }
@@ -2173,8 +2154,6 @@ static int plan_create(eb_fmm_plan* out, int32_t n, const double* x, const doubl
 
   EB_TRY(dmalloc(&P->geom, 1));
   EB_TRY(dmalloc(&P->bbpart, bbblocks));
-  EB_TRY(dmalloc(&P->bbtick, 1));
-  EB_HIP_CHECK(hipMemsetAsync(P->bbtick, 0, sizeof(unsigned), P->st));
   EB_TRY(dmalloc(&P->zs0, n));
   EB_TRY(dmalloc(&P->zs, n));
   EB_TRY(dmalloc(&P->wdzs, n));
@@ -2795,8 +2774,7 @@ extern "C" int eb_cadap_tables(int32_t ilev, int32_t n, const double* x, const d
     cleanup();
     return rc;
   }
-  hipLaunchKernelGGL(k_bbox_part, dim3(bbblocks), dim3(BB_THREADS), 0, st, x, y, n, part, (Geom*)nullptr,
-                     (unsigned*)nullptr);
+  hipLaunchKernelGGL(k_bbox_part, dim3(bbblocks), dim3(BB_THREADS), 0, st, x, y, n, part);
   hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(BB_THREADS), 0, st, part, bbblocks, g);
   bool ok = hipMemsetAsync(icnt, 0, sizeof(int) * nb, st) == hipSuccess &&
             hipMemsetAsync(iloc, 0, sizeof(int) * nb, st) == hipSuccess &&

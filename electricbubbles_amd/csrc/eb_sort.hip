@@ -6,30 +6,32 @@
 
 namespace eb {
 
-constexpr int RS_THREADS = 256;
-constexpr int RS_ITEMS = 16;                  // elements per thread per pass
-constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
+constexpr int RS_THREADS = 256;                 // k_rs_prefix: one thread per digit
+constexpr int RS_BT = 1024;                     // k_rs_hist and k_rs_scatter
+constexpr int RS_TILE = 4096;                   // elements per tile (one block of either)
+constexpr int RS_BW = RS_BT / EB_WAVE;          // 16 waves per block
+constexpr int RS_SUB = RS_TILE / RS_BW;         // 256 consecutive elements per wave
+constexpr int RS_ROUNDS = RS_SUB / EB_WAVE;     // 4 rounds of 64
 constexpr int RS_BITS = 8;
 constexpr int RS_RADIX = 1 << RS_BITS;
+static_assert(RS_RADIX == RS_THREADS && RS_RADIX <= RS_BT, "digit-indexed threads");
 
-__global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint32_t* __restrict__ keys, int n, int shift,
-                                                        int nblk, int* __restrict__ hist) {
+// per tile: the count of each digit
+__global__ __launch_bounds__(RS_BT) void k_rs_hist(const uint32_t* __restrict__ keys, int n, int shift, int nblk,
+                                                   int* __restrict__ hist) {
   __shared__ int h[RS_RADIX];
-  for (int d = threadIdx.x; d < RS_RADIX; d += RS_THREADS) h[d] = 0;
+  if (threadIdx.x < RS_RADIX) h[threadIdx.x] = 0;
   __syncthreads();
   const int base = blockIdx.x * RS_TILE;
-  for (int r = 0; r < RS_ITEMS; ++r) {
-    int e = base + r * RS_THREADS + threadIdx.x;
+#pragma unroll
+  for (int r = 0; r < RS_TILE / RS_BT; ++r) {
+    const int e = base + r * RS_BT + threadIdx.x;
     if (e < n) atomicAdd(&h[(keys[e] >> shift) & (RS_RADIX - 1)], 1);
   }
   __syncthreads();
-  for (int d = threadIdx.x; d < RS_RADIX; d += RS_THREADS) hist[d * nblk + blockIdx.x] = h[d];
+  if (threadIdx.x < RS_RADIX) hist[threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
 }
 
-// Each block takes its tile in order, RS_THREADS elements per round; within a round the rank
-// among equal digits is found with wave ballots (64-lane match) plus per-wave counts.  The
-// tile's keys and values are all loaded before the first round (one memory latency per block
-// instead of one per round: 20 -> 15 us per pass at 2^20 keys).
 // Per digit d (one block each): the exclusive prefix over the tiles of d's tile counts
 // (hist[d * nblk + b] -> pre[d * nblk + b]) and d's total over all tiles (tot[d]).  The
 // scatter adds the digits' own prefix, so the three launches of a general scan of the
@@ -60,52 +62,68 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_prefix(const int* __restrict_
   }
   if (t == RS_THREADS - 1) tot[d] = part[t];
 }
-__global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint32_t* __restrict__ kin,
-                                                           const int32_t* __restrict__ vin,
-                                                           uint32_t* __restrict__ kout,
-                                                           int32_t* __restrict__ vout, int n, int shift,
-                                                           int nblk, const int* __restrict__ pre,
-                                                           const int* __restrict__ tot) {
-  static_assert(RS_RADIX == RS_THREADS, "one thread per digit in the digit-total scan");
-  constexpr int NW = RS_THREADS / EB_WAVE;
-  __shared__ int base[RS_RADIX];
-  __shared__ int wcnt[NW][RS_RADIX];
+
+// Exclusive scan of v over threads 0..RS_RADIX-1 of the block (the others pass 0 and get 0):
+// wave scans by shuffles, then the wave totals.  Every thread of the block calls it.
+__device__ __forceinline__ int rs_scan_digits(int v, int* wsum) {
+  const int lane = threadIdx.x & (EB_WAVE - 1), wid = threadIdx.x / EB_WAVE;
+  int x = v;
+#pragma unroll
+  for (int off = 1; off < EB_WAVE; off <<= 1) {
+    const int y = __shfl_up(x, off, EB_WAVE);
+    if (lane >= off) x += y;
+  }
+  if (lane == EB_WAVE - 1 && wid < RS_RADIX / EB_WAVE) wsum[wid] = x;
+  __syncthreads();
+  int before = 0;
+  for (int w = 0; w < wid && w < RS_RADIX / EB_WAVE; ++w) before += wsum[w];
+  __syncthreads();  // wsum is free again for the next call
+  return (int)threadIdx.x < RS_RADIX ? before + x - v : 0;
+}
+
+// One tile per block.  Wave w ranks its own RS_SUB consecutive elements in RS_ROUNDS rounds
+// (ballots find the equal digits of a round; per-wave digit counters in LDS carry the rounds,
+// so the ranking needs no block barrier); the waves' counts are then summed per digit, the
+// tile is reordered by digit in LDS, and the threads write it out in that order, so each
+// digit's run of the tile goes to consecutive addresses.  Stable: within a digit the output
+// follows the input order (tile, then wave, then round, then lane).
+__global__ __launch_bounds__(RS_BT) void k_rs_scatter(const uint32_t* __restrict__ kin,
+                                                      const int32_t* __restrict__ vin, uint32_t* __restrict__ kout,
+                                                      int32_t* __restrict__ vout, int n, int shift, int nblk,
+                                                      const int* __restrict__ pre, const int* __restrict__ tot) {
+  __shared__ int wc[RS_BW][RS_RADIX];  // per-wave digit counts, then each wave's offset within the digit
+  __shared__ int dglob[RS_RADIX];      // output position of the tile's first element of digit d
+  __shared__ int dloc[RS_RADIX];       // tile-local position of digit d's first element
+  __shared__ int wsum[RS_RADIX / EB_WAVE];
+  __shared__ uint32_t sk[RS_TILE];
+  __shared__ int32_t sv[RS_TILE];
   const int lane = threadIdx.x & (EB_WAVE - 1);
   const int wid = threadIdx.x / EB_WAVE;
   const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
   const int tile = blockIdx.x * RS_TILE;
-  uint32_t kr[RS_ITEMS];
-  int32_t vr[RS_ITEMS];
+  const int w0 = tile + wid * RS_SUB;
+  uint32_t kr[RS_ROUNDS];
+  int32_t vr[RS_ROUNDS];
+  int rk[RS_ROUNDS];
 #pragma unroll
-  for (int r = 0; r < RS_ITEMS; ++r) {
-    const int e = tile + r * RS_THREADS + threadIdx.x;
+  for (int r = 0; r < RS_ROUNDS; ++r) {
+    const int e = w0 + r * EB_WAVE + lane;
     kr[r] = e < n ? kin[e] : 0u;
     vr[r] = e < n ? vin[e] : 0;
   }
+  for (int i = threadIdx.x; i < RS_BW * RS_RADIX; i += RS_BT) (&wc[0][0])[i] = 0;
   {
-    // digit d's first output position: the totals of the digits below it, then this tile's
-    // place among d's tiles (the tile's loads above are in flight meanwhile)
+    // digits below d over all tiles, then this tile's place among d's tiles
     const int d = threadIdx.x;
-    const int td = tot[d];
-    base[d] = td;
-    __syncthreads();
-    for (int off = 1; off < RS_RADIX; off <<= 1) {
-      const int v = d >= off ? base[d - off] : 0;
-      __syncthreads();
-      base[d] += v;
-      __syncthreads();
-    }
-    base[d] = base[d] - td + pre[(size_t)d * nblk + blockIdx.x];
+    const int td = d < RS_RADIX ? tot[d] : 0;
+    const int below = rs_scan_digits(td, wsum);  // also orders the clearing of wc before its use
+    if (d < RS_RADIX) dglob[d] = below + pre[(size_t)d * nblk + blockIdx.x];
   }
 #pragma unroll
-  for (int r = 0; r < RS_ITEMS; ++r) {
-    for (int d = threadIdx.x; d < NW * RS_RADIX; d += RS_THREADS) (&wcnt[0][0])[d] = 0;
-    __syncthreads();
-    const int e = tile + r * RS_THREADS + threadIdx.x;
+  for (int r = 0; r < RS_ROUNDS; ++r) {
+    const int e = w0 + r * EB_WAVE + lane;
     const bool valid = e < n;
-    const uint32_t k = kr[r];
-    const int32_t v = vr[r];
-    const int dig = valid ? (int)((k >> shift) & (RS_RADIX - 1)) : 0;
+    const int dig = valid ? (int)((kr[r] >> shift) & (RS_RADIX - 1)) : 0;
     uint64_t peers = __ballot(valid);
 #pragma unroll
     for (int b = 0; b < RS_BITS; ++b) {
@@ -114,22 +132,43 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint32_t* __res
       peers &= bit ? m : ~m;
     }
     const int rank = __popcll(peers & lt);
-    if (valid && rank == 0) wcnt[wid][dig] = __popcll(peers);
-    __syncthreads();
-    if (valid) {
-      int off = base[dig] + rank;
-      for (int w = 0; w < wid; ++w) off += wcnt[w][dig];
-      kout[off] = k;
-      vout[off] = v;
+    // every lane reads the count before the digit's first lane raises it (one wave: the LDS
+    // read instruction completes for all lanes before the write instruction)
+    const int c = wc[wid][dig];
+    rk[r] = c + rank;
+    if (valid && rank == 0) wc[wid][dig] = c + __popcll(peers);
+  }
+  __syncthreads();
+  int cnt = 0;
+  if (threadIdx.x < RS_RADIX) {
+    const int d = threadIdx.x;
+    for (int w = 0; w < RS_BW; ++w) {
+      const int c = wc[w][d];
+      wc[w][d] = cnt;
+      cnt += c;
     }
-    __syncthreads();
-    for (int d = threadIdx.x; d < RS_RADIX; d += RS_THREADS) {
-      int t = 0;
+  }
+  const int loc = rs_scan_digits(cnt, wsum);
+  if (threadIdx.x < RS_RADIX) dloc[threadIdx.x] = loc;
+  __syncthreads();
 #pragma unroll
-      for (int w = 0; w < NW; ++w) t += wcnt[w][d];
-      base[d] += t;
+  for (int r = 0; r < RS_ROUNDS; ++r) {
+    const int e = w0 + r * EB_WAVE + lane;
+    if (e < n) {
+      const int dig = (int)((kr[r] >> shift) & (RS_RADIX - 1));
+      const int at = dloc[dig] + wc[wid][dig] + rk[r];
+      sk[at] = kr[r];
+      sv[at] = vr[r];
     }
-    __syncthreads();
+  }
+  __syncthreads();
+  const int nt = min(RS_TILE, n - tile);
+  for (int i = threadIdx.x; i < nt; i += RS_BT) {
+    const uint32_t k = sk[i];
+    const int dig = (int)((k >> shift) & (RS_RADIX - 1));
+    const int at = dglob[dig] + (i - dloc[dig]);
+    kout[at] = k;
+    vout[at] = sv[i];
   }
 }
 
@@ -266,10 +305,9 @@ int radix_sort_pairs(uint32_t* k0, int32_t* v0, uint32_t* k1, int32_t* v1, int n
   for (int p = 0; p < passes; ++p) {
     const int shift = p * RS_BITS;
     if (shift < 32) {
-      hipLaunchKernelGGL(k_rs_hist, dim3(nblk), dim3(RS_THREADS), 0, st, ka, n, shift, nblk, hist);
+      hipLaunchKernelGGL(k_rs_hist, dim3(nblk), dim3(RS_BT), 0, st, ka, n, shift, nblk, hist);
       hipLaunchKernelGGL(k_rs_prefix, dim3(RS_RADIX), dim3(RS_THREADS), 0, st, hist, nblk, pre, tot);
-      hipLaunchKernelGGL(k_rs_scatter, dim3(nblk), dim3(RS_THREADS), 0, st, ka, va, kb, vb, n, shift, nblk, pre,
-                         tot);
+      hipLaunchKernelGGL(k_rs_scatter, dim3(nblk), dim3(RS_BT), 0, st, ka, va, kb, vb, n, shift, nblk, pre, tot);
       EB_HIP_CHECK(hipGetLastError());
     } else {
       EB_HIP_CHECK(hipMemcpyAsync(kb, ka, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st));
