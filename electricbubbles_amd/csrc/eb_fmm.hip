@@ -137,24 +137,74 @@ __global__ __launch_bounds__(BB_THREADS) void k_bbox_final(const double4* __rest
 // Scaled coordinates (cadapdrop.f:203-205) and the finest-level Morton key of ASSIGN1's box
 // (cadap2.f:312-319).  H = 64/2^L is a power of two, so X/H is exact and the integer
 // truncation matches the reference bit for bit; coarser levels are key >> 2(L-l).
-__global__ void k_keys(const double* __restrict__ x, const double* __restrict__ y, int n, int L,
-                       const Geom* __restrict__ g, double2* __restrict__ zs0, uint32_t* __restrict__ keys,
-                       int32_t* __restrict__ vals) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double rs = g->rscal;
-  const double X = (x[i] - g->rminx) * rs;
-  const double Y = (y[i] - g->rminy) * rs;
-  zs0[i] = make_double2(X, Y);
+// Two neighbours of this step run in the same launch: every block first reduces the np <= 1024
+// bounding-box partials of k_bbox_part itself (what k_bbox_final does; block 0 stores the
+// result in gout for the later kernels), and it counts the low 8 bits of its RS_TILE_KEYS keys
+// for the radix sort's first pass (hist[d * nblk + block]).
+__global__ __launch_bounds__(1024) void k_keys(const double* __restrict__ x, const double* __restrict__ y,
+                                                    int n, int L, const double4* __restrict__ part, int np,
+                                                    Geom* __restrict__ gout, double2* __restrict__ zs0,
+                                                    uint32_t* __restrict__ keys, int32_t* __restrict__ vals,
+                                                    int* __restrict__ hist, int nblk) {
+  __shared__ double gs[3];  // RSCAL, RMINX, RMINY
+  __shared__ double4 wred[16];
+  __shared__ int h[256];
+  if (threadIdx.x < 256) h[threadIdx.x] = 0;
+  {
+    // one partial per thread (np <= 1024), then the waves' results
+    double4 a = part[threadIdx.x < np ? threadIdx.x : 0];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      a.x = fmin(a.x, __shfl_xor(a.x, off, 64));
+      a.y = fmax(a.y, __shfl_xor(a.y, off, 64));
+      a.z = fmin(a.z, __shfl_xor(a.z, off, 64));
+      a.w = fmax(a.w, __shfl_xor(a.w, off, 64));
+    }
+    if ((threadIdx.x & 63) == 0) wred[threadIdx.x >> 6] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      a = wred[0];
+      for (int w = 1; w < 16; ++w) {
+        const double4 b = wred[w];
+        a = make_double4(fmin(a.x, b.x), fmax(a.y, b.y), fmin(a.z, b.z), fmax(a.w, b.w));
+      }
+      // RSCAL = 64 / max(RMAXX-RMINX, RMAXY-RMINY)  (cadapdrop.f:165-171)
+      const double rsx = a.y - a.x, rsy = a.w - a.z;
+      const double r = (rsx > rsy) ? rsx : rsy;
+      gs[0] = 64.0 / r;
+      gs[1] = a.x;
+      gs[2] = a.z;
+      if (blockIdx.x == 0) {
+        gout->rscal = gs[0];
+        gout->rminx = a.x;
+        gout->rminy = a.z;
+      }
+    }
+  }
+  __syncthreads();
+  const double rs = gs[0], mx = gs[1], my = gs[2];
   const int nside = 1 << L;
-  const double h = 64.0 / nside;
-  int ix = (int)(X / h), iy = (int)(Y / h);
-  ix = ix >= nside ? nside - 1 : ix;
-  iy = iy >= nside ? nside - 1 : iy;
-  ix = ix < 0 ? 0 : ix;
-  iy = iy < 0 ? 0 : iy;
-  keys[i] = morton((uint32_t)ix, (uint32_t)iy);
-  vals[i] = i;
+  const double hh = 64.0 / nside;
+#pragma unroll
+  for (int r = 0; r < RS_TILE_KEYS / 1024; ++r) {
+    const int i = blockIdx.x * RS_TILE_KEYS + r * 1024 + (int)threadIdx.x;
+    if (i < n) {
+      const double X = (x[i] - mx) * rs;
+      const double Y = (y[i] - my) * rs;
+      zs0[i] = make_double2(X, Y);
+      int ix = (int)(X / hh), iy = (int)(Y / hh);
+      ix = ix >= nside ? nside - 1 : ix;
+      iy = iy >= nside ? nside - 1 : iy;
+      ix = ix < 0 ? 0 : ix;
+      iy = iy < 0 ? 0 : iy;
+      const uint32_t k = morton((uint32_t)ix, (uint32_t)iy);
+      keys[i] = k;
+      vals[i] = i;
+      atomicAdd(&h[k & 255u], 1);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 256) hist[threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
 }
 
 // Segmented plans: each segment's bounding box (cadapdrop.f:147-158 on the segment's points)
@@ -1869,6 +1919,7 @@ struct eb_fmm_plan_s {
   // direct mode: the rebuild leaves the points in the caller's order (the O(N^2) sum needs no
   // tree); the sorted tree is made when eb_fmm_plan_ntbox or eb_fmm_plan_info asks for it
   bool tree_sorted = true;
+  bool hist0_ready = false;  // the radix sort's first histogram of the current keys is in sorttmp
   double2* wdz0 = nullptr;  // direct mode: the weights in the caller's order
   int* ntbox = nullptr;
   double2* dir_part = nullptr;  // direct mode: 2 x n partial sums per source chunk
@@ -2022,7 +2073,8 @@ static int sort_tree(eb_fmm_plan P, const double2* wsrc) {
   hipStream_t st = P->st;
   const int n = P->n, L = P->L;
   const int nleaf = 1 << (2 * L);
-  int rc = radix_sort_pairs(P->keys, P->perm, P->keys2, P->vals2, n, 2 * L, P->sorttmp, st);
+  int rc = radix_sort_pairs(P->keys, P->perm, P->keys2, P->vals2, n, 2 * L, P->sorttmp, st, P->hist0_ready);
+  P->hist0_ready = false;
   if (rc) return rc;
   const int t = std::max(n, nleaf + 1);
   hipLaunchKernelGGL(k_leaf_gather, dim3((t + 255) / 256), dim3(256), 0, st, P->keys, n, nleaf, P->leaf_start,
@@ -2054,6 +2106,7 @@ static int build_tree(eb_fmm_plan P, const double* x, const double* y, const dou
   hipStream_t st = P->st;
   const int n = P->n, L = P->L;
   const int nleaf = 1 << (2 * L);
+  // at most 1024 bounding-box partials: every block of k_keys reduces them all, one per thread
   const int bbblocks = std::min(1024, (n + BB_THREADS - 1) / BB_THREADS);
   if (P->nseg) {
     hipLaunchKernelGGL(k_seg_bbox, dim3(P->nseg), dim3(BB_THREADS), 0, st, P->nseg, P->seg_off, x, y,
@@ -2062,10 +2115,11 @@ static int build_tree(eb_fmm_plan P, const double* x, const double* y, const dou
                        P->seg_geo, P->zs0, P->keys, P->perm);
   } else {
     hipLaunchKernelGGL(k_bbox_part, dim3(bbblocks), dim3(BB_THREADS), 0, st, x, y, n, P->bbpart);
-    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(BB_THREADS), 0, st, P->bbpart, bbblocks, P->geom);
-    hipLaunchKernelGGL(k_keys, dim3((n + 255) / 256), dim3(256), 0, st, x, y, n, L, P->geom, P->zs0, P->keys,
-                       P->perm);
+    const int nblk = (n + RS_TILE_KEYS - 1) / RS_TILE_KEYS;
+    hipLaunchKernelGGL(k_keys, dim3(nblk), dim3(1024), 0, st, x, y, n, L, P->bbpart, bbblocks, P->geom, P->zs0,
+                       P->keys, P->perm, P->sorttmp, nblk);
   }
+  P->hist0_ready = P->nseg == 0;
   EB_HIP_CHECK(hipGetLastError());
   if (P->mode == EB_MODE_DIRECT) {
     // the O(N^2) sum runs on the points in the caller's order: four launches per rebuild

@@ -13,8 +13,12 @@ namespace eb {
 size_t scan_temp_ints(int n);
 int scan_exclusive(const int* in, int* out, int n, bool write_total, int* tmp, hipStream_t st);
 size_t radix_sort_temp_ints(int n);
+// The radix sort's tile: its first pass counts the low 8 key bits of keys [RS_TILE_KEYS b,
+// RS_TILE_KEYS (b + 1)) into tmp[d * nblk + b] (nblk tiles).  hist0_ready: the caller has
+// already written those counts (the FMM's k_keys), so the first pass skips its histogram.
+constexpr int RS_TILE_KEYS = 4096;
 int radix_sort_pairs(uint32_t* k0, int32_t* v0, uint32_t* k1, int32_t* v1, int n, int nbits, int* tmp,
-                     hipStream_t st);
+                     hipStream_t st, bool hist0_ready = false);
 
 // MKINTL offsets (cadap2.f:800-827) for a target whose 0-based column/row parity is
 // (px, py): column-outer, row-inner, skipping the 3x3 neighbourhood.  Returns 27.

@@ -15,6 +15,7 @@ constexpr int RS_ROUNDS = RS_SUB / EB_WAVE;     // 4 rounds of 64
 constexpr int RS_BITS = 8;
 constexpr int RS_RADIX = 1 << RS_BITS;
 static_assert(RS_RADIX == RS_THREADS && RS_RADIX <= RS_BT, "digit-indexed threads");
+static_assert(RS_TILE == RS_TILE_KEYS, "the tile that callers count ahead of the sort");
 
 // per tile: the count of each digit
 __global__ __launch_bounds__(RS_BT) void k_rs_hist(const uint32_t* __restrict__ keys, int n, int shift, int nblk,
@@ -289,7 +290,7 @@ size_t radix_sort_temp_ints(int n) {
 // Sorts (k0, v0) by the low `nbits` of the keys; the result ends in (k0, v0).  k1/v1 are
 // scratch of n entries, tmp holds radix_sort_temp_ints(n) ints.
 int radix_sort_pairs(uint32_t* k0, int32_t* v0, uint32_t* k1, int32_t* v1, int n, int nbits, int* tmp,
-                     hipStream_t st) {
+                     hipStream_t st, bool hist0_ready) {
   if (n <= 0) return EB_OK;
   const int nblk = (n + RS_TILE - 1) / RS_TILE;
   int* hist = tmp;
@@ -305,7 +306,8 @@ int radix_sort_pairs(uint32_t* k0, int32_t* v0, uint32_t* k1, int32_t* v1, int n
   for (int p = 0; p < passes; ++p) {
     const int shift = p * RS_BITS;
     if (shift < 32) {
-      hipLaunchKernelGGL(k_rs_hist, dim3(nblk), dim3(RS_BT), 0, st, ka, n, shift, nblk, hist);
+      if (p > 0 || !hist0_ready)
+        hipLaunchKernelGGL(k_rs_hist, dim3(nblk), dim3(RS_BT), 0, st, ka, n, shift, nblk, hist);
       hipLaunchKernelGGL(k_rs_prefix, dim3(RS_RADIX), dim3(RS_THREADS), 0, st, hist, nblk, pre, tot);
       hipLaunchKernelGGL(k_rs_scatter, dim3(nblk), dim3(RS_BT), 0, st, ka, va, kb, vb, n, shift, nblk, pre, tot);
       EB_HIP_CHECK(hipGetLastError());
