@@ -805,15 +805,22 @@ __device__ __forceinline__ void xlat_wave(const double* Cs, const double2* Ps, c
       xb[ks] = fma(w.x, cur[ks].x, w.y * cur[ks].y);
     }
     if constexpr (decltype(ksn)::value > 0) load_src(o + 1, cur, ksn);
-    double2 qw[MG][4];
-    double2 tw[TR ? TR : 1];
+    // L2L reads its weights where they are used: the registers the batch would hold cost it
+    // occupancy (43 -> 38 us for its three launches at 2^20 points), while the M2L's batch
+    // pays (and three waves per SIMD for the M2L, with the weights read where used, measured
+    // 381 -> 389 us)
+    constexpr bool JIT = MODE == XL_L2L;
+    double2 qw[JIT ? 1 : MG][4];
+    double2 tw[TR && !JIT ? TR : 1];
+    if constexpr (!JIT) {
 #pragma unroll
-    for (int mt = 0; mt < MG; ++mt)
+      for (int mt = 0; mt < MG; ++mt)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) qw[mt][r] = Qs[o * NM + TR + 16 * mt + kr + 4 * r];
+        for (int r = 0; r < 4; ++r) qw[mt][r] = Qs[o * NM + TR + 16 * mt + kr + 4 * r];
 #pragma unroll
-    for (int r = 0; r < TR; ++r) tw[r] = Qs[o * NM + r];
-    __builtin_amdgcn_sched_barrier(0);
+      for (int r = 0; r < TR; ++r) tw[r] = Qs[o * NM + r];
+      __builtin_amdgcn_sched_barrier(0);
+    }
     v4d acc[MG];
 #pragma unroll
     for (int mt = 0; mt < MG; ++mt) acc[mt] = v4d{0.0, 0.0, 0.0, 0.0};
@@ -844,16 +851,18 @@ __device__ __forceinline__ void xlat_wave(const double* Cs, const double2* Ps, c
       }
 #pragma unroll
       for (int r = 0; r < TR; ++r) {
-        ta[r] = fma(tw[r].x, part[r], ta[r]);
-        tb[r] = fma(tw[r].y, part[r], tb[r]);
+        const double2 w = JIT ? Qs[o * NM + r] : tw[JIT ? 0 : r];
+        ta[r] = fma(w.x, part[r], ta[r]);
+        tb[r] = fma(w.y, part[r], tb[r]);
       }
     }
 #pragma unroll
     for (int mt = 0; mt < MG; ++mt)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        sa[mt][r] = fma(qw[mt][r].x, acc[mt][r], sa[mt][r]);
-        sb[mt][r] = fma(qw[mt][r].y, acc[mt][r], sb[mt][r]);
+        const double2 w = JIT ? Qs[o * NM + TR + 16 * mt + kr + 4 * r] : qw[JIT ? 0 : mt][r];
+        sa[mt][r] = fma(w.x, acc[mt][r], sa[mt][r]);
+        sb[mt][r] = fma(w.y, acc[mt][r], sb[mt][r]);
       }
   };
   using IKS = std::integral_constant<int, KS>;
