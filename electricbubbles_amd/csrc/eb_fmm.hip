@@ -713,10 +713,12 @@ __device__ __forceinline__ double swap_adjacent(double v) {
 // source level (src, nsrc_boxes boxes; an index past it reads zeros); live lanes write.
 // Cs, Ps and Qs are the LDS tables (Ps and Qs at this wave's rows).  No barrier inside: a
 // kernel may call it for several target sets between its own barriers.
-template <int MODE, int KMAX, int KS, int MT, int TR, typename SrcBox>
+// MTW < MT: the wave computes only output tiles mt0 .. mt0 + MTW - 1 (and the tail rows when
+// tail is set), so that several waves can share one target set's rows.
+template <int MODE, int KMAX, int KS, int MT, int TR, int MTW = MT, typename SrcBox>
 __device__ __forceinline__ void xlat_wave(const double* Cs, const double2* Ps, const double2* Qs, int pt, bool live,
                                           SrcBox src_box, const double2* __restrict__ src, int nsrc_boxes,
-                                          double2* __restrict__ dst) {
+                                          double2* __restrict__ dst, int mt0 = 0, bool tail = true) {
   constexpr int NS = MODE == XL_M2L ? 27 : (MODE == XL_M2M ? 4 : 1);
   constexpr int NM = MT * 16 + TR, NK = KS * 4;
   constexpr int CS = TR ? (NM + 16) / 32 * 32 + 16 : NM + 4;
@@ -750,9 +752,9 @@ __device__ __forceinline__ void xlat_wave(const double* Cs, const double2* Ps, c
   // out_m = Re/Im(w_m y_m) with y_m split over this lane (v) and its partner (v'): summed
   // over the sources it is A -+ B' with A = sum w.x v and B = sum w.y v of the partner lane,
   // so one lane swap after the last source replaces one per source.
-  double sa[MT][4], sb[MT][4];
+  double sa[MTW][4], sb[MTW][4];
 #pragma unroll
-  for (int mt = 0; mt < MT; ++mt)
+  for (int mt = 0; mt < MTW; ++mt)
 #pragma unroll
     for (int r = 0; r < 4; ++r) sa[mt][r] = sb[mt][r] = 0.0;
   // Tail rows 0..3 (TR = 4; the tiles then hold rows 4..4 + 16 MT - 1): this lane's partial
@@ -816,7 +818,7 @@ __device__ __forceinline__ void xlat_wave(const double* Cs, const double2* Ps, c
 #pragma unroll
       for (int mt = 0; mt < MG; ++mt)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) qw[mt][r] = Qs[o * NM + TR + 16 * mt + kr + 4 * r];
+        for (int r = 0; r < 4; ++r) qw[mt][r] = Qs[o * NM + TR + 16 * (mt0 + mt) + kr + 4 * r];
 #pragma unroll
       for (int r = 0; r < TR; ++r) tw[r] = Qs[o * NM + r];
       __builtin_amdgcn_sched_barrier(0);
@@ -828,7 +830,7 @@ __device__ __forceinline__ void xlat_wave(const double* Cs, const double2* Ps, c
     for (int ks = 0; ks < KG; ++ks)
 #pragma unroll
       for (int mt = 0; mt < MG; ++mt)
-        acc[mt] = __builtin_amdgcn_mfma_f64_16x16x4f64(Cs[(4 * ks + kr) * CS + TR + 16 * mt + col], xb[ks], acc[mt],
+        acc[mt] = __builtin_amdgcn_mfma_f64_16x16x4f64(Cs[(4 * ks + kr) * CS + TR + 16 * (mt0 + mt) + col], xb[ks], acc[mt],
                                                        0, 0, 0);
     if constexpr (TR > 0) {
       double part[4];
@@ -860,13 +862,13 @@ __device__ __forceinline__ void xlat_wave(const double* Cs, const double2* Ps, c
     for (int mt = 0; mt < MG; ++mt)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const double2 w = JIT ? Qs[o * NM + TR + 16 * mt + kr + 4 * r] : qw[JIT ? 0 : mt][r];
+        const double2 w = JIT ? Qs[o * NM + TR + 16 * (mt0 + mt) + kr + 4 * r] : qw[JIT ? 0 : mt][r];
         sa[mt][r] = fma(w.x, acc[mt][r], sa[mt][r]);
         sb[mt][r] = fma(w.y, acc[mt][r], sb[mt][r]);
       }
   };
   using IKS = std::integral_constant<int, KS>;
-  using IMT = std::integral_constant<int, MT>;
+  using IMT = std::integral_constant<int, MTW>;
   using IKSF = std::integral_constant<int, KSF>;
   using INONE = std::integral_constant<int, 0>;
   double2 ra[KS];
@@ -884,9 +886,9 @@ __device__ __forceinline__ void xlat_wave(const double* Cs, const double2* Ps, c
     step(NS - 1, ra, IKS{}, IMT{}, INONE{});
   }
   const uint32_t sgn_hi = comp ? 0u : 0x80000000u;  // A - B' on component 0, A + B' on 1
-  double lam[MT][4];
+  double lam[MTW][4];
 #pragma unroll
-  for (int mt = 0; mt < MT; ++mt)
+  for (int mt = 0; mt < MTW; ++mt)
 #pragma unroll
     for (int r = 0; r < 4; ++r) {  // every lane takes part in the swap
       const double bo = swap_adjacent(sb[mt][r]);
@@ -917,18 +919,20 @@ __device__ __forceinline__ void xlat_wave(const double* Cs, const double2* Ps, c
   if (live) {
     double* out = (double*)dst;
 #pragma unroll
-    for (int mt = 0; mt < MT; ++mt)
+    for (int mt = 0; mt < MTW; ++mt)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int m = TR + 16 * mt + kr + 4 * r;
+        const int m = TR + 16 * (mt0 + mt) + kr + 4 * r;
         const size_t at = 2 * ((size_t)pt * 2 * KMAX + s * KMAX + m) + comp;
         out[at] = MODE == XL_L2L ? out[at] + lam[mt][r] : lam[mt][r];
       }
+    if (tail) {  // every wave forms the tail rows; one of several sharing a target set writes them
 #pragma unroll
-    for (int q = 0; q < TR / 4; ++q) {
-      const int m = kr + 4 * q;
-      const size_t at = 2 * ((size_t)pt * 2 * KMAX + s * KMAX + m) + comp;
-      out[at] = MODE == XL_L2L ? out[at] + tlam[q] : tlam[q];
+      for (int q = 0; q < TR / 4; ++q) {
+        const int m = kr + 4 * q;
+        const size_t at = 2 * ((size_t)pt * 2 * KMAX + s * KMAX + m) + comp;
+        out[at] = MODE == XL_L2L ? out[at] + tlam[q] : tlam[q];
+      }
     }
   }
 }
@@ -1150,6 +1154,24 @@ __global__ __launch_bounds__(256) void k_xlat_sub(XlatSub X, const int* __restri
   for (int e = 0; e < X.nlev; ++e) {
     const int l = X.lev[e];
     const int sd = 1 << (l - X.root);           // the subtree's side at level l, in boxes
+    if constexpr (MODE == XL_M2M && MT > 1) {
+      // M2M's tables do not depend on the parity class, so the level's sd^2 <= 4 targets
+      // (the four boxes below the root, or the root) are one wave's target set: its MT
+      // output tiles go to MT waves, whose chains are then one tile long instead of MT
+      if (sd <= 2) {
+        if (wv < MT) {
+          const bool valid = t < sd * sd;
+          const int ix = rx * sd + (valid ? t % sd : 0), iy = ry * sd + (valid ? t / sd : 0);
+          const int pt = pidx(ix, iy, X.Wt[e]);
+          const int Ws = X.Ws[e];
+          auto src_box = [&](int o) -> int { return pidx(2 * ix + (o & 1), 2 * iy + (o >> 1), Ws); };
+          xlat_wave<MODE, KMAX, KS, MT, TR, 1>(Cs, Ps, Qs, pt, valid, src_box, src + X.offs[e], X.NBPs[e],
+                                               dst + X.offt[e], wv, wv == 0);
+        }
+        __syncthreads();
+        continue;
+      }
+    }
     const int hs = sd >= 2 ? sd / 2 : 1;
     const int nper = hs * hs;                   // targets per class (the root alone when sd = 1)
     const int jpc = (nper + 3) / 4;             // wave jobs per class
