@@ -221,10 +221,21 @@ __global__ __launch_bounds__(GM_T) void k_gm_cgs_update(int n, int ll, int nch, 
                                                         double* __restrict__ w, double* red_part, unsigned* red_count,
                                                         double* __restrict__ sn) {
   extern __shared__ double hs[];
-  for (int j = threadIdx.x; j <= ll; j += GM_T) {
-    if (j == ll && !vn) break;
+  // h_j = the chunk partials summed in chunk order (one thread per row), eight loads in flight
+  // at a time: summed one by one, each chunk waited a memory latency
+  const int nj = vn ? ll + 1 : ll;
+  for (int j = threadIdx.x; j < nj; j += GM_T) {
+    const double* pj = part + (size_t)j * nch;
     double h = 0.0;
-    for (int c = 0; c < nch; ++c) h += part[(size_t)j * nch + c];
+    int c = 0;
+    for (; c + 8 <= nch; c += 8) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = pj[c + u];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) h += v[u];
+    }
+    for (; c < nch; ++c) h += pj[c];
     if (j < ll) {
       hs[j] = h;
       if (blockIdx.x == 0) scal[sh + j] = second ? scal[sh + j] + h : h;
@@ -397,7 +408,9 @@ int dpigmr_cycle(eb_gmres_s* g, eb_apply_fn apply, void* ctx, double* x, double 
       const size_t lds = sizeof(double) * (size_t)(ll + 1);
       hipLaunchKernelGGL(k_gm_cgs_dot, dim3(g->nch, ll + 1), dim3(GM_T), 0, st, n, g->nch, ll, g->V, w,
                          g->cgs_part);
-      hipLaunchKernelGGL(k_gm_cgs_update<false>, dim3(nb), dim3(GM_T), lds, st, n, ll, g->nch, g->V, g->cgs_part,
+      // the first pass's update has no norm to finish, so it takes one element per thread
+      const int nbu = std::max(1, std::min(GM_MAXBLK, (n + GM_T - 1) / GM_T));
+      hipLaunchKernelGGL(k_gm_cgs_update<false>, dim3(nbu), dim3(GM_T), lds, st, n, ll, g->nch, g->V, g->cgs_part,
                          g->scal, S_H, 0, g->scal + s_vn(g), w, nullptr, nullptr, nullptr);
       hipLaunchKernelGGL(k_gm_cgs_dot, dim3(g->nch, ll), dim3(GM_T), 0, st, n, g->nch, ll, g->V, w, g->cgs_part);
       hipLaunchKernelGGL(k_gm_cgs_update<true>, dim3(nb), dim3(GM_T), lds, st, n, ll, g->nch, g->V, g->cgs_part,
