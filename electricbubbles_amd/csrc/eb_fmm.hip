@@ -714,8 +714,9 @@ __device__ __forceinline__ double swap_adjacent(double v) {
 // Cs, Ps and Qs are the LDS tables (Ps and Qs at this wave's rows).  No barrier inside: a
 // kernel may call it for several target sets between its own barriers.
 // MTW < MT: the wave computes only output tiles mt0 .. mt0 + MTW - 1 (and the tail rows when
-// tail is set), so that several waves can share one target set's rows.
-template <int MODE, int KMAX, int KS, int MT, int TR, int MTW = MT, typename SrcBox>
+// tail is set), so that several waves can share one target set's rows.  JIT: read the
+// post-scaling weights where they are used instead of in one batch per source (below).
+template <int MODE, int KMAX, int KS, int MT, int TR, int MTW = MT, bool JIT = false, typename SrcBox>
 __device__ __forceinline__ void xlat_wave(const double* Cs, const double2* Ps, const double2* Qs, int pt, bool live,
                                           SrcBox src_box, const double2* __restrict__ src, int nsrc_boxes,
                                           double2* __restrict__ dst, int mt0 = 0, bool tail = true) {
@@ -807,11 +808,10 @@ __device__ __forceinline__ void xlat_wave(const double* Cs, const double2* Ps, c
       xb[ks] = fma(w.x, cur[ks].x, w.y * cur[ks].y);
     }
     if constexpr (decltype(ksn)::value > 0) load_src(o + 1, cur, ksn);
-    // L2L reads its weights where they are used: the registers the batch would hold cost it
-    // occupancy (43 -> 38 us for its three launches at 2^20 points), while the M2L's batch
-    // pays (and three waves per SIMD for the M2L, with the weights read where used, measured
-    // 381 -> 389 us)
-    constexpr bool JIT = MODE == XL_L2L;
+    // JIT (the one-level L2L kernel): the registers the batch would hold cost that kernel
+    // occupancy (the L2L stage 44 -> 38 us at 2^20 points), while the M2L's batch pays (three waves per SIMD
+    // for the M2L, with the weights read where used, measured 381 -> 389 us) and so does the
+    // L2L subtree kernel's (its few waves' chains wait on the reads: 9 -> 12 us at p = 37..43)
     double2 qw[JIT ? 1 : MG][4];
     double2 tw[TR && !JIT ? TR : 1];
     if constexpr (!JIT) {
@@ -1076,7 +1076,8 @@ __device__ __forceinline__ void xlat_body(const XlatLevels& LS, const int* __res
     if (MODE == XL_M2M) return pidx(2 * ix + (o & 1), 2 * iy + (o >> 1), Ws);
     return pidx(ix >> 1, iy >> 1, Ws);
   };
-  xlat_wave<MODE, KMAX, KS, MT, TR>(Cs, Ps, Qs, pt, live, src_box, ADAPT ? src : src + LS.offs[e], ADAPT ? nsrc : NBPs,
+  xlat_wave<MODE, KMAX, KS, MT, TR, MT, MODE == XL_L2L && !ADAPT>(Cs, Ps, Qs, pt, live, src_box,
+                                                                   ADAPT ? src : src + LS.offs[e], ADAPT ? nsrc : NBPs,
                                     ADAPT ? dst : dst + LS.offt[e]);
 }
 
