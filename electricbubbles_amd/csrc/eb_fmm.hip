@@ -808,8 +808,8 @@ __device__ __forceinline__ void xlat_wave(const double* Cs, const double2* Ps, c
       xb[ks] = fma(w.x, cur[ks].x, w.y * cur[ks].y);
     }
     if constexpr (decltype(ksn)::value > 0) load_src(o + 1, cur, ksn);
-    // JIT (the one-level L2L kernel): the registers the batch would hold cost that kernel
-    // occupancy (the L2L stage 44 -> 38 us at 2^20 points), while the M2L's batch pays (three waves per SIMD
+    // JIT (the one-level L2L kernels, uniform and adaptive): the registers the batch would hold
+    // cost them occupancy (the L2L stage 44 -> 38 us at 2^20 points, adaptive 104 -> 97 us), while the M2L's batch pays (three waves per SIMD
     // for the M2L, with the weights read where used, measured 381 -> 389 us) and so does the
     // L2L subtree kernel's (its few waves' chains wait on the reads: 9 -> 12 us at p = 37..43)
     double2 qw[JIT ? 1 : MG][4];
@@ -1076,8 +1076,8 @@ __device__ __forceinline__ void xlat_body(const XlatLevels& LS, const int* __res
     if (MODE == XL_M2M) return pidx(2 * ix + (o & 1), 2 * iy + (o >> 1), Ws);
     return pidx(ix >> 1, iy >> 1, Ws);
   };
-  xlat_wave<MODE, KMAX, KS, MT, TR, MT, MODE == XL_L2L && !ADAPT>(Cs, Ps, Qs, pt, live, src_box,
-                                                                   ADAPT ? src : src + LS.offs[e], ADAPT ? nsrc : NBPs,
+  xlat_wave<MODE, KMAX, KS, MT, TR, MT, MODE == XL_L2L>(Cs, Ps, Qs, pt, live, src_box, ADAPT ? src : src + LS.offs[e],
+                                                       ADAPT ? nsrc : NBPs,
                                     ADAPT ? dst : dst + LS.offt[e]);
 }
 
